@@ -1,0 +1,211 @@
+/*
+ * qsx.h -- C-ABI of the MI355X (gfx950) block-sparse symmetric tensor
+ * contraction library (QSpace contractQS hot path).
+ *
+ * Plain C types only: pointers, sizes, error codes.  Every entry point
+ * returns 0 on success or a QSX_ERR_* code and writes a message into the
+ * caller's err buffer (errlen bytes, may be NULL/0).
+ *
+ * Reference interfaces replaced (citations into moondg/qspace):
+ *   qsx_contract        <- contractQS usage #1, Source/contractQS.cc:505-632
+ *                          (contract_plain + CONTRACT_QS), which calls
+ *                          QSpace::contract, Source/QSpace.cc:4296-4420
+ *   qsx_ctr_plan_*      <- the same QSpace::contract split into its structure
+ *                          half (contract_getIdxSet, Source/QSpace.cc:4131-4294)
+ *                          and its data half (contractDATA_group,
+ *                          Source/QSpace_aux.cc:27-355); the split mirrors the
+ *                          pair-list caching of CPAT (Source/cpat.hh:572-593)
+ *   qsx_hpsi_*          <- two-site H*psi of Class/@Hamilton1D/private/get_HPsi.m:54-57
+ *                          (contractQS({HK_L,Psi},HK_R)) and the legacy C++
+ *                          CPAT::HZTimes, Source/cpat.hh:487-527
+ *   qsx_lanczos         <- RunEigs Lanczos loop, Source/eigs2SiteQS.cc:98-334
+ *   qsx_davidson        <- Davidson loop, Class/@Hamilton1D/private/update_psi_2site.m:223-391
+ *                          and RunDavidson, Source/david2SiteQS.cc:316-535
+ *   qsx_su2_cg / qsx_su2_x3
+ *                       <- SU(2) Clebsch-Gordan data and X-symbols of
+ *                          X3Map::contractCGR, Source/clebsch_aux.cc:4073-4510
+ */
+#ifndef QSX_H
+#define QSX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QSX_MAX_RANK 8
+#define QSX_MAX_SYM 4
+
+/* error codes */
+#define QSX_OK 0
+#define QSX_ERR_ARG 1         /* invalid argument, index, permutation */
+#define QSX_ERR_ITAG 2        /* itag mismatch (contract_getIdxSet) */
+#define QSX_ERR_QDIM 3        /* QDIM / qtype inconsistency */
+#define QSX_ERR_STRUCT 4      /* invalid QSpace structure (dims, labels) */
+#define QSX_ERR_UNSUPPORTED 5 /* outside the supported symmetry set (OM>1 etc.) */
+#define QSX_ERR_DEVICE 6      /* HIP runtime failure / no device */
+#define QSX_ERR_NOMEM 7
+#define QSX_ERR_CG 8          /* Clebsch-Gordan / X-symbol inconsistency */
+
+/* execution modes of the data half of a contraction */
+#define QSX_MODE_BATCHED 0    /* one grouped FP64 MFMA launch per contraction */
+#define QSX_MODE_PLAIN 1      /* one FP64 GEMM launch per (ia,ib) pair, no batching */
+#define QSX_MODE_SPLIT_BF16 2 /* BF16 multi-slice operands on MFMA, FP32 accumulate, FP64 recombine */
+#define QSX_MODE_SPLIT_F32 3  /* FP32 3-slice operands, FP32 accumulate, FP64 recombine */
+
+/*
+ * Symmetry-labelled block-sparse tensor in QSpace layout
+ * (struct {Q, data, info}, Source/QSpace.hh:1455-1475, mx layout 3150-3364).
+ *   Q     : nblk x (rank*qdim) int32 label records, row-major; a record is
+ *           leg-major with the qdim ints of one leg in qtype order
+ *           (QIDX of the reference).  SU(2) labels are 2S.
+ *   dims  : nblk x rank block dimensions; blocks are column-major.
+ *   data  : nblk pointers to the block data (host memory).  May be NULL
+ *           for the structure-only plan calls.
+ *   itags : rank itags, 8 chars packed little-endian into a uint64; the
+ *           conj ("outgoing", trailing '*') flag is bit 7 of byte 0
+ *           (itag_, Source/wbindex.hh:632-700).  May be NULL for purely
+ *           abelian tensors.
+ *   qtype : comma separated symmetry list, e.g. "A", "SU2", "A,SU2"
+ *           ("" = abelian labels without type).
+ *   cgw   : nblk x nsym outer-multiplicity weights (scalar per block and
+ *           non-abelian symmetry; entries of abelian symmetries ignored).
+ *           NULL = external normalisation |cgw| = sqrt(d) of
+ *           CRef::normExt (Source/clebsch.cc:4738-4760).
+ */
+typedef struct qsx_tensor {
+  int32_t nblk;
+  int32_t rank;
+  int32_t qdim;
+  const int32_t* Q;
+  const int64_t* dims;
+  const double* const* data;
+  const uint64_t* itags;
+  const char* qtype;
+  const double* cgw;
+} qsx_tensor;
+
+/* Result tensor, owned by the library; release with qsx_tensor_free. */
+typedef struct qsx_tensor_out {
+  int32_t nblk, rank, qdim, nsym;
+  int32_t* Q;        /* nblk x rank*qdim */
+  int64_t* dims;     /* nblk x rank */
+  int64_t* offs;     /* nblk+1 element offsets of the blocks in data */
+  double* data;      /* offs[nblk] doubles, host memory (NULL for structure only) */
+  uint64_t* itags;   /* rank */
+  double* cgw;       /* nblk x nsym */
+  char qtype[32];
+  /* statistics of the contraction that produced it */
+  int64_t npairs;        /* executed (ia,ib) pairs */
+  double useful_flops;   /* sum over executed pairs of 2*M*N*K */
+  double gpu_ms;         /* device time of the data half (events) */
+} qsx_tensor_out;
+
+typedef struct qsx_opts {
+  int32_t mode;          /* QSX_MODE_* */
+  int32_t verbose;       /* 0 quiet, 1 warnings (default), 2 info */
+  int32_t nslices;       /* split modes: slices per operand (0 = default) */
+  int32_t skip_zero;     /* 1: SkipZeroData + NormCGW as CONTRACT_QS does (default) */
+  int32_t hconj;         /* H*psi: also add the Hermitian-conjugate term (get_HPsi.m:55-56) */
+  const double* fermA;   /* optional per-block sign factors of A (contractDATA_group afac);
+                            for H*psi: per-block signs of HL */
+  const double* fermB;   /* optional per-block sign factors of B */
+} qsx_opts;
+
+/* library / device ------------------------------------------------------ */
+const char* qsx_version(void);
+int qsx_device_init(int device, char* err, size_t errlen);
+int qsx_malloc_device(void** ptr, size_t bytes, char* err, size_t errlen);
+int qsx_free_device(void* ptr);
+int qsx_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream);
+int qsx_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream);
+int qsx_memset_device(void* dst, int value, size_t bytes, void* stream);
+int qsx_synchronize(void);
+
+/* one-shot contraction: contractQS(A,ia,B,ib[,perm]) -------------------- */
+/* ia/ib are 0-based leg indices; conjA/conjB are the trailing '*' flags;
+   perm (0-based, may be shorter than the output rank or NULL) is completed
+   with the identity as in Source/contractQS.cc:505-594. */
+int qsx_contract(const qsx_tensor* A, const int32_t* ia, int32_t na, int32_t conjA,
+                 const qsx_tensor* B, const int32_t* ib, int32_t nb, int32_t conjB,
+                 const int32_t* perm, int32_t nperm, const qsx_opts* opts,
+                 qsx_tensor_out** C, char* err, size_t errlen);
+void qsx_tensor_free(qsx_tensor_out* t);
+
+/* parse a ctr-idx string ("13", "1,3", "[1 3]*") into 0-based indices;
+   returns the number of indices (>0) or a negative error */
+int qsx_parse_ctridx(const char* s, int32_t* idx, int32_t maxlen, int32_t* conj);
+
+/* device-resident contraction plans ------------------------------------- */
+/* Device data of a tensor is one buffer holding the blocks back to back in
+   block order, column-major, at the element offsets returned by
+   qsx_packed_offsets (no padding). */
+int qsx_packed_offsets(const qsx_tensor* T, int64_t* offs /* nblk+1 */);
+
+typedef struct qsx_ctr_plan qsx_ctr_plan;
+int qsx_ctr_plan_create(const qsx_tensor* A, const int32_t* ia, int32_t na, int32_t conjA,
+                        const qsx_tensor* B, const int32_t* ib, int32_t nb, int32_t conjB,
+                        const int32_t* perm, int32_t nperm, const qsx_opts* opts,
+                        qsx_ctr_plan** plan, char* err, size_t errlen);
+/* structure of the result (data==NULL); caller frees with qsx_tensor_free */
+int qsx_ctr_plan_output(const qsx_ctr_plan* plan, qsx_tensor_out** C);
+/* pairs grouped by output block: ia/ib/afac have npairs entries,
+   seg has nout+1 entries (may pass NULL to query the sizes) */
+int qsx_ctr_plan_pairs(const qsx_ctr_plan* plan, int64_t* npairs, int64_t* nout,
+                       int32_t* ia, int32_t* ib, double* afac, int64_t* seg);
+int qsx_ctr_execute(qsx_ctr_plan* plan, const double* dA, const double* dB, double* dC,
+                    void* stream);
+double qsx_ctr_plan_flops(const qsx_ctr_plan* plan);
+void qsx_ctr_plan_destroy(qsx_ctr_plan* plan);
+
+/* two-site H*psi and eigensolvers ---------------------------------------- */
+/* HL(a,a',w), psi(a',b') [rank 2, itag-matched], HR(b',b,w): H*psi =
+   contractQS({HL,psi},HR) with itag auto-matching.  The environments are
+   copied to the device once; psi/hpsi are device buffers in the packed
+   layout of psi's structure. */
+typedef struct qsx_hpsi_plan qsx_hpsi_plan;
+int qsx_hpsi_plan_create(const qsx_tensor* HL, const qsx_tensor* psi, const qsx_tensor* HR,
+                         const qsx_opts* opts, qsx_hpsi_plan** plan, char* err, size_t errlen);
+int64_t qsx_hpsi_size(const qsx_hpsi_plan* plan);      /* doubles in psi */
+double qsx_hpsi_flops(const qsx_hpsi_plan* plan);      /* useful flops per H*psi */
+double qsx_hpsi_bytes(const qsx_hpsi_plan* plan);      /* algorithmic bytes per H*psi */
+int qsx_hpsi_apply(qsx_hpsi_plan* plan, const double* dpsi, double* dhpsi, void* stream);
+/* full-space norm weight of each element of psi's packed layout (product
+   over non-abelian symmetries of cgw^2 of its block, CGT data normalised
+   to 1): <x,y> = sum_i w_i x_i y_i is the inner product in which H*psi is
+   symmetric; the eigensolvers work in the basis sqrt(w) x */
+int qsx_hpsi_weights(const qsx_hpsi_plan* plan, double* w);
+/* capture/replay the H*psi in a hipGraph (0 = direct launches) */
+int qsx_hpsi_use_graph(qsx_hpsi_plan* plan, int32_t on);
+/* per-kernel device timing of the last apply: [stage1_ms, stage2_ms] */
+int qsx_hpsi_last_timing(const qsx_hpsi_plan* plan, double* ms2);
+void qsx_hpsi_plan_destroy(qsx_hpsi_plan* plan);
+
+/* Lanczos with full FP64 re-orthogonalisation; psi0 is a host vector of
+   qsx_hpsi_size doubles; returns the ground energy, the Ritz values of each
+   iteration (niter entries) and the ground state (host, may be NULL). */
+int qsx_lanczos(qsx_hpsi_plan* plan, const double* psi0, int32_t niter, double* e0,
+                double* ritz, double* gs, char* err, size_t errlen);
+/* Davidson without preconditioner (update_psi_2site.m:223-391) */
+int qsx_davidson(qsx_hpsi_plan* plan, const double* psi0, int32_t niter, double* e0,
+                 double* ritz, double* gs, char* err, size_t errlen);
+
+/* SU(2) coefficient engine (host, exact arithmetic) ---------------------- */
+/* Clebsch-Gordan <j1 m1; j2 m2 | j3 m3> in Condon-Shortley convention,
+   all arguments as two-j / two-m integers */
+double qsx_su2_cg(int32_t tj1, int32_t tm1, int32_t tj2, int32_t tm2, int32_t tj3, int32_t tm3);
+/* X-symbol x3 of contracting the normalised CGTs of two blocks into the
+   CGT of the result block (X3Map::contractCGR, OM=1); labels are 2S per
+   leg, dirs are +1 (in) / -1 (out).  Returns 0 and sets *x3. */
+int qsx_su2_x3(int32_t ra, const int32_t* tja, const int32_t* dira,
+               int32_t rb, const int32_t* tjb, const int32_t* dirb,
+               const int32_t* ia, const int32_t* ib, int32_t nctr, double* x3,
+               char* err, size_t errlen);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QSX_H */

@@ -1,0 +1,1251 @@
+// oracle.cc -- CPU FP64 restatement of QSpace's block-sparse symmetric
+// contraction (contractQS usage #1) and of its SU(2) Clebsch-Gordan /
+// X-symbol data.
+//
+// TEST INFRASTRUCTURE ONLY.  This library is the checker of the HIP path:
+// only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+// load it, and never as the thing measured or shipped.  The product library
+// (qspace_amd/csrc) does not link it.
+//
+// Every function cites the reference file:line it restates (moondg/qspace,
+// Source/...).  Algorithms are restated, not copied:
+//   * label engine: SortRecs/hpsort (wbMatrix.cc:1534-1580, wbsort.cc:88-143),
+//     matchIndex/matchSortedIdx (wbMatrix.cc:2075-2101, wbsort.cc:153-271),
+//     groupRecs (wbMatrix.cc:1682-1731), QC.Cat (QSpace.cc:4278)
+//   * contraction: QSpace::contract + contract_getIdxSet (QSpace.cc:4131-4420),
+//     contractDATA_group (QSpace_aux.cc:27-355), CONTRACT_QS
+//     (contractQS.cc:596-632), SkipZeroData/SkipEmptyData (QSpace.cc:3200-3305),
+//     Permute (QSpace.hh:2669-2719), NormCGW (QSpace.cc:3727-3805)
+//   * SU(2): generators with index 0 <-> m=+j (clebsch_aux.cc:1048-1082),
+//     CGTs normalised to |C|_F=1 with first non-zero entry (column-major
+//     sparse order, wbsparray::Compress lex=-1, wbsparray.cc:3485-3520)
+//     positive (cdata::NormSignC clebsch.cc:1928-1985), canonical leg order
+//     of QSet::Sort (clebsch.cc:1435-1505), x3 = overlap of the contracted
+//     CGTs with the result CGT (X3Map::contractCGR clebsch_aux.cc:4073-4510),
+//     external norm sqrt(d) (CRef::normExt clebsch.cc:4738-4760).
+//   Here the CG coefficients are built by highest-weight + lowering-operator
+//   recursion (the construction the reference's generator path uses), which
+//   is independent of the product's exact Racah evaluation.
+
+#include <algorithm>
+#include <cstdarg>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <dlfcn.h>
+#include <map>
+#include <mutex>
+#include <numeric>
+#include <omp.h>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../include/qsx.h"
+
+namespace orc {
+
+struct Err {
+  int code;
+  std::string msg;
+};
+
+[[noreturn]] static void fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+static void fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  throw Err{code, buf};
+}
+
+enum { SYM_A = 0, SYM_SU2 = 1 };
+
+struct Tensor {
+  int n = 0, r = 0, qdim = 0, nsym = 0;
+  std::vector<int> symtype;          // per symmetry
+  std::string qtype;
+  std::vector<int32_t> Q;            // n x r*qdim
+  std::vector<int64_t> dims;         // n x r
+  std::vector<std::vector<double>> data;
+  std::vector<uint64_t> itags;       // r or empty
+  std::vector<double> cgw;           // n x nsym
+  int64_t numel(int i) const {
+    int64_t s = 1;
+    for (int k = 0; k < r; ++k) s *= dims[(size_t)i * r + k];
+    return s;
+  }
+  bool has_cg() const {
+    for (int t : symtype)
+      if (t == SYM_SU2) return true;
+    return false;
+  }
+};
+
+// qtype string -> symmetry list (QType::init_s, clebsch.cc:118-164; only
+// U(1) "A" and "SU2" are in scope)
+static void parse_qtype(const char* s, int qdim, std::vector<int>& st) {
+  st.clear();
+  std::string q = s ? s : "";
+  if (q.empty()) {
+    for (int i = 0; i < qdim; ++i) st.push_back(SYM_A);
+    return;
+  }
+  size_t p = 0;
+  while (p <= q.size()) {
+    size_t e = q.find(',', p);
+    if (e == std::string::npos) e = q.size();
+    std::string t = q.substr(p, e - p);
+    while (!t.empty() && t.front() == ' ') t.erase(t.begin());
+    while (!t.empty() && t.back() == ' ') t.pop_back();
+    if (t == "A" || t == "U1") st.push_back(SYM_A);
+    else if (t == "SU2") st.push_back(SYM_SU2);
+    else fail(QSX_ERR_UNSUPPORTED, "unsupported symmetry '%s' in qtype '%s'", t.c_str(), q.c_str());
+    p = e + 1;
+  }
+  if ((int)st.size() != qdim)
+    fail(QSX_ERR_QDIM, "qtype '%s' has %d symmetries but QDIM=%d", q.c_str(), (int)st.size(), qdim);
+}
+
+static Tensor from_c(const qsx_tensor* t, const char* name) {
+  if (!t) fail(QSX_ERR_ARG, "null tensor %s", name);
+  Tensor T;
+  T.n = t->nblk;
+  T.r = t->rank;
+  T.qdim = t->qdim;
+  if (T.n < 0 || T.r < 0 || T.r > QSX_MAX_RANK || T.qdim <= 0)
+    fail(QSX_ERR_STRUCT, "invalid tensor %s (n=%d r=%d qdim=%d)", name, T.n, T.r, T.qdim);
+  parse_qtype(t->qtype, T.qdim, T.symtype);
+  T.qtype = t->qtype ? t->qtype : "";
+  T.nsym = (int)T.symtype.size();
+  size_t rq = (size_t)T.r * T.qdim;
+  T.Q.assign(t->Q, t->Q + (size_t)T.n * rq);
+  T.dims.assign(t->dims, t->dims + (size_t)T.n * T.r);
+  if (t->itags) T.itags.assign(t->itags, t->itags + T.r);
+  T.data.resize(T.n);
+  for (int i = 0; i < T.n; ++i) {
+    int64_t s = T.numel(i);
+    if (s < 0) fail(QSX_ERR_STRUCT, "negative dims in %s", name);
+    if (t->data && t->data[i]) T.data[i].assign(t->data[i], t->data[i] + s);
+    else T.data[i].assign(s, 0.0);
+  }
+  T.cgw.assign((size_t)T.n * T.nsym, 1.0);
+  // records must be unique (QSpace.hh:3214-3215)
+  std::vector<int> P(T.n);
+  std::iota(P.begin(), P.end(), 0);
+  std::sort(P.begin(), P.end(), [&](int a, int b) {
+    return std::lexicographical_compare(&T.Q[a * rq], &T.Q[a * rq] + rq, &T.Q[b * rq], &T.Q[b * rq] + rq);
+  });
+  for (int i = 1; i < T.n; ++i)
+    if (std::equal(&T.Q[P[i] * rq], &T.Q[P[i] * rq] + rq, &T.Q[P[i - 1] * rq]))
+      fail(QSX_ERR_STRUCT, "non-unique Q records in %s (blocks %d, %d)", name, P[i - 1], P[i]);
+  return T;
+}
+
+// ---------------------------------------------------------------------
+// label engine
+// ---------------------------------------------------------------------
+
+// lexicographic record comparison, column 0 most significant
+// (recCompare lex=1, gathered.hh:538-553)
+static int rec_cmp(const int32_t* a, const int32_t* b, int m) {
+  for (int i = 0; i < m; ++i) {
+    if (a[i] < b[i]) return -1;
+    if (a[i] > b[i]) return 1;
+  }
+  return 0;
+}
+
+// Heap sort of record indices, ties by original index (Wb::hpsort,
+// wbsort.cc:88-143): the order is the total order (record asc, index asc).
+static std::vector<int> sort_recs(const std::vector<int32_t>& R, int n, int m) {
+  std::vector<int> p(n);
+  std::iota(p.begin(), p.end(), 0);
+  auto less = [&](int a, int b) {
+    int c = rec_cmp(&R[(size_t)a * m], &R[(size_t)b * m], m);
+    return c ? c < 0 : a < b;
+  };
+  auto sift = [&](int start, int end) {
+    int root = start;
+    for (;;) {
+      int child = 2 * root + 1;
+      if (child > end) break;
+      if (child + 1 <= end && less(p[child], p[child + 1])) ++child;
+      if (less(p[root], p[child])) {
+        std::swap(p[root], p[child]);
+        root = child;
+      } else break;
+    }
+  };
+  if (n < 2) return p;
+  for (int s = n / 2 - 1; s >= 0; --s) sift(s, n - 1);
+  for (int e = n - 1; e > 0; --e) {
+    std::swap(p[0], p[e]);
+    sift(0, e - 1);
+  }
+  return p;
+}
+
+// matchIndex: sort both record sets, sorted merge, cross product per equal
+// key with A outer, B inner (wbMatrix.cc:2075-2101, wbsort.cc:153-271)
+static void match_index(const std::vector<int32_t>& QA, int na, const std::vector<int32_t>& QB, int nb, int m,
+                        std::vector<int>& Ia, std::vector<int>& Ib) {
+  Ia.clear();
+  Ib.clear();
+  if (!na || !nb) return;
+  std::vector<int> pa = sort_recs(QA, na, m), pb = sort_recs(QB, nb, m);
+  int a = 0, b = 0;
+  while (a < na && b < nb) {
+    int c = rec_cmp(&QA[(size_t)pa[a] * m], &QB[(size_t)pb[b] * m], m);
+    if (c < 0) { ++a; continue; }
+    if (c > 0) { ++b; continue; }
+    int a2 = a + 1, b2 = b + 1;
+    while (a2 < na && !rec_cmp(&QA[(size_t)pa[a2] * m], &QA[(size_t)pa[a] * m], m)) ++a2;
+    while (b2 < nb && !rec_cmp(&QB[(size_t)pb[b2] * m], &QB[(size_t)pb[b] * m], m)) ++b2;
+    for (int i = a; i < a2; ++i)
+      for (int j = b; j < b2; ++j) {
+        Ia.push_back(pa[i]);
+        Ib.push_back(pb[j]);
+      }
+    a = a2;
+    b = b2;
+  }
+}
+
+// ---------------------------------------------------------------------
+// SU(2) Clebsch-Gordan data
+// ---------------------------------------------------------------------
+
+// full table <j1 m1; j2 m2 | j3 m3>, index i1 + d1*(i2 + d2*i3) with
+// i = j - m (index 0 <-> m=+j, clebsch_aux.cc:1048-1082); built from the
+// highest-weight state (J+ psi = 0) and repeated J- lowering, Condon-Shortley
+// phase <j1 j1; j2 j3-j1 | j3 j3> > 0.
+static std::vector<double> cg_table(int tj1, int tj2, int tj3) {
+  int d1 = tj1 + 1, d2 = tj2 + 1, d3 = tj3 + 1;
+  std::vector<double> T((size_t)d1 * d2 * d3, 0.0);
+  if (tj3 > tj1 + tj2 || tj3 < std::abs(tj1 - tj2) || ((tj1 + tj2 + tj3) & 1)) return T;
+  auto raise = [](int tj, int tm) {  // <m+1|J+|m> with two-j ints
+    double v = (double)(tj * (tj + 2) - tm * (tm + 2)) / 4.0;
+    return v > 0 ? std::sqrt(v) : 0.0;
+  };
+  auto lower = [](int tj, int tm) {  // <m-1|J-|m>
+    double v = (double)(tj * (tj + 2) - tm * (tm - 2)) / 4.0;
+    return v > 0 ? std::sqrt(v) : 0.0;
+  };
+  // highest weight: coefficients c(tm1) with tm1 + tm2 = tj3
+  std::vector<long double> c(d1, 0.0L);
+  int tm1_lo = std::max(-tj1, tj3 - tj2), tm1_hi = std::min(tj1, tj3 + tj2);
+  c[(tj1 - tm1_lo) / 2] = 1.0L;
+  for (int tm1 = tm1_lo + 2; tm1 <= tm1_hi; tm1 += 2) {
+    long double prev = c[(tj1 - (tm1 - 2)) / 2];
+    long double a1 = raise(tj1, tm1 - 2), a2 = raise(tj2, tj3 - tm1);
+    c[(tj1 - tm1) / 2] = -prev * a1 / a2;
+  }
+  long double nrm = 0;
+  for (auto v : c) nrm += v * v;
+  nrm = std::sqrt(nrm);
+  if (c[0] < 0) nrm = -nrm;  // c[0] <-> m1 = j1 (present: tm1_hi == tj1)
+  std::vector<long double> cur((size_t)d1 * d2, 0.0L), nxt((size_t)d1 * d2);
+  for (int tm1 = tm1_lo; tm1 <= tm1_hi; tm1 += 2) {
+    int i1 = (tj1 - tm1) / 2, i2 = (tj2 - (tj3 - tm1)) / 2;
+    cur[i1 + (size_t)d1 * i2] = c[i1] / nrm;
+  }
+  for (int i3 = 0; i3 < d3; ++i3) {
+    int tm3 = tj3 - 2 * i3;
+    for (int i2 = 0; i2 < d2; ++i2)
+      for (int i1 = 0; i1 < d1; ++i1) T[i1 + (size_t)d1 * (i2 + (size_t)d2 * i3)] = (double)cur[i1 + (size_t)d1 * i2];
+    if (i3 + 1 == d3) break;
+    std::fill(nxt.begin(), nxt.end(), 0.0L);
+    for (int i2 = 0; i2 < d2; ++i2)
+      for (int i1 = 0; i1 < d1; ++i1) {
+        long double v = cur[i1 + (size_t)d1 * i2];
+        if (v == 0) continue;
+        int tm1 = tj1 - 2 * i1, tm2 = tj2 - 2 * i2;
+        if (i1 + 1 < d1) nxt[(i1 + 1) + (size_t)d1 * i2] += v * lower(tj1, tm1);
+        if (i2 + 1 < d2) nxt[i1 + (size_t)d1 * (i2 + 1)] += v * lower(tj2, tm2);
+      }
+    long double f = lower(tj3, tm3);
+    for (auto& v : nxt) v /= f;
+    std::swap(cur, nxt);
+  }
+  return T;
+}
+
+static std::map<std::tuple<int, int, int>, std::vector<double>> g_cg;
+static std::mutex g_cg_mu;
+
+static const std::vector<double>& cg_cached(int tj1, int tj2, int tj3) {
+  std::lock_guard<std::mutex> lk(g_cg_mu);
+  auto key = std::make_tuple(tj1, tj2, tj3);
+  auto it = g_cg.find(key);
+  if (it == g_cg.end()) it = g_cg.emplace(key, cg_table(tj1, tj2, tj3)).first;
+  return it->second;
+}
+
+// normalise to |T|_F = 1 and make the first entry above eps (column-major
+// order) positive (cdata::NormSignC, clebsch.cc:1928-1985)
+static void norm_sign(std::vector<double>& T) {
+  double s = 0;
+  for (double v : T) s += v * v;
+  s = std::sqrt(s);
+  if (s == 0) fail(QSX_ERR_CG, "zero CGT");
+  for (double v : T)
+    if (std::fabs(v) > 1e-14 * s) {
+      if (v < 0) s = -s;
+      break;
+    }
+  for (double& v : T) v /= s;
+}
+
+// Canonical leg order of a CGT (QSet::Sort, clebsch.cc:1435-1505): legs
+// with the majority direction first (all "in" legs first if at least half
+// are in), each group ascending in the label, ties stable.  Returns
+// order[k] = block leg placed at canonical position k.
+static std::vector<int> canon_order(const std::vector<int>& tj, const std::vector<int>& dir) {
+  int r = (int)tj.size(), nin = 0;
+  for (int d : dir) nin += d > 0;
+  bool flip = nin < (r + 1) / 2;
+  std::vector<int> ord(r);
+  std::iota(ord.begin(), ord.end(), 0);
+  auto flag = [&](int k) { return (dir[k] > 0 ? 0 : 1) ^ (flip ? 1 : 0); };
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
+    if (flag(a) != flag(b)) return flag(a) < flag(b);
+    return tj[a] < tj[b];
+  });
+  if (2 * nin == r && r >= 2) {  // equal groups: rotate if first group > second
+    int l = r / 2;
+    for (int i = 0; i < l; ++i) {
+      int a = tj[ord[i]], b = tj[ord[i + l]];
+      if (a != b) {
+        if (a > b) std::rotate(ord.begin(), ord.begin() + l, ord.end());
+        break;
+      }
+    }
+  }
+  return ord;
+}
+
+// CGT of one block for one SU(2) symmetry, dense, column-major in the
+// block's leg order.  tj = 2S per leg, dir = +1 in / -1 out.
+static std::vector<double> block_cgt(const std::vector<int>& tj, const std::vector<int>& dir) {
+  int r = (int)tj.size();
+  if (r == 0) return {1.0};
+  if (r == 1) {
+    if (tj[0] != 0) fail(QSX_ERR_CG, "rank-1 SU(2) block with 2S=%d", tj[0]);
+    return {1.0};
+  }
+  if (r > 3) fail(QSX_ERR_UNSUPPORTED, "SU(2) CGT of rank %d (outer multiplicity) not supported", r);
+  std::vector<int> ord = canon_order(tj, dir);
+  std::vector<int> ctj(r), cflag(r);
+  int nin = 0;
+  for (int d : dir) nin += d > 0;
+  bool flip = nin < (r + 1) / 2;
+  for (int k = 0; k < r; ++k) {
+    ctj[k] = tj[ord[k]];
+    cflag[k] = (dir[ord[k]] > 0 ? 0 : 1) ^ (flip ? 1 : 0);
+  }
+  std::vector<int> cd(r);
+  for (int k = 0; k < r; ++k) cd[k] = ctj[k] + 1;
+  size_t N = 1;
+  for (int k = 0; k < r; ++k) N *= cd[k];
+  std::vector<double> C(N, 0.0);
+  if (r == 2) {
+    if (ctj[0] != ctj[1]) fail(QSX_ERR_CG, "rank-2 SU(2) block with 2S=(%d,%d)", ctj[0], ctj[1]);
+    int d = cd[0];
+    if (cflag[0] != cflag[1]) {
+      for (int i = 0; i < d; ++i) C[i + (size_t)d * i] = 1.0;
+    } else {  // both legs alike: coupling to the singlet, (-1)^{j-m} delta(m,-m')
+      for (int i = 0; i < d; ++i) {
+        int ip = d - 1 - i;
+        C[i + (size_t)d * ip] = (i % 2 == 0) ? 1.0 : -1.0;
+      }
+    }
+  } else {
+    if (cflag[0] == cflag[1] && cflag[1] != cflag[2]) {
+      const std::vector<double>& T = cg_cached(ctj[0], ctj[1], ctj[2]);
+      C = T;
+    } else if (cflag[0] == cflag[1] && cflag[1] == cflag[2]) {
+      // three legs alike: coupling to the singlet, C(m1,m2,m3) ~ <j1 m1 j2 m2|j3 -m3> (-1)^{j3-m3}
+      const std::vector<double>& T = cg_cached(ctj[0], ctj[1], ctj[2]);
+      int d1 = cd[0], d2 = cd[1], d3 = cd[2];
+      for (int i3 = 0; i3 < d3; ++i3) {
+        int i3m = d3 - 1 - i3;
+        double sg = (i3 % 2 == 0) ? 1.0 : -1.0;
+        for (int i2 = 0; i2 < d2; ++i2)
+          for (int i1 = 0; i1 < d1; ++i1)
+            C[i1 + (size_t)d1 * (i2 + (size_t)d2 * i3)] = sg * T[i1 + (size_t)d1 * (i2 + (size_t)d2 * i3m)];
+      }
+    } else {
+      fail(QSX_ERR_CG, "unexpected canonical direction pattern");
+    }
+  }
+  bool allzero = true;
+  for (double v : C)
+    if (v != 0) { allzero = false; break; }
+  if (allzero) fail(QSX_ERR_CG, "SU(2) CGT vanishes (labels violate the triangle rule)");
+  norm_sign(C);
+  // canonical -> block order: block leg ord[k] sits at canonical position k
+  std::vector<int> bd(r);
+  for (int k = 0; k < r; ++k) bd[k] = tj[k] + 1;
+  std::vector<double> B(N);
+  std::vector<int> idx(r, 0);
+  for (size_t lin = 0; lin < N; ++lin) {
+    // idx = block multi-index of lin
+    size_t rem = lin;
+    for (int k = 0; k < r; ++k) {
+      idx[k] = (int)(rem % bd[k]);
+      rem /= bd[k];
+    }
+    size_t cl = 0, st = 1;
+    for (int k = 0; k < r; ++k) {
+      cl += st * idx[ord[k]];
+      st *= cd[k];
+    }
+    B[lin] = C[cl];
+  }
+  return B;
+}
+
+// external norm of a block CGT (CRef::normExt, clebsch.cc:4738-4760)
+static double norm_ext(const std::vector<int>& tj, const std::vector<int>& dir) {
+  int r = (int)tj.size();
+  if (r == 0 || r > 3) return 1.0;
+  bool scalar = true;
+  for (int t : tj) scalar &= (t == 0);
+  if (scalar) return 1.0;
+  if (r == 3) {
+    int nin = 0;
+    for (int d : dir) nin += d > 0;
+    if (nin == 0 || nin == 3) return 1.0;
+    int odd = -1;
+    for (int k = 0; k < 3; ++k)
+      if ((nin == 2 && dir[k] < 0) || (nin == 1 && dir[k] > 0)) odd = k;
+    return std::sqrt((double)(tj[odd] + 1));
+  }
+  int d = 0;
+  for (int t : tj) d = std::max(d, t + 1);
+  return std::sqrt((double)d);
+}
+
+// X-symbol: contract the block CGTs of A and B over (ia,ib), project on the
+// (normalised) CGT of C (X3Map::contractCGR / contract_x3,
+// clebsch_aux.cc:4073-4510, 3099-3775).
+static double x3_dense(const std::vector<int>& tja, const std::vector<int>& dira, const std::vector<int>& tjb,
+                       const std::vector<int>& dirb, const std::vector<int>& ia, const std::vector<int>& ib) {
+  int ra = (int)tja.size(), rb = (int)tjb.size(), k = (int)ia.size();
+  std::vector<double> TA = block_cgt(tja, dira), TB = block_cgt(tjb, dirb);
+  std::vector<int> ka, kb, tjc, dirc;
+  std::vector<char> ca(ra, 0), cb(rb, 0);
+  for (int i = 0; i < k; ++i) {
+    ca[ia[i]] = 1;
+    cb[ib[i]] = 1;
+    if (tja[ia[i]] != tjb[ib[i]]) fail(QSX_ERR_CG, "contracted SU(2) labels differ");
+  }
+  for (int i = 0; i < ra; ++i)
+    if (!ca[i]) { ka.push_back(i); tjc.push_back(tja[i]); dirc.push_back(dira[i]); }
+  for (int i = 0; i < rb; ++i)
+    if (!cb[i]) { kb.push_back(i); tjc.push_back(tjb[i]); dirc.push_back(dirb[i]); }
+  std::vector<int> da(ra), db(rb);
+  for (int i = 0; i < ra; ++i) da[i] = tja[i] + 1;
+  for (int i = 0; i < rb; ++i) db[i] = tjb[i] + 1;
+  size_t Mc = 1, Kc = 1, Na = 1, Nb = 1;
+  for (int i : ka) Mc *= da[i];
+  for (int i : kb) Nb *= db[i];
+  for (int i = 0; i < k; ++i) Kc *= da[ia[i]];
+  (void)Na;
+  std::vector<size_t> sa(ra), sb(rb);
+  size_t s = 1;
+  for (int i = 0; i < ra; ++i) { sa[i] = s; s *= da[i]; }
+  s = 1;
+  for (int i = 0; i < rb; ++i) { sb[i] = s; s *= db[i]; }
+  auto offs = [](const std::vector<int>& legs, const std::vector<int>& d, const std::vector<size_t>& st) {
+    size_t n = 1;
+    for (int l : legs) n *= d[l];
+    std::vector<size_t> o(n);
+    std::vector<int> idx(legs.size(), 0);
+    for (size_t lin = 0; lin < n; ++lin) {
+      size_t rem = lin, off = 0;
+      for (size_t q = 0; q < legs.size(); ++q) {
+        idx[q] = (int)(rem % d[legs[q]]);
+        rem /= d[legs[q]];
+        off += idx[q] * st[legs[q]];
+      }
+      o[lin] = off;
+    }
+    return o;
+  };
+  std::vector<int> ial(ia.begin(), ia.end()), ibl(ib.begin(), ib.end());
+  std::vector<size_t> oam = offs(ka, da, sa), obn = offs(kb, db, sb), oak = offs(ial, da, sa), obk = offs(ibl, db, sb);
+  std::vector<double> TAB(Mc * Nb, 0.0);
+  for (size_t n = 0; n < Nb; ++n)
+    for (size_t m = 0; m < Mc; ++m) {
+      double acc = 0;
+      for (size_t q = 0; q < Kc; ++q) acc += TA[oam[m] + oak[q]] * TB[obn[n] + obk[q]];
+      TAB[m + Mc * n] = acc;
+    }
+  double nab0 = 0;
+  for (double v : TAB) nab0 += v * v;
+  if (nab0 < 1e-24) return 0.0;  // contraction vanishes (CR_CTR_ZERO)
+  std::vector<double> TC = block_cgt(tjc, dirc);
+  double x = 0, nab = 0;
+  for (size_t i = 0; i < TC.size(); ++i) {
+    x += TAB[i] * TC[i];
+    nab += TAB[i] * TAB[i];
+  }
+  double res = 0;
+  for (size_t i = 0; i < TC.size(); ++i) res += (TAB[i] - x * TC[i]) * (TAB[i] - x * TC[i]);
+  if (res > 1e-20 * std::max(1.0, nab))
+    fail(QSX_ERR_UNSUPPORTED, "contracted CGT is not proportional to a single result CGT (OM>1)");
+  if (std::fabs(x) < 1e-14) x = 0;  // CR_CTR_ZERO
+  return x;
+}
+
+static std::map<std::vector<int>, double> g_x3;
+static std::mutex g_x3_mu;
+
+static double x3_cached(const std::vector<int>& tja, const std::vector<int>& dira, const std::vector<int>& tjb,
+                        const std::vector<int>& dirb, const std::vector<int>& ia, const std::vector<int>& ib) {
+  std::vector<int> key;
+  key.push_back((int)tja.size());
+  key.insert(key.end(), tja.begin(), tja.end());
+  key.insert(key.end(), dira.begin(), dira.end());
+  key.push_back((int)tjb.size());
+  key.insert(key.end(), tjb.begin(), tjb.end());
+  key.insert(key.end(), dirb.begin(), dirb.end());
+  key.push_back((int)ia.size());
+  key.insert(key.end(), ia.begin(), ia.end());
+  key.insert(key.end(), ib.begin(), ib.end());
+  {
+    std::lock_guard<std::mutex> lk(g_x3_mu);
+    auto it = g_x3.find(key);
+    if (it != g_x3.end()) return it->second;
+  }
+  double x = x3_dense(tja, dira, tjb, dirb, ia, ib);
+  std::lock_guard<std::mutex> lk(g_x3_mu);
+  g_x3[key] = x;
+  return x;
+}
+
+// ---------------------------------------------------------------------
+// contraction
+// ---------------------------------------------------------------------
+
+static inline uint64_t tag_name(uint64_t t) { return t & ~(uint64_t)0x80; }
+static inline bool tag_out(uint64_t t) { return (t & 0x80) != 0; }
+
+struct Plan {
+  Tensor A, B;                  // inputs
+  std::vector<int> ica, icb;    // contracted legs
+  std::vector<int> ka, kb;      // kept legs
+  int conjA = 0, conjB = 0;
+  std::vector<uint64_t> tagsC;  // output itags (before perm)
+  std::vector<int> dirA, dirB;  // effective leg directions (+1 in, -1 out)
+  std::vector<int> Ia, Ib;      // pairs grouped by output block
+  std::vector<int64_t> seg;     // nC+1
+  std::vector<int32_t> QC;      // nC x rc*qdim
+  std::vector<double> afac;     // per pair (0 => skipped)
+  int rc = 0;
+};
+
+// contract_getIdxSet (QSpace.cc:4131-4294) restated
+static void get_idx_set(Plan& P, int verbose) {
+  Tensor &A = P.A, &B = P.B;
+  int ra = A.r, rb = B.r, k = (int)P.ica.size();
+  if (A.qdim != B.qdim) fail(QSX_ERR_QDIM, "incompatible objects (QDIM=%d/%d)", A.qdim, B.qdim);
+  if (k != (int)P.icb.size() || !k) fail(QSX_ERR_ARG, "invalid contraction (%d <> %d legs)", k, (int)P.icb.size());
+  if (A.symtype != B.symtype) {
+    bool allab = true;
+    for (int t : A.symtype) allab &= t == SYM_A;
+    for (int t : B.symtype) allab &= t == SYM_A;
+    if (!allab || (!A.qtype.empty() && !B.qtype.empty()))
+      fail(QSX_ERR_QDIM, "qtype inconsistency '%s' / '%s'", A.qtype.c_str(), B.qtype.c_str());
+  }
+  std::vector<char> ua(ra, 0), ub(rb, 0);
+  for (int i = 0; i < k; ++i) {
+    if (P.ica[i] < 0 || P.ica[i] >= ra || ua[P.ica[i]]++)
+      fail(QSX_ERR_ARG, "invalid contraction indices for A (rank %d)", ra);
+    if (P.icb[i] < 0 || P.icb[i] >= rb || ub[P.icb[i]]++)
+      fail(QSX_ERR_ARG, "invalid contraction indices for B (rank %d)", rb);
+  }
+  P.ka.clear();
+  P.kb.clear();
+  for (int i = 0; i < ra; ++i)
+    if (!ua[i]) P.ka.push_back(i);
+  for (int i = 0; i < rb; ++i)
+    if (!ub[i]) P.kb.push_back(i);
+  P.rc = (int)(P.ka.size() + P.kb.size());
+  bool cg = A.has_cg();
+  if (cg && (A.itags.empty() || B.itags.empty())) fail(QSX_ERR_ITAG, "missing itags for non-abelian QSpace");
+  // itag check (QSpace.cc:4230-4271)
+  std::vector<uint64_t> ta = A.itags, tb = B.itags;
+  if (!ta.empty() && !tb.empty()) {
+    int zflag = 0;
+    for (int i = 0; i < k; ++i) {
+      uint64_t a = ta[P.ica[i]], b = tb[P.icb[i]];
+      int q = 0;
+      if (tag_name(a) == tag_name(b)) q = 3;
+      else if (!tag_name(b) || !tag_name(a)) q = 1;  // lenient empty tag
+      if (q && (tag_out(a) != tag_out(b))) q = -q;
+      if (q) {
+        q = q > 0 ? 1 : -1;
+        if (zflag && zflag != q) q = 0;
+        else zflag = q;
+      }
+      if (!q) fail(QSX_ERR_ITAG, "itag mismatch at contracted leg %d (A leg %d, B leg %d)", i + 1, P.ica[i] + 1, P.icb[i] + 1);
+    }
+    if (zflag > 0 && !(P.conjA ^ P.conjB)) {
+      if (verbose) fprintf(stderr, "WRN contract() check missing conj-flag!? applying conj(A) flag\n");
+      P.conjA = 1;
+    }
+  }
+  if (!ta.empty() && P.conjA)
+    for (auto& t : ta) t ^= 0x80;
+  if (!tb.empty() && P.conjB)
+    for (auto& t : tb) t ^= 0x80;
+  P.tagsC.clear();
+  if (!ta.empty() || !tb.empty()) {
+    for (int i : P.ka) P.tagsC.push_back(ta.empty() ? 0 : ta[i]);
+    for (int i : P.kb) P.tagsC.push_back(tb.empty() ? 0 : tb[i]);
+  }
+  P.dirA.assign(ra, 1);
+  P.dirB.assign(rb, 1);
+  for (int i = 0; i < ra && !ta.empty(); ++i) P.dirA[i] = tag_out(ta[i]) ? -1 : 1;
+  for (int i = 0; i < rb && !tb.empty(); ++i) P.dirB[i] = tag_out(tb[i]) ? -1 : 1;
+  // getQsub / matchIndex / Cat / groupRecs (QSpace.cc:4273-4291)
+  int q = A.qdim, mc = k * q;
+  std::vector<int32_t> QAc((size_t)A.n * mc), QBc((size_t)B.n * mc);
+  for (int i = 0; i < A.n; ++i)
+    for (int l = 0; l < k; ++l)
+      for (int j = 0; j < q; ++j) QAc[(size_t)i * mc + l * q + j] = A.Q[(size_t)i * ra * q + P.ica[l] * q + j];
+  for (int i = 0; i < B.n; ++i)
+    for (int l = 0; l < k; ++l)
+      for (int j = 0; j < q; ++j) QBc[(size_t)i * mc + l * q + j] = B.Q[(size_t)i * rb * q + P.icb[l] * q + j];
+  std::vector<int> Ia, Ib;
+  match_index(QAc, A.n, QBc, B.n, mc, Ia, Ib);
+  int np = (int)Ia.size(), m = P.rc * q;
+  std::vector<int32_t> QCp((size_t)np * m);
+  for (int p = 0; p < np; ++p) {
+    int32_t* d = &QCp[(size_t)p * m];
+    for (int i : P.ka)
+      for (int j = 0; j < q; ++j) *d++ = A.Q[(size_t)Ia[p] * ra * q + i * q + j];
+    for (int i : P.kb)
+      for (int j = 0; j < q; ++j) *d++ = B.Q[(size_t)Ib[p] * rb * q + i * q + j];
+  }
+  P.Ia.clear();
+  P.Ib.clear();
+  P.seg.assign(1, 0);
+  P.QC.clear();
+  if (!np) return;
+  if (m == 0) {
+    P.Ia = Ia;
+    P.Ib = Ib;
+    P.seg.push_back(np);
+    return;
+  }
+  std::vector<int> perm = sort_recs(QCp, np, m);
+  for (int i = 0; i < np; ++i) {
+    int p = perm[i];
+    if (i && rec_cmp(&QCp[(size_t)p * m], &QCp[(size_t)perm[i - 1] * m], m)) P.seg.push_back(i);
+    if (i == 0 || rec_cmp(&QCp[(size_t)p * m], &QCp[(size_t)perm[i - 1] * m], m))
+      P.QC.insert(P.QC.end(), &QCp[(size_t)p * m], &QCp[(size_t)p * m] + m);
+    P.Ia.push_back(Ia[p]);
+    P.Ib.push_back(Ib[p]);
+  }
+  P.seg.push_back(np);
+}
+
+static std::vector<int> su2_labels(const Tensor& T, int blk, int sym_int_off) {
+  std::vector<int> tj(T.r);
+  for (int l = 0; l < T.r; ++l) tj[l] = T.Q[(size_t)blk * T.r * T.qdim + l * T.qdim + sym_int_off];
+  return tj;
+}
+
+// afac per pair: fermionic signs x prod_j x3_j wA_j wB_j (QSpace_aux.cc:100-230)
+static void compute_afac(Plan& P, const double* fA, const double* fB) {
+  Tensor &A = P.A, &B = P.B;
+  size_t np = P.Ia.size();
+  P.afac.assign(np, 1.0);
+  for (size_t p = 0; p < np; ++p) {
+    int ia = P.Ia[p], ib = P.Ib[p];
+    double af = (fA ? fA[ia] : 1.0) * (fB ? fB[ib] : 1.0);
+    for (int j = 0; j < A.nsym; ++j) {
+      if (A.symtype[j] != SYM_SU2) continue;
+      std::vector<int> tja = su2_labels(A, ia, j), tjb = su2_labels(B, ib, j);
+      double x = x3_cached(tja, P.dirA, tjb, P.dirB, P.ica, P.icb);
+      af *= x * A.cgw[(size_t)ia * A.nsym + j] * B.cgw[(size_t)ib * B.nsym + j];
+    }
+    if (std::fabs(af) < 1e-18) af = 0;  // CG_SKIP_EPS2
+    P.afac[p] = af;
+  }
+}
+
+// one pair: Ci = A_ia (x)_{ica,icb} B_ib with output legs [kept A | kept B]
+// (wbarray::contract, wbarray.cc:3386-3500), naive loops
+static void pair_product(const Plan& P, int ia, int ib, std::vector<double>& Ci, std::vector<int64_t>& dimsC) {
+  const Tensor &A = P.A, &B = P.B;
+  int ra = A.r, rb = B.r;
+  const int64_t* da = &A.dims[(size_t)ia * ra];
+  const int64_t* db = &B.dims[(size_t)ib * rb];
+  for (size_t l = 0; l < P.ica.size(); ++l)
+    if (da[P.ica[l]] != db[P.icb[l]])
+      fail(QSX_ERR_STRUCT, "contracted block dims differ (%ld/%ld)", (long)da[P.ica[l]], (long)db[P.icb[l]]);
+  std::vector<int64_t> sa(ra), sb(rb);
+  int64_t s = 1;
+  for (int i = 0; i < ra; ++i) { sa[i] = s; s *= da[i]; }
+  s = 1;
+  for (int i = 0; i < rb; ++i) { sb[i] = s; s *= db[i]; }
+  auto offs = [](const std::vector<int>& legs, const int64_t* d, const std::vector<int64_t>& st) {
+    int64_t n = 1;
+    for (int l : legs) n *= d[l];
+    std::vector<int64_t> o(n);
+    for (int64_t lin = 0; lin < n; ++lin) {
+      int64_t rem = lin, off = 0;
+      for (int l : legs) {
+        off += (rem % d[l]) * st[l];
+        rem /= d[l];
+      }
+      o[lin] = off;
+    }
+    return o;
+  };
+  std::vector<int64_t> om = offs(P.ka, da, sa), ok = offs(P.ica, da, sa), on = offs(P.kb, db, sb), okb = offs(P.icb, db, sb);
+  dimsC.clear();
+  for (int i : P.ka) dimsC.push_back(da[i]);
+  for (int i : P.kb) dimsC.push_back(db[i]);
+  int64_t M = om.size(), N = on.size(), K = ok.size();
+  Ci.assign(M * N, 0.0);
+  const double* a = A.data[ia].data();
+  const double* b = B.data[ib].data();
+  for (int64_t n = 0; n < N; ++n)
+    for (int64_t m = 0; m < M; ++m) {
+      double acc = 0;
+      for (int64_t q = 0; q < K; ++q) acc += a[om[m] + ok[q]] * b[on[n] + okb[q]];
+      Ci[m + M * n] = acc;
+    }
+}
+
+struct Result {
+  Tensor C;
+  int64_t npairs = 0;
+  double flops = 0;
+};
+
+static void permute_block(const std::vector<double>& src, const std::vector<int64_t>& d, const std::vector<int>& perm,
+                          std::vector<double>& dst) {
+  int r = (int)d.size();
+  std::vector<int64_t> nd(r), st(r);
+  int64_t s = 1;
+  for (int k = 0; k < r; ++k) { st[k] = s; s *= d[k]; }
+  for (int k = 0; k < r; ++k) nd[k] = d[perm[k]];
+  dst.resize(src.size());
+  std::vector<int64_t> idx(r, 0);
+  for (size_t lin = 0; lin < src.size(); ++lin) {
+    int64_t rem = lin, off = 0;
+    for (int k = 0; k < r; ++k) {
+      int64_t i = rem % nd[k];
+      rem /= nd[k];
+      off += i * st[perm[k]];
+    }
+    dst[lin] = src[off];
+  }
+}
+
+static Result contract(const qsx_tensor* cA, const int32_t* ia, int na, int conjA, const qsx_tensor* cB,
+                       const int32_t* ib, int nb, int conjB, const int32_t* perm, int nperm, const double* fA,
+                       const double* fB, int skip_zero, int verbose) {
+  Plan P;
+  P.A = from_c(cA, "A");
+  P.B = from_c(cB, "B");
+  if (cA->cgw)
+    for (size_t i = 0; i < P.A.cgw.size(); ++i) P.A.cgw[i] = cA->cgw[i];
+  if (cB->cgw)
+    for (size_t i = 0; i < P.B.cgw.size(); ++i) P.B.cgw[i] = cB->cgw[i];
+  P.ica.assign(ia, ia + na);
+  P.icb.assign(ib, ib + nb);
+  P.conjA = conjA;
+  P.conjB = conjB;
+  Result R;
+  Tensor& C = R.C;
+  if (P.A.n == 0 || P.B.n == 0) {  // empty input -> empty output (QSpace.cc:4311)
+    C.r = 0;
+    C.qdim = P.A.qdim;
+    C.symtype = P.A.symtype;
+    C.nsym = P.A.nsym;
+    C.qtype = P.A.qtype;
+    return R;
+  }
+  get_idx_set(P, verbose);
+  // default cgw: external norm (CRef::normExt) for inputs without cgw
+  auto default_cgw = [&](Tensor& T, const std::vector<int>& dir, const double* given) {
+    if (given) return;
+    for (int i = 0; i < T.n; ++i)
+      for (int j = 0; j < T.nsym; ++j)
+        T.cgw[(size_t)i * T.nsym + j] = T.symtype[j] == SYM_SU2 ? norm_ext(su2_labels(T, i, j), dir) : 1.0;
+  };
+  {
+    // directions of the stored tensors (before the conj flags)
+    std::vector<int> da(P.A.r, 1), db(P.B.r, 1);
+    for (int i = 0; i < P.A.r && !P.A.itags.empty(); ++i) da[i] = tag_out(P.A.itags[i]) ? -1 : 1;
+    for (int i = 0; i < P.B.r && !P.B.itags.empty(); ++i) db[i] = tag_out(P.B.itags[i]) ? -1 : 1;
+    default_cgw(P.A, da, cA->cgw);
+    default_cgw(P.B, db, cB->cgw);
+  }
+  compute_afac(P, fA, fB);
+  int q = P.A.qdim, rc = P.rc;
+  C.r = rc;
+  C.qdim = q;
+  C.symtype = P.A.symtype;
+  C.nsym = P.A.nsym;
+  C.qtype = P.A.qtype;
+  C.itags = P.tagsC;
+  int nC = (int)P.seg.size() - 1;
+  std::vector<char> keep(nC, 0);
+  std::vector<std::vector<double>> blocks(nC);
+  std::vector<std::vector<int64_t>> bdims(nC);
+  for (int c = 0; c < nC; ++c) {
+    std::vector<double> Ck, Ci;
+    bool empty = true;
+    for (int64_t p = P.seg[c]; p < P.seg[c + 1]; ++p) {
+      double af = P.afac[p];
+      if (af == 0) continue;
+      std::vector<int64_t> dC;
+      pair_product(P, P.Ia[p], P.Ib[p], Ci, dC);
+      int64_t K = 1;
+      for (int l : P.ica) K *= P.A.dims[(size_t)P.Ia[p] * P.A.r + l];
+      R.flops += 2.0 * (double)Ci.size() * (double)K;
+      R.npairs++;
+      if (empty) {
+        Ck.resize(Ci.size());
+        for (size_t i = 0; i < Ci.size(); ++i) Ck[i] = af * Ci[i];
+        bdims[c] = dC;
+        empty = false;
+      } else {
+        if (bdims[c] != dC) fail(QSX_ERR_STRUCT, "inconsistent output block dims");
+        for (size_t i = 0; i < Ci.size(); ++i) Ck[i] += af * Ci[i];
+      }
+    }
+    if (!empty) {
+      keep[c] = 1;
+      blocks[c] = std::move(Ck);
+    }
+  }
+  // SkipEmptyData / SkipZeroData (QSpace.cc:3200-3305): eps=0, zero blocks
+  // removed unless the result is a rank-2 block-diagonal matrix
+  bool blockdiag = false;
+  if (rc == 2) {
+    blockdiag = true;
+    for (int c = 0; c < nC && blockdiag; ++c)
+      for (int j = 0; j < q; ++j)
+        if (P.QC[(size_t)c * 2 * q + j] != P.QC[(size_t)c * 2 * q + q + j]) blockdiag = false;
+  }
+  if (skip_zero && !(rc == 2 && blockdiag)) {
+    for (int c = 0; c < nC; ++c) {
+      if (!keep[c]) continue;
+      bool z = true;
+      for (double v : blocks[c])
+        if (v != 0) { z = false; break; }
+      if (z) keep[c] = 0;
+    }
+  }
+  std::vector<int> cperm(rc);
+  std::iota(cperm.begin(), cperm.end(), 0);
+  if (nperm > 0) {
+    if (nperm > rc) fail(QSX_ERR_ARG, "invalid permutation (length %d > rank %d)", nperm, rc);
+    std::vector<char> used(rc, 0);
+    for (int i = 0; i < nperm; ++i) {
+      if (perm[i] < 0 || perm[i] >= rc || used[perm[i]]) fail(QSX_ERR_ARG, "invalid permutation");
+      used[perm[i]] = 1;
+      cperm[i] = perm[i];
+    }
+    int l = nperm;
+    for (int i = 0; i < rc; ++i)  // complete with the identity (bin/contractQS.m:154-170)
+      if (!used[i]) cperm[l++] = i;
+  }
+  bool idperm = true;
+  for (int i = 0; i < rc; ++i) idperm &= cperm[i] == i;
+  for (int c = 0; c < nC; ++c) {
+    if (!keep[c]) continue;
+    std::vector<int32_t> rec(P.QC.begin() + (size_t)c * rc * q, P.QC.begin() + (size_t)(c + 1) * rc * q);
+    std::vector<int64_t> d = bdims[c];
+    std::vector<double> data = std::move(blocks[c]);
+    if (!idperm) {  // QSpace::Permute (QSpace.hh:2669-2719): Q columns, data, itags
+      std::vector<int32_t> r2(rec.size());
+      for (int k2 = 0; k2 < rc; ++k2)
+        for (int j = 0; j < q; ++j) r2[k2 * q + j] = rec[cperm[k2] * q + j];
+      rec = r2;
+      std::vector<double> pd;
+      permute_block(data, d, cperm, pd);
+      data = std::move(pd);
+      std::vector<int64_t> d2(rc);
+      for (int k2 = 0; k2 < rc; ++k2) d2[k2] = d[cperm[k2]];
+      d = d2;
+    }
+    C.Q.insert(C.Q.end(), rec.begin(), rec.end());
+    C.dims.insert(C.dims.end(), d.begin(), d.end());
+    C.data.push_back(std::move(data));
+  }
+  if (!idperm && !C.itags.empty()) {
+    std::vector<uint64_t> t2(rc);
+    for (int k2 = 0; k2 < rc; ++k2) t2[k2] = C.itags[cperm[k2]];
+    C.itags = t2;
+  }
+  C.n = (int)C.data.size();
+  // NormCGW (QSpace.cc:3727-3805): cgw -> sqrt(d) external norm, data /= norm
+  C.cgw.assign((size_t)C.n * C.nsym, 1.0);
+  std::vector<int> dirC(rc, 1);
+  for (int i = 0; i < rc && !C.itags.empty(); ++i) dirC[i] = tag_out(C.itags[i]) ? -1 : 1;
+  for (int i = 0; i < C.n; ++i) {
+    double cfac = 1;
+    for (int j = 0; j < C.nsym; ++j) {
+      if (C.symtype[j] != SYM_SU2) continue;
+      double ne = norm_ext(su2_labels(C, i, j), dirC);
+      C.cgw[(size_t)i * C.nsym + j] = ne;
+      cfac /= ne;
+    }
+    if (cfac != 1)
+      for (double& v : C.data[i]) v *= cfac;
+  }
+  return R;
+}
+
+static qsx_tensor_out* to_out(const Result& R) {
+  const Tensor& C = R.C;
+  auto* o = (qsx_tensor_out*)calloc(1, sizeof(qsx_tensor_out));
+  o->nblk = C.n;
+  o->rank = C.r;
+  o->qdim = C.qdim;
+  o->nsym = C.nsym;
+  snprintf(o->qtype, sizeof o->qtype, "%s", C.qtype.c_str());
+  size_t rq = (size_t)C.r * C.qdim;
+  o->Q = (int32_t*)malloc(sizeof(int32_t) * std::max<size_t>(1, C.n * rq));
+  if (C.n > 0 && rq > 0) memcpy(o->Q, C.Q.data(), sizeof(int32_t) * C.n * rq);
+  o->dims = (int64_t*)malloc(sizeof(int64_t) * std::max<size_t>(1, (size_t)C.n * C.r));
+  if (C.n > 0 && C.r > 0) memcpy(o->dims, C.dims.data(), sizeof(int64_t) * C.n * C.r);
+  o->offs = (int64_t*)malloc(sizeof(int64_t) * (C.n + 1));
+  o->offs[0] = 0;
+  for (int i = 0; i < C.n; ++i) o->offs[i + 1] = o->offs[i] + (int64_t)C.data[i].size();
+  o->data = (double*)malloc(sizeof(double) * std::max<int64_t>(1, o->offs[C.n]));
+  for (int i = 0; i < C.n; ++i)
+    if (!C.data[i].empty()) memcpy(o->data + o->offs[i], C.data[i].data(), sizeof(double) * C.data[i].size());
+  o->itags = (uint64_t*)calloc(std::max(1, C.r), sizeof(uint64_t));
+  for (size_t i = 0; i < C.itags.size(); ++i) o->itags[i] = C.itags[i];
+  o->cgw = (double*)malloc(sizeof(double) * std::max<size_t>(1, C.cgw.size()));
+  if (!C.cgw.empty()) memcpy(o->cgw, C.cgw.data(), sizeof(double) * C.cgw.size());
+  o->npairs = R.npairs;
+  o->useful_flops = R.flops;
+  return o;
+}
+
+static void seterr(char* err, size_t errlen, const std::string& m) {
+  if (err && errlen) snprintf(err, errlen, "%s", m.c_str());
+}
+
+// ---------------------------------------------------------------------
+// CPU baseline with the reference's cost structure: OpenMP over output
+// blocks (QSpace.cc:4362-4393), per pair a host BLAS dgemm into a
+// temporary followed by an afac-scaled axpy (QSpace_aux.cc:229-235,
+// wbarray_blas.cc:121-160).  Pairs and afac come from the oracle's
+// structure pass (X-symbol cache warm), only the data half is timed.
+// ---------------------------------------------------------------------
+
+typedef void (*dgemm_ilp64_t)(const char*, const char*, const int64_t*, const int64_t*, const int64_t*,
+                              const double*, const double*, const int64_t*, const double*, const int64_t*,
+                              const double*, double*, const int64_t*, size_t, size_t);
+static dgemm_ilp64_t g_dgemm = nullptr;
+static void (*g_blas_threads)(int) = nullptr;
+static std::string g_blas_name = "internal";
+
+static void blocked_gemm(int64_t M, int64_t N, int64_t K, const double* A, int64_t lda, bool ta, const double* B,
+                         int64_t ldb, bool tb, double* C) {
+  // C(MxN, ld M) = op(A) op(B); simple cache-blocked loops (fallback)
+  std::fill(C, C + M * N, 0.0);
+  const int64_t BK = 64;
+  for (int64_t k0 = 0; k0 < K; k0 += BK) {
+    int64_t k1 = std::min(K, k0 + BK);
+    for (int64_t n = 0; n < N; ++n)
+      for (int64_t k = k0; k < k1; ++k) {
+        double bv = tb ? B[n + ldb * k] : B[k + ldb * n];
+        if (bv == 0) continue;
+        double* c = C + M * n;
+        if (!ta) {
+          const double* a = A + lda * k;
+          for (int64_t m = 0; m < M; ++m) c[m] += a[m] * bv;
+        } else {
+          for (int64_t m = 0; m < M; ++m) c[m] += A[k + lda * m] * bv;
+        }
+      }
+  }
+}
+
+struct Baseline {
+  Plan P;
+  // per pair matricisation: A as (M x K) or (K x M), B as (K x N) or (N x K)
+  std::vector<int64_t> M, N, K;
+  std::vector<char> ta, tb;
+  std::vector<std::vector<double>> Ap, Bp;  // operands permuted to matricisable layout
+  std::vector<std::vector<double>> Cb;      // output blocks
+  double flops = 0;
+};
+
+}  // namespace orc
+
+using namespace orc;
+
+extern "C" {
+
+int oracle_contract(const qsx_tensor* A, const int32_t* ia, int32_t na, int32_t conjA, const qsx_tensor* B,
+                    const int32_t* ib, int32_t nb, int32_t conjB, const int32_t* perm, int32_t nperm,
+                    const double* fermA, const double* fermB, int32_t skip_zero, qsx_tensor_out** C, char* err,
+                    size_t errlen) {
+  try {
+    Result R = contract(A, ia, na, conjA, B, ib, nb, conjB, perm, nperm, fermA, fermB, skip_zero, 0);
+    *C = to_out(R);
+    return 0;
+  } catch (const Err& e) {
+    seterr(err, errlen, e.msg);
+    return e.code;
+  } catch (const std::exception& e) {
+    seterr(err, errlen, e.what());
+    return QSX_ERR_ARG;
+  }
+}
+
+// structure half only: pairs grouped by output block, afac, output records
+int oracle_pairs(const qsx_tensor* A, const int32_t* ia, int32_t na, int32_t conjA, const qsx_tensor* B,
+                 const int32_t* ib, int32_t nb, int32_t conjB, int64_t* npairs, int64_t* nout, int32_t* Ia,
+                 int32_t* Ib, double* afac, int64_t* seg, int32_t* QC, char* err, size_t errlen) {
+  try {
+    Plan P;
+    P.A = from_c(A, "A");
+    P.B = from_c(B, "B");
+    if (A->cgw)
+      for (size_t i = 0; i < P.A.cgw.size(); ++i) P.A.cgw[i] = A->cgw[i];
+    if (B->cgw)
+      for (size_t i = 0; i < P.B.cgw.size(); ++i) P.B.cgw[i] = B->cgw[i];
+    P.ica.assign(ia, ia + na);
+    P.icb.assign(ib, ib + nb);
+    P.conjA = conjA;
+    P.conjB = conjB;
+    get_idx_set(P, 0);
+    std::vector<int> da(P.A.r, 1), db(P.B.r, 1);
+    for (int i = 0; i < P.A.r && !P.A.itags.empty(); ++i) da[i] = tag_out(P.A.itags[i]) ? -1 : 1;
+    for (int i = 0; i < P.B.r && !P.B.itags.empty(); ++i) db[i] = tag_out(P.B.itags[i]) ? -1 : 1;
+    if (!A->cgw)
+      for (int i = 0; i < P.A.n; ++i)
+        for (int j = 0; j < P.A.nsym; ++j)
+          P.A.cgw[(size_t)i * P.A.nsym + j] = P.A.symtype[j] == SYM_SU2 ? norm_ext(su2_labels(P.A, i, j), da) : 1.0;
+    if (!B->cgw)
+      for (int i = 0; i < P.B.n; ++i)
+        for (int j = 0; j < P.B.nsym; ++j)
+          P.B.cgw[(size_t)i * P.B.nsym + j] = P.B.symtype[j] == SYM_SU2 ? norm_ext(su2_labels(P.B, i, j), db) : 1.0;
+    compute_afac(P, nullptr, nullptr);
+    int64_t np = (int64_t)P.Ia.size(), nc = (int64_t)P.seg.size() - 1;
+    if (npairs) *npairs = np;
+    if (nout) *nout = nc;
+    if (Ia) std::copy(P.Ia.begin(), P.Ia.end(), Ia);
+    if (Ib) std::copy(P.Ib.begin(), P.Ib.end(), Ib);
+    if (afac) std::copy(P.afac.begin(), P.afac.end(), afac);
+    if (seg) std::copy(P.seg.begin(), P.seg.end(), seg);
+    if (QC) std::copy(P.QC.begin(), P.QC.end(), QC);
+    return 0;
+  } catch (const Err& e) {
+    seterr(err, errlen, e.msg);
+    return e.code;
+  }
+}
+
+void oracle_tensor_free(qsx_tensor_out* t) {
+  if (!t) return;
+  free(t->Q);
+  free(t->dims);
+  free(t->offs);
+  free(t->data);
+  free(t->itags);
+  free(t->cgw);
+  free(t);
+}
+
+// <j1 m1; j2 m2 | j3 m3> from the lowering-operator table (two-j ints)
+double oracle_su2_cg(int32_t tj1, int32_t tm1, int32_t tj2, int32_t tm2, int32_t tj3, int32_t tm3) {
+  if (tj1 < 0 || tj2 < 0 || tj3 < 0) return 0;
+  if (std::abs(tm1) > tj1 || std::abs(tm2) > tj2 || std::abs(tm3) > tj3) return 0;
+  if (((tj1 - tm1) & 1) || ((tj2 - tm2) & 1) || ((tj3 - tm3) & 1)) return 0;
+  if (tm1 + tm2 != tm3) return 0;
+  const std::vector<double>& T = cg_cached(tj1, tj2, tj3);
+  int d1 = tj1 + 1, d2 = tj2 + 1;
+  int i1 = (tj1 - tm1) / 2, i2 = (tj2 - tm2) / 2, i3 = (tj3 - tm3) / 2;
+  return T[i1 + (size_t)d1 * (i2 + (size_t)d2 * i3)];
+}
+
+// dense block CGT (column-major in the block's leg order); returns numel
+int64_t oracle_block_cgt(int32_t r, const int32_t* tj, const int32_t* dir, double* out, int64_t outlen, char* err,
+                         size_t errlen) {
+  try {
+    std::vector<int> t(tj, tj + r), d(dir, dir + r);
+    std::vector<double> T = block_cgt(t, d);
+    if (out && outlen >= (int64_t)T.size()) std::copy(T.begin(), T.end(), out);
+    return (int64_t)T.size();
+  } catch (const Err& e) {
+    seterr(err, errlen, e.msg);
+    return -e.code;
+  }
+}
+
+double oracle_norm_ext(int32_t r, const int32_t* tj, const int32_t* dir) {
+  std::vector<int> t(tj, tj + r), d(dir, dir + r);
+  return norm_ext(t, d);
+}
+
+int oracle_x3(int32_t ra, const int32_t* tja, const int32_t* dira, int32_t rb, const int32_t* tjb,
+              const int32_t* dirb, const int32_t* ia, const int32_t* ib, int32_t nctr, double* x3, char* err,
+              size_t errlen) {
+  try {
+    std::vector<int> a(tja, tja + ra), da(dira, dira + ra), b(tjb, tjb + rb), db(dirb, dirb + rb);
+    std::vector<int> ca(ia, ia + nctr), cb(ib, ib + nctr);
+    *x3 = x3_dense(a, da, b, db, ca, cb);
+    return 0;
+  } catch (const Err& e) {
+    seterr(err, errlen, e.msg);
+    return e.code;
+  }
+}
+
+// --- CPU baseline ---------------------------------------------------------
+
+// Load a host BLAS (ILP64 OpenBLAS as shipped with numpy) by path; falls
+// back to internal loops when absent.  Returns 1 if loaded.
+int oracle_load_blas(const char* path, const char* dgemm_sym, const char* threads_sym) {
+  void* h = dlopen(path, RTLD_NOW | RTLD_LOCAL);
+  if (!h) return 0;
+  g_dgemm = (dgemm_ilp64_t)dlsym(h, dgemm_sym);
+  if (threads_sym) g_blas_threads = (void (*)(int))dlsym(h, threads_sym);
+  if (!g_dgemm) return 0;
+  if (g_blas_threads) g_blas_threads(1);
+  g_blas_name = path;
+  return 1;
+}
+
+void* oracle_baseline_create(const qsx_tensor* A, const int32_t* ia, int32_t na, int32_t conjA, const qsx_tensor* B,
+                             const int32_t* ib, int32_t nb, int32_t conjB, char* err, size_t errlen) {
+  try {
+    auto* bl = new Baseline;
+    Plan& P = bl->P;
+    P.A = from_c(A, "A");
+    P.B = from_c(B, "B");
+    P.ica.assign(ia, ia + na);
+    P.icb.assign(ib, ib + nb);
+    P.conjA = conjA;
+    P.conjB = conjB;
+    get_idx_set(P, 0);
+    std::vector<int> da(P.A.r, 1), db(P.B.r, 1);
+    for (int i = 0; i < P.A.r && !P.A.itags.empty(); ++i) da[i] = tag_out(P.A.itags[i]) ? -1 : 1;
+    for (int i = 0; i < P.B.r && !P.B.itags.empty(); ++i) db[i] = tag_out(P.B.itags[i]) ? -1 : 1;
+    for (int i = 0; i < P.A.n; ++i)
+      for (int j = 0; j < P.A.nsym; ++j)
+        P.A.cgw[(size_t)i * P.A.nsym + j] = P.A.symtype[j] == SYM_SU2 ? norm_ext(su2_labels(P.A, i, j), da) : 1.0;
+    for (int i = 0; i < P.B.n; ++i)
+      for (int j = 0; j < P.B.nsym; ++j)
+        P.B.cgw[(size_t)i * P.B.nsym + j] = P.B.symtype[j] == SYM_SU2 ? norm_ext(su2_labels(P.B, i, j), db) : 1.0;
+    compute_afac(P, nullptr, nullptr);
+    // permute_to (QSpace.hh:2823-2865): kept legs first / contracted legs
+    // first, done once outside the timed region
+    std::vector<int> pa(P.ka), pb(P.icb);
+    pa.insert(pa.end(), P.ica.begin(), P.ica.end());
+    pb.insert(pb.end(), P.kb.begin(), P.kb.end());
+    bl->Ap.resize(P.A.n);
+    bl->Bp.resize(P.B.n);
+    for (int i = 0; i < P.A.n; ++i) {
+      std::vector<int64_t> d(P.A.dims.begin() + (size_t)i * P.A.r, P.A.dims.begin() + (size_t)(i + 1) * P.A.r);
+      permute_block(P.A.data[i], d, pa, bl->Ap[i]);
+    }
+    for (int i = 0; i < P.B.n; ++i) {
+      std::vector<int64_t> d(P.B.dims.begin() + (size_t)i * P.B.r, P.B.dims.begin() + (size_t)(i + 1) * P.B.r);
+      permute_block(P.B.data[i], d, pb, bl->Bp[i]);
+    }
+    size_t np = P.Ia.size();
+    bl->M.resize(np);
+    bl->N.resize(np);
+    bl->K.resize(np);
+    for (size_t p = 0; p < np; ++p) {
+      int64_t M = 1, N = 1, K = 1;
+      for (int l : P.ka) M *= P.A.dims[(size_t)P.Ia[p] * P.A.r + l];
+      for (int l : P.kb) N *= P.B.dims[(size_t)P.Ib[p] * P.B.r + l];
+      for (int l : P.ica) K *= P.A.dims[(size_t)P.Ia[p] * P.A.r + l];
+      bl->M[p] = M;
+      bl->N[p] = N;
+      bl->K[p] = K;
+      if (P.afac[p] != 0) bl->flops += 2.0 * M * N * K;
+    }
+    int nC = (int)P.seg.size() - 1;
+    bl->Cb.resize(nC);
+    for (int c = 0; c < nC; ++c) {
+      int64_t p = P.seg[c];
+      bl->Cb[c].assign(bl->M[p] * bl->N[p], 0.0);
+    }
+    return bl;
+  } catch (const Err& e) {
+    seterr(err, errlen, e.msg);
+    return nullptr;
+  }
+}
+
+double oracle_baseline_flops(void* h) { return h ? ((Baseline*)h)->flops : 0; }
+
+// one timed pass of the data half; returns the number of output blocks
+int oracle_baseline_run(void* h, int32_t nthreads) {
+  auto* bl = (Baseline*)h;
+  Plan& P = bl->P;
+  int nC = (int)P.seg.size() - 1;
+  int np = std::max(1, std::min(nthreads, nC));
+#pragma omp parallel num_threads(np)
+  {
+    std::vector<double> Ci;
+#pragma omp for schedule(dynamic, 1)
+    for (int c = 0; c < nC; ++c) {
+      std::vector<double>& Ck = bl->Cb[c];
+      bool first = true;
+      for (int64_t p = P.seg[c]; p < P.seg[c + 1]; ++p) {
+        double af = P.afac[p];
+        if (af == 0) continue;
+        int64_t M = bl->M[p], N = bl->N[p], K = bl->K[p];
+        Ci.resize(M * N);
+        const double* a = bl->Ap[P.Ia[p]].data();
+        const double* b = bl->Bp[P.Ib[p]].data();
+        if (g_dgemm) {
+          const double one = 1.0, zero = 0.0;
+          int64_t lda = std::max<int64_t>(1, M), ldb = std::max<int64_t>(1, K), ldc = std::max<int64_t>(1, M);
+          g_dgemm("N", "N", &M, &N, &K, &one, a, &lda, b, &ldb, &zero, Ci.data(), &ldc, 1, 1);
+        } else {
+          blocked_gemm(M, N, K, a, M, false, b, K, false, Ci.data());
+        }
+        if (first) {
+          for (int64_t i = 0; i < M * N; ++i) Ck[i] = af * Ci[i];
+          first = false;
+        } else {
+          for (int64_t i = 0; i < M * N; ++i) Ck[i] += af * Ci[i];
+        }
+      }
+    }
+  }
+  return nC;
+}
+
+// copy the baseline's output blocks (structure order of the pairs) for checks
+int64_t oracle_baseline_output(void* h, double* out, int64_t outlen) {
+  auto* bl = (Baseline*)h;
+  int64_t n = 0;
+  for (auto& c : bl->Cb) {
+    if (out && n + (int64_t)c.size() <= outlen) std::copy(c.begin(), c.end(), out + n);
+    n += (int64_t)c.size();
+  }
+  return n;
+}
+
+const char* oracle_blas_name(void) { return g_blas_name.c_str(); }
+
+void oracle_baseline_destroy(void* h) { delete (Baseline*)h; }
+
+}  // extern "C"

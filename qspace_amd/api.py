@@ -1,0 +1,311 @@
+"""Python mirror of the reference's contraction interface.
+
+``contractQS(A, ica, B, icb [, perm] [, opts])`` follows the MATLAB usage #1
+of bin/contractQS.m:20-191 (1-based numeric indices or strings like '13*';
+trailing '*' = complex conjugate of that operand), and the cell usage #2
+``contractQS([A, B], C, ...)`` contracts by itag auto-matching
+(contractQS_itags, Source/contractQS.cc:634-808).  Computation always runs
+through libqsx.so on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _capi
+from .qspace import QSpace, QsxOpts, QsxTensorOut, from_out, parse_ctridx, to_c
+
+__all__ = ["contractQS", "DeviceBuffer", "CtrPlan", "HPsi", "su2_cg", "su2_x3", "make_opts"]
+
+
+def make_opts(mode="batched", nslices=0, skip_zero=1, hconj=0, fermA=None, fermB=None, verbose=1):
+    o = QsxOpts()
+    o.mode = _capi.MODES[mode] if isinstance(mode, str) else int(mode)
+    o.verbose = verbose
+    o.nslices = nslices
+    o.skip_zero = skip_zero
+    o.hconj = hconj
+    keep = []
+    if fermA is not None:
+        fa = np.ascontiguousarray(fermA, dtype=np.float64)
+        keep.append(fa)
+        o.fermA = fa.ctypes.data_as(C.POINTER(C.c_double))
+    if fermB is not None:
+        fb = np.ascontiguousarray(fermB, dtype=np.float64)
+        keep.append(fb)
+        o.fermB = fb.ctypes.data_as(C.POINTER(C.c_double))
+    return o, keep
+
+
+def _ivec(v):
+    a = np.ascontiguousarray(np.asarray(v, dtype=np.int32).reshape(-1))
+    return a if a.size else np.zeros(1, np.int32)
+
+
+def _itag_match(A: QSpace, B: QSpace, conjA: int, conjB: int):
+    """iTags::getCtr (Source/wbindex.cc:1512-1590): legs with equal tag names
+    and opposite direction (same direction if exactly one side is conj)."""
+    cflag = conjA ^ conjB
+    for it in range(2):
+        ia, ib, used = [], [], set()
+        for i, ta in enumerate(A.itags):
+            na, oa = ta.rstrip("*"), ta.endswith("*")
+            for j, tb in enumerate(B.itags):
+                nb, ob = tb.rstrip("*"), tb.endswith("*")
+                if na and na == nb and ((oa == ob) == bool(cflag)) and j not in used:
+                    ia.append(i)
+                    ib.append(j)
+                    used.add(j)
+                    break
+        if ia:
+            return ia, ib, (conjA if it == 0 else 1 - conjA)
+        cflag = 1 - cflag
+    raise _capi.QsxError(2, "empty set of matching itags")
+
+
+def _contract_pair(A, ia, conjA, B, ib, conjB, perm=None, mode="batched", **kw) -> QSpace:
+    L = _capi.lib()
+    ca, ka = to_c(A)
+    cb, kb = to_c(B)
+    opts, keep = make_opts(mode=mode, **kw)
+    ia_ = _ivec(ia)
+    ib_ = _ivec(ib)
+    p = _ivec(perm if perm is not None else [])
+    out = C.POINTER(QsxTensorOut)()
+    err = C.create_string_buffer(1024)
+    rc = L.qsx_contract(C.byref(ca), ia_.ctypes.data_as(C.POINTER(C.c_int32)), len(ia), int(conjA), C.byref(cb),
+                        ib_.ctypes.data_as(C.POINTER(C.c_int32)), len(ib), int(conjB),
+                        p.ctypes.data_as(C.POINTER(C.c_int32)), 0 if perm is None else len(perm), C.byref(opts),
+                        C.byref(out), err, 1024)
+    _capi.check(rc, err)
+    try:
+        T = from_out(out.contents)
+        T.npairs = int(out.contents.npairs)
+        T.flops = float(out.contents.useful_flops)
+        T.gpu_ms = float(out.contents.gpu_ms)
+    finally:
+        L.qsx_tensor_free(out)
+    return T
+
+
+def contractQS(*args, mode="batched", **kw) -> QSpace:
+    """contractQS(A, ica, B, icb [,perm]) or contractQS([A, B, ...], C [,perm])."""
+    if len(args) >= 4 and isinstance(args[0], QSpace) and isinstance(args[2], QSpace):
+        A, ica, B, icb = args[:4]
+        perm = None
+        if len(args) > 4 and args[4] is not None and not isinstance(args[4], str):
+            perm = [int(x) - 1 for x in np.atleast_1d(args[4])]
+        ia, ca = parse_ctridx(ica)
+        ib, cb = parse_ctridx(icb)
+        return _contract_pair(A, ia, ca, B, ib, cb, perm, mode=mode, **kw)
+    # usage #2: itag auto-contraction, operands may carry a '*' flag
+    ops = []
+    perm = None
+    for a in args:
+        if isinstance(a, (list, tuple)) and a and isinstance(a[0], QSpace):
+            ops.append(_fold(list(a), mode, kw))
+        elif isinstance(a, QSpace):
+            ops.append([a, 0])
+        elif isinstance(a, str) and a.strip() == "*":
+            ops[-1][1] ^= 1
+        elif a is not None:
+            perm = [int(x) - 1 for x in np.atleast_1d(a)]
+    if len(ops) < 2:
+        raise ValueError("contractQS: need at least two operands")
+    # right-to-left fold A.(B.(C...)) (contractQS.cc:722-805)
+    cur, cflag = ops[-1]
+    for A, conjA in reversed(ops[:-1]):
+        ia, ib, conjA = _itag_match(A, cur, conjA, cflag)
+        cur = _contract_pair(A, ia, conjA, cur, ib, cflag, None, mode=mode, **kw)
+        cflag = 0
+    if perm is not None:
+        cur = permuteQS(cur, perm)
+    return cur
+
+
+def _fold(items, mode, kw):
+    ops = []
+    for a in items:
+        if isinstance(a, QSpace):
+            ops.append([a, 0])
+        elif isinstance(a, str) and a.strip() == "*":
+            ops[-1][1] ^= 1
+    cur, cflag = ops[-1]
+    for A, conjA in reversed(ops[:-1]):
+        ia, ib, conjA = _itag_match(A, cur, conjA, cflag)
+        cur = _contract_pair(A, ia, conjA, cur, ib, cflag, None, mode=mode, **kw)
+        cflag = 0
+    return [cur, 0]
+
+
+def permuteQS(A: QSpace, perm) -> QSpace:
+    p = list(perm) + [i for i in range(A.rank) if i not in perm]
+    data = [np.asfortranarray(np.transpose(d, p)) for d in A.data]
+    return QSpace([A.Q[i] for i in p], data, A.qtype, [A.itags[i] for i in p] if A.itags else [], A.info["cgw"])
+
+
+class DeviceBuffer:
+    """A device allocation of n doubles owned by libqsx."""
+
+    def __init__(self, n: int):
+        L = _capi.lib()
+        self.n = int(n)
+        self.ptr = C.c_void_p()
+        err = C.create_string_buffer(256)
+        _capi.check(L.qsx_malloc_device(C.byref(self.ptr), 8 * max(1, self.n), err, 256), err)
+
+    @classmethod
+    def from_array(cls, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        b = cls(a.size)
+        b.upload(a)
+        return b
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        _capi.check(_capi.lib().qsx_memcpy_h2d(self.ptr, a.ctypes.data, 8 * a.size, None), None)
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.n)
+        _capi.check(_capi.lib().qsx_memcpy_d2h(out.ctypes.data, self.ptr, 8 * self.n, None), None)
+        return out
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and self.ptr.value:
+            _capi.lib().qsx_free_device(self.ptr)
+            self.ptr = None
+
+
+class CtrPlan:
+    """Device-resident contraction plan (structure once, execute many)."""
+
+    def __init__(self, A: QSpace, ica, B: QSpace, icb, perm=None, mode="batched", **kw):
+        L = _capi.lib()
+        ia, ca = parse_ctridx(ica)
+        ib, cb = parse_ctridx(icb)
+        cA, self._ka = to_c(A, with_data=False)
+        cB, self._kb = to_c(B, with_data=False)
+        self._opts, self._okeep = make_opts(mode=mode, **kw)
+        ia_ = _ivec(ia)
+        ib_ = _ivec(ib)
+        p = _ivec(perm if perm is not None else [])
+        self.h = C.c_void_p()
+        err = C.create_string_buffer(1024)
+        rc = L.qsx_ctr_plan_create(C.byref(cA), ia_.ctypes.data_as(C.POINTER(C.c_int32)), len(ia), ca, C.byref(cB),
+                                   ib_.ctypes.data_as(C.POINTER(C.c_int32)), len(ib), cb,
+                                   p.ctypes.data_as(C.POINTER(C.c_int32)), 0 if perm is None else len(perm),
+                                   C.byref(self._opts), C.byref(self.h), err, 1024)
+        _capi.check(rc, err)
+        out = C.POINTER(QsxTensorOut)()
+        L.qsx_ctr_plan_output(self.h, C.byref(out))
+        self.out = from_out(out.contents, with_data=False)
+        L.qsx_tensor_free(out)
+        self.flops = L.qsx_ctr_plan_flops(self.h)
+        self.nout = self.out.numel()
+
+    def pairs(self):
+        L = _capi.lib()
+        n = C.c_int64()
+        m = C.c_int64()
+        L.qsx_ctr_plan_pairs(self.h, C.byref(n), C.byref(m), None, None, None, None)
+        ia = np.zeros(max(1, n.value), np.int32)
+        ib = np.zeros(max(1, n.value), np.int32)
+        af = np.zeros(max(1, n.value))
+        seg = np.zeros(m.value + 1, np.int64)
+        P = C.POINTER
+        L.qsx_ctr_plan_pairs(self.h, C.byref(n), C.byref(m), ia.ctypes.data_as(P(C.c_int32)),
+                             ib.ctypes.data_as(P(C.c_int32)), af.ctypes.data_as(P(C.c_double)),
+                             seg.ctypes.data_as(P(C.c_int64)))
+        return ia[: n.value], ib[: n.value], af[: n.value], seg
+
+    def execute(self, dA: DeviceBuffer, dB: DeviceBuffer, dC: DeviceBuffer):
+        _capi.check(_capi.lib().qsx_ctr_execute(self.h, dA.ptr, dB.ptr, dC.ptr, None), None)
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value:
+            _capi.lib().qsx_ctr_plan_destroy(self.h)
+            self.h = None
+
+
+class HPsi:
+    """Two-site H*psi = contractQS({HL,psi},HR) [+ hconj term], device resident."""
+
+    def __init__(self, HL: QSpace, psi: QSpace, HR: QSpace, mode="batched", hconj=1, fermA=None, nslices=0):
+        L = _capi.lib()
+        cL, self._k1 = to_c(HL)
+        cP, self._k2 = to_c(psi, with_data=False)
+        cR, self._k3 = to_c(HR)
+        self._opts, self._okeep = make_opts(mode=mode, hconj=hconj, fermA=fermA, nslices=nslices, skip_zero=0)
+        self.h = C.c_void_p()
+        err = C.create_string_buffer(1024)
+        rc = L.qsx_hpsi_plan_create(C.byref(cL), C.byref(cP), C.byref(cR), C.byref(self._opts), C.byref(self.h),
+                                    err, 1024)
+        _capi.check(rc, err)
+        self.psi = psi
+        self.N = int(L.qsx_hpsi_size(self.h))
+        self.flops = float(L.qsx_hpsi_flops(self.h))
+        self.bytes = float(L.qsx_hpsi_bytes(self.h))
+
+    def weights(self) -> np.ndarray:
+        w = np.zeros(self.N)
+        _capi.check(_capi.lib().qsx_hpsi_weights(self.h, w.ctypes.data_as(C.POINTER(C.c_double))), None)
+        return w
+
+    def use_graph(self, on: bool):
+        _capi.lib().qsx_hpsi_use_graph(self.h, int(on))
+
+    def apply_dev(self, dpsi: DeviceBuffer, dhpsi: DeviceBuffer):
+        _capi.check(_capi.lib().qsx_hpsi_apply(self.h, dpsi.ptr, dhpsi.ptr, None), None)
+
+    def apply(self, v: np.ndarray) -> np.ndarray:
+        d = DeviceBuffer.from_array(v)
+        o = DeviceBuffer(self.N)
+        self.apply_dev(d, o)
+        return o.download()
+
+    def last_timing(self):
+        ms = (C.c_double * 2)()
+        _capi.check(_capi.lib().qsx_hpsi_last_timing(self.h, ms), None)
+        return ms[0], ms[1]
+
+    def lanczos(self, psi0: np.ndarray, niter: int):
+        return self._eig(_capi.lib().qsx_lanczos, psi0, niter)
+
+    def davidson(self, psi0: np.ndarray, niter: int):
+        return self._eig(_capi.lib().qsx_davidson, psi0, niter)
+
+    def _eig(self, fn, psi0, niter):
+        p0 = np.ascontiguousarray(psi0, dtype=np.float64)
+        e0 = C.c_double()
+        ritz = np.zeros(niter)
+        gs = np.zeros(self.N)
+        err = C.create_string_buffer(512)
+        P = C.POINTER
+        rc = fn(self.h, p0.ctypes.data_as(P(C.c_double)), niter, C.byref(e0), ritz.ctypes.data_as(P(C.c_double)),
+                gs.ctypes.data_as(P(C.c_double)), err, 512)
+        _capi.check(rc, err)
+        return e0.value, ritz, gs
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value:
+            _capi.lib().qsx_hpsi_plan_destroy(self.h)
+            self.h = None
+
+
+def su2_cg(tj1, tm1, tj2, tm2, tj3, tm3) -> float:
+    """<j1 m1; j2 m2 | j3 m3> (two-j / two-m ints), exact-arithmetic host engine."""
+    return float(_capi.lib().qsx_su2_cg(tj1, tm1, tj2, tm2, tj3, tm3))
+
+
+def su2_x3(tja, dira, tjb, dirb, ia, ib) -> float:
+    L = _capi.lib()
+    P = C.POINTER
+    a, da, b, db, ca, cb = (_ivec(v) for v in (tja, dira, tjb, dirb, ia, ib))
+    out = C.c_double()
+    err = C.create_string_buffer(256)
+    rc = L.qsx_su2_x3(len(tja), a.ctypes.data_as(P(C.c_int32)), da.ctypes.data_as(P(C.c_int32)), len(tjb),
+                      b.ctypes.data_as(P(C.c_int32)), db.ctypes.data_as(P(C.c_int32)), ca.ctypes.data_as(P(C.c_int32)),
+                      cb.ctypes.data_as(P(C.c_int32)), len(ia), C.byref(out), err, 256)
+    _capi.check(rc, err)
+    return out.value

@@ -1,0 +1,1048 @@
+// contract.cc -- contractQS on the device: structure half (itags, device
+// label engine, X-symbols, output bookkeeping, operand layouts, GEMM work
+// lists) and data half (grouped FP64 MFMA GEMM or the plain per-pair path).
+//
+// Reference path restated (moondg/qspace):
+//   contract_plain / CONTRACT_QS      Source/contractQS.cc:505-632
+//   QSpace::contract                  Source/QSpace.cc:4296-4420
+//   contract_getIdxSet                Source/QSpace.cc:4131-4294
+//   permute_to                        Source/QSpace.hh:2823-2865
+//   contractDATA_group                Source/QSpace_aux.cc:27-355
+//   SkipZeroData / SkipEmptyData      Source/QSpace.cc:3200-3305
+//   QSpace::Permute                   Source/QSpace.hh:2669-2719
+//   NormCGW                           Source/QSpace.cc:3727-3805
+
+#include "contract.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <numeric>
+
+namespace qsx {
+
+void fail(int code, const char* fmt, ...) {
+  char buf[768];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  throw Error{code, buf};
+}
+
+void parse_qtype(const char* s, int qdim, std::vector<int>& st) {
+  st.clear();
+  std::string q = s ? s : "";
+  if (q.empty()) {
+    for (int i = 0; i < qdim; ++i) st.push_back(SYM_A);
+    return;
+  }
+  size_t p = 0;
+  while (p <= q.size()) {
+    size_t e = q.find(',', p);
+    if (e == std::string::npos) e = q.size();
+    std::string t = q.substr(p, e - p);
+    while (!t.empty() && t.front() == ' ') t.erase(t.begin());
+    while (!t.empty() && t.back() == ' ') t.pop_back();
+    if (t == "A" || t == "U1") st.push_back(SYM_A);
+    else if (t == "SU2") st.push_back(SYM_SU2);
+    else fail(QSX_ERR_UNSUPPORTED, "unsupported symmetry '%s' in qtype '%s' (supported: A, SU2)", t.c_str(), q.c_str());
+    p = e + 1;
+  }
+  if ((int)st.size() != qdim)
+    fail(QSX_ERR_QDIM, "qtype '%s' has %d symmetries but QDIM=%d", q.c_str(), (int)st.size(), qdim);
+}
+
+TensorDesc make_desc(const qsx_tensor* t, const char* name) {
+  if (!t) fail(QSX_ERR_ARG, "null tensor %s", name);
+  TensorDesc T;
+  T.n = t->nblk;
+  T.r = t->rank;
+  T.qdim = t->qdim;
+  if (T.n < 0 || T.r < 0 || T.r > QSX_MAX_RANK || T.qdim <= 0)
+    fail(QSX_ERR_STRUCT, "invalid tensor %s (nblk=%d rank=%d qdim=%d)", name, T.n, T.r, T.qdim);
+  if (T.n && (!t->Q || !t->dims)) fail(QSX_ERR_ARG, "tensor %s: null Q or dims", name);
+  parse_qtype(t->qtype, T.qdim, T.symtype);
+  T.qtype = t->qtype ? t->qtype : "";
+  T.nsym = (int)T.symtype.size();
+  const size_t rq = (size_t)T.r * T.qdim;
+  if (T.n) {
+    T.Q.assign(t->Q, t->Q + (size_t)T.n * rq);
+    T.dims.assign(t->dims, t->dims + (size_t)T.n * T.r);
+  }
+  if (t->itags) T.itags.assign(t->itags, t->itags + T.r);
+  T.offs.assign(T.n + 1, 0);
+  for (int i = 0; i < T.n; ++i) {
+    int64_t s = 1;
+    for (int k = 0; k < T.r; ++k) {
+      int64_t d = T.dims[(size_t)i * T.r + k];
+      if (d < 0) fail(QSX_ERR_STRUCT, "tensor %s: negative block dim", name);
+      s *= d;
+    }
+    T.offs[i + 1] = T.offs[i] + s;
+  }
+  T.cgw.assign((size_t)T.n * T.nsym, 1.0);
+  if (t->cgw) {
+    T.cgw.assign(t->cgw, t->cgw + (size_t)T.n * T.nsym);
+    T.cgw_given = true;
+  }
+  T.hdata = t->data;
+  // QIDX rows must be unique (QSpace.hh:3214-3215)
+  if (T.n > 1 && rq) {
+    std::vector<int> p(T.n);
+    std::iota(p.begin(), p.end(), 0);
+    std::sort(p.begin(), p.end(), [&](int a, int b) {
+      return std::lexicographical_compare(&T.Q[a * rq], &T.Q[a * rq] + rq, &T.Q[b * rq], &T.Q[b * rq] + rq);
+    });
+    for (int i = 1; i < T.n; ++i)
+      if (std::equal(&T.Q[p[i] * rq], &T.Q[p[i] * rq] + rq, &T.Q[p[i - 1] * rq]))
+        fail(QSX_ERR_STRUCT, "tensor %s: non-unique Q records (blocks %d and %d)", name, p[i - 1] + 1, p[i] + 1);
+  }
+  return T;
+}
+
+// ---------------------------------------------------------------- device
+static int g_dev = -1;
+static hipStream_t g_stream = nullptr;
+
+void ensure_device() {
+  if (g_dev >= 0) return;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) fail(QSX_ERR_DEVICE, "no HIP device available");
+  QSX_HIP(hipGetDevice(&g_dev));
+  QSX_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+}
+
+hipStream_t default_stream() {
+  ensure_device();
+  return g_stream;
+}
+
+void* dev_alloc(size_t bytes) {
+  ensure_device();
+  void* p = nullptr;
+  if (bytes == 0) bytes = 8;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) fail(QSX_ERR_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  return p;
+}
+
+void dev_free(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+CtrPlan::~CtrPlan() {
+  dev_free(dpairs);
+  for (auto& t : dtiles) dev_free(t);
+  dev_free(dAperm);
+  dev_free(dBperm);
+  dev_free(dCtmp);
+  dev_free(dpbA);
+  dev_free(dpbB);
+  dev_free(dpbC);
+}
+
+// ---------------------------------------------------------------- itags
+static inline uint64_t tag_name(uint64_t t) { return t & ~(uint64_t)0x80; }
+static inline bool tag_out(uint64_t t) { return (t & 0x80) != 0; }
+
+void match_itags(const TensorDesc& A, const TensorDesc& B, int conjA, int conjB, std::vector<int32_t>& ia,
+                 std::vector<int32_t>& ib, int& conjA_out) {
+  if ((int)A.itags.size() != A.r || (int)B.itags.size() != B.r)
+    fail(QSX_ERR_ITAG, "itag auto-contraction needs itags on both operands");
+  conjA_out = conjA;
+  int cflag = conjA ^ conjB;
+  for (int it = 0; it < 2; ++it) {
+    ia.clear();
+    ib.clear();
+    std::vector<char> mb(B.r, 0);
+    int e = 0;
+    const bool want_same = cflag != 0;  // sameAs sign +1 (same dir) if one side is conjugated
+    for (int i = 0; i < A.r; ++i) {
+      int hit = -1;
+      for (int j = 0; j < B.r; ++j) {
+        uint64_t a = A.itags[i], b = B.itags[j];
+        if (tag_name(a) != tag_name(b) || !tag_name(a)) continue;
+        bool same = tag_out(a) == tag_out(b);
+        if (same != want_same) continue;
+        if (hit >= 0 || mb[j]) ++e;
+        else hit = j;
+      }
+      if (hit >= 0) {
+        ia.push_back(i);
+        ib.push_back(hit);
+        mb[hit] = 1;
+      }
+    }
+    if (e) fail(QSX_ERR_ITAG, "non-unique itags in auto-contraction");
+    if (!ia.empty()) {
+      if (it) conjA_out = !conjA;  // "check missing conj flag" -> conj(A)
+      return;
+    }
+    cflag = !cflag;
+  }
+  fail(QSX_ERR_ITAG, "empty set of matching itags");
+}
+
+// ---------------------------------------------------------------- structure
+static std::vector<int> su2_col(const TensorDesc& T, int blk, int sym) {
+  std::vector<int> tj(T.r);
+  for (int l = 0; l < T.r; ++l) tj[l] = T.label(blk, l, sym);
+  return tj;
+}
+
+// contract_getIdxSet checks + output itags (QSpace.cc:4131-4271)
+static void check_and_tags(CtrPlan& P) {
+  TensorDesc &A = P.A, &B = P.B;
+  const int ra = A.r, rb = B.r, k = (int)P.ica.size();
+  if (A.qdim != B.qdim) fail(QSX_ERR_QDIM, "incompatible objects (QDIM=%d/%d)", A.qdim, B.qdim);
+  if (k != (int)P.icb.size() || !k) fail(QSX_ERR_ARG, "invalid contraction (%d <> %d legs)", k, (int)P.icb.size());
+  if (A.symtype != B.symtype) {
+    bool allab = true;
+    for (int t : A.symtype) allab &= t == SYM_A;
+    for (int t : B.symtype) allab &= t == SYM_A;
+    if (!allab || (!A.qtype.empty() && !B.qtype.empty()))
+      fail(QSX_ERR_QDIM, "qtype inconsistency '%s' / '%s'", A.qtype.c_str(), B.qtype.c_str());
+  }
+  std::vector<char> ua(ra, 0), ub(rb, 0);
+  for (int i = 0; i < k; ++i) {
+    if (P.ica[i] < 0 || P.ica[i] >= ra || ua[P.ica[i]]++)
+      fail(QSX_ERR_ARG, "invalid contraction indices for A (rank %d)", ra);
+    if (P.icb[i] < 0 || P.icb[i] >= rb || ub[P.icb[i]]++)
+      fail(QSX_ERR_ARG, "invalid contraction indices for B (rank %d)", rb);
+  }
+  P.ka.clear();
+  P.kb.clear();
+  for (int i = 0; i < ra; ++i)
+    if (!ua[i]) P.ka.push_back(i);
+  for (int i = 0; i < rb; ++i)
+    if (!ub[i]) P.kb.push_back(i);
+  P.rc = (int)(P.ka.size() + P.kb.size());
+  const bool cg = A.has_cg();
+  if (cg && ((int)A.itags.size() != ra || (int)B.itags.size() != rb))
+    fail(QSX_ERR_ITAG, "missing itags for non-abelian QSpace");
+  std::vector<uint64_t> ta = A.itags, tb = B.itags;
+  if (!ta.empty() && !tb.empty()) {
+    int zflag = 0;
+    for (int i = 0; i < k; ++i) {
+      uint64_t a = ta[P.ica[i]], b = tb[P.icb[i]];
+      int q = 0;
+      if (tag_name(a) == tag_name(b)) q = 3;
+      else if (!tag_name(b) || !tag_name(a)) q = 1;
+      if (q && tag_out(a) != tag_out(b)) q = -q;
+      if (q) {
+        q = q > 0 ? 1 : -1;
+        if (zflag && zflag != q) q = 0;
+        else zflag = q;
+      }
+      if (!q)
+        fail(QSX_ERR_ITAG, "itag mismatch at contracted leg %d (A leg %d, B leg %d)", i + 1, P.ica[i] + 1,
+             P.icb[i] + 1);
+    }
+    if (zflag > 0 && !(P.conjA ^ P.conjB)) {
+      if (P.verbose) fprintf(stderr, "WRN contract() check missing conj-flag!? applying conj(A) flag\n");
+      P.conjA = 1;
+    }
+  }
+  if (!ta.empty() && P.conjA)
+    for (auto& t : ta) t ^= 0x80;
+  if (!tb.empty() && P.conjB)
+    for (auto& t : tb) t ^= 0x80;
+  P.tagsC.clear();
+  if (!ta.empty() || !tb.empty()) {
+    for (int i : P.ka) P.tagsC.push_back(ta.empty() ? 0 : ta[i]);
+    for (int i : P.kb) P.tagsC.push_back(tb.empty() ? 0 : tb[i]);
+  }
+  P.dirA.assign(ra, 1);
+  P.dirB.assign(rb, 1);
+  for (int i = 0; i < ra && !ta.empty(); ++i) P.dirA[i] = tag_out(ta[i]) ? -1 : 1;
+  for (int i = 0; i < rb && !tb.empty(); ++i) P.dirB[i] = tag_out(tb[i]) ? -1 : 1;
+  P.dirC.assign(P.rc, 1);
+  for (int i = 0; i < P.rc && !P.tagsC.empty(); ++i) P.dirC[i] = tag_out(P.tagsC[i]) ? -1 : 1;
+  // default cgw: external norm of the stored tensors (before conj flags)
+  auto defcgw = [](TensorDesc& T) {
+    if (T.cgw_given) return;
+    std::vector<int> dir(T.r, 1);
+    for (int i = 0; i < T.r && !T.itags.empty(); ++i) dir[i] = tag_out(T.itags[i]) ? -1 : 1;
+    for (int i = 0; i < T.n; ++i)
+      for (int j = 0; j < T.nsym; ++j)
+        if (T.symtype[j] == SYM_SU2) T.cgw[(size_t)i * T.nsym + j] = su2_norm_ext(su2_col(T, i, j), dir);
+  };
+  defcgw(A);
+  defcgw(B);
+}
+
+// joint ordering of the contracted legs and the operand layouts (permute_to,
+// QSpace.hh:2823-2865): a zero-copy matricisation when the contracted legs
+// are contiguous at the front or end of both operands in a common order,
+// otherwise a permuted copy of the operand(s)
+static void choose_layouts(CtrPlan& P) {
+  const int ra = P.A.r, rb = P.B.r, k = (int)P.ica.size();
+  auto is_front_end = [](const std::vector<int>& legs, int r, int& where) {
+    std::vector<int> s(legs);
+    std::sort(s.begin(), s.end());
+    int k = (int)s.size();
+    bool front = true, end = true;
+    for (int i = 0; i < k; ++i) {
+      front &= s[i] == i;
+      end &= s[i] == r - k + i;
+    }
+    where = end ? 2 : (front ? 1 : 0);  // prefer 'end' (MxK / NxK with kept legs first)
+    return front || end;
+  };
+  auto increasing = [](const std::vector<int>& v) {
+    for (size_t i = 1; i < v.size(); ++i)
+      if (v[i] <= v[i - 1]) return false;
+    return true;
+  };
+  std::vector<int> ordA(k), ordB(k);
+  std::iota(ordA.begin(), ordA.end(), 0);
+  std::iota(ordB.begin(), ordB.end(), 0);
+  std::stable_sort(ordA.begin(), ordA.end(), [&](int x, int y) { return P.ica[x] < P.ica[y]; });
+  std::stable_sort(ordB.begin(), ordB.end(), [&](int x, int y) { return P.icb[x] < P.icb[y]; });
+  auto apply = [&](const std::vector<int>& o) {
+    std::vector<int> a(k), b(k);
+    for (int i = 0; i < k; ++i) {
+      a[i] = P.ica[o[i]];
+      b[i] = P.icb[o[i]];
+    }
+    P.ica = a;
+    P.icb = b;
+  };
+  int wa = 0, wb = 0;
+  bool fa = is_front_end(P.ica, ra, wa), fb = is_front_end(P.icb, rb, wb);
+  // native layouts: A with contracted legs at its end is MxK, at its front
+  // KxM; B with contracted legs at its front is KxN, at its end NxK
+  auto front_of = [&](const std::vector<int>& legs) {
+    std::vector<int> s(legs);
+    std::sort(s.begin(), s.end());
+    bool front = true;
+    for (int i = 0; i < (int)s.size(); ++i) front &= s[i] == i;
+    return front;
+  };
+  auto set_native = [&](void) {
+    P.la.permute = false;
+    P.la.kmajor = (wa == 1);
+    P.lb.permute = false;
+    P.lb.kmajor = !front_of(P.icb);
+  };
+  if (fa && fb) {
+    std::vector<int> ica0 = P.ica, icb0 = P.icb;
+    apply(ordA);
+    if (increasing(P.icb)) {
+      set_native();
+      return;
+    }
+    P.ica = ica0;
+    P.icb = icb0;
+    apply(ordB);
+    if (increasing(P.ica)) {
+      set_native();
+      return;
+    }
+    P.ica = ica0;
+    P.icb = icb0;
+  }
+  if (fa) {  // keep A, permute B to [contracted (A order) | kept]
+    apply(ordA);
+    P.la.permute = false;
+    P.la.kmajor = (wa == 1);
+    P.lb.permute = true;
+    P.lb.perm.assign(P.icb.begin(), P.icb.end());
+    P.lb.perm.insert(P.lb.perm.end(), P.kb.begin(), P.kb.end());
+    P.lb.kmajor = false;
+    return;
+  }
+  if (fb) {  // keep B, permute A to [kept | contracted (B order)]
+    apply(ordB);
+    P.lb.permute = false;
+    P.lb.kmajor = !front_of(P.icb);
+    P.la.permute = true;
+    P.la.perm.assign(P.ka.begin(), P.ka.end());
+    P.la.perm.insert(P.la.perm.end(), P.ica.begin(), P.ica.end());
+    P.la.kmajor = false;
+    return;
+  }
+  apply(ordA);  // permute both
+  P.la.permute = true;
+  P.la.perm.assign(P.ka.begin(), P.ka.end());
+  P.la.perm.insert(P.la.perm.end(), P.ica.begin(), P.ica.end());
+  P.la.kmajor = false;
+  P.lb.permute = true;
+  P.lb.perm.assign(P.icb.begin(), P.icb.end());
+  P.lb.perm.insert(P.lb.perm.end(), P.kb.begin(), P.kb.end());
+  P.lb.kmajor = false;
+}
+
+static void build_structure(CtrPlan& P, const int32_t* perm, int nperm, const double* fA, const double* fB) {
+  check_and_tags(P);
+  TensorDesc &A = P.A, &B = P.B;
+  const int q = A.qdim, k = (int)P.ica.size();
+  const int kc = k * q, kma = (int)P.ka.size() * q, kmb = (int)P.kb.size() * q;
+  // label records (getQsub, QSpace.hh:2315-2340)
+  std::vector<int32_t> QAc((size_t)A.n * kc), QAk((size_t)A.n * kma), QBc((size_t)B.n * kc), QBk((size_t)B.n * kmb);
+  for (int i = 0; i < A.n; ++i) {
+    for (int l = 0; l < k; ++l)
+      for (int j = 0; j < q; ++j) QAc[(size_t)i * kc + l * q + j] = A.label(i, P.ica[l], j);
+    for (size_t l = 0; l < P.ka.size(); ++l)
+      for (int j = 0; j < q; ++j) QAk[(size_t)i * kma + l * q + j] = A.label(i, P.ka[l], j);
+  }
+  for (int i = 0; i < B.n; ++i) {
+    for (int l = 0; l < k; ++l)
+      for (int j = 0; j < q; ++j) QBc[(size_t)i * kc + l * q + j] = B.label(i, P.icb[l], j);
+    for (size_t l = 0; l < P.kb.size(); ++l)
+      for (int j = 0; j < q; ++j) QBk[(size_t)i * kmb + l * q + j] = B.label(i, P.kb[l], j);
+  }
+  P.L = device_label_engine(QAc, QAk, A.n, QBc, QBk, B.n, kc, kma, kmb, default_stream());
+  const int nL = (int)P.L.seg.size() - 1;
+  const size_t np = P.L.Ia.size();
+  // afac per pair (contractDATA_group, QSpace_aux.cc:100-230)
+  P.afac.assign(np, 1.0);
+  for (size_t p = 0; p < np; ++p) {
+    const int ia = P.L.Ia[p], ib = P.L.Ib[p];
+    double af = (fA ? fA[ia] : 1.0) * (fB ? fB[ib] : 1.0);
+    for (int j = 0; j < A.nsym && af != 0; ++j) {
+      if (A.symtype[j] != SYM_SU2) continue;
+      double x = su2_x3(su2_col(A, ia, j), P.dirA, su2_col(B, ib, j), P.dirB, P.ica, P.icb);
+      af *= x * A.cgw[(size_t)ia * A.nsym + j] * B.cgw[(size_t)ib * B.nsym + j];
+    }
+    if (std::fabs(af) < 1e-18) af = 0;  // CG_SKIP_EPS2
+    P.afac[p] = af;
+  }
+  // output permutation (completed with the identity, bin/contractQS.m:154-170)
+  P.perm.resize(P.rc);
+  std::iota(P.perm.begin(), P.perm.end(), 0);
+  if (nperm > 0) {
+    if (nperm > P.rc) fail(QSX_ERR_ARG, "invalid permutation (length %d > rank %d)", nperm, P.rc);
+    std::vector<char> used(P.rc, 0);
+    for (int i = 0; i < nperm; ++i) {
+      if (perm[i] < 0 || perm[i] >= P.rc || used[perm[i]]) fail(QSX_ERR_ARG, "invalid permutation");
+      used[perm[i]] = 1;
+      P.perm[i] = perm[i];
+    }
+    int l = nperm;
+    for (int i = 0; i < P.rc; ++i)
+      if (!used[i]) P.perm[l++] = i;
+  }
+  P.has_perm = false;
+  for (int i = 0; i < P.rc; ++i) P.has_perm |= P.perm[i] != i;
+  // layouts (may reorder ica/icb jointly; afac is independent of that order)
+  choose_layouts(P);
+  // executed output blocks, pair descriptors, output normalisation
+  P.nC = 0;
+  P.Qout.clear();
+  P.dimsOut.clear();
+  P.offsOut.assign(1, 0);
+  P.cgwOut.clear();
+  P.blkL.clear();
+  P.pairs.clear();
+  P.pIa.clear();
+  P.pIb.clear();
+  P.pseg.assign(1, 0);
+  P.M.clear();
+  P.N.clear();
+  P.flops = 0;
+  const int rc = P.rc, mc = rc * q;
+  for (int c = 0; c < nL; ++c) {
+    int64_t p0 = P.L.seg[c], p1 = P.L.seg[c + 1];
+    bool any = false;
+    for (int64_t p = p0; p < p1; ++p) any |= P.afac[p] != 0;
+    if (!any) continue;  // SkipEmptyData
+    // output block dims [kept A | kept B] from the first executed pair
+    std::vector<int64_t> d;
+    int64_t first = -1;
+    for (int64_t p = p0; p < p1; ++p)
+      if (P.afac[p] != 0) { first = p; break; }
+    const int ia0 = P.L.Ia[first], ib0 = P.L.Ib[first];
+    for (int l : P.ka) d.push_back(A.dims[(size_t)ia0 * A.r + l]);
+    for (int l : P.kb) d.push_back(B.dims[(size_t)ib0 * B.r + l]);
+    int64_t Mc = 1, Nc = 1;
+    for (int l : P.ka) Mc *= A.dims[(size_t)ia0 * A.r + l];
+    for (int l : P.kb) Nc *= B.dims[(size_t)ib0 * B.r + l];
+    if (Mc > INT32_MAX || Nc > INT32_MAX) fail(QSX_ERR_UNSUPPORTED, "output block too large");
+    // output normalisation of NormCGW folded into afac: cgw -> normExt, data /= normExt
+    std::vector<int32_t> rec(P.L.QC.begin() + (size_t)c * mc, P.L.QC.begin() + (size_t)(c + 1) * mc);
+    double cfac = 1.0;
+    std::vector<double> cgw(A.nsym, 1.0);
+    for (int j = 0; j < A.nsym; ++j) {
+      if (A.symtype[j] != SYM_SU2) continue;
+      std::vector<int> tj(rc);
+      for (int l = 0; l < rc; ++l) tj[l] = rec[l * q + j];
+      double ne = su2_norm_ext(tj, P.dirC);
+      cgw[j] = ne;
+      cfac /= ne;
+    }
+    for (int64_t p = p0; p < p1; ++p) {
+      if (P.afac[p] == 0) continue;
+      const int ia = P.L.Ia[p], ib = P.L.Ib[p];
+      int64_t K = 1, Mp = 1, Np = 1;
+      for (int l = 0; l < k; ++l) {
+        int64_t da = A.dims[(size_t)ia * A.r + P.ica[l]], db = B.dims[(size_t)ib * B.r + P.icb[l]];
+        if (da != db) fail(QSX_ERR_STRUCT, "contracted block dims differ (%ld/%ld)", (long)da, (long)db);
+        K *= da;
+      }
+      for (int l : P.ka) Mp *= A.dims[(size_t)ia * A.r + l];
+      for (int l : P.kb) Np *= B.dims[(size_t)ib * B.r + l];
+      if (Mp != Mc || Np != Nc) fail(QSX_ERR_STRUCT, "inconsistent output block dims within block %d", c + 1);
+      if (K > INT32_MAX) fail(QSX_ERR_UNSUPPORTED, "contracted dim too large");
+      PairDesc pd{};
+      pd.a_off = A.offs[ia];
+      pd.b_off = B.offs[ib];
+      pd.K = (int32_t)K;
+      pd.flags = (P.la.kmajor ? 1 : 0) | (P.lb.kmajor ? 2 : 0);
+      pd.lda = (int32_t)std::max<int64_t>(1, P.la.kmajor ? K : Mc);
+      pd.ldb = (int32_t)std::max<int64_t>(1, P.lb.kmajor ? Nc : K);
+      pd.afac = P.afac[p] * cfac;
+      P.pairs.push_back(pd);
+      P.pIa.push_back(ia);
+      P.pIb.push_back(ib);
+      P.flops += 2.0 * (double)Mc * (double)Nc * (double)K;
+    }
+    P.pseg.push_back((int64_t)P.pairs.size());
+    P.M.push_back((int32_t)Mc);
+    P.N.push_back((int32_t)Nc);
+    // output record / dims after the user permutation
+    std::vector<int32_t> r2(mc);
+    std::vector<int64_t> d2(rc);
+    for (int l = 0; l < rc; ++l) {
+      for (int j = 0; j < q; ++j) r2[l * q + j] = rec[P.perm[l] * q + j];
+      d2[l] = d[P.perm[l]];
+    }
+    P.Qout.insert(P.Qout.end(), r2.begin(), r2.end());
+    P.dimsOut.insert(P.dimsOut.end(), d2.begin(), d2.end());
+    P.offsOut.push_back(P.offsOut.back() + Mc * Nc);
+    P.cgwOut.insert(P.cgwOut.end(), cgw.begin(), cgw.end());
+    P.blkL.push_back(c);
+    ++P.nC;
+  }
+  if (P.has_perm && !P.tagsC.empty()) {
+    std::vector<uint64_t> t2(rc);
+    for (int l = 0; l < rc; ++l) t2[l] = P.tagsC[P.perm[l]];
+    P.tagsC = t2;
+  }
+}
+
+void plan_build_tiles(const CtrPlan& P, int policy, std::vector<TileDesc> (&tiles)[TILE_NCFG],
+                      const std::vector<int64_t>* c_offsets, int64_t pair_base) {
+  struct Item {
+    TileDesc t;
+    double cost;
+    int cfg;
+  };
+  std::vector<Item> items;
+  for (int c = 0; c < P.nC; ++c) {
+    const int M = P.M[c], N = P.N[c];
+    const int cfg = std::max(M, N) <= policy ? TILE_32 : TILE_64;
+    const int T = tile_dim(cfg);
+    double ksum = 0;
+    for (int64_t p = P.pseg[c]; p < P.pseg[c + 1]; ++p) ksum += P.pairs[p].K + 8;
+    for (int m0 = 0; m0 < std::max(M, 1); m0 += T)
+      for (int n0 = 0; n0 < std::max(N, 1); n0 += T) {
+        Item it;
+        it.t.c_off = c_offsets ? (*c_offsets)[c] : P.offsOut[c];
+        it.t.M = M;
+        it.t.N = N;
+        it.t.m0 = m0;
+        it.t.n0 = n0;
+        it.t.p0 = (int32_t)(pair_base + P.pseg[c]);
+        it.t.p1 = (int32_t)(pair_base + P.pseg[c + 1]);
+        it.cost = ksum * (double)T * T;
+        it.cfg = cfg;
+        items.push_back(it);
+      }
+  }
+  // longest first: the dispatcher hands out workgroups in index order
+  std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) { return a.cost > b.cost; });
+  for (auto& v : tiles) v.clear();
+  for (auto& it : items) tiles[it.cfg].push_back(it.t);
+}
+
+static std::vector<PermBlock> perm_blocks(const TensorDesc& T) {
+  std::vector<PermBlock> v(T.n);
+  for (int i = 0; i < T.n; ++i) {
+    v[i].src_off = T.offs[i];
+    v[i].dst_off = T.offs[i];
+    for (int k = 0; k < T.r; ++k) v[i].dims[k] = (int32_t)T.dims[(size_t)i * T.r + k];
+  }
+  return v;
+}
+
+void plan_upload(CtrPlan& P) {
+  hipStream_t s = default_stream();
+  dev_free(P.dpairs);
+  P.dpairs = (PairDesc*)dev_alloc(sizeof(PairDesc) * std::max<size_t>(1, P.pairs.size()));
+  if (!P.pairs.empty())
+    QSX_HIP(hipMemcpyAsync(P.dpairs, P.pairs.data(), sizeof(PairDesc) * P.pairs.size(), hipMemcpyHostToDevice, s));
+  std::vector<TileDesc> tiles[TILE_NCFG];
+  std::vector<int64_t> coffs(P.offsOut.begin(), P.offsOut.end());
+  plan_build_tiles(P, P.tile_policy, tiles, &coffs, 0);
+  for (int c = 0; c < TILE_NCFG; ++c) {
+    dev_free(P.dtiles[c]);
+    P.dtiles[c] = nullptr;
+    P.ntiles[c] = (int)tiles[c].size();
+    if (P.ntiles[c]) {
+      P.dtiles[c] = (TileDesc*)dev_alloc(sizeof(TileDesc) * tiles[c].size());
+      QSX_HIP(hipMemcpyAsync(P.dtiles[c], tiles[c].data(), sizeof(TileDesc) * tiles[c].size(), hipMemcpyHostToDevice, s));
+    }
+  }
+  auto upload_perm = [&](const TensorDesc& T, PermBlock*& d, int64_t& mx) {
+    std::vector<PermBlock> v = perm_blocks(T);
+    mx = 0;
+    for (int i = 0; i < T.n; ++i) mx = std::max(mx, T.numel(i));
+    dev_free(d);
+    d = (PermBlock*)dev_alloc(sizeof(PermBlock) * std::max<size_t>(1, v.size()));
+    if (!v.empty()) QSX_HIP(hipMemcpyAsync(d, v.data(), sizeof(PermBlock) * v.size(), hipMemcpyHostToDevice, s));
+  };
+  if (P.la.permute && !P.a_prepermuted) {
+    upload_perm(P.A, P.dpbA, P.maxA);
+    dev_free(P.dAperm);
+    P.dAperm = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, P.A.total()));
+  }
+  if (P.lb.permute && !P.b_prepermuted) {
+    upload_perm(P.B, P.dpbB, P.maxB);
+    dev_free(P.dBperm);
+    P.dBperm = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, P.B.total()));
+  }
+  if (P.has_perm) {
+    // unpermuted output blocks have dims [kept A | kept B]
+    std::vector<PermBlock> v(P.nC);
+    P.maxC = 0;
+    for (int c = 0; c < P.nC; ++c) {
+      v[c].src_off = P.offsOut[c];
+      v[c].dst_off = P.offsOut[c];
+      // recover unpermuted dims: d[perm[l]] = dimsOut[l]
+      for (int l = 0; l < P.rc; ++l) v[c].dims[P.perm[l]] = (int32_t)P.dimsOut[(size_t)c * P.rc + l];
+      P.maxC = std::max(P.maxC, P.offsOut[c + 1] - P.offsOut[c]);
+    }
+    dev_free(P.dpbC);
+    P.dpbC = (PermBlock*)dev_alloc(sizeof(PermBlock) * std::max<size_t>(1, v.size()));
+    if (!v.empty()) QSX_HIP(hipMemcpyAsync(P.dpbC, v.data(), sizeof(PermBlock) * v.size(), hipMemcpyHostToDevice, s));
+    dev_free(P.dCtmp);
+    P.dCtmp = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, P.offsOut.back()));
+  }
+  QSX_HIP(hipStreamSynchronize(s));
+}
+
+void plan_permute_operand(const CtrPlan& P, int which, const double* src, double* dst, hipStream_t s) {
+  const TensorDesc& T = which == 0 ? P.A : P.B;
+  const OperandLayout& L = which == 0 ? P.la : P.lb;
+  if (!L.permute) fail(QSX_ERR_ARG, "operand needs no permute");
+  std::vector<PermBlock> v = perm_blocks(T);
+  int64_t mx = 0;
+  for (int i = 0; i < T.n; ++i) mx = std::max(mx, T.numel(i));
+  PermBlock* d = (PermBlock*)dev_alloc(sizeof(PermBlock) * std::max<size_t>(1, v.size()));
+  if (!v.empty()) QSX_HIP(hipMemcpyAsync(d, v.data(), sizeof(PermBlock) * v.size(), hipMemcpyHostToDevice, s));
+  launch_permute_blocks(d, T.n, T.r, L.perm.data(), src, dst, mx, s);
+  QSX_HIP(hipStreamSynchronize(s));
+  dev_free(d);
+}
+
+std::unique_ptr<CtrPlan> plan_create(const qsx_tensor* A, const int32_t* ia, int na, int conjA, const qsx_tensor* B,
+                                     const int32_t* ib, int nb, int conjB, const int32_t* perm, int nperm,
+                                     const qsx_opts* o, bool upload) {
+  ensure_device();
+  auto P = std::make_unique<CtrPlan>();
+  P->A = make_desc(A, "A");
+  P->B = make_desc(B, "B");
+  if (!ia || !ib || na <= 0 || nb <= 0) fail(QSX_ERR_ARG, "empty contraction index");
+  P->ica.assign(ia, ia + na);
+  P->icb.assign(ib, ib + nb);
+  P->conjA = conjA ? 1 : 0;
+  P->conjB = conjB ? 1 : 0;
+  if (o) {
+    P->mode = o->mode;
+    P->nslices = o->nslices;
+    P->verbose = o->verbose;
+  }
+  if (P->mode < QSX_MODE_BATCHED || P->mode > QSX_MODE_SPLIT_F32) fail(QSX_ERR_ARG, "invalid mode %d", P->mode);
+  build_structure(*P, perm, nperm, o ? o->fermA : nullptr, o ? o->fermB : nullptr);
+  if (upload) plan_upload(*P);
+  return P;
+}
+
+void plan_execute(CtrPlan& P, const double* dA, const double* dB, double* dC, hipStream_t s) {
+  const double* a = dA;
+  const double* b = dB;
+  if (P.la.permute && !P.a_prepermuted) {
+    launch_permute_blocks(P.dpbA, P.A.n, P.A.r, P.la.perm.data(), dA, P.dAperm, P.maxA, s);
+    a = P.dAperm;
+  }
+  if (P.lb.permute && !P.b_prepermuted) {
+    launch_permute_blocks(P.dpbB, P.B.n, P.B.r, P.lb.perm.data(), dB, P.dBperm, P.maxB, s);
+    b = P.dBperm;
+  }
+  double* c = P.has_perm ? P.dCtmp : dC;
+  if (P.mode == QSX_MODE_PLAIN) {
+    for (int blk = 0; blk < P.nC; ++blk) {
+      bool first = true;
+      for (int64_t p = P.pseg[blk]; p < P.pseg[blk + 1]; ++p) {
+        launch_pair_gemm_f64(P.pairs[p], P.offsOut[blk], P.M[blk], P.N[blk], first ? 0 : 1, a, b, c, s);
+        first = false;
+      }
+      if (P.pseg[blk] == P.pseg[blk + 1]) launch_fill_zero(c + P.offsOut[blk], P.offsOut[blk + 1] - P.offsOut[blk], s);
+    }
+  } else if (P.mode == QSX_MODE_BATCHED) {
+    for (int cfg = TILE_NCFG - 1; cfg >= 0; --cfg)
+      launch_grouped_gemm_f64(cfg, P.dtiles[cfg], P.ntiles[cfg], P.dpairs, a, b, c, s);
+  } else {
+    for (int cfg = TILE_NCFG - 1; cfg >= 0; --cfg)
+      launch_grouped_gemm_split(P.mode, P.nslices, cfg, P.dtiles[cfg], P.ntiles[cfg], P.dpairs, a, b, c, s);
+  }
+  if (P.has_perm) launch_permute_blocks(P.dpbC, P.nC, P.rc, P.perm.data(), P.dCtmp, dC, P.maxC, s);
+}
+
+// ---------------------------------------------------------------- one-shot
+static qsx_tensor_out* make_out(const CtrPlan& P, const std::vector<int>& keep, const std::vector<double>* data) {
+  auto* o = (qsx_tensor_out*)calloc(1, sizeof(qsx_tensor_out));
+  const int q = P.A.qdim, rc = P.rc;
+  int n = (int)keep.size();
+  o->nblk = n;
+  o->rank = rc;
+  o->qdim = q;
+  o->nsym = P.A.nsym;
+  snprintf(o->qtype, sizeof o->qtype, "%s", P.A.qtype.c_str());
+  o->Q = (int32_t*)malloc(sizeof(int32_t) * std::max<size_t>(1, (size_t)n * rc * q));
+  o->dims = (int64_t*)malloc(sizeof(int64_t) * std::max<size_t>(1, (size_t)n * rc));
+  o->offs = (int64_t*)malloc(sizeof(int64_t) * (n + 1));
+  o->cgw = (double*)malloc(sizeof(double) * std::max<size_t>(1, (size_t)n * P.A.nsym));
+  o->itags = (uint64_t*)calloc(std::max(1, rc), sizeof(uint64_t));
+  o->offs[0] = 0;
+  for (int i = 0; i < n; ++i) {
+    int c = keep[i];
+    for (int l = 0; l < rc * q; ++l) o->Q[(size_t)i * rc * q + l] = P.Qout[(size_t)c * rc * q + l];
+    for (int l = 0; l < rc; ++l) o->dims[(size_t)i * rc + l] = P.dimsOut[(size_t)c * rc + l];
+    for (int j = 0; j < P.A.nsym; ++j) o->cgw[(size_t)i * P.A.nsym + j] = P.cgwOut[(size_t)c * P.A.nsym + j];
+    o->offs[i + 1] = o->offs[i] + (P.offsOut[c + 1] - P.offsOut[c]);
+  }
+  for (size_t l = 0; l < P.tagsC.size(); ++l) o->itags[l] = P.tagsC[l];
+  if (data) {
+    o->data = (double*)malloc(sizeof(double) * std::max<int64_t>(1, o->offs[n]));
+    for (int i = 0; i < n; ++i) {
+      int c = keep[i];
+      std::copy(data->begin() + P.offsOut[c], data->begin() + P.offsOut[c + 1], o->data + o->offs[i]);
+    }
+  }
+  o->npairs = (int64_t)P.pairs.size();
+  o->useful_flops = P.flops;
+  return o;
+}
+
+static void upload_tensor(const TensorDesc& T, double* d, hipStream_t s) {
+  if (!T.hdata) fail(QSX_ERR_ARG, "tensor without data");
+  std::vector<double> buf(std::max<int64_t>(1, T.total()));
+  for (int i = 0; i < T.n; ++i) {
+    if (!T.hdata[i] && T.numel(i)) fail(QSX_ERR_ARG, "null data block %d", i + 1);
+    if (T.numel(i)) memcpy(buf.data() + T.offs[i], T.hdata[i], sizeof(double) * T.numel(i));
+  }
+  QSX_HIP(hipMemcpyAsync(d, buf.data(), sizeof(double) * T.total(), hipMemcpyHostToDevice, s));
+  QSX_HIP(hipStreamSynchronize(s));
+}
+
+static qsx_tensor_out* contract_oneshot(const qsx_tensor* A, const int32_t* ia, int na, int conjA, const qsx_tensor* B,
+                                        const int32_t* ib, int nb, int conjB, const int32_t* perm, int nperm,
+                                        const qsx_opts* o) {
+  ensure_device();
+  hipStream_t s = default_stream();
+  if (A && B && (A->nblk == 0 || B->nblk == 0)) {  // empty input -> empty output
+    auto* out = (qsx_tensor_out*)calloc(1, sizeof(qsx_tensor_out));
+    out->offs = (int64_t*)calloc(1, sizeof(int64_t));
+    out->qdim = A->qdim;
+    return out;
+  }
+  auto P = plan_create(A, ia, na, conjA, B, ib, nb, conjB, perm, nperm, o, true);
+  double *dA = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, P->A.total()));
+  double *dB = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, P->B.total()));
+  double *dC = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, P->offsOut.back()));
+  int32_t* dz = nullptr;
+  int64_t* doffs = nullptr;
+  qsx_tensor_out* out = nullptr;
+  try {
+    upload_tensor(P->A, dA, s);
+    upload_tensor(P->B, dB, s);
+    hipEvent_t e0, e1;
+    QSX_HIP(hipEventCreate(&e0));
+    QSX_HIP(hipEventCreate(&e1));
+    QSX_HIP(hipEventRecord(e0, s));
+    plan_execute(*P, dA, dB, dC, s);
+    QSX_HIP(hipEventRecord(e1, s));
+    QSX_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    QSX_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    std::vector<int> keep(P->nC);
+    std::iota(keep.begin(), keep.end(), 0);
+    const int skip_zero = o ? o->skip_zero : 1;
+    if (skip_zero && P->nC) {
+      // SkipZeroData (eps=0): exact-zero blocks are dropped unless the result
+      // is a rank-2 block-diagonal matrix (QSpace.cc:3200-3270)
+      bool blockdiag = false;
+      const int q = P->A.qdim;
+      if (P->rc == 2) {
+        blockdiag = true;
+        for (int c = 0; c < P->nC && blockdiag; ++c)
+          for (int j = 0; j < q; ++j)
+            if (P->Qout[(size_t)c * 2 * q + j] != P->Qout[(size_t)c * 2 * q + q + j]) blockdiag = false;
+      }
+      if (!(P->rc == 2 && blockdiag)) {
+        dz = (int32_t*)dev_alloc(sizeof(int32_t) * P->nC);
+        doffs = (int64_t*)dev_alloc(sizeof(int64_t) * (P->nC + 1));
+        QSX_HIP(hipMemcpyAsync(doffs, P->offsOut.data(), sizeof(int64_t) * (P->nC + 1), hipMemcpyHostToDevice, s));
+        launch_block_zero_flags(doffs, P->nC, dC, dz, s);
+        std::vector<int32_t> z(P->nC);
+        QSX_HIP(hipMemcpyAsync(z.data(), dz, sizeof(int32_t) * P->nC, hipMemcpyDeviceToHost, s));
+        QSX_HIP(hipStreamSynchronize(s));
+        keep.clear();
+        for (int c = 0; c < P->nC; ++c)
+          if (!z[c]) keep.push_back(c);
+      }
+    }
+    std::vector<double> host(std::max<int64_t>(1, P->offsOut.back()));
+    QSX_HIP(hipMemcpyAsync(host.data(), dC, sizeof(double) * P->offsOut.back(), hipMemcpyDeviceToHost, s));
+    QSX_HIP(hipStreamSynchronize(s));
+    out = make_out(*P, keep, &host);
+    out->gpu_ms = ms;
+  } catch (...) {
+    dev_free(dA);
+    dev_free(dB);
+    dev_free(dC);
+    dev_free(dz);
+    dev_free(doffs);
+    throw;
+  }
+  dev_free(dA);
+  dev_free(dB);
+  dev_free(dC);
+  dev_free(dz);
+  dev_free(doffs);
+  return out;
+}
+
+}  // namespace qsx
+
+// ====================================================================== C-ABI
+using namespace qsx;
+
+struct qsx_ctr_plan {
+  std::unique_ptr<CtrPlan> p;
+};
+
+static void set_err(char* err, size_t errlen, const std::string& m) {
+  if (err && errlen) snprintf(err, errlen, "%s", m.c_str());
+}
+
+template <class F>
+static int guarded(char* err, size_t errlen, F&& f) {
+  try {
+    return f();
+  } catch (const Error& e) {
+    set_err(err, errlen, e.msg);
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_err(err, errlen, "out of host memory");
+    return QSX_ERR_NOMEM;
+  }
+}
+
+extern "C" {
+
+const char* qsx_version(void) { return "qsx 0.1 (gfx950)"; }
+
+int qsx_device_init(int device, char* err, size_t errlen) {
+  return guarded(err, errlen, [&]() -> int {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) fail(QSX_ERR_DEVICE, "no HIP device available");
+    if (device < 0 || device >= n) fail(QSX_ERR_ARG, "device %d out of range (%d devices)", device, n);
+    QSX_HIP(hipSetDevice(device));
+    ensure_device();
+    return 0;
+  });
+}
+
+int qsx_malloc_device(void** ptr, size_t bytes, char* err, size_t errlen) {
+  return guarded(err, errlen, [&]() -> int {
+    *ptr = dev_alloc(bytes);
+    return 0;
+  });
+}
+
+int qsx_free_device(void* ptr) {
+  dev_free(ptr);
+  return 0;
+}
+
+int qsx_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream) {
+  hipStream_t s = stream ? (hipStream_t)stream : default_stream();
+  if (hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s) != hipSuccess) return QSX_ERR_DEVICE;
+  return hipStreamSynchronize(s) == hipSuccess ? 0 : QSX_ERR_DEVICE;
+}
+
+int qsx_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream) {
+  hipStream_t s = stream ? (hipStream_t)stream : default_stream();
+  if (hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s) != hipSuccess) return QSX_ERR_DEVICE;
+  return hipStreamSynchronize(s) == hipSuccess ? 0 : QSX_ERR_DEVICE;
+}
+
+int qsx_memset_device(void* dst, int value, size_t bytes, void* stream) {
+  hipStream_t s = stream ? (hipStream_t)stream : default_stream();
+  return hipMemsetAsync(dst, value, bytes, s) == hipSuccess ? 0 : QSX_ERR_DEVICE;
+}
+
+int qsx_synchronize(void) {
+  if (g_stream && hipStreamSynchronize(g_stream) != hipSuccess) return QSX_ERR_DEVICE;
+  return hipDeviceSynchronize() == hipSuccess ? 0 : QSX_ERR_DEVICE;
+}
+
+int qsx_contract(const qsx_tensor* A, const int32_t* ia, int32_t na, int32_t conjA, const qsx_tensor* B,
+                 const int32_t* ib, int32_t nb, int32_t conjB, const int32_t* perm, int32_t nperm,
+                 const qsx_opts* opts, qsx_tensor_out** C, char* err, size_t errlen) {
+  return guarded(err, errlen, [&]() -> int {
+    if (!C) fail(QSX_ERR_ARG, "null output pointer");
+    *C = contract_oneshot(A, ia, na, conjA, B, ib, nb, conjB, perm, nperm, opts);
+    return 0;
+  });
+}
+
+void qsx_tensor_free(qsx_tensor_out* t) {
+  if (!t) return;
+  free(t->Q);
+  free(t->dims);
+  free(t->offs);
+  free(t->data);
+  free(t->itags);
+  free(t->cgw);
+  free(t);
+}
+
+int qsx_parse_ctridx(const char* s, int32_t* idx, int32_t maxlen, int32_t* conj) {
+  // ctrIdx::init (Source/wbindex.cc:482-530), Wb::Str2Idx (Source/wbio.c:182-297)
+  if (!s || !s[0]) return -1;
+  std::string t(s);
+  int c = 0;
+  while (!t.empty() && (t.back() == '*' || isspace((unsigned char)t.back()))) {
+    if (t.back() == '*') c ^= 1;
+    t.pop_back();
+  }
+  while (!t.empty() && (t.back() == ',' || t.back() == ';')) t.pop_back();
+  std::vector<std::string> tok;
+  std::string cur;
+  for (char ch : t) {
+    if (isspace((unsigned char)ch) || ch == ',' || ch == ';' || ch == '[' || ch == ']') {
+      if (!cur.empty()) tok.push_back(cur);
+      cur.clear();
+    } else if (isalnum((unsigned char)ch)) {
+      cur.push_back(ch);
+    } else {
+      return -2;
+    }
+  }
+  if (!cur.empty()) tok.push_back(cur);
+  std::vector<int> v;
+  if (tok.size() == 1) {
+    for (char ch : tok[0]) {
+      if (ch >= '0' && ch <= '9') v.push_back(ch - '0');
+      else if (ch >= 'A' && ch <= 'Z') v.push_back(ch - 'A' + 10);
+      else v.push_back(ch - 'a' + 36);
+    }
+  } else {
+    for (auto& x : tok) {
+      char* end = nullptr;
+      long y = strtol(x.c_str(), &end, 0);
+      if (*end) return -2;
+      v.push_back((int)y);
+    }
+  }
+  if (v.empty()) return -1;
+  std::vector<int> u(v);
+  std::sort(u.begin(), u.end());
+  if (std::adjacent_find(u.begin(), u.end()) != u.end()) return -3;
+  if ((int)v.size() > maxlen) return -4;
+  for (size_t i = 0; i < v.size(); ++i) {
+    if (v[i] < 1) return -5;
+    idx[i] = v[i] - 1;
+  }
+  if (conj) *conj = c;
+  return (int)v.size();
+}
+
+int qsx_packed_offsets(const qsx_tensor* T, int64_t* offs) {
+  try {
+    TensorDesc d = make_desc(T, "T");
+    std::copy(d.offs.begin(), d.offs.end(), offs);
+    return 0;
+  } catch (const Error& e) {
+    return e.code;
+  }
+}
+
+int qsx_ctr_plan_create(const qsx_tensor* A, const int32_t* ia, int32_t na, int32_t conjA, const qsx_tensor* B,
+                        const int32_t* ib, int32_t nb, int32_t conjB, const int32_t* perm, int32_t nperm,
+                        const qsx_opts* opts, qsx_ctr_plan** plan, char* err, size_t errlen) {
+  return guarded(err, errlen, [&]() -> int {
+    if (!plan) fail(QSX_ERR_ARG, "null plan pointer");
+    auto* h = new qsx_ctr_plan;
+    try {
+      h->p = plan_create(A, ia, na, conjA, B, ib, nb, conjB, perm, nperm, opts, true);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *plan = h;
+    return 0;
+  });
+}
+
+int qsx_ctr_plan_output(const qsx_ctr_plan* plan, qsx_tensor_out** C) {
+  if (!plan || !C) return QSX_ERR_ARG;
+  std::vector<int> keep(plan->p->nC);
+  std::iota(keep.begin(), keep.end(), 0);
+  *C = make_out(*plan->p, keep, nullptr);
+  return 0;
+}
+
+int qsx_ctr_plan_pairs(const qsx_ctr_plan* plan, int64_t* npairs, int64_t* nout, int32_t* ia, int32_t* ib,
+                       double* afac, int64_t* seg) {
+  if (!plan) return QSX_ERR_ARG;
+  const CtrPlan& P = *plan->p;
+  if (npairs) *npairs = (int64_t)P.pairs.size();
+  if (nout) *nout = P.nC;
+  if (ia) std::copy(P.pIa.begin(), P.pIa.end(), ia);
+  if (ib) std::copy(P.pIb.begin(), P.pIb.end(), ib);
+  if (afac)
+    for (size_t i = 0; i < P.pairs.size(); ++i) afac[i] = P.pairs[i].afac;
+  if (seg) std::copy(P.pseg.begin(), P.pseg.end(), seg);
+  return 0;
+}
+
+int qsx_ctr_execute(qsx_ctr_plan* plan, const double* dA, const double* dB, double* dC, void* stream) {
+  if (!plan) return QSX_ERR_ARG;
+  try {
+    plan_execute(*plan->p, dA, dB, dC, stream ? (hipStream_t)stream : default_stream());
+    return 0;
+  } catch (const Error& e) {
+    fprintf(stderr, "qsx_ctr_execute: %s\n", e.msg.c_str());
+    return e.code;
+  }
+}
+
+double qsx_ctr_plan_flops(const qsx_ctr_plan* plan) { return plan ? plan->p->flops : 0; }
+
+void qsx_ctr_plan_destroy(qsx_ctr_plan* plan) { delete plan; }
+
+double qsx_su2_cg(int32_t tj1, int32_t tm1, int32_t tj2, int32_t tm2, int32_t tj3, int32_t tm3) {
+  return su2_cg_exact(tj1, tm1, tj2, tm2, tj3, tm3);
+}
+
+int qsx_su2_x3(int32_t ra, const int32_t* tja, const int32_t* dira, int32_t rb, const int32_t* tjb,
+               const int32_t* dirb, const int32_t* ia, const int32_t* ib, int32_t nctr, double* x3, char* err,
+               size_t errlen) {
+  return guarded(err, errlen, [&]() -> int {
+    std::vector<int> a(tja, tja + ra), da(dira, dira + ra), b(tjb, tjb + rb), db(dirb, dirb + rb);
+    std::vector<int> ca(ia, ia + nctr), cb(ib, ib + nctr);
+    *x3 = su2_x3(a, da, b, db, ca, cb);
+    return 0;
+  });
+}
+
+}  // extern "C"

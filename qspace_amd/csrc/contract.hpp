@@ -1,0 +1,82 @@
+// contract.hpp -- contraction plans (structure half on host+device, data
+// half as grouped GEMM launches).
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "qsx_internal.hpp"
+
+namespace qsx {
+
+struct OperandLayout {
+  bool permute = false;        // a permuted packed copy is used
+  std::vector<int> perm;       // leg order of the copy (new leg k = old leg perm[k])
+  bool kmajor = false;         // A: stored KxM ; B: stored NxK
+};
+
+struct CtrPlan {
+  TensorDesc A, B;
+  std::vector<int> ica, icb, ka, kb;  // contracted legs (pair order used for K), kept legs
+  int conjA = 0, conjB = 0;
+  int mode = QSX_MODE_BATCHED;
+  int nslices = 0;
+  int verbose = 1;
+  std::vector<uint64_t> tagsC;        // output itags (after perm)
+  std::vector<int> dirA, dirB, dirC;  // effective directions (+1 in, -1 out); dirC before perm
+  // structure
+  LabelResult L;                      // all matched pairs grouped by output block
+  std::vector<double> afac;           // per pair of L (0 = skipped)
+  // executed output
+  int nC = 0, rc = 0;
+  std::vector<int32_t> Qout;          // nC x rc*qdim (after perm)
+  std::vector<int64_t> dimsOut;       // nC x rc (after perm)
+  std::vector<int64_t> offsOut;       // nC+1
+  std::vector<double> cgwOut;         // nC x nsym
+  std::vector<int> blkL;              // output block -> index into L's blocks
+  std::vector<int> perm;              // output permutation (identity if none)
+  bool has_perm = false;
+  // GEMM work
+  std::vector<PairDesc> pairs;        // executed pairs, grouped by output block
+  std::vector<int64_t> pseg;          // nC+1 ranges into pairs
+  std::vector<int32_t> pIa, pIb;      // executed pairs' block indices
+  std::vector<int32_t> M, N;          // per output block matrix dims
+  OperandLayout la, lb;
+  bool a_prepermuted = false, b_prepermuted = false;
+  double flops = 0;                   // useful flops (2MNK over executed pairs)
+  // device resources
+  PairDesc* dpairs = nullptr;
+  TileDesc* dtiles[TILE_NCFG] = {nullptr, nullptr};
+  int ntiles[TILE_NCFG] = {0, 0};
+  int tile_policy = 32;               // blocks with max(M,N) <= policy use 32x32 tiles
+  double* dAperm = nullptr;
+  double* dBperm = nullptr;
+  double* dCtmp = nullptr;
+  PermBlock* dpbA = nullptr;
+  PermBlock* dpbB = nullptr;
+  PermBlock* dpbC = nullptr;
+  int64_t maxA = 0, maxB = 0, maxC = 0;
+  ~CtrPlan();
+};
+
+std::unique_ptr<CtrPlan> plan_create(const qsx_tensor* A, const int32_t* ia, int na, int conjA, const qsx_tensor* B,
+                                     const int32_t* ib, int nb, int conjB, const int32_t* perm, int nperm,
+                                     const qsx_opts* o, bool upload = true);
+// build tile lists for the chosen policy and upload pairs/tiles
+void plan_upload(CtrPlan& P);
+void plan_build_tiles(const CtrPlan& P, int policy, std::vector<TileDesc> (&tiles)[TILE_NCFG],
+                      const std::vector<int64_t>* c_offsets = nullptr, int64_t pair_base = 0);
+void plan_execute(CtrPlan& P, const double* dA, const double* dB, double* dC, hipStream_t s);
+// permute an operand's packed device data into the plan's layout
+void plan_permute_operand(const CtrPlan& P, int which, const double* src, double* dst, hipStream_t s);
+
+// itag auto-matching (iTags::getCtr, Source/wbindex.cc:1512-1590)
+void match_itags(const TensorDesc& A, const TensorDesc& B, int conjA, int conjB, std::vector<int32_t>& ia,
+                 std::vector<int32_t>& ib, int& conjA_out);
+
+hipStream_t default_stream();
+void ensure_device();
+void* dev_alloc(size_t bytes);
+void dev_free(void* p);
+
+}  // namespace qsx

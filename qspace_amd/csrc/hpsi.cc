@@ -1,0 +1,742 @@
+// hpsi.cc -- two-site H*psi as two grouped-GEMM launches, replayed from a
+// hipGraph, and the FP64 Lanczos / Davidson loops around it.
+//
+// Reference: get_HPsi.m:54-57 (Class/@Hamilton1D/private),
+//   HPsi = contractQS({HK_L,Psi},HK_R) [+ contractQS({HK_L,'*',Psi},HK_R,'*')]
+// with itag auto-matching (contractQS_itags, Source/contractQS.cc:634-808);
+// the legacy C++ CPAT::HZTimes (Source/cpat.hh:487-527) reuses its pair
+// lists across iterations (check_contraction, cpat.hh:572-593) -- here the
+// whole plan (pairs, X-symbol factors, tiles, operand layouts) is built once
+// and the per-iteration work is two kernel launches:
+//   stage 1: T_t = HL_t . psi      for both terms t (one grouped launch)
+//   stage 2: Hpsi = sum_t T_t . HR_t  (one grouped launch; the pairs of both
+//            terms of an output block run in one workgroup, fixed order)
+// Environments are copied to the device once, in the layouts the GEMM
+// consumes (permuted copies made once at plan time).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <numeric>
+
+#include "contract.hpp"
+#include "vec.hpp"
+
+namespace qsx {
+
+struct HpsiPlan {
+  TensorDesc HL, psi, HR;
+  int nterm = 1;
+  int mode = QSX_MODE_BATCHED;
+  int nslices = 0;
+  std::unique_ptr<CtrPlan> s1[2], s2[2];
+  std::vector<int32_t> t1Q[2];
+  std::vector<int64_t> t1dims[2];
+  std::vector<uint64_t> t1tags[2];
+  int64_t N = 0;
+  double* dHL = nullptr;   // arena: [HL | HL permuted copies]
+  double* dHR = nullptr;   // arena: [HR copies in the stage-2 layouts]
+  double* dT = nullptr;    // arena: [T_0 | T_1]
+  int64_t hl_base[2] = {0, 0}, hr_base[2] = {0, 0}, t_base[2] = {0, 0};
+  int64_t t_total = 0, hr_total = 0, hl_total = 0;
+  std::vector<PairDesc> pairs1, pairs2;
+  std::vector<TileDesc> tiles1[TILE_NCFG], tiles2[TILE_NCFG];
+  PairDesc *dpairs1 = nullptr, *dpairs2 = nullptr;
+  // the same work with afac scaled to the orthonormal reduced basis
+  // x~ = sqrt(w) x, w = full-space norm of each psi block's CGT, in which
+  // H is a symmetric matrix (used by the eigensolvers)
+  PairDesc *dpairs1s = nullptr, *dpairs2s = nullptr;
+  std::vector<double> welem;  // per-element weight w of psi's packed layout
+  std::vector<double> wblk;
+  std::vector<PairDesc> pairs1s, pairs2s;
+  TileDesc* dtiles1[TILE_NCFG] = {nullptr, nullptr};
+  TileDesc* dtiles2[TILE_NCFG] = {nullptr, nullptr};
+  int policy = 32;
+  double flops = 0, bytes = 0;
+  // graph
+  bool use_graph = true;
+  hipGraphExec_t gexec = nullptr;
+  const double* g_psi = nullptr;
+  double* g_hpsi = nullptr;
+  bool g_sym = false;
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  float last_ms[2] = {0, 0};
+  ~HpsiPlan() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    dev_free(dHL);
+    dev_free(dHR);
+    dev_free(dT);
+    dev_free(dpairs1);
+    dev_free(dpairs2);
+    dev_free(dpairs1s);
+    dev_free(dpairs2s);
+    for (auto p : dtiles1) dev_free(p);
+    for (auto p : dtiles2) dev_free(p);
+    for (auto e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+};
+
+namespace {
+
+struct HostTensor {  // qsx_tensor view over owned vectors
+  std::vector<int32_t> Q;
+  std::vector<int64_t> dims;
+  std::vector<uint64_t> tags;
+  std::string qtype;
+  qsx_tensor t{};
+  void bind(int n, int r, int q) {
+    t.nblk = n;
+    t.rank = r;
+    t.qdim = q;
+    t.Q = Q.empty() ? nullptr : Q.data();
+    t.dims = dims.empty() ? nullptr : dims.data();
+    t.data = nullptr;
+    t.itags = tags.empty() ? nullptr : tags.data();
+    t.qtype = qtype.c_str();
+    t.cgw = nullptr;  // results carry the external normalisation
+  }
+};
+
+// multi-term grouped GEMM work list: output block c of term t's plan goes to
+// target block target[t][c]; pairs of all terms of one target run together
+void build_multi(const std::vector<CtrPlan*>& plans, const std::vector<int64_t>& a_base,
+                 const std::vector<int64_t>& b_base, const std::vector<std::vector<int>>& target,
+                 const std::vector<int64_t>& t_off, const std::vector<int>& tM, const std::vector<int>& tN, int policy,
+                 std::vector<PairDesc>& pairs, std::vector<TileDesc> (&tiles)[TILE_NCFG],
+                 std::vector<int>* pair_target = nullptr, std::vector<std::pair<int, int64_t>>* pair_src = nullptr) {
+  const int nt = (int)t_off.size();
+  std::vector<std::vector<std::pair<int, int>>> src(nt);  // (term, block)
+  for (size_t t = 0; t < plans.size(); ++t)
+    for (int c = 0; c < plans[t]->nC; ++c) {
+      int g = target[t][c];
+      if (g < 0) continue;
+      if (plans[t]->M[c] != tM[g] || plans[t]->N[c] != tN[g])
+        fail(QSX_ERR_STRUCT, "H*psi term %zu block %d: dims %dx%d differ from the target %dx%d", t, c, plans[t]->M[c],
+             plans[t]->N[c], tM[g], tN[g]);
+      src[g].push_back({(int)t, c});
+    }
+  pairs.clear();
+  if (pair_target) pair_target->clear();
+  if (pair_src) pair_src->clear();
+  struct Item {
+    TileDesc td;
+    double cost;
+    int cfg;
+  };
+  std::vector<Item> items;
+  for (int g = 0; g < nt; ++g) {
+    int64_t p0 = (int64_t)pairs.size();
+    double ksum = 0;
+    for (auto [t, c] : src[g]) {
+      const CtrPlan& P = *plans[t];
+      for (int64_t p = P.pseg[c]; p < P.pseg[c + 1]; ++p) {
+        PairDesc pd = P.pairs[p];
+        pd.a_off += a_base[t];
+        pd.b_off += b_base[t];
+        pairs.push_back(pd);
+        if (pair_target) pair_target->push_back(g);
+        if (pair_src) pair_src->push_back({t, p});
+        ksum += pd.K + 8;
+      }
+    }
+    int64_t p1 = (int64_t)pairs.size();
+    const int M = tM[g], N = tN[g];
+    const int cfg = std::max(M, N) <= policy ? TILE_32 : TILE_64;
+    const int T = tile_dim(cfg);
+    for (int m0 = 0; m0 < std::max(M, 1); m0 += T)
+      for (int n0 = 0; n0 < std::max(N, 1); n0 += T) {
+        Item it;
+        it.td.c_off = t_off[g];
+        it.td.M = M;
+        it.td.N = N;
+        it.td.m0 = m0;
+        it.td.n0 = n0;
+        it.td.p0 = (int32_t)p0;
+        it.td.p1 = (int32_t)p1;
+        it.cost = ksum * (double)T * T;
+        it.cfg = cfg;
+        items.push_back(it);
+      }
+  }
+  std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) { return a.cost > b.cost; });
+  for (auto& v : tiles) v.clear();
+  for (auto& it : items) tiles[it.cfg].push_back(it.td);
+}
+
+template <class T>
+T* upload_vec(const std::vector<T>& v, hipStream_t s) {
+  T* d = (T*)dev_alloc(sizeof(T) * std::max<size_t>(1, v.size()));
+  if (!v.empty()) QSX_HIP(hipMemcpyAsync(d, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, s));
+  return d;
+}
+
+void upload_blocks(const TensorDesc& T, double* d, hipStream_t s) {
+  if (!T.hdata) fail(QSX_ERR_ARG, "environment tensor without data");
+  std::vector<double> buf(std::max<int64_t>(1, T.total()));
+  for (int i = 0; i < T.n; ++i)
+    if (T.numel(i)) memcpy(buf.data() + T.offs[i], T.hdata[i], sizeof(double) * T.numel(i));
+  QSX_HIP(hipMemcpyAsync(d, buf.data(), sizeof(double) * T.total(), hipMemcpyHostToDevice, s));
+  QSX_HIP(hipStreamSynchronize(s));
+}
+
+std::string rec_key(const int32_t* r, int m) { return std::string((const char*)r, sizeof(int32_t) * m); }
+
+}  // namespace
+
+static void hpsi_launch(HpsiPlan& H, const double* dpsi, double* dhpsi, hipStream_t s, bool timed, bool sym = false) {
+  const PairDesc* p1 = sym ? H.dpairs1s : H.dpairs1;
+  const PairDesc* p2 = sym ? H.dpairs2s : H.dpairs2;
+  if (timed) QSX_HIP(hipEventRecord(H.ev[0], s));
+  for (int cfg = TILE_NCFG - 1; cfg >= 0; --cfg) {
+    if (H.mode == QSX_MODE_BATCHED || H.mode == QSX_MODE_PLAIN)
+      launch_grouped_gemm_f64(cfg, H.dtiles1[cfg], (int)H.tiles1[cfg].size(), p1, H.dHL, dpsi, H.dT, s);
+    else
+      launch_grouped_gemm_split(H.mode, H.nslices, cfg, H.dtiles1[cfg], (int)H.tiles1[cfg].size(), p1, H.dHL,
+                                dpsi, H.dT, s);
+  }
+  if (timed) QSX_HIP(hipEventRecord(H.ev[1], s));
+  for (int cfg = TILE_NCFG - 1; cfg >= 0; --cfg) {
+    if (H.mode == QSX_MODE_BATCHED || H.mode == QSX_MODE_PLAIN)
+      launch_grouped_gemm_f64(cfg, H.dtiles2[cfg], (int)H.tiles2[cfg].size(), p2, H.dT, H.dHR, dhpsi, s);
+    else
+      launch_grouped_gemm_split(H.mode, H.nslices, cfg, H.dtiles2[cfg], (int)H.tiles2[cfg].size(), p2, H.dT,
+                                H.dHR, dhpsi, s);
+  }
+  if (timed) QSX_HIP(hipEventRecord(H.ev[2], s));
+}
+
+// plain per-pair path (no batching, no graph): the baseline
+static void hpsi_plain(HpsiPlan& H, const double* dpsi, double* dhpsi, hipStream_t s) {
+  auto run = [&](const std::vector<PairDesc>& pairs, const std::vector<TileDesc>* tiles, const double* A,
+                 const double* B, double* C) {
+    // walk the output blocks once: tiles of one block share c_off/pair range
+    std::map<int64_t, const TileDesc*> blocks;
+    for (int cfg = 0; cfg < TILE_NCFG; ++cfg)
+      for (auto& t : tiles[cfg]) blocks.emplace(t.c_off, &t);
+    for (auto& [off, t] : blocks) {
+      if (t->p0 == t->p1) {
+        launch_fill_zero(C + off, (int64_t)t->M * t->N, s);
+        continue;
+      }
+      for (int p = t->p0; p < t->p1; ++p)
+        launch_pair_gemm_f64(pairs[p], off, t->M, t->N, p == t->p0 ? 0 : 1, A, B, C, s);
+    }
+  };
+  run(H.pairs1, H.tiles1, H.dHL, dpsi, H.dT);
+  run(H.pairs2, H.tiles2, H.dT, H.dHR, dhpsi);
+}
+
+static void hpsi_create(HpsiPlan& H, const qsx_tensor* HL, const qsx_tensor* psi, const qsx_tensor* HR,
+                        const qsx_opts* o, int hconj) {
+  ensure_device();
+  hipStream_t s = default_stream();
+  H.HL = make_desc(HL, "HL");
+  H.psi = make_desc(psi, "psi");
+  H.HR = make_desc(HR, "HR");
+  H.nterm = hconj ? 2 : 1;
+  if (o) {
+    H.mode = o->mode;
+    H.nslices = o->nslices;
+  }
+  H.N = H.psi.total();
+  qsx_opts so{};
+  so.mode = QSX_MODE_BATCHED;
+  so.verbose = 0;
+  so.skip_zero = 0;
+  so.fermA = o ? o->fermA : nullptr;  // exact +-1 factors of HL blocks (parity operators)
+  // stage 1 plans: ({HL [,'*'], psi}) with itag matching
+  for (int t = 0; t < H.nterm; ++t) {
+    std::vector<int32_t> ia, ib;
+    int cA = 0;
+    match_itags(H.HL, H.psi, t == 1, 0, ia, ib, cA);
+    H.s1[t] = plan_create(HL, ia.data(), (int)ia.size(), cA, psi, ib.data(), (int)ib.size(), 0, nullptr, 0, &so, false);
+    if (H.s1[t]->lb.permute) fail(QSX_ERR_UNSUPPORTED, "H*psi: psi needs a permute in stage 1");
+  }
+  // stage 2 plans: (T_t, HR [,'*'])
+  HostTensor Tt[2];
+  for (int t = 0; t < H.nterm; ++t) {
+    CtrPlan& P = *H.s1[t];
+    Tt[t].Q = P.Qout;
+    Tt[t].dims = P.dimsOut;
+    Tt[t].tags = P.tagsC;
+    Tt[t].qtype = P.A.qtype;
+    Tt[t].bind(P.nC, P.rc, P.A.qdim);
+    TensorDesc Td = make_desc(&Tt[t].t, "T");
+    std::vector<int32_t> ia, ib;
+    int cA = 0;
+    match_itags(Td, H.HR, 0, t == 1, ia, ib, cA);
+    qsx_opts s2o = so;
+    s2o.fermA = nullptr;
+    H.s2[t] = plan_create(&Tt[t].t, ia.data(), (int)ia.size(), cA, HR, ib.data(), (int)ib.size(), t == 1, nullptr, 0,
+                          &s2o, false);
+    if (H.s2[t]->la.permute) fail(QSX_ERR_UNSUPPORTED, "H*psi: intermediate needs a permute in stage 2");
+  }
+  // device arenas: HL (+ permuted copies), HR copies, T_t
+  H.hl_total = H.HL.total();
+  int64_t off = H.hl_total;
+  for (int t = 0; t < H.nterm; ++t) {
+    if (H.s1[t]->la.permute) {
+      H.hl_base[t] = off;
+      off += H.hl_total;
+    } else H.hl_base[t] = 0;
+  }
+  H.dHL = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, off));
+  upload_blocks(H.HL, H.dHL, s);
+  for (int t = 0; t < H.nterm; ++t)
+    if (H.s1[t]->la.permute) plan_permute_operand(*H.s1[t], 0, H.dHL, H.dHL + H.hl_base[t], s);
+  int64_t roff = 0;
+  for (int t = 0; t < H.nterm; ++t) {
+    H.hr_base[t] = roff;
+    roff += H.HR.total();
+  }
+  H.hr_total = roff;
+  H.dHR = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, roff));
+  {
+    double* tmp = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, H.HR.total()));
+    upload_blocks(H.HR, tmp, s);
+    for (int t = 0; t < H.nterm; ++t) {
+      if (H.s2[t]->lb.permute) plan_permute_operand(*H.s2[t], 1, tmp, H.dHR + H.hr_base[t], s);
+      else QSX_HIP(hipMemcpyAsync(H.dHR + H.hr_base[t], tmp, sizeof(double) * H.HR.total(), hipMemcpyDeviceToDevice, s));
+    }
+    QSX_HIP(hipStreamSynchronize(s));
+    dev_free(tmp);
+  }
+  int64_t toff = 0;
+  for (int t = 0; t < H.nterm; ++t) {
+    H.t_base[t] = toff;
+    toff += H.s1[t]->offsOut.back();
+  }
+  H.t_total = toff;
+  H.dT = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, toff));
+  // stage 1 work: every T_t block is its own target
+  {
+    std::vector<CtrPlan*> plans;
+    std::vector<int64_t> ab, bb, toffs;
+    std::vector<std::vector<int>> target;
+    std::vector<int> tM, tN;
+    for (int t = 0; t < H.nterm; ++t) {
+      CtrPlan& P = *H.s1[t];
+      plans.push_back(&P);
+      ab.push_back(H.hl_base[t]);
+      bb.push_back(0);
+      std::vector<int> tg(P.nC);
+      for (int c = 0; c < P.nC; ++c) {
+        tg[c] = (int)toffs.size();
+        toffs.push_back(H.t_base[t] + P.offsOut[c]);
+        tM.push_back(P.M[c]);
+        tN.push_back(P.N[c]);
+      }
+      target.push_back(tg);
+    }
+    std::vector<std::pair<int, int64_t>> src1;
+    build_multi(plans, ab, bb, target, toffs, tM, tN, H.policy, H.pairs1, H.tiles1, nullptr, &src1);
+    // weights of psi's blocks: prod_j cgw_j^2 (|CData|_F = 1)
+    std::vector<double> wblk(H.psi.n, 1.0);
+    {
+      std::vector<int> dir(H.psi.r, 1);
+      for (int i = 0; i < H.psi.r && !H.psi.itags.empty(); ++i) dir[i] = (H.psi.itags[i] & 0x80) ? -1 : 1;
+      for (int i = 0; i < H.psi.n; ++i)
+        for (int j = 0; j < H.psi.nsym; ++j) {
+          if (H.psi.symtype[j] != SYM_SU2) continue;
+          double c;
+          if (H.psi.cgw_given) c = H.psi.cgw[(size_t)i * H.psi.nsym + j];
+          else {
+            std::vector<int> tj(H.psi.r);
+            for (int l = 0; l < H.psi.r; ++l) tj[l] = H.psi.label(i, l, j);
+            c = su2_norm_ext(tj, dir);
+          }
+          wblk[i] *= c * c;
+        }
+    }
+    H.welem.assign(H.N, 1.0);
+    for (int i = 0; i < H.psi.n; ++i)
+      for (int64_t e = H.psi.offs[i]; e < H.psi.offs[i + 1]; ++e) H.welem[e] = wblk[i];
+    H.wblk = wblk;
+    H.pairs1s = H.pairs1;
+    for (size_t p = 0; p < H.pairs1s.size(); ++p) {
+      int ib = H.s1[src1[p].first]->pIb[src1[p].second];
+      H.pairs1s[p].afac /= std::sqrt(wblk[ib]);
+    }
+  }
+  // stage 2 work: targets are psi's blocks (H*psi in psi's packed layout)
+  {
+    const int q = H.psi.qdim, r = H.psi.r;
+    std::map<std::string, int> psi_idx;
+    for (int i = 0; i < H.psi.n; ++i) psi_idx[rec_key(&H.psi.Q[(size_t)i * r * q], r * q)] = i;
+    std::vector<CtrPlan*> plans;
+    std::vector<int64_t> ab, bb, toffs(H.psi.n);
+    std::vector<std::vector<int>> target;
+    std::vector<int> tM(H.psi.n), tN(H.psi.n);
+    for (int i = 0; i < H.psi.n; ++i) {
+      toffs[i] = H.psi.offs[i];
+      int64_t m = H.psi.dims[(size_t)i * r], n = 1;
+      for (int k = 1; k < r; ++k) n *= H.psi.dims[(size_t)i * r + k];
+      tM[i] = (int)m;
+      tN[i] = (int)n;
+    }
+    for (int t = 0; t < H.nterm; ++t) {
+      CtrPlan& P = *H.s2[t];
+      if (P.rc != r) fail(QSX_ERR_STRUCT, "H*psi result has rank %d, psi rank %d", P.rc, r);
+      for (int l = 0; l < r; ++l)
+        if (P.tagsC.size() == (size_t)r && H.psi.itags.size() == (size_t)r && P.tagsC[l] != H.psi.itags[l])
+          fail(QSX_ERR_ITAG, "H*psi result itags differ from psi's at leg %d", l + 1);
+      plans.push_back(&P);
+      ab.push_back(H.t_base[t]);
+      bb.push_back(H.hr_base[t]);
+      std::vector<int> tg(P.nC);
+      for (int c = 0; c < P.nC; ++c) {
+        auto it = psi_idx.find(rec_key(&P.Qout[(size_t)c * r * q], r * q));
+        if (it == psi_idx.end()) fail(QSX_ERR_STRUCT, "H*psi produces a block outside psi's block structure");
+        tg[c] = it->second;
+      }
+      target.push_back(tg);
+    }
+    std::vector<int> tgt2;
+    build_multi(plans, ab, bb, target, toffs, tM, tN, H.policy, H.pairs2, H.tiles2, &tgt2, nullptr);
+    H.pairs2s = H.pairs2;
+    for (size_t p = 0; p < H.pairs2s.size(); ++p) H.pairs2s[p].afac *= std::sqrt(H.wblk[tgt2[p]]);
+  }
+  H.dpairs1 = upload_vec(H.pairs1, s);
+  H.dpairs2 = upload_vec(H.pairs2, s);
+  H.dpairs1s = upload_vec(H.pairs1s, s);
+  H.dpairs2s = upload_vec(H.pairs2s, s);
+  for (int c = 0; c < TILE_NCFG; ++c) {
+    H.dtiles1[c] = upload_vec(H.tiles1[c], s);
+    H.dtiles2[c] = upload_vec(H.tiles2[c], s);
+  }
+  QSX_HIP(hipStreamSynchronize(s));
+  H.flops = 0;
+  for (int t = 0; t < H.nterm; ++t) H.flops += H.s1[t]->flops + H.s2[t]->flops;
+  // algorithmic bytes: unique environments + psi + H*psi (+ the T_t write and re-read)
+  H.bytes = 8.0 * ((double)H.HL.total() + (double)H.HR.total() + 2.0 * (double)H.N + 2.0 * (double)H.t_total);
+  for (auto& e : H.ev) QSX_HIP(hipEventCreate(&e));
+}
+
+static void hpsi_apply(HpsiPlan& H, const double* dpsi, double* dhpsi, hipStream_t s, bool sym = false) {
+  if (H.mode == QSX_MODE_PLAIN) {
+    if (sym) fail(QSX_ERR_ARG, "plain mode has no symmetric-basis variant");
+    hpsi_plain(H, dpsi, dhpsi, s);
+    return;
+  }
+  if (!H.use_graph) {
+    hpsi_launch(H, dpsi, dhpsi, s, true, sym);
+    return;
+  }
+  if (!H.gexec || H.g_psi != dpsi || H.g_hpsi != dhpsi || H.g_sym != sym) {
+    if (H.gexec) {
+      (void)hipGraphExecDestroy(H.gexec);
+      H.gexec = nullptr;
+    }
+    hipGraph_t g;
+    QSX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hpsi_launch(H, dpsi, dhpsi, s, false, sym);
+    QSX_HIP(hipStreamEndCapture(s, &g));
+    QSX_HIP(hipGraphInstantiate(&H.gexec, g, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(g);
+    H.g_psi = dpsi;
+    H.g_hpsi = dhpsi;
+    H.g_sym = sym;
+  }
+  QSX_HIP(hipGraphLaunch(H.gexec, s));
+}
+
+// ---------------------------------------------------------------- eigensolvers
+// symmetric eigen-decomposition by cyclic Jacobi (small dense matrices)
+static void jacobi_eig(int n, std::vector<double> a, std::vector<double>& w, std::vector<double>& v) {
+  v.assign((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) v[(size_t)i * n + i] = 1.0;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0;
+    for (int p = 0; p < n; ++p)
+      for (int q = p + 1; q < n; ++q) off += a[(size_t)p * n + q] * a[(size_t)p * n + q];
+    if (off < 1e-30) break;
+    for (int p = 0; p < n; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        double apq = a[(size_t)p * n + q];
+        if (std::fabs(apq) < 1e-300) continue;
+        double app = a[(size_t)p * n + p], aqq = a[(size_t)q * n + q];
+        double th = (aqq - app) / (2 * apq);
+        double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1));
+        double c = 1 / std::sqrt(t * t + 1), sn = t * c;
+        for (int k = 0; k < n; ++k) {
+          double akp = a[(size_t)k * n + p], akq = a[(size_t)k * n + q];
+          a[(size_t)k * n + p] = c * akp - sn * akq;
+          a[(size_t)k * n + q] = sn * akp + c * akq;
+        }
+        for (int k = 0; k < n; ++k) {
+          double apk = a[(size_t)p * n + k], aqk = a[(size_t)q * n + k];
+          a[(size_t)p * n + k] = c * apk - sn * aqk;
+          a[(size_t)q * n + k] = sn * apk + c * aqk;
+        }
+        for (int k = 0; k < n; ++k) {
+          double vkp = v[(size_t)k * n + p], vkq = v[(size_t)k * n + q];
+          v[(size_t)k * n + p] = c * vkp - sn * vkq;
+          v[(size_t)k * n + q] = sn * vkp + c * vkq;
+        }
+      }
+  }
+  w.resize(n);
+  for (int i = 0; i < n; ++i) w[i] = a[(size_t)i * n + i];
+}
+
+static int argmin(const std::vector<double>& w) {
+  return (int)(std::min_element(w.begin(), w.end()) - w.begin());
+}
+
+static void lanczos(HpsiPlan& H, const double* psi0, int niter, double* e0, double* ritz, double* gs) {
+  hipStream_t s = default_stream();
+  const int64_t N = H.N;
+  if (niter < 1) fail(QSX_ERR_ARG, "niter must be >= 1");
+  double* V = (double*)dev_alloc(sizeof(double) * N * (niter + 1));
+  double* w = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, N));
+  double* part = (double*)dev_alloc(sizeof(double) * DOT_BLOCKS * (niter + 2));
+  double* c1 = (double*)dev_alloc(sizeof(double) * (niter + 2));
+  double* c2 = (double*)dev_alloc(sizeof(double) * (niter + 2));
+  double* alpha = (double*)dev_alloc(sizeof(double) * (niter + 1));
+  double* beta = (double*)dev_alloc(sizeof(double) * (niter + 1));
+  double* nrm = (double*)dev_alloc(sizeof(double) * 2);
+  hipGraphExec_t exec = nullptr;
+  try {
+    std::vector<double> x0(N);
+    for (int64_t i = 0; i < N; ++i) x0[i] = psi0[i] * std::sqrt(H.welem[i]);
+    QSX_HIP(hipMemcpyAsync(w, x0.data(), sizeof(double) * N, hipMemcpyHostToDevice, s));
+    vec_mdot(w, 0, 1, w, N, part, nrm, 0, s);
+    vec_normalize(w, V, nrm, nullptr, N, s);
+    QSX_HIP(hipStreamSynchronize(s));
+    // the whole iteration sequence is captured once and replayed as one graph
+    hipGraph_t g;
+    QSX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    for (int j = 0; j < niter; ++j) {
+      double* vj = V + (int64_t)j * N;
+      hpsi_launch(H, vj, w, s, false, true);
+      vec_mdot(vj, 0, 1, w, N, part, alpha + j, 0, s);  // alpha_j = <v_j, H v_j>
+      // full re-orthogonalisation, classical Gram-Schmidt twice
+      vec_mdot(V, N, j + 1, w, N, part, c1, 0, s);
+      vec_gs_update(w, V, N, c1, j + 1, N, s);
+      vec_mdot(V, N, j + 1, w, N, part, c2, 0, s);
+      vec_gs_update(w, V, N, c2, j + 1, N, s);
+      vec_mdot(w, 0, 1, w, N, part, nrm, 0, s);
+      vec_normalize(w, V + (int64_t)(j + 1) * N, nrm, beta + j, N, s);
+    }
+    QSX_HIP(hipStreamEndCapture(s, &g));
+    QSX_HIP(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(g);
+    QSX_HIP(hipGraphLaunch(exec, s));
+    QSX_HIP(hipStreamSynchronize(s));
+    std::vector<double> b(niter + 1), al(niter);
+    QSX_HIP(hipMemcpy(b.data(), beta, sizeof(double) * niter, hipMemcpyDeviceToHost));
+    QSX_HIP(hipMemcpy(al.data(), alpha, sizeof(double) * niter, hipMemcpyDeviceToHost));
+    std::vector<double> wv, vv;
+    for (int k = 1; k <= niter; ++k) {
+      std::vector<double> T((size_t)k * k, 0.0);
+      for (int i = 0; i < k; ++i) {
+        T[(size_t)i * k + i] = al[i];
+        if (i + 1 < k) T[(size_t)i * k + i + 1] = T[(size_t)(i + 1) * k + i] = b[i];
+      }
+      jacobi_eig(k, T, wv, vv);
+      if (ritz) ritz[k - 1] = wv[argmin(wv)];
+    }
+    int im = argmin(wv);
+    if (e0) *e0 = wv[im];
+    if (gs) {
+      std::vector<double> y(niter);
+      for (int i = 0; i < niter; ++i) y[i] = vv[(size_t)i * niter + im];
+      double* dy = upload_vec(y, s);
+      vec_combine(w, V, N, dy, niter, N, s);
+      QSX_HIP(hipMemcpyAsync(gs, w, sizeof(double) * N, hipMemcpyDeviceToHost, s));
+      QSX_HIP(hipStreamSynchronize(s));
+      dev_free(dy);
+      for (int64_t i = 0; i < N; ++i) gs[i] /= std::sqrt(H.welem[i]);
+    }
+  } catch (...) {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    for (void* p : {(void*)V, (void*)w, (void*)part, (void*)c1, (void*)c2, (void*)alpha, (void*)beta, (void*)nrm})
+      dev_free(p);
+    throw;
+  }
+  if (exec) (void)hipGraphExecDestroy(exec);
+  for (void* p : {(void*)V, (void*)w, (void*)part, (void*)c1, (void*)c2, (void*)alpha, (void*)beta, (void*)nrm})
+    dev_free(p);
+}
+
+static void davidson(HpsiPlan& H, const double* psi0, int niter, double* e0, double* ritz, double* gs) {
+  hipStream_t s = default_stream();
+  const int64_t N = H.N;
+  if (niter < 1) fail(QSX_ERR_ARG, "niter must be >= 1");
+  const int mmax = niter + 1;
+  double* V = (double*)dev_alloc(sizeof(double) * N * mmax);
+  double* HV = (double*)dev_alloc(sizeof(double) * N * mmax);
+  double* u = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, N));
+  double* hu = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, N));
+  double* part = (double*)dev_alloc(sizeof(double) * DOT_BLOCKS * (mmax + 1));
+  double* c = (double*)dev_alloc(sizeof(double) * (mmax + 1));
+  double* nrm = (double*)dev_alloc(sizeof(double) * 2);
+  double* dy = (double*)dev_alloc(sizeof(double) * (mmax + 1));
+  try {
+    std::vector<double> x0(N);
+    for (int64_t i = 0; i < N; ++i) x0[i] = psi0[i] * std::sqrt(H.welem[i]);
+    QSX_HIP(hipMemcpyAsync(u, x0.data(), sizeof(double) * N, hipMemcpyHostToDevice, s));
+    vec_mdot(u, 0, 1, u, N, part, nrm, 0, s);
+    vec_normalize(u, V, nrm, nullptr, N, s);
+    hpsi_apply(H, V, HV, s, true);
+    std::vector<double> Hk(1);
+    vec_mdot(V, 0, 1, HV, N, part, c, 0, s);
+    QSX_HIP(hipMemcpyAsync(&Hk[0], c, sizeof(double), hipMemcpyDeviceToHost, s));
+    QSX_HIP(hipStreamSynchronize(s));
+    int k = 1;
+    std::vector<double> w, v;
+    for (int it = 0; it < niter; ++it) {
+      jacobi_eig(k, Hk, w, v);
+      int im = argmin(w);
+      double th = w[im];
+      if (ritz) ritz[it] = th;
+      if (e0) *e0 = th;
+      std::vector<double> y(k);
+      for (int i = 0; i < k; ++i) y[i] = v[(size_t)i * k + im];
+      QSX_HIP(hipMemcpyAsync(dy, y.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
+      vec_combine(u, V, N, dy, k, N, s);
+      vec_combine(hu, HV, N, dy, k, N, s);
+      if (it + 1 == niter) break;
+      // residual r = H u - theta u, orthogonalised twice against V
+      QSX_HIP(hipMemcpyAsync(c, &th, sizeof(double), hipMemcpyHostToDevice, s));
+      vec_axmy(hu, hu, u, c, N, s);
+      for (int rep = 0; rep < 2; ++rep) {
+        vec_mdot(V, N, k, hu, N, part, c, 0, s);
+        vec_gs_update(hu, V, N, c, k, N, s);
+      }
+      vec_mdot(hu, 0, 1, hu, N, part, nrm, 0, s);
+      double r2 = 0;
+      QSX_HIP(hipMemcpyAsync(&r2, nrm, sizeof(double), hipMemcpyDeviceToHost, s));
+      QSX_HIP(hipStreamSynchronize(s));
+      if (r2 < 1e-28) {
+        for (int j = it + 1; j < niter && ritz; ++j) ritz[j] = th;
+        break;
+      }
+      vec_normalize(hu, V + (int64_t)k * N, nrm, nullptr, N, s);
+      hpsi_apply(H, V + (int64_t)k * N, HV + (int64_t)k * N, s, true);
+      // grow the projected matrix by one row/column
+      vec_mdot(V, N, k + 1, HV + (int64_t)k * N, N, part, c, 0, s);
+      std::vector<double> col(k + 1);
+      QSX_HIP(hipMemcpyAsync(col.data(), c, sizeof(double) * (k + 1), hipMemcpyDeviceToHost, s));
+      QSX_HIP(hipStreamSynchronize(s));
+      std::vector<double> Hn((size_t)(k + 1) * (k + 1));
+      for (int i = 0; i < k; ++i)
+        for (int j = 0; j < k; ++j) Hn[(size_t)i * (k + 1) + j] = Hk[(size_t)i * k + j];
+      for (int i = 0; i <= k; ++i) {
+        Hn[(size_t)i * (k + 1) + k] = col[i];
+        Hn[(size_t)k * (k + 1) + i] = col[i];
+      }
+      Hk.swap(Hn);
+      ++k;
+    }
+    if (gs) {
+      QSX_HIP(hipMemcpyAsync(gs, u, sizeof(double) * N, hipMemcpyDeviceToHost, s));
+      QSX_HIP(hipStreamSynchronize(s));
+      for (int64_t i = 0; i < N; ++i) gs[i] /= std::sqrt(H.welem[i]);
+    }
+  } catch (...) {
+    for (void* p : {(void*)V, (void*)HV, (void*)u, (void*)hu, (void*)part, (void*)c, (void*)nrm, (void*)dy}) dev_free(p);
+    throw;
+  }
+  for (void* p : {(void*)V, (void*)HV, (void*)u, (void*)hu, (void*)part, (void*)c, (void*)nrm, (void*)dy}) dev_free(p);
+}
+
+}  // namespace qsx
+
+using namespace qsx;
+
+struct qsx_hpsi_plan {
+  std::unique_ptr<HpsiPlan> h;
+};
+
+static void set_err2(char* err, size_t errlen, const std::string& m) {
+  if (err && errlen) snprintf(err, errlen, "%s", m.c_str());
+}
+
+extern "C" {
+
+int qsx_hpsi_plan_create(const qsx_tensor* HL, const qsx_tensor* psi, const qsx_tensor* HR, const qsx_opts* opts,
+                         qsx_hpsi_plan** plan, char* err, size_t errlen) {
+  try {
+    if (!plan) fail(QSX_ERR_ARG, "null plan pointer");
+    auto* p = new qsx_hpsi_plan;
+    p->h = std::make_unique<HpsiPlan>();
+    try {
+      hpsi_create(*p->h, HL, psi, HR, opts, opts ? opts->hconj : 0);
+    } catch (...) {
+      delete p;
+      throw;
+    }
+    *plan = p;
+    return 0;
+  } catch (const Error& e) {
+    set_err2(err, errlen, e.msg);
+    return e.code;
+  }
+}
+
+int64_t qsx_hpsi_size(const qsx_hpsi_plan* plan) { return plan ? plan->h->N : 0; }
+double qsx_hpsi_flops(const qsx_hpsi_plan* plan) { return plan ? plan->h->flops : 0; }
+double qsx_hpsi_bytes(const qsx_hpsi_plan* plan) { return plan ? plan->h->bytes : 0; }
+
+int qsx_hpsi_apply(qsx_hpsi_plan* plan, const double* dpsi, double* dhpsi, void* stream) {
+  if (!plan) return QSX_ERR_ARG;
+  try {
+    hpsi_apply(*plan->h, dpsi, dhpsi, stream ? (hipStream_t)stream : default_stream());
+    return 0;
+  } catch (const Error& e) {
+    fprintf(stderr, "qsx_hpsi_apply: %s\n", e.msg.c_str());
+    return e.code;
+  }
+}
+
+int qsx_hpsi_weights(const qsx_hpsi_plan* plan, double* w) {
+  if (!plan || !w) return QSX_ERR_ARG;
+  std::copy(plan->h->welem.begin(), plan->h->welem.end(), w);
+  return 0;
+}
+
+int qsx_hpsi_use_graph(qsx_hpsi_plan* plan, int32_t on) {
+  if (!plan) return QSX_ERR_ARG;
+  plan->h->use_graph = on != 0;
+  return 0;
+}
+
+int qsx_hpsi_last_timing(const qsx_hpsi_plan* plan, double* ms2) {
+  if (!plan || !ms2) return QSX_ERR_ARG;
+  float a = 0, b = 0;
+  if (hipEventElapsedTime(&a, plan->h->ev[0], plan->h->ev[1]) != hipSuccess) return QSX_ERR_DEVICE;
+  if (hipEventElapsedTime(&b, plan->h->ev[1], plan->h->ev[2]) != hipSuccess) return QSX_ERR_DEVICE;
+  ms2[0] = a;
+  ms2[1] = b;
+  return 0;
+}
+
+void qsx_hpsi_plan_destroy(qsx_hpsi_plan* plan) { delete plan; }
+
+int qsx_lanczos(qsx_hpsi_plan* plan, const double* psi0, int32_t niter, double* e0, double* ritz, double* gs,
+                char* err, size_t errlen) {
+  try {
+    if (!plan || !psi0) fail(QSX_ERR_ARG, "null argument");
+    lanczos(*plan->h, psi0, niter, e0, ritz, gs);
+    return 0;
+  } catch (const Error& e) {
+    set_err2(err, errlen, e.msg);
+    return e.code;
+  }
+}
+
+int qsx_davidson(qsx_hpsi_plan* plan, const double* psi0, int32_t niter, double* e0, double* ritz, double* gs,
+                 char* err, size_t errlen) {
+  try {
+    if (!plan || !psi0) fail(QSX_ERR_ARG, "null argument");
+    davidson(*plan->h, psi0, niter, e0, ritz, gs);
+    return 0;
+  } catch (const Error& e) {
+    set_err2(err, errlen, e.msg);
+    return e.code;
+  }
+}
+
+}  // extern "C"

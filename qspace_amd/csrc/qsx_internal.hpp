@@ -1,0 +1,117 @@
+// qsx_internal.hpp -- internal types of the qsx library (host side).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/qsx.h"
+
+namespace qsx {
+
+struct Error {
+  int code;
+  std::string msg;
+};
+
+[[noreturn]] void fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+
+#define QSX_HIP(call)                                                                          \
+  do {                                                                                         \
+    hipError_t e_ = (call);                                                                    \
+    if (e_ != hipSuccess)                                                                      \
+      ::qsx::fail(QSX_ERR_DEVICE, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, __LINE__); \
+  } while (0)
+
+enum SymType { SYM_A = 0, SYM_SU2 = 1 };
+
+// Host view of one tensor (structure; data pointers optional)
+struct TensorDesc {
+  int n = 0, r = 0, qdim = 0, nsym = 0;
+  std::vector<int> symtype;
+  std::vector<int> symcol;           // int offset of each symmetry inside a leg's qdim ints
+  std::string qtype;
+  std::vector<int32_t> Q;            // n x r*qdim
+  std::vector<int64_t> dims;         // n x r
+  std::vector<int64_t> offs;         // n+1 packed element offsets
+  std::vector<uint64_t> itags;       // r or empty
+  std::vector<double> cgw;           // n x nsym
+  bool cgw_given = false;
+  const double* const* hdata = nullptr;  // host block pointers (may be null)
+  int64_t numel(int i) const { return offs[i + 1] - offs[i]; }
+  int64_t total() const { return offs.empty() ? 0 : offs.back(); }
+  bool has_cg() const {
+    for (int t : symtype)
+      if (t == SYM_SU2) return true;
+    return false;
+  }
+  int label(int blk, int leg, int col) const { return Q[(size_t)blk * r * qdim + (size_t)leg * qdim + col]; }
+};
+
+TensorDesc make_desc(const qsx_tensor* t, const char* name);
+void parse_qtype(const char* s, int qdim, std::vector<int>& st);
+
+// ---------------------------------------------------------------- labels
+// Device label engine (labels.hip): block matching and output grouping.
+struct LabelResult {
+  std::vector<int32_t> Ia, Ib;   // pairs grouped by output block, reference order
+  std::vector<int64_t> seg;      // nC+1
+  std::vector<int32_t> QC;       // nC x mc records
+  int mc = 0;                    // ints per output record
+};
+// QAc/QBc: contracted-label records (na x k, nb x k); QAk/QBk kept records.
+LabelResult device_label_engine(const std::vector<int32_t>& QAc, const std::vector<int32_t>& QAk, int na,
+                                const std::vector<int32_t>& QBc, const std::vector<int32_t>& QBk, int nb, int kc,
+                                int ka, int kb, hipStream_t stream);
+
+// ---------------------------------------------------------------- SU(2)
+struct Canon;  // canonical CGT description (su2.cc)
+double su2_cg_exact(int tj1, int tm1, int tj2, int tm2, int tj3, int tm3);
+double su2_x3(const std::vector<int>& tja, const std::vector<int>& dira, const std::vector<int>& tjb,
+              const std::vector<int>& dirb, const std::vector<int>& ia, const std::vector<int>& ib);
+double su2_norm_ext(const std::vector<int>& tj, const std::vector<int>& dir);
+
+// ---------------------------------------------------------------- kernels
+// Work descriptors of the grouped GEMM (gemm.hip)
+struct PairDesc {
+  int64_t a_off;   // element offset of the A operand block
+  int64_t b_off;   // element offset of the B operand block
+  int32_t K;       // contracted dimension
+  int32_t lda;     // leading dim of A as stored (M if A is MxK, K if KxM)
+  int32_t ldb;     // leading dim of B as stored (K if B is KxN, N if NxK)
+  int32_t flags;   // bit0: A stored KxM; bit1: B stored NxK
+  double afac;     // X-symbol x cgw x sign x output normalisation
+};
+
+struct TileDesc {
+  int64_t c_off;   // element offset of the output block
+  int32_t M, N;    // output block as MxN column-major (ldc = M)
+  int32_t m0, n0;  // tile origin
+  int32_t p0, p1;  // pair range of the output block
+};
+
+enum TileCfg { TILE_32 = 0, TILE_64 = 1, TILE_NCFG = 2 };
+int tile_dim(int cfg);
+
+void launch_grouped_gemm_f64(int cfg, const TileDesc* dtiles, int ntiles, const PairDesc* dpairs, const double* A,
+                             const double* B, double* C, hipStream_t s);
+void launch_pair_gemm_f64(const PairDesc& p, int64_t c_off, int M, int N, int beta, const double* A,
+                          const double* B, double* C, hipStream_t s);
+void launch_grouped_gemm_split(int mode, int nslices, int cfg, const TileDesc* dtiles, int ntiles,
+                               const PairDesc* dpairs, const double* A, const double* B, double* C, hipStream_t s);
+
+// generic block permute: dst block b = permute(src block b, perm) for all blocks
+struct PermBlock {
+  int64_t src_off, dst_off;
+  int32_t dims[QSX_MAX_RANK];  // source dims
+};
+void launch_permute_blocks(const PermBlock* dblocks, int nblk, int rank, const int* perm, const double* src,
+                           double* dst, int64_t max_numel, hipStream_t s);
+// per-block "all entries exactly zero" flags
+void launch_block_zero_flags(const int64_t* doffs, int nblk, const double* data, int32_t* flags, hipStream_t s);
+void launch_fill_zero(double* p, int64_t n, hipStream_t s);
+
+}  // namespace qsx

@@ -1,0 +1,118 @@
+// vec.hip -- FP64 vector kernels of the Lanczos / Davidson loops
+// (RunEigs Source/eigs2SiteQS.cc:98-334, update_psi_2site.m:223-391).
+// Reductions are two-stage with a fixed partition (DOT_BLOCKS partials
+// summed in index order) so every dot product is deterministic; scalars
+// stay in device memory so an iteration needs no host round trip and can
+// be captured in a hipGraph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "vec.hpp"
+
+namespace qsx {
+namespace {
+
+constexpr int RT = 256;
+
+__device__ __forceinline__ double block_sum(double v) {
+  __shared__ double red[RT / 64];
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  double s = 0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < RT / 64; ++i) s += red[i];
+  __syncthreads();
+  return s;  // valid in thread 0
+}
+
+// partial[j*DOT_BLOCKS + b] = sum over the b-th slice of V_j . w
+__global__ __launch_bounds__(RT) void k_mdot_partial(const double* __restrict__ V, int64_t ldv,
+                                                      const double* __restrict__ w, int64_t n,
+                                                      double* __restrict__ partial) {
+  const int j = blockIdx.y, b = blockIdx.x;
+  const double* v = V + (int64_t)j * ldv;
+  double s = 0;
+  for (int64_t i = (int64_t)b * RT + threadIdx.x; i < n; i += (int64_t)DOT_BLOCKS * RT) s += v[i] * w[i];
+  s = block_sum(s);
+  if (threadIdx.x == 0) partial[(int64_t)j * DOT_BLOCKS + b] = s;
+}
+
+// out[j] = (accumulate ? out[j] : 0) + sum_b partial[j][b]
+__global__ __launch_bounds__(RT) void k_mdot_final(const double* __restrict__ partial, int m, double* __restrict__ out,
+                                                    int accumulate) {
+  const int j = blockIdx.x;
+  double s = 0;
+  for (int b = threadIdx.x; b < DOT_BLOCKS; b += RT) s += partial[(int64_t)j * DOT_BLOCKS + b];
+  s = block_sum(s);
+  if (threadIdx.x == 0) out[j] = accumulate ? out[j] + s : s;
+}
+
+// w -= sum_j c[j] V_j   (j < m)
+__global__ __launch_bounds__(RT) void k_gs_update(double* __restrict__ w, const double* __restrict__ V, int64_t ldv,
+                                                   const double* __restrict__ c, int m, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < n; i += (int64_t)gridDim.x * RT) {
+    double s = 0;
+    for (int j = 0; j < m; ++j) s += c[j] * V[(int64_t)j * ldv + i];
+    w[i] -= s;
+  }
+}
+
+// y = sum_j c[j] V_j
+__global__ __launch_bounds__(RT) void k_combine(double* __restrict__ y, const double* __restrict__ V, int64_t ldv,
+                                                 const double* __restrict__ c, int m, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < n; i += (int64_t)gridDim.x * RT) {
+    double s = 0;
+    for (int j = 0; j < m; ++j) s += c[j] * V[(int64_t)j * ldv + i];
+    y[i] = s;
+  }
+}
+
+// beta = sqrt(*nrm2); dst = src / beta
+__global__ __launch_bounds__(RT) void k_normalize(const double* __restrict__ src, double* __restrict__ dst,
+                                                   const double* __restrict__ nrm2, double* __restrict__ beta,
+                                                   int64_t n) {
+  const double b = sqrt(*nrm2);
+  const double inv = b > 0 ? 1.0 / b : 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && beta) *beta = b;
+  for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < n; i += (int64_t)gridDim.x * RT) dst[i] = src[i] * inv;
+}
+
+// y = x - a * z  with scalar a on the device
+__global__ __launch_bounds__(RT) void k_axmy(double* __restrict__ y, const double* __restrict__ x,
+                                              const double* __restrict__ z, const double* __restrict__ a, int64_t n) {
+  const double s = *a;
+  for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < n; i += (int64_t)gridDim.x * RT) y[i] = x[i] - s * z[i];
+}
+
+int grid_for(int64_t n) { return (int)std::min<int64_t>(2048, std::max<int64_t>(1, (n + RT - 1) / RT)); }
+
+}  // namespace
+
+void vec_mdot(const double* V, int64_t ldv, int m, const double* w, int64_t n, double* partial, double* out,
+              int accumulate, hipStream_t s) {
+  if (m <= 0) return;
+  hipLaunchKernelGGL(k_mdot_partial, dim3(DOT_BLOCKS, m), dim3(RT), 0, s, V, ldv, w, n, partial);
+  hipLaunchKernelGGL(k_mdot_final, dim3(m), dim3(RT), 0, s, partial, m, out, accumulate);
+}
+
+void vec_gs_update(double* w, const double* V, int64_t ldv, const double* c, int m, int64_t n, hipStream_t s) {
+  if (m <= 0) return;
+  hipLaunchKernelGGL(k_gs_update, dim3(grid_for(n)), dim3(RT), 0, s, w, V, ldv, c, m, n);
+}
+
+void vec_combine(double* y, const double* V, int64_t ldv, const double* c, int m, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_combine, dim3(grid_for(n)), dim3(RT), 0, s, y, V, ldv, c, m, n);
+}
+
+void vec_normalize(const double* src, double* dst, const double* nrm2, double* beta, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_normalize, dim3(grid_for(n)), dim3(RT), 0, s, src, dst, nrm2, beta, n);
+}
+
+void vec_axmy(double* y, const double* x, const double* z, const double* a, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_axmy, dim3(grid_for(n)), dim3(RT), 0, s, y, x, z, a, n);
+}
+
+}  // namespace qsx

@@ -1,0 +1,19 @@
+// vec.hpp -- FP64 device vector operations of the eigensolvers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace qsx {
+
+constexpr int DOT_BLOCKS = 128;  // fixed partition of every reduction
+
+// out[j] (+)= <V_j, w> for j < m; partial needs m*DOT_BLOCKS doubles
+void vec_mdot(const double* V, int64_t ldv, int m, const double* w, int64_t n, double* partial, double* out,
+              int accumulate, hipStream_t s);
+void vec_gs_update(double* w, const double* V, int64_t ldv, const double* c, int m, int64_t n, hipStream_t s);
+void vec_combine(double* y, const double* V, int64_t ldv, const double* c, int m, int64_t n, hipStream_t s);
+void vec_normalize(const double* src, double* dst, const double* nrm2, double* beta, int64_t n, hipStream_t s);
+void vec_axmy(double* y, const double* x, const double* z, const double* a, int64_t n, hipStream_t s);
+
+}  // namespace qsx

@@ -1,0 +1,164 @@
+"""Seeded synthetic inputs for the five contraction workloads.
+
+The reference ships no data, so every input is generated: labels and block
+dims follow the workload definitions, block entries are FP64 N(0,1).  The
+random stream is numpy's legacy MT19937 ``RandomState`` (frozen across numpy
+versions).  SU(2) labels are 2S; U(1) labels are integer charges.
+
+Workloads (sizes at full scale):
+  u1_mps       U(1) rank-3 A(L,s,R), L charges -10..10 step 2 (dims U[4,32]),
+               s = +-1 (dim 1), R charges -11..11 step 2 (dims U[4,32]);
+               C = contractQS(A,'12*',A,'12').  The odd R charges are needed
+               for blocks to exist (qL+qs=qR with even qL and odd qs).
+  su2_env      SU(2) L(L,L') . A(L,s,R) . conj(A): 2S in 0..16 on both bonds
+               (17 labels x mean dim 36 ~ 600 multiplets), dims U[8,64],
+               local spin 1/2.
+  heis_hpsi    two-site H*psi of the spin-1/2 Heisenberg chain in SU(2), in the
+               enlarged-bond form psi(E1l,E2r) with rank-3 environments
+               HK(E,E',w); MPO bond w = {2S=0: dim 2, 2S=2: dim 1} (3 multiplets);
+               enlarged-bond sectors 2S = 0,2,4,... with lognormal(mu,sigma)
+               dims clipped to [1,dmax], drawn until sum(d) >= D*.
+  hubbard_hpsi SU(2)xU(1) ("A,SU2") Hubbard two-site H*psi, MPO bond
+               {(0,0): 2, (+1,1): 2, (-1,1): 2} (6 multiplets), sectors (Q,2S)
+               with Q in [-12,12], Q+2S even, ordered by |Q|+2S, lognormal dims.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .qspace import QSpace
+
+
+def _rng(seed):
+    return np.random.RandomState(seed)
+
+
+def _normal_block(rng, shape):
+    return np.asfortranarray(rng.standard_normal(int(np.prod(shape))).reshape(shape, order="F"))
+
+
+# ---------------------------------------------------------------------------
+def u1_mps(seed=1001, Lq=tuple(range(-10, 11, 2)), Rq=tuple(range(-11, 12, 2)), dmin=4, dmax=32):
+    """Rank-3 U(1) MPS tensor A(L,s,R) with qL + qs = qR."""
+    rng = _rng(seed)
+    dL = {q: int(rng.randint(dmin, dmax + 1)) for q in Lq}
+    dR = {q: int(rng.randint(dmin, dmax + 1)) for q in Rq}
+    Q = [[], [], []]
+    data = []
+    for qL in Lq:
+        for qs in (-1, 1):
+            qR = qL + qs
+            if qR not in dR:
+                continue
+            Q[0].append([qL]); Q[1].append([qs]); Q[2].append([qR])
+            data.append(_normal_block(rng, (dL[qL], 1, dR[qR])))
+    return QSpace([np.array(q) for q in Q], data, "A", ["L", "s", "R*"])
+
+
+# ---------------------------------------------------------------------------
+def su2_labels_dims(rng, labels, dmin, dmax):
+    return {q: int(rng.randint(dmin, dmax + 1)) for q in labels}
+
+
+def su2_mps(seed=1002, tS=tuple(range(0, 17)), dmin=8, dmax=64):
+    """Rank-3 SU(2) A(L,s,R): L in, s (2S=1) in, R out; blocks |S_L-S_R|=1/2."""
+    rng = _rng(seed)
+    dL = su2_labels_dims(rng, tS, dmin, dmax)
+    dR = su2_labels_dims(rng, tS, dmin, dmax)
+    Q = [[], [], []]
+    data = []
+    for a in tS:
+        for b in tS:
+            if abs(a - b) != 1:
+                continue
+            Q[0].append([a]); Q[1].append([1]); Q[2].append([b])
+            data.append(_normal_block(rng, (dL[a], 1, dR[b])))
+    A = QSpace([np.array(q) for q in Q], data, "SU2", ["L", "s", "R*"])
+    L = QSpace([np.array([[a] for a in tS]), np.array([[a] for a in tS])],
+               [_normal_block(rng, (dL[a], dL[a])) for a in tS], "SU2", ["L", "L*"])
+    return A, L
+
+
+# ---------------------------------------------------------------------------
+def lognormal_sectors(rng, labels, Dstar, mu, sigma, dmax):
+    """Assign lognormal dims round(exp(mu + sigma Z)) clipped to [1,dmax] to
+    labels in order until the dims sum to at least Dstar."""
+    out = []
+    tot = 0
+    for q in labels:
+        if tot >= Dstar:
+            break
+        d = int(np.clip(np.rint(np.exp(mu + sigma * rng.standard_normal())), 1, dmax))
+        out.append((q, d))
+        tot += d
+    if tot < Dstar:
+        raise ValueError("label set too small for D*")
+    return out
+
+
+def _env_blocks(rng, secA, secB, wsec, rule, scale=1.0):
+    """rank-3 environment blocks (a, a', w) for the label rule."""
+    Q = [[], [], []]
+    data = []
+    for qa, da in secA:
+        for qb, db in secB:
+            for qw, dw in wsec:
+                if rule(qa, qb, qw):
+                    Q[0].append(list(np.atleast_1d(qa)))
+                    Q[1].append(list(np.atleast_1d(qb)))
+                    Q[2].append(list(np.atleast_1d(qw)))
+                    data.append(scale * _normal_block(rng, (da, db, dw)))
+    return Q, data
+
+
+def _tri(a, b, c):
+    return abs(a - b) <= c <= a + b and (a + b + c) % 2 == 0
+
+
+def heis_hpsi(seed=1003, Dstar=2000, mu=3.0, sigma=0.8, dmax=256):
+    """SU(2) Heisenberg two-site H*psi operands: (HL, psi, HR)."""
+    rng = _rng(seed)
+    labels = list(range(0, 4000, 2))
+    E1 = lognormal_sectors(rng, labels, Dstar, mu, sigma, dmax)
+    E2 = [(q, int(np.clip(np.rint(np.exp(mu + sigma * rng.standard_normal())), 1, dmax))) for q, _ in E1]
+    wsec = [(0, 2), (2, 1)]
+    rule = lambda a, b, w: _tri(a, b, w)
+    QL, dL = _env_blocks(rng, E1, E1, wsec, rule)
+    HL = QSpace([np.array(q) for q in QL], dL, "SU2", ["E1l", "E1l*", "w"])
+    QR, dR = _env_blocks(rng, E2, E2, wsec, rule)
+    HR = QSpace([np.array(q) for q in QR], dR, "SU2", ["E2r", "E2r*", "w*"])
+    psi = QSpace([np.array([[q] for q, _ in E1]), np.array([[q] for q, _ in E2])],
+                 [_normal_block(rng, (d1, d2)) for (_, d1), (_, d2) in zip(E1, E2)], "SU2", ["E1l", "E2r"])
+    nrm = psi.norm()
+    psi = psi.with_packed(psi.packed() / nrm)
+    return HL, psi, HR
+
+
+def hubbard_hpsi(seed=1005, Dstar=4000, mu=3.2, sigma=0.9, dmax=384, qmax=12):
+    """SU(2)xU(1) Hubbard two-site H*psi operands: (HL, psi, HR) with qtype 'A,SU2'."""
+    rng = _rng(seed)
+    cand = [(Q, s) for Q in range(-qmax, qmax + 1) for s in range(0, 64) if (Q + s) % 2 == 0]
+    cand.sort(key=lambda x: (abs(x[0]) + x[1], x[1], x[0]))
+    E1 = lognormal_sectors(rng, cand, Dstar, mu, sigma, dmax)
+    E2 = [((-Q, s), int(np.clip(np.rint(np.exp(mu + sigma * rng.standard_normal())), 1, dmax))) for (Q, s), _ in E1]
+    wsec = [((0, 0), 2), ((1, 1), 2), ((-1, 1), 2)]
+    # HL(a in, a' out, w in): Q_a + Q_w = Q_a'; HR(b in, b' out, w out): Q_b = Q_b' + Q_w
+    ruleL = lambda a, b, w: a[0] + w[0] == b[0] and _tri(a[1], b[1], w[1])
+    ruleR = lambda a, b, w: a[0] == b[0] + w[0] and _tri(a[1], b[1], w[1])
+    QL, dL = _env_blocks(rng, E1, E1, wsec, ruleL)
+    HL = QSpace([np.array(q) for q in QL], dL, "A,SU2", ["E1l", "E1l*", "w"])
+    QR, dR = _env_blocks(rng, E2, E2, wsec, ruleR)
+    HR = QSpace([np.array(q) for q in QR], dR, "A,SU2", ["E2r", "E2r*", "w*"])
+    psi = QSpace([np.array([list(q) for q, _ in E1]), np.array([list(q) for q, _ in E2])],
+                 [_normal_block(rng, (d1, d2)) for (_, d1), (_, d2) in zip(E1, E2)], "A,SU2", ["E1l", "E2r"])
+    nrm = psi.norm()
+    psi = psi.with_packed(psi.packed() / nrm)
+    return HL, psi, HR
+
+
+def fermion_parity_signs(HL: QSpace) -> np.ndarray:
+    """Exact +-1 per HL block: parity (-1)^{Q_w} of the operator leg, the
+    Jordan-Wigner string an explicit parity operator Z contributes when a
+    fermionic operator acts through the left block."""
+    qw = HL.Q[2][:, 0]
+    return np.where(qw % 2 == 0, 1.0, -1.0)

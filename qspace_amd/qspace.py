@@ -1,0 +1,270 @@
+"""QSpace tensor layout and the ctypes marshalling of the qsx C-ABI structs.
+
+A QSpace is the reference's struct {Q, data, info} (Source/QSpace.hh:1455-1475,
+mx layout Source/QSpace.hh:3150-3364):
+
+* ``Q``     list of ``rank`` int32 arrays, each ``nblk x qdim`` (MATLAB's 1 x r
+            cell of label matrices; SU(2) labels are 2S);
+* ``data``  list of ``nblk`` column-major float64 blocks with ``rank`` dims;
+* ``info``  dict with ``qtype`` ("A", "SU2", "A,SU2", ...), ``itags`` (list of
+            leg tags, trailing '*' = outgoing leg, Source/wbindex.hh:632-700)
+            and ``cgw`` (nblk x nsym outer-multiplicity weights, or None for
+            the external normalisation sqrt(d) of CRef::normExt,
+            Source/clebsch.cc:4738-4760).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import re
+
+import numpy as np
+
+__all__ = [
+    "QSpace", "QsxTensor", "QsxTensorOut", "QsxOpts", "pack_itag", "unpack_itag",
+    "parse_ctridx", "qtype_syms", "to_c", "from_out",
+]
+
+
+class QsxTensor(C.Structure):
+    _fields_ = [
+        ("nblk", C.c_int32), ("rank", C.c_int32), ("qdim", C.c_int32),
+        ("Q", C.POINTER(C.c_int32)), ("dims", C.POINTER(C.c_int64)),
+        ("data", C.POINTER(C.POINTER(C.c_double))), ("itags", C.POINTER(C.c_uint64)),
+        ("qtype", C.c_char_p), ("cgw", C.POINTER(C.c_double)),
+    ]
+
+
+class QsxTensorOut(C.Structure):
+    _fields_ = [
+        ("nblk", C.c_int32), ("rank", C.c_int32), ("qdim", C.c_int32), ("nsym", C.c_int32),
+        ("Q", C.POINTER(C.c_int32)), ("dims", C.POINTER(C.c_int64)), ("offs", C.POINTER(C.c_int64)),
+        ("data", C.POINTER(C.c_double)), ("itags", C.POINTER(C.c_uint64)), ("cgw", C.POINTER(C.c_double)),
+        ("qtype", C.c_char * 32), ("npairs", C.c_int64), ("useful_flops", C.c_double),
+        ("gpu_ms", C.c_double),
+    ]
+
+
+class QsxOpts(C.Structure):
+    _fields_ = [
+        ("mode", C.c_int32), ("verbose", C.c_int32), ("nslices", C.c_int32), ("skip_zero", C.c_int32),
+        ("hconj", C.c_int32), ("fermA", C.POINTER(C.c_double)), ("fermB", C.POINTER(C.c_double)),
+    ]
+
+
+def pack_itag(tag: str) -> int:
+    """'L*' -> uint64 with the 8 chars little-endian and bit 7 of byte 0 set
+    for an outgoing leg (itag_::Conj, Source/wbindex.hh:632-700)."""
+    t = tag or ""
+    conj = 0
+    while t.endswith("*"):
+        conj ^= 1
+        t = t[:-1]
+    b = t.encode()[:8]
+    v = int.from_bytes(b.ljust(8, b"\0"), "little")
+    if conj:
+        v ^= 0x80
+    return v
+
+
+def unpack_itag(v: int) -> str:
+    conj = bool(v & 0x80)
+    b = (v & ~0x80).to_bytes(8, "little").rstrip(b"\0")
+    return b.decode(errors="replace") + ("*" if conj else "")
+
+
+def qtype_syms(qtype: str, qdim: int) -> list[str]:
+    if not qtype:
+        return ["A"] * qdim
+    s = [x.strip() for x in qtype.split(",")]
+    return ["A" if x in ("A", "U1") else x for x in s]
+
+
+def parse_ctridx(s) -> tuple[list[int], int]:
+    """ctr-idx -> (0-based indices, conj flag).  Accepts 1-based numeric
+    sequences or strings like '13', '1,3', '[1 3]', '13*' (ctrIdx::init,
+    Source/wbindex.cc:482-530 and Wb::Str2Idx, Source/wbio.c:182-297)."""
+    if isinstance(s, str):
+        t = s.rstrip()
+        conj = 0
+        while t.endswith("*"):
+            conj ^= 1
+            t = t[:-1].rstrip()
+        t = t.rstrip(",; ")
+        tok = [x for x in re.split(r"[\s,;\[\]]+", t) if x]
+        if len(tok) == 1 and re.fullmatch(r"[0-9A-Za-z]+", tok[0]):
+            # compact form: one alnum token, each char an index (0-9, A-Z=10.., a-z=36..)
+            idx = []
+            for c in tok[0]:
+                if c.isdigit():
+                    idx.append(int(c))
+                elif "A" <= c <= "Z":
+                    idx.append(ord(c) - ord("A") + 10)
+                else:
+                    idx.append(ord(c) - ord("a") + 36)
+        else:
+            idx = [int(x, 0) for x in tok]
+        if not idx or min(idx) < 1:
+            raise ValueError(f"invalid contraction index '{s}' (1-based)")
+        if len(set(idx)) != len(idx):
+            raise ValueError(f"invalid index set '{s}' (not unique)")
+        return [i - 1 for i in idx], conj
+    idx = [int(x) for x in np.atleast_1d(s)]
+    if not idx or min(idx) < 1:
+        raise ValueError("contraction index must be 1-based")
+    if len(set(idx)) != len(idx):
+        raise ValueError("contraction index not unique")
+    return [i - 1 for i in idx], 0
+
+
+class QSpace:
+    """Block-sparse symmetric tensor, reference struct layout."""
+
+    def __init__(self, Q, data, qtype: str = "", itags=None, cgw=None):
+        if isinstance(Q, np.ndarray) and Q.ndim == 2:
+            raise TypeError("Q must be a list of per-leg label matrices")
+        self.Q = [np.ascontiguousarray(np.asarray(q, dtype=np.int32)) for q in Q]
+        n = self.Q[0].shape[0] if self.Q else 0
+        self.Q = [q.reshape(n, -1) for q in self.Q]
+        r = len(self.Q)
+        self.data = []
+        for d in data:
+            a = np.asarray(d, dtype=np.float64)
+            if a.ndim < r:
+                a = a.reshape(a.shape + (1,) * (r - a.ndim), order="F")
+            self.data.append(np.asfortranarray(a))
+        self.info = {"qtype": qtype, "itags": list(itags) if itags is not None else [],
+                     "cgw": None if cgw is None else np.asarray(cgw, dtype=np.float64).reshape(n, -1)}
+        if len(self.data) != n:
+            raise ValueError(f"QSpace: {n} label records but {len(self.data)} data blocks")
+
+    # -- shape -------------------------------------------------------------
+    @property
+    def rank(self) -> int:
+        return len(self.Q)
+
+    @property
+    def nblk(self) -> int:
+        return len(self.data)
+
+    @property
+    def qdim(self) -> int:
+        return self.Q[0].shape[1] if self.Q else 1
+
+    @property
+    def qtype(self) -> str:
+        return self.info["qtype"]
+
+    @property
+    def itags(self) -> list[str]:
+        return self.info["itags"]
+
+    @property
+    def QIDX(self) -> np.ndarray:
+        if not self.Q:
+            return np.zeros((self.nblk, 0), np.int32)
+        return np.ascontiguousarray(np.concatenate(self.Q, axis=1))
+
+    def dims(self) -> np.ndarray:
+        return np.array([d.shape[: self.rank] for d in self.data], dtype=np.int64).reshape(self.nblk, self.rank)
+
+    def numel(self) -> int:
+        return int(sum(d.size for d in self.data))
+
+    def norm(self) -> float:
+        return float(np.sqrt(sum(float(np.sum(d * d)) for d in self.data)))
+
+    def packed(self) -> np.ndarray:
+        """blocks back to back, column-major (the device layout)."""
+        if not self.data:
+            return np.zeros(0)
+        return np.concatenate([d.ravel(order="F") for d in self.data])
+
+    def with_packed(self, v: np.ndarray) -> "QSpace":
+        out, o = [], 0
+        for d in self.data:
+            out.append(np.asarray(v[o: o + d.size]).reshape(d.shape, order="F"))
+            o += d.size
+        return QSpace(self.Q, out, self.qtype, self.itags, self.info["cgw"])
+
+    def copy(self) -> "QSpace":
+        return QSpace([q.copy() for q in self.Q], [d.copy(order="F") for d in self.data], self.qtype,
+                      list(self.itags), None if self.info["cgw"] is None else self.info["cgw"].copy())
+
+    def __repr__(self):
+        return f"QSpace(rank={self.rank}, nblk={self.nblk}, qtype='{self.qtype}', itags={self.itags})"
+
+
+def to_c(T: QSpace, with_data: bool = True):
+    """QSpace -> (QsxTensor, keepalive list)."""
+    keep = []
+    n, r = T.nblk, T.rank
+    Q = np.ascontiguousarray(T.QIDX, dtype=np.int32).reshape(-1)
+    if Q.size == 0:
+        Q = np.zeros(1, np.int32)
+    dims = np.ascontiguousarray(T.dims(), dtype=np.int64).reshape(-1)
+    if dims.size == 0:
+        dims = np.zeros(1, np.int64)
+    keep += [Q, dims]
+    ct = QsxTensor()
+    ct.nblk, ct.rank, ct.qdim = n, r, T.qdim
+    ct.Q = Q.ctypes.data_as(C.POINTER(C.c_int32))
+    ct.dims = dims.ctypes.data_as(C.POINTER(C.c_int64))
+    if with_data:
+        ptrs = (C.POINTER(C.c_double) * max(1, n))()
+        for i, d in enumerate(T.data):
+            a = np.asfortranarray(d, dtype=np.float64)
+            keep.append(a)
+            ptrs[i] = a.ctypes.data_as(C.POINTER(C.c_double))
+        keep.append(ptrs)
+        ct.data = C.cast(ptrs, C.POINTER(C.POINTER(C.c_double)))
+    else:
+        ct.data = None
+    if T.itags:
+        it = np.array([pack_itag(t) for t in T.itags], dtype=np.uint64)
+        keep.append(it)
+        ct.itags = it.ctypes.data_as(C.POINTER(C.c_uint64))
+    else:
+        ct.itags = None
+    qt = (T.qtype or "").encode()
+    keep.append(qt)
+    ct.qtype = qt
+    if T.info.get("cgw") is not None:
+        cg = np.ascontiguousarray(T.info["cgw"], dtype=np.float64).reshape(-1)
+        keep.append(cg)
+        ct.cgw = cg.ctypes.data_as(C.POINTER(C.c_double))
+    else:
+        ct.cgw = None
+    return ct, keep
+
+
+def from_out(o: QsxTensorOut, with_data: bool = True) -> QSpace:
+    n, r, q = o.nblk, o.rank, o.qdim
+    Qflat = np.ctypeslib.as_array(o.Q, shape=(max(1, n * r * q),))[: n * r * q].copy().reshape(n, r * q)
+    dims = np.ctypeslib.as_array(o.dims, shape=(max(1, n * r),))[: n * r].copy().reshape(n, r)
+    offs = np.ctypeslib.as_array(o.offs, shape=(n + 1,)).copy()
+    data = []
+    if with_data and o.data:
+        buf = np.ctypeslib.as_array(o.data, shape=(max(1, int(offs[-1])),))
+        for i in range(n):
+            shp = tuple(int(x) for x in dims[i]) if r else ()
+            data.append(buf[offs[i]: offs[i + 1]].copy().reshape(shp, order="F"))
+    else:
+        for i in range(n):
+            data.append(np.zeros(tuple(int(x) for x in dims[i]), order="F"))
+    itags = [unpack_itag(int(o.itags[i])) for i in range(r)] if o.itags else []
+    if itags and all(t == "" for t in itags):
+        itags = []
+    cgw = None
+    if o.nsym and o.cgw:
+        cgw = np.ctypeslib.as_array(o.cgw, shape=(max(1, n * o.nsym),))[: n * o.nsym].copy().reshape(n, o.nsym)
+    Qlist = [Qflat[:, k * q:(k + 1) * q] for k in range(r)]
+    T = QSpace(Qlist, data, o.qtype.decode(), itags, cgw) if r else _scalar_qspace(data, o.qtype.decode(), cgw)
+    return T
+
+
+def _scalar_qspace(data, qtype, cgw):
+    T = QSpace.__new__(QSpace)
+    T.Q = []
+    T.data = [np.asarray(d, dtype=np.float64) for d in data]
+    T.info = {"qtype": qtype, "itags": [], "cgw": cgw}
+    return T

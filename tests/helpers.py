@@ -1,0 +1,21 @@
+"""Comparison helpers shared by the parity tests."""
+import numpy as np
+
+
+def rel_err(A, B):
+    """relative Frobenius error over all blocks (same structure assumed)."""
+    num = sum(float(np.sum((a - b) ** 2)) for a, b in zip(A.data, B.data))
+    den = sum(float(np.sum(b * b)) for b in B.data)
+    return np.sqrt(num / max(den, 1e-300))
+
+
+def assert_same_structure(C, R):
+    """bit-exact Q records, block order, dims, itags, cgw of two QSpaces."""
+    assert C.rank == R.rank, (C.rank, R.rank)
+    assert C.nblk == R.nblk, (C.nblk, R.nblk)
+    if C.rank:
+        assert np.array_equal(C.QIDX, R.QIDX)
+        assert np.array_equal(C.dims(), R.dims())
+    assert C.itags == R.itags, (C.itags, R.itags)
+    if R.info["cgw"] is not None or C.info["cgw"] is not None:
+        assert np.array_equal(np.asarray(C.info["cgw"]), np.asarray(R.info["cgw"]))

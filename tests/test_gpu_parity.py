@@ -1,0 +1,132 @@
+"""HIP path vs the CPU oracle on the same seeded inputs (GPU tests).
+
+Bar: Q records, block order, dims, itags and cgw bit-exact; data within
+1e-12 relative Frobenius error (FP64 paths).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import qspace_amd as Q
+from qspace_amd import gen
+from helpers import assert_same_structure, rel_err
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def _both(A, ia, B, ib, perm=None, conjA=0, conjB=0, mode="batched"):
+    C = Q.api._contract_pair(A, ia, conjA, B, ib, conjB, perm, mode=mode)
+    R = O.contract(A, ia, B, ib, perm=perm, conjA=conjA, conjB=conjB)
+    return C, R
+
+
+def test_u1_mps_overlap(gpu):
+    """workload u1_mps at full size: C = contractQS(A,'12*',A,'12')."""
+    A = gen.u1_mps()
+    C = Q.contractQS(A, "12*", A, "12")
+    R = O.contract(A, [0, 1], A, [0, 1], conjA=1)
+    assert_same_structure(C, R)
+    assert rel_err(C, R) < TOL
+    assert C.itags == ["R", "R*"]
+
+
+@pytest.mark.parametrize("mode", ["batched", "plain"])
+def test_u1_modes_and_perm(gpu, mode):
+    A = gen.u1_mps(seed=7)
+    C, R = _both(A, [2], A, [2], perm=[2, 0, 1, 3], conjB=1, mode=mode)
+    assert_same_structure(C, R)
+    assert rel_err(C, R) < TOL
+
+
+def test_su2_env_small(gpu):
+    """SU(2) L.A then X.conj(A) at reduced size (2S in 0..6, dims 4..12)."""
+    A, L = gen.su2_mps(seed=11, tS=tuple(range(7)), dmin=4, dmax=12)
+    X, XR = _both(L, [1], A, [0])
+    assert_same_structure(X, XR)
+    assert rel_err(X, XR) < TOL
+    C, CR = _both(XR, [0, 1], A, [0, 1], conjB=1)
+    assert_same_structure(C, CR)
+    assert rel_err(C, CR) < TOL
+
+
+def test_su2_noncontiguous_legs(gpu):
+    """an operator on the middle (local) leg forces an operand permute."""
+    A, L = gen.su2_mps(seed=12, tS=tuple(range(6)), dmin=3, dmax=9)
+    rng = np.random.RandomState(4)
+    W = Q.QSpace([np.array([[1]]), np.array([[1]])], [rng.standard_normal((1, 1))], "SU2", ["s", "s*"])
+    C, R = _both(A, [1], W, [1])
+    assert_same_structure(C, R)
+    assert rel_err(C, R) < TOL
+    C2, R2 = _both(W, [1], A, [1])
+    assert_same_structure(C2, R2)
+    assert rel_err(C2, R2) < TOL
+
+
+def test_single_block_gemm_layouts(gpu):
+    """one block per operand, all four matricisations vs numpy (FP64)."""
+    rng = np.random.RandomState(9)
+    M, K, N = 37, 29, 41
+    for sa in ("mk", "km"):
+        for sb in ("kn", "nk"):
+            a = rng.standard_normal((M, K) if sa == "mk" else (K, M))
+            b = rng.standard_normal((K, N) if sb == "kn" else (N, K))
+            A = Q.QSpace([[[0]], [[0]]], [a], "A")
+            B = Q.QSpace([[[0]], [[0]]], [b], "A")
+            ia = [1] if sa == "mk" else [0]
+            ib = [0] if sb == "kn" else [1]
+            C = Q.api._contract_pair(A, ia, 0, B, ib, 0)
+            ref = (a if sa == "mk" else a.T) @ (b if sb == "kn" else b.T)
+            assert np.abs(C.data[0] - ref).max() / np.abs(ref).max() < 1e-14
+
+
+def _hpsi_oracle(HL, psi, HR, hconj=1):
+    T = O.contract(HL, [1], psi, [0], skip_zero=0)
+    H = O.contract(T, [1, 2], HR, [2, 1], skip_zero=0)
+    terms = [H]
+    if hconj:
+        T2 = O.contract(HL, [0], psi, [0], conjA=1, skip_zero=0)
+        H2 = O.contract(T2, [1, 2], HR, [2, 0], conjB=1, skip_zero=0)
+        terms.append(H2)
+    # map onto psi's block order
+    key = {tuple(r): i for i, r in enumerate(psi.QIDX)}
+    out = [np.zeros_like(d) for d in psi.data]
+    for Ht in terms:
+        for r, d in zip(Ht.QIDX, Ht.data):
+            out[key[tuple(r)]] += d
+    return psi.with_packed(np.concatenate([o.ravel(order="F") for o in out]))
+
+
+@pytest.mark.parametrize("hconj", [0, 1])
+def test_hpsi_heisenberg_small(gpu, hconj):
+    HL, psi, HR = gen.heis_hpsi(seed=21, Dstar=40, mu=1.5, sigma=0.5, dmax=12)
+    H = Q.HPsi(HL, psi, HR, hconj=hconj)
+    got = H.apply(psi.packed())
+    ref = _hpsi_oracle(HL, psi, HR, hconj).packed()
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < TOL
+
+
+def test_hpsi_plain_equals_batched(gpu):
+    HL, psi, HR = gen.heis_hpsi(seed=22, Dstar=60, mu=2.0, sigma=0.6, dmax=40)
+    v = psi.packed()
+    a = Q.HPsi(HL, psi, HR, hconj=1, mode="batched").apply(v)
+    b = Q.HPsi(HL, psi, HR, hconj=1, mode="plain").apply(v)
+    assert np.linalg.norm(a - b) / np.linalg.norm(b) < TOL
+
+
+def test_hpsi_hermitian_full_size(gpu):
+    """size-independent property at the D*=2000 workload: <x,Hy>_w = <Hx,y>_w
+    in the full-space (CGT-weighted) inner product."""
+    HL, psi, HR = gen.heis_hpsi(seed=1003, Dstar=2000)
+    H = Q.HPsi(HL, psi, HR, hconj=1)
+    w = H.weights()
+    rng = np.random.RandomState(5)
+    x = rng.standard_normal(H.N)
+    y = rng.standard_normal(H.N)
+    hx = H.apply(x)
+    hy = H.apply(y)
+    nrm = np.sqrt((w * hx) @ hx) * np.sqrt((w * y) @ y)
+    assert abs((w * x) @ hy - (w * hx) @ y) / nrm < 1e-12
+    # linearity
+    z = H.apply(2.0 * x - 3.0 * y)
+    assert np.linalg.norm(z - (2.0 * hx - 3.0 * hy)) / np.linalg.norm(z) < 1e-12
