@@ -123,6 +123,9 @@ int qsx_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream);
 int qsx_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream);
 int qsx_memset_device(void* dst, int value, size_t bytes, void* stream);
 int qsx_synchronize(void);
+/* device copy kernel of n doubles (a plain streaming kernel; also the
+   calibration reference for hardware byte counters) */
+int qsx_copy_device(double* dst, const double* src, int64_t n, void* stream);
 
 /* one-shot contraction: contractQS(A,ia,B,ib[,perm]) -------------------- */
 /* ia/ib are 0-based leg indices; conjA/conjB are the trailing '*' flags;
@@ -177,6 +180,10 @@ int qsx_hpsi_apply(qsx_hpsi_plan* plan, const double* dpsi, double* dhpsi, void*
    to 1): <x,y> = sum_i w_i x_i y_i is the inner product in which H*psi is
    symmetric; the eigensolvers work in the basis sqrt(w) x */
 int qsx_hpsi_weights(const qsx_hpsi_plan* plan, double* w);
+/* per-stage work of one H*psi: info[0..7] = flops stage 1, flops stage 2,
+   algorithmic bytes stage 1, stage 2, tiles stage 1, stage 2, pairs stage 1,
+   stage 2 */
+int qsx_hpsi_stage_info(const qsx_hpsi_plan* plan, double* info);
 /* capture/replay the H*psi in a hipGraph (0 = direct launches) */
 int qsx_hpsi_use_graph(qsx_hpsi_plan* plan, int32_t on);
 /* per-kernel device timing of the last apply: [stage1_ms, stage2_ms] */

@@ -219,59 +219,80 @@ static void match_index(const std::vector<int32_t>& QA, int na, const std::vecto
 // SU(2) Clebsch-Gordan data
 // ---------------------------------------------------------------------
 
-// full table <j1 m1; j2 m2 | j3 m3>, index i1 + d1*(i2 + d2*i3) with
-// i = j - m (index 0 <-> m=+j, clebsch_aux.cc:1048-1082); built from the
-// highest-weight state (J+ psi = 0) and repeated J- lowering, Condon-Shortley
-// phase <j1 j1; j2 j3-j1 | j3 j3> > 0.
+// Full tables <j1 m1; j2 m2 | J M> for every J of j1 (x) j2, index
+// i1 + d1*(i2 + d2*iM) with i = j - m (index 0 <-> m=+j,
+// clebsch_aux.cc:1048-1082).  Built level by level from M = j1+j2 down: each
+// irrep's state at level M is the J- lowering of its state at M+1,
+// re-orthogonalised against the higher irreps of the same level (lowering
+// alone amplifies rounding in the higher-J directions); a new irrep J enters
+// at M = J as the unit vector orthogonal to all higher irreps, with the
+// Condon-Shortley phase <j1 j1; j2 J-j1 | J J> > 0.
+static std::map<int, std::vector<double>> cg_all(int tj1, int tj2) {
+  const int d1 = tj1 + 1, d2 = tj2 + 1, n = d1 * d2;
+  const int tJmin = std::abs(tj1 - tj2), tJmax = tj1 + tj2;
+  auto lower = [](int tj, int tm) {  // <m-1|J-|m> with two-j ints
+    long double v = (long double)(tj * (tj + 2) - tm * (tm - 2)) / 4.0L;
+    return v > 0 ? std::sqrt(v) : 0.0L;
+  };
+  std::map<int, std::vector<double>> out;
+  std::map<int, std::vector<long double>> cur;  // state of irrep tJ at the current level
+  for (int tJ = tJmin; tJ <= tJmax; tJ += 2) out[tJ].assign((size_t)n * (tJ + 1), 0.0);
+  for (int tM = tJmax; tM >= -tJmax; tM -= 2) {
+    std::map<int, std::vector<long double>> nxt;
+    for (int tJ = tJmax; tJ >= std::max(tJmin, std::abs(tM)); tJ -= 2) {
+      std::vector<long double> v(n, 0.0L);
+      if (tJ == tM) {
+        // unit vector of the level orthogonal to the higher irreps: start
+        // from the basis state with m1 = j1 and project the others out
+        int i1 = 0, i2 = (tj2 - (tM - tj1)) / 2;
+        v[i1 + d1 * i2] = 1.0L;
+      } else {
+        const std::vector<long double>& u = cur[tJ];
+        for (int i2 = 0; i2 < d2; ++i2)
+          for (int i1 = 0; i1 < d1; ++i1) {
+            long double x = u[i1 + d1 * i2];
+            if (x == 0) continue;
+            int tm1 = tj1 - 2 * i1, tm2 = tj2 - 2 * i2;
+            if (i1 + 1 < d1) v[(i1 + 1) + d1 * i2] += x * lower(tj1, tm1);
+            if (i2 + 1 < d2) v[i1 + d1 * (i2 + 1)] += x * lower(tj2, tm2);
+          }
+      }
+      for (int rep = 0; rep < 2; ++rep)
+        for (auto& [tJp, w] : nxt) {  // higher irreps of this level
+          long double dot = 0;
+          for (int i = 0; i < n; ++i) dot += w[i] * v[i];
+          for (int i = 0; i < n; ++i) v[i] -= dot * w[i];
+        }
+      long double nrm = 0;
+      for (int i = 0; i < n; ++i) nrm += v[i] * v[i];
+      nrm = std::sqrt(nrm);
+      if (tJ == tM) {
+        int i2 = (tj2 - (tM - tj1)) / 2;
+        if (v[0 + d1 * i2] < 0) nrm = -nrm;  // Condon-Shortley phase
+      }
+      for (int i = 0; i < n; ++i) v[i] /= nrm;
+      const int iM = (tJ - tM) / 2;
+      std::vector<double>& T = out[tJ];
+      for (int i = 0; i < n; ++i) T[(size_t)i + (size_t)n * iM] = (double)v[i];
+      nxt[tJ] = std::move(v);
+    }
+    cur.swap(nxt);
+  }
+  return out;
+}
+
+static std::map<std::pair<int, int>, std::map<int, std::vector<double>>> g_cgall;
+static std::mutex g_cgall_mu;
+
 static std::vector<double> cg_table(int tj1, int tj2, int tj3) {
   int d1 = tj1 + 1, d2 = tj2 + 1, d3 = tj3 + 1;
-  std::vector<double> T((size_t)d1 * d2 * d3, 0.0);
-  if (tj3 > tj1 + tj2 || tj3 < std::abs(tj1 - tj2) || ((tj1 + tj2 + tj3) & 1)) return T;
-  auto raise = [](int tj, int tm) {  // <m+1|J+|m> with two-j ints
-    double v = (double)(tj * (tj + 2) - tm * (tm + 2)) / 4.0;
-    return v > 0 ? std::sqrt(v) : 0.0;
-  };
-  auto lower = [](int tj, int tm) {  // <m-1|J-|m>
-    double v = (double)(tj * (tj + 2) - tm * (tm - 2)) / 4.0;
-    return v > 0 ? std::sqrt(v) : 0.0;
-  };
-  // highest weight: coefficients c(tm1) with tm1 + tm2 = tj3
-  std::vector<long double> c(d1, 0.0L);
-  int tm1_lo = std::max(-tj1, tj3 - tj2), tm1_hi = std::min(tj1, tj3 + tj2);
-  c[(tj1 - tm1_lo) / 2] = 1.0L;
-  for (int tm1 = tm1_lo + 2; tm1 <= tm1_hi; tm1 += 2) {
-    long double prev = c[(tj1 - (tm1 - 2)) / 2];
-    long double a1 = raise(tj1, tm1 - 2), a2 = raise(tj2, tj3 - tm1);
-    c[(tj1 - tm1) / 2] = -prev * a1 / a2;
-  }
-  long double nrm = 0;
-  for (auto v : c) nrm += v * v;
-  nrm = std::sqrt(nrm);
-  if (c[0] < 0) nrm = -nrm;  // c[0] <-> m1 = j1 (present: tm1_hi == tj1)
-  std::vector<long double> cur((size_t)d1 * d2, 0.0L), nxt((size_t)d1 * d2);
-  for (int tm1 = tm1_lo; tm1 <= tm1_hi; tm1 += 2) {
-    int i1 = (tj1 - tm1) / 2, i2 = (tj2 - (tj3 - tm1)) / 2;
-    cur[i1 + (size_t)d1 * i2] = c[i1] / nrm;
-  }
-  for (int i3 = 0; i3 < d3; ++i3) {
-    int tm3 = tj3 - 2 * i3;
-    for (int i2 = 0; i2 < d2; ++i2)
-      for (int i1 = 0; i1 < d1; ++i1) T[i1 + (size_t)d1 * (i2 + (size_t)d2 * i3)] = (double)cur[i1 + (size_t)d1 * i2];
-    if (i3 + 1 == d3) break;
-    std::fill(nxt.begin(), nxt.end(), 0.0L);
-    for (int i2 = 0; i2 < d2; ++i2)
-      for (int i1 = 0; i1 < d1; ++i1) {
-        long double v = cur[i1 + (size_t)d1 * i2];
-        if (v == 0) continue;
-        int tm1 = tj1 - 2 * i1, tm2 = tj2 - 2 * i2;
-        if (i1 + 1 < d1) nxt[(i1 + 1) + (size_t)d1 * i2] += v * lower(tj1, tm1);
-        if (i2 + 1 < d2) nxt[i1 + (size_t)d1 * (i2 + 1)] += v * lower(tj2, tm2);
-      }
-    long double f = lower(tj3, tm3);
-    for (auto& v : nxt) v /= f;
-    std::swap(cur, nxt);
-  }
-  return T;
+  if (tj3 > tj1 + tj2 || tj3 < std::abs(tj1 - tj2) || ((tj1 + tj2 + tj3) & 1))
+    return std::vector<double>((size_t)d1 * d2 * d3, 0.0);
+  std::lock_guard<std::mutex> lk(g_cgall_mu);
+  auto key = std::make_pair(tj1, tj2);
+  auto it = g_cgall.find(key);
+  if (it == g_cgall.end()) it = g_cgall.emplace(key, cg_all(tj1, tj2)).first;
+  return it->second.at(tj3);
 }
 
 static std::map<std::tuple<int, int, int>, std::vector<double>> g_cg;
@@ -451,41 +472,44 @@ static double x3_dense(const std::vector<int>& tja, const std::vector<int>& dira
   std::vector<int> da(ra), db(rb);
   for (int i = 0; i < ra; ++i) da[i] = tja[i] + 1;
   for (int i = 0; i < rb; ++i) db[i] = tjb[i] + 1;
-  size_t Mc = 1, Kc = 1, Na = 1, Nb = 1;
+  size_t Mc = 1, Nb = 1;
   for (int i : ka) Mc *= da[i];
   for (int i : kb) Nb *= db[i];
-  for (int i = 0; i < k; ++i) Kc *= da[ia[i]];
-  (void)Na;
-  std::vector<size_t> sa(ra), sb(rb);
-  size_t s = 1;
-  for (int i = 0; i < ra; ++i) { sa[i] = s; s *= da[i]; }
-  s = 1;
-  for (int i = 0; i < rb; ++i) { sb[i] = s; s *= db[i]; }
-  auto offs = [](const std::vector<int>& legs, const std::vector<int>& d, const std::vector<size_t>& st) {
-    size_t n = 1;
-    for (int l : legs) n *= d[l];
-    std::vector<size_t> o(n);
-    std::vector<int> idx(legs.size(), 0);
-    for (size_t lin = 0; lin < n; ++lin) {
-      size_t rem = lin, off = 0;
-      for (size_t q = 0; q < legs.size(); ++q) {
-        idx[q] = (int)(rem % d[legs[q]]);
-        rem /= d[legs[q]];
-        off += idx[q] * st[legs[q]];
-      }
-      o[lin] = off;
+  // brute force over all non-zero entries: B's entries hashed by their
+  // contracted multi-index, every A entry meets its matching B entries
+  auto unrank = [](size_t lin, const std::vector<int>& d, std::vector<int>& idx) {
+    for (size_t q = 0; q < d.size(); ++q) {
+      idx[q] = (int)(lin % d[q]);
+      lin /= d[q];
     }
-    return o;
+  };
+  auto rank_of = [](const std::vector<int>& legs, const std::vector<int>& d, const std::vector<int>& idx) {
+    size_t r = 0, st = 1;
+    for (int l : legs) {
+      r += st * idx[l];
+      st *= d[l];
+    }
+    return r;
   };
   std::vector<int> ial(ia.begin(), ia.end()), ibl(ib.begin(), ib.end());
-  std::vector<size_t> oam = offs(ka, da, sa), obn = offs(kb, db, sb), oak = offs(ial, da, sa), obk = offs(ibl, db, sb);
+  std::map<size_t, std::vector<std::pair<size_t, double>>> bmap;  // contracted rank -> (kept rank, value)
+  std::vector<int> idx(std::max(ra, rb));
+  for (size_t lin = 0; lin < TB.size(); ++lin) {
+    if (TB[lin] == 0) continue;
+    std::vector<int> ix(rb);
+    unrank(lin, db, ix);
+    bmap[rank_of(ibl, db, ix)].push_back({rank_of(kb, db, ix), TB[lin]});
+  }
   std::vector<double> TAB(Mc * Nb, 0.0);
-  for (size_t n = 0; n < Nb; ++n)
-    for (size_t m = 0; m < Mc; ++m) {
-      double acc = 0;
-      for (size_t q = 0; q < Kc; ++q) acc += TA[oam[m] + oak[q]] * TB[obn[n] + obk[q]];
-      TAB[m + Mc * n] = acc;
-    }
+  for (size_t lin = 0; lin < TA.size(); ++lin) {
+    if (TA[lin] == 0) continue;
+    std::vector<int> ix(ra);
+    unrank(lin, da, ix);
+    auto it = bmap.find(rank_of(ial, da, ix));
+    if (it == bmap.end()) continue;
+    size_t m = rank_of(ka, da, ix);
+    for (auto& [n, v] : it->second) TAB[m + Mc * n] += TA[lin] * v;
+  }
   double nab0 = 0;
   for (double v : TAB) nab0 += v * v;
   if (nab0 < 1e-24) return 0.0;  // contraction vanishes (CR_CTR_ZERO)

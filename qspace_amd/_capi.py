@@ -47,6 +47,7 @@ def lib():
         "qsx_memcpy_d2h": ([vp, vp, sz, vp], C.c_int),
         "qsx_memset_device": ([vp, C.c_int, sz, vp], C.c_int),
         "qsx_synchronize": ([], C.c_int),
+        "qsx_copy_device": ([vp, vp, i64, vp], C.c_int),
         "qsx_contract": ([P(QsxTensor), P(i32), i32, i32, P(QsxTensor), P(i32), i32, i32, P(i32), i32, P(QsxOpts),
                           P(P(QsxTensorOut)), cs, sz], C.c_int),
         "qsx_tensor_free": ([P(QsxTensorOut)], None),
@@ -66,6 +67,7 @@ def lib():
         "qsx_hpsi_apply": ([vp, vp, vp, vp], C.c_int),
         "qsx_hpsi_use_graph": ([vp, i32], C.c_int),
         "qsx_hpsi_weights": ([vp, P(f64)], C.c_int),
+        "qsx_hpsi_stage_info": ([vp, P(f64)], C.c_int),
         "qsx_hpsi_last_timing": ([vp, P(f64)], C.c_int),
         "qsx_hpsi_plan_destroy": ([vp], None),
         "qsx_lanczos": ([vp, P(f64), i32, P(f64), P(f64), P(f64), cs, sz], C.c_int),
@@ -84,11 +86,11 @@ def lib():
 def exported_symbols() -> list[str]:
     """names declared in include/qsx.h that the loaded library must export."""
     return ["qsx_version", "qsx_device_init", "qsx_malloc_device", "qsx_free_device", "qsx_memcpy_h2d",
-            "qsx_memcpy_d2h", "qsx_memset_device", "qsx_synchronize", "qsx_contract", "qsx_tensor_free",
+            "qsx_memcpy_d2h", "qsx_memset_device", "qsx_synchronize", "qsx_copy_device", "qsx_contract", "qsx_tensor_free",
             "qsx_parse_ctridx", "qsx_packed_offsets", "qsx_ctr_plan_create", "qsx_ctr_plan_output",
             "qsx_ctr_plan_pairs", "qsx_ctr_execute", "qsx_ctr_plan_flops", "qsx_ctr_plan_destroy",
             "qsx_hpsi_plan_create", "qsx_hpsi_size", "qsx_hpsi_flops", "qsx_hpsi_bytes", "qsx_hpsi_apply",
-            "qsx_hpsi_use_graph", "qsx_hpsi_weights", "qsx_hpsi_last_timing", "qsx_hpsi_plan_destroy", "qsx_lanczos", "qsx_davidson",
+            "qsx_hpsi_use_graph", "qsx_hpsi_weights", "qsx_hpsi_stage_info", "qsx_hpsi_last_timing", "qsx_hpsi_plan_destroy", "qsx_lanczos", "qsx_davidson",
             "qsx_su2_cg", "qsx_su2_x3"]
 
 

@@ -247,6 +247,12 @@ class HPsi:
         self.flops = float(L.qsx_hpsi_flops(self.h))
         self.bytes = float(L.qsx_hpsi_bytes(self.h))
 
+    def stage_info(self) -> dict:
+        v = (C.c_double * 8)()
+        _capi.check(_capi.lib().qsx_hpsi_stage_info(self.h, v), None)
+        return {"flops": (v[0], v[1]), "bytes": (v[2], v[3]), "tiles": (int(v[4]), int(v[5])),
+                "pairs": (int(v[6]), int(v[7]))}
+
     def weights(self) -> np.ndarray:
         w = np.zeros(self.N)
         _capi.check(_capi.lib().qsx_hpsi_weights(self.h, w.ctypes.data_as(C.POINTER(C.c_double))), None)

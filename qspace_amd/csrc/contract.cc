@@ -134,7 +134,7 @@ void dev_free(void* p) {
 
 CtrPlan::~CtrPlan() {
   dev_free(dpairs);
-  for (auto& t : dtiles) dev_free(t);
+  dev_free(dtiles);
   dev_free(dAperm);
   dev_free(dBperm);
   dev_free(dCtmp);
@@ -523,7 +523,7 @@ static void build_structure(CtrPlan& P, const int32_t* perm, int nperm, const do
   }
 }
 
-void plan_build_tiles(const CtrPlan& P, int policy, std::vector<TileDesc> (&tiles)[TILE_NCFG],
+void plan_build_tiles(const CtrPlan& P, int policy, std::vector<TileDesc>& tiles,
                       const std::vector<int64_t>* c_offsets, int64_t pair_base) {
   struct Item {
     TileDesc t;
@@ -547,6 +547,8 @@ void plan_build_tiles(const CtrPlan& P, int policy, std::vector<TileDesc> (&tile
         it.t.n0 = n0;
         it.t.p0 = (int32_t)(pair_base + P.pseg[c]);
         it.t.p1 = (int32_t)(pair_base + P.pseg[c + 1]);
+        it.t.cfg = cfg;
+        it.t.pad = 0;
         it.cost = ksum * (double)T * T;
         it.cfg = cfg;
         items.push_back(it);
@@ -554,8 +556,8 @@ void plan_build_tiles(const CtrPlan& P, int policy, std::vector<TileDesc> (&tile
   }
   // longest first: the dispatcher hands out workgroups in index order
   std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) { return a.cost > b.cost; });
-  for (auto& v : tiles) v.clear();
-  for (auto& it : items) tiles[it.cfg].push_back(it.t);
+  tiles.clear();
+  for (auto& it : items) tiles.push_back(it.t);
 }
 
 static std::vector<PermBlock> perm_blocks(const TensorDesc& T) {
@@ -574,17 +576,15 @@ void plan_upload(CtrPlan& P) {
   P.dpairs = (PairDesc*)dev_alloc(sizeof(PairDesc) * std::max<size_t>(1, P.pairs.size()));
   if (!P.pairs.empty())
     QSX_HIP(hipMemcpyAsync(P.dpairs, P.pairs.data(), sizeof(PairDesc) * P.pairs.size(), hipMemcpyHostToDevice, s));
-  std::vector<TileDesc> tiles[TILE_NCFG];
+  std::vector<TileDesc> tiles;
   std::vector<int64_t> coffs(P.offsOut.begin(), P.offsOut.end());
   plan_build_tiles(P, P.tile_policy, tiles, &coffs, 0);
-  for (int c = 0; c < TILE_NCFG; ++c) {
-    dev_free(P.dtiles[c]);
-    P.dtiles[c] = nullptr;
-    P.ntiles[c] = (int)tiles[c].size();
-    if (P.ntiles[c]) {
-      P.dtiles[c] = (TileDesc*)dev_alloc(sizeof(TileDesc) * tiles[c].size());
-      QSX_HIP(hipMemcpyAsync(P.dtiles[c], tiles[c].data(), sizeof(TileDesc) * tiles[c].size(), hipMemcpyHostToDevice, s));
-    }
+  dev_free(P.dtiles);
+  P.dtiles = nullptr;
+  P.ntiles = (int)tiles.size();
+  if (P.ntiles) {
+    P.dtiles = (TileDesc*)dev_alloc(sizeof(TileDesc) * tiles.size());
+    QSX_HIP(hipMemcpyAsync(P.dtiles, tiles.data(), sizeof(TileDesc) * tiles.size(), hipMemcpyHostToDevice, s));
   }
   auto upload_perm = [&](const TensorDesc& T, PermBlock*& d, int64_t& mx) {
     std::vector<PermBlock> v = perm_blocks(T);
@@ -683,11 +683,9 @@ void plan_execute(CtrPlan& P, const double* dA, const double* dB, double* dC, hi
       if (P.pseg[blk] == P.pseg[blk + 1]) launch_fill_zero(c + P.offsOut[blk], P.offsOut[blk + 1] - P.offsOut[blk], s);
     }
   } else if (P.mode == QSX_MODE_BATCHED) {
-    for (int cfg = TILE_NCFG - 1; cfg >= 0; --cfg)
-      launch_grouped_gemm_f64(cfg, P.dtiles[cfg], P.ntiles[cfg], P.dpairs, a, b, c, s);
+    launch_grouped_gemm_f64(P.dtiles, P.ntiles, P.dpairs, a, b, c, s);
   } else {
-    for (int cfg = TILE_NCFG - 1; cfg >= 0; --cfg)
-      launch_grouped_gemm_split(P.mode, P.nslices, cfg, P.dtiles[cfg], P.ntiles[cfg], P.dpairs, a, b, c, s);
+    launch_grouped_gemm_split(P.mode, P.nslices, P.dtiles, P.ntiles, P.dpairs, a, b, c, s);
   }
   if (P.has_perm) launch_permute_blocks(P.dpbC, P.nC, P.rc, P.perm.data(), P.dCtmp, dC, P.maxC, s);
 }
@@ -887,6 +885,15 @@ int qsx_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream) {
 int qsx_memset_device(void* dst, int value, size_t bytes, void* stream) {
   hipStream_t s = stream ? (hipStream_t)stream : default_stream();
   return hipMemsetAsync(dst, value, bytes, s) == hipSuccess ? 0 : QSX_ERR_DEVICE;
+}
+
+int qsx_copy_device(double* dst, const double* src, int64_t n, void* stream) {
+  try {
+    launch_copy(dst, src, n, stream ? (hipStream_t)stream : default_stream());
+    return 0;
+  } catch (const Error& e) {
+    return e.code;
+  }
 }
 
 int qsx_synchronize(void) {

@@ -46,8 +46,8 @@ struct CtrPlan {
   double flops = 0;                   // useful flops (2MNK over executed pairs)
   // device resources
   PairDesc* dpairs = nullptr;
-  TileDesc* dtiles[TILE_NCFG] = {nullptr, nullptr};
-  int ntiles[TILE_NCFG] = {0, 0};
+  TileDesc* dtiles = nullptr;
+  int ntiles = 0;
   int tile_policy = 32;               // blocks with max(M,N) <= policy use 32x32 tiles
   double* dAperm = nullptr;
   double* dBperm = nullptr;
@@ -64,7 +64,7 @@ std::unique_ptr<CtrPlan> plan_create(const qsx_tensor* A, const int32_t* ia, int
                                      const qsx_opts* o, bool upload = true);
 // build tile lists for the chosen policy and upload pairs/tiles
 void plan_upload(CtrPlan& P);
-void plan_build_tiles(const CtrPlan& P, int policy, std::vector<TileDesc> (&tiles)[TILE_NCFG],
+void plan_build_tiles(const CtrPlan& P, int policy, std::vector<TileDesc>& tiles,
                       const std::vector<int64_t>* c_offsets = nullptr, int64_t pair_base = 0);
 void plan_execute(CtrPlan& P, const double* dA, const double* dB, double* dC, hipStream_t s);
 // permute an operand's packed device data into the plan's layout

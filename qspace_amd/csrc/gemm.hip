@@ -114,20 +114,28 @@ __device__ __forceinline__ void tile_store(const d4 (&acc)[TM / 32][TN / 32], do
       }
 }
 
-template <int TM, int TN>
+// one launch for all tiles: TileDesc.cfg selects the 64x64 or the 32x32
+// instantiation (uniform per workgroup); LDS is sized for the larger one
 __global__ __launch_bounds__(256) void k_grouped_gemm_f64(const TileDesc* __restrict__ tiles,
                                                           const PairDesc* __restrict__ pairs,
                                                           const double* __restrict__ A, const double* __restrict__ B,
                                                           double* __restrict__ C) {
-  __shared__ Smem<TM, TN> sm;
+  __shared__ Smem<64, 64> sm;
   const TileDesc t = tiles[blockIdx.x];
-  d4 acc[TM / 32][TN / 32];
+  if (t.cfg == TILE_64) {
+    d4 acc[2][2];
 #pragma unroll
-  for (int i = 0; i < TM / 32; ++i)
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < TN / 32; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
-  tile_accumulate<TM, TN>(sm, acc, pairs, t.p0, t.p1, t.M, t.N, t.m0, t.n0, A, B);
-  tile_store<TM, TN>(acc, C + t.c_off, t.M, t.N, t.m0, t.n0, 0);
+      for (int j = 0; j < 2; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+    tile_accumulate<64, 64>(sm, acc, pairs, t.p0, t.p1, t.M, t.N, t.m0, t.n0, A, B);
+    tile_store<64, 64>(acc, C + t.c_off, t.M, t.N, t.m0, t.n0, 0);
+  } else {
+    Smem<32, 32>& s32 = *reinterpret_cast<Smem<32, 32>*>(&sm);
+    d4 acc[1][1] = {{d4{0.0, 0.0, 0.0, 0.0}}};
+    tile_accumulate<32, 32>(s32, acc, pairs, t.p0, t.p1, t.M, t.N, t.m0, t.n0, A, B);
+    tile_store<32, 32>(acc, C + t.c_off, t.M, t.N, t.m0, t.n0, 0);
+  }
 }
 
 // plain path: one launch per pair; the grid covers the tiles of its output block
@@ -190,6 +198,10 @@ __global__ void k_block_zero_flags(const int64_t* __restrict__ offs, const doubl
   if (threadIdx.x == 0) flags[blockIdx.x] = nz ? 0 : 1;
 }
 
+__global__ void k_copy(double* __restrict__ d, const double* __restrict__ s, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) d[i] = s[i];
+}
+
 __global__ void k_fill_zero(double* p, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = 0.0;
 }
@@ -198,13 +210,10 @@ __global__ void k_fill_zero(double* p, int64_t n) {
 
 int tile_dim(int cfg) { return cfg == TILE_64 ? 64 : 32; }
 
-void launch_grouped_gemm_f64(int cfg, const TileDesc* dtiles, int ntiles, const PairDesc* dpairs, const double* A,
+void launch_grouped_gemm_f64(const TileDesc* dtiles, int ntiles, const PairDesc* dpairs, const double* A,
                              const double* B, double* C, hipStream_t s) {
   if (ntiles <= 0) return;
-  if (cfg == TILE_64)
-    hipLaunchKernelGGL((k_grouped_gemm_f64<64, 64>), dim3(ntiles), dim3(256), 0, s, dtiles, dpairs, A, B, C);
-  else
-    hipLaunchKernelGGL((k_grouped_gemm_f64<32, 32>), dim3(ntiles), dim3(256), 0, s, dtiles, dpairs, A, B, C);
+  hipLaunchKernelGGL(k_grouped_gemm_f64, dim3(ntiles), dim3(256), 0, s, dtiles, dpairs, A, B, C);
   QSX_HIP(hipGetLastError());
 }
 
@@ -230,6 +239,13 @@ void launch_permute_blocks(const PermBlock* dblocks, int nblk, int rank, const i
 void launch_block_zero_flags(const int64_t* doffs, int nblk, const double* data, int32_t* flags, hipStream_t s) {
   if (nblk <= 0) return;
   hipLaunchKernelGGL(k_block_zero_flags, dim3(nblk), dim3(256), 0, s, doffs, data, flags);
+  QSX_HIP(hipGetLastError());
+}
+
+void launch_copy(double* d, const double* src, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  int g = (int)std::min<int64_t>(8192, (n + 255) / 256);
+  hipLaunchKernelGGL(k_copy, dim3(g), dim3(256), 0, s, d, src, n);
   QSX_HIP(hipGetLastError());
 }
 

@@ -40,7 +40,7 @@ struct HpsiPlan {
   int64_t hl_base[2] = {0, 0}, hr_base[2] = {0, 0}, t_base[2] = {0, 0};
   int64_t t_total = 0, hr_total = 0, hl_total = 0;
   std::vector<PairDesc> pairs1, pairs2;
-  std::vector<TileDesc> tiles1[TILE_NCFG], tiles2[TILE_NCFG];
+  std::vector<TileDesc> tiles1, tiles2;
   PairDesc *dpairs1 = nullptr, *dpairs2 = nullptr;
   // the same work with afac scaled to the orthonormal reduced basis
   // x~ = sqrt(w) x, w = full-space norm of each psi block's CGT, in which
@@ -49,10 +49,11 @@ struct HpsiPlan {
   std::vector<double> welem;  // per-element weight w of psi's packed layout
   std::vector<double> wblk;
   std::vector<PairDesc> pairs1s, pairs2s;
-  TileDesc* dtiles1[TILE_NCFG] = {nullptr, nullptr};
-  TileDesc* dtiles2[TILE_NCFG] = {nullptr, nullptr};
+  TileDesc* dtiles1 = nullptr;
+  TileDesc* dtiles2 = nullptr;
   int policy = 32;
   double flops = 0, bytes = 0;
+  double stage_flops[2] = {0, 0}, stage_bytes[2] = {0, 0};
   // graph
   bool use_graph = true;
   hipGraphExec_t gexec = nullptr;
@@ -70,8 +71,8 @@ struct HpsiPlan {
     dev_free(dpairs2);
     dev_free(dpairs1s);
     dev_free(dpairs2s);
-    for (auto p : dtiles1) dev_free(p);
-    for (auto p : dtiles2) dev_free(p);
+    dev_free(dtiles1);
+    dev_free(dtiles2);
     for (auto e : ev)
       if (e) (void)hipEventDestroy(e);
   }
@@ -103,7 +104,7 @@ struct HostTensor {  // qsx_tensor view over owned vectors
 void build_multi(const std::vector<CtrPlan*>& plans, const std::vector<int64_t>& a_base,
                  const std::vector<int64_t>& b_base, const std::vector<std::vector<int>>& target,
                  const std::vector<int64_t>& t_off, const std::vector<int>& tM, const std::vector<int>& tN, int policy,
-                 std::vector<PairDesc>& pairs, std::vector<TileDesc> (&tiles)[TILE_NCFG],
+                 std::vector<PairDesc>& pairs, std::vector<TileDesc>& tiles,
                  std::vector<int>* pair_target = nullptr, std::vector<std::pair<int, int64_t>>* pair_src = nullptr) {
   const int nt = (int)t_off.size();
   std::vector<std::vector<std::pair<int, int>>> src(nt);  // (term, block)
@@ -154,14 +155,16 @@ void build_multi(const std::vector<CtrPlan*>& plans, const std::vector<int64_t>&
         it.td.n0 = n0;
         it.td.p0 = (int32_t)p0;
         it.td.p1 = (int32_t)p1;
+        it.td.cfg = cfg;
+        it.td.pad = 0;
         it.cost = ksum * (double)T * T;
         it.cfg = cfg;
         items.push_back(it);
       }
   }
   std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) { return a.cost > b.cost; });
-  for (auto& v : tiles) v.clear();
-  for (auto& it : items) tiles[it.cfg].push_back(it.td);
+  tiles.clear();
+  for (auto& it : items) tiles.push_back(it.td);
 }
 
 template <class T>
@@ -188,32 +191,25 @@ static void hpsi_launch(HpsiPlan& H, const double* dpsi, double* dhpsi, hipStrea
   const PairDesc* p1 = sym ? H.dpairs1s : H.dpairs1;
   const PairDesc* p2 = sym ? H.dpairs2s : H.dpairs2;
   if (timed) QSX_HIP(hipEventRecord(H.ev[0], s));
-  for (int cfg = TILE_NCFG - 1; cfg >= 0; --cfg) {
-    if (H.mode == QSX_MODE_BATCHED || H.mode == QSX_MODE_PLAIN)
-      launch_grouped_gemm_f64(cfg, H.dtiles1[cfg], (int)H.tiles1[cfg].size(), p1, H.dHL, dpsi, H.dT, s);
-    else
-      launch_grouped_gemm_split(H.mode, H.nslices, cfg, H.dtiles1[cfg], (int)H.tiles1[cfg].size(), p1, H.dHL,
-                                dpsi, H.dT, s);
-  }
+  if (H.mode == QSX_MODE_BATCHED || H.mode == QSX_MODE_PLAIN)
+    launch_grouped_gemm_f64(H.dtiles1, (int)H.tiles1.size(), p1, H.dHL, dpsi, H.dT, s);
+  else
+    launch_grouped_gemm_split(H.mode, H.nslices, H.dtiles1, (int)H.tiles1.size(), p1, H.dHL, dpsi, H.dT, s);
   if (timed) QSX_HIP(hipEventRecord(H.ev[1], s));
-  for (int cfg = TILE_NCFG - 1; cfg >= 0; --cfg) {
-    if (H.mode == QSX_MODE_BATCHED || H.mode == QSX_MODE_PLAIN)
-      launch_grouped_gemm_f64(cfg, H.dtiles2[cfg], (int)H.tiles2[cfg].size(), p2, H.dT, H.dHR, dhpsi, s);
-    else
-      launch_grouped_gemm_split(H.mode, H.nslices, cfg, H.dtiles2[cfg], (int)H.tiles2[cfg].size(), p2, H.dT,
-                                H.dHR, dhpsi, s);
-  }
+  if (H.mode == QSX_MODE_BATCHED || H.mode == QSX_MODE_PLAIN)
+    launch_grouped_gemm_f64(H.dtiles2, (int)H.tiles2.size(), p2, H.dT, H.dHR, dhpsi, s);
+  else
+    launch_grouped_gemm_split(H.mode, H.nslices, H.dtiles2, (int)H.tiles2.size(), p2, H.dT, H.dHR, dhpsi, s);
   if (timed) QSX_HIP(hipEventRecord(H.ev[2], s));
 }
 
 // plain per-pair path (no batching, no graph): the baseline
 static void hpsi_plain(HpsiPlan& H, const double* dpsi, double* dhpsi, hipStream_t s) {
-  auto run = [&](const std::vector<PairDesc>& pairs, const std::vector<TileDesc>* tiles, const double* A,
+  auto run = [&](const std::vector<PairDesc>& pairs, const std::vector<TileDesc>& tiles, const double* A,
                  const double* B, double* C) {
     // walk the output blocks once: tiles of one block share c_off/pair range
     std::map<int64_t, const TileDesc*> blocks;
-    for (int cfg = 0; cfg < TILE_NCFG; ++cfg)
-      for (auto& t : tiles[cfg]) blocks.emplace(t.c_off, &t);
+    for (auto& t : tiles) blocks.emplace(t.c_off, &t);
     for (auto& [off, t] : blocks) {
       if (t->p0 == t->p1) {
         launch_fill_zero(C + off, (int64_t)t->M * t->N, s);
@@ -401,13 +397,18 @@ static void hpsi_create(HpsiPlan& H, const qsx_tensor* HL, const qsx_tensor* psi
   H.dpairs2 = upload_vec(H.pairs2, s);
   H.dpairs1s = upload_vec(H.pairs1s, s);
   H.dpairs2s = upload_vec(H.pairs2s, s);
-  for (int c = 0; c < TILE_NCFG; ++c) {
-    H.dtiles1[c] = upload_vec(H.tiles1[c], s);
-    H.dtiles2[c] = upload_vec(H.tiles2[c], s);
-  }
+  H.dtiles1 = upload_vec(H.tiles1, s);
+  H.dtiles2 = upload_vec(H.tiles2, s);
   QSX_HIP(hipStreamSynchronize(s));
   H.flops = 0;
-  for (int t = 0; t < H.nterm; ++t) H.flops += H.s1[t]->flops + H.s2[t]->flops;
+  for (int t = 0; t < H.nterm; ++t) {
+    H.flops += H.s1[t]->flops + H.s2[t]->flops;
+    H.stage_flops[0] += H.s1[t]->flops;
+    H.stage_flops[1] += H.s2[t]->flops;
+  }
+  // algorithmic bytes per stage: unique operand and result elements
+  H.stage_bytes[0] = 8.0 * ((double)H.HL.total() + (double)H.N + (double)H.t_total);
+  H.stage_bytes[1] = 8.0 * ((double)H.t_total + (double)H.HR.total() + (double)H.N);
   // algorithmic bytes: unique environments + psi + H*psi (+ the T_t write and re-read)
   H.bytes = 8.0 * ((double)H.HL.total() + (double)H.HR.total() + 2.0 * (double)H.N + 2.0 * (double)H.t_total);
   for (auto& e : H.ev) QSX_HIP(hipEventCreate(&e));
@@ -691,6 +692,20 @@ int qsx_hpsi_apply(qsx_hpsi_plan* plan, const double* dpsi, double* dhpsi, void*
   }
 }
 
+int qsx_hpsi_stage_info(const qsx_hpsi_plan* plan, double* info) {
+  if (!plan || !info) return QSX_ERR_ARG;
+  const HpsiPlan& H = *plan->h;
+  info[0] = H.stage_flops[0];
+  info[1] = H.stage_flops[1];
+  info[2] = H.stage_bytes[0];
+  info[3] = H.stage_bytes[1];
+  info[4] = (double)H.tiles1.size();
+  info[5] = (double)H.tiles2.size();
+  info[6] = (double)H.pairs1.size();
+  info[7] = (double)H.pairs2.size();
+  return 0;
+}
+
 int qsx_hpsi_weights(const qsx_hpsi_plan* plan, double* w) {
   if (!plan || !w) return QSX_ERR_ARG;
   std::copy(plan->h->welem.begin(), plan->h->welem.end(), w);
@@ -706,6 +721,7 @@ int qsx_hpsi_use_graph(qsx_hpsi_plan* plan, int32_t on) {
 int qsx_hpsi_last_timing(const qsx_hpsi_plan* plan, double* ms2) {
   if (!plan || !ms2) return QSX_ERR_ARG;
   float a = 0, b = 0;
+  if (hipEventSynchronize(plan->h->ev[2]) != hipSuccess) return QSX_ERR_DEVICE;
   if (hipEventElapsedTime(&a, plan->h->ev[0], plan->h->ev[1]) != hipSuccess) return QSX_ERR_DEVICE;
   if (hipEventElapsedTime(&b, plan->h->ev[1], plan->h->ev[2]) != hipSuccess) return QSX_ERR_DEVICE;
   ms2[0] = a;

@@ -86,21 +86,24 @@ struct PairDesc {
   double afac;     // X-symbol x cgw x sign x output normalisation
 };
 
+enum TileCfg { TILE_32 = 0, TILE_64 = 1, TILE_NCFG = 2 };
+
 struct TileDesc {
   int64_t c_off;   // element offset of the output block
   int32_t M, N;    // output block as MxN column-major (ldc = M)
   int32_t m0, n0;  // tile origin
   int32_t p0, p1;  // pair range of the output block
+  int32_t cfg;     // TILE_32 / TILE_64
+  int32_t pad;
 };
-
-enum TileCfg { TILE_32 = 0, TILE_64 = 1, TILE_NCFG = 2 };
 int tile_dim(int cfg);
 
-void launch_grouped_gemm_f64(int cfg, const TileDesc* dtiles, int ntiles, const PairDesc* dpairs, const double* A,
+void launch_grouped_gemm_f64(const TileDesc* dtiles, int ntiles, const PairDesc* dpairs, const double* A,
                              const double* B, double* C, hipStream_t s);
+void launch_copy(double* d, const double* src, int64_t n, hipStream_t s);
 void launch_pair_gemm_f64(const PairDesc& p, int64_t c_off, int M, int N, int beta, const double* A,
                           const double* B, double* C, hipStream_t s);
-void launch_grouped_gemm_split(int mode, int nslices, int cfg, const TileDesc* dtiles, int ntiles,
+void launch_grouped_gemm_split(int mode, int nslices, const TileDesc* dtiles, int ntiles,
                                const PairDesc* dpairs, const double* A, const double* B, double* C, hipStream_t s);
 
 // generic block permute: dst block b = permute(src block b, perm) for all blocks

@@ -4,9 +4,9 @@
 
 namespace qsx {
 
-void launch_grouped_gemm_split(int mode, int nslices, int cfg, const TileDesc* dtiles, int ntiles,
+void launch_grouped_gemm_split(int mode, int nslices, const TileDesc* dtiles, int ntiles,
                                const PairDesc* dpairs, const double* A, const double* B, double* C, hipStream_t s) {
-  (void)nslices; (void)cfg; (void)dtiles; (void)ntiles; (void)dpairs; (void)A; (void)B; (void)C; (void)s;
+  (void)nslices; (void)dtiles; (void)ntiles; (void)dpairs; (void)A; (void)B; (void)C; (void)s;
   fail(QSX_ERR_UNSUPPORTED, "split-precision mode %d not available", mode);
 }
 

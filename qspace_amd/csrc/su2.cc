@@ -429,6 +429,7 @@ double x3_compute(const std::vector<int>& tja, const std::vector<int>& dira, con
     ca[ia[i]] = 1;
     cb[ib[i]] = 1;
     if (tja[ia[i]] != tjb[ib[i]]) fail(QSX_ERR_CG, "contracted SU(2) labels differ");
+    if (dira[ia[i]] == dirb[ib[i]]) fail(QSX_ERR_ITAG, "contracted SU(2) legs must have opposite directions");
   }
   std::vector<int> ka, kb, tjc, dirc;
   for (int i = 0; i < ra; ++i)

@@ -1,0 +1,135 @@
+"""Host-side checks of libqsx.so that need no GPU: the C-ABI exports every
+entry point declared in include/qsx.h, ctr-idx parsing, and the exact SU(2)
+coefficient engine (CG and X-symbols) against sympy fixtures and the oracle."""
+import ctypes as C
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle as O
+import qspace_amd as Q
+from qspace_amd import _capi
+from qspace_amd.qspace import parse_ctridx
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    with open(os.path.join(ROOT, "include", "qsx.h")) as f:
+        src = f.read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(qsx_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_header():
+    L = _capi.lib()
+    names = declared_symbols()
+    assert len(names) >= 30
+    for n in names:
+        assert hasattr(L, n), f"libqsx.so does not export {n}"
+    assert set(names) == set(_capi.exported_symbols())
+    assert L.qsx_version().decode().startswith("qsx")
+
+
+@pytest.mark.parametrize("s,idx,conj", [("13", [0, 2], 0), ("1,3", [0, 2], 0), ("[1 3]*", [0, 2], 1),
+                                         ("12**", [0, 1], 0), ("2", [1], 0), ("1, 2 ;3", [0, 1, 2], 0),
+                                         ("10,11", [9, 10], 0), ("A", [9], 0)])
+def test_parse_ctridx(s, idx, conj):
+    L = _capi.lib()
+    out = (C.c_int32 * 16)()
+    cj = C.c_int32()
+    n = L.qsx_parse_ctridx(s.encode(), out, 16, C.byref(cj))
+    assert n == len(idx)
+    assert list(out[:n]) == idx and cj.value == conj
+    assert parse_ctridx(s) == (idx, conj)
+
+
+@pytest.mark.parametrize("s", ["", "11", "0", "1,x"])
+def test_parse_ctridx_errors(s):
+    out = (C.c_int32 * 16)()
+    assert _capi.lib().qsx_parse_ctridx(s.encode(), out, 16, None) < 0
+    with pytest.raises(ValueError):
+        parse_ctridx(s)
+
+
+def test_cg_exact_against_sympy_fixture():
+    with open(os.path.join(ROOT, "tests", "golden", "su2_cg.json")) as f:
+        rows = json.load(f)["rows"]
+    err = max(abs(Q.su2_cg(*r[:6]) - r[6]) for r in rows)
+    assert err < 2e-16
+
+
+def test_cg_exact_large_spin_identities():
+    # <j m; j -m | 0 0> = (-1)^{j-m} / sqrt(2j+1) at j = 100
+    tj = 200
+    for tm in (200, 0, -198):
+        v = Q.su2_cg(tj, tm, tj, -tm, 0, 0)
+        ref = (-1) ** ((tj - tm) // 2) / np.sqrt(tj + 1)
+        assert abs(v - ref) < 1e-16
+    # stretched coupling <j j; 1/2 1/2 | j+1/2 j+1/2> = 1
+    assert Q.su2_cg(291, 291, 1, 1, 292, 292) == 1.0
+
+
+def _random_valid_x3(rng):
+    """random OM=1 SU(2) contraction patterns with consistent labels."""
+    kind = rng.randint(4)
+    tjs = int(rng.randint(0, 9))
+    if kind == 0:   # rank2 (in,out) . rank3 (in,in,out) over leg 0
+        ts = int(rng.randint(1, 3))
+        tc = tjs + ts - 2 * int(rng.randint(0, min(tjs, ts) + 1))
+        return [tjs, tjs], [1, -1], [tjs, ts, tc], [1, 1, -1], [1], [0]
+    if kind == 1:   # rank3 . conj rank3 over two legs -> rank 2
+        ts = int(rng.randint(1, 3))
+        tc = tjs + ts - 2 * int(rng.randint(0, min(tjs, ts) + 1))
+        return [tjs, ts, tc], [1, 1, -1], [tjs, ts, tc], [-1, -1, 1], [0, 1], [0, 1]
+    if kind == 2:   # H*psi stage 1: HL(a, a'*, w) . psi(a', b) over a'
+        tw = 2 * int(rng.randint(0, 2))
+        ta = tjs + tw - 2 * int(rng.randint(0, min(tjs, tw) + 1))
+        return [ta, tjs, tw], [1, -1, 1], [tjs, tjs], [1, 1], [1], [0]
+    # H*psi stage 2: T(a, w, b') . HR(b', b*, w*)... contracted over (w, b')
+    tw = 2 * int(rng.randint(0, 2))
+    tb = tjs + tw - 2 * int(rng.randint(0, min(tjs, tw) + 1))
+    return [tjs, tw, tb], [1, 1, 1], [tjs, tb, tw], [1, -1, -1], [1, 2], [2, 1]
+
+
+def test_x3_product_vs_oracle():
+    rng = np.random.RandomState(11)
+    n = 0
+    for _ in range(300):
+        case = _random_valid_x3(rng)
+        try:
+            ref = O.x3(*case)
+        except O.OracleError:
+            with pytest.raises(Q.QsxError):
+                Q.su2_x3(*case)
+            continue
+        got = Q.su2_x3(*case)
+        assert abs(got - ref) < 1e-13 * max(1.0, abs(ref)), case
+        n += 1
+    assert n > 150
+
+
+def test_x3_large_spin_vs_oracle():
+    cases = [([140, 2, 142], [1, 1, -1], [142, 140, 2], [1, -1, -1], [0, 1], [1, 2]),
+             ([100, 2, 100], [1, 1, -1], [100, 100], [-1, 1], [0], [0]),
+             ([290, 2, 290], [1, 1, -1], [290, 290], [1, 1], [2], [0])]
+    for c in cases:
+        assert abs(Q.su2_x3(*c) - O.x3(*c)) < 1e-13
+
+
+def test_x3_rejects_same_direction():
+    with pytest.raises(Q.QsxError) as e:
+        Q.su2_x3([2, 2], [1, -1], [2, 2], [1, -1], [1], [1])  # both contracted legs outgoing
+    assert e.value.code in (2, 8)
+
+
+def test_no_device_is_a_clean_error():
+    """on a host without a GPU the device entry points return QSX_ERR_DEVICE."""
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("GPU device present")
+    err = C.create_string_buffer(256)
+    rc = _capi.lib().qsx_device_init(0, err, 256)
+    assert rc == 6 and b"device" in err.value
