@@ -184,9 +184,16 @@ int qsx_hpsi_weights(const qsx_hpsi_plan* plan, double* w);
    algorithmic bytes stage 1, stage 2, tiles stage 1, stage 2, pairs stage 1,
    stage 2 */
 int qsx_hpsi_stage_info(const qsx_hpsi_plan* plan, double* info);
+/* autotune the work decomposition of both stages (tile shapes, k-split
+   length) by timing candidates on this plan; stage_ms[2] = best times (or -1
+   when taken from the per-process cache), policy[6] = chosen split,
+   tile overhead of stage 1 and of stage 2, then the persistent grid size of
+   stage 1 and of stage 2 (either pointer may be NULL) */
+int qsx_hpsi_autotune(qsx_hpsi_plan* plan, int32_t reps, double* stage_ms, int32_t* policy);
 /* capture/replay the H*psi in a hipGraph (0 = direct launches) */
 int qsx_hpsi_use_graph(qsx_hpsi_plan* plan, int32_t on);
-/* per-kernel device timing of the last apply: [stage1_ms, stage2_ms] */
+/* per-kernel device timing of the last direct (non-graph) apply, HIP events
+   on the launch stream: [grouped GEMM stage 1, grouped GEMM stage 2] in ms */
 int qsx_hpsi_last_timing(const qsx_hpsi_plan* plan, double* ms2);
 void qsx_hpsi_plan_destroy(qsx_hpsi_plan* plan);
 

@@ -288,6 +288,23 @@ static std::vector<double> cg_table(int tj1, int tj2, int tj3) {
   int d1 = tj1 + 1, d2 = tj2 + 1, d3 = tj3 + 1;
   if (tj3 > tj1 + tj2 || tj3 < std::abs(tj1 - tj2) || ((tj1 + tj2 + tj3) & 1))
     return std::vector<double>((size_t)d1 * d2 * d3, 0.0);
+  if (tj3 < tj2 && tj2 > 8) {
+    // two large spins coupled to a small one: use the textbook symmetry
+    // <j1 m1 j2 m2|J M> = (-1)^{j1-m1} sqrt((2J+1)/(2j2+1)) <j1 m1 J -M|j2 -m2>
+    // so the table is built from the (j1 (x) J -> j2) coupling
+    std::vector<double> S = cg_table(tj1, tj3, tj2), T((size_t)d1 * d2 * d3, 0.0);
+    const double f = std::sqrt((double)(tj3 + 1) / (double)(tj2 + 1));
+    for (int i3 = 0; i3 < d3; ++i3)
+      for (int i2 = 0; i2 < d2; ++i2)
+        for (int i1 = 0; i1 < d1; ++i1) {
+          // S index: (i1, iJ of -M, i2' of -m2) with i = j - m
+          const int iMneg = d3 - 1 - i3, im2neg = d2 - 1 - i2;
+          double v = S[i1 + (size_t)d1 * (iMneg + (size_t)d3 * im2neg)];
+          if (v == 0) continue;
+          T[i1 + (size_t)d1 * (i2 + (size_t)d2 * i3)] = ((i1 & 1) ? -f : f) * v;
+        }
+    return T;
+  }
   std::lock_guard<std::mutex> lk(g_cgall_mu);
   auto key = std::make_pair(tj1, tj2);
   auto it = g_cgall.find(key);

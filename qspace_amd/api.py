@@ -253,6 +253,13 @@ class HPsi:
         return {"flops": (v[0], v[1]), "bytes": (v[2], v[3]), "tiles": (int(v[4]), int(v[5])),
                 "pairs": (int(v[6]), int(v[7]))}
 
+    def autotune(self, reps: int = 5):
+        ms = (C.c_double * 2)()
+        pol = (C.c_int32 * 6)()
+        _capi.check(_capi.lib().qsx_hpsi_autotune(self.h, reps, ms, pol), None)
+        return {"stage_ms": (ms[0], ms[1]), "split_steps": (pol[0], pol[2]), "tile_overhead": (pol[1], pol[3]),
+                "grid": (pol[4], pol[5])}
+
     def weights(self) -> np.ndarray:
         w = np.zeros(self.N)
         _capi.check(_capi.lib().qsx_hpsi_weights(self.h, w.ctypes.data_as(C.POINTER(C.c_double))), None)
@@ -271,9 +278,10 @@ class HPsi:
         return o.download()
 
     def last_timing(self):
+        """(stage 1, stage 2) grouped-GEMM ms of the last direct apply"""
         ms = (C.c_double * 2)()
         _capi.check(_capi.lib().qsx_hpsi_last_timing(self.h, ms), None)
-        return ms[0], ms[1]
+        return tuple(ms)
 
     def lanczos(self, psi0: np.ndarray, niter: int):
         return self._eig(_capi.lib().qsx_lanczos, psi0, niter)

@@ -119,6 +119,19 @@ hipStream_t default_stream() {
   return g_stream;
 }
 
+// resident workgroups of the persistent grouped GEMM: 4 per compute unit
+// (the kernel's LDS and register use allow 4 per CU)
+int resident_slots() {
+  static int slots = 0;
+  if (!slots) {
+    ensure_device();
+    hipDeviceProp_t prop;
+    QSX_HIP(hipGetDeviceProperties(&prop, g_dev));
+    slots = 4 * std::max(1, prop.multiProcessorCount);
+  }
+  return slots;
+}
+
 void* dev_alloc(size_t bytes) {
   ensure_device();
   void* p = nullptr;
@@ -134,7 +147,10 @@ void dev_free(void* p) {
 
 CtrPlan::~CtrPlan() {
   dev_free(dpairs);
-  dev_free(dtiles);
+  dev_free(ditems);
+  dev_free(dreds);
+  dev_free(dqueues);
+  dev_free(dscratch);
   dev_free(dAperm);
   dev_free(dBperm);
   dev_free(dCtmp);
@@ -523,43 +539,6 @@ static void build_structure(CtrPlan& P, const int32_t* perm, int nperm, const do
   }
 }
 
-void plan_build_tiles(const CtrPlan& P, int policy, std::vector<TileDesc>& tiles,
-                      const std::vector<int64_t>* c_offsets, int64_t pair_base) {
-  struct Item {
-    TileDesc t;
-    double cost;
-    int cfg;
-  };
-  std::vector<Item> items;
-  for (int c = 0; c < P.nC; ++c) {
-    const int M = P.M[c], N = P.N[c];
-    const int cfg = std::max(M, N) <= policy ? TILE_32 : TILE_64;
-    const int T = tile_dim(cfg);
-    double ksum = 0;
-    for (int64_t p = P.pseg[c]; p < P.pseg[c + 1]; ++p) ksum += P.pairs[p].K + 8;
-    for (int m0 = 0; m0 < std::max(M, 1); m0 += T)
-      for (int n0 = 0; n0 < std::max(N, 1); n0 += T) {
-        Item it;
-        it.t.c_off = c_offsets ? (*c_offsets)[c] : P.offsOut[c];
-        it.t.M = M;
-        it.t.N = N;
-        it.t.m0 = m0;
-        it.t.n0 = n0;
-        it.t.p0 = (int32_t)(pair_base + P.pseg[c]);
-        it.t.p1 = (int32_t)(pair_base + P.pseg[c + 1]);
-        it.t.cfg = cfg;
-        it.t.pad = 0;
-        it.cost = ksum * (double)T * T;
-        it.cfg = cfg;
-        items.push_back(it);
-      }
-  }
-  // longest first: the dispatcher hands out workgroups in index order
-  std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) { return a.cost > b.cost; });
-  tiles.clear();
-  for (auto& it : items) tiles.push_back(it.t);
-}
-
 static std::vector<PermBlock> perm_blocks(const TensorDesc& T) {
   std::vector<PermBlock> v(T.n);
   for (int i = 0; i < T.n; ++i) {
@@ -576,16 +555,23 @@ void plan_upload(CtrPlan& P) {
   P.dpairs = (PairDesc*)dev_alloc(sizeof(PairDesc) * std::max<size_t>(1, P.pairs.size()));
   if (!P.pairs.empty())
     QSX_HIP(hipMemcpyAsync(P.dpairs, P.pairs.data(), sizeof(PairDesc) * P.pairs.size(), hipMemcpyHostToDevice, s));
-  std::vector<TileDesc> tiles;
-  std::vector<int64_t> coffs(P.offsOut.begin(), P.offsOut.end());
-  plan_build_tiles(P, P.tile_policy, tiles, &coffs, 0);
-  dev_free(P.dtiles);
-  P.dtiles = nullptr;
-  P.ntiles = (int)tiles.size();
-  if (P.ntiles) {
-    P.dtiles = (TileDesc*)dev_alloc(sizeof(TileDesc) * tiles.size());
-    QSX_HIP(hipMemcpyAsync(P.dtiles, tiles.data(), sizeof(TileDesc) * tiles.size(), hipMemcpyHostToDevice, s));
-  }
+  std::vector<WorkTarget> targets(P.nC);
+  for (int c = 0; c < P.nC; ++c) targets[c] = WorkTarget{P.offsOut[c], P.M[c], P.N[c], P.pseg[c], P.pseg[c + 1]};
+  P.work = build_work(targets, P.pairs, P.policy);
+  dev_free(P.ditems);
+  dev_free(P.dreds);
+  dev_free(P.dqueues);
+  dev_free(P.dscratch);
+  P.ditems = (WorkItem*)dev_alloc(sizeof(WorkItem) * std::max<size_t>(1, P.work.items.size()));
+  P.dreds = (ReduceItem*)dev_alloc(sizeof(ReduceItem) * std::max<size_t>(1, P.work.reds.size()));
+  P.dqueues = (WorkQueue*)dev_alloc(sizeof(WorkQueue) * std::max<size_t>(1, P.work.queues.size()));
+  if (!P.work.queues.empty())
+    QSX_HIP(hipMemcpyAsync(P.dqueues, P.work.queues.data(), sizeof(WorkQueue) * P.work.queues.size(), hipMemcpyHostToDevice, s));
+  if (!P.work.reds.empty())
+    QSX_HIP(hipMemcpyAsync(P.dreds, P.work.reds.data(), sizeof(ReduceItem) * P.work.reds.size(), hipMemcpyHostToDevice, s));
+  P.dscratch = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, P.work.scratch_elems));
+  if (!P.work.items.empty())
+    QSX_HIP(hipMemcpyAsync(P.ditems, P.work.items.data(), sizeof(WorkItem) * P.work.items.size(), hipMemcpyHostToDevice, s));
   auto upload_perm = [&](const TensorDesc& T, PermBlock*& d, int64_t& mx) {
     std::vector<PermBlock> v = perm_blocks(T);
     mx = 0;
@@ -650,6 +636,7 @@ std::unique_ptr<CtrPlan> plan_create(const qsx_tensor* A, const int32_t* ia, int
   P->icb.assign(ib, ib + nb);
   P->conjA = conjA ? 1 : 0;
   P->conjB = conjB ? 1 : 0;
+  P->policy.slots = resident_slots();
   if (o) {
     P->mode = o->mode;
     P->nslices = o->nslices;
@@ -683,9 +670,11 @@ void plan_execute(CtrPlan& P, const double* dA, const double* dB, double* dC, hi
       if (P.pseg[blk] == P.pseg[blk + 1]) launch_fill_zero(c + P.offsOut[blk], P.offsOut[blk + 1] - P.offsOut[blk], s);
     }
   } else if (P.mode == QSX_MODE_BATCHED) {
-    launch_grouped_gemm_f64(P.dtiles, P.ntiles, P.dpairs, a, b, c, s);
+    launch_work(P.dqueues, (int)P.work.queues.size(), P.ditems, P.dreds, (int)P.work.reds.size(), P.dpairs, a, b, c,
+                P.dscratch, s);
   } else {
-    launch_grouped_gemm_split(P.mode, P.nslices, P.dtiles, P.ntiles, P.dpairs, a, b, c, s);
+    launch_work_split(P.mode, P.nslices, P.dqueues, (int)P.work.queues.size(), P.ditems, P.dreds,
+                      (int)P.work.reds.size(), P.dpairs, a, b, c, P.dscratch, s);
   }
   if (P.has_perm) launch_permute_blocks(P.dpbC, P.nC, P.rc, P.perm.data(), P.dCtmp, dC, P.maxC, s);
 }

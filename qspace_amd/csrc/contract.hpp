@@ -46,9 +46,12 @@ struct CtrPlan {
   double flops = 0;                   // useful flops (2MNK over executed pairs)
   // device resources
   PairDesc* dpairs = nullptr;
-  TileDesc* dtiles = nullptr;
-  int ntiles = 0;
-  int tile_policy = 32;               // blocks with max(M,N) <= policy use 32x32 tiles
+  WorkPolicy policy;
+  WorkList work;
+  WorkItem* ditems = nullptr;
+  ReduceItem* dreds = nullptr;
+  WorkQueue* dqueues = nullptr;
+  double* dscratch = nullptr;
   double* dAperm = nullptr;
   double* dBperm = nullptr;
   double* dCtmp = nullptr;
@@ -62,10 +65,8 @@ struct CtrPlan {
 std::unique_ptr<CtrPlan> plan_create(const qsx_tensor* A, const int32_t* ia, int na, int conjA, const qsx_tensor* B,
                                      const int32_t* ib, int nb, int conjB, const int32_t* perm, int nperm,
                                      const qsx_opts* o, bool upload = true);
-// build tile lists for the chosen policy and upload pairs/tiles
+// build the work list for the plan's policy and upload pairs/items
 void plan_upload(CtrPlan& P);
-void plan_build_tiles(const CtrPlan& P, int policy, std::vector<TileDesc>& tiles,
-                      const std::vector<int64_t>* c_offsets = nullptr, int64_t pair_base = 0);
 void plan_execute(CtrPlan& P, const double* dA, const double* dB, double* dC, hipStream_t s);
 // permute an operand's packed device data into the plan's layout
 void plan_permute_operand(const CtrPlan& P, int which, const double* src, double* dst, hipStream_t s);
@@ -77,6 +78,7 @@ void match_itags(const TensorDesc& A, const TensorDesc& B, int conjA, int conjB,
 hipStream_t default_stream();
 void ensure_device();
 void* dev_alloc(size_t bytes);
+int resident_slots();
 void dev_free(void* p);
 
 }  // namespace qsx

@@ -114,30 +114,6 @@ __device__ __forceinline__ void tile_store(const d4 (&acc)[TM / 32][TN / 32], do
       }
 }
 
-// one launch for all tiles: TileDesc.cfg selects the 64x64 or the 32x32
-// instantiation (uniform per workgroup); LDS is sized for the larger one
-__global__ __launch_bounds__(256) void k_grouped_gemm_f64(const TileDesc* __restrict__ tiles,
-                                                          const PairDesc* __restrict__ pairs,
-                                                          const double* __restrict__ A, const double* __restrict__ B,
-                                                          double* __restrict__ C) {
-  __shared__ Smem<64, 64> sm;
-  const TileDesc t = tiles[blockIdx.x];
-  if (t.cfg == TILE_64) {
-    d4 acc[2][2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-      for (int j = 0; j < 2; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
-    tile_accumulate<64, 64>(sm, acc, pairs, t.p0, t.p1, t.M, t.N, t.m0, t.n0, A, B);
-    tile_store<64, 64>(acc, C + t.c_off, t.M, t.N, t.m0, t.n0, 0);
-  } else {
-    Smem<32, 32>& s32 = *reinterpret_cast<Smem<32, 32>*>(&sm);
-    d4 acc[1][1] = {{d4{0.0, 0.0, 0.0, 0.0}}};
-    tile_accumulate<32, 32>(s32, acc, pairs, t.p0, t.p1, t.M, t.N, t.m0, t.n0, A, B);
-    tile_store<32, 32>(acc, C + t.c_off, t.M, t.N, t.m0, t.n0, 0);
-  }
-}
-
 // plain path: one launch per pair; the grid covers the tiles of its output block
 template <int TM, int TN>
 __global__ __launch_bounds__(256) void k_pair_gemm_f64(PairDesc pd, int64_t c_off, int M, int N, int beta,
@@ -207,15 +183,6 @@ __global__ void k_fill_zero(double* p, int64_t n) {
 }
 
 }  // namespace
-
-int tile_dim(int cfg) { return cfg == TILE_64 ? 64 : 32; }
-
-void launch_grouped_gemm_f64(const TileDesc* dtiles, int ntiles, const PairDesc* dpairs, const double* A,
-                             const double* B, double* C, hipStream_t s) {
-  if (ntiles <= 0) return;
-  hipLaunchKernelGGL(k_grouped_gemm_f64, dim3(ntiles), dim3(256), 0, s, dtiles, dpairs, A, B, C);
-  QSX_HIP(hipGetLastError());
-}
 
 void launch_pair_gemm_f64(const PairDesc& p, int64_t c_off, int M, int N, int beta, const double* A, const double* B,
                           double* C, hipStream_t s) {

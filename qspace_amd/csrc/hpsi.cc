@@ -40,7 +40,13 @@ struct HpsiPlan {
   int64_t hl_base[2] = {0, 0}, hr_base[2] = {0, 0}, t_base[2] = {0, 0};
   int64_t t_total = 0, hr_total = 0, hl_total = 0;
   std::vector<PairDesc> pairs1, pairs2;
-  std::vector<TileDesc> tiles1, tiles2;
+  std::vector<WorkTarget> targets1, targets2;
+  WorkList work1, work2;
+  WorkPolicy pol1, pol2;
+  WorkItem *ditems1 = nullptr, *ditems2 = nullptr;
+  ReduceItem *dreds1 = nullptr, *dreds2 = nullptr;
+  WorkQueue *dq1 = nullptr, *dq2 = nullptr;
+  double* dscratch = nullptr;
   PairDesc *dpairs1 = nullptr, *dpairs2 = nullptr;
   // the same work with afac scaled to the orthonormal reduced basis
   // x~ = sqrt(w) x, w = full-space norm of each psi block's CGT, in which
@@ -49,9 +55,6 @@ struct HpsiPlan {
   std::vector<double> welem;  // per-element weight w of psi's packed layout
   std::vector<double> wblk;
   std::vector<PairDesc> pairs1s, pairs2s;
-  TileDesc* dtiles1 = nullptr;
-  TileDesc* dtiles2 = nullptr;
-  int policy = 32;
   double flops = 0, bytes = 0;
   double stage_flops[2] = {0, 0}, stage_bytes[2] = {0, 0};
   // graph
@@ -71,8 +74,13 @@ struct HpsiPlan {
     dev_free(dpairs2);
     dev_free(dpairs1s);
     dev_free(dpairs2s);
-    dev_free(dtiles1);
-    dev_free(dtiles2);
+    dev_free(ditems1);
+    dev_free(ditems2);
+    dev_free(dreds1);
+    dev_free(dreds2);
+    dev_free(dq1);
+    dev_free(dq2);
+    dev_free(dscratch);
     for (auto e : ev)
       if (e) (void)hipEventDestroy(e);
   }
@@ -99,12 +107,13 @@ struct HostTensor {  // qsx_tensor view over owned vectors
   }
 };
 
-// multi-term grouped GEMM work list: output block c of term t's plan goes to
-// target block target[t][c]; pairs of all terms of one target run together
+// multi-term work: output block c of term t's plan goes to target block
+// target[t][c]; the pairs of all terms of one target are concatenated (term
+// order, then the plan's reference pair order) into one pair range
 void build_multi(const std::vector<CtrPlan*>& plans, const std::vector<int64_t>& a_base,
                  const std::vector<int64_t>& b_base, const std::vector<std::vector<int>>& target,
-                 const std::vector<int64_t>& t_off, const std::vector<int>& tM, const std::vector<int>& tN, int policy,
-                 std::vector<PairDesc>& pairs, std::vector<TileDesc>& tiles,
+                 const std::vector<int64_t>& t_off, const std::vector<int>& tM, const std::vector<int>& tN,
+                 std::vector<PairDesc>& pairs, std::vector<WorkTarget>& targets,
                  std::vector<int>* pair_target = nullptr, std::vector<std::pair<int, int64_t>>* pair_src = nullptr) {
   const int nt = (int)t_off.size();
   std::vector<std::vector<std::pair<int, int>>> src(nt);  // (term, block)
@@ -118,17 +127,11 @@ void build_multi(const std::vector<CtrPlan*>& plans, const std::vector<int64_t>&
       src[g].push_back({(int)t, c});
     }
   pairs.clear();
+  targets.clear();
   if (pair_target) pair_target->clear();
   if (pair_src) pair_src->clear();
-  struct Item {
-    TileDesc td;
-    double cost;
-    int cfg;
-  };
-  std::vector<Item> items;
   for (int g = 0; g < nt; ++g) {
     int64_t p0 = (int64_t)pairs.size();
-    double ksum = 0;
     for (auto [t, c] : src[g]) {
       const CtrPlan& P = *plans[t];
       for (int64_t p = P.pseg[c]; p < P.pseg[c + 1]; ++p) {
@@ -137,34 +140,11 @@ void build_multi(const std::vector<CtrPlan*>& plans, const std::vector<int64_t>&
         pd.b_off += b_base[t];
         pairs.push_back(pd);
         if (pair_target) pair_target->push_back(g);
-        if (pair_src) pair_src->push_back({t, p});
-        ksum += pd.K + 8;
+        if (pair_src) pair_src->push_back({(int)t, p});
       }
     }
-    int64_t p1 = (int64_t)pairs.size();
-    const int M = tM[g], N = tN[g];
-    const int cfg = std::max(M, N) <= policy ? TILE_32 : TILE_64;
-    const int T = tile_dim(cfg);
-    for (int m0 = 0; m0 < std::max(M, 1); m0 += T)
-      for (int n0 = 0; n0 < std::max(N, 1); n0 += T) {
-        Item it;
-        it.td.c_off = t_off[g];
-        it.td.M = M;
-        it.td.N = N;
-        it.td.m0 = m0;
-        it.td.n0 = n0;
-        it.td.p0 = (int32_t)p0;
-        it.td.p1 = (int32_t)p1;
-        it.td.cfg = cfg;
-        it.td.pad = 0;
-        it.cost = ksum * (double)T * T;
-        it.cfg = cfg;
-        items.push_back(it);
-      }
+    targets.push_back(WorkTarget{t_off[g], tM[g], tN[g], p0, (int64_t)pairs.size()});
   }
-  std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) { return a.cost > b.cost; });
-  tiles.clear();
-  for (auto& it : items) tiles.push_back(it.td);
 }
 
 template <class T>
@@ -187,40 +167,163 @@ std::string rec_key(const int32_t* r, int m) { return std::string((const char*)r
 
 }  // namespace
 
+// (re)build and upload the two stages' work lists for the current policies
+static void hpsi_build_work(HpsiPlan& H) {
+  hipStream_t s = default_stream();
+  H.work1 = build_work(H.targets1, H.pairs1, H.pol1);
+  H.work2 = build_work(H.targets2, H.pairs2, H.pol2);
+  for (void* p : {(void*)H.ditems1, (void*)H.ditems2, (void*)H.dreds1, (void*)H.dreds2, (void*)H.dq1, (void*)H.dq2,
+                  (void*)H.dscratch})
+    dev_free(p);
+  H.ditems1 = upload_vec(H.work1.items, s);
+  H.ditems2 = upload_vec(H.work2.items, s);
+  H.dreds1 = upload_vec(H.work1.reds, s);
+  H.dreds2 = upload_vec(H.work2.reds, s);
+  H.dq1 = upload_vec(H.work1.queues, s);
+  H.dq2 = upload_vec(H.work2.queues, s);
+  H.dscratch = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, std::max(H.work1.scratch_elems, H.work2.scratch_elems)));
+  QSX_HIP(hipStreamSynchronize(s));
+  if (H.gexec) {
+    (void)hipGraphExecDestroy(H.gexec);
+    H.gexec = nullptr;
+  }
+}
+
+// ---------------------------------------------------------------- autotuner
+// Tile shapes and k-splits come from timing candidate work decompositions of
+// each stage on the plan itself (HIP events, median of reps).  The result is
+// cached per process under a key built from the stage's block-size
+// histogram (log2 buckets of M, N and K, the STAT_DGEMM idea of
+// Source/wbblas.hh:235-268), so plans with the same size profile reuse it.
+static std::map<std::string, std::pair<WorkPolicy, WorkPolicy>> g_tune_cache;
+
+static std::string hist_key(const std::vector<WorkTarget>& targets, const std::vector<PairDesc>& pairs) {
+  std::map<int, int64_t> h;
+  auto lg = [](int64_t x) {
+    int b = 0;
+    while ((1ll << b) < x) ++b;
+    return b;
+  };
+  for (auto& t : targets)
+    for (int64_t p = t.p0; p < t.p1; ++p) h[lg(t.M) * 400 + lg(t.N) * 20 + lg(pairs[p].K)]++;
+  std::string k;
+  for (auto& [b, n] : h) k += std::to_string(b) + ":" + std::to_string(lg(n)) + ";";
+  return k;
+}
+
+static double time_stage(HpsiPlan& H, int stage, const WorkList& W, const double* dpsi, double* dout, int reps) {
+  hipStream_t s = default_stream();
+  WorkItem* di = upload_vec(W.items, s);
+  ReduceItem* dr = upload_vec(W.reds, s);
+  WorkQueue* dq = upload_vec(W.queues, s);
+  double* sc = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, W.scratch_elems));
+  const PairDesc* p = stage == 0 ? H.dpairs1 : H.dpairs2;
+  std::vector<float> ms;
+  hipEvent_t e0, e1;
+  QSX_HIP(hipEventCreate(&e0));
+  QSX_HIP(hipEventCreate(&e1));
+  for (int r = 0; r < reps + 2; ++r) {
+    QSX_HIP(hipEventRecord(e0, s));
+    if (stage == 0)
+      launch_work(dq, (int)W.queues.size(), di, dr, (int)W.reds.size(), p, H.dHL, dpsi, H.dT, sc, s);
+    else
+      launch_work(dq, (int)W.queues.size(), di, dr, (int)W.reds.size(), p, H.dT, H.dHR, dout, sc, s);
+    QSX_HIP(hipEventRecord(e1, s));
+    QSX_HIP(hipEventSynchronize(e1));
+    float t = 0;
+    QSX_HIP(hipEventElapsedTime(&t, e0, e1));
+    if (r >= 2) ms.push_back(t);
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  dev_free(di);
+  dev_free(dr);
+  dev_free(dq);
+  dev_free(sc);
+  std::sort(ms.begin(), ms.end());
+  return ms[ms.size() / 2];
+}
+
+static void hpsi_autotune(HpsiPlan& H, int reps, double* out) {
+  const std::string key = hist_key(H.targets1, H.pairs1) + "|" + hist_key(H.targets2, H.pairs2);
+  auto it = g_tune_cache.find(key);
+  if (it != g_tune_cache.end()) {
+    H.pol1 = it->second.first;
+    H.pol2 = it->second.second;
+    hpsi_build_work(H);
+    if (out) out[0] = out[1] = -1;
+    return;
+  }
+  double* dpsi = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, H.N));
+  double* dout = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, H.N));
+  launch_fill_zero(dpsi, H.N, default_stream());
+  const int splits[] = {2, 4, 8, 16, 1 << 28};
+  const double overheads[] = {0, 2048, 8192};
+  const int slots[] = {resident_slots(), 2 * resident_slots()};
+  for (int stage = 0; stage < 2; ++stage) {
+    const auto& tg = stage == 0 ? H.targets1 : H.targets2;
+    const auto& pr = stage == 0 ? H.pairs1 : H.pairs2;
+    double best = 1e300;
+    WorkPolicy bp;
+    for (int sp : splits)
+      for (double ov : overheads)
+        for (int sl : slots) {
+        WorkPolicy pol;
+        pol.split_steps = sp;
+        pol.tile_overhead = ov;
+        pol.slots = sl;
+        WorkList W = build_work(tg, pr, pol);
+        double t = time_stage(H, stage, W, dpsi, dout, reps);
+        if (t < best) {
+          best = t;
+          bp = pol;
+        }
+      }
+    (stage == 0 ? H.pol1 : H.pol2) = bp;
+    if (out) out[stage] = best;
+  }
+  dev_free(dpsi);
+  dev_free(dout);
+  g_tune_cache[key] = {H.pol1, H.pol2};
+  hpsi_build_work(H);
+}
+
 static void hpsi_launch(HpsiPlan& H, const double* dpsi, double* dhpsi, hipStream_t s, bool timed, bool sym = false) {
   const PairDesc* p1 = sym ? H.dpairs1s : H.dpairs1;
   const PairDesc* p2 = sym ? H.dpairs2s : H.dpairs2;
+  // events: [0] start, [1] stage 1 done, [2] stage 2 done
   if (timed) QSX_HIP(hipEventRecord(H.ev[0], s));
   if (H.mode == QSX_MODE_BATCHED || H.mode == QSX_MODE_PLAIN)
-    launch_grouped_gemm_f64(H.dtiles1, (int)H.tiles1.size(), p1, H.dHL, dpsi, H.dT, s);
+    launch_work(H.dq1, (int)H.work1.queues.size(), H.ditems1, H.dreds1, (int)H.work1.reds.size(), p1, H.dHL, dpsi,
+                H.dT, H.dscratch, s);
   else
-    launch_grouped_gemm_split(H.mode, H.nslices, H.dtiles1, (int)H.tiles1.size(), p1, H.dHL, dpsi, H.dT, s);
+    launch_work_split(H.mode, H.nslices, H.dq1, (int)H.work1.queues.size(), H.ditems1, H.dreds1,
+                      (int)H.work1.reds.size(), p1, H.dHL, dpsi, H.dT, H.dscratch, s);
   if (timed) QSX_HIP(hipEventRecord(H.ev[1], s));
   if (H.mode == QSX_MODE_BATCHED || H.mode == QSX_MODE_PLAIN)
-    launch_grouped_gemm_f64(H.dtiles2, (int)H.tiles2.size(), p2, H.dT, H.dHR, dhpsi, s);
+    launch_work(H.dq2, (int)H.work2.queues.size(), H.ditems2, H.dreds2, (int)H.work2.reds.size(), p2, H.dT, H.dHR,
+                dhpsi, H.dscratch, s);
   else
-    launch_grouped_gemm_split(H.mode, H.nslices, H.dtiles2, (int)H.tiles2.size(), p2, H.dT, H.dHR, dhpsi, s);
+    launch_work_split(H.mode, H.nslices, H.dq2, (int)H.work2.queues.size(), H.ditems2, H.dreds2,
+                      (int)H.work2.reds.size(), p2, H.dT, H.dHR, dhpsi, H.dscratch, s);
   if (timed) QSX_HIP(hipEventRecord(H.ev[2], s));
 }
 
 // plain per-pair path (no batching, no graph): the baseline
 static void hpsi_plain(HpsiPlan& H, const double* dpsi, double* dhpsi, hipStream_t s) {
-  auto run = [&](const std::vector<PairDesc>& pairs, const std::vector<TileDesc>& tiles, const double* A,
+  auto run = [&](const std::vector<PairDesc>& pairs, const std::vector<WorkTarget>& targets, const double* A,
                  const double* B, double* C) {
-    // walk the output blocks once: tiles of one block share c_off/pair range
-    std::map<int64_t, const TileDesc*> blocks;
-    for (auto& t : tiles) blocks.emplace(t.c_off, &t);
-    for (auto& [off, t] : blocks) {
-      if (t->p0 == t->p1) {
-        launch_fill_zero(C + off, (int64_t)t->M * t->N, s);
+    for (const WorkTarget& t : targets) {
+      if (t.p0 == t.p1) {
+        launch_fill_zero(C + t.c_off, (int64_t)t.M * t.N, s);
         continue;
       }
-      for (int p = t->p0; p < t->p1; ++p)
-        launch_pair_gemm_f64(pairs[p], off, t->M, t->N, p == t->p0 ? 0 : 1, A, B, C, s);
+      for (int64_t p = t.p0; p < t.p1; ++p)
+        launch_pair_gemm_f64(pairs[p], t.c_off, t.M, t.N, p == t.p0 ? 0 : 1, A, B, C, s);
     }
   };
-  run(H.pairs1, H.tiles1, H.dHL, dpsi, H.dT);
-  run(H.pairs2, H.tiles2, H.dT, H.dHR, dhpsi);
+  run(H.pairs1, H.targets1, H.dHL, dpsi, H.dT);
+  run(H.pairs2, H.targets2, H.dT, H.dHR, dhpsi);
 }
 
 static void hpsi_create(HpsiPlan& H, const qsx_tensor* HL, const qsx_tensor* psi, const qsx_tensor* HR,
@@ -236,6 +339,7 @@ static void hpsi_create(HpsiPlan& H, const qsx_tensor* HL, const qsx_tensor* psi
     H.nslices = o->nslices;
   }
   H.N = H.psi.total();
+  H.pol1.slots = H.pol2.slots = resident_slots();
   qsx_opts so{};
   so.mode = QSX_MODE_BATCHED;
   so.verbose = 0;
@@ -326,7 +430,7 @@ static void hpsi_create(HpsiPlan& H, const qsx_tensor* HL, const qsx_tensor* psi
       target.push_back(tg);
     }
     std::vector<std::pair<int, int64_t>> src1;
-    build_multi(plans, ab, bb, target, toffs, tM, tN, H.policy, H.pairs1, H.tiles1, nullptr, &src1);
+    build_multi(plans, ab, bb, target, toffs, tM, tN, H.pairs1, H.targets1, nullptr, &src1);
     // weights of psi's blocks: prod_j cgw_j^2 (|CData|_F = 1)
     std::vector<double> wblk(H.psi.n, 1.0);
     {
@@ -389,7 +493,7 @@ static void hpsi_create(HpsiPlan& H, const qsx_tensor* HL, const qsx_tensor* psi
       target.push_back(tg);
     }
     std::vector<int> tgt2;
-    build_multi(plans, ab, bb, target, toffs, tM, tN, H.policy, H.pairs2, H.tiles2, &tgt2, nullptr);
+    build_multi(plans, ab, bb, target, toffs, tM, tN, H.pairs2, H.targets2, &tgt2, nullptr);
     H.pairs2s = H.pairs2;
     for (size_t p = 0; p < H.pairs2s.size(); ++p) H.pairs2s[p].afac *= std::sqrt(H.wblk[tgt2[p]]);
   }
@@ -397,8 +501,7 @@ static void hpsi_create(HpsiPlan& H, const qsx_tensor* HL, const qsx_tensor* psi
   H.dpairs2 = upload_vec(H.pairs2, s);
   H.dpairs1s = upload_vec(H.pairs1s, s);
   H.dpairs2s = upload_vec(H.pairs2s, s);
-  H.dtiles1 = upload_vec(H.tiles1, s);
-  H.dtiles2 = upload_vec(H.tiles2, s);
+  hpsi_build_work(H);
   QSX_HIP(hipStreamSynchronize(s));
   H.flops = 0;
   for (int t = 0; t < H.nterm; ++t) {
@@ -699,11 +802,30 @@ int qsx_hpsi_stage_info(const qsx_hpsi_plan* plan, double* info) {
   info[1] = H.stage_flops[1];
   info[2] = H.stage_bytes[0];
   info[3] = H.stage_bytes[1];
-  info[4] = (double)H.tiles1.size();
-  info[5] = (double)H.tiles2.size();
+  info[4] = (double)H.work1.items.size();
+  info[5] = (double)H.work2.items.size();
   info[6] = (double)H.pairs1.size();
   info[7] = (double)H.pairs2.size();
   return 0;
+}
+
+int qsx_hpsi_autotune(qsx_hpsi_plan* plan, int32_t reps, double* stage_ms, int32_t* policy) {
+  if (!plan) return QSX_ERR_ARG;
+  try {
+    hpsi_autotune(*plan->h, reps > 0 ? reps : 5, stage_ms);
+    if (policy) {
+      policy[0] = plan->h->pol1.split_steps;
+      policy[1] = (int32_t)plan->h->pol1.tile_overhead;
+      policy[2] = plan->h->pol2.split_steps;
+      policy[3] = (int32_t)plan->h->pol2.tile_overhead;
+      policy[4] = plan->h->pol1.slots;
+      policy[5] = plan->h->pol2.slots;
+    }
+    return 0;
+  } catch (const Error& e) {
+    fprintf(stderr, "qsx_hpsi_autotune: %s\n", e.msg.c_str());
+    return e.code;
+  }
 }
 
 int qsx_hpsi_weights(const qsx_hpsi_plan* plan, double* w) {
@@ -720,12 +842,12 @@ int qsx_hpsi_use_graph(qsx_hpsi_plan* plan, int32_t on) {
 
 int qsx_hpsi_last_timing(const qsx_hpsi_plan* plan, double* ms2) {
   if (!plan || !ms2) return QSX_ERR_ARG;
-  float a = 0, b = 0;
   if (hipEventSynchronize(plan->h->ev[2]) != hipSuccess) return QSX_ERR_DEVICE;
-  if (hipEventElapsedTime(&a, plan->h->ev[0], plan->h->ev[1]) != hipSuccess) return QSX_ERR_DEVICE;
-  if (hipEventElapsedTime(&b, plan->h->ev[1], plan->h->ev[2]) != hipSuccess) return QSX_ERR_DEVICE;
-  ms2[0] = a;
-  ms2[1] = b;
+  for (int i = 0; i < 2; ++i) {
+    float t = 0;
+    if (hipEventElapsedTime(&t, plan->h->ev[i], plan->h->ev[i + 1]) != hipSuccess) return QSX_ERR_DEVICE;
+    ms2[i] = t;
+  }
   return 0;
 }
 

@@ -86,25 +86,75 @@ struct PairDesc {
   double afac;     // X-symbol x cgw x sign x output normalisation
 };
 
-enum TileCfg { TILE_32 = 0, TILE_64 = 1, TILE_NCFG = 2 };
+// tile shapes of the grouped GEMM (TM x TN, 4 waves per workgroup)
+enum TileShape { SH_64x64 = 0, SH_32x32 = 1, SH_16x64 = 2, SH_64x16 = 3, SH_32x64 = 4, SH_64x32 = 5, SH_NUM = 6 };
 
-struct TileDesc {
-  int64_t c_off;   // element offset of the output block
-  int32_t M, N;    // output block as MxN column-major (ldc = M)
-  int32_t m0, n0;  // tile origin
-  int32_t p0, p1;  // pair range of the output block
-  int32_t cfg;     // TILE_32 / TILE_64
-  int32_t pad;
+__host__ __device__ inline void shape_dims(int sh, int& tm, int& tn) {
+  switch (sh) {
+    case SH_64x64: tm = 64; tn = 64; break;
+    case SH_32x32: tm = 32; tn = 32; break;
+    case SH_16x64: tm = 16; tn = 64; break;
+    case SH_64x16: tm = 64; tn = 16; break;
+    case SH_32x64: tm = 32; tn = 64; break;
+    default: tm = 64; tn = 32; break;
+  }
+}
+
+// one work item: a TMxTN tile of one output block over a k-range that runs
+// from pair p0 at k0 to pair p1 (inclusive) ending at k1 (exclusive)
+struct WorkItem {
+  int64_t c_off;     // element offset of the output block
+  int32_t M, N;      // output block as MxN column-major (ldc = M)
+  int32_t m0, n0;    // tile origin
+  int32_t p0, k0, p1, k1;
+  int32_t shape;     // TileShape
+  int32_t slot;      // -1: store into C; >= 0: partial tile into scratch slot
 };
-int tile_dim(int cfg);
 
-void launch_grouped_gemm_f64(const TileDesc* dtiles, int ntiles, const PairDesc* dpairs, const double* A,
-                             const double* B, double* C, hipStream_t s);
+// sum of the partial tiles slot0 .. slot0+nslot-1 into one tile of C
+struct ReduceItem {
+  int64_t c_off;
+  int32_t M, N, m0, n0;
+  int32_t shape, slot0, nslot, pad;
+};
+
+// the items [i0, i1) one persistent workgroup runs; all of one shape
+struct WorkQueue {
+  int32_t i0, i1, shape, pad;
+};
+
+struct WorkPolicy {
+  int split_steps = 16;          // k-steps (of 16) per chunk before a tile is split
+  double tile_overhead = 1024;   // per-tile cost (in MACs per k) when choosing shapes
+  int shape_mask = (1 << SH_NUM) - 1;
+  int slots = 1024;              // resident workgroups of the persistent grid
+};
+
+// an output block and the range of its pairs in a pair array
+struct WorkTarget {
+  int64_t c_off;
+  int32_t M, N;
+  int64_t p0, p1;  // [p0, p1)
+};
+
+struct WorkList {
+  std::vector<WorkItem> items;   // in queue order
+  std::vector<WorkQueue> queues; // one per persistent workgroup
+  std::vector<ReduceItem> reds;  // one per split tile
+  int64_t scratch_elems = 0;     // partial-tile scratch (doubles)
+};
+
+WorkList build_work(const std::vector<WorkTarget>& targets, const std::vector<PairDesc>& pairs, const WorkPolicy& pol);
+// persistent grouped GEMM over the queues, then (if any tile is split) the reduction
+void launch_work(const WorkQueue* dqueues, int nqueues, const WorkItem* ditems, const ReduceItem* dreds, int nreds,
+                 const PairDesc* dpairs, const double* A, const double* B, double* C, double* scratch, hipStream_t s);
 void launch_copy(double* d, const double* src, int64_t n, hipStream_t s);
 void launch_pair_gemm_f64(const PairDesc& p, int64_t c_off, int M, int N, int beta, const double* A,
                           const double* B, double* C, hipStream_t s);
-void launch_grouped_gemm_split(int mode, int nslices, const TileDesc* dtiles, int ntiles,
-                               const PairDesc* dpairs, const double* A, const double* B, double* C, hipStream_t s);
+void launch_work_split(int mode, int nslices, const WorkQueue* dqueues, int nqueues, const WorkItem* ditems,
+                       const ReduceItem* dreds, int nreds,
+                       const PairDesc* dpairs, const double* A, const double* B, double* C, double* scratch,
+                       hipStream_t s);
 
 // generic block permute: dst block b = permute(src block b, perm) for all blocks
 struct PermBlock {

@@ -4,9 +4,11 @@
 
 namespace qsx {
 
-void launch_grouped_gemm_split(int mode, int nslices, const TileDesc* dtiles, int ntiles,
-                               const PairDesc* dpairs, const double* A, const double* B, double* C, hipStream_t s) {
-  (void)nslices; (void)dtiles; (void)ntiles; (void)dpairs; (void)A; (void)B; (void)C; (void)s;
+void launch_work_split(int mode, int nslices, const WorkQueue* dqueues, int nqueues, const WorkItem* ditems,
+                       const ReduceItem* dreds, int nreds, const PairDesc* dpairs, const double* A, const double* B,
+                       double* C, double* scratch, hipStream_t s) {
+  (void)nslices; (void)dqueues; (void)nqueues; (void)ditems; (void)dreds; (void)nreds; (void)dpairs; (void)A; (void)B; (void)C;
+  (void)scratch; (void)s;
   fail(QSX_ERR_UNSUPPORTED, "split-precision mode %d not available", mode);
 }
 

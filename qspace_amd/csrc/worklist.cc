@@ -1,0 +1,174 @@
+// worklist.cc -- host builder of the grouped-GEMM work items.
+//
+// Per output block: pick the tile shape that minimises padded work plus a
+// per-tile overhead, cut the block into tiles, and cut each tile's k-range
+// (the concatenated k-steps of the block's pairs, in the reference pair
+// order) into chunks of at most split_steps steps.  The items are then dealt
+// to the queues of the persistent grid (one queue per resident workgroup,
+// one tile shape per queue, longest-processing-time first).
+#include <algorithm>
+#include <cmath>
+
+#include "qsx_internal.hpp"
+
+namespace qsx {
+
+static constexpr int TKS = 16;  // k-step of the kernels
+
+WorkList build_work(const std::vector<WorkTarget>& targets, const std::vector<PairDesc>& pairs, const WorkPolicy& pol) {
+  struct Group {
+    std::vector<WorkItem> items;
+    double cost;  // per item
+  };
+  WorkList out;
+  std::vector<Group> groups;
+  int64_t slot = 0;
+  for (const WorkTarget& t : targets) {
+    const int M = t.M, N = t.N;
+    if (M <= 0 || N <= 0) continue;
+    int64_t steps = 0;
+    for (int64_t p = t.p0; p < t.p1; ++p) steps += (pairs[p].K + TKS - 1) / TKS;
+    int best = SH_64x64;
+    double bestc = 1e300;
+    for (int sh = 0; sh < SH_NUM; ++sh) {
+      if (!(pol.shape_mask & (1 << sh))) continue;
+      int tm, tn;
+      shape_dims(sh, tm, tn);
+      double nt = std::ceil((double)M / tm) * std::ceil((double)N / tn);
+      double c = nt * ((double)tm * tn * std::max<int64_t>(1, steps) + pol.tile_overhead);
+      if (c < bestc) {
+        bestc = c;
+        best = sh;
+      }
+    }
+    int tm, tn;
+    shape_dims(best, tm, tn);
+    const int nchunk = steps <= pol.split_steps ? 1 : (int)((steps + pol.split_steps - 1) / pol.split_steps);
+    std::vector<int64_t> bounds(nchunk + 1);
+    for (int q = 0; q <= nchunk; ++q) bounds[q] = steps * q / nchunk;
+    auto locate = [&](int64_t s, int& p, int& k) {  // global step -> (pair, k)
+      for (int64_t q = t.p0; q < t.p1; ++q) {
+        int64_t n = (pairs[q].K + TKS - 1) / TKS;
+        if (s < n) {
+          p = (int)q;
+          k = (int)(s * TKS);
+          return;
+        }
+        s -= n;
+      }
+      p = (int)t.p1;
+      k = 0;
+    };
+    for (int m0 = 0; m0 < M; m0 += tm)
+      for (int n0 = 0; n0 < N; n0 += tn) {
+        WorkItem base{};
+        base.c_off = t.c_off;
+        base.M = M;
+        base.N = N;
+        base.m0 = m0;
+        base.n0 = n0;
+        base.shape = best;
+        base.slot = -1;
+        const double tile_cost = (double)tm * tn;
+        Group g;
+        if (steps == 0) {  // no pairs: the tile stores zeros
+          base.p0 = 0;
+          base.p1 = -1;
+          g.items.push_back(base);
+          g.cost = 1.0;
+        } else if (nchunk == 1) {
+          base.p0 = (int)t.p0;
+          base.k0 = 0;
+          base.p1 = (int)t.p1 - 1;
+          base.k1 = pairs[t.p1 - 1].K;
+          g.items.push_back(base);
+          g.cost = tile_cost * steps;
+        } else {
+          for (int q = 0; q < nchunk; ++q) {
+            WorkItem w = base;
+            int pa, ka, pb, kb;
+            locate(bounds[q], pa, ka);
+            locate(bounds[q + 1] - 1, pb, kb);  // last step of the chunk
+            w.p0 = pa;
+            w.k0 = ka;
+            w.p1 = pb;
+            w.k1 = std::min(kb + TKS, pairs[pb].K);
+            w.slot = (int32_t)(slot + q);
+            g.items.push_back(w);
+          }
+          ReduceItem r{};
+          r.c_off = t.c_off;
+          r.M = M;
+          r.N = N;
+          r.m0 = m0;
+          r.n0 = n0;
+          r.shape = best;
+          r.slot0 = (int32_t)slot;
+          r.nslot = nchunk;
+          out.reds.push_back(r);
+          g.cost = tile_cost * (double)steps / nchunk;
+          slot += nchunk;
+        }
+        groups.push_back(std::move(g));
+      }
+  }
+  out.scratch_elems = slot * 64 * 64;  // uniform slot of 64x64 doubles (any shape fits)
+  // persistent queues: each shape gets a share of the resident workgroups in
+  // proportion to its work; within a shape, items are dealt longest first to
+  // the least loaded queue (LPT), so the queues finish close together
+  std::vector<std::vector<std::pair<WorkItem, double>>> byshape(SH_NUM);
+  for (auto& g : groups)
+    for (auto& it : g.items) byshape[it.shape].push_back({it, g.cost + pol.tile_overhead + 2.0 * 64 * 64});
+  double total = 0;
+  std::vector<double> scost(SH_NUM, 0.0);
+  for (int s = 0; s < SH_NUM; ++s)
+    for (auto& x : byshape[s]) scost[s] += x.second;
+  for (double c : scost) total += c;
+  const int slots = std::max(1, pol.slots);
+  std::vector<int> nq(SH_NUM, 0);
+  int used = 0;
+  for (int s = 0; s < SH_NUM; ++s) {
+    if (byshape[s].empty()) continue;
+    int n = (int)std::floor(slots * scost[s] / std::max(total, 1e-300));
+    n = std::max(1, std::min<int>(n, (int)byshape[s].size()));
+    nq[s] = n;
+    used += n;
+  }
+  // hand out the rounding remainder to the shapes with the longest queues
+  while (used < slots) {
+    int bs = -1;
+    double bl = 0;
+    for (int s = 0; s < SH_NUM; ++s)
+      if (nq[s] && nq[s] < (int)byshape[s].size() && scost[s] / nq[s] > bl) {
+        bl = scost[s] / nq[s];
+        bs = s;
+      }
+    if (bs < 0) break;
+    ++nq[bs];
+    ++used;
+  }
+  for (int s = 0; s < SH_NUM; ++s) {
+    if (!nq[s]) continue;
+    auto& v = byshape[s];
+    std::stable_sort(v.begin(), v.end(), [](const auto& x, const auto& y) { return x.second > y.second; });
+    std::vector<std::vector<WorkItem>> qs(nq[s]);
+    std::vector<double> load(nq[s], 0.0);
+    for (auto& x : v) {
+      int b = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      qs[b].push_back(x.first);
+      load[b] += x.second;
+    }
+    for (auto& qv : qs) {
+      if (qv.empty()) continue;
+      WorkQueue q{};
+      q.i0 = (int32_t)out.items.size();
+      for (auto& it : qv) out.items.push_back(it);
+      q.i1 = (int32_t)out.items.size();
+      q.shape = s;
+      out.queues.push_back(q);
+    }
+  }
+  return out;
+}
+
+}  // namespace qsx
