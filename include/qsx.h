@@ -161,6 +161,8 @@ int qsx_ctr_plan_pairs(const qsx_ctr_plan* plan, int64_t* npairs, int64_t* nout,
 int qsx_ctr_execute(qsx_ctr_plan* plan, const double* dA, const double* dB, double* dC,
                     void* stream);
 double qsx_ctr_plan_flops(const qsx_ctr_plan* plan);
+/* set the work decomposition policy (int32[QSX_POLICY_INTS], see below) */
+int qsx_ctr_plan_set_policy(qsx_ctr_plan* plan, const int32_t* policy);
 void qsx_ctr_plan_destroy(qsx_ctr_plan* plan);
 
 /* two-site H*psi and eigensolvers ---------------------------------------- */
@@ -184,12 +186,23 @@ int qsx_hpsi_weights(const qsx_hpsi_plan* plan, double* w);
    algorithmic bytes stage 1, stage 2, tiles stage 1, stage 2, pairs stage 1,
    stage 2 */
 int qsx_hpsi_stage_info(const qsx_hpsi_plan* plan, double* info);
-/* autotune the work decomposition of both stages (tile shapes, k-split
-   length) by timing candidates on this plan; stage_ms[2] = best times (or -1
-   when taken from the per-process cache), policy[6] = chosen split,
-   tile overhead of stage 1 and of stage 2, then the persistent grid size of
-   stage 1 and of stage 2 (either pointer may be NULL) */
+/* Work decomposition policy of one batched contraction, as
+   int32[QSX_POLICY_INTS]: [0] engine (0 workgroup tiles, 1 wave tiles),
+   [1] workgroup tiles: k-steps of 16 per chunk before a tile is split,
+   [2] workgroup tiles: per-tile overhead used when choosing shapes,
+   [3] workgroup tiles: persistent grid size, [4] wave tiles: max fragments
+   (of 16) per tile side, [5] wave tiles: MFMAs per tile above which the 4
+   waves of a workgroup share it, [6] wave tiles: MFMAs per workgroup before
+   the k-range is split over workgroups, [7] reserved (0).  Any policy gives
+   the same result to rounding (k-splits change the summation grouping). */
+#define QSX_POLICY_INTS 8
+/* autotune the work decomposition of both stages by timing candidates on
+   this plan; stage_ms[2] = best times (or -1 when taken from the
+   per-process cache), policy[2*QSX_POLICY_INTS] = the chosen policies of
+   stage 1 and stage 2 (either pointer may be NULL) */
 int qsx_hpsi_autotune(qsx_hpsi_plan* plan, int32_t reps, double* stage_ms, int32_t* policy);
+/* set the policies of both stages (policy[2*QSX_POLICY_INTS]) */
+int qsx_hpsi_set_policy(qsx_hpsi_plan* plan, const int32_t* policy);
 /* capture/replay the H*psi in a hipGraph (0 = direct launches) */
 int qsx_hpsi_use_graph(qsx_hpsi_plan* plan, int32_t on);
 /* per-kernel device timing of the last direct (non-graph) apply, HIP events

@@ -38,6 +38,26 @@ def make_opts(mode="batched", nslices=0, skip_zero=1, hconj=0, fermA=None, fermB
     return o, keep
 
 
+POLICY_KEYS = ("engine", "split_steps", "tile_overhead", "slots", "wave_max", "group_mfma", "split_mfma", "reserved")
+ENGINES = {"grouped": 0, "wave": 1}
+_POLICY_DEFAULT = {"engine": 1, "split_steps": 16, "tile_overhead": 1024, "slots": 1024, "wave_max": 4,
+                   "group_mfma": 1 << 30, "split_mfma": 1 << 30, "reserved": 0}
+
+
+def _policy_ints(p: dict):
+    """int32[8] exchange form of a work-decomposition policy (qsx.h)."""
+    d = dict(_POLICY_DEFAULT)
+    for k, v in p.items():
+        if k not in d:
+            raise KeyError(f"unknown policy key {k}")
+        d[k] = ENGINES[v] if k == "engine" and isinstance(v, str) else int(v)
+    return [d[k] for k in POLICY_KEYS]
+
+
+def _policy_dict(v):
+    return {k: int(x) for k, x in zip(POLICY_KEYS[:7], v)}
+
+
 def _ivec(v):
     a = np.ascontiguousarray(np.asarray(v, dtype=np.int32).reshape(-1))
     return a if a.size else np.zeros(1, np.int32)
@@ -222,6 +242,11 @@ class CtrPlan:
     def execute(self, dA: DeviceBuffer, dB: DeviceBuffer, dC: DeviceBuffer):
         _capi.check(_capi.lib().qsx_ctr_execute(self.h, dA.ptr, dB.ptr, dC.ptr, None), None)
 
+    def set_policy(self, **policy):
+        """work decomposition of the batched GEMM (keys of POLICY_KEYS)"""
+        v = (C.c_int32 * 8)(*_policy_ints(policy))
+        _capi.check(_capi.lib().qsx_ctr_plan_set_policy(self.h, v), None)
+
     def __del__(self):
         if getattr(self, "h", None) and self.h.value:
             _capi.lib().qsx_ctr_plan_destroy(self.h)
@@ -231,12 +256,14 @@ class CtrPlan:
 class HPsi:
     """Two-site H*psi = contractQS({HL,psi},HR) [+ hconj term], device resident."""
 
-    def __init__(self, HL: QSpace, psi: QSpace, HR: QSpace, mode="batched", hconj=1, fermA=None, nslices=0):
+    def __init__(self, HL: QSpace, psi: QSpace, HR: QSpace, mode="batched", hconj=1, fermA=None, nslices=0,
+                 verbose=0):
         L = _capi.lib()
         cL, self._k1 = to_c(HL)
         cP, self._k2 = to_c(psi, with_data=False)
         cR, self._k3 = to_c(HR)
-        self._opts, self._okeep = make_opts(mode=mode, hconj=hconj, fermA=fermA, nslices=nslices, skip_zero=0)
+        self._opts, self._okeep = make_opts(mode=mode, hconj=hconj, fermA=fermA, nslices=nslices, skip_zero=0,
+                                            verbose=verbose)
         self.h = C.c_void_p()
         err = C.create_string_buffer(1024)
         rc = L.qsx_hpsi_plan_create(C.byref(cL), C.byref(cP), C.byref(cR), C.byref(self._opts), C.byref(self.h),
@@ -254,11 +281,16 @@ class HPsi:
                 "pairs": (int(v[6]), int(v[7]))}
 
     def autotune(self, reps: int = 5):
+        """time candidate work decompositions of both stages and keep the
+        fastest: {"stage_ms": (t1, t2), "policy": (stage-1 dict, stage-2 dict)}"""
         ms = (C.c_double * 2)()
-        pol = (C.c_int32 * 6)()
+        pol = (C.c_int32 * 16)()
         _capi.check(_capi.lib().qsx_hpsi_autotune(self.h, reps, ms, pol), None)
-        return {"stage_ms": (ms[0], ms[1]), "split_steps": (pol[0], pol[2]), "tile_overhead": (pol[1], pol[3]),
-                "grid": (pol[4], pol[5])}
+        return {"stage_ms": (ms[0], ms[1]), "policy": (_policy_dict(pol[:8]), _policy_dict(pol[8:]))}
+
+    def set_policy(self, stage1: dict, stage2: dict):
+        v = (C.c_int32 * 16)(*(_policy_ints(stage1) + _policy_ints(stage2)))
+        _capi.check(_capi.lib().qsx_hpsi_set_policy(self.h, v), None)
 
     def weights(self) -> np.ndarray:
         w = np.zeros(self.N)

@@ -670,8 +670,7 @@ void plan_execute(CtrPlan& P, const double* dA, const double* dB, double* dC, hi
       if (P.pseg[blk] == P.pseg[blk + 1]) launch_fill_zero(c + P.offsOut[blk], P.offsOut[blk + 1] - P.offsOut[blk], s);
     }
   } else if (P.mode == QSX_MODE_BATCHED) {
-    launch_work(P.dqueues, (int)P.work.queues.size(), P.ditems, P.dreds, (int)P.work.reds.size(), P.dpairs, a, b, c,
-                P.dscratch, s);
+    launch_worklist(P.work, P.dqueues, P.ditems, P.dreds, P.dpairs, a, b, c, P.dscratch, s);
   } else {
     launch_work_split(P.mode, P.nslices, P.dqueues, (int)P.work.queues.size(), P.ditems, P.dreds,
                       (int)P.work.reds.size(), P.dpairs, a, b, c, P.dscratch, s);
@@ -1023,6 +1022,19 @@ int qsx_ctr_execute(qsx_ctr_plan* plan, const double* dA, const double* dB, doub
 }
 
 double qsx_ctr_plan_flops(const qsx_ctr_plan* plan) { return plan ? plan->p->flops : 0; }
+
+int qsx_ctr_plan_set_policy(qsx_ctr_plan* plan, const int32_t* policy) {
+  if (!plan || !policy) return QSX_ERR_ARG;
+  try {
+    plan->p->policy = policy_from_ints(policy);
+    plan_upload(*plan->p);
+    QSX_HIP(hipStreamSynchronize(default_stream()));
+    return 0;
+  } catch (const Error& e) {
+    fprintf(stderr, "qsx_ctr_plan_set_policy: %s\n", e.msg.c_str());
+    return e.code;
+  }
+}
 
 void qsx_ctr_plan_destroy(qsx_ctr_plan* plan) { delete plan; }
 

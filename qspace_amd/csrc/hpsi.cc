@@ -27,6 +27,7 @@ namespace qsx {
 struct HpsiPlan {
   TensorDesc HL, psi, HR;
   int nterm = 1;
+  int verbose = 0;
   int mode = QSX_MODE_BATCHED;
   int nslices = 0;
   std::unique_ptr<CtrPlan> s1[2], s2[2];
@@ -225,9 +226,9 @@ static double time_stage(HpsiPlan& H, int stage, const WorkList& W, const double
   for (int r = 0; r < reps + 2; ++r) {
     QSX_HIP(hipEventRecord(e0, s));
     if (stage == 0)
-      launch_work(dq, (int)W.queues.size(), di, dr, (int)W.reds.size(), p, H.dHL, dpsi, H.dT, sc, s);
+      launch_worklist(W, dq, di, dr, p, H.dHL, dpsi, H.dT, sc, s);
     else
-      launch_work(dq, (int)W.queues.size(), di, dr, (int)W.reds.size(), p, H.dT, H.dHR, dout, sc, s);
+      launch_worklist(W, dq, di, dr, p, H.dT, H.dHR, dout, sc, s);
     QSX_HIP(hipEventRecord(e1, s));
     QSX_HIP(hipEventSynchronize(e1));
     float t = 0;
@@ -265,20 +266,41 @@ static void hpsi_autotune(HpsiPlan& H, int reps, double* out) {
     const auto& pr = stage == 0 ? H.pairs1 : H.pairs2;
     double best = 1e300;
     WorkPolicy bp;
+    std::vector<WorkPolicy> cands;
     for (int sp : splits)
       for (double ov : overheads)
         for (int sl : slots) {
-        WorkPolicy pol;
-        pol.split_steps = sp;
-        pol.tile_overhead = ov;
-        pol.slots = sl;
-        WorkList W = build_work(tg, pr, pol);
-        double t = time_stage(H, stage, W, dpsi, dout, reps);
-        if (t < best) {
-          best = t;
-          bp = pol;
+          WorkPolicy pol;
+          pol.engine = ENG_GROUPED;
+          pol.split_steps = sp;
+          pol.tile_overhead = ov;
+          pol.slots = sl;
+          cands.push_back(pol);
         }
+    for (int wm : {1, 2})
+      for (int gm : {1 << 30, 256, 128, 64, 32})
+        for (int sm : {1 << 30, 512, 256, 128}) {
+          WorkPolicy pol;
+          pol.engine = ENG_WAVE;
+          pol.wave_max = wm;
+          pol.group_mfma = gm;
+          pol.split_mfma = sm;
+          cands.push_back(pol);
+        }
+    for (const WorkPolicy& pol : cands) {
+      WorkList W = build_work(tg, pr, pol);
+      double t = time_stage(H, stage, W, dpsi, dout, reps);
+      if (H.verbose)
+        fprintf(stderr,
+                "qsx autotune stage %d engine %d split %d/%d group %d overhead %g slots %d wave_max %d: %zu items "
+                "%zu reds %.4f ms\n",
+                stage + 1, W.engine, pol.split_steps, pol.split_mfma, pol.group_mfma, pol.tile_overhead, pol.slots,
+                pol.wave_max, W.items.size(), W.reds.size(), t);
+      if (t < best) {
+        best = t;
+        bp = pol;
       }
+    }
     (stage == 0 ? H.pol1 : H.pol2) = bp;
     if (out) out[stage] = best;
   }
@@ -294,15 +316,13 @@ static void hpsi_launch(HpsiPlan& H, const double* dpsi, double* dhpsi, hipStrea
   // events: [0] start, [1] stage 1 done, [2] stage 2 done
   if (timed) QSX_HIP(hipEventRecord(H.ev[0], s));
   if (H.mode == QSX_MODE_BATCHED || H.mode == QSX_MODE_PLAIN)
-    launch_work(H.dq1, (int)H.work1.queues.size(), H.ditems1, H.dreds1, (int)H.work1.reds.size(), p1, H.dHL, dpsi,
-                H.dT, H.dscratch, s);
+    launch_worklist(H.work1, H.dq1, H.ditems1, H.dreds1, p1, H.dHL, dpsi, H.dT, H.dscratch, s);
   else
     launch_work_split(H.mode, H.nslices, H.dq1, (int)H.work1.queues.size(), H.ditems1, H.dreds1,
                       (int)H.work1.reds.size(), p1, H.dHL, dpsi, H.dT, H.dscratch, s);
   if (timed) QSX_HIP(hipEventRecord(H.ev[1], s));
   if (H.mode == QSX_MODE_BATCHED || H.mode == QSX_MODE_PLAIN)
-    launch_work(H.dq2, (int)H.work2.queues.size(), H.ditems2, H.dreds2, (int)H.work2.reds.size(), p2, H.dT, H.dHR,
-                dhpsi, H.dscratch, s);
+    launch_worklist(H.work2, H.dq2, H.ditems2, H.dreds2, p2, H.dT, H.dHR, dhpsi, H.dscratch, s);
   else
     launch_work_split(H.mode, H.nslices, H.dq2, (int)H.work2.queues.size(), H.ditems2, H.dreds2,
                       (int)H.work2.reds.size(), p2, H.dT, H.dHR, dhpsi, H.dscratch, s);
@@ -337,6 +357,7 @@ static void hpsi_create(HpsiPlan& H, const qsx_tensor* HL, const qsx_tensor* psi
   if (o) {
     H.mode = o->mode;
     H.nslices = o->nslices;
+    H.verbose = o->verbose;
   }
   H.N = H.psi.total();
   H.pol1.slots = H.pol2.slots = resident_slots();
@@ -814,16 +835,25 @@ int qsx_hpsi_autotune(qsx_hpsi_plan* plan, int32_t reps, double* stage_ms, int32
   try {
     hpsi_autotune(*plan->h, reps > 0 ? reps : 5, stage_ms);
     if (policy) {
-      policy[0] = plan->h->pol1.split_steps;
-      policy[1] = (int32_t)plan->h->pol1.tile_overhead;
-      policy[2] = plan->h->pol2.split_steps;
-      policy[3] = (int32_t)plan->h->pol2.tile_overhead;
-      policy[4] = plan->h->pol1.slots;
-      policy[5] = plan->h->pol2.slots;
+      policy_to_ints(plan->h->pol1, policy);
+      policy_to_ints(plan->h->pol2, policy + QSX_POLICY_INTS);
     }
     return 0;
   } catch (const Error& e) {
     fprintf(stderr, "qsx_hpsi_autotune: %s\n", e.msg.c_str());
+    return e.code;
+  }
+}
+
+int qsx_hpsi_set_policy(qsx_hpsi_plan* plan, const int32_t* policy) {
+  if (!plan || !policy) return QSX_ERR_ARG;
+  try {
+    plan->h->pol1 = policy_from_ints(policy);
+    plan->h->pol2 = policy_from_ints(policy + QSX_POLICY_INTS);
+    hpsi_build_work(*plan->h);
+    return 0;
+  } catch (const Error& e) {
+    fprintf(stderr, "qsx_hpsi_set_policy: %s\n", e.msg.c_str());
     return e.code;
   }
 }

@@ -88,8 +88,17 @@ struct PairDesc {
 
 // tile shapes of the grouped GEMM (TM x TN, 4 waves per workgroup)
 enum TileShape { SH_64x64 = 0, SH_32x32 = 1, SH_16x64 = 2, SH_64x16 = 3, SH_32x64 = 4, SH_64x32 = 5, SH_NUM = 6 };
+// wave tiles (one wave per item): shape SH_WAVE + (fm-1)*4 + (fn-1) is a
+// (16*fm) x (16*fn) tile, fm, fn in 1..4
+constexpr int SH_WAVE = 64;
+__host__ __device__ inline int wave_shape(int fm, int fn) { return SH_WAVE + (fm - 1) * 4 + (fn - 1); }
 
 __host__ __device__ inline void shape_dims(int sh, int& tm, int& tn) {
+  if (sh >= SH_WAVE) {
+    tm = 16 * ((sh - SH_WAVE) / 4 + 1);
+    tn = 16 * ((sh - SH_WAVE) % 4 + 1);
+    return;
+  }
   switch (sh) {
     case SH_64x64: tm = 64; tn = 64; break;
     case SH_32x32: tm = 32; tn = 32; break;
@@ -108,7 +117,11 @@ struct WorkItem {
   int32_t m0, n0;    // tile origin
   int32_t p0, k0, p1, k1;
   int32_t shape;     // TileShape
-  int32_t slot;      // -1: store into C; >= 0: partial tile into scratch slot
+  int32_t slot;      // -1: store into C; >= 0: partial tile into scratch slot;
+                     // -2 (wave engine): no work (padding of a workgroup)
+  int32_t group;     // wave engine: 1 if the workgroup's 4 waves share this
+                     // tile, each over its own k-range, summed through LDS
+  int32_t pad;
 };
 
 // sum of the partial tiles slot0 .. slot0+nslot-1 into one tile of C
@@ -123,12 +136,41 @@ struct WorkQueue {
   int32_t i0, i1, shape, pad;
 };
 
+enum Engine { ENG_GROUPED = 0, ENG_WAVE = 1 };
+
 struct WorkPolicy {
+  int engine = ENG_WAVE;         // workgroup tiles (grouped.hip) or wave tiles (wave.hip)
+  int wave_max = 4;              // wave tiles: at most wave_max x wave_max fragments of 16x16
+  int split_mfma = 1 << 30;      // wave tiles: MFMAs per workgroup item before the k-range is split
+  int group_mfma = 1 << 30;      // wave tiles: MFMAs per tile above which 4 waves share it
   int split_steps = 16;          // k-steps (of 16) per chunk before a tile is split
   double tile_overhead = 1024;   // per-tile cost (in MACs per k) when choosing shapes
   int shape_mask = (1 << SH_NUM) - 1;
   int slots = 1024;              // resident workgroups of the persistent grid
 };
+
+// exchange form of a policy: int32[QSX_POLICY_INTS] (see qsx.h)
+inline void policy_to_ints(const WorkPolicy& p, int32_t* v) {
+  v[0] = p.engine;
+  v[1] = p.split_steps;
+  v[2] = (int32_t)p.tile_overhead;
+  v[3] = p.slots;
+  v[4] = p.wave_max;
+  v[5] = p.group_mfma;
+  v[6] = p.split_mfma;
+  v[7] = 0;
+}
+inline WorkPolicy policy_from_ints(const int32_t* v) {
+  WorkPolicy p;
+  p.engine = v[0] == ENG_GROUPED ? ENG_GROUPED : ENG_WAVE;
+  p.split_steps = v[1] > 0 ? v[1] : 1;
+  p.tile_overhead = v[2] >= 0 ? v[2] : 0;
+  p.slots = v[3] > 0 ? v[3] : 1;
+  p.wave_max = v[4] >= 1 && v[4] <= 4 ? v[4] : 4;
+  p.group_mfma = v[5] > 0 ? v[5] : 1;
+  p.split_mfma = v[6] > 0 ? v[6] : 1;
+  return p;
+}
 
 // an output block and the range of its pairs in a pair array
 struct WorkTarget {
@@ -138,7 +180,9 @@ struct WorkTarget {
 };
 
 struct WorkList {
-  std::vector<WorkItem> items;   // in queue order
+  int engine = ENG_GROUPED;
+  int maxf = 4;                  // wave engine: largest fragment count per tile side
+  std::vector<WorkItem> items;   // in queue order (wave engine: longest first)
   std::vector<WorkQueue> queues; // one per persistent workgroup
   std::vector<ReduceItem> reds;  // one per split tile
   int64_t scratch_elems = 0;     // partial-tile scratch (doubles)
@@ -148,6 +192,19 @@ WorkList build_work(const std::vector<WorkTarget>& targets, const std::vector<Pa
 // persistent grouped GEMM over the queues, then (if any tile is split) the reduction
 void launch_work(const WorkQueue* dqueues, int nqueues, const WorkItem* ditems, const ReduceItem* dreds, int nreds,
                  const PairDesc* dpairs, const double* A, const double* B, double* C, double* scratch, hipStream_t s);
+// one wave per item, then (if any tile is split) the reduction
+void launch_wave(const WorkItem* ditems, int nitems, int maxf, const ReduceItem* dreds, int nreds,
+                 const PairDesc* dpairs, const double* A, const double* B, double* C, double* scratch,
+                 hipStream_t s);
+// the launch that matches the work list's engine
+inline void launch_worklist(const WorkList& W, const WorkQueue* dqueues, const WorkItem* ditems,
+                            const ReduceItem* dreds, const PairDesc* dpairs, const double* A, const double* B,
+                            double* C, double* scratch, hipStream_t s) {
+  if (W.engine == ENG_WAVE)
+    launch_wave(ditems, (int)W.items.size(), W.maxf, dreds, (int)W.reds.size(), dpairs, A, B, C, scratch, s);
+  else
+    launch_work(dqueues, (int)W.queues.size(), ditems, dreds, (int)W.reds.size(), dpairs, A, B, C, scratch, s);
+}
 void launch_copy(double* d, const double* src, int64_t n, hipStream_t s);
 void launch_pair_gemm_f64(const PairDesc& p, int64_t c_off, int M, int N, int beta, const double* A,
                           const double* B, double* C, hipStream_t s);

@@ -13,9 +13,161 @@
 
 namespace qsx {
 
-static constexpr int TKS = 16;  // k-step of the kernels
+static constexpr int TKS = 16;  // k-step of the workgroup-tile kernel
+static constexpr int WKS = 4;   // k-step of the wave-tile kernel (one MFMA)
+
+// Wave tiles: per output block, ceil(M/16) x ceil(N/16) fragments cut into
+// tiles of at most wave_max x wave_max fragments (fragments spread evenly
+// over the tiles).  A tile's k-range is the concatenation of its pairs'
+// k-steps (of 4, reference pair order).  A tile with more than group_mfma
+// MFMAs is shared by the 4 waves of a workgroup (contiguous k-ranges, partials
+// summed through LDS); one with more than split_mfma MFMAs per workgroup is
+// further cut into chunks whose partial tiles the reduction sums.  Other
+// tiles run one per wave, four to a workgroup.  Workgroups are ordered
+// longest first: the hardware issues them in order to free slots, which is
+// list scheduling with the longest-processing-time rule.
+static WorkList build_wave_work(const std::vector<WorkTarget>& targets, const std::vector<PairDesc>& pairs,
+                                const WorkPolicy& pol) {
+  WorkList out;
+  out.engine = ENG_WAVE;
+  const int maxf = std::max(1, std::min(4, pol.wave_max));
+  out.maxf = maxf;
+  struct WG {
+    double cost;
+    std::vector<WorkItem> it;
+  };
+  std::vector<WG> groups;                          // workgroups of 4 tasks of one tile
+  std::vector<std::pair<double, WorkItem>> solo;   // one-wave items
+  int64_t slot = 0;
+  for (const WorkTarget& t : targets) {
+    const int M = t.M, N = t.N;
+    if (M <= 0 || N <= 0) continue;
+    int64_t steps = 0;
+    for (int64_t p = t.p0; p < t.p1; ++p) steps += (pairs[p].K + WKS - 1) / WKS;
+    const int mf = (M + 15) / 16, nf = (N + 15) / 16;
+    const int ntm = (mf + maxf - 1) / maxf, ntn = (nf + maxf - 1) / maxf;
+    // item over the global k-steps [s0, s1) of the tile
+    auto range = [&](WorkItem w, int64_t s0, int64_t s1) {
+      if (s0 >= s1) {
+        w.p0 = 0;
+        w.p1 = -1;
+        return w;
+      }
+      auto locate = [&](int64_t s, int& p, int& k) {
+        for (int64_t q = t.p0; q < t.p1; ++q) {
+          const int64_t n = (pairs[q].K + WKS - 1) / WKS;
+          if (s < n) {
+            p = (int)q;
+            k = (int)(s * WKS);
+            return;
+          }
+          s -= n;
+        }
+        p = (int)t.p1;
+        k = 0;
+      };
+      int kb;
+      locate(s0, w.p0, w.k0);
+      locate(s1 - 1, w.p1, kb);
+      w.k1 = std::min(kb + WKS, pairs[w.p1].K);
+      return w;
+    };
+    for (int ti = 0, m0f = 0; ti < ntm; ++ti) {
+      const int fm = mf / ntm + (ti < mf % ntm ? 1 : 0);
+      for (int tj = 0, n0f = 0; tj < ntn; ++tj) {
+        const int fn = nf / ntn + (tj < nf % ntn ? 1 : 0);
+        WorkItem base{};
+        base.c_off = t.c_off;
+        base.M = M;
+        base.N = N;
+        base.m0 = 16 * m0f;
+        base.n0 = 16 * n0f;
+        base.shape = wave_shape(fm, fn);
+        base.slot = -1;
+        base.group = 0;
+        const double per_step = (double)fm * fn;
+        const double cost = per_step * (double)steps;
+        const bool grp = fm * fn <= 4 && cost > pol.group_mfma && steps >= 4;
+        const int ways = grp ? 4 : 1;
+        const double per_wg = std::max<double>(1.0, pol.split_mfma);
+        const int nchunk = (int)std::max<double>(1.0, std::ceil(cost / ways / per_wg));
+        const int64_t nch = std::min<int64_t>(nchunk, std::max<int64_t>(1, steps / ways));
+        for (int64_t q = 0; q < nch; ++q) {
+          const int64_t c0 = steps * q / nch, c1 = steps * (q + 1) / nch;
+          WorkItem w = base;
+          if (nch > 1) w.slot = (int32_t)(slot + q);
+          if (grp) {
+            WG g;
+            g.cost = per_step * (double)((c1 - c0 + 3) / 4) + 16.0;
+            for (int v = 0; v < 4; ++v) {
+              WorkItem x = range(w, c0 + (c1 - c0) * v / 4, c0 + (c1 - c0) * (v + 1) / 4);
+              x.group = 1;
+              g.it.push_back(x);
+            }
+            groups.push_back(std::move(g));
+          } else {
+            solo.push_back({per_step * (double)(c1 - c0) + 8.0, range(w, c0, c1)});
+          }
+        }
+        if (nch > 1) {
+          ReduceItem r{};
+          r.c_off = t.c_off;
+          r.M = M;
+          r.N = N;
+          r.m0 = base.m0;
+          r.n0 = base.n0;
+          r.shape = base.shape;
+          r.slot0 = (int32_t)slot;
+          r.nslot = (int32_t)nch;
+          out.reds.push_back(r);
+          slot += nch;
+        }
+        n0f += fn;
+      }
+      m0f += fm;
+    }
+  }
+  // solo items longest first, four to a workgroup (the last padded)
+  std::stable_sort(solo.begin(), solo.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
+  for (size_t i = 0; i < solo.size(); i += 4) {
+    WG g;
+    g.cost = solo[i].first;
+    for (size_t j = i; j < i + 4; ++j) {
+      if (j < solo.size()) {
+        g.it.push_back(solo[j].second);
+      } else {
+        WorkItem pad{};
+        pad.slot = -2;
+        pad.p0 = 0;
+        pad.p1 = -1;
+        pad.shape = wave_shape(1, 1);
+        g.it.push_back(pad);
+      }
+    }
+    groups.push_back(std::move(g));
+  }
+  std::stable_sort(groups.begin(), groups.end(), [](const WG& x, const WG& y) { return x.cost > y.cost; });
+  for (const WG& g : groups)
+    for (const WorkItem& w : g.it) out.items.push_back(w);
+  out.scratch_elems = slot * 64 * 64;
+  return out;
+}
+
+// the wave kernel addresses operand elements with 32-bit byte offsets inside
+// a block
+static bool wave_fits(const std::vector<WorkTarget>& targets, const std::vector<PairDesc>& pairs) {
+  for (const WorkTarget& t : targets)
+    for (int64_t p = t.p0; p < t.p1; ++p) {
+      const PairDesc& d = pairs[p];
+      const int64_t ea = (int64_t)d.lda * ((d.flags & 1) ? t.M : d.K);
+      const int64_t eb = (int64_t)d.ldb * ((d.flags & 2) ? d.K : t.N);
+      if (std::max(ea, eb) >= (int64_t(1) << 26)) return false;
+    }
+  return true;
+}
 
 WorkList build_work(const std::vector<WorkTarget>& targets, const std::vector<PairDesc>& pairs, const WorkPolicy& pol) {
+  if (pol.engine == ENG_WAVE && wave_fits(targets, pairs)) return build_wave_work(targets, pairs, pol);
   struct Group {
     std::vector<WorkItem> items;
     double cost;  // per item

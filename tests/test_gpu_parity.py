@@ -134,7 +134,7 @@ def test_hpsi_hermitian_full_size(gpu):
 
 def test_hpsi_deterministic_graph_replays(gpu):
     """split-k tiles reduce in a fixed order: graph replays and direct launches
-    give bitwise identical H*psi (and the arrival counters reset)."""
+    give bitwise identical H*psi."""
     HL, psi, HR = gen.heis_hpsi(seed=1003, Dstar=2000)
     H = Q.HPsi(HL, psi, HR, hconj=1)
     x = np.random.RandomState(3).standard_normal(H.N)
@@ -148,3 +148,47 @@ def test_hpsi_deterministic_graph_replays(gpu):
         outs.append(dy.download())
     for o in outs[1:]:
         assert np.array_equal(o, outs[0])
+
+
+# every work decomposition (engine, tile size, workgroup sharing, k-splits)
+# is the same contraction: each must meet the oracle bar
+POLICIES = [
+    {"engine": "grouped", "split_steps": 1 << 28},
+    {"engine": "grouped", "split_steps": 2, "tile_overhead": 0},
+    {"engine": "wave", "wave_max": 4},
+    {"engine": "wave", "wave_max": 1, "group_mfma": 1},
+    {"engine": "wave", "wave_max": 2, "group_mfma": 8, "split_mfma": 16},
+    {"engine": "wave", "wave_max": 2, "split_mfma": 8},
+    {"engine": "wave", "wave_max": 3, "group_mfma": 4, "split_mfma": 64},
+]
+
+
+@pytest.mark.parametrize("pi", range(len(POLICIES)))
+def test_hpsi_policies_match_oracle(gpu, pi):
+    """blocks up to 90 wide (tiles of several fragments, long k-ranges) under
+    each policy vs the oracle; two applies bitwise equal (fixed sum order)."""
+    HL, psi, HR = gen.heis_hpsi(seed=23, Dstar=400, mu=3.0, sigma=0.7, dmax=90)
+    H = Q.HPsi(HL, psi, HR, hconj=1)
+    H.set_policy(POLICIES[pi], POLICIES[pi])
+    v = psi.packed()
+    got = H.apply(v)
+    ref = _hpsi_oracle(HL, psi, HR, 1).packed()
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < TOL
+    assert np.array_equal(H.apply(v), got)
+
+
+@pytest.mark.parametrize("pi", range(len(POLICIES)))
+def test_contract_policies_match_oracle(gpu, pi):
+    """SU(2) environment step L.A (long k: the L bond) under each policy."""
+    A, L = gen.su2_mps(seed=13, tS=tuple(range(9)), dmin=20, dmax=70)
+    P = Q.CtrPlan(L, "2", A, "1")
+    P.set_policy(**POLICIES[pi])
+    dA = Q.DeviceBuffer.from_array(L.packed())
+    dB = Q.DeviceBuffer.from_array(A.packed())
+    dC = Q.DeviceBuffer(P.nout)
+    P.execute(dA, dB, dC)
+    got = dC.download()
+    R = O.contract(L, [1], A, [0], skip_zero=0)
+    ref = R.packed()
+    assert got.shape == ref.shape
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < TOL
