@@ -3,7 +3,12 @@
 Default workload: spin-1/2 Heisenberg two-site H*psi in SU(2) at D* = 4000
 kept multiplets (enlarged-bond psi, rank-3 environments, hconj term), FP64.
 A "step" is one H*psi on device-resident inputs (graph replay of the two
-grouped-GEMM launches).  Prints ONE JSON line.
+batched-GEMM launches).  Prints ONE JSON line.
+
+Before the timed region the run also makes one device copy of 64 MiB
+(k_copy, 64 MiB read + 64 MiB written): counter runs of this same command
+(rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE) calibrate the per-dispatch HBM
+byte counts of the GEMM kernels against it.
 
     python bench.py [--steps K] [--warmup W] [--workload heis4000|heis2000|hubbard4000]
                     [--no-cpu] [--extra]
@@ -55,6 +60,21 @@ def restrict(T, keep_fn):
     from qspace_amd import QSpace
     idx = [i for i, r in enumerate(T.QIDX) if keep_fn(r)]
     return QSpace([q[idx] for q in T.Q], [T.data[i] for i in idx], T.qtype, T.itags)
+
+
+CALIB_MIB = 64
+
+
+def calibration_copy():
+    """one k_copy of CALIB_MIB MiB (known HBM traffic) for counter calibration"""
+    import qspace_amd as Q
+    n = CALIB_MIB * (1 << 20) // 8
+    a = Q.DeviceBuffer(n)
+    b = Q.DeviceBuffer(n)
+    Q._capi.lib().qsx_memset_device(a.ptr, 0, n * 8, None)
+    Q._capi.lib().qsx_copy_device(b.ptr, a.ptr, n, None)
+    Q._capi.lib().qsx_synchronize()
+    return {"kernel": "k_copy", "read_bytes": n * 8, "write_bytes": n * 8}
 
 
 def time_hpsi(H, dpsi, dout, steps, warmup, graph=True):
@@ -146,6 +166,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="heis4000", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--lite", action="store_true",
+                    help="timed H*psi only (no CPU baseline, plain path, Lanczos): for profiler runs")
+    ap.add_argument("--policy", default=None,
+                    help="JSON file {stage1: {...}, stage2: {...}} fixing the work decomposition (no autotune)")
     ap.add_argument("--extra", action="store_true", help="also time Lanczos/plain path at extra sizes")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -166,12 +190,19 @@ def main():
     H = Q.HPsi(HL, psi, HR, hconj=1, fermA=ferm)
     plan_s = time.perf_counter() - t0
     t0 = time.perf_counter()
-    tune = H.autotune(5)
+    if args.policy:
+        with open(args.policy) as f:
+            pj = json.load(f)
+        H.set_policy(pj["stage1"], pj["stage2"])
+        tune = {"stage_ms": (-1.0, -1.0), "policy": (pj["stage1"], pj["stage2"])}
+    else:
+        tune = H.autotune(5)
     tune_s = time.perf_counter() - t0
     log(f"[bench] autotune {tune_s:.2f} s: {tune}")
     info = H.stage_info()
     log(f"[bench] plan {plan_s:.2f} s; N={H.N} flops/H*psi={H.flops / 1e9:.3f} GFLOP pairs={info['pairs']} "
-        f"tiles={info['tiles']}")
+        f"items={info['tiles']} kernels={info['kernel']} grid={info['grid']} reduce={info['reduce_grid']}")
+    calib = calibration_copy()
     rng = np.random.RandomState(7)
     x = rng.standard_normal(H.N)
     dpsi = Q.DeviceBuffer.from_array(x)
@@ -181,27 +212,29 @@ def main():
     ms_nograph = time_hpsi(H, dpsi, dout, max(5, args.steps // 5), 2, graph=False)
     kt = stage_timing(H, dpsi, dout, 20)
     t1, t2 = kt[0], kt[1]
-    log(f"[bench] kernels grouped1 {kt[0]:.4f} grouped2 {kt[1]:.4f} ms")
-    # plain FP64 per-pair GPU path (one launch per pair, no graph)
-    Hp = Q.HPsi(HL, psi, HR, hconj=1, fermA=ferm, mode="plain")
-    ms_plain = time_hpsi(Hp, dpsi, dout, 3, 1, graph=False)
-    hp = dout.download()
-    H.apply_dev(dpsi, dout)
-    hb = dout.download()
-    err_plain = float(np.linalg.norm(hb - hp) / np.linalg.norm(hp))
-    log(f"[bench] plain per-pair FP64 H*psi {ms_plain:.3f} ms (batched/plain rel diff {err_plain:.2e})")
-    del Hp
-    # Hermiticity in the full-space inner product (size-independent check)
-    wts = H.weights()
-    y = rng.standard_normal(H.N)
-    hx = H.apply(x)
-    hy = H.apply(y)
-    herm = float(abs((wts * x) @ hy - (wts * hx) @ y) / (np.sqrt((wts * hx) @ hx) * np.sqrt((wts * y) @ y)))
-    # Lanczos, 20 iterations in one graph
-    t0 = time.perf_counter()
-    e0, ritz, gs = H.lanczos(psi.packed(), 20)
-    lan_ms = (time.perf_counter() - t0) * 1e3
-    log(f"[bench] lanczos 20 it: {lan_ms:.1f} ms, E0={e0:.10f}")
+    log(f"[bench] stage kernels {info['kernel'][0]} {kt[0]:.4f} ms, {info['kernel'][1]} {kt[1]:.4f} ms")
+    ms_plain = err_plain = herm = lan_ms = e0 = None
+    if not args.lite:
+        # plain FP64 per-pair GPU path (one launch per pair, no graph)
+        Hp = Q.HPsi(HL, psi, HR, hconj=1, fermA=ferm, mode="plain")
+        ms_plain = time_hpsi(Hp, dpsi, dout, 3, 1, graph=False)
+        hp = dout.download()
+        H.apply_dev(dpsi, dout)
+        hb = dout.download()
+        err_plain = float(np.linalg.norm(hb - hp) / np.linalg.norm(hp))
+        log(f"[bench] plain per-pair FP64 H*psi {ms_plain:.3f} ms (batched/plain rel diff {err_plain:.2e})")
+        del Hp
+        # Hermiticity in the full-space inner product (size-independent check)
+        wts = H.weights()
+        y = rng.standard_normal(H.N)
+        hx = H.apply(x)
+        hy = H.apply(y)
+        herm = float(abs((wts * x) @ hy - (wts * hx) @ y) / (np.sqrt((wts * hx) @ hx) * np.sqrt((wts * y) @ y)))
+        # Lanczos, 20 iterations in one graph
+        t0 = time.perf_counter()
+        e0, ritz, gs = H.lanczos(psi.packed(), 20)
+        lan_ms = (time.perf_counter() - t0) * 1e3
+        log(f"[bench] lanczos 20 it: {lan_ms:.1f} ms, E0={e0:.10f}")
     # dominant kernel roofline
     dom = 0 if t1 >= t2 else 1
     tk = (t1, t2)[dom]
@@ -221,7 +254,9 @@ def main():
         roof = {"bound": "hbm", "achieved": by / (tk * 1e-3) / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s"}
     roof["frac"] = roof["achieved"] / roof["peak"]
     roof["traffic"] = traffic
-    roof["kernel"] = f"k_grouped (H*psi stage {dom + 1})"
+    roof["kernel"] = f"{info['kernel'][dom]} (H*psi stage {dom + 1})"
+    roof["kernel_grid"] = info["grid"][dom]
+    roof["reduce_grid"] = info["reduce_grid"][dom]
     roof["kernel_ms"] = {"stage1": kt[0], "stage2": kt[1]}
     roof["algorithmic_flops"] = fl
     roof["algorithmic_bytes"] = by
@@ -229,7 +264,7 @@ def main():
     roof["fp64_mfma_frac"] = fl / (tk * 1e-3) / 1e12 / PEAK_FP64_TFLOPS
     roof["hbm_frac"] = by / (tk * 1e-3) / 1e9 / PEAK_HBM_GBS
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and not args.lite:
         log("[bench] cpu baseline (sample) ...")
         cpu = cpu_baseline(w)
         log(f"[bench] cpu {cpu['value']:.2f} GFLOP/s on {cpu['cores']} cores")
@@ -249,19 +284,21 @@ def main():
         "data": "synthetic (seeded lognormal sector dims, N(0,1) block entries)",
         "config": {"workload": w["name"], "Dstar": w["Dstar"], "psi_elements": H.N, "hconj": 1,
                    "useful_gflop_per_hpsi": H.flops / 1e9, "pairs": list(info["pairs"]),
-                   "tiles": list(info["tiles"]), "path": "batched grouped FP64 MFMA + hipGraph"},
+                   "items": list(info["tiles"]), "kernels": list(info["kernel"]), "grid": list(info["grid"]),
+                   "path": "batched FP64 MFMA (wave-tile or workgroup-tile engine, autotuned) + hipGraph"},
         "roofline": roof,
         "cpu_baseline": cpu,
         "hpsi_ms": ms,
         "hpsi_ms_no_graph": ms_nograph,
         "plain_fp64_per_pair_ms": ms_plain,
-        "speedup_vs_plain_fp64": ms_plain / ms,
+        "speedup_vs_plain_fp64": ms_plain / ms if ms_plain else None,
         "rel_diff_batched_vs_plain": err_plain,
         "hermiticity_err_full_size": herm,
         "lanczos20_ms": lan_ms,
         "lanczos20_E0": e0,
         "plan_create_s": plan_s,
         "autotune": {"seconds": tune_s, **{k: list(v) for k, v in tune.items()}},
+        "calibration": calib,
     }
     if cpu is not None:
         out["rel_err_vs_fp64_oracle"] = cpu["rel_err_gpu_vs_fp64_oracle_sample"]

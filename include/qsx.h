@@ -182,9 +182,12 @@ int qsx_hpsi_apply(qsx_hpsi_plan* plan, const double* dpsi, double* dhpsi, void*
    to 1): <x,y> = sum_i w_i x_i y_i is the inner product in which H*psi is
    symmetric; the eigensolvers work in the basis sqrt(w) x */
 int qsx_hpsi_weights(const qsx_hpsi_plan* plan, double* w);
-/* per-stage work of one H*psi: info[0..7] = flops stage 1, flops stage 2,
-   algorithmic bytes stage 1, stage 2, tiles stage 1, stage 2, pairs stage 1,
-   stage 2 */
+/* per-stage work of one H*psi, info[QSX_STAGE_INFO]: [0..1] useful flops
+   of stage 1, 2; [2..3] algorithmic bytes; [4..5] work items; [6..7] pairs;
+   [8..9] engine (0 workgroup tiles, 1 wave tiles); [10..11] workgroups of
+   the main GEMM launch; [12..13] largest fragments per tile side (wave
+   tiles); [14..15] reduction workgroups (0: no split tiles) */
+#define QSX_STAGE_INFO 16
 int qsx_hpsi_stage_info(const qsx_hpsi_plan* plan, double* info);
 /* Work decomposition policy of one batched contraction, as
    int32[QSX_POLICY_INTS]: [0] engine (0 workgroup tiles, 1 wave tiles),
@@ -193,9 +196,12 @@ int qsx_hpsi_stage_info(const qsx_hpsi_plan* plan, double* info);
    [3] workgroup tiles: persistent grid size, [4] wave tiles: max fragments
    (of 16) per tile side, [5] wave tiles: MFMAs per tile above which the 4
    waves of a workgroup share it, [6] wave tiles: MFMAs per workgroup before
-   the k-range is split over workgroups, [7] reserved (0).  Any policy gives
+   the k-range is split over workgroups, [7] wave tiles: XCD-aware
+   workgroup order over this many L2 domains (0 or 1: longest first only),
+   [8] wave tiles of at most 2x2 fragments: waves per workgroup = waves
+   sharing one long tile (4, 8 or 16), [9..11] reserved (0).  Any policy gives
    the same result to rounding (k-splits change the summation grouping). */
-#define QSX_POLICY_INTS 8
+#define QSX_POLICY_INTS 12
 /* autotune the work decomposition of both stages by timing candidates on
    this plan; stage_ms[2] = best times (or -1 when taken from the
    per-process cache), policy[2*QSX_POLICY_INTS] = the chosen policies of

@@ -38,10 +38,13 @@ def make_opts(mode="batched", nslices=0, skip_zero=1, hconj=0, fermA=None, fermB
     return o, keep
 
 
-POLICY_KEYS = ("engine", "split_steps", "tile_overhead", "slots", "wave_max", "group_mfma", "split_mfma", "reserved")
+POLICY_KEYS = ("engine", "split_steps", "tile_overhead", "slots", "wave_max", "group_mfma", "split_mfma", "xcd_bins",
+               "group_waves", "r9", "r10", "r11")
+NPOL = len(POLICY_KEYS)
 ENGINES = {"grouped": 0, "wave": 1}
-_POLICY_DEFAULT = {"engine": 1, "split_steps": 16, "tile_overhead": 1024, "slots": 1024, "wave_max": 4,
-                   "group_mfma": 1 << 30, "split_mfma": 1 << 30, "reserved": 0}
+_POLICY_DEFAULT = {"engine": 1, "split_steps": 16, "tile_overhead": 1024, "slots": 1024, "wave_max": 2,
+                   "group_mfma": 64, "split_mfma": 1 << 30, "xcd_bins": 8, "group_waves": 4, "r9": 0,
+                   "r10": 0, "r11": 0}
 
 
 def _policy_ints(p: dict):
@@ -55,7 +58,7 @@ def _policy_ints(p: dict):
 
 
 def _policy_dict(v):
-    return {k: int(x) for k, x in zip(POLICY_KEYS[:7], v)}
+    return {k: int(x) for k, x in zip(POLICY_KEYS[:9], v)}
 
 
 def _ivec(v):
@@ -244,7 +247,7 @@ class CtrPlan:
 
     def set_policy(self, **policy):
         """work decomposition of the batched GEMM (keys of POLICY_KEYS)"""
-        v = (C.c_int32 * 8)(*_policy_ints(policy))
+        v = (C.c_int32 * NPOL)(*_policy_ints(policy))
         _capi.check(_capi.lib().qsx_ctr_plan_set_policy(self.h, v), None)
 
     def __del__(self):
@@ -275,21 +278,30 @@ class HPsi:
         self.bytes = float(L.qsx_hpsi_bytes(self.h))
 
     def stage_info(self) -> dict:
-        v = (C.c_double * 8)()
+        """per stage: useful flops, algorithmic bytes, work items, pairs, the
+        GEMM kernel that runs it, its workgroups and reduction workgroups"""
+        v = (C.c_double * 16)()
         _capi.check(_capi.lib().qsx_hpsi_stage_info(self.h, v), None)
+        kern = []
+        for t in (0, 1):
+            if int(v[8 + t]) == 0:
+                kern.append("k_grouped")
+            else:
+                kern.append("k_wave_s" if int(v[12 + t]) <= 2 else "k_wave_l")
         return {"flops": (v[0], v[1]), "bytes": (v[2], v[3]), "tiles": (int(v[4]), int(v[5])),
-                "pairs": (int(v[6]), int(v[7]))}
+                "pairs": (int(v[6]), int(v[7])), "kernel": tuple(kern), "grid": (int(v[10]), int(v[11])),
+                "reduce_grid": (int(v[14]), int(v[15]))}
 
     def autotune(self, reps: int = 5):
         """time candidate work decompositions of both stages and keep the
         fastest: {"stage_ms": (t1, t2), "policy": (stage-1 dict, stage-2 dict)}"""
         ms = (C.c_double * 2)()
-        pol = (C.c_int32 * 16)()
+        pol = (C.c_int32 * (2 * NPOL))()
         _capi.check(_capi.lib().qsx_hpsi_autotune(self.h, reps, ms, pol), None)
-        return {"stage_ms": (ms[0], ms[1]), "policy": (_policy_dict(pol[:8]), _policy_dict(pol[8:]))}
+        return {"stage_ms": (ms[0], ms[1]), "policy": (_policy_dict(pol[:NPOL]), _policy_dict(pol[NPOL:]))}
 
     def set_policy(self, stage1: dict, stage2: dict):
-        v = (C.c_int32 * 16)(*(_policy_ints(stage1) + _policy_ints(stage2)))
+        v = (C.c_int32 * (2 * NPOL))(*(_policy_ints(stage1) + _policy_ints(stage2)))
         _capi.check(_capi.lib().qsx_hpsi_set_policy(self.h, v), None)
 
     def weights(self) -> np.ndarray:

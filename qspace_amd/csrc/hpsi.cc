@@ -277,25 +277,31 @@ static void hpsi_autotune(HpsiPlan& H, int reps, double* out) {
           pol.slots = sl;
           cands.push_back(pol);
         }
-    for (int wm : {1, 2})
-      for (int gm : {1 << 30, 256, 128, 64, 32})
-        for (int sm : {1 << 30, 512, 256, 128}) {
-          WorkPolicy pol;
-          pol.engine = ENG_WAVE;
-          pol.wave_max = wm;
-          pol.group_mfma = gm;
-          pol.split_mfma = sm;
-          cands.push_back(pol);
+    for (int xb : {8, 0})
+      for (int wm : {1, 2})
+        for (int gw : {4, 8, 16}) {
+          if (wm * wm * gw > 16) continue;
+          for (int gm : {1 << 30, 256, 128, 64, 32})
+            for (int sm : {1 << 30, 256, 128}) {
+              WorkPolicy pol;
+              pol.engine = ENG_WAVE;
+              pol.wave_max = wm;
+              pol.group_waves = gw;
+              pol.group_mfma = gm;
+              pol.split_mfma = sm;
+              pol.xcd_bins = xb;
+              cands.push_back(pol);
+            }
         }
     for (const WorkPolicy& pol : cands) {
       WorkList W = build_work(tg, pr, pol);
       double t = time_stage(H, stage, W, dpsi, dout, reps);
       if (H.verbose)
         fprintf(stderr,
-                "qsx autotune stage %d engine %d split %d/%d group %d overhead %g slots %d wave_max %d: %zu items "
-                "%zu reds %.4f ms\n",
-                stage + 1, W.engine, pol.split_steps, pol.split_mfma, pol.group_mfma, pol.tile_overhead, pol.slots,
-                pol.wave_max, W.items.size(), W.reds.size(), t);
+                "qsx autotune stage %d engine %d split %d/%d group %d/%d overhead %g slots %d wave_max %d xcd %d: "
+                "%zu items %zu reds %.4f ms\n",
+                stage + 1, W.engine, pol.split_steps, pol.split_mfma, pol.group_mfma, W.wg_waves, pol.tile_overhead,
+                pol.slots, pol.wave_max, pol.xcd_bins, W.items.size(), W.reds.size(), t);
       if (t < best) {
         best = t;
         bp = pol;
@@ -827,6 +833,14 @@ int qsx_hpsi_stage_info(const qsx_hpsi_plan* plan, double* info) {
   info[5] = (double)H.work2.items.size();
   info[6] = (double)H.pairs1.size();
   info[7] = (double)H.pairs2.size();
+  const WorkList* W[2] = {&H.work1, &H.work2};
+  for (int t = 0; t < 2; ++t) {
+    info[8 + t] = W[t]->engine;
+    info[10 + t] = W[t]->engine == ENG_WAVE ? (double)((W[t]->items.size() + W[t]->wg_waves - 1) / W[t]->wg_waves)
+                                            : (double)W[t]->queues.size();
+    info[12 + t] = W[t]->maxf;
+    info[14 + t] = (double)W[t]->reds.size();
+  }
   return 0;
 }
 

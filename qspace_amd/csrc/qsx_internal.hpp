@@ -140,9 +140,11 @@ enum Engine { ENG_GROUPED = 0, ENG_WAVE = 1 };
 
 struct WorkPolicy {
   int engine = ENG_WAVE;         // workgroup tiles (grouped.hip) or wave tiles (wave.hip)
-  int wave_max = 4;              // wave tiles: at most wave_max x wave_max fragments of 16x16
+  int wave_max = 2;              // wave tiles: at most wave_max x wave_max fragments of 16x16
   int split_mfma = 1 << 30;      // wave tiles: MFMAs per workgroup item before the k-range is split
-  int group_mfma = 1 << 30;      // wave tiles: MFMAs per tile above which 4 waves share it
+  int group_mfma = 64;           // wave tiles: MFMAs per tile above which a workgroup's waves share it
+  int xcd_bins = 8;              // wave tiles: XCD-aware workgroup order over this many L2s (<= 1: off)
+  int group_waves = 4;           // wave tiles of <= 2x2 fragments: waves per workgroup (4, 8 or 16)
   int split_steps = 16;          // k-steps (of 16) per chunk before a tile is split
   double tile_overhead = 1024;   // per-tile cost (in MACs per k) when choosing shapes
   int shape_mask = (1 << SH_NUM) - 1;
@@ -158,7 +160,9 @@ inline void policy_to_ints(const WorkPolicy& p, int32_t* v) {
   v[4] = p.wave_max;
   v[5] = p.group_mfma;
   v[6] = p.split_mfma;
-  v[7] = 0;
+  v[7] = p.xcd_bins;
+  v[8] = p.group_waves;
+  v[9] = v[10] = v[11] = 0;
 }
 inline WorkPolicy policy_from_ints(const int32_t* v) {
   WorkPolicy p;
@@ -169,6 +173,8 @@ inline WorkPolicy policy_from_ints(const int32_t* v) {
   p.wave_max = v[4] >= 1 && v[4] <= 4 ? v[4] : 4;
   p.group_mfma = v[5] > 0 ? v[5] : 1;
   p.split_mfma = v[6] > 0 ? v[6] : 1;
+  p.xcd_bins = v[7] >= 0 && v[7] <= 64 ? v[7] : 8;
+  p.group_waves = v[8] == 8 || v[8] == 16 ? v[8] : 4;
   return p;
 }
 
@@ -182,6 +188,7 @@ struct WorkTarget {
 struct WorkList {
   int engine = ENG_GROUPED;
   int maxf = 4;                  // wave engine: largest fragment count per tile side
+  int wg_waves = 4;              // wave engine: waves per workgroup (the group size)
   std::vector<WorkItem> items;   // in queue order (wave engine: longest first)
   std::vector<WorkQueue> queues; // one per persistent workgroup
   std::vector<ReduceItem> reds;  // one per split tile
@@ -193,7 +200,7 @@ WorkList build_work(const std::vector<WorkTarget>& targets, const std::vector<Pa
 void launch_work(const WorkQueue* dqueues, int nqueues, const WorkItem* ditems, const ReduceItem* dreds, int nreds,
                  const PairDesc* dpairs, const double* A, const double* B, double* C, double* scratch, hipStream_t s);
 // one wave per item, then (if any tile is split) the reduction
-void launch_wave(const WorkItem* ditems, int nitems, int maxf, const ReduceItem* dreds, int nreds,
+void launch_wave(const WorkItem* ditems, int nitems, int maxf, int wg_waves, const ReduceItem* dreds, int nreds,
                  const PairDesc* dpairs, const double* A, const double* B, double* C, double* scratch,
                  hipStream_t s);
 // the launch that matches the work list's engine
@@ -201,7 +208,8 @@ inline void launch_worklist(const WorkList& W, const WorkQueue* dqueues, const W
                             const ReduceItem* dreds, const PairDesc* dpairs, const double* A, const double* B,
                             double* C, double* scratch, hipStream_t s) {
   if (W.engine == ENG_WAVE)
-    launch_wave(ditems, (int)W.items.size(), W.maxf, dreds, (int)W.reds.size(), dpairs, A, B, C, scratch, s);
+    launch_wave(ditems, (int)W.items.size(), W.maxf, W.wg_waves, dreds, (int)W.reds.size(), dpairs, A, B, C, scratch,
+                s);
   else
     launch_work(dqueues, (int)W.queues.size(), ditems, dreds, (int)W.reds.size(), dpairs, A, B, C, scratch, s);
 }

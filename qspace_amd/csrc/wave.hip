@@ -35,7 +35,8 @@ namespace {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
-constexpr int NT = 256;          // 4 waves (4 independent items) per workgroup
+constexpr int NT = 256;          // large-tile kernel and reduce: 4 waves per workgroup
+constexpr int kMaxWaves = 16;    // small-tile kernel: up to 16 waves per workgroup
 constexpr int kSlot = 64 * 64;   // scratch doubles per partial tile
 
 __device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
@@ -185,7 +186,7 @@ __device__ __forceinline__ void store_acc(const WorkItem& w, const d4 (&acc)[FM]
 template <int FM, int FN>
 struct Pipe {
   static constexpr int KS = (FM + FN) >= 6 ? 1 : (FM + FN) >= 3 ? 2 : 4;
-  static constexpr int D = (FM + FN) >= 6 ? 2 : (FM + FN) >= 4 ? 3 : 4;
+  static constexpr int D = (FM + FN) >= 6 ? 2 : (FM + FN) >= 4 ? 3 : (FM + FN) == 3 ? 4 : 6;
 };
 
 template <int FM, int FN>
@@ -228,10 +229,12 @@ __device__ __forceinline__ void run_k(const WorkItem& w, const PairDesc* __restr
   }
 }
 
-// partial tiles of waves 1..3 of a group, lane-fastest (conflict-free)
-constexpr int kGroupFrags = 4;  // groups are formed for tiles of at most 4 fragments
+// Partial tiles of waves 1..G-1 of a group (G = the workgroup's waves),
+// lane-fastest (conflict-free).  Groups are formed only for tiles with
+// G * fragments <= kGroupBudget.
+constexpr int kGroupBudget = 16;
 struct GroupLds {
-  double part[3][kGroupFrags * 4][64];
+  double part[(kGroupBudget - 1) * 4][64];
 };
 
 template <int FM, int FN>
@@ -240,29 +243,29 @@ __device__ __forceinline__ void run_item(const WorkItem& w, const PairDesc* __re
                                          double* __restrict__ C, double* __restrict__ scratch, GroupLds& lds) {
   d4 acc[FM][FN];
   run_k<FM, FN>(w, pairs, A, B, acc);
-  if constexpr (FM * FN <= kGroupFrags) {
-    if (w.group) {  // uniform over the workgroup
-      // the four waves' partials are summed in wave order 0,1,2,3: one fixed
-      // order per element, deterministic
-      const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if constexpr (FM * FN <= 4) {
+    if (w.group) {  // uniform over the workgroup: all its waves share this tile
+      // the partials are summed in wave order 0, 1, ..., G-1: one fixed order
+      // per element, deterministic
+      const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, G = blockDim.x >> 6;
+      constexpr int F = FM * FN * 4;
       if (wave > 0) {
 #pragma unroll
         for (int i = 0; i < FM; ++i)
 #pragma unroll
           for (int j = 0; j < FN; ++j)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) lds.part[wave - 1][(i * FN + j) * 4 + v][lane] = acc[i][j][v];
+            for (int v = 0; v < 4; ++v) lds.part[(wave - 1) * F + (i * FN + j) * 4 + v][lane] = acc[i][j][v];
       }
       __syncthreads();
       if (wave > 0) return;
-#pragma unroll
-      for (int q = 0; q < 3; ++q)
+      for (int q = 0; q < G - 1; ++q)
 #pragma unroll
         for (int i = 0; i < FM; ++i)
 #pragma unroll
           for (int j = 0; j < FN; ++j)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) acc[i][j][v] += lds.part[q][(i * FN + j) * 4 + v][lane];
+            for (int v = 0; v < 4; ++v) acc[i][j][v] += lds.part[q * F + (i * FN + j) * 4 + v][lane];
     }
   }
   store_acc<FM, FN>(w, acc, C, scratch);
@@ -278,7 +281,7 @@ __device__ __forceinline__ void wave_dispatch(const WorkItem* __restrict__ items
                                               const double* __restrict__ B, double* __restrict__ C,
                                               double* __restrict__ scratch) {
   __shared__ GroupLds lds;
-  const int wid = uniform(blockIdx.x * (NT / 64) + (threadIdx.x >> 6));
+  const int wid = uniform(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
   if (wid >= nitems) return;
   const WorkItem w = items[wid];
   if (w.slot == -2) return;  // padding (never part of a group)
@@ -296,7 +299,7 @@ __device__ __forceinline__ void wave_dispatch(const WorkItem* __restrict__ items
   }
 }
 
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ __launch_bounds__(64 * kMaxWaves) __attribute__((amdgpu_waves_per_eu(4)))
 void k_wave_s(const WorkItem* __restrict__ items, int nitems, const PairDesc* __restrict__ pairs,
               const double* __restrict__ A, const double* __restrict__ B, double* __restrict__ C,
               double* __restrict__ scratch) {
@@ -337,15 +340,19 @@ __global__ __launch_bounds__(NT) void k_wave_reduce(const ReduceItem* __restrict
 
 }  // namespace
 
-void launch_wave(const WorkItem* ditems, int nitems, int maxf, const ReduceItem* dreds, int nreds,
+void launch_wave(const WorkItem* ditems, int nitems, int maxf, int wg_waves, const ReduceItem* dreds, int nreds,
                  const PairDesc* dpairs, const double* A, const double* B, double* C, double* scratch,
                  hipStream_t s) {
   if (nitems > 0) {
-    const int nb = (nitems + NT / 64 - 1) / (NT / 64);
-    if (maxf <= 2)
-      hipLaunchKernelGGL(k_wave_s, dim3(nb), dim3(NT), 0, s, ditems, nitems, dpairs, A, B, C, scratch);
-    else
+    if (maxf <= 2) {
+      if (wg_waves < 1 || wg_waves > kMaxWaves)
+        fail(QSX_ERR_ARG, "wave engine: %d waves per workgroup", wg_waves);
+      const int nb = (nitems + wg_waves - 1) / wg_waves;
+      hipLaunchKernelGGL(k_wave_s, dim3(nb), dim3(64 * wg_waves), 0, s, ditems, nitems, dpairs, A, B, C, scratch);
+    } else {
+      const int nb = (nitems + NT / 64 - 1) / (NT / 64);
       hipLaunchKernelGGL(k_wave_l, dim3(nb), dim3(NT), 0, s, ditems, nitems, dpairs, A, B, C, scratch);
+    }
     QSX_HIP(hipGetLastError());
   }
   if (nreds > 0) {

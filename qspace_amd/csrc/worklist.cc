@@ -8,6 +8,7 @@
 // one tile shape per queue, longest-processing-time first).
 #include <algorithm>
 #include <cmath>
+#include <map>
 
 #include "qsx_internal.hpp"
 
@@ -20,10 +21,11 @@ static constexpr int WKS = 4;   // k-step of the wave-tile kernel (one MFMA)
 // tiles of at most wave_max x wave_max fragments (fragments spread evenly
 // over the tiles).  A tile's k-range is the concatenation of its pairs'
 // k-steps (of 4, reference pair order).  A tile with more than group_mfma
-// MFMAs is shared by the 4 waves of a workgroup (contiguous k-ranges, partials
+// MFMAs is shared by the G waves of a workgroup (G = group_waves: 4, 8 or 16,
+// with G x fragments <= 16; contiguous k-ranges, partials
 // summed through LDS); one with more than split_mfma MFMAs per workgroup is
 // further cut into chunks whose partial tiles the reduction sums.  Other
-// tiles run one per wave, four to a workgroup.  Workgroups are ordered
+// tiles run one per wave, G to a workgroup.  Workgroups are ordered
 // longest first: the hardware issues them in order to free slots, which is
 // list scheduling with the longest-processing-time rule.
 static WorkList build_wave_work(const std::vector<WorkTarget>& targets, const std::vector<PairDesc>& pairs,
@@ -32,12 +34,23 @@ static WorkList build_wave_work(const std::vector<WorkTarget>& targets, const st
   out.engine = ENG_WAVE;
   const int maxf = std::max(1, std::min(4, pol.wave_max));
   out.maxf = maxf;
+  // waves per workgroup: the group size; a group's partial tiles live in LDS,
+  // at most 16 fragments per workgroup (the kernel's budget)
+  int G = maxf <= 2 ? pol.group_waves : 4;
+  while (G > 4 && G * maxf * maxf > 16) G /= 2;
+  out.wg_waves = G;
   struct WG {
     double cost;
+    int64_t key;  // locality key: the larger operand's block of the target's first pair
     std::vector<WorkItem> it;
   };
-  std::vector<WG> groups;                          // workgroups of 4 tasks of one tile
-  std::vector<std::pair<double, WorkItem>> solo;   // one-wave items
+  struct Solo {
+    double cost;
+    int64_t key;
+    WorkItem w;
+  };
+  std::vector<WG> groups;   // workgroups of 4 tasks of one tile
+  std::vector<Solo> solo;   // one-wave items
   int64_t slot = 0;
   for (const WorkTarget& t : targets) {
     const int M = t.M, N = t.N;
@@ -46,6 +59,12 @@ static WorkList build_wave_work(const std::vector<WorkTarget>& targets, const st
     for (int64_t p = t.p0; p < t.p1; ++p) steps += (pairs[p].K + WKS - 1) / WKS;
     const int mf = (M + 15) / 16, nf = (N + 15) / 16;
     const int ntm = (mf + maxf - 1) / maxf, ntn = (nf + maxf - 1) / maxf;
+    int64_t key = -1;
+    if (t.p1 > t.p0) {
+      const PairDesc& d = pairs[t.p0];
+      const int64_t ea = (int64_t)M * d.K, eb = (int64_t)d.K * N;
+      key = eb >= ea ? (d.b_off << 1) | 1 : (d.a_off << 1);
+    }
     // item over the global k-steps [s0, s1) of the tile
     auto range = [&](WorkItem w, int64_t s0, int64_t s1) {
       if (s0 >= s1) {
@@ -87,8 +106,8 @@ static WorkList build_wave_work(const std::vector<WorkTarget>& targets, const st
         base.group = 0;
         const double per_step = (double)fm * fn;
         const double cost = per_step * (double)steps;
-        const bool grp = fm * fn <= 4 && cost > pol.group_mfma && steps >= 4;
-        const int ways = grp ? 4 : 1;
+        const bool grp = fm * fn * G <= 16 && cost > pol.group_mfma && steps >= G;
+        const int ways = grp ? G : 1;
         const double per_wg = std::max<double>(1.0, pol.split_mfma);
         const int nchunk = (int)std::max<double>(1.0, std::ceil(cost / ways / per_wg));
         const int64_t nch = std::min<int64_t>(nchunk, std::max<int64_t>(1, steps / ways));
@@ -98,15 +117,16 @@ static WorkList build_wave_work(const std::vector<WorkTarget>& targets, const st
           if (nch > 1) w.slot = (int32_t)(slot + q);
           if (grp) {
             WG g;
-            g.cost = per_step * (double)((c1 - c0 + 3) / 4) + 16.0;
-            for (int v = 0; v < 4; ++v) {
-              WorkItem x = range(w, c0 + (c1 - c0) * v / 4, c0 + (c1 - c0) * (v + 1) / 4);
+            g.key = key;
+            g.cost = per_step * (double)((c1 - c0 + G - 1) / G) + 16.0;
+            for (int v = 0; v < G; ++v) {
+              WorkItem x = range(w, c0 + (c1 - c0) * v / G, c0 + (c1 - c0) * (v + 1) / G);
               x.group = 1;
               g.it.push_back(x);
             }
             groups.push_back(std::move(g));
           } else {
-            solo.push_back({per_step * (double)(c1 - c0) + 8.0, range(w, c0, c1)});
+            solo.push_back({per_step * (double)(c1 - c0) + 8.0, key, range(w, c0, c1)});
           }
         }
         if (nch > 1) {
@@ -127,28 +147,87 @@ static WorkList build_wave_work(const std::vector<WorkTarget>& targets, const st
       m0f += fm;
     }
   }
-  // solo items longest first, four to a workgroup (the last padded)
-  std::stable_sort(solo.begin(), solo.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
-  for (size_t i = 0; i < solo.size(); i += 4) {
+  // solo items: by locality key, longest first within a key, G to a
+  // workgroup (a key's last workgroup padded)
+  std::stable_sort(solo.begin(), solo.end(), [](const Solo& x, const Solo& y) {
+    return x.key != y.key ? x.key < y.key : x.cost > y.cost;
+  });
+  auto pad_item = []() {
+    WorkItem pad{};
+    pad.slot = -2;
+    pad.p0 = 0;
+    pad.p1 = -1;
+    pad.shape = wave_shape(1, 1);
+    return pad;
+  };
+  for (size_t i = 0; i < solo.size();) {
     WG g;
-    g.cost = solo[i].first;
-    for (size_t j = i; j < i + 4; ++j) {
-      if (j < solo.size()) {
-        g.it.push_back(solo[j].second);
+    g.cost = solo[i].cost;
+    g.key = solo[i].key;
+    size_t j = i;
+    for (; j < solo.size() && j < i + G && solo[j].key == g.key; ++j) g.it.push_back(solo[j].w);
+    while ((int)g.it.size() < G) g.it.push_back(pad_item());
+    groups.push_back(std::move(g));
+    i = j;
+  }
+  auto by_cost = [](const WG& x, const WG& y) { return x.cost > y.cost; };
+  const int nbins = pol.xcd_bins;
+  if (nbins <= 1) {
+    std::stable_sort(groups.begin(), groups.end(), by_cost);
+    for (const WG& g : groups)
+      for (const WorkItem& w : g.it) out.items.push_back(w);
+  } else {
+    // XCD-aware order: workgroup i runs on XCD i % nbins (round-robin
+    // dispatch), each XCD with its own L2.  Workgroups of one locality key
+    // go to one XCD so its operand blocks are fetched into one L2; keys are
+    // dealt to XCDs longest first to the least loaded (a key heavier than
+    // an XCD's share is spread workgroup by workgroup); each XCD's list is
+    // longest first and the lists are interleaved (short lists padded with
+    // empty workgroups).
+    std::map<int64_t, std::vector<WG>> bykey;
+    double total = 0;
+    for (WG& g : groups) {
+      total += g.cost;
+      bykey[g.key].push_back(std::move(g));
+    }
+    std::vector<std::pair<double, std::vector<WG>*>> keys;
+    for (auto& kv : bykey) {
+      double c = 0;
+      for (const WG& g : kv.second) c += g.cost;
+      keys.push_back({c, &kv.second});
+    }
+    std::stable_sort(keys.begin(), keys.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
+    std::vector<std::vector<WG>> bins(nbins);
+    std::vector<double> load(nbins, 0.0);
+    const double share = total / nbins;
+    for (auto& kc : keys) {
+      if (kc.first > share) {
+        std::stable_sort(kc.second->begin(), kc.second->end(), by_cost);
+        for (WG& g : *kc.second) {
+          const int b = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+          load[b] += g.cost;
+          bins[b].push_back(std::move(g));
+        }
       } else {
-        WorkItem pad{};
-        pad.slot = -2;
-        pad.p0 = 0;
-        pad.p1 = -1;
-        pad.shape = wave_shape(1, 1);
-        g.it.push_back(pad);
+        const int b = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        load[b] += kc.first;
+        for (WG& g : *kc.second) bins[b].push_back(std::move(g));
       }
     }
-    groups.push_back(std::move(g));
+    size_t depth = 0;
+    for (auto& b : bins) {
+      std::stable_sort(b.begin(), b.end(), by_cost);
+      depth = std::max(depth, b.size());
+    }
+    for (size_t r = 0; r < depth; ++r)
+      for (int b = 0; b < nbins; ++b) {
+        if (r < bins[b].size()) {
+          for (const WorkItem& w : bins[b][r].it) out.items.push_back(w);
+        } else {
+          for (int v = 0; v < G; ++v) out.items.push_back(pad_item());
+        }
+      }
   }
-  std::stable_sort(groups.begin(), groups.end(), [](const WG& x, const WG& y) { return x.cost > y.cost; });
-  for (const WG& g : groups)
-    for (const WorkItem& w : g.it) out.items.push_back(w);
   out.scratch_elems = slot * 64 * 64;
   return out;
 }

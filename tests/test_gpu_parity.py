@@ -160,6 +160,10 @@ POLICIES = [
     {"engine": "wave", "wave_max": 2, "group_mfma": 8, "split_mfma": 16},
     {"engine": "wave", "wave_max": 2, "split_mfma": 8},
     {"engine": "wave", "wave_max": 3, "group_mfma": 4, "split_mfma": 64},
+    {"engine": "wave", "wave_max": 1, "group_mfma": 16, "xcd_bins": 0},
+    {"engine": "wave", "wave_max": 2, "group_mfma": 16, "split_mfma": 32, "xcd_bins": 3},
+    {"engine": "wave", "wave_max": 1, "group_mfma": 8, "group_waves": 16},
+    {"engine": "wave", "wave_max": 1, "group_mfma": 8, "group_waves": 8, "split_mfma": 16, "xcd_bins": 5},
 ]
 
 
