@@ -80,12 +80,12 @@ def test_single_block_gemm_layouts(gpu):
             assert np.abs(C.data[0] - ref).max() / np.abs(ref).max() < 1e-14
 
 
-def _hpsi_oracle(HL, psi, HR, hconj=1):
-    T = O.contract(HL, [1], psi, [0], skip_zero=0)
+def _hpsi_oracle(HL, psi, HR, hconj=1, fermA=None):
+    T = O.contract(HL, [1], psi, [0], fermA=fermA, skip_zero=0)
     H = O.contract(T, [1, 2], HR, [2, 1], skip_zero=0)
     terms = [H]
     if hconj:
-        T2 = O.contract(HL, [0], psi, [0], conjA=1, skip_zero=0)
+        T2 = O.contract(HL, [0], psi, [0], conjA=1, fermA=fermA, skip_zero=0)
         H2 = O.contract(T2, [1, 2], HR, [2, 0], conjB=1, skip_zero=0)
         terms.append(H2)
     # map onto psi's block order
@@ -196,3 +196,53 @@ def test_contract_policies_match_oracle(gpu, pi):
     ref = R.packed()
     assert got.shape == ref.shape
     assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < TOL
+
+
+def test_hpsi_hubbard_small(gpu):
+    """SU(2)xU(1) Hubbard H*psi with exact +-1 parity signs on HL vs the oracle."""
+    HL, psi, HR = gen.hubbard_hpsi(seed=24, Dstar=120, mu=1.5, sigma=0.6, dmax=12, qmax=4)
+    ferm = gen.fermion_parity_signs(HL)
+    assert (ferm < 0).any() and (ferm > 0).any()
+    H = Q.HPsi(HL, psi, HR, hconj=1, fermA=ferm)
+    got = H.apply(psi.packed())
+    ref = _hpsi_oracle(HL, psi, HR, 1, fermA=ferm).packed()
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < TOL
+
+
+def _dense_weighted(H):
+    """H as a dense matrix in the orthonormal basis sqrt(w) x (symmetric)."""
+    N = H.N
+    cols = []
+    for i in range(N):
+        e = np.zeros(N)
+        e[i] = 1.0
+        cols.append(H.apply(e))
+    M = np.column_stack(cols)
+    sw = np.sqrt(H.weights())
+    S = (sw[:, None] * M) / sw[None, :]
+    return S
+
+
+@pytest.mark.parametrize("kind", ["heis", "hubbard"])
+def test_eigensolvers_vs_dense(gpu, kind):
+    """Lanczos and Davidson with a Krylov space of the full dimension give the
+    lowest eigenvalue of the dense operator; their ground states are
+    eigenvectors (weighted residual)."""
+    if kind == "heis":
+        HL, psi, HR = gen.heis_hpsi(seed=25, Dstar=40, mu=1.2, sigma=0.4, dmax=6)
+        ferm = None
+    else:
+        HL, psi, HR = gen.hubbard_hpsi(seed=26, Dstar=30, mu=1.0, sigma=0.4, dmax=4, qmax=3)
+        ferm = gen.fermion_parity_signs(HL)
+    H = Q.HPsi(HL, psi, HR, hconj=1, fermA=ferm)
+    S = _dense_weighted(H)
+    assert np.abs(S - S.T).max() < 1e-12 * np.abs(S).max()
+    ev = np.linalg.eigvalsh(0.5 * (S + S.T))
+    e_ref = ev[0]
+    w = H.weights()
+    x0 = np.random.RandomState(1).standard_normal(H.N)
+    for solver in (H.lanczos, H.davidson):
+        e0, ritz, gs = solver(x0, H.N)
+        assert abs(e0 - e_ref) < 1e-10 * max(1.0, abs(e_ref)), (solver.__name__, e0, e_ref)
+        r = H.apply(gs) - e0 * gs
+        assert np.sqrt((w * r) @ r) / np.sqrt((w * gs) @ gs) < 1e-8 * max(1.0, abs(e_ref))
