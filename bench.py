@@ -11,7 +11,12 @@ Before the timed region the run also makes one device copy of 64 MiB
 byte counts of the GEMM kernels against it.
 
     python bench.py [--steps K] [--warmup W] [--workload heis4000|heis2000|hubbard4000]
-                    [--no-cpu] [--extra]
+                    [--no-cpu] [--lite] [--policy FILE]
+
+Besides the headline H*psi the JSON line carries "configs": the survey's
+configurations 1 (U(1) MPS overlap, CPU oracle and GPU), 2 (SU(2)
+environment step), 3 (Heisenberg D*=2000, 20 Lanczos iterations) and 5
+(SU(2)xU(1) Hubbard D*=4000 with parity signs, 30 Davidson iterations).
 """
 from __future__ import annotations
 
@@ -159,6 +164,105 @@ def cpu_baseline(w, sample_smax=None, target_s=15.0):
     }
 
 
+def _rel(C, R):
+    """relative Frobenius error of C vs R; structure (records, dims) must agree"""
+    if not (np.array_equal(C.QIDX, R.QIDX) and [d.shape for d in C.data] == [d.shape for d in R.data]):
+        return float("inf")
+    c, r = C.packed(), R.packed()
+    return float(np.linalg.norm(c - r) / max(np.linalg.norm(r), 1e-300))
+
+
+def _time_plan(P, A, B, reps=20):
+    """device time of one batched execution of a contraction plan (inputs resident)"""
+    import qspace_amd as Q
+    dA = Q.DeviceBuffer.from_array(A.packed())
+    dB = Q.DeviceBuffer.from_array(B.packed())
+    dC = Q.DeviceBuffer(P.nout)
+    for _ in range(3):
+        P.execute(dA, dB, dC)
+    Q._capi.lib().qsx_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        P.execute(dA, dB, dC)
+    Q._capi.lib().qsx_synchronize()
+    return (time.perf_counter() - t0) / reps * 1e3
+
+
+def extra_configs():
+    """the survey's five configurations (config 4 = the headline workload)"""
+    import qspace_amd as Q
+    from qspace_amd import gen
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    out = {}
+    # 1: U(1) MPS overlap contractQS(A,'12*',A,'12'): CPU oracle and GPU
+    A = gen.u1_mps()
+    t0 = time.perf_counter()
+    R = O.contract(A, [0, 1], A, [0, 1], conjA=1)
+    cpu_ms = (time.perf_counter() - t0) * 1e3
+    C = Q.contractQS(A, "12*", A, "12")
+    P = Q.CtrPlan(A, "12*", A, "12")
+    out["1_u1_mps_overlap"] = {"blocks": A.nblk, "useful_flops": C.flops, "oracle_cpu_ms": cpu_ms,
+                               "gpu_plan_execute_ms": _time_plan(P, A, A), "rel_err_vs_oracle": _rel(C, R)}
+    log(f"[bench] config 1: {out['1_u1_mps_overlap']}")
+    # 2: SU(2) environment step X = L.A, C = X.conj(A)
+    A, L = gen.su2_mps()
+    X = Q.contractQS(L, "2", A, "1")
+    XR = O.contract(L, [1], A, [0])
+    Cg = Q.contractQS(X, "12", A, "12*")
+    CR = O.contract(XR, [0, 1], A, [0, 1], conjB=1)
+    P1 = Q.CtrPlan(L, "2", A, "1")
+    P2 = Q.CtrPlan(X, "12", A, "12*")
+    out["2_su2_env"] = {"multiplets_per_bond": int(sum(d.shape[0] for d in L.data)),
+                        "useful_flops": X.flops + Cg.flops,
+                        "gpu_plan_execute_ms": _time_plan(P1, L, A) + _time_plan(P2, X, A),
+                        "rel_err_vs_oracle": max(_rel(X, XR), _rel(Cg, CR))}
+    log(f"[bench] config 2: {out['2_su2_env']}")
+    # 3: Heisenberg H*psi at D*=2000, 20 Lanczos iterations
+    w = WORKLOADS["heis2000"]
+    HL, psi, HR, ferm = make_workload(w)
+    H = Q.HPsi(HL, psi, HR, hconj=1)
+    H.autotune(5)
+    x = Q.DeviceBuffer.from_array(psi.packed())
+    y = Q.DeviceBuffer(H.N)
+    ms = time_hpsi(H, x, y, 50, 5)
+    t0 = time.perf_counter()
+    e0, ritz, gs = H.lanczos(psi.packed(), 20)
+    lan = (time.perf_counter() - t0) * 1e3
+    out["3_heis2000_lanczos20"] = {"psi_elements": H.N, "useful_gflop_per_hpsi": H.flops / 1e9, "hpsi_ms": ms,
+                                   "gflops": H.flops / ms / 1e6, "lanczos20_ms": lan, "E0": e0,
+                                   "ritz_last3": [float(r) for r in ritz[-3:]]}
+    log(f"[bench] config 3: {out['3_heis2000_lanczos20']}")
+    del H
+    # 5: SU(2)xU(1) Hubbard H*psi at D*=4000 with parity signs, 30 Davidson iterations
+    w = WORKLOADS["hubbard4000"]
+    HL, psi, HR, ferm = make_workload(w)
+    H = Q.HPsi(HL, psi, HR, hconj=1, fermA=ferm)
+    H.autotune(5)
+    x = Q.DeviceBuffer.from_array(psi.packed())
+    y = Q.DeviceBuffer(H.N)
+    ms = time_hpsi(H, x, y, 50, 5)
+    got = y.download()
+    T1 = O.contract(HL, [1], psi, [0], fermA=ferm, skip_zero=0)
+    T2 = O.contract(HL, [0], psi, [0], conjA=1, fermA=ferm, skip_zero=0)
+    key = {tuple(r): i for i, r in enumerate(psi.QIDX)}
+    acc = [np.zeros_like(d) for d in psi.data]
+    for Ht in (O.contract(T1, [1, 2], HR, [2, 1], skip_zero=0), O.contract(T2, [1, 2], HR, [2, 0], conjB=1,
+                                                                                  skip_zero=0)):
+        for r, d in zip(Ht.QIDX, Ht.data):
+            acc[key[tuple(r)]] += d
+    ref = np.concatenate([a.ravel(order="F") for a in acc])
+    t0 = time.perf_counter()
+    e0, ritz, gs = H.davidson(psi.packed(), 30)
+    dav = (time.perf_counter() - t0) * 1e3
+    out["5_hubbard4000_davidson30"] = {"psi_elements": H.N, "useful_gflop_per_hpsi": H.flops / 1e9, "hpsi_ms": ms,
+                                       "gflops": H.flops / ms / 1e6, "davidson30_ms": dav, "E0": e0,
+                                       "rel_err_hpsi_vs_oracle": float(np.linalg.norm(got - ref) /
+                                                                       np.linalg.norm(ref))}
+    log(f"[bench] config 5: {out['5_hubbard4000_davidson30']}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -170,7 +274,6 @@ def main():
                     help="timed H*psi only (no CPU baseline, plain path, Lanczos): for profiler runs")
     ap.add_argument("--policy", default=None,
                     help="JSON file {stage1: {...}, stage2: {...}} fixing the work decomposition (no autotune)")
-    ap.add_argument("--extra", action="store_true", help="also time Lanczos/plain path at extra sizes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (got --gpus {args.gpus}); "
@@ -263,6 +366,9 @@ def main():
     roof["avg_launch_ms"] = tk
     roof["fp64_mfma_frac"] = fl / (tk * 1e-3) / 1e12 / PEAK_FP64_TFLOPS
     roof["hbm_frac"] = by / (tk * 1e-3) / 1e9 / PEAK_HBM_GBS
+    configs = None
+    if not args.lite:
+        configs = extra_configs()
     cpu = None
     if not args.no_cpu and not args.lite:
         log("[bench] cpu baseline (sample) ...")
@@ -299,6 +405,7 @@ def main():
         "plan_create_s": plan_s,
         "autotune": {"seconds": tune_s, **{k: list(v) for k, v in tune.items()}},
         "calibration": calib,
+        "configs": configs,
     }
     if cpu is not None:
         out["rel_err_vs_fp64_oracle"] = cpu["rel_err_gpu_vs_fp64_oracle_sample"]
