@@ -15,7 +15,8 @@ byte counts of the GEMM kernels against it.
 
 Besides the headline H*psi the JSON line carries "configs": the survey's
 configurations 1 (U(1) MPS overlap, CPU oracle and GPU), 2 (SU(2)
-environment step), 3 (Heisenberg D*=2000, 20 Lanczos iterations) and 5
+environment step), 3 (Heisenberg D*=2000, 20 Lanczos iterations), 4 (split
+precision: BF16-slice H*psi error/time table on the headline operator) and 5
 (SU(2)xU(1) Hubbard D*=4000 with parity signs, 30 Davidson iterations).
 """
 from __future__ import annotations
@@ -234,6 +235,32 @@ def extra_configs():
                                    "ritz_last3": [float(r) for r in ritz[-3:]]}
     log(f"[bench] config 3: {out['3_heis2000_lanczos20']}")
     del H
+    # 4: split precision (BF16 slices, exact FP32 accumulation, FP64
+    # recombination) on the headline operator: error table per slice count
+    w = WORKLOADS["heis4000"]
+    HL, psi, HR, ferm = make_workload(w)
+    Hd = Q.HPsi(HL, psi, HR, hconj=1)
+    Hd.autotune(5)
+    xv = psi.packed()
+    ref = Hd.apply(xv)
+    e_ref, _, _ = Hd.lanczos(xv, 20)
+    x = Q.DeviceBuffer.from_array(xv)
+    y = Q.DeviceBuffer(Hd.N)
+    ms_fp64 = time_hpsi(Hd, x, y, 50, 5)
+    del Hd
+    table = {}
+    for S in (5, 6, 7, 8):
+        Hs = Q.HPsi(HL, psi, HR, hconj=1, mode="split_bf16", nslices=S)
+        got = Hs.apply(xv)
+        ms = time_hpsi(Hs, x, y, 20, 3)
+        es, _, _ = Hs.lanczos(xv, 20)
+        inf = Hs.stage_info()
+        table[str(S)] = {"rel_err_hpsi_vs_fp64": float(np.linalg.norm(got - ref) / np.linalg.norm(ref)),
+                         "E0_rel_err_lanczos20": float(abs(es - e_ref) / abs(e_ref)), "hpsi_ms": ms,
+                         "executed_slice_gflop": float(sum(inf["split_exec_flops"]) / 1e9)}
+        del Hs
+    out["4_split_bf16_heis4000"] = {"fp64_hpsi_ms": ms_fp64, "E0_fp64_lanczos20": e_ref, "slices": table}
+    log(f"[bench] config 4: {out['4_split_bf16_heis4000']}")
     # 5: SU(2)xU(1) Hubbard H*psi at D*=4000 with parity signs, 30 Davidson iterations
     w = WORKLOADS["hubbard4000"]
     HL, psi, HR, ferm = make_workload(w)

@@ -186,8 +186,10 @@ int qsx_hpsi_weights(const qsx_hpsi_plan* plan, double* w);
    of stage 1, 2; [2..3] algorithmic bytes; [4..5] work items; [6..7] pairs;
    [8..9] engine (0 workgroup tiles, 1 wave tiles); [10..11] workgroups of
    the main GEMM launch; [12..13] largest fragments per tile side (wave
-   tiles); [14..15] reduction workgroups (0: no split tiles) */
-#define QSX_STAGE_INFO 16
+   tiles); [14..15] reduction workgroups (0: no split tiles); [16..17]
+   executed slice-product flops (split precision, else 0); engine 2 = the
+   split-precision kernel */
+#define QSX_STAGE_INFO 18
 int qsx_hpsi_stage_info(const qsx_hpsi_plan* plan, double* info);
 /* Work decomposition policy of one batched contraction, as
    int32[QSX_POLICY_INTS]: [0] engine (0 workgroup tiles, 1 wave tiles),

@@ -643,6 +643,8 @@ std::unique_ptr<CtrPlan> plan_create(const qsx_tensor* A, const int32_t* ia, int
     P->verbose = o->verbose;
   }
   if (P->mode < QSX_MODE_BATCHED || P->mode > QSX_MODE_SPLIT_F32) fail(QSX_ERR_ARG, "invalid mode %d", P->mode);
+  if (P->mode == QSX_MODE_SPLIT_BF16 || P->mode == QSX_MODE_SPLIT_F32)
+    fail(QSX_ERR_UNSUPPORTED, "split precision is available for H*psi plans only (QSX_MODE_SPLIT_BF16)");
   build_structure(*P, perm, nperm, o ? o->fermA : nullptr, o ? o->fermB : nullptr);
   if (upload) plan_upload(*P);
   return P;
@@ -671,9 +673,6 @@ void plan_execute(CtrPlan& P, const double* dA, const double* dB, double* dC, hi
     }
   } else if (P.mode == QSX_MODE_BATCHED) {
     launch_worklist(P.work, P.dqueues, P.ditems, P.dreds, P.dpairs, a, b, c, P.dscratch, s);
-  } else {
-    launch_work_split(P.mode, P.nslices, P.dqueues, (int)P.work.queues.size(), P.ditems, P.dreds,
-                      (int)P.work.reds.size(), P.dpairs, a, b, c, P.dscratch, s);
   }
   if (P.has_perm) launch_permute_blocks(P.dpbC, P.nC, P.rc, P.perm.data(), P.dCtmp, dC, P.maxC, s);
 }

@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <tuple>
 #include <numeric>
 
 #include "contract.hpp"
@@ -66,8 +67,29 @@ struct HpsiPlan {
   bool g_sym = false;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   float last_ms[2] = {0, 0};
+  // split precision (QSX_MODE_SPLIT_BF16): BF16 slice planes and line
+  // exponents of the four operand arenas, slice-addressed pairs, 16x16 tiles
+  struct SplitSide {
+    uint16_t* planes = nullptr;
+    int64_t plane = 0;
+    int32_t* exps = nullptr;
+    SplitLine* dlines = nullptr;
+    int nlines = 0;
+  };
+  int nsl = 6;
+  SplitSide spHL, spPsi, spT, spHR;
+  SplitPair *dsp1 = nullptr, *dsp2 = nullptr, *dsp1s = nullptr, *dsp2s = nullptr;
+  WorkList swork1, swork2;
+  WorkItem *dsitems1 = nullptr, *dsitems2 = nullptr;
+  double split_exec_flops[2] = {0, 0};
   ~HpsiPlan() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
+    for (SplitSide* sp : {&spHL, &spPsi, &spT, &spHR}) {
+      dev_free(sp->planes);
+      dev_free(sp->exps);
+      dev_free(sp->dlines);
+    }
+    for (void* p : {(void*)dsp1, (void*)dsp2, (void*)dsp1s, (void*)dsp2s, (void*)dsitems1, (void*)dsitems2}) dev_free(p);
     dev_free(dHL);
     dev_free(dHR);
     dev_free(dT);
@@ -246,6 +268,10 @@ static double time_stage(HpsiPlan& H, int stage, const WorkList& W, const double
 }
 
 static void hpsi_autotune(HpsiPlan& H, int reps, double* out) {
+  if (H.mode != QSX_MODE_BATCHED) {  // the other modes have fixed work lists
+    if (out) out[0] = out[1] = -1;
+    return;
+  }
   const std::string key = hist_key(H.targets1, H.pairs1) + "|" + hist_key(H.targets2, H.pairs2);
   auto it = g_tune_cache.find(key);
   if (it != g_tune_cache.end()) {
@@ -316,22 +342,129 @@ static void hpsi_autotune(HpsiPlan& H, int reps, double* out) {
   hpsi_build_work(H);
 }
 
+// ---------------------------------------------------------------- split precision
+// Lines of one operand side of a pair list: A side = rows of each A block view
+// (over its K), B side = columns of each B block view.  Blocks shared by
+// several pairs (same offset and strides) share their lines.
+static std::vector<SplitLine> split_lines(const std::vector<PairDesc>& pairs, const std::vector<WorkTarget>& targets,
+                                          bool aside, std::vector<int32_t>& ebase) {
+  std::map<std::tuple<int64_t, int64_t, int64_t, int>, int32_t> seen;
+  std::vector<SplitLine> lines;
+  ebase.assign(pairs.size(), 0);
+  for (const WorkTarget& t : targets)
+    for (int64_t p = t.p0; p < t.p1; ++p) {
+      const PairDesc& d = pairs[p];
+      const int64_t off = aside ? d.a_off : d.b_off;
+      const int64_t sl = aside ? ((d.flags & 1) ? d.lda : 1) : ((d.flags & 2) ? 1 : d.ldb);  // between lines
+      const int64_t sk = aside ? ((d.flags & 1) ? 1 : d.lda) : ((d.flags & 2) ? d.ldb : 1);  // along a line
+      const int n = aside ? t.M : t.N;
+      auto key = std::make_tuple(off, sl, sk, n);
+      auto it = seen.find(key);
+      if (it != seen.end()) {
+        ebase[p] = it->second;
+        continue;
+      }
+      const int32_t base = (int32_t)lines.size();
+      seen[key] = base;
+      ebase[p] = base;
+      for (int i = 0; i < n; ++i) lines.push_back(SplitLine{off + i * sl, d.K, (int32_t)sk, base + i, 0});
+    }
+  return lines;
+}
+
+static std::vector<SplitPair> split_pairs(const std::vector<PairDesc>& pairs, const std::vector<int32_t>& ea,
+                                          const std::vector<int32_t>& eb) {
+  std::vector<SplitPair> v(pairs.size());
+  for (size_t p = 0; p < pairs.size(); ++p) {
+    const PairDesc& d = pairs[p];
+    SplitPair& q = v[p];
+    q.a_off = d.a_off;
+    q.b_off = d.b_off;
+    q.K = d.K;
+    q.sra = (d.flags & 1) ? d.lda : 1;
+    q.ska = (d.flags & 1) ? 1 : d.lda;
+    q.skb = (d.flags & 2) ? d.ldb : 1;
+    q.scb = (d.flags & 2) ? 1 : d.ldb;
+    q.ea = ea[p];
+    q.eb = eb[p];
+    q.pad = 0;
+    q.afac = d.afac;
+  }
+  return v;
+}
+
+static void split_side_alloc(HpsiPlan::SplitSide& sp, const std::vector<SplitLine>& lines, int64_t plane, int S,
+                             hipStream_t s) {
+  sp.plane = plane;
+  sp.nlines = (int)lines.size();
+  sp.planes = (uint16_t*)dev_alloc(sizeof(uint16_t) * std::max<int64_t>(1, plane * S));
+  sp.exps = (int32_t*)dev_alloc(sizeof(int32_t) * std::max<size_t>(1, lines.size()));
+  sp.dlines = upload_vec(lines, s);
+}
+
+// slice planes of the constant environments, lines of psi and T, the
+// slice-addressed pairs and the 16x16-tile work lists
+static void hpsi_split_setup(HpsiPlan& H) {
+  hipStream_t s = default_stream();
+  const int S = H.nsl;
+  std::vector<int32_t> ea1, eb1, ea2, eb2;
+  std::vector<SplitLine> lA1 = split_lines(H.pairs1, H.targets1, true, ea1);
+  std::vector<SplitLine> lB1 = split_lines(H.pairs1, H.targets1, false, eb1);
+  std::vector<SplitLine> lA2 = split_lines(H.pairs2, H.targets2, true, ea2);
+  std::vector<SplitLine> lB2 = split_lines(H.pairs2, H.targets2, false, eb2);
+  int64_t hl_arena = H.hl_total;
+  for (int t = 0; t < H.nterm; ++t) hl_arena = std::max(hl_arena, H.hl_base[t] + H.hl_total);
+  split_side_alloc(H.spHL, lA1, hl_arena, S, s);
+  split_side_alloc(H.spPsi, lB1, H.N, S, s);
+  split_side_alloc(H.spT, lA2, H.t_total, S, s);
+  split_side_alloc(H.spHR, lB2, H.hr_total, S, s);
+  launch_split_lines(H.spHL.dlines, H.spHL.nlines, H.dHL, H.spHL.planes, H.spHL.plane, S, H.spHL.exps, s);
+  launch_split_lines(H.spHR.dlines, H.spHR.nlines, H.dHR, H.spHR.planes, H.spHR.plane, S, H.spHR.exps, s);
+  H.dsp1 = upload_vec(split_pairs(H.pairs1, ea1, eb1), s);
+  H.dsp2 = upload_vec(split_pairs(H.pairs2, ea2, eb2), s);
+  H.dsp1s = upload_vec(split_pairs(H.pairs1s, ea1, eb1), s);
+  H.dsp2s = upload_vec(split_pairs(H.pairs2s, ea2, eb2), s);
+  WorkPolicy pol;
+  pol.engine = ENG_WAVE;
+  pol.wave_max = 1;
+  pol.group_mfma = 1 << 30;
+  pol.split_mfma = 1 << 30;
+  H.swork1 = build_work(H.targets1, H.pairs1, pol);
+  H.swork2 = build_work(H.targets2, H.pairs2, pol);
+  H.dsitems1 = upload_vec(H.swork1.items, s);
+  H.dsitems2 = upload_vec(H.swork2.items, s);
+  // executed slice-product flops: per 16x16 tile and k-step of 16 of a pair,
+  // S(S+1)/2 MFMAs of 2*16*16*16
+  const WorkList* W[2] = {&H.swork1, &H.swork2};
+  const std::vector<PairDesc>* P[2] = {&H.pairs1, &H.pairs2};
+  for (int st = 0; st < 2; ++st) {
+    double steps = 0;
+    for (const WorkItem& w : W[st]->items)
+      for (int p = w.p0; p <= w.p1 && w.slot != -2; ++p) steps += ((*P[st])[p].K + 15) / 16;
+    H.split_exec_flops[st] = steps * (S * (S + 1) / 2) * 2.0 * 16 * 16 * 16;
+  }
+  QSX_HIP(hipStreamSynchronize(s));
+}
+
 static void hpsi_launch(HpsiPlan& H, const double* dpsi, double* dhpsi, hipStream_t s, bool timed, bool sym = false) {
   const PairDesc* p1 = sym ? H.dpairs1s : H.dpairs1;
   const PairDesc* p2 = sym ? H.dpairs2s : H.dpairs2;
   // events: [0] start, [1] stage 1 done, [2] stage 2 done
   if (timed) QSX_HIP(hipEventRecord(H.ev[0], s));
-  if (H.mode == QSX_MODE_BATCHED || H.mode == QSX_MODE_PLAIN)
+  if (H.mode == QSX_MODE_SPLIT_BF16) {
+    const int S = H.nsl;
+    launch_split_lines(H.spPsi.dlines, H.spPsi.nlines, dpsi, H.spPsi.planes, H.spPsi.plane, S, H.spPsi.exps, s);
+    launch_split_gemm(S, H.dsitems1, (int)H.swork1.items.size(), sym ? H.dsp1s : H.dsp1, H.spHL.planes,
+                      H.spHL.plane, H.spHL.exps, H.spPsi.planes, H.spPsi.plane, H.spPsi.exps, H.dT, s);
+    if (timed) QSX_HIP(hipEventRecord(H.ev[1], s));
+    launch_split_lines(H.spT.dlines, H.spT.nlines, H.dT, H.spT.planes, H.spT.plane, S, H.spT.exps, s);
+    launch_split_gemm(S, H.dsitems2, (int)H.swork2.items.size(), sym ? H.dsp2s : H.dsp2, H.spT.planes, H.spT.plane,
+                      H.spT.exps, H.spHR.planes, H.spHR.plane, H.spHR.exps, dhpsi, s);
+  } else {
     launch_worklist(H.work1, H.dq1, H.ditems1, H.dreds1, p1, H.dHL, dpsi, H.dT, H.dscratch, s);
-  else
-    launch_work_split(H.mode, H.nslices, H.dq1, (int)H.work1.queues.size(), H.ditems1, H.dreds1,
-                      (int)H.work1.reds.size(), p1, H.dHL, dpsi, H.dT, H.dscratch, s);
-  if (timed) QSX_HIP(hipEventRecord(H.ev[1], s));
-  if (H.mode == QSX_MODE_BATCHED || H.mode == QSX_MODE_PLAIN)
+    if (timed) QSX_HIP(hipEventRecord(H.ev[1], s));
     launch_worklist(H.work2, H.dq2, H.ditems2, H.dreds2, p2, H.dT, H.dHR, dhpsi, H.dscratch, s);
-  else
-    launch_work_split(H.mode, H.nslices, H.dq2, (int)H.work2.queues.size(), H.ditems2, H.dreds2,
-                      (int)H.work2.reds.size(), p2, H.dT, H.dHR, dhpsi, H.dscratch, s);
+  }
   if (timed) QSX_HIP(hipEventRecord(H.ev[2], s));
 }
 
@@ -364,6 +497,15 @@ static void hpsi_create(HpsiPlan& H, const qsx_tensor* HL, const qsx_tensor* psi
     H.mode = o->mode;
     H.nslices = o->nslices;
     H.verbose = o->verbose;
+  }
+  if (H.mode < QSX_MODE_BATCHED || H.mode > QSX_MODE_SPLIT_F32) fail(QSX_ERR_ARG, "invalid mode %d", H.mode);
+  if (H.mode == QSX_MODE_SPLIT_F32)
+    fail(QSX_ERR_UNSUPPORTED,
+         "FP32 slices: an exact FP32 accumulation allows ~8-bit slices, so 3 FP32 slices carry ~24 bits and "
+         "cannot reach 1e-12; use QSX_MODE_SPLIT_BF16 (nslices >= 6)");
+  if (H.mode == QSX_MODE_SPLIT_BF16) {
+    H.nsl = H.nslices > 0 ? H.nslices : 6;
+    if (H.nsl < 3 || H.nsl > 8) fail(QSX_ERR_ARG, "split precision: %d slices (3..8 supported)", H.nsl);
   }
   H.N = H.psi.total();
   H.pol1.slots = H.pol2.slots = resident_slots();
@@ -529,6 +671,7 @@ static void hpsi_create(HpsiPlan& H, const qsx_tensor* HL, const qsx_tensor* psi
   H.dpairs1s = upload_vec(H.pairs1s, s);
   H.dpairs2s = upload_vec(H.pairs2s, s);
   hpsi_build_work(H);
+  if (H.mode == QSX_MODE_SPLIT_BF16) hpsi_split_setup(H);
   QSX_HIP(hipStreamSynchronize(s));
   H.flops = 0;
   for (int t = 0; t < H.nterm; ++t) {
@@ -840,6 +983,11 @@ int qsx_hpsi_stage_info(const qsx_hpsi_plan* plan, double* info) {
                                             : (double)W[t]->queues.size();
     info[12 + t] = W[t]->maxf;
     info[14 + t] = (double)W[t]->reds.size();
+    info[16 + t] = H.split_exec_flops[t];
+    if (H.mode == QSX_MODE_SPLIT_BF16) {
+      info[10 + t] = (double)((H.swork1.items.size() * (t == 0) + H.swork2.items.size() * (t == 1) + 3) / 4);
+      info[8 + t] = 2;
+    }
   }
   return 0;
 }

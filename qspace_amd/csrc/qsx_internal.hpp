@@ -216,10 +216,29 @@ inline void launch_worklist(const WorkList& W, const WorkQueue* dqueues, const W
 void launch_copy(double* d, const double* src, int64_t n, hipStream_t s);
 void launch_pair_gemm_f64(const PairDesc& p, int64_t c_off, int M, int N, int beta, const double* A,
                           const double* B, double* C, hipStream_t s);
-void launch_work_split(int mode, int nslices, const WorkQueue* dqueues, int nqueues, const WorkItem* ditems,
-                       const ReduceItem* dreds, int nreds,
-                       const PairDesc* dpairs, const double* A, const double* B, double* C, double* scratch,
-                       hipStream_t s);
+// split precision (split.hip): a contraction line of an operand block (a row
+// of A or a column of B over its k-range) and a pair in slice addressing
+struct SplitLine {
+  int64_t off;    // element offset of the line's first element
+  int32_t K;      // elements on the line
+  int32_t sk;     // element stride along the line
+  int32_t eidx;   // index of the line's exponent
+  int32_t pad;
+};
+struct SplitPair {
+  int64_t a_off, b_off;
+  int32_t K;
+  int32_t sra, ska;   // A element (i, k) at a_off + i*sra + k*ska
+  int32_t skb, scb;   // B element (k, j) at b_off + k*skb + j*scb
+  int32_t ea, eb;     // exponent index of A row 0 / B column 0 of the pair's blocks
+  int32_t pad;
+  double afac;
+};
+void launch_split_lines(const SplitLine* dlines, int nlines, const double* X, uint16_t* planes, int64_t plane,
+                        int S, int32_t* exps, hipStream_t s);
+void launch_split_gemm(int S, const WorkItem* ditems, int nitems, const SplitPair* dpairs, const uint16_t* Ap,
+                       int64_t apl, const int32_t* expA, const uint16_t* Bp, int64_t bpl, const int32_t* expB,
+                       double* C, hipStream_t s);
 
 // generic block permute: dst block b = permute(src block b, perm) for all blocks
 struct PermBlock {
