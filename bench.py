@@ -418,6 +418,7 @@ def main():
         "config": {"workload": w["name"], "Dstar": w["Dstar"], "psi_elements": H.N, "hconj": 1,
                    "useful_gflop_per_hpsi": H.flops / 1e9, "pairs": list(info["pairs"]),
                    "items": list(info["tiles"]), "kernels": list(info["kernel"]), "grid": list(info["grid"]),
+                   "stage_flops": list(info["flops"]), "stage_bytes": list(info["bytes"]),
                    "path": "batched FP64 MFMA (wave-tile or workgroup-tile engine, autotuned) + hipGraph"},
         "roofline": roof,
         "cpu_baseline": cpu,
