@@ -39,11 +39,11 @@ def make_opts(mode="batched", nslices=0, skip_zero=1, hconj=0, fermA=None, fermB
 
 
 POLICY_KEYS = ("engine", "split_steps", "tile_overhead", "slots", "wave_max", "group_mfma", "split_mfma", "xcd_bins",
-               "group_waves", "r9", "r10", "r11")
+               "group_waves", "fused", "r10", "r11")
 NPOL = len(POLICY_KEYS)
 ENGINES = {"grouped": 0, "wave": 1}
 _POLICY_DEFAULT = {"engine": 1, "split_steps": 16, "tile_overhead": 1024, "slots": 1024, "wave_max": 2,
-                   "group_mfma": 64, "split_mfma": 1 << 30, "xcd_bins": 8, "group_waves": 4, "r9": 0,
+                   "group_mfma": 64, "split_mfma": 1 << 30, "xcd_bins": 8, "group_waves": 4, "fused": 0,
                    "r10": 0, "r11": 0}
 
 
@@ -58,7 +58,7 @@ def _policy_ints(p: dict):
 
 
 def _policy_dict(v):
-    return {k: int(x) for k, x in zip(POLICY_KEYS[:9], v)}
+    return {k: int(x) for k, x in zip(POLICY_KEYS[:10], v)}
 
 
 def _ivec(v):
@@ -303,8 +303,14 @@ class HPsi:
         return {"stage_ms": (ms[0], ms[1]), "policy": (_policy_dict(pol[:NPOL]), _policy_dict(pol[NPOL:]))}
 
     def set_policy(self, stage1: dict, stage2: dict):
+        """work decomposition per stage (keys of POLICY_KEYS); stage1["fused"]
+        = 1 runs both stages in one launch"""
         v = (C.c_int32 * (2 * NPOL))(*(_policy_ints(stage1) + _policy_ints(stage2)))
         _capi.check(_capi.lib().qsx_hpsi_set_policy(self.h, v), None)
+
+    def status(self) -> int:
+        """0, or QSX_ERR_DEVICE if a bounded wait of the single-launch path gave up"""
+        return _capi.lib().qsx_hpsi_status(self.h)
 
     def weights(self) -> np.ndarray:
         w = np.zeros(self.N)

@@ -328,6 +328,8 @@ static void choose_layouts(CtrPlan& P) {
   };
   int wa = 0, wb = 0;
   bool fa = is_front_end(P.ica, ra, wa), fb = is_front_end(P.icb, rb, wb);
+  if (P.force_layout & 1) fa = false;
+  if (P.force_layout & 2) fb = false;
   // native layouts: A with contracted legs at its end is MxK, at its front
   // KxM; B with contracted legs at its front is KxN, at its end NxK
   auto front_of = [&](const std::vector<int>& legs) {
@@ -360,14 +362,17 @@ static void choose_layouts(CtrPlan& P) {
     P.ica = ica0;
     P.icb = icb0;
   }
-  if (fa) {  // keep A, permute B to [contracted (A order) | kept]
+  // permuted copies: A as [kept | contracted] (M x K), B as [kept |
+  // contracted] (N x K), the orientations whose MFMA fragments are 16
+  // consecutive elements per k
+  if (fa) {  // keep A, permute B to [kept | contracted (A order)]
     apply(ordA);
     P.la.permute = false;
     P.la.kmajor = (wa == 1);
     P.lb.permute = true;
-    P.lb.perm.assign(P.icb.begin(), P.icb.end());
-    P.lb.perm.insert(P.lb.perm.end(), P.kb.begin(), P.kb.end());
-    P.lb.kmajor = false;
+    P.lb.perm.assign(P.kb.begin(), P.kb.end());
+    P.lb.perm.insert(P.lb.perm.end(), P.icb.begin(), P.icb.end());
+    P.lb.kmajor = true;
     return;
   }
   if (fb) {  // keep B, permute A to [kept | contracted (B order)]
@@ -386,9 +391,9 @@ static void choose_layouts(CtrPlan& P) {
   P.la.perm.insert(P.la.perm.end(), P.ica.begin(), P.ica.end());
   P.la.kmajor = false;
   P.lb.permute = true;
-  P.lb.perm.assign(P.icb.begin(), P.icb.end());
-  P.lb.perm.insert(P.lb.perm.end(), P.kb.begin(), P.kb.end());
-  P.lb.kmajor = false;
+  P.lb.perm.assign(P.kb.begin(), P.kb.end());
+  P.lb.perm.insert(P.lb.perm.end(), P.icb.begin(), P.icb.end());
+  P.lb.kmajor = true;
 }
 
 static void build_structure(CtrPlan& P, const int32_t* perm, int nperm, const double* fA, const double* fB) {
@@ -626,9 +631,10 @@ void plan_permute_operand(const CtrPlan& P, int which, const double* src, double
 
 std::unique_ptr<CtrPlan> plan_create(const qsx_tensor* A, const int32_t* ia, int na, int conjA, const qsx_tensor* B,
                                      const int32_t* ib, int nb, int conjB, const int32_t* perm, int nperm,
-                                     const qsx_opts* o, bool upload) {
+                                     const qsx_opts* o, bool upload, int force) {
   ensure_device();
   auto P = std::make_unique<CtrPlan>();
+  P->force_layout = force;
   P->A = make_desc(A, "A");
   P->B = make_desc(B, "B");
   if (!ia || !ib || na <= 0 || nb <= 0) fail(QSX_ERR_ARG, "empty contraction index");

@@ -17,6 +17,7 @@ struct OperandLayout {
 
 struct CtrPlan {
   TensorDesc A, B;
+  int force_layout = 0;  // bit 0: A as M x K copy, bit 1: B as N x K copy
   std::vector<int> ica, icb, ka, kb;  // contracted legs (pair order used for K), kept legs
   int conjA = 0, conjB = 0;
   int mode = QSX_MODE_BATCHED;
@@ -62,9 +63,12 @@ struct CtrPlan {
   ~CtrPlan();
 };
 
+// force: bit 0 / bit 1 -- give A / B a permuted copy in the orientation the
+// GEMM loads best (A as M x K, B as N x K: 16 consecutive rows / columns per
+// k) even when the native layout would do (plans of constant operands)
 std::unique_ptr<CtrPlan> plan_create(const qsx_tensor* A, const int32_t* ia, int na, int conjA, const qsx_tensor* B,
                                      const int32_t* ib, int nb, int conjB, const int32_t* perm, int nperm,
-                                     const qsx_opts* o, bool upload = true);
+                                     const qsx_opts* o, bool upload = true, int force = 0);
 // build the work list for the plan's policy and upload pairs/items
 void plan_upload(CtrPlan& P);
 void plan_execute(CtrPlan& P, const double* dA, const double* dB, double* dC, hipStream_t s);

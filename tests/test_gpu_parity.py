@@ -164,6 +164,8 @@ POLICIES = [
     {"engine": "wave", "wave_max": 2, "group_mfma": 16, "split_mfma": 32, "xcd_bins": 3},
     {"engine": "wave", "wave_max": 1, "group_mfma": 8, "group_waves": 16},
     {"engine": "wave", "wave_max": 1, "group_mfma": 8, "group_waves": 8, "split_mfma": 16, "xcd_bins": 5},
+    {"engine": "wave", "wave_max": 2, "group_mfma": 16, "fused": 1},
+    {"engine": "wave", "wave_max": 1, "group_mfma": 1 << 30, "fused": 1},
 ]
 
 
@@ -179,6 +181,7 @@ def test_hpsi_policies_match_oracle(gpu, pi):
     ref = _hpsi_oracle(HL, psi, HR, 1).packed()
     assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < TOL
     assert np.array_equal(H.apply(v), got)
+    assert H.status() == 0
 
 
 @pytest.mark.parametrize("pi", range(len(POLICIES)))
@@ -263,3 +266,40 @@ def test_split_precision_slices(gpu):
         assert errs[S + 1] < errs[S] * 0.1 or errs[S + 1] < 1e-14, errs
     assert errs[7] < TOL, errs
     assert errs[3] > 1e-9, errs  # the slices really are narrow
+
+
+@pytest.mark.parametrize("kind", ["heis", "hubbard"])
+def test_hpsi_single_launch(gpu, kind):
+    """both stages in one launch (stage 2 waits per XCD partition on an L2
+    counter): equal to the two-launch H*psi and the oracle, bitwise stable
+    over 40 graph replays and direct launches, no bounded wait gave up."""
+    if kind == "heis":
+        HL, psi, HR = gen.heis_hpsi(seed=1003, Dstar=2000)
+        ferm = None
+    else:
+        HL, psi, HR = gen.hubbard_hpsi(seed=28, Dstar=1500, mu=3.0, sigma=0.8, dmax=200)
+        ferm = gen.fermion_parity_signs(HL)
+    H = Q.HPsi(HL, psi, HR, hconj=1, fermA=ferm)
+    pol = {"engine": "wave", "wave_max": 2, "group_mfma": 64}
+    H.set_policy(pol, pol)
+    x = np.random.RandomState(9).standard_normal(H.N)
+    two = H.apply(x)
+    H.set_policy(dict(pol, fused=1), pol)
+    one = H.apply(x)
+    assert np.linalg.norm(one - two) / np.linalg.norm(two) < 1e-14
+    dx = Q.DeviceBuffer.from_array(x)
+    dy = Q.DeviceBuffer(H.N)
+    outs = []
+    for graph in (True, False):
+        H.use_graph(graph)
+        for _ in range(20):
+            H.apply_dev(dx, dy)
+            outs.append(dy.download())
+    for o in outs:
+        assert np.array_equal(o, outs[0])
+    assert H.status() == 0
+    if kind == "hubbard":
+        ref = _hpsi_oracle(HL, psi, HR, 1, fermA=ferm)
+        xs = psi.with_packed(x)
+        ref = _hpsi_oracle(HL, xs, HR, 1, fermA=ferm).packed()
+        assert np.linalg.norm(outs[0] - ref) / np.linalg.norm(ref) < TOL
