@@ -900,12 +900,10 @@ static int argmin(const std::vector<double>& w) {
 static void lanczos(HpsiPlan& H, const double* psi0, int niter, double* e0, double* ritz, double* gs) {
   hipStream_t s = default_stream();
   const int64_t N = H.N;
-  if (niter < 1) fail(QSX_ERR_ARG, "niter must be >= 1");
+  if (niter < 1 || niter > 500) fail(QSX_ERR_ARG, "niter must be in 1..500");
   double* V = (double*)dev_alloc(sizeof(double) * N * (niter + 1));
   double* w = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, N));
   double* part = (double*)dev_alloc(sizeof(double) * DOT_BLOCKS * (niter + 2));
-  double* c1 = (double*)dev_alloc(sizeof(double) * (niter + 2));
-  double* c2 = (double*)dev_alloc(sizeof(double) * (niter + 2));
   double* alpha = (double*)dev_alloc(sizeof(double) * (niter + 1));
   double* beta = (double*)dev_alloc(sizeof(double) * (niter + 1));
   double* nrm = (double*)dev_alloc(sizeof(double) * 2);
@@ -923,14 +921,14 @@ static void lanczos(HpsiPlan& H, const double* psi0, int niter, double* e0, doub
     for (int j = 0; j < niter; ++j) {
       double* vj = V + (int64_t)j * N;
       hpsi_launch(H, vj, w, s, false, true);
-      vec_mdot(vj, 0, 1, w, N, part, alpha + j, 0, s);  // alpha_j = <v_j, H v_j>
-      // full re-orthogonalisation, classical Gram-Schmidt twice
-      vec_mdot(V, N, j + 1, w, N, part, c1, 0, s);
-      vec_gs_update(w, V, N, c1, j + 1, N, s);
-      vec_mdot(V, N, j + 1, w, N, part, c2, 0, s);
-      vec_gs_update(w, V, N, c2, j + 1, N, s);
-      vec_mdot(w, 0, 1, w, N, part, nrm, 0, s);
-      vec_normalize(w, V + (int64_t)(j + 1) * N, nrm, beta + j, N, s);
+      // full re-orthogonalisation, classical Gram-Schmidt twice; the first
+      // pass's coefficient of v_j is alpha_j = <v_j, H v_j>
+      vec_mdot_partial(V, N, j + 1, w, N, part, s);
+      vec_gs_update_partial(w, V, N, part, j + 1, N, alpha + j, j, s);
+      vec_mdot_partial(V, N, j + 1, w, N, part, s);
+      vec_gs_update_partial(w, V, N, part, j + 1, N, nullptr, 0, s);
+      vec_mdot_partial(w, 0, 1, w, N, part, s);
+      vec_normalize_partial(w, V + (int64_t)(j + 1) * N, part, beta + j, N, s);
     }
     QSX_HIP(hipStreamEndCapture(s, &g));
     QSX_HIP(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
@@ -964,25 +962,23 @@ static void lanczos(HpsiPlan& H, const double* psi0, int niter, double* e0, doub
     }
   } catch (...) {
     if (exec) (void)hipGraphExecDestroy(exec);
-    for (void* p : {(void*)V, (void*)w, (void*)part, (void*)c1, (void*)c2, (void*)alpha, (void*)beta, (void*)nrm})
-      dev_free(p);
+    for (void* p : {(void*)V, (void*)w, (void*)part, (void*)alpha, (void*)beta, (void*)nrm}) dev_free(p);
     throw;
   }
   if (exec) (void)hipGraphExecDestroy(exec);
-  for (void* p : {(void*)V, (void*)w, (void*)part, (void*)c1, (void*)c2, (void*)alpha, (void*)beta, (void*)nrm})
-    dev_free(p);
+  for (void* p : {(void*)V, (void*)w, (void*)part, (void*)alpha, (void*)beta, (void*)nrm}) dev_free(p);
 }
 
 static void davidson(HpsiPlan& H, const double* psi0, int niter, double* e0, double* ritz, double* gs) {
   hipStream_t s = default_stream();
   const int64_t N = H.N;
-  if (niter < 1) fail(QSX_ERR_ARG, "niter must be >= 1");
+  if (niter < 1 || niter > 500) fail(QSX_ERR_ARG, "niter must be in 1..500");
   const int mmax = niter + 1;
   double* V = (double*)dev_alloc(sizeof(double) * N * mmax);
   double* HV = (double*)dev_alloc(sizeof(double) * N * mmax);
   double* u = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, N));
   double* hu = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, N));
-  double* part = (double*)dev_alloc(sizeof(double) * DOT_BLOCKS * (mmax + 1));
+  double* part = (double*)dev_alloc(sizeof(double) * DOT_BLOCKS * (mmax + 2));
   double* c = (double*)dev_alloc(sizeof(double) * (mmax + 1));
   double* nrm = (double*)dev_alloc(sizeof(double) * 2);
   double* dy = (double*)dev_alloc(sizeof(double) * (mmax + 1));
@@ -1011,28 +1007,31 @@ static void davidson(HpsiPlan& H, const double* psi0, int niter, double* e0, dou
       vec_combine(u, V, N, dy, k, N, s);
       vec_combine(hu, HV, N, dy, k, N, s);
       if (it + 1 == niter) break;
-      // residual r = H u - theta u, orthogonalised twice against V
+      // residual r = H u - theta u, orthogonalised twice against V (the
+      // reductions' final sums are folded into their consumers)
       QSX_HIP(hipMemcpyAsync(c, &th, sizeof(double), hipMemcpyHostToDevice, s));
       vec_axmy(hu, hu, u, c, N, s);
       for (int rep = 0; rep < 2; ++rep) {
-        vec_mdot(V, N, k, hu, N, part, c, 0, s);
-        vec_gs_update(hu, V, N, c, k, N, s);
+        vec_mdot_partial(V, N, k, hu, N, part, s);
+        vec_gs_update_partial(hu, V, N, part, k, N, nullptr, 0, s);
       }
-      vec_mdot(hu, 0, 1, hu, N, part, nrm, 0, s);
-      double r2 = 0;
-      QSX_HIP(hipMemcpyAsync(&r2, nrm, sizeof(double), hipMemcpyDeviceToHost, s));
+      double* rpart = part + (int64_t)DOT_BLOCKS * mmax;  // |r|^2 partials
+      vec_mdot_partial(hu, 0, 1, hu, N, rpart, s);
+      vec_normalize_partial(hu, V + (int64_t)k * N, rpart, nullptr, N, s);
+      hpsi_apply(H, V + (int64_t)k * N, HV + (int64_t)k * N, s, true);
+      // grow the projected matrix by one row/column; one host round trip
+      // brings the new column and the residual norm
+      vec_mdot(V, N, k + 1, HV + (int64_t)k * N, N, part, c, 0, s);
+      std::vector<double> col(k + 1), rp(DOT_BLOCKS);
+      QSX_HIP(hipMemcpyAsync(col.data(), c, sizeof(double) * (k + 1), hipMemcpyDeviceToHost, s));
+      QSX_HIP(hipMemcpyAsync(rp.data(), rpart, sizeof(double) * DOT_BLOCKS, hipMemcpyDeviceToHost, s));
       QSX_HIP(hipStreamSynchronize(s));
-      if (r2 < 1e-28) {
+      double r2 = 0;
+      for (double x : rp) r2 += x;
+      if (r2 < 1e-28) {  // converged: the new direction is noise, keep u
         for (int j = it + 1; j < niter && ritz; ++j) ritz[j] = th;
         break;
       }
-      vec_normalize(hu, V + (int64_t)k * N, nrm, nullptr, N, s);
-      hpsi_apply(H, V + (int64_t)k * N, HV + (int64_t)k * N, s, true);
-      // grow the projected matrix by one row/column
-      vec_mdot(V, N, k + 1, HV + (int64_t)k * N, N, part, c, 0, s);
-      std::vector<double> col(k + 1);
-      QSX_HIP(hipMemcpyAsync(col.data(), c, sizeof(double) * (k + 1), hipMemcpyDeviceToHost, s));
-      QSX_HIP(hipStreamSynchronize(s));
       std::vector<double> Hn((size_t)(k + 1) * (k + 1));
       for (int i = 0; i < k; ++i)
         for (int j = 0; j < k; ++j) Hn[(size_t)i * (k + 1) + j] = Hk[(size_t)i * k + j];

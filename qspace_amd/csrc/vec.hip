@@ -8,6 +8,7 @@
 
 #include <algorithm>
 
+#include "qsx_internal.hpp"
 #include "vec.hpp"
 
 namespace qsx {
@@ -80,6 +81,46 @@ __global__ __launch_bounds__(RT) void k_normalize(const double* __restrict__ src
   for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < n; i += (int64_t)gridDim.x * RT) dst[i] = src[i] * inv;
 }
 
+// w -= sum_j c[j] V_j with c[j] = sum_b partial[j][b] summed by every block
+// (the same fixed order everywhere: deterministic); block 0 also writes
+// c[cj] to *cout when cout is given
+constexpr int kMaxM = 512;
+__global__ __launch_bounds__(RT) void k_gs_update_p(double* __restrict__ w, const double* __restrict__ V, int64_t ldv,
+                                                     const double* __restrict__ partial, int m, int64_t n,
+                                                     double* __restrict__ cout, int cj) {
+  __shared__ double c[kMaxM];
+  for (int j = threadIdx.x; j < m; j += RT) {
+    double s = 0;
+    for (int b = 0; b < DOT_BLOCKS; ++b) s += partial[(int64_t)j * DOT_BLOCKS + b];
+    c[j] = s;
+  }
+  __syncthreads();
+  if (cout && blockIdx.x == 0 && threadIdx.x == 0) *cout = c[cj];
+  for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < n; i += (int64_t)gridDim.x * RT) {
+    double s = 0;
+    for (int j = 0; j < m; ++j) s += c[j] * V[(int64_t)j * ldv + i];
+    w[i] -= s;
+  }
+}
+
+// beta = sqrt(sum_b partial[b]); dst = src / beta (every block sums the
+// partials in the same order)
+__global__ __launch_bounds__(RT) void k_normalize_p(const double* __restrict__ src, double* __restrict__ dst,
+                                                     const double* __restrict__ partial, double* __restrict__ beta,
+                                                     int64_t n) {
+  __shared__ double nb;
+  if (threadIdx.x == 0) {
+    double s = 0;
+    for (int b = 0; b < DOT_BLOCKS; ++b) s += partial[b];
+    nb = s;
+  }
+  __syncthreads();
+  const double b = sqrt(nb);
+  const double inv = b > 0 ? 1.0 / b : 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && beta) *beta = b;
+  for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < n; i += (int64_t)gridDim.x * RT) dst[i] = src[i] * inv;
+}
+
 // y = x - a * z  with scalar a on the device
 __global__ __launch_bounds__(RT) void k_axmy(double* __restrict__ y, const double* __restrict__ x,
                                               const double* __restrict__ z, const double* __restrict__ a, int64_t n) {
@@ -89,6 +130,8 @@ __global__ __launch_bounds__(RT) void k_axmy(double* __restrict__ y, const doubl
 
 int grid_for(int64_t n) { return (int)std::min<int64_t>(2048, std::max<int64_t>(1, (n + RT - 1) / RT)); }
 
+[[noreturn]] void fail_vec(int m) { fail(QSX_ERR_ARG, "eigensolver subspace of %d vectors exceeds %d", m, kMaxM); }
+
 }  // namespace
 
 void vec_mdot(const double* V, int64_t ldv, int m, const double* w, int64_t n, double* partial, double* out,
@@ -96,6 +139,24 @@ void vec_mdot(const double* V, int64_t ldv, int m, const double* w, int64_t n, d
   if (m <= 0) return;
   hipLaunchKernelGGL(k_mdot_partial, dim3(DOT_BLOCKS, m), dim3(RT), 0, s, V, ldv, w, n, partial);
   hipLaunchKernelGGL(k_mdot_final, dim3(m), dim3(RT), 0, s, partial, m, out, accumulate);
+}
+
+void vec_mdot_partial(const double* V, int64_t ldv, int m, const double* w, int64_t n, double* partial,
+                      hipStream_t s) {
+  if (m <= 0) return;
+  hipLaunchKernelGGL(k_mdot_partial, dim3(DOT_BLOCKS, m), dim3(RT), 0, s, V, ldv, w, n, partial);
+}
+
+void vec_gs_update_partial(double* w, const double* V, int64_t ldv, const double* partial, int m, int64_t n,
+                           double* cout, int cj, hipStream_t s) {
+  if (m <= 0) return;
+  if (m > kMaxM) fail_vec(m);
+  hipLaunchKernelGGL(k_gs_update_p, dim3(grid_for(n)), dim3(RT), 0, s, w, V, ldv, partial, m, n, cout, cj);
+}
+
+void vec_normalize_partial(const double* src, double* dst, const double* partial, double* beta, int64_t n,
+                           hipStream_t s) {
+  hipLaunchKernelGGL(k_normalize_p, dim3(grid_for(n)), dim3(RT), 0, s, src, dst, partial, beta, n);
 }
 
 void vec_gs_update(double* w, const double* V, int64_t ldv, const double* c, int m, int64_t n, hipStream_t s) {
