@@ -903,7 +903,7 @@ static void lanczos(HpsiPlan& H, const double* psi0, int niter, double* e0, doub
   if (niter < 1 || niter > 500) fail(QSX_ERR_ARG, "niter must be in 1..500");
   double* V = (double*)dev_alloc(sizeof(double) * N * (niter + 1));
   double* w = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, N));
-  double* part = (double*)dev_alloc(sizeof(double) * DOT_BLOCKS * (niter + 2));
+  double* part = (double*)dev_alloc(sizeof(double) * DOT_BLOCKS * 2 * (niter + 2));
   double* alpha = (double*)dev_alloc(sizeof(double) * (niter + 1));
   double* beta = (double*)dev_alloc(sizeof(double) * (niter + 1));
   double* nrm = (double*)dev_alloc(sizeof(double) * 2);
@@ -923,12 +923,12 @@ static void lanczos(HpsiPlan& H, const double* psi0, int niter, double* e0, doub
       hpsi_launch(H, vj, w, s, false, true);
       // full re-orthogonalisation, classical Gram-Schmidt twice; the first
       // pass's coefficient of v_j is alpha_j = <v_j, H v_j>
-      vec_mdot_partial(V, N, j + 1, w, N, part, s);
-      vec_gs_update_partial(w, V, N, part, j + 1, N, alpha + j, j, s);
-      vec_mdot_partial(V, N, j + 1, w, N, part, s);
-      vec_gs_update_partial(w, V, N, part, j + 1, N, nullptr, 0, s);
-      vec_mdot_partial(w, 0, 1, w, N, part, s);
-      vec_normalize_partial(w, V + (int64_t)(j + 1) * N, part, beta + j, N, s);
+      double* pa = part;
+      double* pb = part + (int64_t)DOT_BLOCKS * (niter + 1);
+      vec_mdot_partial(V, N, j + 1, w, N, pa, s);
+      vec_gs_update_mdot(w, V, N, pa, j + 1, N, alpha + j, j, V, j + 1, pb, s);
+      vec_gs_update_mdot(w, V, N, pb, j + 1, N, nullptr, 0, nullptr, 1, pa, s);
+      vec_normalize_partial(w, V + (int64_t)(j + 1) * N, pa, beta + j, N, s);
     }
     QSX_HIP(hipStreamEndCapture(s, &g));
     QSX_HIP(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));

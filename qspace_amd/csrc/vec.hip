@@ -103,6 +103,52 @@ __global__ __launch_bounds__(RT) void k_gs_update_p(double* __restrict__ w, cons
   }
 }
 
+// One Gram-Schmidt step fused with the next reduction: w -= sum_j c[j] V_j
+// (c from the partials pin[j][b], every block summing them in the same
+// order), then pout[j][b] = <V_j, w> over block b's slice (jn vectors from
+// V, or <w, w> if V2 is null).  The grid is exactly DOT_BLOCKS blocks with
+// the slice layout of k_mdot_partial, so the partials are the same sums in
+// the same order: deterministic.
+__global__ __launch_bounds__(RT) void k_gs_update_mdot(double* __restrict__ w, const double* __restrict__ V,
+                                                        int64_t ldv, const double* __restrict__ pin, int m,
+                                                        int64_t n, double* __restrict__ cout, int cj,
+                                                        const double* __restrict__ V2, int jn,
+                                                        double* __restrict__ pout) {
+  __shared__ double c[kMaxM];
+  __shared__ double red[RT / 64];
+  for (int j = threadIdx.x; j < m; j += RT) {
+    double s = 0;
+    for (int b = 0; b < DOT_BLOCKS; ++b) s += pin[(int64_t)j * DOT_BLOCKS + b];
+    c[j] = s;
+  }
+  __syncthreads();
+  if (cout && blockIdx.x == 0 && threadIdx.x == 0) *cout = c[cj];
+  const int b = blockIdx.x;
+  for (int64_t i = (int64_t)b * RT + threadIdx.x; i < n; i += (int64_t)DOT_BLOCKS * RT) {
+    double s = 0;
+    for (int j = 0; j < m; ++j) s += c[j] * V[(int64_t)j * ldv + i];
+    w[i] -= s;
+  }
+  // the slice's updated w is this thread's own elements: no barrier needed
+  // before the dots; one block-wide sum per output vector
+  const int nv = V2 ? jn : 1;
+  for (int j = 0; j < nv; ++j) {
+    const double* v = V2 ? V2 + (int64_t)j * ldv : w;
+    double s = 0;
+    for (int64_t i = (int64_t)b * RT + threadIdx.x; i < n; i += (int64_t)DOT_BLOCKS * RT) s += v[i] * w[i];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) red[wv] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0;
+      for (int q = 0; q < RT / 64; ++q) t += red[q];
+      pout[(int64_t)j * DOT_BLOCKS + b] = t;
+    }
+    __syncthreads();
+  }
+}
+
 // beta = sqrt(sum_b partial[b]); dst = src / beta (every block sums the
 // partials in the same order)
 __global__ __launch_bounds__(RT) void k_normalize_p(const double* __restrict__ src, double* __restrict__ dst,
@@ -152,6 +198,14 @@ void vec_gs_update_partial(double* w, const double* V, int64_t ldv, const double
   if (m <= 0) return;
   if (m > kMaxM) fail_vec(m);
   hipLaunchKernelGGL(k_gs_update_p, dim3(grid_for(n)), dim3(RT), 0, s, w, V, ldv, partial, m, n, cout, cj);
+}
+
+void vec_gs_update_mdot(double* w, const double* V, int64_t ldv, const double* pin, int m, int64_t n, double* cout,
+                        int cj, const double* V2, int jn, double* pout, hipStream_t s) {
+  if (m <= 0) return;
+  if (m > kMaxM) fail_vec(m);
+  hipLaunchKernelGGL(k_gs_update_mdot, dim3(DOT_BLOCKS), dim3(RT), 0, s, w, V, ldv, pin, m, n, cout, cj, V2, jn,
+                     pout);
 }
 
 void vec_normalize_partial(const double* src, double* dst, const double* partial, double* beta, int64_t n,

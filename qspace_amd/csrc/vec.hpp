@@ -20,6 +20,11 @@ void vec_gs_update_partial(double* w, const double* V, int64_t ldv, const double
                            double* cout, int cj, hipStream_t s);
 void vec_normalize_partial(const double* src, double* dst, const double* partial, double* beta, int64_t n,
                            hipStream_t s);
+// w -= V c(pin) (block 0 writes c[cj] to cout if given), then pout = the
+// partials of <V2_j, w> (j < jn) or of <w, w> (V2 null); pin and pout must
+// not overlap
+void vec_gs_update_mdot(double* w, const double* V, int64_t ldv, const double* pin, int m, int64_t n, double* cout,
+                        int cj, const double* V2, int jn, double* pout, hipStream_t s);
 void vec_combine(double* y, const double* V, int64_t ldv, const double* c, int m, int64_t n, hipStream_t s);
 void vec_normalize(const double* src, double* dst, const double* nrm2, double* beta, int64_t n, hipStream_t s);
 void vec_axmy(double* y, const double* x, const double* z, const double* a, int64_t n, hipStream_t s);
