@@ -147,11 +147,6 @@ static WorkList build_wave_work(const std::vector<WorkTarget>& targets, const st
       m0f += fm;
     }
   }
-  // solo items: by locality key, longest first within a key, G to a
-  // workgroup (a key's last workgroup padded)
-  std::stable_sort(solo.begin(), solo.end(), [](const Solo& x, const Solo& y) {
-    return x.key != y.key ? x.key < y.key : x.cost > y.cost;
-  });
   auto pad_item = []() {
     WorkItem pad{};
     pad.slot = -2;
@@ -160,64 +155,87 @@ static WorkList build_wave_work(const std::vector<WorkTarget>& targets, const st
     pad.shape = wave_shape(1, 1);
     return pad;
   };
-  for (size_t i = 0; i < solo.size();) {
-    WG g;
-    g.cost = solo[i].cost;
-    g.key = solo[i].key;
-    size_t j = i;
-    for (; j < solo.size() && j < i + G && solo[j].key == g.key; ++j) g.it.push_back(solo[j].w);
-    while ((int)g.it.size() < G) g.it.push_back(pad_item());
-    groups.push_back(std::move(g));
-    i = j;
-  }
   auto by_cost = [](const WG& x, const WG& y) { return x.cost > y.cost; };
+  // one bin's workgroups: its group workgroups plus its solo items packed G
+  // to a workgroup longest first (only the bin's last workgroup is padded;
+  // locality lives in the L2 of the bin's XCD, so a workgroup may mix keys)
+  auto pack = [&](std::vector<WG>& wgs, std::vector<Solo>& so) {
+    std::stable_sort(so.begin(), so.end(), [](const Solo& x, const Solo& y) { return x.cost > y.cost; });
+    for (size_t i = 0; i < so.size(); i += G) {
+      WG g;
+      g.cost = so[i].cost;
+      g.key = so[i].key;
+      for (size_t j = i; j < i + G; ++j) g.it.push_back(j < so.size() ? so[j].w : pad_item());
+      wgs.push_back(std::move(g));
+    }
+    std::stable_sort(wgs.begin(), wgs.end(), by_cost);
+  };
   const int nbins = pol.xcd_bins;
   if (nbins <= 1) {
-    std::stable_sort(groups.begin(), groups.end(), by_cost);
+    pack(groups, solo);
     for (const WG& g : groups)
       for (const WorkItem& w : g.it) out.items.push_back(w);
   } else {
     // XCD-aware order: workgroup i runs on XCD i % nbins (round-robin
-    // dispatch), each XCD with its own L2.  Workgroups of one locality key
-    // go to one XCD so its operand blocks are fetched into one L2; keys are
-    // dealt to XCDs longest first to the least loaded (a key heavier than
-    // an XCD's share is spread workgroup by workgroup); each XCD's list is
+    // dispatch), each XCD with its own L2.  The work of one locality key
+    // goes to one XCD so its operand blocks are fetched into one L2; keys
+    // are dealt to XCDs longest first to the least loaded (a key heavier
+    // than an XCD's share is spread item by item); each XCD's list is
     // longest first and the lists are interleaved (short lists padded with
     // empty workgroups).
-    std::map<int64_t, std::vector<WG>> bykey;
+    struct Key {
+      double cost = 0;
+      std::vector<WG> wgs;
+      std::vector<Solo> so;
+    };
+    std::map<int64_t, Key> bykey;
     double total = 0;
     for (WG& g : groups) {
       total += g.cost;
-      bykey[g.key].push_back(std::move(g));
+      Key& k = bykey[g.key];
+      k.cost += g.cost;
+      k.wgs.push_back(std::move(g));
     }
-    std::vector<std::pair<double, std::vector<WG>*>> keys;
-    for (auto& kv : bykey) {
-      double c = 0;
-      for (const WG& g : kv.second) c += g.cost;
-      keys.push_back({c, &kv.second});
+    for (Solo& x : solo) {
+      const double c = x.cost / G;  // a solo item is one wave of a workgroup
+      total += c;
+      Key& k = bykey[x.key];
+      k.cost += c;
+      k.so.push_back(x);
     }
-    std::stable_sort(keys.begin(), keys.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
+    std::vector<Key*> keys;
+    for (auto& kv : bykey) keys.push_back(&kv.second);
+    std::stable_sort(keys.begin(), keys.end(), [](const Key* x, const Key* y) { return x->cost > y->cost; });
     std::vector<std::vector<WG>> bins(nbins);
+    std::vector<std::vector<Solo>> bsolo(nbins);
     std::vector<double> load(nbins, 0.0);
     const double share = total / nbins;
-    for (auto& kc : keys) {
-      if (kc.first > share) {
-        std::stable_sort(kc.second->begin(), kc.second->end(), by_cost);
-        for (WG& g : *kc.second) {
-          const int b = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    auto least = [&]() { return (int)(std::min_element(load.begin(), load.end()) - load.begin()); };
+    for (Key* k : keys) {
+      if (k->cost > share) {
+        std::stable_sort(k->wgs.begin(), k->wgs.end(), by_cost);
+        for (WG& g : k->wgs) {
+          const int b = least();
           load[b] += g.cost;
           bins[b].push_back(std::move(g));
         }
+        std::stable_sort(k->so.begin(), k->so.end(), [](const Solo& x, const Solo& y) { return x.cost > y.cost; });
+        for (Solo& x : k->so) {
+          const int b = least();
+          load[b] += x.cost / G;
+          bsolo[b].push_back(x);
+        }
       } else {
-        const int b = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-        load[b] += kc.first;
-        for (WG& g : *kc.second) bins[b].push_back(std::move(g));
+        const int b = least();
+        load[b] += k->cost;
+        for (WG& g : k->wgs) bins[b].push_back(std::move(g));
+        for (Solo& x : k->so) bsolo[b].push_back(x);
       }
     }
     size_t depth = 0;
-    for (auto& b : bins) {
-      std::stable_sort(b.begin(), b.end(), by_cost);
-      depth = std::max(depth, b.size());
+    for (int b = 0; b < nbins; ++b) {
+      pack(bins[b], bsolo[b]);
+      depth = std::max(depth, bins[b].size());
     }
     for (size_t r = 0; r < depth; ++r)
       for (int b = 0; b < nbins; ++b) {
