@@ -280,7 +280,7 @@ static void hpsi_build_fused(HpsiPlan& H) {
       WorkList W = build_work(sub, pr, p);
       if (W.engine != ENG_WAVE || W.wg_waves != 4 || !W.reds.empty() || W.items.size() % 4)
         fail(QSX_ERR_STRUCT, "fused H*psi: unexpected work list");
-      for (WorkItem& w : W.items) w.pad = 1;  // these workgroups count for the partition
+      for (WorkItem& w : W.items) w.pad |= kCounts;  // these workgroups count for the partition
       n12[st * P + b] = (int32_t)(W.items.size() / 4);
       wl[st][b] = std::move(W.items);
     }

@@ -121,8 +121,16 @@ struct WorkItem {
                      // -2 (wave engine): no work (padding of a workgroup)
   int32_t group;     // wave engine: 1 if the workgroup's 4 waves share this
                      // tile, each over its own k-range, summed through LDS
-  int32_t pad;
+  int32_t pad;       // wave engine flags: kCounts (single-launch H*psi),
+                     // kShortItem (at most two k-chunks)
 };
+constexpr int32_t kCounts = 1;
+constexpr int32_t kShortItem = 2;
+// k-steps of 4 per chunk of the wave kernel's ring for a tile of fm x fn
+// fragments (the kernel's Pipe<fm, fn>::KS)
+__host__ __device__ inline int wave_chunk_steps(int fm, int fn) {
+  return (fm + fn) >= 6 ? 1 : (fm + fn) >= 3 ? 2 : 4;
+}
 
 // sum of the partial tiles slot0 .. slot0+nslot-1 into one tile of C
 struct ReduceItem {

@@ -149,22 +149,40 @@ __device__ __forceinline__ void mma(int kv, double af, const double (&ra)[KS][FM
   }
 }
 
-// D layout of v_mfma_f64_16x16x4f64: element v of lane l is (row l/16 + 4v, col l%16)
+// D layout of v_mfma_f64_16x16x4f64: element v of lane l is (row l/16 + 4v,
+// col l%16).  A column-major C takes it through a per-wave LDS tile
+// (stage, 16 x 17 doubles): written in the D layout, read back so that each
+// store covers 4 columns x 16 consecutive rows (4 x 128 B instead of 16 x
+// 32 B per instruction).  LDS operations of one wave complete in order, so
+// the wave needs no barrier around its private tile.
+constexpr int kStageLd = 17;
+constexpr int kStageElems = 16 * kStageLd;
 template <int FM, int FN>
 __device__ __forceinline__ void store_acc(const WorkItem& w, const d4 (&acc)[FM][FN], double* __restrict__ C,
-                                          double* __restrict__ scratch) {
+                                          double* __restrict__ scratch, double* stage) {
   const int lane = threadIdx.x & 63, lr = lane & 15, lq = lane >> 4;
   if (w.slot < 0) {
     double* c = C + w.c_off;
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
-      for (int j = 0; j < FN; ++j)
+      for (int j = 0; j < FN; ++j) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) stage[(lq + 4 * v) * kStageLd + lr] = acc[i][j][v];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int row = w.m0 + 16 * i + lr;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          const int row = w.m0 + 16 * i + lq + 4 * v, col = w.n0 + 16 * j + lr;
-          if (row < w.M && col < w.N) c[row + (int64_t)w.M * col] = acc[i][j][v];
+          const int cc = lq + 4 * v, col = w.n0 + 16 * j + cc;
+          const double x = stage[lr * kStageLd + cc];
+          if (row < w.M && col < w.N) c[row + (int64_t)w.M * col] = x;
         }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
   } else {
     double* c = scratch + (int64_t)w.slot * kSlot;
 #pragma unroll
@@ -186,14 +204,15 @@ __device__ __forceinline__ void store_acc(const WorkItem& w, const d4 (&acc)[FM]
 template <int FM, int FN>
 struct Pipe {
   static constexpr int KS = (FM + FN) >= 6 ? 1 : (FM + FN) >= 3 ? 2 : 4;
+  static_assert(KS == ((FM + FN) >= 6 ? 1 : (FM + FN) >= 3 ? 2 : 4), "chunk steps must match the work lists");
   static constexpr int D = (FM + FN) >= 6 ? 2 : (FM + FN) >= 4 ? 3 : (FM + FN) == 3 ? 4 : 6;
 };
 
-template <int FM, int FN>
+template <int FM, int FN, int D>
 __device__ __forceinline__ void run_k(const WorkItem& w, const PairDesc* __restrict__ pairs,
                                       const double* __restrict__ A, const double* __restrict__ B,
                                       d4 (&acc)[FM][FN]) {
-  constexpr int KS = Pipe<FM, FN>::KS, D = Pipe<FM, FN>::D;
+  constexpr int KS = Pipe<FM, FN>::KS;
 #pragma unroll
   for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -231,10 +250,14 @@ __device__ __forceinline__ void run_k(const WorkItem& w, const PairDesc* __restr
 
 // Partial tiles of waves 1..G-1 of a group (G = the workgroup's waves),
 // lane-fastest (conflict-free).  Groups are formed only for tiles with
-// G * fragments <= kGroupBudget.
+// G * fragments <= kGroupBudget.  The same memory holds each wave's store
+// staging tile (wave w at w * kStageElems; after a group's reduction only
+// wave 0 stores).
 constexpr int kGroupBudget = 16;
+constexpr int kLdsDoubles = (kGroupBudget - 1) * 4 * 64 > kMaxWaves * kStageElems ? (kGroupBudget - 1) * 4 * 64
+                                                                                  : kMaxWaves * kStageElems;
 struct GroupLds {
-  double part[(kGroupBudget - 1) * 4][64];
+  double buf[kLdsDoubles];
 };
 
 template <int FM, int FN>
@@ -242,7 +265,13 @@ __device__ __forceinline__ void run_item(const WorkItem& w, const PairDesc* __re
                                          const double* __restrict__ A, const double* __restrict__ B,
                                          double* __restrict__ C, double* __restrict__ scratch, GroupLds& lds) {
   d4 acc[FM][FN];
-  run_k<FM, FN>(w, pairs, A, B, acc);
+  // an item of at most two chunks (flag set by the work-list builder) runs a
+  // two-deep ring: a deeper ring would issue empty chunks' loads (out of
+  // range, no memory access, but address-unit work) for nothing
+  if (w.pad & kShortItem)
+    run_k<FM, FN, 2>(w, pairs, A, B, acc);
+  else
+    run_k<FM, FN, Pipe<FM, FN>::D>(w, pairs, A, B, acc);
   if constexpr (FM * FN <= 4) {
     if (w.group) {  // uniform over the workgroup: all its waves share this tile
       // the partials are summed in wave order 0, 1, ..., G-1: one fixed order
@@ -255,7 +284,7 @@ __device__ __forceinline__ void run_item(const WorkItem& w, const PairDesc* __re
 #pragma unroll
           for (int j = 0; j < FN; ++j)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) lds.part[(wave - 1) * F + (i * FN + j) * 4 + v][lane] = acc[i][j][v];
+            for (int v = 0; v < 4; ++v) lds.buf[((wave - 1) * F + (i * FN + j) * 4 + v) * 64 + lane] = acc[i][j][v];
       }
       __syncthreads();
       if (wave > 0) return;
@@ -265,10 +294,12 @@ __device__ __forceinline__ void run_item(const WorkItem& w, const PairDesc* __re
 #pragma unroll
           for (int j = 0; j < FN; ++j)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) acc[i][j][v] += lds.part[q * F + (i * FN + j) * 4 + v][lane];
+            for (int v = 0; v < 4; ++v) acc[i][j][v] += lds.buf[(q * F + (i * FN + j) * 4 + v) * 64 + lane];
+      store_acc<FM, FN>(w, acc, C, scratch, lds.buf);
+      return;
     }
   }
-  store_acc<FM, FN>(w, acc, C, scratch);
+  store_acc<FM, FN>(w, acc, C, scratch, lds.buf + (threadIdx.x >> 6) * kStageElems);
 }
 
 // MAXF: the largest fragment count per tile side the kernel handles.  Tiles
@@ -391,7 +422,7 @@ void k_hpsi_fused(const WorkItem* __restrict__ items, int nitems, int n1wg, cons
   const int part = (int)(blockIdx.x % (unsigned)sync.nparts);
   const int first = blockIdx.x * (NT / 64);
   // bit 0 of pad: the workgroup counts for its partition (not padding)
-  const bool counts = first < nitems && (items[first].pad & 1);
+  const bool counts = first < nitems && (items[first].pad & kCounts);
   if (stage == 1 && counts) {
     if (threadIdx.x == 0) {
       const unsigned long long gen = l2_add(&sync.cnt2[part], 1ull) / (unsigned long long)sync.n2[part];

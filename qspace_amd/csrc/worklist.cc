@@ -147,6 +147,21 @@ static WorkList build_wave_work(const std::vector<WorkTarget>& targets, const st
       m0f += fm;
     }
   }
+  // items of at most two ring chunks run the kernel's two-deep ring
+  auto mark_short = [&](WorkItem& w) {
+    if (w.p0 > w.p1 || w.slot == -2) return;
+    int64_t chunks = 0;
+    const int fm = (w.shape - SH_WAVE) / 4 + 1, fn = (w.shape - SH_WAVE) % 4 + 1;
+    const int ck = 4 * wave_chunk_steps(fm, fn);
+    for (int p = w.p0; p <= w.p1; ++p) {
+      const int kb = p == w.p0 ? w.k0 : 0, ke = p == w.p1 ? w.k1 : pairs[p].K;
+      chunks += (ke - kb + ck - 1) / ck;
+    }
+    if (chunks <= 2) w.pad |= kShortItem;
+  };
+  for (Solo& x : solo) mark_short(x.w);
+  for (WG& g : groups)
+    for (WorkItem& w : g.it) mark_short(w);
   auto pad_item = []() {
     WorkItem pad{};
     pad.slot = -2;
