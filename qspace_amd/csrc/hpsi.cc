@@ -1,5 +1,6 @@
-// hpsi.cc -- two-site H*psi as two grouped-GEMM launches, replayed from a
-// hipGraph, and the FP64 Lanczos / Davidson loops around it.
+// hpsi.cc -- two-site H*psi as two batched-GEMM launches (or one fused
+// launch, or the split-precision BF16 path), replayed from a hipGraph, and
+// the FP64 Lanczos / Davidson loops around it.
 //
 // Reference: get_HPsi.m:54-57 (Class/@Hamilton1D/private),
 //   HPsi = contractQS({HK_L,Psi},HK_R) [+ contractQS({HK_L,'*',Psi},HK_R,'*')]
@@ -8,9 +9,9 @@
 // lists across iterations (check_contraction, cpat.hh:572-593) -- here the
 // whole plan (pairs, X-symbol factors, tiles, operand layouts) is built once
 // and the per-iteration work is two kernel launches:
-//   stage 1: T_t = HL_t . psi      for both terms t (one grouped launch)
-//   stage 2: Hpsi = sum_t T_t . HR_t  (one grouped launch; the pairs of both
-//            terms of an output block run in one workgroup, fixed order)
+//   stage 1: T_t = HL_t . psi      for both terms t (one launch)
+//   stage 2: Hpsi = sum_t T_t . HR_t  (one launch; the pairs of both terms
+//            of an output block are summed in one fixed order)
 // Environments are copied to the device once, in the layouts the GEMM
 // consumes (permuted copies made once at plan time).
 #include <algorithm>

@@ -16,13 +16,14 @@
 //  * the k-loop runs in chunks of KS k-steps of 4 and double-buffers the
 //    operand registers: the next chunk's loads (also across pair boundaries)
 //    are in flight while the current chunk's MFMAs issue;
-//  * rows / columns outside the block read a clamped in-block address (their
-//    results are never stored); k beyond the pair reads a clamped address and
-//    is zeroed in both operands, so no out-of-bounds access and no garbage
-//    in the stored elements;
-//  * items are issued longest first, four waves (independent items) per
-//    workgroup; split tiles store partial tiles to scratch and the reduce
-//    kernel sums them in chunk order (deterministic, no atomics).
+//  * operand loads are raw buffer loads; a lane whose element lies outside
+//    the block (row >= M, col >= N, k beyond the pair) gets an offset past
+//    the buffer range, for which the hardware returns 0 without a memory
+//    access, so padded rows, columns and k-steps contribute exact zeros;
+//  * items are issued longest first, G waves (independent items, or the
+//    k-shares of one grouped tile) per workgroup; split tiles store partial
+//    tiles to scratch and the reduce kernel sums them in chunk order
+//    (deterministic, no atomics).
 #include <hip/hip_runtime.h>
 
 #include <cstdint>

@@ -1,11 +1,17 @@
-// worklist.cc -- host builder of the grouped-GEMM work items.
+// worklist.cc -- host builders of the work items of the batched data half.
 //
-// Per output block: pick the tile shape that minimises padded work plus a
-// per-tile overhead, cut the block into tiles, and cut each tile's k-range
-// (the concatenated k-steps of the block's pairs, in the reference pair
-// order) into chunks of at most split_steps steps.  The items are then dealt
-// to the queues of the persistent grid (one queue per resident workgroup,
-// one tile shape per queue, longest-processing-time first).
+// Two engines consume them:
+//  * wave engine (build_wave_work, the default): one wave per tile of at
+//    most 4x4 MFMA fragments, grouped or split for long k-ranges, workgroups
+//    binned per XCD by operand locality and ordered longest first;
+//  * workgroup engine (the rest of build_work): per output block, pick the
+//    workgroup tile shape that minimises padded work plus a per-tile
+//    overhead, cut the block into tiles and each tile's k-range (the
+//    concatenated k-steps of the block's pairs, in the reference pair order)
+//    into chunks of at most split_steps steps, then deal the items to the
+//    queues of a persistent grid (one queue per resident workgroup, one tile
+//    shape per queue, longest-processing-time first).
+// The autotuner picks the engine and its parameters per stage.
 #include <algorithm>
 #include <cmath>
 #include <map>
