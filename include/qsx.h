@@ -21,6 +21,8 @@
  *   qsx_lanczos         <- RunEigs Lanczos loop, Source/eigs2SiteQS.cc:98-334
  *   qsx_davidson        <- Davidson loop, Class/@Hamilton1D/private/update_psi_2site.m:223-391
  *                          and RunDavidson, Source/david2SiteQS.cc:316-535
+ *   qsx_permute         <- permuteQS, Source/permuteQS.cc:25-87
+ *   qsx_plus            <- plusQS, Source/plusQS.cc:17-87
  *   qsx_su2_cg / qsx_su2_x3
  *                       <- SU(2) Clebsch-Gordan data and X-symbols of
  *                          X3Map::contractCGR, Source/clebsch_aux.cc:4073-4510
@@ -136,6 +138,22 @@ int qsx_contract(const qsx_tensor* A, const int32_t* ia, int32_t na, int32_t con
                  const int32_t* perm, int32_t nperm, const qsx_opts* opts,
                  qsx_tensor_out** C, char* err, size_t errlen);
 void qsx_tensor_free(qsx_tensor_out* t);
+
+/* whole-QSpace block operations (device kernels over all blocks) -------- */
+/* permuteQS(A, perm [,'conj']) <- Source/permuteQS.cc:25-87 (QSpace::permute,
+   Source/QSpace.hh:2721-2780, then NormCGW, Source/QSpace.cc:3727-3805).
+   perm is 0-based and completed with the identity when shorter than the
+   rank; conj != 0 flips the conj flag of every leg.  An identity
+   permutation without conj returns A unchanged.  B->gpu_ms = device time. */
+int qsx_permute(const qsx_tensor* A, const int32_t* perm, int32_t nperm, int32_t conj,
+                qsx_tensor_out** B, char* err, size_t errlen);
+/* plusQS(A, B [,bfac]) <- Source/plusQS.cc:17-87: C = A + bfac*B
+   (QSpace::plus_plain, Source/QSpace.cc:3593-3705, for abelian labels; Cat +
+   MakeUnique, Source/QSpace.cc:3075-3140,3807-3905, with SU(2) labels), then
+   SkipZeroData and NormCGW (opts->skip_zero = 0 keeps zero blocks; opts
+   may be NULL).  C->gpu_ms = device time of the sum. */
+int qsx_plus(const qsx_tensor* A, const qsx_tensor* B, double bfac, const qsx_opts* opts,
+             qsx_tensor_out** C, char* err, size_t errlen);
 
 /* parse a ctr-idx string ("13", "1,3", "[1 3]*") into 0-based indices;
    returns the number of indices (>0) or a negative error */

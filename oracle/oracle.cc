@@ -774,6 +774,38 @@ struct Result {
   double flops = 0;
 };
 
+// +-1 relating the CGT of a permuted block (canonical CGT of the new leg
+// order) to the transposed CGT of the original block, from the dense CGTs:
+// sum_m C_new(m_new) C_old(m_old) with m_new[k] = m_old[perm[k]]
+static double perm_sign(const std::vector<int>& tj, const std::vector<int>& dir, const std::vector<int>& perm) {
+  const int r = (int)tj.size();
+  if (r < 2) return 1.0;
+  std::vector<int> tj2(r), dir2(r);
+  for (int k = 0; k < r; ++k) {
+    tj2[k] = tj[perm[k]];
+    dir2[k] = dir[perm[k]];
+  }
+  const std::vector<double> A = block_cgt(tj, dir), B = block_cgt(tj2, dir2);
+  std::vector<int64_t> st(r);
+  int64_t n = 1;
+  for (int k = 0; k < r; ++k) {
+    st[k] = n;
+    n *= tj[k] + 1;
+  }
+  double s = 0;
+  for (int64_t lin = 0; lin < n; ++lin) {
+    int64_t rem = lin, off = 0;
+    for (int k = 0; k < r; ++k) {
+      const int64_t i = rem % (tj2[k] + 1);
+      rem /= tj2[k] + 1;
+      off += i * st[perm[k]];
+    }
+    s += B[lin] * A[off];
+  }
+  if (std::fabs(std::fabs(s) - 1.0) > 1e-9) fail(QSX_ERR_CG, "permuted CGT overlap %g", s);
+  return s < 0 ? -1.0 : 1.0;
+}
+
 static void permute_block(const std::vector<double>& src, const std::vector<int64_t>& d, const std::vector<int>& perm,
                           std::vector<double>& dst) {
   int r = (int)d.size();
@@ -907,9 +939,24 @@ static Result contract(const qsx_tensor* cA, const int32_t* ia, int na, int conj
   }
   bool idperm = true;
   for (int i = 0; i < rc; ++i) idperm &= cperm[i] == i;
+  // directions before the permutation, and per kept block and symmetry the
+  // sign of the CGT of the new leg order relative to the transposed one
+  // (the data take it; the weights stay normalised)
+  std::vector<int> dir0(rc, 1);
+  for (int i = 0; i < rc && !C.itags.empty(); ++i) dir0[i] = tag_out(C.itags[i]) ? -1 : 1;
+  std::vector<double> psign;
   for (int c = 0; c < nC; ++c) {
     if (!keep[c]) continue;
     std::vector<int32_t> rec(P.QC.begin() + (size_t)c * rc * q, P.QC.begin() + (size_t)(c + 1) * rc * q);
+    for (int j = 0; j < C.nsym; ++j) {
+      double sg = 1.0;
+      if (!idperm && C.symtype[j] == SYM_SU2) {
+        std::vector<int> tj(rc);
+        for (int k2 = 0; k2 < rc; ++k2) tj[k2] = rec[k2 * q + j];
+        sg = perm_sign(tj, dir0, cperm);
+      }
+      psign.push_back(sg);
+    }
     std::vector<int64_t> d = bdims[c];
     std::vector<double> data = std::move(blocks[c]);
     if (!idperm) {  // QSpace::Permute (QSpace.hh:2669-2719): Q columns, data, itags
@@ -945,6 +992,7 @@ static Result contract(const qsx_tensor* cA, const int32_t* ia, int na, int conj
       double ne = norm_ext(su2_labels(C, i, j), dirC);
       C.cgw[(size_t)i * C.nsym + j] = ne;
       cfac /= ne;
+      cfac *= psign[(size_t)i * C.nsym + j];
     }
     if (cfac != 1)
       for (double& v : C.data[i]) v *= cfac;

@@ -11,6 +11,12 @@ Parity status of the oracle's functions (what pins each one):
   * SU(2) CGT normalisation/sign      -- explicit restated rule checks
   * X-symbols (x3)                    -- full M-resolved space contraction
   * SkipZeroData / Permute / NormCGW  -- closed-form checks (norm, permutation)
+  * permute (permuteQS)               -- full M-resolved space: Full(permute(A))
+                                         = transpose(Full(A)); conj flag vs the
+                                         contraction's conj flag
+  * plus (plusQS)                     -- full M-resolved space: Full(C) = Full(A)
+                                         + bfac Full(B); block order and zero
+                                         handling by hand-built cases
   * reference's FP64 summation order / MKL dgemm rounding: parity unpinned
     (no reference build; tolerance 1e-12 relative instead of bit equality)
 """
@@ -187,6 +193,219 @@ def x3(tja, dira, tjb, dirb, ia, ib) -> float:
     if rc:
         raise OracleError(rc, err.value.decode())
     return out.value
+
+
+# ---------------------------------------------------------------- block ops
+# numpy restatements of permuteQS and plusQS (see the header's status list).
+
+def _syms(T: QSpace):
+    if not T.qtype:
+        return ["A"] * T.qdim
+    return [x.strip() for x in T.qtype.split(",")]
+
+
+def _tag_dirs(tags, r):
+    return [(-1 if t.endswith("*") else 1) for t in tags] if tags else [1] * r
+
+
+def _block_labels(T: QSpace, i, j):
+    return [int(T.Q[k][i, j]) for k in range(T.rank)]
+
+
+def _weights(T: QSpace, i, dirs):
+    """(NormCGW data factor, output weights) of block i: cgw / normExt per
+    SU(2) symmetry (Source/QSpace.cc:3727-3805, CRef::normExt clebsch.cc:4738-4760)."""
+    f, w_out = 1.0, []
+    cgw = T.info.get("cgw")
+    for j, s in enumerate(_syms(T)):
+        if s != "SU2":
+            w_out.append(1.0)
+            continue
+        ne = norm_ext(_block_labels(T, i, j), dirs)
+        w = float(cgw[i, j]) if cgw is not None else ne
+        f *= w / ne
+        w_out.append(ne)
+    return f, w_out
+
+
+def _as_is(T: QSpace, fac=1.0) -> QSpace:
+    dirs = _tag_dirs(T.itags, T.rank)
+    cg = []
+    for i in range(T.nblk):
+        w = []
+        for j, s in enumerate(_syms(T)):
+            if s != "SU2":
+                w.append(1.0 if T.info.get("cgw") is None else float(T.info["cgw"][i, j]))
+            elif T.info.get("cgw") is not None:
+                w.append(float(T.info["cgw"][i, j]))
+            else:
+                w.append(norm_ext(_block_labels(T, i, j), dirs))
+        cg.append(w)
+    data = [d.copy(order="F") if fac == 1.0 else fac * d for d in T.data]
+    return QSpace([q.copy() for q in T.Q], data, T.qtype, list(T.itags),
+                  np.array(cg).reshape(T.nblk, -1) if T.nblk else None)
+
+
+def permute(A: QSpace, perm, conj=0) -> QSpace:
+    """permuteQS(A, perm [,'conj']) -- Source/permuteQS.cc:25-48: a short
+    permutation is completed with the identity (wbperm::Extend), a long one
+    is an error; identity without conj returns A; else QSpace::permute
+    (Source/QSpace.hh:2721-2780: labels, dims, itags and every block's data
+    permuted; conj flips each leg's conj flag) and NormCGW
+    (Source/QSpace.cc:3727-3805)."""
+    r = A.rank
+    perm = [int(x) for x in perm]
+    if len(perm) > r:
+        raise OracleError(1, f"invalid permutation (len={len(perm)}/{r})")
+    p = perm + [i for i in range(r) if i not in perm]
+    if sorted(p) != list(range(r)):
+        raise OracleError(1, "invalid permutation")
+    if r == 0 or (p == list(range(r)) and not conj):
+        return _as_is(A)
+    tags = [A.itags[k] for k in p] if A.itags else []
+    if conj:
+        tags = [t[:-1] if t.endswith("*") else t + "*" for t in tags]
+    din, dout = _tag_dirs(A.itags, r), _tag_dirs(tags, r)
+    Q = [A.Q[k].copy() for k in p]
+    dconj = [-x for x in din] if conj else din
+    data, cg = [], []
+    for i, d in enumerate(A.data):
+        f, _ = _weights(A, i, din)
+        w = []
+        for j, s in enumerate(_syms(A)):
+            if s != "SU2":
+                w.append(1.0)
+                continue
+            # the block's CGT is the canonical CGT of the new leg order: the
+            # transposed CGT times the overlap sign of the two, which the
+            # data take (no CGT permutation in the tensor format)
+            tj0 = _block_labels(A, i, j)
+            c_old = block_cgt(tj0, dconj)
+            c_new = block_cgt([tj0[k] for k in p], [dconj[k] for k in p])
+            f *= 1.0 if np.sum(c_new * np.transpose(c_old, p)) > 0 else -1.0
+            w.append(norm_ext([int(Q[k][i, j]) for k in range(r)], dout))
+        cg.append(w)
+        x = np.transpose(np.asarray(d).reshape(d.shape[:r], order="F"), p)
+        data.append(np.asfortranarray(x * f))
+    return QSpace(Q, data, A.qtype, tags, np.array(cg).reshape(A.nblk, -1) if A.nblk else None)
+
+
+def _sorted_groups(R):
+    """stable ascending lexicographic order of records (column 0 most
+    significant, ties by index: Wb::hpsort / groupRecs, Source/wbsort.cc:88-143,
+    Source/wbMatrix.cc:1682-1731) and the runs of equal records"""
+    order = sorted(range(len(R)), key=lambda i: (tuple(R[i]), i))
+    groups, cur = [], []
+    for i in order:
+        if cur and tuple(R[i]) != tuple(R[cur[0]]):
+            groups.append(cur)
+            cur = []
+        cur.append(i)
+    if cur:
+        groups.append(cur)
+    return groups
+
+
+def _tags_equal(a, b):
+    """iTags == with itag_::sameAs(.,'l') (Source/wbindex.cc:989-1007)"""
+    if len(a) != len(b):
+        return False
+    for x, y in zip(a, b):
+        if x.endswith("*") != y.endswith("*"):
+            return False
+        nx, ny = x.rstrip("*"), y.rstrip("*")
+        if nx != ny and nx and ny:
+            return False
+    return True
+
+
+def plus(A: QSpace, B: QSpace, bfac=1.0, skip_zero=1) -> QSpace:
+    """plusQS(A, B [,bfac]) -- Source/plusQS.cc:17-55: C = A + bfac B.
+    Abelian: QSpace::plus_plain (Source/QSpace.cc:3593-3705) over the sorted
+    union of records (getQOverlapU, QSpace.cc:1266-1320).  SU(2) labels: Cat
+    + MakeUnique (QSpace.cc:3075-3140,3807-3905): no common record -> A's
+    blocks then B's; otherwise sorted unique records, zero blocks of a group
+    not added.  Then SkipZeroData (QSpace.cc:3200-3270) and NormCGW."""
+    if A.nblk == 0 and B.nblk == 0:
+        return _as_is(A)
+    both = A.nblk and B.nblk
+    ref = A if A.nblk else B
+    if both and (A.qdim != B.qdim or A.rank != B.rank):
+        raise OracleError(3, "QSpace mismatch")
+    if both and _syms(A) != _syms(B) and A.qtype and B.qtype:
+        raise OracleError(3, "got inconsistent qtype")
+    syms = _syms(ref)
+    cg = "SU2" in syms
+    r = ref.rank
+    tags = list(B.itags) if (not A.itags or not A.nblk) else list(A.itags)
+    if both and A.itags and B.itags:
+        if cg:
+            if not _tags_equal(A.itags, B.itags):
+                raise OracleError(2, "itag inconsistency")
+        else:
+            if any(x.endswith("*") != y.endswith("*") for x, y in zip(A.itags, B.itags)):
+                raise OracleError(2, "got inconsistent itags")
+            if not _tags_equal(A.itags, B.itags):
+                tags = ["*" if t.endswith("*") else "" for t in tags]
+    RA = [tuple(A.QIDX[i]) for i in range(A.nblk)]
+    RB = [tuple(B.QIDX[i]) for i in range(B.nblk)]
+    groups = _sorted_groups(RA + RB)
+    ga, gb, overlap = [], [], False
+    for g in groups:
+        if len(g) > 2:
+            raise OracleError(4, "got non-unique QIDX")
+        a = [i for i in g if i < A.nblk]
+        b = [i - A.nblk for i in g if i >= A.nblk]
+        if len(a) > 1 or len(b) > 1:
+            raise OracleError(4, "non-unique QIDX")
+        ga.append(a[0] if a else -1)
+        gb.append(b[0] if b else -1)
+        if a and b:
+            overlap = True
+            if A.data[a[0]].shape[:r] != B.data[b[0]].shape[:r]:
+                raise OracleError(4, "block size mismatch")
+    if cg and not overlap:
+        oa = list(range(A.nblk)) + [-1] * B.nblk
+        ob = [-1] * A.nblk + list(range(B.nblk))
+    else:
+        oa, ob = ga, gb
+    dA, dB, dC = _tag_dirs(A.itags, r), _tag_dirs(B.itags, r), _tag_dirs(tags, r)
+    Q, data, cgw = [[] for _ in range(r)], [], []
+    for a, b in zip(oa, ob):
+        if cg and a >= 0 and b >= 0 and overlap:
+            za, zb = not np.any(A.data[a] != 0), not np.any(B.data[b] != 0)
+            if za and not zb:
+                a = -1
+            elif zb:
+                b = -1
+        fa = (_weights(A, a, dA)[0] if cg else 1.0) if a >= 0 else 0.0
+        fb = (bfac * _weights(B, b, dB)[0] if cg else bfac) if b >= 0 else 0.0
+        if a >= 0 and b >= 0:
+            x = fa * A.data[a]
+            y = fb * B.data[b]
+            d = x + y
+        elif a >= 0:
+            d = fa * A.data[a]
+        else:
+            d = fb * B.data[b]
+        S, sb = (A, a) if a >= 0 else (B, b)
+        for k in range(r):
+            Q[k].append(S.Q[k][sb])
+        data.append(np.asfortranarray(d))
+        w = []
+        for j, s in enumerate(syms):
+            w.append(norm_ext(_block_labels(S, sb, j), dC) if s == "SU2" else 1.0)
+        cgw.append(w)
+    # SkipZeroData (eps = 0): exact-zero blocks dropped unless a rank-2
+    # block-diagonal matrix
+    q = ref.qdim
+    blockdiag = r == 2 and all(np.array_equal(Q[0][i], Q[1][i]) for i in range(len(data)))
+    keep = list(range(len(data)))
+    if skip_zero and data and not (r == 2 and blockdiag):
+        keep = [i for i in keep if np.any(data[i] != 0)]
+    Qk = [np.array([Q[k][i] for i in keep], dtype=np.int32).reshape(len(keep), q) for k in range(r)]
+    return QSpace(Qk, [data[i] for i in keep], A.qtype or B.qtype, tags,
+                  np.array([cgw[i] for i in keep]).reshape(len(keep), -1) if keep else None)
 
 
 def find_openblas():

@@ -51,6 +51,8 @@ def lib():
         "qsx_contract": ([P(QsxTensor), P(i32), i32, i32, P(QsxTensor), P(i32), i32, i32, P(i32), i32, P(QsxOpts),
                           P(P(QsxTensorOut)), cs, sz], C.c_int),
         "qsx_tensor_free": ([P(QsxTensorOut)], None),
+        "qsx_permute": ([P(QsxTensor), P(i32), i32, i32, P(P(QsxTensorOut)), cs, sz], C.c_int),
+        "qsx_plus": ([P(QsxTensor), P(QsxTensor), f64, P(QsxOpts), P(P(QsxTensorOut)), cs, sz], C.c_int),
         "qsx_parse_ctridx": ([cs, P(i32), i32, P(i32)], C.c_int),
         "qsx_packed_offsets": ([P(QsxTensor), P(i64)], C.c_int),
         "qsx_ctr_plan_create": ([P(QsxTensor), P(i32), i32, i32, P(QsxTensor), P(i32), i32, i32, P(i32), i32,
@@ -91,6 +93,7 @@ def exported_symbols() -> list[str]:
     """names declared in include/qsx.h that the loaded library must export."""
     return ["qsx_version", "qsx_device_init", "qsx_malloc_device", "qsx_free_device", "qsx_memcpy_h2d",
             "qsx_memcpy_d2h", "qsx_memset_device", "qsx_synchronize", "qsx_copy_device", "qsx_contract", "qsx_tensor_free",
+            "qsx_permute", "qsx_plus",
             "qsx_parse_ctridx", "qsx_packed_offsets", "qsx_ctr_plan_create", "qsx_ctr_plan_output",
             "qsx_ctr_plan_pairs", "qsx_ctr_execute", "qsx_ctr_plan_flops", "qsx_ctr_plan_set_policy", "qsx_ctr_plan_destroy",
             "qsx_hpsi_plan_create", "qsx_hpsi_size", "qsx_hpsi_flops", "qsx_hpsi_bytes", "qsx_hpsi_apply",

@@ -16,7 +16,7 @@ import numpy as np
 from . import _capi
 from .qspace import QSpace, QsxOpts, QsxTensorOut, from_out, parse_ctridx, to_c
 
-__all__ = ["contractQS", "DeviceBuffer", "CtrPlan", "HPsi", "su2_cg", "su2_x3", "make_opts"]
+__all__ = ["contractQS", "permuteQS", "plusQS", "DeviceBuffer", "CtrPlan", "HPsi", "su2_cg", "su2_x3", "make_opts"]
 
 
 def make_opts(mode="batched", nslices=0, skip_zero=1, hconj=0, fermA=None, fermB=None, verbose=1):
@@ -162,10 +162,42 @@ def _fold(items, mode, kw):
     return [cur, 0]
 
 
-def permuteQS(A: QSpace, perm) -> QSpace:
-    p = list(perm) + [i for i in range(A.rank) if i not in perm]
-    data = [np.asfortranarray(np.transpose(d, p)) for d in A.data]
-    return QSpace([A.Q[i] for i in p], data, A.qtype, [A.itags[i] for i in p] if A.itags else [], A.info["cgw"])
+def _out_to_qspace(out) -> QSpace:
+    try:
+        T = from_out(out.contents)
+        T.gpu_ms = float(out.contents.gpu_ms)
+    finally:
+        _capi.lib().qsx_tensor_free(out)
+    return T
+
+
+def permuteQS(A: QSpace, perm, conj: bool = False) -> QSpace:
+    """permuteQS(A, perm [,'conj']) (Source/permuteQS.cc:25-87) on the device.
+
+    ``perm`` is 0-based and may be shorter than the rank (completed with the
+    identity); ``conj`` flips the conj flag of every leg."""
+    L = _capi.lib()
+    ca, ka = to_c(A)
+    p = _ivec(list(perm))
+    out = C.POINTER(QsxTensorOut)()
+    err = C.create_string_buffer(1024)
+    rc = L.qsx_permute(C.byref(ca), p.ctypes.data_as(C.POINTER(C.c_int32)), len(list(perm)), int(bool(conj)),
+                       C.byref(out), err, 1024)
+    _capi.check(rc, err)
+    return _out_to_qspace(out)
+
+
+def plusQS(A: QSpace, B: QSpace, bfac: float = 1.0, skip_zero: int = 1) -> QSpace:
+    """plusQS(A, B [,bfac]) (Source/plusQS.cc:17-87): A + bfac*B on the device."""
+    L = _capi.lib()
+    ca, ka = to_c(A)
+    cb, kb = to_c(B)
+    opts, keep = make_opts(skip_zero=skip_zero)
+    out = C.POINTER(QsxTensorOut)()
+    err = C.create_string_buffer(1024)
+    rc = L.qsx_plus(C.byref(ca), C.byref(cb), float(bfac), C.byref(opts), C.byref(out), err, 1024)
+    _capi.check(rc, err)
+    return _out_to_qspace(out)
 
 
 class DeviceBuffer:

@@ -154,9 +154,6 @@ CtrPlan::~CtrPlan() {
   dev_free(dAperm);
   dev_free(dBperm);
   dev_free(dCtmp);
-  dev_free(dpbA);
-  dev_free(dpbB);
-  dev_free(dpbC);
 }
 
 // ---------------------------------------------------------------- itags
@@ -493,6 +490,10 @@ static void build_structure(CtrPlan& P, const int32_t* perm, int nperm, const do
       double ne = su2_norm_ext(tj, P.dirC);
       cgw[j] = ne;
       cfac /= ne;
+      // after the user permutation the block's CGT is the canonical CGT of
+      // the new leg order: the data take its sign relative to the
+      // transposed one
+      if (P.has_perm) cfac *= su2_perm_sign(tj, P.dirC, P.perm);
     }
     for (int64_t p = p0; p < p1; ++p) {
       if (P.afac[p] == 0) continue;
@@ -544,16 +545,6 @@ static void build_structure(CtrPlan& P, const int32_t* perm, int nperm, const do
   }
 }
 
-static std::vector<PermBlock> perm_blocks(const TensorDesc& T) {
-  std::vector<PermBlock> v(T.n);
-  for (int i = 0; i < T.n; ++i) {
-    v[i].src_off = T.offs[i];
-    v[i].dst_off = T.offs[i];
-    for (int k = 0; k < T.r; ++k) v[i].dims[k] = (int32_t)T.dims[(size_t)i * T.r + k];
-  }
-  return v;
-}
-
 void plan_upload(CtrPlan& P) {
   hipStream_t s = default_stream();
   dev_free(P.dpairs);
@@ -577,38 +568,27 @@ void plan_upload(CtrPlan& P) {
   P.dscratch = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, P.work.scratch_elems));
   if (!P.work.items.empty())
     QSX_HIP(hipMemcpyAsync(P.ditems, P.work.items.data(), sizeof(WorkItem) * P.work.items.size(), hipMemcpyHostToDevice, s));
-  auto upload_perm = [&](const TensorDesc& T, PermBlock*& d, int64_t& mx) {
-    std::vector<PermBlock> v = perm_blocks(T);
-    mx = 0;
-    for (int i = 0; i < T.n; ++i) mx = std::max(mx, T.numel(i));
-    dev_free(d);
-    d = (PermBlock*)dev_alloc(sizeof(PermBlock) * std::max<size_t>(1, v.size()));
-    if (!v.empty()) QSX_HIP(hipMemcpyAsync(d, v.data(), sizeof(PermBlock) * v.size(), hipMemcpyHostToDevice, s));
-  };
   if (P.la.permute && !P.a_prepermuted) {
-    upload_perm(P.A, P.dpbA, P.maxA);
+    perm_plan_build(P.ppA, P.A.n, P.A.r, P.A.dims.data(), P.A.offs.data(), P.A.offs.data(), P.la.perm.data(),
+                    nullptr);
+    perm_plan_upload(P.ppA, s);
     dev_free(P.dAperm);
     P.dAperm = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, P.A.total()));
   }
   if (P.lb.permute && !P.b_prepermuted) {
-    upload_perm(P.B, P.dpbB, P.maxB);
+    perm_plan_build(P.ppB, P.B.n, P.B.r, P.B.dims.data(), P.B.offs.data(), P.B.offs.data(), P.lb.perm.data(),
+                    nullptr);
+    perm_plan_upload(P.ppB, s);
     dev_free(P.dBperm);
     P.dBperm = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, P.B.total()));
   }
   if (P.has_perm) {
-    // unpermuted output blocks have dims [kept A | kept B]
-    std::vector<PermBlock> v(P.nC);
-    P.maxC = 0;
-    for (int c = 0; c < P.nC; ++c) {
-      v[c].src_off = P.offsOut[c];
-      v[c].dst_off = P.offsOut[c];
-      // recover unpermuted dims: d[perm[l]] = dimsOut[l]
-      for (int l = 0; l < P.rc; ++l) v[c].dims[P.perm[l]] = (int32_t)P.dimsOut[(size_t)c * P.rc + l];
-      P.maxC = std::max(P.maxC, P.offsOut[c + 1] - P.offsOut[c]);
-    }
-    dev_free(P.dpbC);
-    P.dpbC = (PermBlock*)dev_alloc(sizeof(PermBlock) * std::max<size_t>(1, v.size()));
-    if (!v.empty()) QSX_HIP(hipMemcpyAsync(P.dpbC, v.data(), sizeof(PermBlock) * v.size(), hipMemcpyHostToDevice, s));
+    // unpermuted output blocks have dims [kept A | kept B]: d[perm[l]] = dimsOut[l]
+    std::vector<int64_t> d((size_t)P.nC * P.rc);
+    for (int c = 0; c < P.nC; ++c)
+      for (int l = 0; l < P.rc; ++l) d[(size_t)c * P.rc + P.perm[l]] = P.dimsOut[(size_t)c * P.rc + l];
+    perm_plan_build(P.ppC, P.nC, P.rc, d.data(), P.offsOut.data(), P.offsOut.data(), P.perm.data(), nullptr);
+    perm_plan_upload(P.ppC, s);
     dev_free(P.dCtmp);
     P.dCtmp = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, P.offsOut.back()));
   }
@@ -619,14 +599,11 @@ void plan_permute_operand(const CtrPlan& P, int which, const double* src, double
   const TensorDesc& T = which == 0 ? P.A : P.B;
   const OperandLayout& L = which == 0 ? P.la : P.lb;
   if (!L.permute) fail(QSX_ERR_ARG, "operand needs no permute");
-  std::vector<PermBlock> v = perm_blocks(T);
-  int64_t mx = 0;
-  for (int i = 0; i < T.n; ++i) mx = std::max(mx, T.numel(i));
-  PermBlock* d = (PermBlock*)dev_alloc(sizeof(PermBlock) * std::max<size_t>(1, v.size()));
-  if (!v.empty()) QSX_HIP(hipMemcpyAsync(d, v.data(), sizeof(PermBlock) * v.size(), hipMemcpyHostToDevice, s));
-  launch_permute_blocks(d, T.n, T.r, L.perm.data(), src, dst, mx, s);
+  PermPlan pp;
+  perm_plan_build(pp, T.n, T.r, T.dims.data(), T.offs.data(), T.offs.data(), L.perm.data(), nullptr);
+  perm_plan_upload(pp, s);
+  launch_perm_plan(pp, src, dst, s);
   QSX_HIP(hipStreamSynchronize(s));
-  dev_free(d);
 }
 
 std::unique_ptr<CtrPlan> plan_create(const qsx_tensor* A, const int32_t* ia, int na, int conjA, const qsx_tensor* B,
@@ -660,11 +637,11 @@ void plan_execute(CtrPlan& P, const double* dA, const double* dB, double* dC, hi
   const double* a = dA;
   const double* b = dB;
   if (P.la.permute && !P.a_prepermuted) {
-    launch_permute_blocks(P.dpbA, P.A.n, P.A.r, P.la.perm.data(), dA, P.dAperm, P.maxA, s);
+    launch_perm_plan(P.ppA, dA, P.dAperm, s);
     a = P.dAperm;
   }
   if (P.lb.permute && !P.b_prepermuted) {
-    launch_permute_blocks(P.dpbB, P.B.n, P.B.r, P.lb.perm.data(), dB, P.dBperm, P.maxB, s);
+    launch_perm_plan(P.ppB, dB, P.dBperm, s);
     b = P.dBperm;
   }
   double* c = P.has_perm ? P.dCtmp : dC;
@@ -680,7 +657,7 @@ void plan_execute(CtrPlan& P, const double* dA, const double* dB, double* dC, hi
   } else if (P.mode == QSX_MODE_BATCHED) {
     launch_worklist(P.work, P.dqueues, P.ditems, P.dreds, P.dpairs, a, b, c, P.dscratch, s);
   }
-  if (P.has_perm) launch_permute_blocks(P.dpbC, P.nC, P.rc, P.perm.data(), P.dCtmp, dC, P.maxC, s);
+  if (P.has_perm) launch_perm_plan(P.ppC, P.dCtmp, dC, s);
 }
 
 // ---------------------------------------------------------------- one-shot

@@ -56,10 +56,7 @@ struct CtrPlan {
   double* dAperm = nullptr;
   double* dBperm = nullptr;
   double* dCtmp = nullptr;
-  PermBlock* dpbA = nullptr;
-  PermBlock* dpbB = nullptr;
-  PermBlock* dpbC = nullptr;
-  int64_t maxA = 0, maxB = 0, maxC = 0;
+  PermPlan ppA, ppB, ppC;            // operand copies and the output permutation
   ~CtrPlan();
 };
 

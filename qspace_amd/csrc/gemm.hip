@@ -133,34 +133,6 @@ __global__ __launch_bounds__(256) void k_pair_gemm_f64(PairDesc pd, int64_t c_of
   tile_store<TM, TN>(acc, C + c_off, M, N, m0, n0, beta);
 }
 
-// ---- permute ---------------------------------------------------------------
-struct PermArgs {
-  int rank;
-  int perm[QSX_MAX_RANK];
-};
-
-__global__ void k_permute_blocks(const PermBlock* __restrict__ blocks, PermArgs pa, const double* __restrict__ src,
-                                 double* __restrict__ dst) {
-  const PermBlock pb = blocks[blockIdx.y];
-  int64_t n = 1;
-  int64_t sst[QSX_MAX_RANK];
-  int nd[QSX_MAX_RANK];
-  for (int k = 0; k < pa.rank; ++k) {
-    sst[k] = n;
-    n *= pb.dims[k];
-  }
-  for (int k = 0; k < pa.rank; ++k) nd[k] = pb.dims[pa.perm[k]];
-  for (int64_t lin = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; lin < n; lin += (int64_t)gridDim.x * blockDim.x) {
-    int64_t rem = lin, off = 0;
-    for (int k = 0; k < pa.rank; ++k) {
-      int64_t i = rem % nd[k];
-      rem /= nd[k];
-      off += i * sst[pa.perm[k]];
-    }
-    dst[pb.dst_off + lin] = src[pb.src_off + off];
-  }
-}
-
 __global__ void k_block_zero_flags(const int64_t* __restrict__ offs, const double* __restrict__ data,
                                    int32_t* __restrict__ flags) {
   __shared__ int nz;
@@ -189,17 +161,6 @@ void launch_pair_gemm_f64(const PairDesc& p, int64_t c_off, int M, int N, int be
   if (M <= 0 || N <= 0) return;
   dim3 g((M + 63) / 64, (N + 63) / 64);
   hipLaunchKernelGGL((k_pair_gemm_f64<64, 64>), g, dim3(256), 0, s, p, c_off, M, N, beta, A, B, C);
-  QSX_HIP(hipGetLastError());
-}
-
-void launch_permute_blocks(const PermBlock* dblocks, int nblk, int rank, const int* perm, const double* src,
-                           double* dst, int64_t max_numel, hipStream_t s) {
-  if (nblk <= 0 || max_numel <= 0) return;
-  PermArgs pa{};
-  pa.rank = rank;
-  for (int k = 0; k < rank; ++k) pa.perm[k] = perm[k];
-  int gx = (int)std::min<int64_t>(1024, (max_numel + 255) / 256);
-  hipLaunchKernelGGL(k_permute_blocks, dim3(gx, nblk), dim3(256), 0, s, dblocks, pa, src, dst);
   QSX_HIP(hipGetLastError());
 }
 

@@ -269,6 +269,13 @@ __global__ void k_group_emit(const uint32_t* head, const uint32_t* hoff, const u
   for (int c = 0; c < mc; ++c) qout[(size_t)g * mc + c] = s[c];
 }
 
+// head flag of each run of equal keys in a sorted key array
+__global__ void k_key_heads(const uint64_t* kh, const uint64_t* kl, int n, uint32_t* head) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  head[i] = (i == 0 || kh[i] != kh[i - 1] || kl[i] != kl[i - 1]) ? 1u : 0u;
+}
+
 // ---- host helpers ----------------------------------------------------------
 
 struct DevBuf {
@@ -385,6 +392,41 @@ void pack_and_sort(const int32_t* drec, int n, int m, const PackSpec& spec, int 
 }
 
 }  // namespace
+
+void device_group_records(const std::vector<int32_t>& R, int n, int m, std::vector<int32_t>& order,
+                          std::vector<int64_t>& seg, hipStream_t s) {
+  order.resize(n);
+  seg.clear();
+  if (n == 0) {
+    seg.push_back(0);
+    return;
+  }
+  if (m == 0) {  // records of no ints are all equal
+    for (int i = 0; i < n; ++i) order[i] = i;
+    seg = {0, (int64_t)n};
+    return;
+  }
+  int nbits = 0;
+  PackSpec spec = make_spec_safe(R, n, m, &nbits);
+  DevBuf dR(sizeof(int32_t) * R.size());
+  QSX_HIP(hipMemcpyAsync(dR.p, R.data(), sizeof(int32_t) * R.size(), hipMemcpyHostToDevice, s));
+  SortedKeys K(n);
+  pack_and_sort(dR.as<int32_t>(), n, m, spec, nbits, K, s);
+  DevBuf head(sizeof(uint32_t) * n);
+  const int tb = 256;
+  hipLaunchKernelGGL(k_key_heads, dim3((n + tb - 1) / tb), dim3(tb), 0, s, K.h.as<uint64_t>(), K.l.as<uint64_t>(), n,
+                     head.as<uint32_t>());
+  QSX_HIP(hipGetLastError());
+  std::vector<uint32_t> hv(n), pv(n);
+  QSX_HIP(hipMemcpyAsync(hv.data(), head.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
+  QSX_HIP(hipMemcpyAsync(pv.data(), K.v.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
+  QSX_HIP(hipStreamSynchronize(s));
+  for (int i = 0; i < n; ++i) {
+    order[i] = (int32_t)pv[i];
+    if (hv[i]) seg.push_back(i);
+  }
+  seg.push_back(n);
+}
 
 LabelResult device_label_engine(const std::vector<int32_t>& QAc, const std::vector<int32_t>& QAk, int na,
                                 const std::vector<int32_t>& QBc, const std::vector<int32_t>& QBk, int nb, int kc,

@@ -73,6 +73,9 @@ double su2_cg_exact(int tj1, int tm1, int tj2, int tm2, int tj3, int tm3);
 double su2_x3(const std::vector<int>& tja, const std::vector<int>& dira, const std::vector<int>& tjb,
               const std::vector<int>& dirb, const std::vector<int>& ia, const std::vector<int>& ib);
 double su2_norm_ext(const std::vector<int>& tj, const std::vector<int>& dir);
+// +-1: the CGT implied by the legs in order perm (new leg k = old leg
+// perm[k]) relative to the transposed CGT of the block
+double su2_perm_sign(const std::vector<int>& tj, const std::vector<int>& dir, const std::vector<int>& perm);
 
 // ---------------------------------------------------------------- kernels
 // Work descriptors of the grouped GEMM (gemm.hip)
@@ -257,13 +260,56 @@ void launch_split_gemm(int S, const WorkItem* ditems, int nitems, const SplitPai
                        int64_t apl, const int32_t* expA, const uint16_t* Bp, int64_t bpl, const int32_t* expB,
                        double* C, hipStream_t s);
 
-// generic block permute: dst block b = permute(src block b, perm) for all blocks
-struct PermBlock {
+// ---------------------------------------------------------------- block ops
+// Block permutation of a whole QSpace (blockops.hip): per block the fused
+// dims in destination order (nd), the source stride (ss) and destination
+// stride (ds) of each destination dim; kind 0 = contiguous runs (fused
+// perm[0] == 0), kind 1 + di = LDS-tiled transpose of source dim 0 (which
+// is destination dim di) against destination dim 0.
+struct PermDesc {
   int64_t src_off, dst_off;
-  int32_t dims[QSX_MAX_RANK];  // source dims
+  double scale;
+  int32_t rank, kind;
+  int32_t nd[QSX_MAX_RANK];
+  int64_t ss[QSX_MAX_RANK];
+  int64_t ds[QSX_MAX_RANK];
 };
-void launch_permute_blocks(const PermBlock* dblocks, int nblk, int rank, const int* perm, const double* src,
-                           double* dst, int64_t max_numel, hipStream_t s);
+// runs: a = first destination element of the chunk, b = elements;
+// tiles: a = tile row (source dim 0), b = tile column (destination dim 0),
+// c = batch index over the other dims
+struct PermItem {
+  int32_t blk, a, b, c;
+};
+struct PermPlan {
+  std::vector<PermDesc> descs;
+  std::vector<PermItem> items;
+  PermDesc* ddescs = nullptr;
+  PermItem* ditems = nullptr;
+  int64_t elems = 0;  // elements moved per launch
+  PermPlan() = default;
+  PermPlan(const PermPlan&) = delete;
+  PermPlan& operator=(const PermPlan&) = delete;
+  ~PermPlan();
+};
+// dst block b = scale[b] * permute(src block b, perm), perm[k] = source dim
+// of destination dim k; dims n x r (source dims)
+void perm_plan_build(PermPlan& pp, int n, int r, const int64_t* dims, const int64_t* src_offs,
+                     const int64_t* dst_offs, const int* perm, const double* scale);
+void perm_plan_upload(PermPlan& pp, hipStream_t s);
+void launch_perm_plan(const PermPlan& pp, const double* src, double* dst, hipStream_t s);
+// C[c_off + e] = fa * A[a_off + e] + fb * B[b_off + e], e < n (a_off / b_off
+// < 0: that term absent)
+struct AxpbyItem {
+  int64_t c_off, a_off, b_off;
+  double fa, fb;
+  int32_t n, pad;
+};
+void launch_axpby(const AxpbyItem* ditems, int nitems, const double* A, const double* B, double* C, hipStream_t s);
+// stable ascending sort of n records of m ints on the device (ties by index,
+// the order of Wb::hpsort / groupRecs): order[i] = index of the i-th record,
+// seg = starts of the runs of equal records (+ n at the end)
+void device_group_records(const std::vector<int32_t>& R, int n, int m, std::vector<int32_t>& order,
+                          std::vector<int64_t>& seg, hipStream_t s);
 // per-block "all entries exactly zero" flags
 void launch_block_zero_flags(const int64_t* doffs, int nblk, const double* data, int32_t* flags, hipStream_t s);
 void launch_fill_zero(double* p, int64_t n, hipStream_t s);

@@ -531,6 +531,40 @@ double su2_x3(const std::vector<int>& tja, const std::vector<int>& dira, const s
   return x;
 }
 
+// A block's CGT is the canonical CGT of its leg set in the block's leg
+// order; the canonical order breaks label ties by leg position, so after a
+// leg permutation the CGT implied by the new leg order is the transposed
+// CGT up to a sign (-1 when two legs of equal label and direction swap
+// order and the coupling is odd).  Returns that sign: the ratio of the new
+// implied CGT to the transposed old one at one non-zero entry.
+double su2_perm_sign(const std::vector<int>& tj, const std::vector<int>& dir, const std::vector<int>& perm) {
+  const int r = (int)tj.size();
+  if (r < 2) return 1.0;
+  std::vector<int> tj2(r), dir2(r);
+  for (int k = 0; k < r; ++k) {
+    tj2[k] = tj[perm[k]];
+    dir2[k] = dir[perm[k]];
+  }
+  const CanonCGT a = make_canon(tj, dir), b = make_canon(tj2, dir2);
+  int tm[3] = {0, 0, 0}, tm2[3] = {0, 0, 0};
+  const int n0 = tj[0] + 1, n1 = tj[1] + 1, n2 = r > 2 ? tj[2] + 1 : 1;
+  for (int i2 = 0; i2 < n2; ++i2)
+    for (int i1 = 0; i1 < n1; ++i1)
+      for (int i0 = 0; i0 < n0; ++i0) {
+        tm[0] = tj[0] - 2 * i0;
+        tm[1] = tj[1] - 2 * i1;
+        if (r > 2) tm[2] = tj[2] - 2 * i2;
+        const double v = block_entry(a, tm);
+        if (std::fabs(v) < 1e-12) continue;
+        for (int k = 0; k < r; ++k) tm2[k] = tm[perm[k]];
+        const double ratio = block_entry(b, tm2) / v;
+        if (std::fabs(std::fabs(ratio) - 1.0) > 1e-9)
+          fail(QSX_ERR_CG, "permuted SU(2) CGT is not the transposed one up to sign (ratio %g)", ratio);
+        return ratio < 0 ? -1.0 : 1.0;
+      }
+  fail(QSX_ERR_CG, "vanishing SU(2) CGT");
+}
+
 double su2_norm_ext(const std::vector<int>& tj, const std::vector<int>& dir) {
   int r = (int)tj.size();
   if (r == 0 || r > 3) return 1.0;

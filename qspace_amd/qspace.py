@@ -124,7 +124,7 @@ class QSpace:
             raise TypeError("Q must be a list of per-leg label matrices")
         self.Q = [np.ascontiguousarray(np.asarray(q, dtype=np.int32)) for q in Q]
         n = self.Q[0].shape[0] if self.Q else 0
-        self.Q = [q.reshape(n, -1) for q in self.Q]
+        self.Q = [q.reshape(n, -1) if n else q.reshape(0, q.shape[-1] if q.ndim == 2 else 1) for q in self.Q]
         r = len(self.Q)
         self.data = []
         for d in data:
