@@ -299,6 +299,32 @@ def extra_configs():
     return out
 
 
+def block_ops():
+    """permuteQS / plusQS on the device over the SU(2) Heisenberg environment
+    HL (rank 3, all blocks in one launch), at D* = 4000 and 32000; device time
+    of the one launch (events), best of 5.  Algorithmic bytes: permute 16 B
+    per element (read + write), plus 24 B (two reads + write)."""
+    import qspace_amd as Q
+    from qspace_amd import gen
+    rng = np.random.RandomState(11)
+    out = {}
+    for D in (4000, 32000):
+        HL, _, _ = gen.heis_hpsi(seed=1003, Dstar=D)
+        n = HL.numel()
+        row = {"blocks": HL.nblk, "elements": n, "max_block": [int(x) for x in max(d.shape for d in HL.data)]}
+        for perm in ([1, 0, 2], [2, 1, 0]):
+            ms = min(Q.permuteQS(HL, perm).gpu_ms for _ in range(5))
+            gbs = 16.0 * n / (ms * 1e-3) / 1e9
+            row[f"permute_{''.join(str(k + 1) for k in perm)}"] = {"ms": ms, "GB/s": gbs, "hbm_frac": gbs / PEAK_HBM_GBS}
+        B = HL.with_packed(rng.standard_normal(n))
+        ms = min(Q.plusQS(HL, B, 0.5).gpu_ms for _ in range(5))
+        gbs = 24.0 * n / (ms * 1e-3) / 1e9
+        row["plus"] = {"ms": ms, "GB/s": gbs, "hbm_frac": gbs / PEAK_HBM_GBS}
+        out[f"su2_heisenberg_env_Dstar{D}"] = row
+        log(f"[bench] block ops D*={D}: {row}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -400,9 +426,10 @@ def main():
     roof["avg_launch_ms"] = tk
     roof["fp64_mfma_frac"] = fl / (tk * 1e-3) / 1e12 / PEAK_FP64_TFLOPS
     roof["hbm_frac"] = by / (tk * 1e-3) / 1e9 / PEAK_HBM_GBS
-    configs = None
+    configs = bops = None
     if not args.lite:
         configs = extra_configs()
+        bops = block_ops()
     cpu = None
     if not args.no_cpu and not args.lite:
         log("[bench] cpu baseline (sample) ...")
@@ -442,6 +469,7 @@ def main():
         "autotune": {"seconds": tune_s, **{k: list(v) for k, v in tune.items()}},
         "calibration": calib,
         "configs": configs,
+        "block_ops": bops,
     }
     if cpu is not None:
         out["rel_err_vs_fp64_oracle"] = cpu["rel_err_gpu_vs_fp64_oracle_sample"]

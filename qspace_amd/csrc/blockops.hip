@@ -18,10 +18,11 @@
 //    destination-linear chunks; each element's source offset follows from its
 //    run index (destination-order outer dims); writes are unit-stride, and
 //    so are reads when the source's fast dim stays first;
-//  * tiles (fused perm[0] != 0): 32 x 32 tiles of (source dim 0, destination
-//    dim 0) staged through LDS, so the read is unit-stride along the source's
-//    fast dim and the write unit-stride along the destination's; the other
-//    dims are a batch index decoded once per tile.
+//  * tiles (fused perm[0] != 0): 32 x 32 tiles of (source dim 0, the
+//    destination dims in front of it flattened) staged through LDS, so the
+//    read is unit-stride along the source's fast dim and the write
+//    unit-stride in the destination; the dims behind it are a batch index
+//    decoded once per tile.
 // The linear combination is a chunked streaming kernel; its products and sum
 // are rounded separately (no contraction into an FMA), so the result is the
 // same double as a*x + b*y evaluated in that order on the host.
@@ -69,15 +70,17 @@ __global__ __launch_bounds__(NT) void k_perm(const PermDesc* __restrict__ descs,
     }
     return;
   }
-  // tile (source dim 0 = destination dim di) x (destination dim 0)
+  // tile: rows i = source dim 0 (destination dim di), columns j = the
+  // destination dims before di flattened (destination-contiguous); the
+  // dims after di are a batch index
   const int di = d.kind - 1;
-  const int ni = d.nd[di], nj = d.nd[0];
+  const int ni = d.nd[di];
+  const int nj = (int)d.ds[di];
   const int i0 = it.a * TILE, j0 = it.b * TILE;
   int64_t sb = 0, db = 0;
   {
     uint32_t c = (uint32_t)it.c;
-    for (int k = 1; k < rank; ++k) {
-      if (k == di) continue;
+    for (int k = di + 1; k < rank; ++k) {
       const uint32_t n = (uint32_t)d.nd[k];
       const uint32_t q = c / n;
       const int64_t idx = c - q * n;
@@ -86,12 +89,24 @@ __global__ __launch_bounds__(NT) void k_perm(const PermDesc* __restrict__ descs,
       c = q;
     }
   }
-  const int64_t sj = d.ss[0], dsi = d.ds[di];
+  const int64_t dsi = d.ds[di];
   const int tx = threadIdx.x & (TILE - 1), ty = threadIdx.x / TILE;
 #pragma unroll
   for (int r = 0; r < TILE; r += NT / TILE) {
     const int i = i0 + tx, j = j0 + ty + r;
-    if (i < ni && j < nj) t[ty + r][tx] = s[sb + i + j * sj];
+    if (i < ni && j < nj) {
+      // source offset of column j (mixed-radix decode over dims 0..di-1)
+      int64_t sj = 0;
+      uint32_t rem = (uint32_t)j;
+      for (int k = 0; k < di - 1; ++k) {
+        const uint32_t n = (uint32_t)d.nd[k];
+        const uint32_t q = rem / n;
+        sj += (int64_t)(rem - q * n) * d.ss[k];
+        rem = q;
+      }
+      sj += (int64_t)rem * d.ss[di - 1];
+      t[ty + r][tx] = s[sb + i + sj];
+    }
   }
   __syncthreads();
 #pragma unroll

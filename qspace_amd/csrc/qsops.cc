@@ -46,7 +46,7 @@ namespace {
 
 constexpr int kRunChunk = 4096;  // destination elements per run item
 constexpr int kTile = 32;        // tile edge of the transpose kernel (blockops.hip)
-constexpr int kTileMin = 8;      // both tiled dims at least this long, else a gather
+constexpr int kTileMin = 8;      // both tile extents at least this long, else a gather
 
 // fuse one block's dims: unit dims dropped, source-adjacent dims that stay
 // adjacent in the destination merged.  Out: fused dims in destination order,
@@ -121,14 +121,15 @@ void perm_plan_build(PermPlan& pp, int n, int r, const int64_t* dims, const int6
       if (ss[k] == 1) di = k;
     }
     const int bi = (int)pp.descs.size();
-    const bool tiled = di > 0 && nd[0] >= kTileMin && nd[di] >= kTileMin;
+    // tile columns: the destination dims in front of source dim 0, flattened
+    const bool tiled = di > 0 && pd.ds[di] >= kTileMin && nd[di] >= kTileMin;
     if (!tiled) {
       pd.kind = 0;
       for (int64_t e0 = 0; e0 < N; e0 += kRunChunk)
         pp.items.push_back(PermItem{bi, (int32_t)e0, (int32_t)std::min<int64_t>(kRunChunk, N - e0), 0});
     } else {
       pd.kind = 1 + di;
-      const int64_t ni = nd[di], nj = nd[0], nb = N / (ni * nj);
+      const int64_t ni = nd[di], nj = pd.ds[di], nb = N / (ni * nj);
       const int ti = (int)((ni + kTile - 1) / kTile), tj = (int)((nj + kTile - 1) / kTile);
       for (int64_t c = 0; c < nb; ++c)
         for (int a = 0; a < ti; ++a)
