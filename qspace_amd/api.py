@@ -66,23 +66,38 @@ def _ivec(v):
     return a if a.size else np.zeros(1, np.int32)
 
 
-def _itag_match(A: QSpace, B: QSpace, conjA: int, conjB: int):
-    """iTags::getCtr (Source/wbindex.cc:1512-1590): legs with equal tag names
-    and opposite direction (same direction if exactly one side is conj)."""
+def _itag_match(A: QSpace, B: QSpace, conjA: int, conjB: int, skipA=(), skipB=()):
+    """iTags::getCtr (Source/wbindex.cc:1512-1590): every pair of legs with
+    equal tag names (two empty names count as equal) and opposite conj bits
+    (equal bits if exactly one side carries a conj flag) is contracted; a
+    leg matching twice is an error; if nothing matches, the match is retried
+    with the other conj relation and conj(A) is applied (the reference's
+    "check missing conj flag" warning).  Legs in skipA / skipB are flagged
+    (itag_::SetFlags) and match nothing."""
+    if len(A.itags) != A.rank or len(B.itags) != B.rank:
+        raise _capi.QsxError(2, "got empty or invalid set of itags")
     cflag = conjA ^ conjB
     for it in range(2):
-        ia, ib, used = [], [], set()
+        ma, mb, err = {}, {}, 0
         for i, ta in enumerate(A.itags):
+            if i in skipA:
+                continue
             na, oa = ta.rstrip("*"), ta.endswith("*")
             for j, tb in enumerate(B.itags):
+                if j in skipB:
+                    continue
                 nb, ob = tb.rstrip("*"), tb.endswith("*")
-                if na and na == nb and ((oa == ob) == bool(cflag)) and j not in used:
-                    ia.append(i)
-                    ib.append(j)
-                    used.add(j)
-                    break
-        if ia:
-            return ia, ib, (conjA if it == 0 else 1 - conjA)
+                if na == nb and ((oa == ob) == bool(cflag)):
+                    if i in ma or j in mb:
+                        err += 1
+                    else:
+                        ma[i] = j
+                        mb[j] = i
+        if err:
+            raise _capi.QsxError(2, "got non-unique itags")
+        if ma:
+            ia = sorted(ma)
+            return ia, [ma[i] for i in ia], (conjA if it == 0 else 1 - conjA)
         cflag = 1 - cflag
     raise _capi.QsxError(2, "empty set of matching itags")
 
@@ -113,8 +128,12 @@ def _contract_pair(A, ia, conjA, B, ib, conjB, perm=None, mode="batched", **kw) 
 
 
 def contractQS(*args, mode="batched", **kw) -> QSpace:
-    """contractQS(A, ica, B, icb [,perm]) or contractQS([A, B, ...], C [,perm])."""
-    if len(args) >= 4 and isinstance(args[0], QSpace) and isinstance(args[2], QSpace):
+    """contractQS(A, ica, B, icb [,perm]) or the cell form
+    contractQS({A, opts.., B, opts..}, C, opts.. [,perm] [,'-v'|'-q'])."""
+    # usage #1 when arguments 2 and 4 are contraction indices (isCtrIdx,
+    # Source/contractQS.cc:369-373)
+    if len(args) >= 4 and isinstance(args[0], QSpace) and isinstance(args[2], QSpace) and \
+            _is_ctridx(args[1]) and _is_ctridx(args[3]):
         A, ica, B, icb = args[:4]
         perm = None
         if len(args) > 4 and args[4] is not None and not isinstance(args[4], str):
@@ -122,44 +141,140 @@ def contractQS(*args, mode="batched", **kw) -> QSpace:
         ia, ca = parse_ctridx(ica)
         ib, cb = parse_ctridx(icb)
         return _contract_pair(A, ia, ca, B, ib, cb, perm, mode=mode, **kw)
-    # usage #2: itag auto-contraction, operands may carry a '*' flag
-    ops = []
+    # usage #2 (mexFunction, Source/contractQS.cc:364-433): trailing -v/-q,
+    # then a trailing numeric vector that is a valid permutation is the
+    # output permutation (a single index only at argument position < 2; a
+    # later one is the preceding operand's option)
+    args = list(args)
+    while args and isinstance(args[-1], str) and args[-1] in ("-v", "-q"):
+        args.pop()
     perm = None
-    for a in args:
-        if isinstance(a, (list, tuple)) and a and isinstance(a[0], QSpace):
-            ops.append(_fold(list(a), mode, kw))
-        elif isinstance(a, QSpace):
-            ops.append([a, 0])
-        elif isinstance(a, str) and a.strip() == "*":
-            ops[-1][1] ^= 1
-        elif a is not None:
-            perm = [int(x) - 1 for x in np.atleast_1d(a)]
-    if len(ops) < 2:
-        raise ValueError("contractQS: need at least two operands")
-    # right-to-left fold A.(B.(C...)) (contractQS.cc:722-805)
-    cur, cflag = ops[-1]
-    for A, conjA in reversed(ops[:-1]):
-        ia, ib, conjA = _itag_match(A, cur, conjA, cflag)
-        cur = _contract_pair(A, ia, conjA, cur, ib, cflag, None, mode=mode, **kw)
-        cflag = 0
+    if len(args) >= 2 and _is_numvec(args[-1]):
+        v = [int(x) for x in np.atleast_1d(args[-1])]
+        if sorted(v) == list(range(1, len(v) + 1)) and (len(v) != 1 or len(args) - 1 < 2):
+            perm = [x - 1 for x in v]
+            args.pop()
+    C = _cell(args, 0, mode, kw)
     if perm is not None:
-        cur = permuteQS(cur, perm)
-    return cur
+        if len(perm) > C.rank:
+            raise _capi.QsxError(1, f"invalid input permutation (len={len(perm)}/{C.rank})")
+        C = permuteQS(C, perm)
+    return C
 
 
-def _fold(items, mode, kw):
+def _is_ctridx(a) -> bool:
+    if isinstance(a, QSpace) or _is_cell(a):
+        return False
+    try:
+        idx, _ = parse_ctridx(a)
+    except Exception:
+        return False
+    return len(idx) > 0
+
+
+def _is_cell(a) -> bool:
+    return isinstance(a, (list, tuple)) and len(a) > 0 and isinstance(a[0], (QSpace, list, tuple))
+
+
+def _is_numvec(a) -> bool:
+    if isinstance(a, (str, QSpace)) or _is_cell(a):
+        return False
+    try:
+        v = np.atleast_1d(np.asarray(a))
+    except (TypeError, ValueError):
+        return False
+    return v.ndim == 1 and v.size > 0 and np.issubdtype(v.dtype, np.number)
+
+
+def _cell_option(op, a):
+    """icFlags::set (Source/contractQS.cc:159-242): '*' = conj; '[!]##[*]' or
+    a numeric vector = the legs (1-based) that may be contracted ('!': the
+    legs that may not)."""
+    if isinstance(a, str):
+        s = a
+        if s.lstrip("-").lower().startswith("op:") or s.startswith("--itag:s/"):
+            raise _capi.QsxError(5, f"cell-contraction option '{a}' (itag operations) not supported")
+        x = s.startswith("!")
+        if x:
+            s = s[1:]
+        n = 0
+        while n < len(s) and s[n].isdigit():
+            n += 1
+        rest = s[n:]
+        if rest == "*":
+            if op["conj"]:
+                raise _capi.QsxError(1, "conj already set")
+            op["conj"] = 1
+        elif rest or (not n and not rest):
+            raise _capi.QsxError(1, f"invalid cell-contraction option (expecting '[!##][*]': got '{a}')")
+        if n:
+            if op["sidx"] is not None:
+                raise _capi.QsxError(1, f"ctr-idx already set ({a})")
+            op["sidx"] = [int(c) - 1 for c in s[:n]]
+            op["xflag"] = x
+        return
+    if not _is_numvec(a):
+        raise _capi.QsxError(1, f"unexpected cell-contraction input {type(a).__name__}")
+    v = [int(x) for x in np.atleast_1d(a)]
+    if any(x < 1 or x > 9 for x in v):
+        raise _capi.QsxError(1, f"invalid ctr-idx {v}")
+    if op["sidx"] is not None:
+        raise _capi.QsxError(1, "ctr-idx already set")
+    op["sidx"] = [x - 1 for x in v]
+    op["xflag"] = False
+
+
+def _skipped(op, T: QSpace):
+    """legs excluded from the itag match (icFlags::apply,
+    Source/contractQS.cc:244-293): without '!' the legs not listed, with '!'
+    the listed ones"""
+    if op["sidx"] is None:
+        return set()
+    r = T.rank
+    if any(j >= r for j in op["sidx"]):
+        raise _capi.QsxError(1, f"cell-ctr index out of bounds (r={r})")
+    if len(set(op["sidx"])) != len(op["sidx"]):
+        raise _capi.QsxError(1, "invalid cell-ctr index (non-unique)")
+    listed = set(op["sidx"])
+    return {i for i in range(r) if (i in listed) == op["xflag"]}
+
+
+def _cell(items, level, mode, kw) -> QSpace:
+    """contractQS_itags (Source/contractQS.cc:634-808): operands (QSpaces or
+    nested cells, evaluated first) each followed by its options, contracted
+    right to left by itag matching."""
     ops = []
     for a in items:
         if isinstance(a, QSpace):
-            ops.append([a, 0])
-        elif isinstance(a, str) and a.strip() == "*":
-            ops[-1][1] ^= 1
-    cur, cflag = ops[-1]
-    for A, conjA in reversed(ops[:-1]):
-        ia, ib, conjA = _itag_match(A, cur, conjA, cflag)
+            ops.append({"T": a, "conj": 0, "sidx": None, "xflag": False})
+        elif _is_cell(a):
+            if len(a) < 2:
+                raise _capi.QsxError(1, "invalid cell-contraction (got cell with fewer than two entries)")
+            ops.append({"T": _cell(list(a), level + 1, mode, kw), "conj": 0, "sidx": None, "xflag": False})
+        else:
+            if not ops:
+                raise _capi.QsxError(1, "invalid QSpace cell-structure (option before any QSpace)")
+            _cell_option(ops[-1], a)
+    if len(ops) < 2:
+        if len(ops) == 1 and level == 0:
+            op = ops[0]
+            T = op["T"]
+            if op["conj"]:
+                T = permuteQS(T, [], conj=True)
+            if op["sidx"] is not None:
+                T = permuteQS(T, op["sidx"])
+            return T
+        raise _capi.QsxError(1, "invalid usage #2: at least two QSpaces required for each contraction")
+    cur = ops[-1]["T"]
+    cflag = ops[-1]["conj"]
+    skipB = _skipped(ops[-1], cur)
+    for op in reversed(ops[:-1]):
+        A = op["T"]
+        ia, ib, conjA = _itag_match(A, cur, op["conj"], cflag, _skipped(op, A), skipB)
         cur = _contract_pair(A, ia, conjA, cur, ib, cflag, None, mode=mode, **kw)
         cflag = 0
-    return [cur, 0]
+        skipB = set()
+    return cur
 
 
 def _out_to_qspace(out) -> QSpace:

@@ -176,7 +176,7 @@ void match_itags(const TensorDesc& A, const TensorDesc& B, int conjA, int conjB,
       int hit = -1;
       for (int j = 0; j < B.r; ++j) {
         uint64_t a = A.itags[i], b = B.itags[j];
-        if (tag_name(a) != tag_name(b) || !tag_name(a)) continue;
+        if (tag_name(a) != tag_name(b)) continue;  // two empty names count as equal (itag_::sameAs)
         bool same = tag_out(a) == tag_out(b);
         if (same != want_same) continue;
         if (hit >= 0 || mb[j]) ++e;

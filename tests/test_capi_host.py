@@ -133,3 +133,51 @@ def test_no_device_is_a_clean_error():
     err = C.create_string_buffer(256)
     rc = _capi.lib().qsx_device_init(0, err, 256)
     assert rc == 6 and b"device" in err.value
+
+
+# ---- cell-form options and itag matching (host logic, no GPU) -------------
+from qspace_amd import api as _api  # noqa: E402
+
+
+def _mk(tags):
+    return Q.QSpace([np.array([[0]])] * len(tags), [np.ones((1,) * len(tags))], "A", tags)
+
+
+@pytest.mark.parametrize("opt,sidx,xflag,conj", [("*", None, False, 1), ("23", [1, 2], False, 0),
+                                                 ("!2*", [1], True, 1), ("!13", [0, 2], True, 0),
+                                                 ([1, 3], [0, 2], False, 0)])
+def test_cell_options(opt, sidx, xflag, conj):
+    op = {"T": None, "conj": 0, "sidx": None, "xflag": False}
+    _api._cell_option(op, opt)
+    assert (op["sidx"], op["xflag"], op["conj"]) == (sidx, xflag, conj)
+
+
+@pytest.mark.parametrize("bad", ["", "2x", "**", "-op:S", "--itag:s/a/b/", [0], [10]])
+def test_cell_option_errors(bad):
+    op = {"T": None, "conj": 0, "sidx": None, "xflag": False}
+    with pytest.raises(Q.QsxError):
+        _api._cell_option(op, bad)
+
+
+def test_cell_skipped_legs():
+    T = _mk(["L", "s", "R*"])
+    op = {"T": T, "conj": 0, "sidx": [1], "xflag": True}     # '!2': leg 2 may not be contracted
+    assert _api._skipped(op, T) == {1}
+    op = {"T": T, "conj": 0, "sidx": [0, 2], "xflag": False}  # '13': only legs 1, 3
+    assert _api._skipped(op, T) == {1}
+
+
+def test_itag_match_rules():
+    A = _mk(["L", "s", "R*"])
+    B = _mk(["L*", "s*", "R"])
+    assert _api._itag_match(A, B, 0, 0)[:2] == ([0, 1, 2], [0, 1, 2])
+    assert _api._itag_match(A, B, 0, 0, skipA={1})[:2] == ([0, 2], [0, 2])
+    # same directions: no match at first, retried with conj(A) (the
+    # reference's "missing conj flag" rule)
+    ia, ib, cA = _api._itag_match(A, A, 0, 0)
+    assert (ia, ib, cA) == ([0, 1, 2], [0, 1, 2], 1)
+    # a leg that matches twice is an error (iTags::getCtr)
+    with pytest.raises(Q.QsxError):
+        _api._itag_match(_mk(["a", "a"]), _mk(["a*", "b"]), 0, 0)
+    # two empty names count as equal
+    assert _api._itag_match(_mk(["", "x"]), _mk(["*", "y"]), 0, 0)[:2] == ([0], [0])

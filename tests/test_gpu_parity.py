@@ -303,3 +303,26 @@ def test_hpsi_single_launch(gpu, kind):
         xs = psi.with_packed(x)
         ref = _hpsi_oracle(HL, xs, HR, 1, fermA=ferm).packed()
         assert np.linalg.norm(outs[0] - ref) / np.linalg.norm(ref) < TOL
+
+
+def test_cell_form_options(gpu):
+    """cell-form leg options (icFlags, contractQS.cc:159-293): {A,'!2*',A},
+    {A,'13*',A} and {A,[1 3],'*',A} contract legs 1 and 3 like usage #1; the
+    nested {A,'!2*',{L,A}} (the environment-update shape of updateHK.m)
+    equals the explicit chain; all against the oracle."""
+    A = gen.u1_mps(seed=31, Lq=tuple(range(-6, 7, 2)), Rq=tuple(range(-7, 8, 2)), dmin=3, dmax=9)
+    R = O.contract(A, [0, 2], A, [0, 2], conjA=1)
+    for form in ([A, "!2*", A], [A, "13*", A], [A, [1, 3], "*", A]):
+        C = Q.contractQS(form)
+        assert_same_structure(C, R)
+        assert rel_err(C, R) < TOL
+    rng = np.random.RandomState(3)
+    Lq = sorted({int(q) for q in A.Q[0][:, 0]})
+    dims = {int(q): A.data[i].shape[0] for i, q in enumerate(A.Q[0][:, 0])}
+    Lenv = Q.QSpace([np.array([[q] for q in Lq]), np.array([[q] for q in Lq])],
+                    [rng.standard_normal((dims[q], dims[q])) for q in Lq], "A", ["L", "L*"])
+    C = Q.contractQS([A, "!2*", [Lenv, A]])
+    Y = O.contract(Lenv, [1], A, [0])
+    Z = O.contract(A, [0, 2], Y, [0, 2], conjA=1)
+    assert_same_structure(C, Z)
+    assert rel_err(C, Z) < TOL
