@@ -179,6 +179,11 @@ int qsx_ctr_plan_pairs(const qsx_ctr_plan* plan, int64_t* npairs, int64_t* nout,
 int qsx_ctr_execute(qsx_ctr_plan* plan, const double* dA, const double* dB, double* dC,
                     void* stream);
 double qsx_ctr_plan_flops(const qsx_ctr_plan* plan);
+/* SkipZeroData's test on device data (Source/QSpace.cc:3200-3270, eps = 0):
+   flags[i] = 1 if block i (elements offs[i]..offs[i+1]-1 of ddata) is all
+   exact zeros; offs / flags are host arrays, ddata device memory. */
+int qsx_block_zero_flags(const int64_t* offs, int32_t nblk, const double* ddata, int32_t* flags, char* err,
+                         size_t errlen);
 /* set the work decomposition policy (int32[QSX_POLICY_INTS], see below) */
 int qsx_ctr_plan_set_policy(qsx_ctr_plan* plan, const int32_t* policy);
 void qsx_ctr_plan_destroy(qsx_ctr_plan* plan);

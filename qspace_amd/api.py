@@ -154,7 +154,14 @@ def contractQS(*args, mode="batched", **kw) -> QSpace:
         if sorted(v) == list(range(1, len(v) + 1)) and (len(v) != 1 or len(args) - 1 < 2):
             perm = [x - 1 for x in v]
             args.pop()
-    C = _cell(args, 0, mode, kw)
+    # the contractions run on device-resident operands (one upload per leaf,
+    # one download of the result); a single top-level operand is a conj /
+    # permutation of it
+    tops = [a for a in args if isinstance(a, QSpace) or _is_cell(a)]
+    if len(tops) >= 2:
+        C = _download_result(_cell_dev(args, 0, mode, kw, {}, []))
+    else:
+        C = _single(args, mode, kw)
     if perm is not None:
         if len(perm) > C.rank:
             raise _capi.QsxError(1, f"invalid input permutation (len={len(perm)}/{C.rank})")
@@ -239,42 +246,115 @@ def _skipped(op, T: QSpace):
     return {i for i in range(r) if (i in listed) == op["xflag"]}
 
 
-def _cell(items, level, mode, kw) -> QSpace:
-    """contractQS_itags (Source/contractQS.cc:634-808): operands (QSpaces or
-    nested cells, evaluated first) each followed by its options, contracted
-    right to left by itag matching."""
+def _single(items, mode, kw) -> QSpace:
+    """a cell form with one top-level operand (contractQS_itags with len 1
+    at level 0, Source/contractQS.cc:676-704): the operand (a nested cell is
+    contracted on the device first), then its conj flag and its index
+    option as a permutation"""
+    op = None
+    for a in items:
+        if isinstance(a, QSpace) or _is_cell(a):
+            if op is not None:
+                raise _capi.QsxError(1, "one top-level operand expected")
+            if isinstance(a, QSpace):
+                T = a
+            else:
+                if len(a) < 2:
+                    raise _capi.QsxError(1, "invalid cell-contraction (got cell with fewer than two entries)")
+                T = _download_result(_cell_dev(list(a), 1, mode, kw, {}, []))
+            op = {"T": T, "conj": 0, "sidx": None, "xflag": False}
+        else:
+            if op is None:
+                raise _capi.QsxError(1, "invalid QSpace cell-structure (option before any QSpace)")
+            _cell_option(op, a)
+    if op is None:
+        raise _capi.QsxError(1, "invalid usage #2: no QSpace")
+    T = op["T"]
+    if op["conj"]:
+        T = permuteQS(T, [], conj=True)
+    if op["sidx"] is not None:
+        T = permuteQS(T, op["sidx"])
+    return T
+
+
+def _ctr_str(idx, conj):
+    """0-based leg list -> compact 1-based ctr-idx string (digits, then A=10..)."""
+    chars = "".join(str(i + 1) if i < 9 else chr(ord("A") + i - 9) for i in idx)
+    return chars + ("*" if conj else "")
+
+
+class _DevT:
+    """a cell operand resident on the device: its structure (a QSpace whose
+    data are not used) and its packed data"""
+
+    def __init__(self, S: QSpace, buf: "DeviceBuffer", keep=None):
+        self.S = S
+        self.buf = buf
+        self.keep = keep
+
+
+def _cell_dev(items, level, mode, kw, cache, plans):
+    """contractQS_itags (Source/contractQS.cc:634-808) on device-resident
+    operands: every leaf is uploaded once, each contraction is a plan
+    executed on device buffers, and only the result comes back.  The
+    intermediates keep their exactly-zero blocks (the reference drops them
+    after every contraction; they add nothing to the result, whose zero
+    blocks are dropped at the end)."""
     ops = []
     for a in items:
         if isinstance(a, QSpace):
-            ops.append({"T": a, "conj": 0, "sidx": None, "xflag": False})
+            if id(a) not in cache:
+                cache[id(a)] = _DevT(a, DeviceBuffer.from_array(a.packed()), keep=a)
+            ops.append({"T": cache[id(a)], "conj": 0, "sidx": None, "xflag": False})
         elif _is_cell(a):
             if len(a) < 2:
                 raise _capi.QsxError(1, "invalid cell-contraction (got cell with fewer than two entries)")
-            ops.append({"T": _cell(list(a), level + 1, mode, kw), "conj": 0, "sidx": None, "xflag": False})
+            ops.append({"T": _cell_dev(list(a), level + 1, mode, kw, cache, plans), "conj": 0, "sidx": None,
+                        "xflag": False})
         else:
             if not ops:
                 raise _capi.QsxError(1, "invalid QSpace cell-structure (option before any QSpace)")
             _cell_option(ops[-1], a)
     if len(ops) < 2:
-        if len(ops) == 1 and level == 0:
-            op = ops[0]
-            T = op["T"]
-            if op["conj"]:
-                T = permuteQS(T, [], conj=True)
-            if op["sidx"] is not None:
-                T = permuteQS(T, op["sidx"])
-            return T
         raise _capi.QsxError(1, "invalid usage #2: at least two QSpaces required for each contraction")
     cur = ops[-1]["T"]
     cflag = ops[-1]["conj"]
-    skipB = _skipped(ops[-1], cur)
+    skipB = _skipped(ops[-1], cur.S)
     for op in reversed(ops[:-1]):
         A = op["T"]
-        ia, ib, conjA = _itag_match(A, cur, op["conj"], cflag, _skipped(op, A), skipB)
-        cur = _contract_pair(A, ia, conjA, cur, ib, cflag, None, mode=mode, **kw)
+        ia, ib, conjA = _itag_match(A.S, cur.S, op["conj"], cflag, _skipped(op, A.S), skipB)
+        P = CtrPlan(A.S, _ctr_str(ia, conjA), cur.S, _ctr_str(ib, cflag), mode=mode, **kw)
+        dC = DeviceBuffer(max(1, P.nout))
+        P.execute(A.buf, cur.buf, dC)
+        plans.append(P)
+        cur = _DevT(P.out, dC)
         cflag = 0
         skipB = set()
     return cur
+
+
+def _download_result(R: _DevT, skip_zero=True) -> QSpace:
+    """the device result back as a QSpace, SkipZeroData applied on the device
+    (eps = 0; zero blocks kept in a rank-2 block-diagonal matrix,
+    Source/QSpace.cc:3200-3270)"""
+    S = R.S
+    n = S.nblk
+    sizes = [int(np.prod(d.shape)) for d in S.data]
+    offs = np.zeros(n + 1, np.int64)
+    offs[1:] = np.cumsum(sizes) if n else []
+    keep = list(range(n))
+    if skip_zero and n:
+        blockdiag = S.rank == 2 and all(np.array_equal(S.Q[0][i], S.Q[1][i]) for i in range(n))
+        if not blockdiag:
+            flags = np.zeros(n, np.int32)
+            err = C.create_string_buffer(256)
+            _capi.check(_capi.lib().qsx_block_zero_flags(offs.ctypes.data_as(C.POINTER(C.c_int64)), n, R.buf.ptr,
+                                                         flags.ctypes.data_as(C.POINTER(C.c_int32)), err, 256), err)
+            keep = [i for i in range(n) if not flags[i]]
+    v = R.buf.download()
+    data = [np.asarray(v[offs[i]:offs[i + 1]]).reshape(S.data[i].shape, order="F") for i in keep]
+    cgw = S.info["cgw"][keep] if S.info["cgw"] is not None and keep else None
+    return QSpace([q[keep] for q in S.Q], data, S.qtype, S.itags, cgw)
 
 
 def _out_to_qspace(out) -> QSpace:

@@ -1003,6 +1003,30 @@ int qsx_ctr_execute(qsx_ctr_plan* plan, const double* dA, const double* dB, doub
   }
 }
 
+int qsx_block_zero_flags(const int64_t* offs, int32_t nblk, const double* ddata, int32_t* flags, char* err,
+                         size_t errlen) {
+  return guarded(err, errlen, [&] {
+    if (nblk < 0 || (nblk && (!offs || !flags))) fail(QSX_ERR_ARG, "invalid block list");
+    if (!nblk) return 0;
+    hipStream_t s = default_stream();
+    int64_t* doffs = (int64_t*)dev_alloc(sizeof(int64_t) * (nblk + 1));
+    int32_t* dz = (int32_t*)dev_alloc(sizeof(int32_t) * nblk);
+    try {
+      QSX_HIP(hipMemcpyAsync(doffs, offs, sizeof(int64_t) * (nblk + 1), hipMemcpyHostToDevice, s));
+      launch_block_zero_flags(doffs, nblk, ddata, dz, s);
+      QSX_HIP(hipMemcpyAsync(flags, dz, sizeof(int32_t) * nblk, hipMemcpyDeviceToHost, s));
+      QSX_HIP(hipStreamSynchronize(s));
+    } catch (...) {
+      dev_free(doffs);
+      dev_free(dz);
+      throw;
+    }
+    dev_free(doffs);
+    dev_free(dz);
+    return 0;
+  });
+}
+
 double qsx_ctr_plan_flops(const qsx_ctr_plan* plan) { return plan ? plan->p->flops : 0; }
 
 int qsx_ctr_plan_set_policy(qsx_ctr_plan* plan, const int32_t* policy) {
