@@ -325,6 +325,36 @@ def block_ops():
     return out
 
 
+def env_update():
+    """U(1) environment update L' = A'.(HK.W.A) in the cell form of
+    updateHK.m, {A,'!2*',{{A,HK},W}}: bond of 21 charge sectors with
+    40..120 states each, MPO bond 3 states per charge.  GPU: the whole
+    contractQS call (upload of the leaves, three plans built and executed on
+    device buffers, download), mean of 3 after one warm-up; CPU: the
+    oracle's explicit chain, one thread."""
+    import oracle as O
+    import qspace_amd as Q
+    from qspace_amd import gen
+    A, HK, W = gen.u1_env_update(seed=1006, Lq=tuple(range(-20, 21, 2)), Rq=tuple(range(-21, 22, 2)), dmin=40,
+                                 dmax=120, dw=3)
+    Q.contractQS([A, "!2*", [[A, HK], W]])
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        C = Q.contractQS([A, "!2*", [[A, HK], W]])
+        ts.append((time.perf_counter() - t0) * 1e3)
+    t0 = time.perf_counter()
+    X1 = O.contract(A, [0], HK, [0])
+    X2 = O.contract(X1, [1, 3], W, [2, 0])
+    R = O.contract(A, [0, 2], X2, [1, 3], conjA=1)
+    cpu_ms = (time.perf_counter() - t0) * 1e3
+    out = {"bond_states": int(sum(A.data[i].shape[0] for i in range(A.nblk) if int(A.Q[2][i, 0]) == 1)),
+           "blocks": [A.nblk, HK.nblk, W.nblk], "useful_gflop": (X1.flops + X2.flops + R.flops) / 1e9,
+           "gpu_wall_ms": sum(ts) / len(ts), "oracle_cpu_ms_1thread": cpu_ms, "rel_err_vs_oracle": _rel(C, R)}
+    log(f"[bench] env update: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -430,6 +460,7 @@ def main():
     if not args.lite:
         configs = extra_configs()
         bops = block_ops()
+        bops["u1_env_update"] = env_update()
     cpu = None
     if not args.no_cpu and not args.lite:
         log("[bench] cpu baseline (sample) ...")

@@ -162,3 +162,35 @@ def fermion_parity_signs(HL: QSpace) -> np.ndarray:
     fermionic operator acts through the left block."""
     qw = HL.Q[2][:, 0]
     return np.where(qw % 2 == 0, 1.0, -1.0)
+
+
+# ---------------------------------------------------------------------------
+def u1_env_update(seed=1006, Lq=tuple(range(-6, 7, 2)), Rq=tuple(range(-7, 8, 2)), dmin=3, dmax=10,
+                  ws=(-2, 0, 2), dw=1):
+    """Operands of the U(1) environment update L' = A'.(HK.W.A) in the leg
+    conventions of updateHK.m: A(L,R,s) (reference leg order), HK(L*,L,w*)
+    and the MPO W(w,w2*,s*,s), charges conserved (in = out)."""
+    rng = _rng(seed + 7)
+    A0 = u1_mps(seed=seed, Lq=Lq, Rq=Rq, dmin=dmin, dmax=dmax)
+    A = QSpace([A0.Q[0], A0.Q[2], A0.Q[1]], [np.asfortranarray(np.transpose(d, (0, 2, 1))) for d in A0.data], "A",
+               ["L", "R*", "s"])
+    dL = {int(q): A.data[i].shape[0] for i, q in enumerate(A.Q[0][:, 0])}
+    hq, hd = [[], [], []], []
+    for a in sorted(dL):
+        for w in ws:
+            b = a + w
+            if b in dL:
+                hq[0].append([a]); hq[1].append([b]); hq[2].append([w])
+                hd.append(_normal_block(rng, (dL[a], dL[b], dw)))
+    HK = QSpace([np.array(q) for q in hq], hd, "A", ["L*", "L", "w*"])
+    wq, wd = [[], [], [], []], []
+    for w in ws:
+        for s_in in (-1, 1):
+            for w2 in ws:
+                s_out = w + s_in - w2
+                if s_out in (-1, 1):
+                    for k, v in enumerate((w, w2, s_out, s_in)):
+                        wq[k].append([v])
+                    wd.append(_normal_block(rng, (dw, dw, 1, 1)))
+    W = QSpace([np.array(q) for q in wq], wd, "A", ["w", "w2*", "s*", "s"])
+    return A, HK, W

@@ -328,41 +328,11 @@ def test_cell_form_options(gpu):
     assert rel_err(C, Z) < TOL
 
 
-def u1_env_operands(seed, Lq=tuple(range(-6, 7, 2)), Rq=tuple(range(-7, 8, 2)), dmin=3, dmax=10):
-    """U(1) MPS tensor A(L,R,s) (reference leg order), environment HK(L*,L,w*)
-    and MPO W(w,w2*,s*,s) with conserved charges (in = out)."""
-    rng = np.random.RandomState(seed)
-    A0 = gen.u1_mps(seed=seed, Lq=Lq, Rq=Rq, dmin=dmin, dmax=dmax)
-    A = Q.QSpace([A0.Q[0], A0.Q[2], A0.Q[1]], [np.transpose(d, (0, 2, 1)) for d in A0.data], "A",
-                 ["L", "R*", "s"])
-    dL = {int(q): A.data[i].shape[0] for i, q in enumerate(A.Q[0][:, 0])}
-    ws = (-2, 0, 2)
-    hq, hd = [[], [], []], []
-    for a in dL:
-        for w in ws:
-            b = a + w            # in: L (leg 2 of HK) = out: L* + w*
-            if b in dL:
-                hq[0].append([a]); hq[1].append([b]); hq[2].append([w])
-                hd.append(rng.standard_normal((dL[a], dL[b], 1)))
-    HK = Q.QSpace([np.array(q) for q in hq], hd, "A", ["L*", "L", "w*"])
-    wq, wd = [[], [], [], []], []
-    for w in ws:
-        for s_in in (-1, 1):
-            for w2 in ws:
-                s_out = w + s_in - w2
-                if s_out in (-1, 1):
-                    for k, v in enumerate((w, w2, s_out, s_in)):
-                        wq[k].append([v])
-                    wd.append(rng.standard_normal((1, 1, 1, 1)))
-    W = Q.QSpace([np.array(q) for q in wq], wd, "A", ["w", "w2*", "s*", "s"])
-    return A, HK, W
-
-
 def test_env_update_u1_cell_form(gpu):
     """environment update L' = A'.(HK.W.A) in the cell form of updateHK.m
     ({A,'!2*',{{A,HK},W}}), contracted on device-resident operands, against
     the oracle's explicit chain."""
-    A, HK, W = u1_env_operands(41)
+    A, HK, W = gen.u1_env_update(seed=41, dw=2)
     C = Q.contractQS([A, "!2*", [[A, HK], W]])
     X1 = O.contract(A, [0], HK, [0])            # (R*, s, L, w*)
     X2 = O.contract(X1, [1, 3], W, [2, 0])      # (R*, L, w2*, s)
