@@ -13,11 +13,13 @@ byte counts of the GEMM kernels against it.
     python bench.py [--steps K] [--warmup W] [--workload heis4000|heis2000|hubbard4000]
                     [--no-cpu] [--lite] [--policy FILE]
 
-Besides the headline H*psi the JSON line carries "configs": the survey's
+Besides the headline H*psi the JSON line carries "configs": the benchmark
 configurations 1 (U(1) MPS overlap, CPU oracle and GPU), 2 (SU(2)
 environment step), 3 (Heisenberg D*=2000, 20 Lanczos iterations), 4 (split
 precision: BF16-slice H*psi error/time table on the headline operator) and 5
-(SU(2)xU(1) Hubbard D*=4000 with parity signs, 30 Davidson iterations).
+(SU(2)xU(1) Hubbard D*=4000 with parity signs, 30 Davidson iterations),
+and "block_ops": permuteQS / plusQS bandwidth on the SU(2) environments and
+the U(1) environment update in the cell form of updateHK.m.
 """
 from __future__ import annotations
 
@@ -200,7 +202,7 @@ def _time_plan(P, A, B, reps=20):
 
 
 def extra_configs():
-    """the survey's five configurations (config 4 = the headline workload)"""
+    """the five benchmark configurations (config 4 = the headline workload)"""
     import qspace_amd as Q
     from qspace_amd import gen
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
