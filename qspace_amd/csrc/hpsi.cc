@@ -22,6 +22,7 @@
 #include <numeric>
 
 #include "contract.hpp"
+#include "smalleig.hpp"
 #include "vec.hpp"
 
 namespace qsx {
@@ -857,46 +858,6 @@ static void hpsi_apply(HpsiPlan& H, const double* dpsi, double* dhpsi, hipStream
 
 // ---------------------------------------------------------------- eigensolvers
 // symmetric eigen-decomposition by cyclic Jacobi (small dense matrices)
-static void jacobi_eig(int n, std::vector<double> a, std::vector<double>& w, std::vector<double>& v) {
-  v.assign((size_t)n * n, 0.0);
-  for (int i = 0; i < n; ++i) v[(size_t)i * n + i] = 1.0;
-  for (int sweep = 0; sweep < 100; ++sweep) {
-    double off = 0;
-    for (int p = 0; p < n; ++p)
-      for (int q = p + 1; q < n; ++q) off += a[(size_t)p * n + q] * a[(size_t)p * n + q];
-    if (off < 1e-30) break;
-    for (int p = 0; p < n; ++p)
-      for (int q = p + 1; q < n; ++q) {
-        double apq = a[(size_t)p * n + q];
-        if (std::fabs(apq) < 1e-300) continue;
-        double app = a[(size_t)p * n + p], aqq = a[(size_t)q * n + q];
-        double th = (aqq - app) / (2 * apq);
-        double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1));
-        double c = 1 / std::sqrt(t * t + 1), sn = t * c;
-        for (int k = 0; k < n; ++k) {
-          double akp = a[(size_t)k * n + p], akq = a[(size_t)k * n + q];
-          a[(size_t)k * n + p] = c * akp - sn * akq;
-          a[(size_t)k * n + q] = sn * akp + c * akq;
-        }
-        for (int k = 0; k < n; ++k) {
-          double apk = a[(size_t)p * n + k], aqk = a[(size_t)q * n + k];
-          a[(size_t)p * n + k] = c * apk - sn * aqk;
-          a[(size_t)q * n + k] = sn * apk + c * aqk;
-        }
-        for (int k = 0; k < n; ++k) {
-          double vkp = v[(size_t)k * n + p], vkq = v[(size_t)k * n + q];
-          v[(size_t)k * n + p] = c * vkp - sn * vkq;
-          v[(size_t)k * n + q] = sn * vkp + c * vkq;
-        }
-      }
-  }
-  w.resize(n);
-  for (int i = 0; i < n; ++i) w[i] = a[(size_t)i * n + i];
-}
-
-static int argmin(const std::vector<double>& w) {
-  return (int)(std::min_element(w.begin(), w.end()) - w.begin());
-}
 
 static void lanczos(HpsiPlan& H, const double* psi0, int niter, double* e0, double* ritz, double* gs) {
   hipStream_t s = default_stream();
@@ -935,21 +896,15 @@ static void lanczos(HpsiPlan& H, const double* psi0, int niter, double* e0, doub
     std::vector<double> b(niter + 1), al(niter);
     QSX_HIP(hipMemcpy(b.data(), beta, sizeof(double) * niter, hipMemcpyDeviceToHost));
     QSX_HIP(hipMemcpy(al.data(), alpha, sizeof(double) * niter, hipMemcpyDeviceToHost));
-    std::vector<double> wv, vv;
-    for (int k = 1; k <= niter; ++k) {
-      std::vector<double> T((size_t)k * k, 0.0);
-      for (int i = 0; i < k; ++i) {
-        T[(size_t)i * k + i] = al[i];
-        if (i + 1 < k) T[(size_t)i * k + i + 1] = T[(size_t)(i + 1) * k + i] = b[i];
-      }
-      jacobi_eig(k, T, wv, vv);
-      if (ritz) ritz[k - 1] = wv[argmin(wv)];
-    }
-    int im = argmin(wv);
-    if (e0) *e0 = wv[im];
+    // Ritz values of every T_k, lowest eigenpair of T_niter
+    if (ritz)
+      for (int k = 1; k < niter; ++k) ritz[k - 1] = tri_min_eigenvalue(k, al.data(), b.data());
+    double lam = 0.0;
+    std::vector<double> y;
+    tri_lowest(niter, al.data(), b.data(), lam, y);
+    if (ritz) ritz[niter - 1] = lam;
+    if (e0) *e0 = lam;
     if (gs) {
-      std::vector<double> y(niter);
-      for (int i = 0; i < niter; ++i) y[i] = vv[(size_t)i * niter + im];
       double* dy = upload_vec(y, s);
       vec_combine(w, V, N, dy, niter, N, s);
       QSX_HIP(hipMemcpyAsync(gs, w, sizeof(double) * N, hipMemcpyDeviceToHost, s));
@@ -989,15 +944,12 @@ static void davidson(HpsiPlan& H, const double* psi0, int niter, double* e0, dou
     QSX_HIP(hipMemcpyAsync(&Hk[0], c, sizeof(double), hipMemcpyDeviceToHost, s));
     QSX_HIP(hipStreamSynchronize(s));
     int k = 1;
-    std::vector<double> w, v;
     for (int it = 0; it < niter; ++it) {
-      jacobi_eig(k, Hk, w, v);
-      int im = argmin(w);
-      double th = w[im];
+      double th = 0.0;
+      std::vector<double> y;
+      sym_lowest(k, Hk, th, y);
       if (ritz) ritz[it] = th;
       if (e0) *e0 = th;
-      std::vector<double> y(k);
-      for (int i = 0; i < k; ++i) y[i] = v[(size_t)i * k + im];
       QSX_HIP(hipMemcpyAsync(dy, y.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
       vec_combine(u, V, N, dy, k, N, s);
       vec_combine(hu, HV, N, dy, k, N, s);
