@@ -928,8 +928,10 @@ static void davidson(HpsiPlan& H, const double* psi0, int niter, double* e0, dou
   double* HV = (double*)dev_alloc(sizeof(double) * N * mmax);
   double* u = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, N));
   double* hu = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, N));
-  double* part = (double*)dev_alloc(sizeof(double) * DOT_BLOCKS * (mmax + 2));
-  double* c = (double*)dev_alloc(sizeof(double) * (mmax + 1));
+  double* part = (double*)dev_alloc(sizeof(double) * DOT_BLOCKS * 2 * (mmax + 1));
+  // ret = [new column of the projected matrix (mmax) | |r|^2 partials]: one
+  // device-to-host copy per step
+  double* ret = (double*)dev_alloc(sizeof(double) * (mmax + DOT_BLOCKS));
   double* nrm = (double*)dev_alloc(sizeof(double) * 2);
   double* dy = (double*)dev_alloc(sizeof(double) * (mmax + 1));
   try {
@@ -940,41 +942,44 @@ static void davidson(HpsiPlan& H, const double* psi0, int niter, double* e0, dou
     vec_normalize(u, V, nrm, nullptr, N, s);
     hpsi_apply(H, V, HV, s, true);
     std::vector<double> Hk(1);
-    vec_mdot(V, 0, 1, HV, N, part, c, 0, s);
-    QSX_HIP(hipMemcpyAsync(&Hk[0], c, sizeof(double), hipMemcpyDeviceToHost, s));
+    vec_mdot(V, 0, 1, HV, N, part, ret, 0, s);
+    QSX_HIP(hipMemcpyAsync(&Hk[0], ret, sizeof(double), hipMemcpyDeviceToHost, s));
     QSX_HIP(hipStreamSynchronize(s));
     int k = 1;
+    std::vector<double> back(mmax + DOT_BLOCKS);
     for (int it = 0; it < niter; ++it) {
       double th = 0.0;
       std::vector<double> y;
       sym_lowest(k, Hk, th, y);
       if (ritz) ritz[it] = th;
       if (e0) *e0 = th;
-      QSX_HIP(hipMemcpyAsync(dy, y.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
-      vec_combine(u, V, N, dy, k, N, s);
-      vec_combine(hu, HV, N, dy, k, N, s);
-      if (it + 1 == niter) break;
-      // residual r = H u - theta u, orthogonalised twice against V (the
-      // reductions' final sums are folded into their consumers)
-      QSX_HIP(hipMemcpyAsync(c, &th, sizeof(double), hipMemcpyHostToDevice, s));
-      vec_axmy(hu, hu, u, c, N, s);
-      for (int rep = 0; rep < 2; ++rep) {
-        vec_mdot_partial(V, N, k, hu, N, part, s);
-        vec_gs_update_partial(hu, V, N, part, k, N, nullptr, 0, s);
+      // u = V y and the residual r = H u - theta u in one pass (y and theta
+      // in one host-to-device copy)
+      y.push_back(th);
+      QSX_HIP(hipMemcpyAsync(dy, y.data(), sizeof(double) * (k + 1), hipMemcpyHostToDevice, s));
+      if (it + 1 == niter) {
+        vec_combine(u, V, N, dy, k, N, s);
+        break;
       }
-      double* rpart = part + (int64_t)DOT_BLOCKS * mmax;  // |r|^2 partials
-      vec_mdot_partial(hu, 0, 1, hu, N, rpart, s);
+      vec_combine_res(u, hu, V, HV, N, dy, k, N, s);
+      // r orthogonalised twice against V, then normalised: classical
+      // Gram-Schmidt with each reduction's final sum folded into its
+      // consumer (the second update also forms the |r|^2 partials)
+      double* pa = part;
+      double* pb = part + (int64_t)DOT_BLOCKS * (mmax + 1);
+      double* rpart = ret + mmax;
+      vec_mdot_partial(V, N, k, hu, N, pa, s);
+      vec_gs_update_mdot(hu, V, N, pa, k, N, nullptr, 0, V, k, pb, s);
+      vec_gs_update_mdot(hu, V, N, pb, k, N, nullptr, 0, nullptr, 1, rpart, s);
       vec_normalize_partial(hu, V + (int64_t)k * N, rpart, nullptr, N, s);
       hpsi_apply(H, V + (int64_t)k * N, HV + (int64_t)k * N, s, true);
       // grow the projected matrix by one row/column; one host round trip
       // brings the new column and the residual norm
-      vec_mdot(V, N, k + 1, HV + (int64_t)k * N, N, part, c, 0, s);
-      std::vector<double> col(k + 1), rp(DOT_BLOCKS);
-      QSX_HIP(hipMemcpyAsync(col.data(), c, sizeof(double) * (k + 1), hipMemcpyDeviceToHost, s));
-      QSX_HIP(hipMemcpyAsync(rp.data(), rpart, sizeof(double) * DOT_BLOCKS, hipMemcpyDeviceToHost, s));
+      vec_mdot(V, N, k + 1, HV + (int64_t)k * N, N, part, ret, 0, s);
+      QSX_HIP(hipMemcpyAsync(back.data(), ret, sizeof(double) * (mmax + DOT_BLOCKS), hipMemcpyDeviceToHost, s));
       QSX_HIP(hipStreamSynchronize(s));
       double r2 = 0;
-      for (double x : rp) r2 += x;
+      for (int b = 0; b < DOT_BLOCKS; ++b) r2 += back[mmax + b];
       if (r2 < 1e-28) {  // converged: the new direction is noise, keep u
         for (int j = it + 1; j < niter && ritz; ++j) ritz[j] = th;
         break;
@@ -983,8 +988,8 @@ static void davidson(HpsiPlan& H, const double* psi0, int niter, double* e0, dou
       for (int i = 0; i < k; ++i)
         for (int j = 0; j < k; ++j) Hn[(size_t)i * (k + 1) + j] = Hk[(size_t)i * k + j];
       for (int i = 0; i <= k; ++i) {
-        Hn[(size_t)i * (k + 1) + k] = col[i];
-        Hn[(size_t)k * (k + 1) + i] = col[i];
+        Hn[(size_t)i * (k + 1) + k] = back[i];
+        Hn[(size_t)k * (k + 1) + i] = back[i];
       }
       Hk.swap(Hn);
       ++k;
@@ -995,10 +1000,12 @@ static void davidson(HpsiPlan& H, const double* psi0, int niter, double* e0, dou
       for (int64_t i = 0; i < N; ++i) gs[i] /= std::sqrt(H.welem[i]);
     }
   } catch (...) {
-    for (void* p : {(void*)V, (void*)HV, (void*)u, (void*)hu, (void*)part, (void*)c, (void*)nrm, (void*)dy}) dev_free(p);
+    for (void* p : {(void*)V, (void*)HV, (void*)u, (void*)hu, (void*)part, (void*)ret, (void*)nrm, (void*)dy})
+      dev_free(p);
     throw;
   }
-  for (void* p : {(void*)V, (void*)HV, (void*)u, (void*)hu, (void*)part, (void*)c, (void*)nrm, (void*)dy}) dev_free(p);
+  for (void* p : {(void*)V, (void*)HV, (void*)u, (void*)hu, (void*)part, (void*)ret, (void*)nrm, (void*)dy})
+    dev_free(p);
 }
 
 }  // namespace qsx

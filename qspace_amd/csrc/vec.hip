@@ -71,6 +71,23 @@ __global__ __launch_bounds__(RT) void k_combine(double* __restrict__ y, const do
   }
 }
 
+// u = V c, r = HV c - theta u with theta = c[m] (the Davidson Ritz vector
+// and its residual in one pass over V and HV)
+__global__ __launch_bounds__(RT) void k_combine_res(double* __restrict__ u, double* __restrict__ r,
+                                                     const double* __restrict__ V, const double* __restrict__ HV,
+                                                     int64_t ldv, const double* __restrict__ c, int m, int64_t n) {
+  const double th = c[m];
+  for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < n; i += (int64_t)gridDim.x * RT) {
+    double su = 0, sh = 0;
+    for (int j = 0; j < m; ++j) {
+      su += c[j] * V[(int64_t)j * ldv + i];
+      sh += c[j] * HV[(int64_t)j * ldv + i];
+    }
+    u[i] = su;
+    r[i] = sh - th * su;
+  }
+}
+
 // beta = sqrt(*nrm2); dst = src / beta
 __global__ __launch_bounds__(RT) void k_normalize(const double* __restrict__ src, double* __restrict__ dst,
                                                    const double* __restrict__ nrm2, double* __restrict__ beta,
@@ -220,6 +237,11 @@ void vec_gs_update(double* w, const double* V, int64_t ldv, const double* c, int
 
 void vec_combine(double* y, const double* V, int64_t ldv, const double* c, int m, int64_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_combine, dim3(grid_for(n)), dim3(RT), 0, s, y, V, ldv, c, m, n);
+}
+
+void vec_combine_res(double* u, double* r, const double* V, const double* HV, int64_t ldv, const double* c, int m,
+                     int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_combine_res, dim3(grid_for(n)), dim3(RT), 0, s, u, r, V, HV, ldv, c, m, n);
 }
 
 void vec_normalize(const double* src, double* dst, const double* nrm2, double* beta, int64_t n, hipStream_t s) {

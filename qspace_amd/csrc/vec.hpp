@@ -26,6 +26,9 @@ void vec_normalize_partial(const double* src, double* dst, const double* partial
 void vec_gs_update_mdot(double* w, const double* V, int64_t ldv, const double* pin, int m, int64_t n, double* cout,
                         int cj, const double* V2, int jn, double* pout, hipStream_t s);
 void vec_combine(double* y, const double* V, int64_t ldv, const double* c, int m, int64_t n, hipStream_t s);
+// u = V c, r = HV c - c[m] u (c holds m coefficients and then theta)
+void vec_combine_res(double* u, double* r, const double* V, const double* HV, int64_t ldv, const double* c, int m,
+                     int64_t n, hipStream_t s);
 void vec_normalize(const double* src, double* dst, const double* nrm2, double* beta, int64_t n, hipStream_t s);
 void vec_axmy(double* y, const double* x, const double* z, const double* a, int64_t n, hipStream_t s);
 
