@@ -386,6 +386,12 @@ def main():
     t0 = time.perf_counter()
     H = Q.HPsi(HL, psi, HR, hconj=1, fermA=ferm)
     plan_s = time.perf_counter() - t0
+    # a second build of the same plan: the X-symbol cache and the device code
+    # are warm (the state of every later site of a sweep)
+    t0 = time.perf_counter()
+    H2 = Q.HPsi(HL, psi, HR, hconj=1, fermA=ferm)
+    plan_warm_s = time.perf_counter() - t0
+    del H2
     t0 = time.perf_counter()
     if args.policy:
         with open(args.policy) as f:
@@ -499,6 +505,7 @@ def main():
         "lanczos20_first_call_ms": lan_first,
         "lanczos20_E0": e0,
         "plan_create_s": plan_s,
+        "plan_create_warm_s": plan_warm_s,
         "autotune": {"seconds": tune_s, **{k: list(v) for k, v in tune.items()}},
         "calibration": calib,
         "configs": configs,

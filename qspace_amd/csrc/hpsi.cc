@@ -15,6 +15,7 @@
 // Environments are copied to the device once, in the layouts the GEMM
 // consumes (permuted copies made once at plan time).
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -625,9 +626,22 @@ static void hpsi_plain(HpsiPlan& H, const double* dpsi, double* dhpsi, hipStream
   run(H.pairs2, H.targets2, H.dT, H.dHR, dhpsi);
 }
 
+// host wall-clock phases of the plan build (printed with verbose >= 2)
+struct PhaseClock {
+  int verbose;
+  std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+  void lap(const char* what) {
+    auto n = std::chrono::steady_clock::now();
+    if (verbose >= 2)
+      fprintf(stderr, "qsx H*psi plan: %-28s %8.3f ms\n", what, std::chrono::duration<double, std::milli>(n - t).count());
+    t = n;
+  }
+};
+
 static void hpsi_create(HpsiPlan& H, const qsx_tensor* HL, const qsx_tensor* psi, const qsx_tensor* HR,
                         const qsx_opts* o, int hconj) {
   ensure_device();
+  PhaseClock clk{o ? o->verbose : 0};
   hipStream_t s = default_stream();
   H.HL = make_desc(HL, "HL");
   H.psi = make_desc(psi, "psi");
@@ -663,6 +677,7 @@ static void hpsi_create(HpsiPlan& H, const qsx_tensor* HL, const qsx_tensor* psi
                           1);
     if (H.s1[t]->lb.permute) fail(QSX_ERR_UNSUPPORTED, "H*psi: psi needs a permute in stage 1");
   }
+  clk.lap("stage-1 structure");
   // stage 2 plans: (T_t, HR [,'*'])
   HostTensor Tt[2];
   for (int t = 0; t < H.nterm; ++t) {
@@ -682,6 +697,7 @@ static void hpsi_create(HpsiPlan& H, const qsx_tensor* HL, const qsx_tensor* psi
                           &s2o, false, 2);
     if (H.s2[t]->la.permute) fail(QSX_ERR_UNSUPPORTED, "H*psi: intermediate needs a permute in stage 2");
   }
+  clk.lap("stage-2 structure");
   // device arenas: HL (+ permuted copies), HR copies, T_t
   H.hl_total = H.HL.total();
   int64_t off = H.hl_total;
@@ -719,6 +735,7 @@ static void hpsi_create(HpsiPlan& H, const qsx_tensor* HL, const qsx_tensor* psi
   }
   H.t_total = toff;
   H.dT = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, toff));
+  clk.lap("environment upload + permute");
   // stage 1 work: every T_t block is its own target
   {
     std::vector<CtrPlan*> plans;
@@ -807,11 +824,13 @@ static void hpsi_create(HpsiPlan& H, const qsx_tensor* HL, const qsx_tensor* psi
     H.pairs2s = H.pairs2;
     for (size_t p = 0; p < H.pairs2s.size(); ++p) H.pairs2s[p].afac *= std::sqrt(H.wblk[tgt2[p]]);
   }
+  clk.lap("pair lists");
   H.dpairs1 = upload_vec(H.pairs1, s);
   H.dpairs2 = upload_vec(H.pairs2, s);
   H.dpairs1s = upload_vec(H.pairs1s, s);
   H.dpairs2s = upload_vec(H.pairs2s, s);
   hpsi_build_work(H);
+  clk.lap("work lists");
   if (H.mode == QSX_MODE_SPLIT_BF16) hpsi_split_setup(H);
   QSX_HIP(hipStreamSynchronize(s));
   H.flops = 0;
