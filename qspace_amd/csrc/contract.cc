@@ -15,6 +15,8 @@
 #include "contract.hpp"
 
 #include <algorithm>
+#include <mutex>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
@@ -103,15 +105,22 @@ TensorDesc make_desc(const qsx_tensor* t, const char* name) {
 }
 
 // ---------------------------------------------------------------- device
-static int g_dev = -1;
+// the library's device and stream, set up once per process (the first call
+// from any thread; the others wait)
+static std::atomic<int> g_dev{-1};
 static hipStream_t g_stream = nullptr;
+static std::mutex g_dev_mu;
 
 void ensure_device() {
-  if (g_dev >= 0) return;
+  if (g_dev.load(std::memory_order_acquire) >= 0) return;
+  std::lock_guard<std::mutex> lk(g_dev_mu);
+  if (g_dev.load(std::memory_order_relaxed) >= 0) return;
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) fail(QSX_ERR_DEVICE, "no HIP device available");
-  QSX_HIP(hipGetDevice(&g_dev));
+  int dev = -1;
+  QSX_HIP(hipGetDevice(&dev));
   QSX_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+  g_dev.store(dev, std::memory_order_release);
 }
 
 hipStream_t default_stream() {

@@ -19,6 +19,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <tuple>
 #include <numeric>
 
@@ -324,7 +325,10 @@ static void hpsi_build_fused(HpsiPlan& H) {
 // cached per process under a key built from the stage's block-size
 // histogram (log2 buckets of M, N and K, the STAT_DGEMM idea of
 // Source/wbblas.hh:235-268), so plans with the same size profile reuse it.
+// autotuned policies per block-size histogram, shared by all plans of the
+// process (guarded: plans may be built and tuned from several host threads)
 static std::map<std::string, std::pair<WorkPolicy, WorkPolicy>> g_tune_cache;
+static std::mutex g_tune_mu;
 
 static std::string hist_key(const std::vector<WorkTarget>& targets, const std::vector<PairDesc>& pairs) {
   std::map<int, int64_t> h;
@@ -396,10 +400,19 @@ static void hpsi_autotune(HpsiPlan& H, int reps, double* out) {
     return;
   }
   const std::string key = hist_key(H.targets1, H.pairs1) + "|" + hist_key(H.targets2, H.pairs2);
-  auto it = g_tune_cache.find(key);
-  if (it != g_tune_cache.end()) {
-    H.pol1 = it->second.first;
-    H.pol2 = it->second.second;
+  std::pair<WorkPolicy, WorkPolicy> cached;
+  bool hit = false;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) {
+      cached = it->second;
+      hit = true;
+    }
+  }
+  if (hit) {
+    H.pol1 = cached.first;
+    H.pol2 = cached.second;
     hpsi_build_work(H);
     H.fused = H.pol1.fused;
     if (H.fused) hpsi_build_fused(H);
@@ -474,7 +487,10 @@ static void hpsi_autotune(HpsiPlan& H, int reps, double* out) {
   if (H.verbose) fprintf(stderr, "qsx autotune H*psi: two launches %.4f ms, one launch %.4f ms\n", t_two, t_one);
   dev_free(dpsi);
   dev_free(dout);
-  g_tune_cache[key] = {H.pol1, H.pol2};
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[key] = {H.pol1, H.pol2};
+  }
   hpsi_build_work(H);
   if (H.fused) hpsi_build_fused(H);
 }
