@@ -333,7 +333,8 @@ def env_update():
     40..120 states each, MPO bond 3 states per charge.  GPU: the whole
     contractQS call (upload of the leaves, three plans built and executed on
     device buffers, download), mean of 3 after one warm-up; CPU: the
-    oracle's explicit chain, one thread."""
+    oracle's explicit chain, one thread (the checker and the CPU baseline)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     import qspace_amd as Q
     from qspace_amd import gen
@@ -366,6 +367,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--lite", action="store_true",
                     help="timed H*psi only (no CPU baseline, plain path, Lanczos): for profiler runs")
+    ap.add_argument("--block-ops", action="store_true",
+                    help="only the block operations (permuteQS / plusQS, environment update): one JSON line")
     ap.add_argument("--policy", default=None,
                     help="JSON file {stage1: {...}, stage2: {...}} fixing the work decomposition (no autotune)")
     args = ap.parse_args()
@@ -380,6 +383,11 @@ def main():
     if rc:
         print(f"bench.py: no GPU: {err.value.decode()}", file=sys.stderr)
         sys.exit(1)
+    if args.block_ops:
+        bops = block_ops()
+        bops["u1_env_update"] = env_update()
+        print(json.dumps({"block_ops": bops}), flush=True)
+        return
     w = WORKLOADS[args.workload]
     log(f"[bench] generating {w['name']}")
     HL, psi, HR, ferm = make_workload(w)

@@ -133,16 +133,31 @@ __global__ __launch_bounds__(256) void k_pair_gemm_f64(PairDesc pd, int64_t c_of
   tile_store<TM, TN>(acc, C + c_off, M, N, m0, n0, beta);
 }
 
-__global__ void k_block_zero_flags(const int64_t* __restrict__ offs, const double* __restrict__ data,
-                                   int32_t* __restrict__ flags) {
-  __shared__ int nz;
-  if (threadIdx.x == 0) nz = 0;
-  __syncthreads();
-  int64_t o0 = offs[blockIdx.x], o1 = offs[blockIdx.x + 1];
-  int any = 0;
-  for (int64_t i = o0 + threadIdx.x; i < o1; i += blockDim.x) any |= (data[i] != 0.0);
-  if (any) nz = 1;  // benign race: every writer stores 1
-  __syncthreads();
+// flags[b] = 1 if block b is all exact zeros.  One workgroup per block;
+// each thread reads 8 elements per round (independent loads in flight), and
+// the workgroup stops at the first round that finds a non-zero (the usual
+// case: a non-zero block costs one round)
+__global__ __launch_bounds__(256) void k_block_zero_flags(const int64_t* __restrict__ offs,
+                                                          const double* __restrict__ data,
+                                                          int32_t* __restrict__ flags) {
+  constexpr int U = 8;
+  const int64_t o0 = offs[blockIdx.x], o1 = offs[blockIdx.x + 1];
+  int nz = 0;
+  for (int64_t base = o0; base < o1; base += (int64_t)U * 256) {
+    double x[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t i = base + (int64_t)u * 256 + threadIdx.x;
+      x[u] = i < o1 ? data[i] : 0.0;
+    }
+    int any = 0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) any |= x[u] != 0.0;
+    if (__syncthreads_or(any)) {
+      nz = 1;
+      break;
+    }
+  }
   if (threadIdx.x == 0) flags[blockIdx.x] = nz ? 0 : 1;
 }
 
