@@ -21,6 +21,7 @@
  *   qsx_lanczos         <- RunEigs Lanczos loop, Source/eigs2SiteQS.cc:98-334
  *   qsx_davidson        <- Davidson loop, Class/@Hamilton1D/private/update_psi_2site.m:223-391
  *                          and RunDavidson, Source/david2SiteQS.cc:316-535
+ *   qsx_block_zero_flags <- SkipZeroData's zero test, Source/QSpace.cc:3200-3270
  *   qsx_permute         <- permuteQS, Source/permuteQS.cc:25-87
  *   qsx_plus            <- plusQS, Source/plusQS.cc:17-87
  *   qsx_su2_cg / qsx_su2_x3
