@@ -88,6 +88,9 @@ typedef struct qsx_tensor {
   const uint64_t* itags;
   const char* qtype;
   const double* cgw;
+  /* imaginary parts of complex data: nblk pointers to blocks of the same
+     layout as data, or NULL for real data (qsx_contract only) */
+  const double* const* data_im;
 } qsx_tensor;
 
 /* Result tensor, owned by the library; release with qsx_tensor_free. */
@@ -104,6 +107,7 @@ typedef struct qsx_tensor_out {
   int64_t npairs;        /* executed (ia,ib) pairs */
   double useful_flops;   /* sum over executed pairs of 2*M*N*K */
   double gpu_ms;         /* device time of the data half (events) */
+  double* data_im;       /* imaginary parts (complex results), else NULL */
 } qsx_tensor_out;
 
 typedef struct qsx_opts {
@@ -133,7 +137,10 @@ int qsx_copy_device(double* dst, const double* src, int64_t n, void* stream);
 /* one-shot contraction: contractQS(A,ia,B,ib[,perm]) -------------------- */
 /* ia/ib are 0-based leg indices; conjA/conjB are the trailing '*' flags;
    perm (0-based, may be shorter than the output rank or NULL) is completed
-   with the identity as in Source/contractQS.cc:505-594. */
+   with the identity as in Source/contractQS.cc:505-594.  Complex operands
+   (data_im set) give a complex result: the real/complex dispatch of
+   contractQS.cc:505-594 (zgemm in QSpace_aux.cc), a conj flag conjugating
+   that operand's data as well as reversing its arrows. */
 int qsx_contract(const qsx_tensor* A, const int32_t* ia, int32_t na, int32_t conjA,
                  const qsx_tensor* B, const int32_t* ib, int32_t nb, int32_t conjB,
                  const int32_t* perm, int32_t nperm, const qsx_opts* opts,

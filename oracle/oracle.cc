@@ -112,6 +112,7 @@ static void parse_qtype(const char* s, int qdim, std::vector<int>& st) {
 
 static Tensor from_c(const qsx_tensor* t, const char* name) {
   if (!t) fail(QSX_ERR_ARG, "null tensor %s", name);
+  if (t->data_im) fail(QSX_ERR_UNSUPPORTED, "tensor %s: complex data (combined from real contractions by the caller)", name);
   Tensor T;
   T.n = t->nblk;
   T.r = t->rank;

@@ -17,6 +17,11 @@ Parity status of the oracle's functions (what pins each one):
   * plus (plusQS)                     -- full M-resolved space: Full(C) = Full(A)
                                          + bfac Full(B); block order and zero
                                          handling by hand-built cases
+  * complex data (contract)           -- four real contractions combined by
+                                         bilinearity; full M-resolved space
+                                         with the conjugated operand's data
+                                         conjugated, zero imaginary parts =
+                                         the real result bit for bit
   * reference's FP64 summation order / MKL dgemm rounding: parity unpinned
     (no reference build; tolerance 1e-12 relative instead of bit equality)
 """
@@ -93,6 +98,8 @@ def _ivec(x):
 def contract(A: QSpace, ia, B: QSpace, ib, perm=None, conjA=0, conjB=0, fermA=None, fermB=None,
              skip_zero=1) -> QSpace:
     """0-based ia/ib; returns the oracle's contractQS result."""
+    if A.is_complex() or B.is_complex():
+        return _contract_complex(A, ia, B, ib, perm, conjA, conjB, fermA, fermB, skip_zero)
     L = lib()
     ca, ka = to_c(A)
     cb, kb = to_c(B)
@@ -117,6 +124,64 @@ def contract(A: QSpace, ia, B: QSpace, ib, perm=None, conjA=0, conjB=0, fermA=No
         T.flops = out.contents.useful_flops
     finally:
         L.oracle_tensor_free(out)
+    return T
+
+
+def _planes(T: QSpace):
+    re = QSpace(T.Q, [np.real(d).astype(np.float64) for d in T.data], T.qtype, T.itags, T.info.get("cgw"))
+    if not T.is_complex():
+        return re, None
+    im = QSpace(T.Q, [np.imag(d).astype(np.float64) for d in T.data], T.qtype, T.itags, T.info.get("cgw"))
+    return re, im
+
+
+def _contract_complex(A, ia, B, ib, perm, conjA, conjB, fermA, fermB, skip_zero) -> QSpace:
+    """complex data by bilinearity.  The reference conjugates the data of an
+    operand contracted with its conj flag (QSpace::contract passes the flag
+    to the block products, Source/QSpace.cc:4186-4187; wbarray::contract
+    conjugates a complex operand, Source/wbarray.cc:3473-3483), so with
+    s = -1 for a conjugated operand
+      Re C = C(Ar,Br) - sA sB C(Ai,Bi),  Im C = sB C(Ar,Bi) + sA C(Ai,Br)
+    from four real contractions of the same structure (all blocks kept),
+    then SkipZeroData on the combined blocks (both planes zero; rank-2
+    block-diagonal results keep theirs, Source/QSpace.cc:3200-3305).
+    The conj flag of the data is the one after the missing-conj rule
+    (contract_getIdxSet, Source/QSpace.cc:4262-4267): contracted legs of
+    equal direction with neither or both operands conjugated mean conj(A)."""
+    if not (conjA ^ conjB) and A.itags and B.itags and len(ia):
+        if all(A.itags[a].endswith("*") == B.itags[b].endswith("*") for a, b in zip(ia, ib)):
+            conjA = 1
+    Ar, Ai = _planes(A)
+    Br, Bi = _planes(B)
+
+    def one(X, Y):
+        return contract(X, ia, Y, ib, perm=perm, conjA=conjA, conjB=conjB, fermA=fermA, fermB=fermB,
+                        skip_zero=0)
+
+    R1 = one(Ar, Br)
+    sA = -1.0 if conjA else 1.0
+    sB = -1.0 if conjB else 1.0
+    re = [np.asarray(d, dtype=np.float64) for d in R1.data]
+    im = [np.zeros_like(d) for d in re]
+    if Ai is not None and Bi is not None:
+        re = [x + (-sA * sB) * y for x, y in zip(re, one(Ai, Bi).data)]
+    if Bi is not None:
+        im = [sB * y for y in one(Ar, Bi).data]
+    if Ai is not None:
+        R4 = one(Ai, Br).data
+        im = [sA * y for y in R4] if Bi is None else [x + sA * y for x, y in zip(im, R4)]
+    data = [x + 1j * y for x, y in zip(re, im)]
+    keep = list(range(len(data)))
+    if skip_zero and data:
+        rc = R1.rank
+        blockdiag = rc == 2 and all(np.array_equal(R1.Q[0][i], R1.Q[1][i]) for i in range(len(data)))
+        if not blockdiag:
+            keep = [i for i, d in enumerate(data) if np.any(d != 0)]
+    cg = R1.info.get("cgw")
+    T = QSpace([q[keep] for q in R1.Q], [data[i] for i in keep], R1.qtype, R1.itags,
+               None if cg is None else cg[keep])
+    T.npairs = R1.npairs
+    T.flops = R1.flops * ((Ai is not None and Bi is not None) + (Ai is not None) + (Bi is not None) + 1)
     return T
 
 

@@ -303,6 +303,9 @@ def _cell_dev(items, level, mode, kw, cache, plans):
     ops = []
     for a in items:
         if isinstance(a, QSpace):
+            if a.is_complex():
+                raise _capi.QsxError(5, "cell-form contractQS takes real QSpaces; contract complex ones pairwise "
+                                        "(contractQS(A, ica, B, icb))")
             if id(a) not in cache:
                 cache[id(a)] = _DevT(a, DeviceBuffer.from_array(a.packed()), keep=a)
             ops.append({"T": cache[id(a)], "conj": 0, "sidx": None, "xflag": False})

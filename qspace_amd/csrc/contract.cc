@@ -90,6 +90,7 @@ TensorDesc make_desc(const qsx_tensor* t, const char* name) {
     T.cgw_given = true;
   }
   T.hdata = t->data;
+  T.hdata_im = t->data_im;
   // QIDX rows must be unique (QSpace.hh:3214-3215)
   if (T.n > 1 && rq) {
     std::vector<int> p(T.n);
@@ -670,7 +671,8 @@ void plan_execute(CtrPlan& P, const double* dA, const double* dB, double* dC, hi
 }
 
 // ---------------------------------------------------------------- one-shot
-static qsx_tensor_out* make_out(const CtrPlan& P, const std::vector<int>& keep, const std::vector<double>* data) {
+static qsx_tensor_out* make_out(const CtrPlan& P, const std::vector<int>& keep, const std::vector<double>* data,
+                                const std::vector<double>* data_im = nullptr) {
   auto* o = (qsx_tensor_out*)calloc(1, sizeof(qsx_tensor_out));
   const int q = P.A.qdim, rc = P.rc;
   int n = (int)keep.size();
@@ -700,20 +702,145 @@ static qsx_tensor_out* make_out(const CtrPlan& P, const std::vector<int>& keep, 
       std::copy(data->begin() + P.offsOut[c], data->begin() + P.offsOut[c + 1], o->data + o->offs[i]);
     }
   }
+  if (data_im) {
+    o->data_im = (double*)malloc(sizeof(double) * std::max<int64_t>(1, o->offs[n]));
+    for (int i = 0; i < n; ++i) {
+      int c = keep[i];
+      std::copy(data_im->begin() + P.offsOut[c], data_im->begin() + P.offsOut[c + 1], o->data_im + o->offs[i]);
+    }
+  }
   o->npairs = (int64_t)P.pairs.size();
   o->useful_flops = P.flops;
   return o;
 }
 
-static void upload_tensor(const TensorDesc& T, double* d, hipStream_t s) {
-  if (!T.hdata) fail(QSX_ERR_ARG, "tensor without data");
+static void upload_tensor(const TensorDesc& T, double* d, hipStream_t s, bool imag = false) {
+  const double* const* h = imag ? T.hdata_im : T.hdata;
+  if (!h) fail(QSX_ERR_ARG, "tensor without data");
   std::vector<double> buf(std::max<int64_t>(1, T.total()));
   for (int i = 0; i < T.n; ++i) {
-    if (!T.hdata[i] && T.numel(i)) fail(QSX_ERR_ARG, "null data block %d", i + 1);
-    if (T.numel(i)) memcpy(buf.data() + T.offs[i], T.hdata[i], sizeof(double) * T.numel(i));
+    if (!h[i] && T.numel(i)) fail(QSX_ERR_ARG, "null data block %d", i + 1);
+    if (T.numel(i)) memcpy(buf.data() + T.offs[i], h[i], sizeof(double) * T.numel(i));
   }
   QSX_HIP(hipMemcpyAsync(d, buf.data(), sizeof(double) * T.total(), hipMemcpyHostToDevice, s));
   QSX_HIP(hipStreamSynchronize(s));
+}
+
+// device buffers freed on scope exit
+struct DevBufs {
+  std::vector<void*> p;
+  double* add(int64_t n) {
+    p.push_back(dev_alloc(sizeof(double) * std::max<int64_t>(1, n)));
+    return (double*)p.back();
+  }
+  ~DevBufs() {
+    for (void* q : p) dev_free(q);
+  }
+};
+
+// complex operands (the zgemm path of contractDATA_group): the structure and
+// its plan do not depend on the data, so one plan runs on the real and
+// imaginary planes and a linear combination forms the result,
+//   Re C = C(Ar,Br) - sA sB C(Ai,Bi),  Im C = sB C(Ar,Bi) + sA C(Ai,Br),
+// with sX = -1 when operand X carries a conj flag (its data are conjugated)
+static qsx_tensor_out* contract_oneshot_complex(const qsx_tensor* A, const int32_t* ia, int na, int conjA,
+                                                const qsx_tensor* B, const int32_t* ib, int nb, int conjB,
+                                                const int32_t* perm, int nperm, const qsx_opts* o) {
+  hipStream_t s = default_stream();
+  auto P = plan_create(A, ia, na, conjA, B, ib, nb, conjB, perm, nperm, o, true);
+  const bool ai = P->A.hdata_im != nullptr, bi = P->B.hdata_im != nullptr;
+  // the plan may have applied the missing-conj rule (contract_getIdxSet)
+  const double sA = P->conjA ? -1.0 : 1.0, sB = P->conjB ? -1.0 : 1.0;
+  const int64_t nC = P->offsOut.back();
+  DevBufs m;
+  double* dAr = m.add(P->A.total());
+  double* dBr = m.add(P->B.total());
+  double* dAi = ai ? m.add(P->A.total()) : nullptr;
+  double* dBi = bi ? m.add(P->B.total()) : nullptr;
+  upload_tensor(P->A, dAr, s);
+  upload_tensor(P->B, dBr, s);
+  if (ai) upload_tensor(P->A, dAi, s, true);
+  if (bi) upload_tensor(P->B, dBi, s, true);
+  double* C1 = m.add(nC);                          // Ar Br
+  double* C2 = ai && bi ? m.add(nC) : nullptr;     // Ai Bi
+  double* C3 = bi ? m.add(nC) : nullptr;           // Ar Bi
+  double* C4 = ai ? m.add(nC) : nullptr;           // Ai Br
+  double* Cr = m.add(nC);
+  double* Ci = m.add(nC);
+  hipEvent_t e0, e1;
+  QSX_HIP(hipEventCreate(&e0));
+  QSX_HIP(hipEventCreate(&e1));
+  QSX_HIP(hipEventRecord(e0, s));
+  plan_execute(*P, dAr, dBr, C1, s);
+  if (C2) plan_execute(*P, dAi, dBi, C2, s);
+  if (C3) plan_execute(*P, dAr, dBi, C3, s);
+  if (C4) plan_execute(*P, dAi, dBr, C4, s);
+  // Re C and Im C in one streaming launch each (offsets relative to the
+  // two source buffers); a missing term is the other source with factor 0
+  auto combine = [&](double* dst, double* x, double fx, double* y, double fy) {
+    std::vector<AxpbyItem> items;
+    for (int64_t e0i = 0; e0i < nC; e0i += 4096) {
+      AxpbyItem it{};
+      it.c_off = e0i;
+      it.a_off = x ? e0i : -1;
+      it.b_off = y ? e0i : -1;
+      it.fa = fx;
+      it.fb = fy;
+      it.n = (int32_t)std::min<int64_t>(4096, nC - e0i);
+      items.push_back(it);
+    }
+    if (items.empty()) return;
+    AxpbyItem* d = (AxpbyItem*)dev_alloc(sizeof(AxpbyItem) * items.size());
+    m.p.push_back(d);
+    QSX_HIP(hipMemcpyAsync(d, items.data(), sizeof(AxpbyItem) * items.size(), hipMemcpyHostToDevice, s));
+    launch_axpby(d, (int)items.size(), x ? x : y, y ? y : x, dst, s);
+  };
+  combine(Cr, C1, 1.0, C2, -sA * sB);
+  if (C3 || C4) combine(Ci, C3, sB, C4, sA);
+  else launch_fill_zero(Ci, nC, s);
+  QSX_HIP(hipEventRecord(e1, s));
+  QSX_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  QSX_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  // SkipZeroData: a block is zero when both planes are
+  std::vector<int> keep(P->nC);
+  std::iota(keep.begin(), keep.end(), 0);
+  const int skip_zero = o ? o->skip_zero : 1;
+  if (skip_zero && P->nC) {
+    bool blockdiag = false;
+    const int q = P->A.qdim;
+    if (P->rc == 2) {
+      blockdiag = true;
+      for (int c = 0; c < P->nC && blockdiag; ++c)
+        for (int j = 0; j < q; ++j)
+          if (P->Qout[(size_t)c * 2 * q + j] != P->Qout[(size_t)c * 2 * q + q + j]) blockdiag = false;
+    }
+    if (!(P->rc == 2 && blockdiag)) {
+      int32_t* dz = (int32_t*)dev_alloc(sizeof(int32_t) * 2 * P->nC);
+      m.p.push_back(dz);
+      int64_t* doffs = (int64_t*)dev_alloc(sizeof(int64_t) * (P->nC + 1));
+      m.p.push_back(doffs);
+      QSX_HIP(hipMemcpyAsync(doffs, P->offsOut.data(), sizeof(int64_t) * (P->nC + 1), hipMemcpyHostToDevice, s));
+      launch_block_zero_flags(doffs, P->nC, Cr, dz, s);
+      launch_block_zero_flags(doffs, P->nC, Ci, dz + P->nC, s);
+      std::vector<int32_t> z(2 * P->nC);
+      QSX_HIP(hipMemcpyAsync(z.data(), dz, sizeof(int32_t) * 2 * P->nC, hipMemcpyDeviceToHost, s));
+      QSX_HIP(hipStreamSynchronize(s));
+      keep.clear();
+      for (int c = 0; c < P->nC; ++c)
+        if (!(z[c] && z[P->nC + c])) keep.push_back(c);
+    }
+  }
+  std::vector<double> hr(std::max<int64_t>(1, nC)), hi(std::max<int64_t>(1, nC));
+  QSX_HIP(hipMemcpyAsync(hr.data(), Cr, sizeof(double) * nC, hipMemcpyDeviceToHost, s));
+  QSX_HIP(hipMemcpyAsync(hi.data(), Ci, sizeof(double) * nC, hipMemcpyDeviceToHost, s));
+  QSX_HIP(hipStreamSynchronize(s));
+  qsx_tensor_out* out = make_out(*P, keep, &hr, &hi);
+  out->gpu_ms = ms;
+  out->useful_flops = P->flops * ((C2 ? 1 : 0) + (C3 ? 1 : 0) + (C4 ? 1 : 0) + 1);
+  return out;
 }
 
 static qsx_tensor_out* contract_oneshot(const qsx_tensor* A, const int32_t* ia, int na, int conjA, const qsx_tensor* B,
@@ -727,6 +854,8 @@ static qsx_tensor_out* contract_oneshot(const qsx_tensor* A, const int32_t* ia, 
     out->qdim = A->qdim;
     return out;
   }
+  if (A && B && (A->data_im || B->data_im))
+    return contract_oneshot_complex(A, ia, na, conjA, B, ib, nb, conjB, perm, nperm, o);
   auto P = plan_create(A, ia, na, conjA, B, ib, nb, conjB, perm, nperm, o, true);
   double *dA = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, P->A.total()));
   double *dB = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, P->B.total()));
@@ -898,6 +1027,7 @@ void qsx_tensor_free(qsx_tensor_out* t) {
   free(t->data);
   free(t->itags);
   free(t->cgw);
+  free(t->data_im);
   free(t);
 }
 

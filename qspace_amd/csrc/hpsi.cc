@@ -656,6 +656,8 @@ struct PhaseClock {
 
 static void hpsi_create(HpsiPlan& H, const qsx_tensor* HL, const qsx_tensor* psi, const qsx_tensor* HR,
                         const qsx_opts* o, int hconj) {
+  for (const qsx_tensor* t : {HL, psi, HR})
+    if (t && t->data_im) fail(QSX_ERR_UNSUPPORTED, "H*psi plans take real tensors (complex data not supported)");
   ensure_device();
   PhaseClock clk{o ? o->verbose : 0};
   hipStream_t s = default_stream();

@@ -255,6 +255,7 @@ qsx_tensor_out* copy_out(const TensorDesc& T, const std::vector<double>& data, d
 }
 
 qsx_tensor_out* permute_oneshot(const qsx_tensor* At, const int32_t* perm, int nperm, int conj) {
+  if (At && At->data_im) fail(QSX_ERR_UNSUPPORTED, "permuteQS: complex data not supported (real QSpaces only)");
   ensure_device();
   hipStream_t s = default_stream();
   TensorDesc A = make_desc(At, "A");
@@ -340,6 +341,8 @@ bool tags_equal(const std::vector<uint64_t>& a, const std::vector<uint64_t>& b) 
 }
 
 qsx_tensor_out* plus_oneshot(const qsx_tensor* At, const qsx_tensor* Bt, double bfac, int skip_zero) {
+  if ((At && At->data_im) || (Bt && Bt->data_im))
+    fail(QSX_ERR_UNSUPPORTED, "plusQS: complex data not supported (real QSpaces only)");
   ensure_device();
   hipStream_t s = default_stream();
   TensorDesc A = make_desc(At, "A"), B = make_desc(Bt, "B");

@@ -41,6 +41,7 @@ struct TensorDesc {
   std::vector<double> cgw;           // n x nsym
   bool cgw_given = false;
   const double* const* hdata = nullptr;  // host block pointers (may be null)
+  const double* const* hdata_im = nullptr;  // imaginary parts (complex data) or null
   int64_t numel(int i) const { return offs[i + 1] - offs[i]; }
   int64_t total() const { return offs.empty() ? 0 : offs.back(); }
   bool has_cg() const {

@@ -31,6 +31,7 @@ class QsxTensor(C.Structure):
         ("Q", C.POINTER(C.c_int32)), ("dims", C.POINTER(C.c_int64)),
         ("data", C.POINTER(C.POINTER(C.c_double))), ("itags", C.POINTER(C.c_uint64)),
         ("qtype", C.c_char_p), ("cgw", C.POINTER(C.c_double)),
+        ("data_im", C.POINTER(C.POINTER(C.c_double))),
     ]
 
 
@@ -40,7 +41,7 @@ class QsxTensorOut(C.Structure):
         ("Q", C.POINTER(C.c_int32)), ("dims", C.POINTER(C.c_int64)), ("offs", C.POINTER(C.c_int64)),
         ("data", C.POINTER(C.c_double)), ("itags", C.POINTER(C.c_uint64)), ("cgw", C.POINTER(C.c_double)),
         ("qtype", C.c_char * 32), ("npairs", C.c_int64), ("useful_flops", C.c_double),
-        ("gpu_ms", C.c_double),
+        ("gpu_ms", C.c_double), ("data_im", C.POINTER(C.c_double)),
     ]
 
 
@@ -128,7 +129,8 @@ class QSpace:
         r = len(self.Q)
         self.data = []
         for d in data:
-            a = np.asarray(d, dtype=np.float64)
+            a = np.asarray(d)
+            a = a.astype(np.complex128 if np.iscomplexobj(a) else np.float64, copy=False)
             if a.ndim < r:
                 a = a.reshape(a.shape + (1,) * (r - a.ndim), order="F")
             self.data.append(np.asfortranarray(a))
@@ -171,7 +173,10 @@ class QSpace:
         return int(sum(d.size for d in self.data))
 
     def norm(self) -> float:
-        return float(np.sqrt(sum(float(np.sum(d * d)) for d in self.data)))
+        return float(np.sqrt(sum(float(np.sum(np.abs(d) ** 2)) for d in self.data)))
+
+    def is_complex(self) -> bool:
+        return any(np.iscomplexobj(d) for d in self.data)
 
     def packed(self) -> np.ndarray:
         """blocks back to back, column-major (the device layout)."""
@@ -209,14 +214,24 @@ def to_c(T: QSpace, with_data: bool = True):
     ct.nblk, ct.rank, ct.qdim = n, r, T.qdim
     ct.Q = Q.ctypes.data_as(C.POINTER(C.c_int32))
     ct.dims = dims.ctypes.data_as(C.POINTER(C.c_int64))
+    ct.data_im = None
     if with_data:
+        cplx = T.is_complex()
         ptrs = (C.POINTER(C.c_double) * max(1, n))()
+        pim = (C.POINTER(C.c_double) * max(1, n))() if cplx else None
         for i, d in enumerate(T.data):
-            a = np.asfortranarray(d, dtype=np.float64)
+            a = np.asfortranarray(np.real(d), dtype=np.float64)
             keep.append(a)
             ptrs[i] = a.ctypes.data_as(C.POINTER(C.c_double))
+            if cplx:
+                b = np.asfortranarray(np.imag(d), dtype=np.float64)
+                keep.append(b)
+                pim[i] = b.ctypes.data_as(C.POINTER(C.c_double))
         keep.append(ptrs)
         ct.data = C.cast(ptrs, C.POINTER(C.POINTER(C.c_double)))
+        if cplx:
+            keep.append(pim)
+            ct.data_im = C.cast(pim, C.POINTER(C.POINTER(C.c_double)))
     else:
         ct.data = None
     if T.itags:
@@ -245,9 +260,12 @@ def from_out(o: QsxTensorOut, with_data: bool = True) -> QSpace:
     data = []
     if with_data and o.data:
         buf = np.ctypeslib.as_array(o.data, shape=(max(1, int(offs[-1])),))
+        bim = np.ctypeslib.as_array(o.data_im, shape=(max(1, int(offs[-1])),)) if o.data_im else None
         for i in range(n):
             shp = tuple(int(x) for x in dims[i]) if r else ()
-            data.append(buf[offs[i]: offs[i + 1]].copy().reshape(shp, order="F"))
+            re = buf[offs[i]: offs[i + 1]]
+            v = re + 1j * bim[offs[i]: offs[i + 1]] if bim is not None else re.copy()
+            data.append(v.reshape(shp, order="F"))
     else:
         for i in range(n):
             data.append(np.zeros(tuple(int(x) for x in dims[i]), order="F"))

@@ -51,7 +51,7 @@ def expand(T, layouts):
     assert len(su2) <= 1
     j = su2[0] if su2 else None
     r = T.rank
-    full = np.zeros(tuple(L[1] for L in layouts))
+    full = np.zeros(tuple(L[1] for L in layouts), dtype=np.result_type(np.float64, *[d.dtype for d in T.data]))
     dirs = _dirs(T)
     cgw = T.info.get("cgw")
     for i, d in enumerate(T.data):

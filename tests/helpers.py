@@ -4,8 +4,8 @@ import numpy as np
 
 def rel_err(A, B):
     """relative Frobenius error over all blocks (same structure assumed)."""
-    num = sum(float(np.sum((a - b) ** 2)) for a, b in zip(A.data, B.data))
-    den = sum(float(np.sum(b * b)) for b in B.data)
+    num = sum(float(np.sum(np.abs(a - b) ** 2)) for a, b in zip(A.data, B.data))
+    den = sum(float(np.sum(np.abs(b) ** 2)) for b in B.data)
     return np.sqrt(num / max(den, 1e-300))
 
 

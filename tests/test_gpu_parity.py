@@ -340,3 +340,47 @@ def test_env_update_u1_cell_form(gpu):
     assert C.itags == ["R", "R*", "w2*"]
     assert_same_structure(C, R)
     assert rel_err(C, R) < TOL
+
+
+def _complexify(T, seed, real_blocks=False):
+    rng = np.random.RandomState(seed)
+    data = [np.asarray(d, dtype=np.complex128) if real_blocks and i % 2 else d + 1j * rng.standard_normal(d.shape)
+            for i, d in enumerate(T.data)]
+    return Q.QSpace(T.Q, [np.asfortranarray(d) for d in data], T.qtype, T.itags, T.info.get("cgw"))
+
+
+def test_complex_u1_mps_overlap(gpu):
+    """complex workload u1_mps: contractQS(A,'12*',A,'12') conjugates the
+    data of the first operand (device vs oracle by bilinearity)"""
+    A = _complexify(gen.u1_mps(seed=21), 1)
+    C = Q.contractQS(A, "12*", A, "12")
+    R = O.contract(A, [0, 1], A, [0, 1], conjA=1)
+    assert C.is_complex()
+    assert_same_structure(C, R)
+    assert rel_err(C, R) < TOL
+
+
+@pytest.mark.parametrize("which", ["both", "A", "B"])
+def test_complex_u1_perm_conjB(gpu, which):
+    A0 = gen.u1_mps(seed=22)
+    A = _complexify(A0, 2, True) if which in ("both", "A") else A0
+    B = _complexify(A0, 3) if which in ("both", "B") else A0
+    C, R = _both(A, [2], B, [2], perm=[2, 0, 1, 3], conjB=1)
+    assert_same_structure(C, R)
+    assert rel_err(C, R) < TOL
+
+
+def test_complex_su2_env(gpu):
+    """complex SU(2) L.A then X.conj(A), and the missing-conj rule"""
+    A, L = gen.su2_mps(seed=23, tS=tuple(range(7)), dmin=4, dmax=12)
+    A = _complexify(A, 4)
+    L = _complexify(L, 5, True)
+    X, XR = _both(L, [1], A, [0])
+    assert_same_structure(X, XR)
+    assert rel_err(X, XR) < TOL
+    C, CR = _both(XR, [0, 1], A, [0, 1], conjB=1)
+    assert_same_structure(C, CR)
+    assert rel_err(C, CR) < TOL
+    N, NR = _both(A, [0, 1], A, [0, 1])
+    assert_same_structure(N, NR)
+    assert rel_err(N, NR) < TOL

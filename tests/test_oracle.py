@@ -247,3 +247,74 @@ def test_skip_zero_and_perm():
     for d in C.data:
         assert np.any(d != 0)
     assert C.itags == ["R", "L*", "L", "R*"]
+
+
+# ---------------------------------------------------------------- complex data
+def _complexify(T, seed, imag_only_some=False):
+    """T with an imaginary part drawn per block (every other block left real
+    when imag_only_some)"""
+    rng = np.random.RandomState(seed)
+    data = []
+    for i, d in enumerate(T.data):
+        if imag_only_some and i % 2:
+            data.append(np.asarray(d, dtype=np.complex128))
+        else:
+            data.append(np.asfortranarray(d + 1j * rng.standard_normal(d.shape)))
+    return QSpace(T.Q, data, T.qtype, T.itags, T.info.get("cgw"))
+
+
+def _conj_data(T):
+    return QSpace(T.Q, [np.conj(d) for d in T.data], T.qtype, T.itags, T.info.get("cgw"))
+
+
+def test_complex_u1_overlap_full_space():
+    """complex U(1) MPS overlap conj(A).A and A.conj(B) against the dense
+    contraction of the full tensors (the conj flag conjugates the data)"""
+    A = _complexify(gen.u1_mps(seed=41, Lq=(-2, 0, 2), Rq=(-3, -1, 1, 3), dmin=2, dmax=4), 1)
+    rng = np.random.RandomState(6)
+    B = _complexify(QSpace(A.Q, [rng.standard_normal(d.shape) for d in A.data], A.qtype, A.itags), 2, True)
+    C = O.contract(A, [0, 1], A, [0, 1], conjA=1)
+    assert C.is_complex()
+    assert check_contraction(_conj_data(A), [0, 1], A, [0, 1], C) < 1e-13
+    C2 = O.contract(A, [2], B, [2], conjB=1)
+    assert check_contraction(A, [2], _conj_data(B), [2], C2) < 1e-13
+
+
+def test_complex_su2_full_space():
+    """complex SU(2) L.A and X.conj(A) against the dense contraction"""
+    A, L = gen.su2_mps(seed=43, tS=tuple(range(5)), dmin=2, dmax=4)
+    A = _complexify(A, 3)
+    L = _complexify(L, 4, True)
+    X = O.contract(L, [1], A, [0])
+    assert check_contraction(L, [1], A, [0], X) < 1e-13
+    C = O.contract(X, [0, 1], A, [0, 1], conjB=1)
+    assert check_contraction(X, [0, 1], _conj_data(A), [0, 1], C) < 1e-13
+
+
+def test_complex_missing_conj_rule():
+    """contracted legs of equal direction and no conj flag: the reference
+    applies conj(A), to the data too"""
+    A = _complexify(gen.u1_mps(seed=44, Lq=(-2, 0, 2), Rq=(-3, -1, 1, 3), dmin=2, dmax=3), 5)
+    C = O.contract(A, [0, 1], A, [0, 1])
+    R = O.contract(A, [0, 1], A, [0, 1], conjA=1)
+    assert C.itags == R.itags
+    for a, b in zip(C.data, R.data):
+        assert np.array_equal(a, b)
+
+
+def test_complex_real_imag_zero_matches_real():
+    """zero imaginary parts: the complex path reproduces the real result
+    bit for bit; a purely imaginary A gives i times the real contraction"""
+    A, L = gen.su2_mps(seed=45, tS=tuple(range(5)), dmin=2, dmax=4)
+    Ac = QSpace(A.Q, [np.asarray(d, dtype=np.complex128) for d in A.data], A.qtype, A.itags, A.info.get("cgw"))
+    R = O.contract(L, [1], A, [0])
+    C = O.contract(L, [1], Ac, [0])
+    assert C.nblk == R.nblk and np.array_equal(C.QIDX, R.QIDX)
+    for a, b in zip(C.data, R.data):
+        assert np.array_equal(a.real, b) and not np.any(a.imag)
+    Ai = QSpace(A.Q, [1j * np.asarray(d) for d in A.data], A.qtype, A.itags, A.info.get("cgw"))
+    C2 = O.contract(Ai, [0, 1], A, [0, 1], conjA=1)
+    R2 = O.contract(A, [0, 1], A, [0, 1], conjA=1)
+    assert C2.nblk == R2.nblk > 0
+    for a, b in zip(C2.data, R2.data):
+        assert np.array_equal(a.imag, -b) and not np.any(a.real)
