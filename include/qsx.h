@@ -24,6 +24,9 @@
  *   qsx_block_zero_flags <- SkipZeroData's zero test, Source/QSpace.cc:3200-3270
  *   qsx_permute         <- permuteQS, Source/permuteQS.cc:25-87
  *   qsx_plus            <- plusQS, Source/plusQS.cc:17-87
+ *   qsx_ortho           <- orthoQS core (SVD_Data::blockSVD + dmrgTruncate),
+ *                          Source/mpsortho.cc:545-954,1031-1078; called by
+ *                          mpsOrthoQS after the eigensolve, eigs2SiteQS.cc:481-484
  *   qsx_su2_cg / qsx_su2_x3
  *                       <- SU(2) Clebsch-Gordan data and X-symbols of
  *                          X3Map::contractCGR, Source/clebsch_aux.cc:4073-4510
@@ -162,6 +165,38 @@ int qsx_permute(const qsx_tensor* A, const int32_t* perm, int32_t nperm, int32_t
    may be NULL).  C->gpu_ms = device time of the sum. */
 int qsx_plus(const qsx_tensor* A, const qsx_tensor* B, double bfac, const qsx_opts* opts,
              qsx_tensor_out** C, char* err, size_t errlen);
+
+/* statistics of a truncation (the Sout struct of dmrgTruncate,
+   Source/mpsortho.cc:758-769); release with qsx_ortho_info_free */
+typedef struct qsx_ortho_info {
+  int64_t ntot;      /* singular values (all groups) */
+  int64_t nkeep;     /* kept */
+  int32_t nkmin;
+  int32_t flag;      /* number of notes dmrgTruncate would print */
+  double stol;
+  double svd2tr;     /* discarded weight / total weight */
+  double sv2tot;     /* total weight, sum of squared singular values */
+  double sfac;       /* norm readjustment of the kept singular values */
+  double* svd;       /* ntot singular values, descending */
+  double gpu_ms;     /* device time (assembly, SVD, result blocks) */
+  int32_t sweeps;    /* largest number of Jacobi sweeps of a group */
+  int32_t ngroups;   /* symmetry groups (block matrices) */
+} qsx_ortho_info;
+
+/* orthoQS(PSI, K, Nkmin, Nkeep, stol, itag) <- Source/mpsortho.cc:1031-1078
+   (SVD_Data::blockSVD, :545-613, and dmrgTruncate, :615-954): PSI's blocks
+   are grouped by the label sum over legs [0] (K=1), [r-1] (K=r-1) or
+   [0..K-1]; each group's block matrix (legs 0..K-1 against the rest) is
+   decomposed; the kept singular values (markSet 'desc': at most nkeep,
+   those above stol, at least nkmin; nkeep < 0: no limit) give
+   A1 = U (legs 0..K-1, new leg) and A2 = V S sfac (legs K..r-1, new leg),
+   PSI ~ A1 . A2 over the new leg.  itag names the new leg (NULL or "":
+   the reference's default).  Abelian symmetries and real data only
+   (QSX_ERR_UNSUPPORTED otherwise, and for a K = r-1 split whose last leg is
+   incoming, which the reference answers with an identity-CG flip). */
+int qsx_ortho(const qsx_tensor* psi, int32_t K, int32_t nkmin, int64_t nkeep, double stol, const char* itag,
+              qsx_tensor_out** A1, qsx_tensor_out** A2, qsx_ortho_info** info, char* err, size_t errlen);
+void qsx_ortho_info_free(qsx_ortho_info* info);
 
 /* parse a ctr-idx string ("13", "1,3", "[1 3]*") into 0-based indices;
    returns the number of indices (>0) or a negative error */

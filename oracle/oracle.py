@@ -473,6 +473,251 @@ def plus(A: QSpace, B: QSpace, bfac=1.0, skip_zero=1) -> QSpace:
                   np.array([cgw[i] for i in keep]).reshape(len(keep), -1) if keep else None)
 
 
+# ------------------------------------------------------------- orthoQS (SVD)
+def _mark_set_desc(E, Nkmin, Nkeep, Etrunc):
+    """markSet(F,L,E,mark,Nkmin,Nkeep,Etrunc,b=0,dmax,'desc')
+    (Source/gathered.cc:1957-2079) on E sorted ascending: marks the states
+    kept from the top; returns (mark, Nkeep)"""
+    N = len(E)
+    mark = np.zeros(N, dtype=bool)
+    if Nkeep < 0 or Nkeep > N:
+        Nkeep = N
+    if Nkeep == 0:
+        return mark, 0
+    if Nkeep >= N and Etrunc <= 0:
+        mark[:] = True
+        return mark, N
+    if Etrunc > 0:
+        lx = N - Nkmin if Nkmin < N else 0
+        l = -1  # the unsigned loop running past index 0
+        for i in range(N - 1, -1, -1):
+            if i and E[i - 1] > E[i]:
+                raise OracleError(1, "data not sorted")
+            if E[i] <= Etrunc and i < lx:
+                l = i
+                break
+        # (N-1-l) > Nkeep in unsigned arithmetic: l = -1 counts as N
+        if (N if l < 0 else N - 1 - l) > Nkeep:
+            l = N - 1 - Nkeep
+    else:
+        l = N - 1 - Nkeep
+    if l < 0:
+        mark[:] = True
+        return mark, N
+    if l == N - 1:
+        return mark, 0
+    mark[l + 1:] = True
+    return mark, N - 1 - l
+
+
+def _mark_set(SV, Nkmin, Nkeep, Etrunc):
+    """markSet(E0,mark,Nkmin,Nkeep,Etrunc,E,eps=0,...,'desc')
+    (Source/gathered.cc:1822-1884): all values sorted ascending (stable,
+    Wb::hpsort), the inner markSet, marks mapped back per block"""
+    E = np.concatenate(SV) if SV else np.zeros(0)
+    N = len(E)
+    marks = [np.zeros(len(s), dtype=bool) for s in SV]
+    if N == 0:
+        return marks, Nkeep
+    if Nkeep < 0:
+        Nkeep = N
+    if Nkeep == 0:
+        return marks, 0
+    if Nkeep >= N and Etrunc <= 0:
+        return [np.ones(len(s), dtype=bool) for s in SV], N
+    P = np.argsort(E, kind="stable")
+    mm, Nkeep = _mark_set_desc(E[P], Nkmin, Nkeep, Etrunc)
+    m = np.zeros(N, dtype=bool)
+    m[P] = mm
+    o = 0
+    for i, s in enumerate(SV):
+        marks[i] = m[o:o + len(s)]
+        o += len(s)
+    return marks, Nkeep
+
+
+def _tag_conj(t):
+    return t[:-1] if t.endswith("*") else t + "*"
+
+
+def ortho(PSI: QSpace, K, Nkmin=0, Nkeep=-1, stol=0.0, itag=""):
+    """orthoQS(PSI,A1,A2,K,Nkmin,Nkeep,stol,tx) -- Source/mpsortho.cc:1031-1078:
+    SVD_Data::blockSVD (:545-613) then dmrgTruncate (:615-954), for abelian
+    symmetries.  Returns (A1, A2, info):
+      * blocks grouped by the label sum over legs [0] (K=1), [r-1] (K=r-1) or
+        [0..K-1] (QSpace::getQsum, Source/QSpace.hh:2385-2420), groups in
+        stable ascending order (groupRecs);
+      * per group the block matrix of QBlock::toBlockMatrix (:30-256): rows
+        the sorted distinct label records of legs 0..K-1, columns those of
+        legs K..r-1, each block in its column-major fused form;
+      * economy SVD per group (wbSVD -> LAPACK gesvd, Source/wbarray_blas.cc:
+        1090-1180), singular values descending;
+      * markSet 'desc' over all singular values (Nkmin, Nkeep, stol);
+        discarded weight s2t, norm readjusted by sfac = |S| / sqrt(|S|^2-s2t)
+        when anything is discarded;
+      * A1 = U[:, kept] per row record with legs (legs 0..K-1, new);
+        A2 = (V S sfac)[:, kept] per column record with legs (legs K..r-1,
+        new); itags tU / tV (:786-818); A2's blocks with norm <= stol*1e-14
+        dropped unless A2 is a rank-2 block-diagonal matrix (SkipZeroData
+        (xtol2,'b'), Source/QSpace.cc:3201-3270).
+    info: svd (all singular values, descending), svd2tr, sv2tot, sfac,
+    Nkeep, Ntot, flag (dmrgTruncate's count of notes)."""
+    r, n = PSI.rank, PSI.nblk
+    if any(s != "A" for s in _syms(PSI)):
+        raise OracleError(5, "non-abelian symmetries (the CGT of the new leg is not restated)")
+    if PSI.is_complex():
+        raise OracleError(5, "complex data")
+    if n == 0:
+        raise OracleError(1, "got empty PSI")
+    if not (1 <= K < r):
+        raise OracleError(1, f"index specs out of bounds ({K}/{r})")
+    tags = list(PSI.itags)
+    if len(tags) != r:
+        raise OracleError(2, "got invalid/empty itags")
+    dirs = _tag_dirs(tags, r)
+    if K != 1:
+        if any(dirs[i] <= 0 for i in range(K)):
+            raise OracleError(2, "only ingoing indices can be fused into outgoing")
+        if any(dirs[i] != dirs[K] for i in range(K + 1, r)):
+            raise OracleError(2, "2nd/right group is mixed in+out")
+    Ib = [0] if K == 1 else ([r - 1] if K == r - 1 else list(range(K)))
+    Qb = [tuple(int(sum(PSI.Q[j][i, q] for j in Ib)) for q in range(PSI.qdim)) for i in range(n)]
+    groups = _sorted_groups(Qb)
+    QB = []
+    for g in groups:
+        rows = [tuple(int(x) for j in range(K) for x in PSI.Q[j][i]) for i in g]
+        cols = [tuple(int(x) for j in range(K, r) for x in PSI.Q[j][i]) for i in g]
+        g1 = _sorted_groups(rows)
+        g2 = _sorted_groups(cols)
+        ri = {}
+        for a, idx in enumerate(g1):
+            for t in idx:
+                ri[t] = a
+        ci = {}
+        for b, idx in enumerate(g2):
+            for t in idx:
+                ci[t] = b
+        S1 = [None] * len(g1)
+        S2 = [None] * len(g2)
+        seen = set()
+        for t, i in enumerate(g):
+            a, b = ri[t], ci[t]
+            if (a, b) in seen:
+                raise OracleError(4, "block not unique")
+            seen.add((a, b))
+            s = PSI.data[i].shape + (1,) * (r - PSI.data[i].ndim)
+            s1, s2 = tuple(s[:K]), tuple(s[K:r])
+            if S1[a] is None:
+                S1[a] = s1
+            elif S1[a] != s1:
+                raise OracleError(4, "block size mismatch")
+            if S2[b] is None:
+                S2[b] = s2
+            elif S2[b] != s2:
+                raise OracleError(4, "block size mismatch")
+        D1 = [int(np.prod(s)) for s in S1]
+        D2 = [int(np.prod(s)) for s in S2]
+        cD1 = np.concatenate([[0], np.cumsum(D1)]).astype(int)
+        cD2 = np.concatenate([[0], np.cumsum(D2)]).astype(int)
+        MM = np.zeros((cD1[-1], cD2[-1]))
+        for t, i in enumerate(g):
+            a, b = ri[t], ci[t]
+            MM[cD1[a]:cD1[a + 1], cD2[b]:cD2[b + 1]] = np.asarray(PSI.data[i]).reshape(D1[a], D2[b], order="F")
+        U, S, Vt = np.linalg.svd(MM, full_matrices=False)
+        QB.append(dict(g=g, rows=[rows[x[0]] for x in g1], cols=[cols[x[0]] for x in g2], S1=S1, S2=S2,
+                       cD1=cD1, cD2=cD2, U=U, S=S, Vc=Vt.T, key=Qb[g[0]]))
+    SV = [b["S"] for b in QB]
+    N = sum(len(s) for s in SV)
+    if N == 0:
+        raise OracleError(1, "got empty SVD data")
+    snorm2 = sum(float(np.sum(s * s)) for s in SV)
+    SM = np.sort(np.concatenate(SV))[::-1]
+    marks, NK = _mark_set(SV, max(Nkmin, 0), Nkeep, stol)
+    s2t = sum(float(np.sum(s[~m] ** 2)) for s, m in zip(SV, marks))
+    nk = sum(int(m.sum()) for m in marks)
+    flag = 0
+    xtol2 = stol * 1e-14 if stol > 0 else 0.0
+    unlimited = Nkeep < 0
+    if SM[-1] > stol:
+        if not unlimited and len(SM) > Nkeep:
+            if stol:
+                flag += 1
+        else:
+            xtol2 = 0.0
+    else:
+        if not unlimited and NK < Nkeep:
+            pass
+        elif NK < len(SM):
+            flag += 1
+    sfac = 1.0
+    if s2t:
+        if snorm2 > s2t:
+            sfac = np.sqrt(snorm2) / np.sqrt(snorm2 - s2t)
+            if abs(sfac - 1) > 0.01:
+                flag += 1
+        elif snorm2 != s2t:
+            raise OracleError(1, "discarded weight exceeds the norm")
+    # itags of the new leg (dmrgTruncate, Source/mpsortho.cc:786-818)
+    tx = itag.rstrip("*")
+    if r - K == 1:
+        last = tags[-1]
+        tu = last if not tx else (tx + "*" if last.endswith("*") else tx)
+        if not tu.endswith("*") and nk:
+            is_op = r == 2 and tags[0].endswith("*") != tags[1].endswith("*")
+            if not is_op:
+                raise OracleError(5, "new leg incoming (the reference's identity-CG flip, revert_Uidx, "
+                                     "is not restated)")
+    elif K == 1:
+        first = tags[0]
+        tu = first if not tx else (tx + "*" if first.endswith("*") else tx)
+        tu = _tag_conj(tu)
+    else:
+        tu = tx if tags[0].endswith("*") else tx + "*"
+    tU = tags[:K] + [tu]
+    tV = tags[K:] + [_tag_conj(tu)]
+    QU, DU, QV, DV = [[] for _ in range(K + 1)], [], [[] for _ in range(r - K + 1)], []
+    q = PSI.qdim
+    for b, m in zip(QB, marks):
+        Ik = np.nonzero(m)[0]
+        if not len(Ik):
+            continue
+        if r - K == 1:
+            new = b["cols"][0]
+        else:
+            left = b["rows"][0]
+            new = tuple(sum(left[j * q + x] for j in range(K)) for x in range(q))
+        for a, rec in enumerate(b["rows"]):
+            if b["cD1"][a + 1] == b["cD1"][a]:
+                continue  # empty sub-block: no block (initFromBlockMatrix)
+            for j in range(K):
+                QU[j].append(rec[j * q:(j + 1) * q])
+            QU[K].append(new)
+            blk = b["U"][b["cD1"][a]:b["cD1"][a + 1], Ik]
+            DU.append(np.asfortranarray(blk.reshape(b["S1"][a] + (len(Ik),), order="F")))
+        VS = b["Vc"][:, Ik] * (b["S"][Ik] * sfac)[None, :]
+        XQ, XD = [], []
+        for c, rec in enumerate(b["cols"]):
+            if b["cD2"][c + 1] == b["cD2"][c]:
+                continue
+            XQ.append([rec[j * q:(j + 1) * q] for j in range(r - K)] + [new])
+            blk = VS[b["cD2"][c]:b["cD2"][c + 1], :]
+            XD.append(np.asfortranarray(blk.reshape(b["S2"][c] + (len(Ik),), order="F")))
+        blockdiag = r - K + 1 == 2 and all(tuple(x[0]) == tuple(x[1]) for x in XQ)
+        for xq, xd in zip(XQ, XD):
+            if not blockdiag and float(np.sum(xd * xd)) <= xtol2 * xtol2:
+                continue
+            for j in range(r - K + 1):
+                QV[j].append(xq[j])
+            DV.append(xd)
+
+    def mk(Qs, D, rank, tg):
+        return QSpace([np.array(x, dtype=np.int32).reshape(len(D), q) for x in Qs], D, PSI.qtype, tg)
+
+    info = dict(svd=SM, svd2tr=s2t / snorm2, sv2tot=snorm2, sfac=sfac, Nkeep=nk, Ntot=len(SM), flag=flag,
+                Nkmin=max(Nkmin, 0), stol=stol)
+    return mk(QU, DU, K + 1, tU), mk(QV, DV, r - K + 1, tV), info
+
+
 def find_openblas():
     """numpy's bundled ILP64 OpenBLAS (path, dgemm symbol, threads symbol)."""
     import glob

@@ -10,6 +10,14 @@ import os
 
 from .qspace import QsxOpts, QsxTensor, QsxTensorOut
 
+
+class QsxOrthoInfo(C.Structure):
+    """qsx_ortho_info (include/qsx.h)"""
+    _fields_ = [("ntot", C.c_int64), ("nkeep", C.c_int64), ("nkmin", C.c_int32), ("flag", C.c_int32),
+                ("stol", C.c_double), ("svd2tr", C.c_double), ("sv2tot", C.c_double), ("sfac", C.c_double),
+                ("svd", C.POINTER(C.c_double)), ("gpu_ms", C.c_double), ("sweeps", C.c_int32),
+                ("ngroups", C.c_int32)]
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libqsx.so")
 
@@ -53,6 +61,9 @@ def lib():
         "qsx_tensor_free": ([P(QsxTensorOut)], None),
         "qsx_permute": ([P(QsxTensor), P(i32), i32, i32, P(P(QsxTensorOut)), cs, sz], C.c_int),
         "qsx_plus": ([P(QsxTensor), P(QsxTensor), f64, P(QsxOpts), P(P(QsxTensorOut)), cs, sz], C.c_int),
+        "qsx_ortho": ([P(QsxTensor), i32, i32, i64, f64, cs, P(P(QsxTensorOut)), P(P(QsxTensorOut)),
+                       P(P(QsxOrthoInfo)), cs, sz], C.c_int),
+        "qsx_ortho_info_free": ([P(QsxOrthoInfo)], None),
         "qsx_parse_ctridx": ([cs, P(i32), i32, P(i32)], C.c_int),
         "qsx_packed_offsets": ([P(QsxTensor), P(i64)], C.c_int),
         "qsx_ctr_plan_create": ([P(QsxTensor), P(i32), i32, i32, P(QsxTensor), P(i32), i32, i32, P(i32), i32,
@@ -94,7 +105,7 @@ def exported_symbols() -> list[str]:
     """names declared in include/qsx.h that the loaded library must export."""
     return ["qsx_version", "qsx_device_init", "qsx_malloc_device", "qsx_free_device", "qsx_memcpy_h2d",
             "qsx_memcpy_d2h", "qsx_memset_device", "qsx_synchronize", "qsx_copy_device", "qsx_contract", "qsx_tensor_free",
-            "qsx_permute", "qsx_plus",
+            "qsx_permute", "qsx_plus", "qsx_ortho", "qsx_ortho_info_free",
             "qsx_parse_ctridx", "qsx_packed_offsets", "qsx_ctr_plan_create", "qsx_ctr_plan_output",
             "qsx_ctr_plan_pairs", "qsx_ctr_execute", "qsx_ctr_plan_flops", "qsx_ctr_plan_set_policy", "qsx_ctr_plan_destroy", "qsx_block_zero_flags",
             "qsx_hpsi_plan_create", "qsx_hpsi_size", "qsx_hpsi_flops", "qsx_hpsi_bytes", "qsx_hpsi_apply",

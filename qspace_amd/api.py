@@ -16,7 +16,7 @@ import numpy as np
 from . import _capi
 from .qspace import QSpace, QsxOpts, QsxTensorOut, from_out, parse_ctridx, to_c
 
-__all__ = ["contractQS", "permuteQS", "plusQS", "DeviceBuffer", "CtrPlan", "HPsi", "su2_cg", "su2_x3", "make_opts"]
+__all__ = ["contractQS", "permuteQS", "plusQS", "orthoQS", "DeviceBuffer", "CtrPlan", "HPsi", "su2_cg", "su2_x3", "make_opts"]
 
 
 def make_opts(mode="batched", nslices=0, skip_zero=1, hconj=0, fermA=None, fermB=None, verbose=1):
@@ -396,6 +396,35 @@ def plusQS(A: QSpace, B: QSpace, bfac: float = 1.0, skip_zero: int = 1) -> QSpac
     rc = L.qsx_plus(C.byref(ca), C.byref(cb), float(bfac), C.byref(opts), C.byref(out), err, 1024)
     _capi.check(rc, err)
     return _out_to_qspace(out)
+
+
+def orthoQS(PSI: QSpace, K: int, Nkeep: int = -1, stol: float = 0.0, Nkmin: int = 0, itag: str = ""):
+    """orthoQS(PSI, K, Nkeep, stol, Nkmin, itag) -- the truncation core of
+    Source/mpsortho.cc:1031-1078 (blockSVD + dmrgTruncate) on the device.
+
+    Returns (A1, A2, info): A1 = U with legs (PSI legs 0..K-1, new leg),
+    A2 = V S sfac with legs (PSI legs K..r-1, new leg), PSI ~ A1 . A2 over
+    the new leg; info holds svd (all singular values, descending), Nkeep,
+    Ntot, svd2tr, sv2tot, sfac, flag, and the device time.  ``Nkeep < 0``:
+    no limit.  Abelian symmetries, real data."""
+    L = _capi.lib()
+    ca, ka = to_c(PSI)
+    o1 = C.POINTER(QsxTensorOut)()
+    o2 = C.POINTER(QsxTensorOut)()
+    oi = C.POINTER(_capi.QsxOrthoInfo)()
+    err = C.create_string_buffer(1024)
+    rc = L.qsx_ortho(C.byref(ca), int(K), int(Nkmin), int(Nkeep), float(stol), (itag or "").encode(),
+                     C.byref(o1), C.byref(o2), C.byref(oi), err, 1024)
+    _capi.check(rc, err)
+    try:
+        x = oi.contents
+        info = dict(svd=np.ctypeslib.as_array(x.svd, shape=(max(1, x.ntot),))[:x.ntot].copy(), Nkeep=int(x.nkeep),
+                    Ntot=int(x.ntot), Nkmin=int(x.nkmin), flag=int(x.flag), stol=float(x.stol),
+                    svd2tr=float(x.svd2tr), sv2tot=float(x.sv2tot), sfac=float(x.sfac), gpu_ms=float(x.gpu_ms),
+                    sweeps=int(x.sweeps), ngroups=int(x.ngroups))
+    finally:
+        L.qsx_ortho_info_free(oi)
+    return _out_to_qspace(o1), _out_to_qspace(o2), info
 
 
 class DeviceBuffer:

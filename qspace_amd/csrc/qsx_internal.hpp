@@ -315,4 +315,38 @@ void device_group_records(const std::vector<int32_t>& R, int n, int m, std::vect
 void launch_block_zero_flags(const int64_t* doffs, int nblk, const double* data, int32_t* flags, hipStream_t s);
 void launch_fill_zero(double* p, int64_t n, hipStream_t s);
 
+// ---------------------------------------------------------------- block SVD
+// (svd.hip) group matrices of the truncation, one-sided Jacobi SVD, result
+// blocks.  A group matrix is stored m x n column-major with m >= n (the
+// transpose of a wide matrix); it is decomposed in LDS when
+// m*n + n*n <= SVD_LDS_DOUBLES, in place in the work buffer otherwise.
+constexpr int SVD_LDS_DOUBLES = 20224;
+// block -> its place in a group matrix (transposed for a wide group)
+struct SvdGather {
+  int64_t src_off, dst_off;
+  int32_t d1, d2, ld, trans;
+};
+// one group matrix: A (m x n) at a_off, V (n x n) at v_off, singular values
+// (unsorted) at s_off; on return A holds the left vectors
+struct SvdItem {
+  int64_t a_off, v_off, s_off;
+  int32_t m, n, lds, pad;
+};
+struct SvdCol {
+  int32_t col, pad;
+  double scale;
+};
+// result block (rows x ncols, column-major) from the kept columns
+// cols[col0 .. col0+ncols-1] of the matrix at src_off (leading dim ld)
+struct SvdScatter {
+  int64_t dst_off, src_off;
+  int32_t rows, ncols, ld, col0;
+};
+void launch_svd_gather(const SvdGather* ditems, int nitems, const double* src, double* work, hipStream_t s);
+// sweeps[i] = sweeps that rotated (jacobi_max_sweeps(): not converged)
+void launch_jacobi_svd(const SvdItem* ditems, int nitems, double* work, double* sig, int32_t* sweeps, hipStream_t s);
+int jacobi_max_sweeps();
+void launch_svd_scatter(const SvdScatter* ditems, int nitems, const SvdCol* dcols, const double* work, double* out,
+                        hipStream_t s);
+
 }  // namespace qsx

@@ -194,3 +194,28 @@ def u1_env_update(seed=1006, Lq=tuple(range(-6, 7, 2)), Rq=tuple(range(-7, 8, 2)
                     wd.append(_normal_block(rng, (dw, dw, 1, 1)))
     W = QSpace([np.array(q) for q in wq], wd, "A", ["w", "w2*", "s*", "s"])
     return A, HK, W
+
+
+# ---------------------------------------------------------------------------
+def u1_two_site_psi(seed=1007, Lq=tuple(range(-10, 11, 2)), Rq=tuple(range(-10, 11, 2)), dmin=4, dmax=32,
+                    itags=("L", "s1", "s2*", "R*")):
+    """Rank-4 U(1) two-site wave function psi(L,s1,s2,R) (spin-1/2 sites,
+    charges 2Sz = +-1, dim 1) with qL + qs1 = qs2 + qR when the itags are
+    (in, in, out, out); the group of legs (L,s1) meets (s2,R) in the
+    truncation, which the reference requires to share one direction."""
+    rng = _rng(seed)
+    dL = {q: int(rng.randint(dmin, dmax + 1)) for q in Lq}
+    dR = {q: int(rng.randint(dmin, dmax + 1)) for q in Rq}
+    sgn = [(-1 if t.endswith("*") else 1) for t in itags]
+    Q = [[], [], [], []]
+    data = []
+    for qL in Lq:
+        for q1 in (-1, 1):
+            for q2 in (-1, 1):
+                for qR in Rq:
+                    if sgn[0] * qL + sgn[1] * q1 + sgn[2] * q2 + sgn[3] * qR != 0:
+                        continue
+                    for k, v in enumerate((qL, q1, q2, qR)):
+                        Q[k].append([v])
+                    data.append(_normal_block(rng, (dL[qL], 1, 1, dR[qR])))
+    return QSpace([np.array(q) for q in Q], data, "A", list(itags))
