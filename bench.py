@@ -18,8 +18,9 @@ configurations 1 (U(1) MPS overlap, CPU oracle and GPU), 2 (SU(2)
 environment step), 3 (Heisenberg D*=2000, 20 Lanczos iterations), 4 (split
 precision: BF16-slice H*psi error/time table on the headline operator) and 5
 (SU(2)xU(1) Hubbard D*=4000 with parity signs, 30 Davidson iterations),
-and "block_ops": permuteQS / plusQS bandwidth on the SU(2) environments and
-the U(1) environment update in the cell form of updateHK.m.
+and "block_ops": permuteQS / plusQS bandwidth on the SU(2) environments,
+the U(1) environment update in the cell form of updateHK.m and the
+orthoQS truncation of a U(1) two-site wave function.
 """
 from __future__ import annotations
 
@@ -358,6 +359,45 @@ def env_update():
     return out
 
 
+def truncation():
+    """orthoQS (block SVD truncation after the eigensolve, mpsOrthoQS) of a
+    U(1) two-site wave function psi(L,s1,s2*,R*) split between (L,s1) and
+    (s2,R): 11 charge sectors per bond, "lds" with U[4,32] states per sector
+    (group matrices up to 64 x 64, decomposed in LDS), "bond1000" with
+    U[40,140] (about 1000 states per bond, group matrices about 180 x 180,
+    decomposed in place in HBM/L2); Nkeep = 512.  GPU: device time of the
+    call (events: assembly, Jacobi SVD, result blocks) and its wall time,
+    best of 3; CPU: the oracle (numpy LAPACK gesdd), wall time."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    import qspace_amd as Q
+    from qspace_amd import gen
+    out = {}
+    for name, dmin, dmax in (("lds", 4, 32), ("bond1000", 40, 140)):
+        P = gen.u1_two_site_psi(seed=1009, dmin=dmin, dmax=dmax)
+        Q.orthoQS(P, 2, Nkeep=512)
+        best, wall = None, []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            A1, A2, info = Q.orthoQS(P, 2, Nkeep=512)
+            wall.append((time.perf_counter() - t0) * 1e3)
+            best = info if best is None or info["gpu_ms"] < best["gpu_ms"] else best
+        t0 = time.perf_counter()
+        R1, R2, rinfo = O.ortho(P, 2, Nkeep=512)
+        cpu_ms = (time.perf_counter() - t0) * 1e3
+        smax = float(rinfo["svd"][0])
+        X = O.contract(A1, [2], A2, [2])
+        XR = O.contract(R1, [2], R2, [2])
+        out[f"u1_two_site_psi_{name}"] = {
+            "blocks": P.nblk, "elements": P.numel(), "groups": best["ngroups"], "sweeps": best["sweeps"],
+            "Ntot": best["Ntot"], "Nkeep": best["Nkeep"], "gpu_ms": best["gpu_ms"], "gpu_wall_ms": min(wall),
+            "oracle_cpu_ms": cpu_ms,
+            "svd_max_err_rel": float(np.abs(best["svd"] - rinfo["svd"]).max() / smax),
+            "product_rel_err_vs_oracle": _rel(X, XR)}
+        log(f"[bench] truncation {name}: {out[f'u1_two_site_psi_{name}']}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -386,6 +426,7 @@ def main():
     if args.block_ops:
         bops = block_ops()
         bops["u1_env_update"] = env_update()
+        bops["truncation"] = truncation()
         print(json.dumps({"block_ops": bops}), flush=True)
         return
     w = WORKLOADS[args.workload]
@@ -477,6 +518,7 @@ def main():
         configs = extra_configs()
         bops = block_ops()
         bops["u1_env_update"] = env_update()
+        bops["truncation"] = truncation()
     cpu = None
     if not args.no_cpu and not args.lite:
         log("[bench] cpu baseline (sample) ...")

@@ -110,6 +110,7 @@ TensorDesc make_desc(const qsx_tensor* t, const char* name) {
 // from any thread; the others wait)
 static std::atomic<int> g_dev{-1};
 static hipStream_t g_stream = nullptr;
+static hipStream_t g_side = nullptr;  // independent work beside g_stream (same priority)
 static std::mutex g_dev_mu;
 
 void ensure_device() {
@@ -121,12 +122,18 @@ void ensure_device() {
   int dev = -1;
   QSX_HIP(hipGetDevice(&dev));
   QSX_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+  QSX_HIP(hipStreamCreateWithFlags(&g_side, hipStreamNonBlocking));
   g_dev.store(dev, std::memory_order_release);
 }
 
 hipStream_t default_stream() {
   ensure_device();
   return g_stream;
+}
+
+hipStream_t side_stream() {
+  ensure_device();
+  return g_side;
 }
 
 // resident workgroups of the persistent grouped GEMM: 4 per compute unit
@@ -1006,6 +1013,7 @@ int qsx_copy_device(double* dst, const double* src, int64_t n, void* stream) {
 
 int qsx_synchronize(void) {
   if (g_stream && hipStreamSynchronize(g_stream) != hipSuccess) return QSX_ERR_DEVICE;
+  if (g_side && hipStreamSynchronize(g_side) != hipSuccess) return QSX_ERR_DEVICE;
   return hipDeviceSynchronize() == hipSuccess ? 0 : QSX_ERR_DEVICE;
 }
 

@@ -77,6 +77,9 @@ void match_itags(const TensorDesc& A, const TensorDesc& B, int conjA, int conjB,
                  std::vector<int32_t>& ib, int& conjA_out);
 
 hipStream_t default_stream();
+// a second stream of the library (default priority) for work that may
+// overlap the default stream's; joined back with events
+hipStream_t side_stream();
 void ensure_device();
 void* dev_alloc(size_t bytes);
 int resident_slots();

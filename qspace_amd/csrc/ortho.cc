@@ -279,23 +279,131 @@ OrthoResult ortho_oneshot(const qsx_tensor* pt, int K, int Nkmin, int64_t Nkeep,
       for (void* x : p) dev_free(x);
     }
   } fr;
-  int32_t* dsw = (int32_t*)fr.add(sizeof(int32_t) * items.size());
+  // groups that fit in LDS: one workgroup each (k_jacobi_svd); the others:
+  // block one-sided Jacobi over column blocks of at most cb columns, two
+  // blocks (and their columns of V) staged in LDS per workgroup
+  std::vector<SvdItem> small;
+  std::vector<int32_t> big, nbk, cbk;
+  for (size_t i = 0; i < items.size(); ++i) {
+    if (items[i].lds) {
+      small.push_back(items[i]);
+      continue;
+    }
+    const int64_t m = items[i].m, n = items[i].n;
+    const int64_t cmax = SVD_LDS_DOUBLES / (2 * (m + n));
+    if (cmax < 1) fail(QSX_ERR_UNSUPPORTED, "orthoQS: group matrix %lld x %lld too large", (long long)m, (long long)n);
+    int64_t nb = (n + cmax - 1) / cmax;
+    nb += nb & 1;
+    if (nb < 2) nb = 2;
+    const int64_t cb = (n + nb - 1) / nb;
+    if (2 * cb * (m + n) > SVD_LDS_DOUBLES) fail(QSX_ERR_STRUCT, "orthoQS: internal column block too large");
+    big.push_back((int32_t)i);
+    nbk.push_back((int32_t)nb);
+    cbk.push_back((int32_t)cb);
+  }
+  int32_t* dsw = (int32_t*)fr.add(sizeof(int32_t) * std::max<size_t>(1, small.size()));
+  SvdItem* dsmall = (SvdItem*)fr.add(sizeof(SvdItem) * std::max<size_t>(1, small.size()));
   SvdItem* ditems = (SvdItem*)fr.add(sizeof(SvdItem) * items.size());
   SvdGather* dgath = (SvdGather*)fr.add(sizeof(SvdGather) * std::max<size_t>(1, gath.size()));
+  int32_t* drot = (int32_t*)fr.add(sizeof(int32_t) * items.size());
+  int32_t* dbig = (int32_t*)fr.add(sizeof(int32_t) * std::max<size_t>(1, big.size()));
   QSX_HIP(hipMemcpyAsync(dpsi.p, host.data(), sizeof(double) * P.total(), hipMemcpyHostToDevice, s));
   QSX_HIP(hipMemcpyAsync(ditems, items.data(), sizeof(SvdItem) * items.size(), hipMemcpyHostToDevice, s));
+  if (!small.empty())
+    QSX_HIP(hipMemcpyAsync(dsmall, small.data(), sizeof(SvdItem) * small.size(), hipMemcpyHostToDevice, s));
+  if (!big.empty())
+    QSX_HIP(hipMemcpyAsync(dbig, big.data(), sizeof(int32_t) * big.size(), hipMemcpyHostToDevice, s));
   if (!gath.empty())
     QSX_HIP(hipMemcpyAsync(dgath, gath.data(), sizeof(SvdGather) * gath.size(), hipMemcpyHostToDevice, s));
   Timer tm(s);
   launch_fill_zero(dwork.p, work, s);
   launch_svd_gather(dgath, (int)gath.size(), dpsi.p, dwork.p, s);
-  launch_jacobi_svd(ditems, (int)items.size(), dwork.p, dsig.p, dsw, s);
-  std::vector<double> sig(nsig);
-  std::vector<int32_t> sweeps(items.size());
-  QSX_HIP(hipMemcpyAsync(sig.data(), dsig.p, sizeof(double) * nsig, hipMemcpyDeviceToHost, s));
-  QSX_HIP(hipMemcpyAsync(sweeps.data(), dsw, sizeof(int32_t) * items.size(), hipMemcpyDeviceToHost, s));
-  QSX_HIP(hipStreamSynchronize(s));
+  // the groups in LDS run on the side stream, beside the block steps
+  hipStream_t s2 = side_stream();
+  struct Ev {
+    hipEvent_t e = nullptr;
+    Ev() { QSX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming)); }
+    ~Ev() { (void)hipEventDestroy(e); }
+  } ready, done;
+  QSX_HIP(hipEventRecord(ready.e, s));
+  QSX_HIP(hipStreamWaitEvent(s2, ready.e, 0));
+  launch_jacobi_svd(dsmall, (int)small.size(), dwork.p, dsig.p, dsw, s2);
+  QSX_HIP(hipEventRecord(done.e, s2));
   int maxsw = 0;
+  if (!big.empty()) {
+    launch_svd_init_v(ditems, dbig, (int)big.size(), dwork.p, s);
+    std::vector<int> active(big.size());
+    std::iota(active.begin(), active.end(), 0);
+    SvdBlockStep* dsteps = nullptr;
+    size_t cap = 0;
+    int sw = 0;
+    std::vector<int32_t> rot(items.size());
+    for (; sw < jacobi_max_sweeps() && !active.empty(); ++sw) {
+      // a sweep: the pairs within every column block (one launch), then
+      // the pairs across blocks P, Q in the round-robin order of the blocks
+      // (nb - 1 launches of nb / 2 disjoint block pairs per group)
+      std::vector<SvdBlockStep> steps;
+      std::vector<size_t> off(1, 0);
+      int maxnb = 0;
+      for (int a : active) maxnb = std::max(maxnb, nbk[a]);
+      for (int a : active) {
+        const int nb = nbk[a], cb = cbk[a], n = items[big[a]].n;
+        for (int k = 0; k < nb; ++k) {
+          SvdBlockStep b{};
+          b.grp = big[a];
+          b.p0 = std::min(n, k * cb);
+          b.p1 = std::min(n, (k + 1) * cb);
+          b.q0 = b.q1 = b.p1;
+          if (b.p1 - b.p0 >= 2) steps.push_back(b);
+        }
+      }
+      off.push_back(steps.size());
+      for (int st = 0; st < maxnb - 1; ++st) {
+        for (int a : active) {
+          const int nb = nbk[a], cb = cbk[a], n = items[big[a]].n;
+          if (st >= nb - 1) continue;
+          for (int k = 0; k < nb / 2; ++k) {
+            const int P0 = k == 0 ? nb - 1 : (st + k) % (nb - 1);
+            const int Q0 = k == 0 ? st : (st - k + nb - 1) % (nb - 1);
+            SvdBlockStep b{};
+            b.grp = big[a];
+            b.p0 = std::min(n, P0 * cb);
+            b.p1 = std::min(n, (P0 + 1) * cb);
+            b.q0 = std::min(n, Q0 * cb);
+            b.q1 = std::min(n, (Q0 + 1) * cb);
+            if (b.p1 == b.p0 || b.q1 == b.q0) continue;
+            steps.push_back(b);
+          }
+        }
+        off.push_back(steps.size());
+      }
+      if (steps.size() > cap) {
+        cap = steps.size();
+        dsteps = (SvdBlockStep*)fr.add(sizeof(SvdBlockStep) * cap);
+      }
+      if (!steps.empty())
+        QSX_HIP(hipMemcpyAsync(dsteps, steps.data(), sizeof(SvdBlockStep) * steps.size(), hipMemcpyHostToDevice, s));
+      QSX_HIP(hipMemsetAsync(drot, 0, sizeof(int32_t) * items.size(), s));
+      for (size_t st = 0; st + 1 < off.size(); ++st)
+        launch_jacobi_block(ditems, dsteps + off[st], (int)(off[st + 1] - off[st]), dwork.p, drot, s);
+      QSX_HIP(hipMemcpyAsync(rot.data(), drot, sizeof(int32_t) * items.size(), hipMemcpyDeviceToHost, s));
+      QSX_HIP(hipStreamSynchronize(s));
+      std::vector<int> next;
+      for (int a : active)
+        if (rot[big[a]]) next.push_back(a);
+      active.swap(next);
+    }
+    if (!active.empty()) fail(QSX_ERR_ARG, "orthoQS: block Jacobi SVD did not converge in %d sweeps", sw);
+    maxsw = sw;
+    launch_svd_finalize(ditems, dbig, (int)big.size(), dwork.p, dsig.p, s);
+  }
+  QSX_HIP(hipStreamWaitEvent(s, done.e, 0));
+  std::vector<double> sig(nsig);
+  std::vector<int32_t> sweeps(small.size());
+  QSX_HIP(hipMemcpyAsync(sig.data(), dsig.p, sizeof(double) * nsig, hipMemcpyDeviceToHost, s));
+  if (!small.empty())
+    QSX_HIP(hipMemcpyAsync(sweeps.data(), dsw, sizeof(int32_t) * small.size(), hipMemcpyDeviceToHost, s));
+  QSX_HIP(hipStreamSynchronize(s));
   for (int x : sweeps) {
     if (x >= jacobi_max_sweeps()) fail(QSX_ERR_ARG, "orthoQS: Jacobi SVD did not converge in %d sweeps", x);
     maxsw = std::max(maxsw, x + 1);

@@ -348,5 +348,16 @@ void launch_jacobi_svd(const SvdItem* ditems, int nitems, double* work, double* 
 int jacobi_max_sweeps();
 void launch_svd_scatter(const SvdScatter* ditems, int nitems, const SvdCol* dcols, const double* work, double* out,
                         hipStream_t s);
+// block one-sided Jacobi of group matrices beyond the LDS: one step = the
+// column-block pairs (P, Q) of a round-robin over a group's column blocks,
+// one workgroup each; rot[grp] set when a pair rotated
+struct SvdBlockStep {
+  int32_t grp, p0, p1, q0, q1, pad;
+};
+void launch_jacobi_block(const SvdItem* dgroups, const SvdBlockStep* dsteps, int nsteps, double* work, int32_t* rot,
+                         hipStream_t s);
+void launch_svd_finalize(const SvdItem* dgroups, const int32_t* dwhich, int n, double* work, double* sig,
+                         hipStream_t s);
+void launch_svd_init_v(const SvdItem* dgroups, const int32_t* dwhich, int n, double* work, hipStream_t s);
 
 }  // namespace qsx
