@@ -285,7 +285,7 @@ OrthoResult ortho_oneshot(const qsx_tensor* pt, int K, int Nkmin, int64_t Nkeep,
   std::vector<SvdItem> small;
   std::vector<int32_t> big, nbk, cbk;
   for (size_t i = 0; i < items.size(); ++i) {
-    if (items[i].lds) {
+    if (items[i].lds) {  // checked above: fits in LDS
       small.push_back(items[i]);
       continue;
     }

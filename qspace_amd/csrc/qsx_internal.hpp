@@ -318,8 +318,9 @@ void launch_fill_zero(double* p, int64_t n, hipStream_t s);
 // ---------------------------------------------------------------- block SVD
 // (svd.hip) group matrices of the truncation, one-sided Jacobi SVD, result
 // blocks.  A group matrix is stored m x n column-major with m >= n (the
-// transpose of a wide matrix); it is decomposed in LDS when
-// m*n + n*n <= SVD_LDS_DOUBLES, in place in the work buffer otherwise.
+// transpose of a wide matrix); it is decomposed by one workgroup in LDS
+// when m*n + n*n <= SVD_LDS_DOUBLES (k_jacobi_svd takes only such items),
+// by column-block steps otherwise.
 constexpr int SVD_LDS_DOUBLES = 20224;
 // block -> its place in a group matrix (transposed for a wide group)
 struct SvdGather {

@@ -248,10 +248,11 @@ __global__ __launch_bounds__(SVD_NT) void k_jacobi_svd(const SvdItem* __restrict
   const SvdItem it = items[blockIdx.x];
   const int m = it.m, n = it.n;  // m >= n >= 1
   const int tid = threadIdx.x;
-  double* A = it.lds ? lds : work + it.a_off;
-  double* V = it.lds ? lds + (int64_t)m * n : work + it.v_off;
-  if (it.lds)
-    for (int64_t e = tid; e < (int64_t)m * n; e += SVD_NT) A[e] = work[it.a_off + e];
+  // LDS only (m*n + n*n <= SVD_LDS_DOUBLES, checked by the host), so that
+  // every access is an LDS instruction
+  double* A = lds;
+  double* V = lds + (int64_t)m * n;
+  for (int64_t e = tid; e < (int64_t)m * n; e += SVD_NT) A[e] = work[it.a_off + e];
   for (int64_t e = tid; e < (int64_t)n * n; e += SVD_NT) V[e] = (e % (n + 1)) == 0 ? 1.0 : 0.0;
   __syncthreads();
 
@@ -273,10 +274,8 @@ __global__ __launch_bounds__(SVD_NT) void k_jacobi_svd(const SvdItem* __restrict
     if (lane == 0) sig[it.s_off + j] = sg;
   }
   __syncthreads();
-  if (it.lds) {
-    for (int64_t e = tid; e < (int64_t)m * n; e += SVD_NT) work[it.a_off + e] = A[e];
-    for (int64_t e = tid; e < (int64_t)n * n; e += SVD_NT) work[it.v_off + e] = V[e];
-  }
+  for (int64_t e = tid; e < (int64_t)m * n; e += SVD_NT) work[it.a_off + e] = A[e];
+  for (int64_t e = tid; e < (int64_t)n * n; e += SVD_NT) work[it.v_off + e] = V[e];
   if (tid == 0) sweeps[blockIdx.x] = sw;
 }
 
