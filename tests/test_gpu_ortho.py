@@ -114,3 +114,18 @@ def test_ortho_unsupported_flip(gpu):
     with pytest.raises(Q.QsxError) as e:
         Q.orthoQS(M2, 2)
     assert e.value.code == 5
+
+
+def test_ortho_edge_cases(gpu):
+    """nothing kept (Nkeep = 0; stol above every value), a group with a zero
+    block, a rank-2 block-diagonal psi (K = 1 = r-1)"""
+    from test_oracle_ortho import u1_rank2
+    P = _tensor("psi")
+    for kw in ({"Nkeep": 0}, {"Nkeep": 50, "stol": 1e3}):
+        A1, A2, info = _check(P, 2, **kw)
+        assert A1.nblk == 0 and A2.nblk == 0 and info["Nkeep"] == 0
+    Z = gen.u1_two_site_psi(seed=5, Lq=(-4, -2, 0, 2, 4), Rq=(-4, -2, 0, 2, 4), dmin=2, dmax=7)
+    Z = Q.QSpace(Z.Q, [np.zeros_like(d) if i == 3 else d for i, d in enumerate(Z.data)], Z.qtype, Z.itags)
+    _check(Z, 2, Nkeep=20)
+    _check(u1_rank2(), 1, Nkeep=5)
+    _check(u1_rank2(seed=6, qs=(-4, -2, 0, 2, 4), dmin=20, dmax=150), 1, Nkeep=200, stol=1e-3)

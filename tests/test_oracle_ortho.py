@@ -167,3 +167,52 @@ def test_errors():
     with pytest.raises(O.OracleError) as e:
         O.ortho(M2, 2)
     assert e.value.code == 5
+
+
+def u1_rank2(seed=4, qs=(-2, 0, 2), dmin=2, dmax=6):
+    """block-diagonal U(1) matrix psi(L, R*), one block per charge"""
+    rng = np.random.RandomState(seed)
+    Q, data = [[], []], []
+    for q in qs:
+        d1, d2 = int(rng.randint(dmin, dmax + 1)), int(rng.randint(dmin, dmax + 1))
+        Q[0].append([q])
+        Q[1].append([q])
+        data.append(rng.standard_normal((d1, d2)))
+    return QSpace([np.array(x) for x in Q], data, "A", ["L", "R*"])
+
+
+def test_nothing_kept():
+    P = _psi()
+    for kw in ({"Nkeep": 0}, {"Nkeep": 50, "stol": 1e3}):
+        A1, A2, info = O.ortho(P, 2, **kw)
+        assert A1.nblk == 0 and A2.nblk == 0 and info["Nkeep"] == 0
+        assert info["svd2tr"] == 1.0 and info["sfac"] == 1.0
+        assert A1.itags == ["L", "s1", "*"] and A2.itags == ["s2*", "R*", ""]
+
+
+def test_rank_deficient_group():
+    """a zero block makes its group rank deficient: the zero singular values
+    are reported and never kept ahead of non-zero ones"""
+    P = _psi(5)
+    data = [np.zeros_like(d) if i == 3 else d for i, d in enumerate(P.data)]
+    Z = QSpace(P.Q, data, P.qtype, P.itags)
+    A1, A2, info = O.ortho(Z, 2, Nkeep=20)
+    F, lay = _dense(Z)
+    s = np.linalg.svd(F.reshape(F.shape[0] * F.shape[1], -1, order="F"), compute_uv=False)
+    assert np.allclose(info["svd"][:20], s[:20], atol=1e-12 * s[0])
+    assert info["Nkeep"] == 20
+
+
+def test_rank2_block_diagonal():
+    """rank-2 psi (K = 1 = r-1): the new leg takes the last leg's itag, A2 is
+    a rank-2 block-diagonal matrix (its zero blocks are kept)"""
+    P = u1_rank2()
+    A1, A2, info = O.ortho(P, 1, Nkeep=5)
+    assert A1.itags == ["L", "R*"] and A2.itags == ["R*", "R"]
+    for i in range(A2.nblk):
+        assert A2.Q[0][i, 0] == A2.Q[1][i, 0]
+    X = O.contract(A1, [1], A2, [1])
+    F, lay = _dense(P)
+    U, s, Vt = np.linalg.svd(F)
+    best = (U[:, :5] * s[:5]) @ Vt[:5] * (np.sqrt(np.sum(s * s)) / np.sqrt(np.sum(s[:5] ** 2)))
+    assert np.abs(expand(X, lay) - best).max() < 1e-12 * np.abs(best).max()
