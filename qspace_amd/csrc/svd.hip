@@ -213,11 +213,105 @@ __device__ __forceinline__ bool jacobi_pairs(double* A, double* V, int np, int n
   return any;
 }
 
+// the cross pairs with each segment's own column of P held in registers for
+// the whole schedule: segment k owns column k of P (A and V) and meets the
+// columns of Q one step at a time, so only Q's columns move through LDS
+// (half the LDS traffic of jacobi_pairs).  m, n <= 16 R.
+template <int NT, int R>
+__device__ __forceinline__ bool jacobi_cross_reg(double* A, double* V, int np, int nq, int m, int n, double tol,
+                                                 int* flag) {
+  constexpr int L = SVD_SEG;
+  static_assert(R > 0, "register form");
+  const int tid = threadIdx.x;
+  const int seg = tid / L, lane = tid % L;
+  const int cmax = np > nq ? np : nq;
+  if (tid == 0) *flag = 0;
+  __syncthreads();
+  // one pass of segments over P's columns (np <= NT / 16, the caller's check)
+  const int k = seg;
+  const bool own = k < np;
+  double x[R], u[R];
+  double* ap = A + (int64_t)k * m;
+  double* vp = V + (int64_t)k * n;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int i = lane + L * r;
+    x[r] = own && i < m ? ap[i] : 0.0;
+    u[r] = own && i < n ? vp[i] : 0.0;
+  }
+  for (int step = 0; step < cmax; ++step) {
+    const int j = (k + step) % cmax;
+    if (own && j < nq) {
+      double* aq = A + (int64_t)(np + j) * m;
+      double* vq = V + (int64_t)(np + j) * n;
+      double y[R], z[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int i = lane + L * r;
+        y[r] = i < m ? aq[i] : 0.0;
+      }
+      double al = 0.0, be = 0.0, ga = 0.0;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        al += x[r] * x[r];
+        be += y[r] * y[r];
+        ga += x[r] * y[r];
+      }
+      al = seg_sum(al);
+      be = seg_sum(be);
+      ga = seg_sum(ga);
+      double c, s;
+      if (jacobi_cs(al, be, ga, tol, c, s)) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int i = lane + L * r;
+          const double xn = c * x[r] - s * y[r];
+          if (i < m) aq[i] = s * x[r] + c * y[r];
+          x[r] = xn;
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int i = lane + L * r;
+          z[r] = i < n ? vq[i] : 0.0;
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int i = lane + L * r;
+          const double un = c * u[r] - s * z[r];
+          if (i < n) vq[i] = s * u[r] + c * z[r];
+          u[r] = un;
+        }
+        if (lane == 0) *flag = 1;
+      }
+    }
+    __syncthreads();
+  }
+  if (own) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int i = lane + L * r;
+      if (i < m) ap[i] = x[r];
+      if (i < n) vp[i] = u[r];
+    }
+  }
+  __syncthreads();
+  const bool any = *flag != 0;
+  __syncthreads();  // every thread has read the flag before it is reused
+  return any;
+}
+
 // the variant for column length m (NT threads; 512 leave the registers for
 // 32 elements per lane)
 template <int NT, bool CROSS>
 __device__ __forceinline__ bool jacobi_pairs_m(double* A, double* V, int np, int nq, int m, int n, double tol,
                                                int* flag) {
+  if constexpr (CROSS && NT <= 512) {
+    if (np <= NT / SVD_SEG) {
+      if (m <= 64) return jacobi_cross_reg<NT, 4>(A, V, np, nq, m, n, tol, flag);
+      if (m <= 128) return jacobi_cross_reg<NT, 8>(A, V, np, nq, m, n, tol, flag);
+      if (m <= 256) return jacobi_cross_reg<NT, 16>(A, V, np, nq, m, n, tol, flag);
+    }
+  }
   if (m <= 16) return jacobi_pairs<NT, 1, CROSS>(A, V, np, nq, m, n, tol, flag);
   if (m <= 32) return jacobi_pairs<NT, 2, CROSS>(A, V, np, nq, m, n, tol, flag);
   if (m <= 64) return jacobi_pairs<NT, 4, CROSS>(A, V, np, nq, m, n, tol, flag);
