@@ -325,6 +325,15 @@ OrthoResult ortho_oneshot(const qsx_tensor* pt, int K, int Nkmin, int64_t Nkeep,
     Ev() { QSX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming)); }
     ~Ev() { (void)hipEventDestroy(e); }
   } ready, done;
+  // on an error below, both streams drain before the buffers above are
+  // freed (declared after them, so destroyed first)
+  struct Drain {
+    hipStream_t a, b;
+    ~Drain() {
+      (void)hipStreamSynchronize(a);
+      (void)hipStreamSynchronize(b);
+    }
+  } drain{s, s2};
   QSX_HIP(hipEventRecord(ready.e, s));
   QSX_HIP(hipStreamWaitEvent(s2, ready.e, 0));
   launch_jacobi_svd(dsmall, (int)small.size(), dwork.p, dsig.p, dsw, s2);
@@ -579,13 +588,13 @@ OrthoResult ortho_oneshot(const qsx_tensor* pt, int K, int Nkmin, int64_t Nkeep,
   }
   std::vector<double> out(std::max<int64_t>(1, totU + totX));
   if (!sc.empty()) {
-    DevMem dout(totU + totX);
+    double* dout = (double*)fr.add(sizeof(double) * (totU + totX));  // freed after the drain
     SvdScatter* dsc = (SvdScatter*)fr.add(sizeof(SvdScatter) * sc.size());
     SvdCol* dcol = (SvdCol*)fr.add(sizeof(SvdCol) * cols.size());
     QSX_HIP(hipMemcpyAsync(dsc, sc.data(), sizeof(SvdScatter) * sc.size(), hipMemcpyHostToDevice, s));
     QSX_HIP(hipMemcpyAsync(dcol, cols.data(), sizeof(SvdCol) * cols.size(), hipMemcpyHostToDevice, s));
-    launch_svd_scatter(dsc, (int)sc.size(), dcol, dwork.p, dout.p, s);
-    QSX_HIP(hipMemcpyAsync(out.data(), dout.p, sizeof(double) * (totU + totX), hipMemcpyDeviceToHost, s));
+    launch_svd_scatter(dsc, (int)sc.size(), dcol, dwork.p, dout, s);
+    QSX_HIP(hipMemcpyAsync(out.data(), dout, sizeof(double) * (totU + totX), hipMemcpyDeviceToHost, s));
     QSX_HIP(hipStreamSynchronize(s));
   }
   const float ms = tm.stop(s);
