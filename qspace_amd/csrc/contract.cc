@@ -721,16 +721,49 @@ static qsx_tensor_out* make_out(const CtrPlan& P, const std::vector<int>& keep, 
   return o;
 }
 
-static void upload_tensor(const TensorDesc& T, double* d, hipStream_t s, bool imag = false) {
-  const double* const* h = imag ? T.hdata_im : T.hdata;
+// Host blocks -> one packed device buffer.  The blocks are gathered (OpenMP
+// threads for large tensors) into a pinned staging buffer that the library
+// keeps for the life of the process, then copied in one DMA per chunk: a
+// pageable source would be staged by the driver at a fraction of the link
+// rate, and fresh host vectors page-fault on first touch.
+static std::mutex g_stage_mu;
+static double* g_stage = nullptr;
+static int64_t g_stage_n = 0;
+constexpr int64_t kStageMax = (int64_t)1 << 26;  // 512 MB of doubles per chunk
+
+void upload_packed(const TensorDesc& T, const double* const* h, double* d, hipStream_t s) {
+  const int64_t tot = T.total();
+  if (!tot) return;
   if (!h) fail(QSX_ERR_ARG, "tensor without data");
-  std::vector<double> buf(std::max<int64_t>(1, T.total()));
-  for (int i = 0; i < T.n; ++i) {
+  for (int i = 0; i < T.n; ++i)
     if (!h[i] && T.numel(i)) fail(QSX_ERR_ARG, "null data block %d", i + 1);
-    if (T.numel(i)) memcpy(buf.data() + T.offs[i], h[i], sizeof(double) * T.numel(i));
+  std::lock_guard<std::mutex> lk(g_stage_mu);
+  const int64_t want = std::min(tot, kStageMax);
+  if (g_stage_n < want) {
+    if (g_stage) (void)hipHostFree(g_stage);
+    g_stage = nullptr;
+    g_stage_n = 0;
+    QSX_HIP(hipHostMalloc((void**)&g_stage, sizeof(double) * want, hipHostMallocDefault));
+    g_stage_n = want;
   }
-  QSX_HIP(hipMemcpyAsync(d, buf.data(), sizeof(double) * T.total(), hipMemcpyHostToDevice, s));
-  QSX_HIP(hipStreamSynchronize(s));
+  for (int64_t e0 = 0; e0 < tot; e0 += g_stage_n) {
+    const int64_t e1 = std::min(tot, e0 + g_stage_n);
+    // blocks overlapping [e0, e1)
+    const int b0 = (int)(std::upper_bound(T.offs.begin(), T.offs.end(), e0) - T.offs.begin()) - 1;
+    const int b1 = (int)(std::lower_bound(T.offs.begin(), T.offs.end(), e1) - T.offs.begin());
+    double* st = g_stage;
+#pragma omp parallel for schedule(dynamic, 8) if (e1 - e0 > (1 << 18))
+    for (int i = b0; i < b1; ++i) {
+      const int64_t lo = std::max(e0, T.offs[i]), hi = std::min(e1, T.offs[i + 1]);
+      if (hi > lo) memcpy(st + (lo - e0), h[i] + (lo - T.offs[i]), sizeof(double) * (hi - lo));
+    }
+    QSX_HIP(hipMemcpyAsync(d + e0, st, sizeof(double) * (e1 - e0), hipMemcpyHostToDevice, s));
+    QSX_HIP(hipStreamSynchronize(s));  // the staging buffer is reused by the next chunk / call
+  }
+}
+
+static void upload_tensor(const TensorDesc& T, double* d, hipStream_t s, bool imag = false) {
+  upload_packed(T, imag ? T.hdata_im : T.hdata, d, s);
 }
 
 // device buffers freed on scope exit

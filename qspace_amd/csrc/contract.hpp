@@ -77,6 +77,9 @@ void match_itags(const TensorDesc& A, const TensorDesc& B, int conjA, int conjB,
                  std::vector<int32_t>& ib, int& conjA_out);
 
 hipStream_t default_stream();
+// host blocks h[i] (T's dims/offsets) -> the packed device buffer d, through
+// the library's pinned staging buffer; returns when the copy is complete
+void upload_packed(const TensorDesc& T, const double* const* h, double* d, hipStream_t s);
 // a second stream of the library (default priority) for work that may
 // overlap the default stream's; joined back with events
 hipStream_t side_stream();

@@ -193,11 +193,7 @@ T* upload_vec(const std::vector<T>& v, hipStream_t s) {
 
 void upload_blocks(const TensorDesc& T, double* d, hipStream_t s) {
   if (!T.hdata) fail(QSX_ERR_ARG, "environment tensor without data");
-  std::vector<double> buf(std::max<int64_t>(1, T.total()));
-  for (int i = 0; i < T.n; ++i)
-    if (T.numel(i)) memcpy(buf.data() + T.offs[i], T.hdata[i], sizeof(double) * T.numel(i));
-  QSX_HIP(hipMemcpyAsync(d, buf.data(), sizeof(double) * T.total(), hipMemcpyHostToDevice, s));
-  QSX_HIP(hipStreamSynchronize(s));
+  upload_packed(T, T.hdata, d, s);
 }
 
 std::string rec_key(const int32_t* r, int m) { return std::string((const char*)r, sizeof(int32_t) * m); }

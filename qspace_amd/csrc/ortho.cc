@@ -267,7 +267,6 @@ OrthoResult ortho_oneshot(const qsx_tensor* pt, int K, int Nkmin, int64_t Nkeep,
         g.src_off + (int64_t)g.d1 * g.d2 > P.total() || (g.trans ? g.d2 : g.d1) > g.ld)
       fail(QSX_ERR_STRUCT, "orthoQS: internal gather item out of range");
   }
-  std::vector<double> host = packed_host(P);
   DevMem dpsi(P.total()), dwork(work), dsig(nsig);
   struct Free {
     std::vector<void*> p;
@@ -307,7 +306,7 @@ OrthoResult ortho_oneshot(const qsx_tensor* pt, int K, int Nkmin, int64_t Nkeep,
   SvdGather* dgath = (SvdGather*)fr.add(sizeof(SvdGather) * std::max<size_t>(1, gath.size()));
   int32_t* drot = (int32_t*)fr.add(sizeof(int32_t) * items.size());
   int32_t* dbig = (int32_t*)fr.add(sizeof(int32_t) * std::max<size_t>(1, big.size()));
-  QSX_HIP(hipMemcpyAsync(dpsi.p, host.data(), sizeof(double) * P.total(), hipMemcpyHostToDevice, s));
+  upload_packed(P, P.hdata, dpsi.p, s);
   QSX_HIP(hipMemcpyAsync(ditems, items.data(), sizeof(SvdItem) * items.size(), hipMemcpyHostToDevice, s));
   if (!small.empty())
     QSX_HIP(hipMemcpyAsync(dsmall, small.data(), sizeof(SvdItem) * small.size(), hipMemcpyHostToDevice, s));

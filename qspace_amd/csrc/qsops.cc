@@ -352,10 +352,9 @@ qsx_tensor_out* plus_oneshot(const qsx_tensor* At, const qsx_tensor* Bt, double 
   std::vector<int64_t> offs(nC + 1, 0);
   for (int c = 0; c < nC; ++c) offs[c + 1] = offs[c] + (oa[c] >= 0 ? A.numel(oa[c]) : B.numel(ob[c]));
   // zero blocks of a group are not added (MakeUnique's mark): flags per operand block
-  std::vector<double> hA = packed_host(A), hB = packed_host(B);
   DevMem dAm(A.total()), dBm(B.total()), dCm(offs[nC]);
-  QSX_HIP(hipMemcpyAsync(dAm.p, hA.data(), sizeof(double) * A.total(), hipMemcpyHostToDevice, s));
-  QSX_HIP(hipMemcpyAsync(dBm.p, hB.data(), sizeof(double) * B.total(), hipMemcpyHostToDevice, s));
+  upload_packed(A, A.hdata, dAm.p, s);
+  upload_packed(B, B.hdata, dBm.p, s);
   std::vector<int32_t> zA(A.n, 0), zB(B.n, 0);
   if (cg && overlap) {
     int32_t* dz = (int32_t*)dev_alloc(sizeof(int32_t) * std::max(A.n, B.n));
