@@ -15,6 +15,7 @@
 #include "contract.hpp"
 
 #include <algorithm>
+#include <functional>
 #include <mutex>
 #include <atomic>
 #include <cmath>
@@ -678,8 +679,8 @@ void plan_execute(CtrPlan& P, const double* dA, const double* dB, double* dC, hi
 }
 
 // ---------------------------------------------------------------- one-shot
-static qsx_tensor_out* make_out(const CtrPlan& P, const std::vector<int>& keep, const std::vector<double>* data,
-                                const std::vector<double>* data_im = nullptr) {
+static qsx_tensor_out* make_out(const CtrPlan& P, const std::vector<int>& keep, const double* data,
+                                const double* data_im = nullptr) {
   auto* o = (qsx_tensor_out*)calloc(1, sizeof(qsx_tensor_out));
   const int q = P.A.qdim, rc = P.rc;
   int n = (int)keep.size();
@@ -702,20 +703,17 @@ static qsx_tensor_out* make_out(const CtrPlan& P, const std::vector<int>& keep, 
     o->offs[i + 1] = o->offs[i] + (P.offsOut[c + 1] - P.offsOut[c]);
   }
   for (size_t l = 0; l < P.tagsC.size(); ++l) o->itags[l] = P.tagsC[l];
-  if (data) {
-    o->data = (double*)malloc(sizeof(double) * std::max<int64_t>(1, o->offs[n]));
+  // the kept blocks, in parallel for large results
+  auto gather = [&](const double* src, double*& dst) {
+    dst = (double*)malloc(sizeof(double) * std::max<int64_t>(1, o->offs[n]));
+#pragma omp parallel for schedule(dynamic, 16) if (o->offs[n] > (1 << 18))
     for (int i = 0; i < n; ++i) {
-      int c = keep[i];
-      std::copy(data->begin() + P.offsOut[c], data->begin() + P.offsOut[c + 1], o->data + o->offs[i]);
+      const int c = keep[i];
+      memcpy(dst + o->offs[i], src + P.offsOut[c], sizeof(double) * (P.offsOut[c + 1] - P.offsOut[c]));
     }
-  }
-  if (data_im) {
-    o->data_im = (double*)malloc(sizeof(double) * std::max<int64_t>(1, o->offs[n]));
-    for (int i = 0; i < n; ++i) {
-      int c = keep[i];
-      std::copy(data_im->begin() + P.offsOut[c], data_im->begin() + P.offsOut[c + 1], o->data_im + o->offs[i]);
-    }
-  }
+  };
+  if (data) gather(data, o->data);
+  if (data_im) gather(data_im, o->data_im);
   o->npairs = (int64_t)P.pairs.size();
   o->useful_flops = P.flops;
   return o;
@@ -731,6 +729,52 @@ static double* g_stage = nullptr;
 static int64_t g_stage_n = 0;
 constexpr int64_t kStageMax = (int64_t)1 << 26;  // 512 MB of doubles per chunk
 
+// the staging buffer holds at least want doubles (caller holds g_stage_mu)
+static void stage_reserve(int64_t want) {
+  if (g_stage_n >= want) return;
+  if (g_stage) (void)hipHostFree(g_stage);
+  g_stage = nullptr;
+  g_stage_n = 0;
+  QSX_HIP(hipHostMalloc((void**)&g_stage, sizeof(double) * want, hipHostMallocDefault));
+  g_stage_n = want;
+}
+
+// dst[0..n) = src[0..n), by OpenMP threads for large copies
+static void par_copy(double* dst, const double* src, int64_t n) {
+  constexpr int64_t kChunk = 1 << 17;
+  const int64_t nch = (n + kChunk - 1) / kChunk;
+#pragma omp parallel for schedule(static) if (nch > 2)
+  for (int64_t c = 0; c < nch; ++c) {
+    const int64_t lo = c * kChunk, hi = std::min(n, lo + kChunk);
+    memcpy(dst + lo, src + lo, sizeof(double) * (hi - lo));
+  }
+}
+
+void download_staged(const double* d, int64_t n, hipStream_t s, const std::function<void(const double*)>& use) {
+  if (n <= 0) {
+    use(nullptr);
+    return;
+  }
+  if (n > kStageMax) {  // beyond the staging buffer: a plain host vector
+    std::vector<double> v(n);
+    QSX_HIP(hipMemcpyAsync(v.data(), d, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    QSX_HIP(hipStreamSynchronize(s));
+    use(v.data());
+    return;
+  }
+  std::lock_guard<std::mutex> lk(g_stage_mu);
+  stage_reserve(n);
+  QSX_HIP(hipMemcpyAsync(g_stage, d, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  QSX_HIP(hipStreamSynchronize(s));
+  use(g_stage);
+}
+
+void download_packed(const double* d, int64_t n, double* dst, hipStream_t s) {
+  download_staged(d, n, s, [&](const double* h) {
+    if (h) par_copy(dst, h, n);
+  });
+}
+
 void upload_packed(const TensorDesc& T, const double* const* h, double* d, hipStream_t s) {
   const int64_t tot = T.total();
   if (!tot) return;
@@ -738,14 +782,7 @@ void upload_packed(const TensorDesc& T, const double* const* h, double* d, hipSt
   for (int i = 0; i < T.n; ++i)
     if (!h[i] && T.numel(i)) fail(QSX_ERR_ARG, "null data block %d", i + 1);
   std::lock_guard<std::mutex> lk(g_stage_mu);
-  const int64_t want = std::min(tot, kStageMax);
-  if (g_stage_n < want) {
-    if (g_stage) (void)hipHostFree(g_stage);
-    g_stage = nullptr;
-    g_stage_n = 0;
-    QSX_HIP(hipHostMalloc((void**)&g_stage, sizeof(double) * want, hipHostMallocDefault));
-    g_stage_n = want;
-  }
+  stage_reserve(std::min(tot, kStageMax));
   for (int64_t e0 = 0; e0 < tot; e0 += g_stage_n) {
     const int64_t e1 = std::min(tot, e0 + g_stage_n);
     // blocks overlapping [e0, e1)
@@ -877,7 +914,7 @@ static qsx_tensor_out* contract_oneshot_complex(const qsx_tensor* A, const int32
   QSX_HIP(hipMemcpyAsync(hr.data(), Cr, sizeof(double) * nC, hipMemcpyDeviceToHost, s));
   QSX_HIP(hipMemcpyAsync(hi.data(), Ci, sizeof(double) * nC, hipMemcpyDeviceToHost, s));
   QSX_HIP(hipStreamSynchronize(s));
-  qsx_tensor_out* out = make_out(*P, keep, &hr, &hi);
+  qsx_tensor_out* out = make_out(*P, keep, hr.data(), hi.data());
   out->gpu_ms = ms;
   out->useful_flops = P->flops * ((C2 ? 1 : 0) + (C3 ? 1 : 0) + (C4 ? 1 : 0) + 1);
   return out;
@@ -944,10 +981,9 @@ static qsx_tensor_out* contract_oneshot(const qsx_tensor* A, const int32_t* ia, 
           if (!z[c]) keep.push_back(c);
       }
     }
-    std::vector<double> host(std::max<int64_t>(1, P->offsOut.back()));
-    QSX_HIP(hipMemcpyAsync(host.data(), dC, sizeof(double) * P->offsOut.back(), hipMemcpyDeviceToHost, s));
-    QSX_HIP(hipStreamSynchronize(s));
-    out = make_out(*P, keep, &host);
+    static const double kNone = 0.0;
+    download_staged(dC, P->offsOut.back(), s,
+                    [&](const double* h) { out = make_out(*P, keep, h ? h : &kNone); });
     out->gpu_ms = ms;
   } catch (...) {
     dev_free(dA);

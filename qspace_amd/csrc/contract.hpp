@@ -2,6 +2,8 @@
 // half as grouped GEMM launches).
 #pragma once
 
+#include <functional>
+
 #include <memory>
 #include <vector>
 
@@ -80,6 +82,11 @@ hipStream_t default_stream();
 // host blocks h[i] (T's dims/offsets) -> the packed device buffer d, through
 // the library's pinned staging buffer; returns when the copy is complete
 void upload_packed(const TensorDesc& T, const double* const* h, double* d, hipStream_t s);
+// n doubles of device memory -> host, through the pinned staging buffer:
+// use(h) reads them (h = nullptr when n == 0) while the buffer is held
+void download_staged(const double* d, int64_t n, hipStream_t s, const std::function<void(const double*)>& use);
+// n doubles of device memory -> host memory dst
+void download_packed(const double* d, int64_t n, double* dst, hipStream_t s);
 // a second stream of the library (default priority) for work that may
 // overlap the default stream's; joined back with events
 hipStream_t side_stream();

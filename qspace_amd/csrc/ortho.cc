@@ -593,8 +593,7 @@ OrthoResult ortho_oneshot(const qsx_tensor* pt, int K, int Nkmin, int64_t Nkeep,
     QSX_HIP(hipMemcpyAsync(dsc, sc.data(), sizeof(SvdScatter) * sc.size(), hipMemcpyHostToDevice, s));
     QSX_HIP(hipMemcpyAsync(dcol, cols.data(), sizeof(SvdCol) * cols.size(), hipMemcpyHostToDevice, s));
     launch_svd_scatter(dsc, (int)sc.size(), dcol, dwork.p, dout, s);
-    QSX_HIP(hipMemcpyAsync(out.data(), dout, sizeof(double) * (totU + totX), hipMemcpyDeviceToHost, s));
-    QSX_HIP(hipStreamSynchronize(s));
+    download_packed(dout, totU + totX, out.data(), s);
   }
   const float ms = tm.stop(s);
   // SkipZeroData(xtol2, 'b') of each group's A2 part: blocks with

@@ -255,8 +255,7 @@ qsx_tensor_out* permute_oneshot(const qsx_tensor* At, const int32_t* perm, int n
     Timer t(s);
     launch_perm_plan(pp, dA.p, dB.p, s);
     o->gpu_ms = t.stop(s);
-    QSX_HIP(hipMemcpyAsync(o->data, dB.p, sizeof(double) * A.total(), hipMemcpyDeviceToHost, s));
-    QSX_HIP(hipStreamSynchronize(s));
+    download_packed(dB.p, A.total(), o->data, s);
   } catch (...) {
     qsx_tensor_free(o);
     throw;
@@ -399,7 +398,6 @@ qsx_tensor_out* plus_oneshot(const qsx_tensor* At, const qsx_tensor* Bt, double 
     }
   }
   AxpbyItem* ditems = (AxpbyItem*)dev_alloc(sizeof(AxpbyItem) * std::max<size_t>(1, items.size()));
-  std::vector<double> hC(std::max<int64_t>(1, offs[nC]));
   float ms = 0;
   std::vector<int32_t> zC(nC, 0);
   try {
@@ -434,7 +432,6 @@ qsx_tensor_out* plus_oneshot(const qsx_tensor* At, const qsx_tensor* Bt, double 
       dev_free(dz);
       dev_free(doff);
     }
-    QSX_HIP(hipMemcpyAsync(hC.data(), dCm.p, sizeof(double) * offs[nC], hipMemcpyDeviceToHost, s));
     QSX_HIP(hipStreamSynchronize(s));
   } catch (...) {
     dev_free(ditems);
@@ -454,7 +451,6 @@ qsx_tensor_out* plus_oneshot(const qsx_tensor* At, const qsx_tensor* Bt, double 
     std::copy(&S.Q[(size_t)sb * rq], &S.Q[(size_t)sb * rq] + rq, o->Q + i * rq);
     std::copy(&S.dims[(size_t)sb * r], &S.dims[(size_t)sb * r] + r, o->dims + i * r);
     o->offs[i + 1] = o->offs[i] + (offs[c + 1] - offs[c]);
-    std::copy(hC.begin() + offs[c], hC.begin() + offs[c + 1], o->data + o->offs[i]);
     for (int j = 0; j < ref.nsym; ++j) {
       double w = 1.0;
       if (ref.symtype[j] == SYM_SU2) {
@@ -466,6 +462,20 @@ qsx_tensor_out* plus_oneshot(const qsx_tensor* At, const qsx_tensor* Bt, double 
     }
   }
   for (size_t l = 0; l < tags.size(); ++l) o->itags[l] = tags[l];
+  // the kept blocks straight from the pinned staging copy of the result
+  try {
+    download_staged(dCm.p, offs[nC], s, [&](const double* h) {
+      if (!h) return;
+#pragma omp parallel for schedule(dynamic, 16) if (offs[nC] > (1 << 18))
+      for (size_t i = 0; i < keep.size(); ++i) {
+        const int c = keep[i];
+        memcpy(o->data + o->offs[i], h + offs[c], sizeof(double) * (offs[c + 1] - offs[c]));
+      }
+    });
+  } catch (...) {
+    qsx_tensor_free(o);
+    throw;
+  }
   o->gpu_ms = ms;
   return o;
 }

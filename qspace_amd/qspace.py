@@ -216,22 +216,27 @@ def to_c(T: QSpace, with_data: bool = True):
     ct.dims = dims.ctypes.data_as(C.POINTER(C.c_int64))
     ct.data_im = None
     if with_data:
+        # block addresses gathered into one array (no per-block ctypes
+        # objects); a block that is not column-major float64 gets a copy
         cplx = T.is_complex()
-        ptrs = (C.POINTER(C.c_double) * max(1, n))()
-        pim = (C.POINTER(C.c_double) * max(1, n))() if cplx else None
+        addrs = np.zeros(max(1, n), dtype=np.uintp)
+        aim = np.zeros(max(1, n), dtype=np.uintp) if cplx else None
         for i, d in enumerate(T.data):
-            a = np.asfortranarray(np.real(d), dtype=np.float64)
-            keep.append(a)
-            ptrs[i] = a.ctypes.data_as(C.POINTER(C.c_double))
             if cplx:
+                a = np.asfortranarray(np.real(d), dtype=np.float64)
                 b = np.asfortranarray(np.imag(d), dtype=np.float64)
-                keep.append(b)
-                pim[i] = b.ctypes.data_as(C.POINTER(C.c_double))
-        keep.append(ptrs)
-        ct.data = C.cast(ptrs, C.POINTER(C.POINTER(C.c_double)))
+                keep += [a, b]
+                aim[i] = b.__array_interface__["data"][0]
+            else:
+                a = d if (d.dtype == np.float64 and d.flags.f_contiguous) else np.asfortranarray(d, dtype=np.float64)
+                if a is not d:
+                    keep.append(a)
+            addrs[i] = a.__array_interface__["data"][0]
+        keep += [addrs, T.data]
+        ct.data = addrs.ctypes.data_as(C.POINTER(C.POINTER(C.c_double)))
         if cplx:
-            keep.append(pim)
-            ct.data_im = C.cast(pim, C.POINTER(C.POINTER(C.c_double)))
+            keep.append(aim)
+            ct.data_im = aim.ctypes.data_as(C.POINTER(C.POINTER(C.c_double)))
     else:
         ct.data = None
     if T.itags:
@@ -258,17 +263,18 @@ def from_out(o: QsxTensorOut, with_data: bool = True) -> QSpace:
     dims = np.ctypeslib.as_array(o.dims, shape=(max(1, n * r),))[: n * r].copy().reshape(n, r)
     offs = np.ctypeslib.as_array(o.offs, shape=(n + 1,)).copy()
     data = []
+    dl = dims.tolist()
     if with_data and o.data:
-        buf = np.ctypeslib.as_array(o.data, shape=(max(1, int(offs[-1])),))
-        bim = np.ctypeslib.as_array(o.data_im, shape=(max(1, int(offs[-1])),)) if o.data_im else None
-        for i in range(n):
-            shp = tuple(int(x) for x in dims[i]) if r else ()
-            re = buf[offs[i]: offs[i + 1]]
-            v = re + 1j * bim[offs[i]: offs[i + 1]] if bim is not None else re.copy()
-            data.append(v.reshape(shp, order="F"))
+        # one copy of the whole result; the blocks are column-major views of it
+        tot = int(offs[-1])
+        buf = np.ctypeslib.as_array(o.data, shape=(max(1, tot),))[:tot].copy()
+        if o.data_im:
+            buf = buf + 1j * np.ctypeslib.as_array(o.data_im, shape=(max(1, tot),))[:tot]
+        ol = offs.tolist()
+        data = [buf[ol[i]: ol[i + 1]].reshape(dl[i] if r else (), order="F") for i in range(n)]
     else:
         for i in range(n):
-            data.append(np.zeros(tuple(int(x) for x in dims[i]), order="F"))
+            data.append(np.zeros(tuple(dl[i]), order="F"))
     itags = [unpack_itag(int(o.itags[i])) for i in range(r)] if o.itags else []
     if itags and all(t == "" for t in itags):
         itags = []
