@@ -83,7 +83,8 @@ hipStream_t default_stream();
 // the library's pinned staging buffer; returns when the copy is complete
 void upload_packed(const TensorDesc& T, const double* const* h, double* d, hipStream_t s);
 // n doubles of device memory -> host, through the pinned staging buffer:
-// use(h) reads them (h = nullptr when n == 0) while the buffer is held
+// use(h) reads them (h = nullptr when n == 0) while the buffer is held, so
+// use must not upload or download itself
 void download_staged(const double* d, int64_t n, hipStream_t s, const std::function<void(const double*)>& use);
 // n doubles of device memory -> host memory dst
 void download_packed(const double* d, int64_t n, double* dst, hipStream_t s);
