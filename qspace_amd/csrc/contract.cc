@@ -433,19 +433,45 @@ static void build_structure(CtrPlan& P, const int32_t* perm, int nperm, const do
   P.L = device_label_engine(QAc, QAk, A.n, QBc, QBk, B.n, kc, kma, kmb, default_stream());
   const int nL = (int)P.L.seg.size() - 1;
   const size_t np = P.L.Ia.size();
-  // afac per pair (contractDATA_group, QSpace_aux.cc:100-230)
+  // afac per pair (contractDATA_group, QSpace_aux.cc:100-230).  The
+  // X-symbols are exact and memoised; a structure's new label sets are
+  // evaluated by OpenMP threads (the exact-arithmetic tables are grown
+  // first, so the threads only read them; an error is rethrown after the
+  // loop)
   P.afac.assign(np, 1.0);
-  for (size_t p = 0; p < np; ++p) {
-    const int ia = P.L.Ia[p], ib = P.L.Ib[p];
-    double af = (fA ? fA[ia] : 1.0) * (fB ? fB[ib] : 1.0);
-    for (int j = 0; j < A.nsym && af != 0; ++j) {
-      if (A.symtype[j] != SYM_SU2) continue;
-      double x = su2_x3(su2_col(A, ia, j), P.dirA, su2_col(B, ib, j), P.dirB, P.ica, P.icb);
-      af *= x * A.cgw[(size_t)ia * A.nsym + j] * B.cgw[(size_t)ib * B.nsym + j];
-    }
-    if (std::fabs(af) < 1e-18) af = 0;  // CG_SKIP_EPS2
-    P.afac[p] = af;
+  int max_tj = 0;
+  bool su2 = false;
+  for (int j = 0; j < A.nsym; ++j) {
+    if (A.symtype[j] != SYM_SU2) continue;
+    su2 = true;
+    for (int i = 0; i < A.n; ++i)
+      for (int l = 0; l < A.r; ++l) max_tj = std::max(max_tj, A.label(i, l, j));
+    for (int i = 0; i < B.n; ++i)
+      for (int l = 0; l < B.r; ++l) max_tj = std::max(max_tj, B.label(i, l, j));
   }
+  if (su2) su2_reserve_labels(max_tj);
+  std::atomic<bool> bad{false};
+  Error first{0, ""};
+  std::mutex err_mu;
+#pragma omp parallel for schedule(dynamic, 32) if (su2 && np > 256)
+  for (size_t p = 0; p < np; ++p) {
+    if (bad.load(std::memory_order_relaxed)) continue;
+    try {
+      const int ia = P.L.Ia[p], ib = P.L.Ib[p];
+      double af = (fA ? fA[ia] : 1.0) * (fB ? fB[ib] : 1.0);
+      for (int j = 0; j < A.nsym && af != 0; ++j) {
+        if (A.symtype[j] != SYM_SU2) continue;
+        double x = su2_x3(su2_col(A, ia, j), P.dirA, su2_col(B, ib, j), P.dirB, P.ica, P.icb);
+        af *= x * A.cgw[(size_t)ia * A.nsym + j] * B.cgw[(size_t)ib * B.nsym + j];
+      }
+      if (std::fabs(af) < 1e-18) af = 0;  // CG_SKIP_EPS2
+      P.afac[p] = af;
+    } catch (const Error& e) {
+      std::lock_guard<std::mutex> lk(err_mu);
+      if (!bad.exchange(true)) first = e;
+    }
+  }
+  if (bad.load()) throw first;
   // output permutation (completed with the identity, bin/contractQS.m:154-170)
   P.perm.resize(P.rc);
   std::iota(P.perm.begin(), P.perm.end(), 0);

@@ -74,6 +74,9 @@ double su2_cg_exact(int tj1, int tm1, int tj2, int tm2, int tj3, int tm3);
 double su2_x3(const std::vector<int>& tja, const std::vector<int>& dira, const std::vector<int>& tjb,
               const std::vector<int>& dirb, const std::vector<int>& ia, const std::vector<int>& ib);
 double su2_norm_ext(const std::vector<int>& tj, const std::vector<int>& dir);
+// grow the exact-arithmetic tables for labels 2S <= max_tj up front, so that
+// su2_x3 / su2_cg_exact on such labels only read them (concurrent callers)
+void su2_reserve_labels(int max_tj);
 // +-1: the CGT implied by the legs in order perm (new leg k = old leg
 // perm[k]) relative to the transposed CGT of the block
 double su2_perm_sign(const std::vector<int>& tj, const std::vector<int>& dir, const std::vector<int>& perm);

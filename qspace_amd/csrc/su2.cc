@@ -28,6 +28,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <shared_mutex>
 #include <unordered_map>
 #include <vector>
 #include <string>
@@ -420,7 +421,7 @@ struct VecHash {
 };
 
 std::unordered_map<std::vector<int>, double, VecHash> g_x3;
-std::mutex g_x3_mu;
+std::shared_mutex g_x3_mu;  // lookups shared, inserts exclusive
 
 double x3_compute(const std::vector<int>& tja, const std::vector<int>& dira, const std::vector<int>& tjb,
                   const std::vector<int>& dirb, const std::vector<int>& ia, const std::vector<int>& ib) {
@@ -508,6 +509,11 @@ double x3_compute(const std::vector<int>& tja, const std::vector<int>& dira, con
 
 }  // namespace
 
+// su2_cg_exact asks for primes up to (tj1+tj2+tj3)/2 + tj3 + 3 <= 2.5 max + 3;
+// the table then ends at the largest prime <= its limit, which is above half
+// the limit (Bertrand): a limit of 6 max + 64 covers every such request
+void su2_reserve_labels(int max_tj) { ensure_primes(6 * std::max(0, max_tj) + 64); }
+
 double su2_x3(const std::vector<int>& tja, const std::vector<int>& dira, const std::vector<int>& tjb,
               const std::vector<int>& dirb, const std::vector<int>& ia, const std::vector<int>& ib) {
   std::vector<int> key;
@@ -522,12 +528,12 @@ double su2_x3(const std::vector<int>& tja, const std::vector<int>& dira, const s
   key.insert(key.end(), ia.begin(), ia.end());
   key.insert(key.end(), ib.begin(), ib.end());
   {
-    std::lock_guard<std::mutex> lk(g_x3_mu);
+    std::shared_lock<std::shared_mutex> lk(g_x3_mu);
     auto it = g_x3.find(key);
     if (it != g_x3.end()) return it->second;
   }
   double x = x3_compute(tja, dira, tjb, dirb, ia, ib);
-  std::lock_guard<std::mutex> lk(g_x3_mu);
+  std::unique_lock<std::shared_mutex> lk(g_x3_mu);
   g_x3.emplace(std::move(key), x);
   return x;
 }
