@@ -134,8 +134,11 @@ class QSpace:
             if a.ndim < r:
                 a = a.reshape(a.shape + (1,) * (r - a.ndim), order="F")
             self.data.append(np.asfortranarray(a))
-        self.info = {"qtype": qtype, "itags": list(itags) if itags is not None else [],
-                     "cgw": None if cgw is None else np.asarray(cgw, dtype=np.float64).reshape(n, -1)}
+        if cgw is not None:
+            cgw = np.asarray(cgw, dtype=np.float64)
+            # (n, nsym); an empty tensor keeps the column count it was given
+            cgw = cgw.reshape(n, -1) if n else cgw.reshape(0, cgw.shape[-1] if cgw.ndim == 2 else 0)
+        self.info = {"qtype": qtype, "itags": list(itags) if itags is not None else [], "cgw": cgw}
         if len(self.data) != n:
             raise ValueError(f"QSpace: {n} label records but {len(self.data)} data blocks")
 

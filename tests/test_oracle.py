@@ -318,3 +318,15 @@ def test_complex_real_imag_zero_matches_real():
     assert C2.nblk == R2.nblk > 0
     for a, b in zip(C2.data, R2.data):
         assert np.array_equal(a.imag, -b) and not np.any(a.real)
+
+
+def test_empty_result_keeps_weight_columns():
+    """a contraction whose result has no blocks (two U(1) labels per leg):
+    the tensor is built, with a (0, nsym) weight array"""
+    rng = np.random.RandomState(0)
+    A = QSpace([np.array([[1, 0]]), np.array([[1, 0]])], [rng.standard_normal((2, 3))], "A,A", ["a", "b*"])
+    B = QSpace([np.array([[2, 0]]), np.array([[2, 0]])], [rng.standard_normal((3, 2))], "A,A", ["b", "c*"])
+    C = O.contract(A, [1], B, [0])
+    assert C.nblk == 0 and C.rank == 2
+    E = QSpace(C.Q, [], "A,A", C.itags, np.zeros((0, 2)))
+    assert E.info["cgw"].shape == (0, 2)
