@@ -138,3 +138,63 @@ def test_random_truncations(gpu, seed):
         XR = O.contract(R1, [K], R2, [r - K])
         assert_same_structure(X, XR)
         assert rel_err(X, XR) < 1e-12
+
+
+def _su2_leg(rng):
+    """a leg: {2S: dim}"""
+    tss = rng.choice(6, size=rng.randint(2, 5), replace=False)
+    return {int(t): int(rng.randint(1, 5)) for t in tss}
+
+
+def _su2_tensor(rng, legs, dirs, tags):
+    """blocks of every label set allowed by SU(2) fusion (triangle rule and
+    parity for rank 3, equal labels for rank 2)"""
+    r = len(legs)
+    Q_, data = [[] for _ in legs], []
+    for combo in itertools.product(*[sorted(l) for l in legs]):
+        if r == 2 and combo[0] != combo[1]:
+            continue
+        if r == 3:
+            a, b, c = combo
+            if (a + b + c) % 2 or c > a + b or c < abs(a - b):
+                continue
+        for k, t in enumerate(combo):
+            Q_[k].append([t])
+        data.append(np.asfortranarray(rng.standard_normal(tuple(l[t] for l, t in zip(legs, combo)))))
+    return QSpace([np.array(q, dtype=np.int32).reshape(len(data), 1) for q in Q_], data, "SU2", tags)
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_random_su2_contractions(gpu, seed):
+    """SU(2) contractions with results of rank <= 3 (outer multiplicity 1):
+    (3,2) over one leg, (3,3) over two or three, (2,2) over one; random
+    directions and leg positions"""
+    rng = np.random.RandomState(300 + seed)
+    ra, rb, k = [(3, 2, 1), (3, 3, 2), (2, 2, 1), (3, 3, 3)][seed % 4]
+    shared = [_su2_leg(rng) for _ in range(k)]
+    la = shared + [_su2_leg(rng) for _ in range(ra - k)]
+    lb = shared + [_su2_leg(rng) for _ in range(rb - k)]
+    da = [1 if rng.rand() < 0.5 else -1 for _ in la]
+    db = [-da[i] if i < k else (1 if rng.rand() < 0.5 else -1) for i in range(rb)]
+    if ra == 2:
+        da[1] = -da[0]
+    if rb == 2:
+        db[1] = -db[0] if k < 2 else db[1]
+    pa, pb = rng.permutation(ra), rng.permutation(rb)
+    ta = [(f"c{i}" if i < k else f"a{i}") + ("*" if da[i] < 0 else "") for i in range(ra)]
+    tb = [(f"c{i}" if i < k else f"b{i}") + ("*" if db[i] < 0 else "") for i in range(rb)]
+    A = _su2_tensor(rng, [la[i] for i in pa], [da[i] for i in pa], [ta[i] for i in pa])
+    B = _su2_tensor(rng, [lb[i] for i in pb], [db[i] for i in pb], [tb[i] for i in pb])
+    if not A.nblk or not B.nblk:
+        pytest.skip("empty draw")
+    ia = [int(np.nonzero(pa == i)[0][0]) for i in range(k)]
+    ib = [int(np.nonzero(pb == i)[0][0]) for i in range(k)]
+    try:
+        R = O.contract(A, ia, B, ib)
+    except O.OracleError as e:
+        pytest.skip(f"outside the oracle's scope: {e}")
+    print(f"seed {seed}: A {A.itags} {A.nblk} ia {ia}, B {B.itags} {B.nblk} ib {ib} -> {R.nblk}", flush=True)
+    C = Q.api._contract_pair(A, ia, 0, B, ib, 0, None)
+    assert_same_structure(C, R)
+    if R.nblk:
+        assert rel_err(C, R) < 1e-12
