@@ -92,7 +92,8 @@ typedef struct qsx_tensor {
   const char* qtype;
   const double* cgw;
   /* imaginary parts of complex data: nblk pointers to blocks of the same
-     layout as data, or NULL for real data (qsx_contract only) */
+     layout as data, or NULL for real data (qsx_contract, qsx_permute,
+     qsx_plus; the other entry points reject complex data) */
   const double* const* data_im;
 } qsx_tensor;
 
@@ -154,7 +155,8 @@ void qsx_tensor_free(qsx_tensor_out* t);
 /* permuteQS(A, perm [,'conj']) <- Source/permuteQS.cc:25-87 (QSpace::permute,
    Source/QSpace.hh:2721-2780, then NormCGW, Source/QSpace.cc:3727-3805).
    perm is 0-based and completed with the identity when shorter than the
-   rank; conj != 0 flips the conj flag of every leg.  An identity
+   rank; conj != 0 flips the conj flag of every leg and conjugates complex
+   data (wbarray::permute, Source/wbarray.cc:2738).  An identity
    permutation without conj returns A unchanged.  B->gpu_ms = device time. */
 int qsx_permute(const qsx_tensor* A, const int32_t* perm, int32_t nperm, int32_t conj,
                 qsx_tensor_out** B, char* err, size_t errlen);
@@ -162,7 +164,9 @@ int qsx_permute(const qsx_tensor* A, const int32_t* perm, int32_t nperm, int32_t
    (QSpace::plus_plain, Source/QSpace.cc:3593-3705, for abelian labels; Cat +
    MakeUnique, Source/QSpace.cc:3075-3140,3807-3905, with SU(2) labels), then
    SkipZeroData and NormCGW (opts->skip_zero = 0 keeps zero blocks; opts
-   may be NULL).  C->gpu_ms = device time of the sum. */
+   may be NULL).  Complex operands: both planes (a real operand's
+   imaginary plane is zero; a block is zero when both planes are).
+   C->gpu_ms = device time of the sum. */
 int qsx_plus(const qsx_tensor* A, const qsx_tensor* B, double bfac, const qsx_opts* opts,
              qsx_tensor_out** C, char* err, size_t errlen);
 

@@ -351,6 +351,8 @@ def permute(A: QSpace, perm, conj=0) -> QSpace:
             w.append(norm_ext([int(Q[k][i, j]) for k in range(r)], dout))
         cg.append(w)
         x = np.transpose(np.asarray(d).reshape(d.shape[:r], order="F"), p)
+        if conj:  # conj conjugates complex data (wbarray::permute, Source/wbarray.cc:2738)
+            x = np.conj(x)
         data.append(np.asfortranarray(x * f))
     return QSpace(Q, data, A.qtype, tags, np.array(cg).reshape(A.nblk, -1) if A.nblk else None)
 

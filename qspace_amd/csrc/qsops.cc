@@ -170,8 +170,13 @@ double norm_factor(const TensorDesc& T, int blk, const std::vector<int>& dir) {
 }
 
 // the input returned unchanged (identity permutation, or an empty operand)
-qsx_tensor_out* copy_out(const TensorDesc& T, const std::vector<double>& data, double fac) {
+qsx_tensor_out* copy_out(const TensorDesc& T, const std::vector<double>& data, double fac,
+                         const std::vector<double>* data_im = nullptr) {
   qsx_tensor_out* o = alloc_out(T.n, T.r, T.qdim, T.nsym, T.qtype, T.total());
+  if (data_im) {
+    o->data_im = (double*)calloc(std::max<int64_t>(1, T.total()), sizeof(double));
+    for (int64_t e = 0; e < T.total(); ++e) o->data_im[e] = fac * (*data_im)[e];
+  }
   std::copy(T.Q.begin(), T.Q.end(), o->Q);
   std::copy(T.dims.begin(), T.dims.end(), o->dims);
   std::copy(T.offs.begin(), T.offs.end(), o->offs);
@@ -192,7 +197,6 @@ qsx_tensor_out* copy_out(const TensorDesc& T, const std::vector<double>& data, d
 }
 
 qsx_tensor_out* permute_oneshot(const qsx_tensor* At, const int32_t* perm, int nperm, int conj) {
-  if (At && At->data_im) fail(QSX_ERR_UNSUPPORTED, "permuteQS: complex data not supported (real QSpaces only)");
   ensure_device();
   hipStream_t s = default_stream();
   TensorDesc A = make_desc(At, "A");
@@ -213,8 +217,16 @@ qsx_tensor_out* permute_oneshot(const qsx_tensor* At, const int32_t* perm, int n
   }
   bool ident = true;
   for (int i = 0; i < r; ++i) ident &= p[i] == i;
-  std::vector<double> host = packed_host(A);
-  if (!r || (ident && !conj)) return copy_out(A, host, 1.0);
+  const bool cplx = A.hdata_im != nullptr;
+  if (!r || (ident && !conj)) {
+    std::vector<double> host = packed_host(A), him;
+    if (cplx) {
+      TensorDesc Ai = A;
+      Ai.hdata = A.hdata_im;
+      him = packed_host(Ai);
+    }
+    return copy_out(A, host, 1.0, cplx ? &him : nullptr);
+  }
   // output structure
   std::vector<uint64_t> tags(A.itags.size());
   for (size_t l = 0; l < tags.size(); ++l) tags[l] = A.itags[p[l]] ^ (conj ? 0x80u : 0u);
@@ -251,11 +263,25 @@ qsx_tensor_out* permute_oneshot(const qsx_tensor* At, const int32_t* perm, int n
     perm_plan_build(pp, A.n, r, A.dims.data(), A.offs.data(), A.offs.data(), p.data(), scale.data());
     perm_plan_upload(pp, s);
     DevMem dA(A.total()), dB(A.total());
-    QSX_HIP(hipMemcpyAsync(dA.p, host.data(), sizeof(double) * A.total(), hipMemcpyHostToDevice, s));
+    upload_packed(A, A.hdata, dA.p, s);
     Timer t(s);
     launch_perm_plan(pp, dA.p, dB.p, s);
     o->gpu_ms = t.stop(s);
     download_packed(dB.p, A.total(), o->data, s);
+    if (cplx) {
+      // the imaginary plane through the same permutation; conj conjugates
+      // complex data (wbarray::permute with P.conj, Source/wbarray.cc:2738)
+      PermPlan pi;
+      std::vector<double> si(scale);
+      if (conj)
+        for (double& x : si) x = -x;
+      perm_plan_build(pi, A.n, r, A.dims.data(), A.offs.data(), A.offs.data(), p.data(), si.data());
+      perm_plan_upload(pi, s);
+      upload_packed(A, A.hdata_im, dA.p, s);
+      launch_perm_plan(pi, dA.p, dB.p, s);
+      o->data_im = (double*)calloc(std::max<int64_t>(1, A.total()), sizeof(double));
+      download_packed(dB.p, A.total(), o->data_im, s);
+    }
   } catch (...) {
     qsx_tensor_free(o);
     throw;
@@ -277,8 +303,6 @@ bool tags_equal(const std::vector<uint64_t>& a, const std::vector<uint64_t>& b) 
 }
 
 qsx_tensor_out* plus_oneshot(const qsx_tensor* At, const qsx_tensor* Bt, double bfac, int skip_zero) {
-  if ((At && At->data_im) || (Bt && Bt->data_im))
-    fail(QSX_ERR_UNSUPPORTED, "plusQS: complex data not supported (real QSpaces only)");
   ensure_device();
   hipStream_t s = default_stream();
   TensorDesc A = make_desc(At, "A"), B = make_desc(Bt, "B");
@@ -354,19 +378,29 @@ qsx_tensor_out* plus_oneshot(const qsx_tensor* At, const qsx_tensor* Bt, double 
   DevMem dAm(A.total()), dBm(B.total()), dCm(offs[nC]);
   upload_packed(A, A.hdata, dAm.p, s);
   upload_packed(B, B.hdata, dBm.p, s);
-  std::vector<int32_t> zA(A.n, 0), zB(B.n, 0);
-  if (cg && overlap) {
-    int32_t* dz = (int32_t*)dev_alloc(sizeof(int32_t) * std::max(A.n, B.n));
-    int64_t* doff = (int64_t*)dev_alloc(sizeof(int64_t) * (std::max(A.n, B.n) + 1));
+  // complex data: the imaginary planes take the same items (a real operand
+  // has a zero plane); a block is zero when both planes are
+  const bool cplx = A.hdata_im || B.hdata_im;
+  DevMem dAi(cplx ? A.total() : 0), dBi(cplx ? B.total() : 0), dCi(cplx ? offs[nC] : 0);
+  if (cplx) {
+    if (A.hdata_im) upload_packed(A, A.hdata_im, dAi.p, s);
+    else launch_fill_zero(dAi.p, A.total(), s);
+    if (B.hdata_im) upload_packed(B, B.hdata_im, dBi.p, s);
+    else launch_fill_zero(dBi.p, B.total(), s);
+  }
+  // per-block "all exact zeros" over both planes
+  auto zero_flags = [&](const std::vector<int64_t>& of, int nb, const double* re, const double* im,
+                        std::vector<int32_t>& z) {
+    int32_t* dz = (int32_t*)dev_alloc(sizeof(int32_t) * std::max(1, 2 * nb));
+    int64_t* doff = (int64_t*)dev_alloc(sizeof(int64_t) * (nb + 1));
     try {
-      QSX_HIP(hipMemcpyAsync(doff, A.offs.data(), sizeof(int64_t) * (A.n + 1), hipMemcpyHostToDevice, s));
-      launch_block_zero_flags(doff, A.n, dAm.p, dz, s);
-      QSX_HIP(hipMemcpyAsync(zA.data(), dz, sizeof(int32_t) * A.n, hipMemcpyDeviceToHost, s));
+      QSX_HIP(hipMemcpyAsync(doff, of.data(), sizeof(int64_t) * (nb + 1), hipMemcpyHostToDevice, s));
+      launch_block_zero_flags(doff, nb, re, dz, s);
+      if (im) launch_block_zero_flags(doff, nb, im, dz + nb, s);
+      std::vector<int32_t> zz(2 * nb, 1);
+      QSX_HIP(hipMemcpyAsync(zz.data(), dz, sizeof(int32_t) * (im ? 2 : 1) * nb, hipMemcpyDeviceToHost, s));
       QSX_HIP(hipStreamSynchronize(s));
-      QSX_HIP(hipMemcpyAsync(doff, B.offs.data(), sizeof(int64_t) * (B.n + 1), hipMemcpyHostToDevice, s));
-      launch_block_zero_flags(doff, B.n, dBm.p, dz, s);
-      QSX_HIP(hipMemcpyAsync(zB.data(), dz, sizeof(int32_t) * B.n, hipMemcpyDeviceToHost, s));
-      QSX_HIP(hipStreamSynchronize(s));
+      for (int i = 0; i < nb; ++i) z[i] = zz[i] && zz[nb + i];
     } catch (...) {
       dev_free(dz);
       dev_free(doff);
@@ -374,6 +408,11 @@ qsx_tensor_out* plus_oneshot(const qsx_tensor* At, const qsx_tensor* Bt, double 
     }
     dev_free(dz);
     dev_free(doff);
+  };
+  std::vector<int32_t> zA(A.n, 0), zB(B.n, 0);
+  if (cg && overlap) {
+    zero_flags(A.offs, A.n, dAm.p, cplx ? dAi.p : nullptr, zA);
+    zero_flags(B.offs, B.n, dBm.p, cplx ? dBi.p : nullptr, zB);
   }
   std::vector<AxpbyItem> items;
   for (int c = 0; c < nC; ++c) {
@@ -405,6 +444,7 @@ qsx_tensor_out* plus_oneshot(const qsx_tensor* At, const qsx_tensor* Bt, double 
       QSX_HIP(hipMemcpyAsync(ditems, items.data(), sizeof(AxpbyItem) * items.size(), hipMemcpyHostToDevice, s));
     Timer t(s);
     launch_axpby(ditems, (int)items.size(), dAm.p, dBm.p, dCm.p, s);
+    if (cplx) launch_axpby(ditems, (int)items.size(), dAi.p, dBi.p, dCi.p, s);
     ms = t.stop(s);
     // SkipZeroData: exact-zero blocks dropped unless a rank-2 block-diagonal matrix
     bool blockdiag = false;
@@ -416,22 +456,7 @@ qsx_tensor_out* plus_oneshot(const qsx_tensor* At, const qsx_tensor* Bt, double 
           if (rec[j] != rec[q + j]) blockdiag = false;
       }
     }
-    if (skip_zero && nC && !(r == 2 && blockdiag)) {
-      int32_t* dz = (int32_t*)dev_alloc(sizeof(int32_t) * nC);
-      int64_t* doff = (int64_t*)dev_alloc(sizeof(int64_t) * (nC + 1));
-      try {
-        QSX_HIP(hipMemcpyAsync(doff, offs.data(), sizeof(int64_t) * (nC + 1), hipMemcpyHostToDevice, s));
-        launch_block_zero_flags(doff, nC, dCm.p, dz, s);
-        QSX_HIP(hipMemcpyAsync(zC.data(), dz, sizeof(int32_t) * nC, hipMemcpyDeviceToHost, s));
-      } catch (...) {
-        dev_free(dz);
-        dev_free(doff);
-        throw;
-      }
-      QSX_HIP(hipStreamSynchronize(s));
-      dev_free(dz);
-      dev_free(doff);
-    }
+    if (skip_zero && nC && !(r == 2 && blockdiag)) zero_flags(offs, nC, dCm.p, cplx ? dCi.p : nullptr, zC);
     QSX_HIP(hipStreamSynchronize(s));
   } catch (...) {
     dev_free(ditems);
@@ -464,14 +489,21 @@ qsx_tensor_out* plus_oneshot(const qsx_tensor* At, const qsx_tensor* Bt, double 
   for (size_t l = 0; l < tags.size(); ++l) o->itags[l] = tags[l];
   // the kept blocks straight from the pinned staging copy of the result
   try {
-    download_staged(dCm.p, offs[nC], s, [&](const double* h) {
-      if (!h) return;
+    auto fetch = [&](const double* src, double* dst) {
+      download_staged(src, offs[nC], s, [&](const double* h) {
+        if (!h) return;
 #pragma omp parallel for schedule(dynamic, 16) if (offs[nC] > (1 << 18))
-      for (size_t i = 0; i < keep.size(); ++i) {
-        const int c = keep[i];
-        memcpy(o->data + o->offs[i], h + offs[c], sizeof(double) * (offs[c + 1] - offs[c]));
-      }
-    });
+        for (size_t i = 0; i < keep.size(); ++i) {
+          const int c = keep[i];
+          memcpy(dst + o->offs[i], h + offs[c], sizeof(double) * (offs[c + 1] - offs[c]));
+        }
+      });
+    };
+    fetch(dCm.p, o->data);
+    if (cplx) {
+      o->data_im = (double*)calloc(std::max<int64_t>(1, tot), sizeof(double));
+      fetch(dCi.p, o->data_im);
+    }
   } catch (...) {
     qsx_tensor_free(o);
     throw;

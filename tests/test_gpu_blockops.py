@@ -124,3 +124,32 @@ def test_cell_form_perm_on_device(gpu):
     C0 = Q.contractQS([HL, psi], HR)
     C = Q.contractQS([HL, psi], HR, [2, 1])
     _exact(C, O.permute(C0, [1, 0]))
+
+
+def _cplx(T, seed):
+    rng = np.random.RandomState(seed)
+    return QSpace(T.Q, [np.asfortranarray(d + 1j * rng.standard_normal(d.shape)) for d in T.data], T.qtype,
+                  T.itags, T.info.get("cgw"))
+
+
+@pytest.mark.parametrize("kind,perm,conj", [("u1_small", [1, 0, 2, 3], 0), ("u1_big", [3, 1, 2, 0], 1),
+                                             ("u1_small", [], 1), ("su2", [2, 1, 0], 0), ("su2", [0, 2, 1], 1)])
+def test_complex_permute_matches_oracle(gpu, kind, perm, conj):
+    """both planes through the same permutation; conj conjugates the data"""
+    A = _cplx(_tensor(kind), 5)
+    C = Q.permuteQS(A, perm, conj=bool(conj))
+    assert C.is_complex()
+    _exact(C, O.permute(A, perm, conj=conj))
+
+
+@pytest.mark.parametrize("kind,which", [("u1", "both"), ("u1", "B"), ("su2", "A"), ("su2", "both")])
+def test_complex_plus_matches_oracle(gpu, kind, which):
+    T = u1_rank4(5, dmin=2, dmax=12) if kind == "u1" else su2_env(9, Dstar=300)
+    n = T.nblk
+    A = _split(T, 0, 2 * n // 3, 1)
+    B = _split(T, n // 3, n, 2)
+    if which in ("both", "A"):
+        A = _cplx(A, 3)
+    if which in ("both", "B"):
+        B = _cplx(B, 4)
+    _exact(Q.plusQS(A, B, -0.75), O.plus(A, B, -0.75))

@@ -245,3 +245,26 @@ def _union_layouts_pair(A, ia, B, ib):
         LA[a] = u
         LB[b] = u
     return [_layout(s) for s in LA], [_layout(s) for s in LB]
+
+
+def _cplx(T, seed):
+    rng = np.random.RandomState(seed)
+    return QSpace(T.Q, [d + 1j * rng.standard_normal(d.shape) for d in T.data], T.qtype, T.itags,
+                  T.info.get("cgw"))
+
+
+def test_complex_permute_and_plus_full_space():
+    """complex U(1) data: Full(permute(A,p)) = transpose(Full(A)), with conj
+    the complex conjugate; Full(A + b B) = Full(A) + b Full(B)"""
+    A = _cplx(u1_rank4(21), 1)
+    lay = [_layout(_sectors(A, k, None)) for k in range(A.rank)]
+    F = expand(A, lay)
+    for perm, conj in (([2, 0, 3, 1], 0), ([1, 0, 2, 3], 1), ([0, 1, 2, 3], 1)):
+        P = O.permute(A, perm, conj=conj)
+        ref = np.transpose(F, perm)
+        if conj:
+            ref = np.conj(ref)
+        assert np.abs(expand(P, [lay[k] for k in perm]) - ref).max() < 1e-14
+    B = QSpace(A.Q, [np.asarray(d).real * 0.5 for d in A.data], A.qtype, A.itags)
+    C = O.plus(A, B, -2.0)
+    assert np.abs(expand(C, lay) - (F - 2.0 * expand(B, lay))).max() < 1e-13
