@@ -88,7 +88,8 @@ def qspace_from_mx(S) -> QSpace:
         raise ValueError(f"dimension mismatch (QIDX: {n} records, data: {len(dcells)})")
     data = []
     for d in dcells:
-        a = np.asarray(d, dtype=np.float64)
+        a = np.asarray(d)
+        a = a.astype(np.complex128 if np.iscomplexobj(a) else np.float64, copy=False)
         if a.ndim > r:
             raise ValueError(f"data block of rank {a.ndim} > {r}")
         data.append(np.asfortranarray(a.reshape(a.shape + (1,) * (r - a.ndim), order="F")))

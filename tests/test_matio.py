@@ -73,3 +73,17 @@ def test_invalid_structs():
         qspace_from_mx({"Q": Q, "data": D})                    # non-integer labels
     with pytest.raises(ValueError):
         qspace_from_mx({"Q": Q})                               # no data
+
+
+def test_complex_round_trip(tmp_path):
+    """complex blocks keep their imaginary parts through a .mat file"""
+    rng = np.random.RandomState(7)
+    T = QSpace([np.array([[0], [1]]), np.array([[0], [1]])],
+               [rng.standard_normal((2, 3)) + 1j * rng.standard_normal((2, 3)), rng.standard_normal((1, 2)) + 0j],
+               "A", ["a", "a*"])
+    path = str(tmp_path / "c.mat")
+    save_qspace(path, {"T": T})
+    R = load_qspace(path, "T")
+    assert R.is_complex()
+    for a, b in zip(R.data, T.data):
+        assert np.array_equal(a, b)
