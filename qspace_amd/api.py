@@ -159,7 +159,10 @@ def contractQS(*args, mode="batched", **kw) -> QSpace:
     # permutation of it
     tops = [a for a in args if isinstance(a, QSpace) or _is_cell(a)]
     if len(tops) >= 2:
-        C = _download_result(_cell_dev(args, 0, mode, kw, {}, []))
+        if _any_complex(args):
+            C = _cell_host(args, mode, kw)
+        else:
+            C = _download_result(_cell_dev(args, 0, mode, kw, {}, []))
     else:
         C = _single(args, mode, kw)
     if perm is not None:
@@ -261,7 +264,8 @@ def _single(items, mode, kw) -> QSpace:
             else:
                 if len(a) < 2:
                     raise _capi.QsxError(1, "invalid cell-contraction (got cell with fewer than two entries)")
-                T = _download_result(_cell_dev(list(a), 1, mode, kw, {}, []))
+                T = _cell_host(list(a), mode, kw) if _any_complex(a) else \
+                    _download_result(_cell_dev(list(a), 1, mode, kw, {}, []))
             op = {"T": T, "conj": 0, "sidx": None, "xflag": False}
         else:
             if op is None:
@@ -303,9 +307,6 @@ def _cell_dev(items, level, mode, kw, cache, plans):
     ops = []
     for a in items:
         if isinstance(a, QSpace):
-            if a.is_complex():
-                raise _capi.QsxError(5, "cell-form contractQS takes real QSpaces; contract complex ones pairwise "
-                                        "(contractQS(A, ica, B, icb))")
             if id(a) not in cache:
                 cache[id(a)] = _DevT(a, DeviceBuffer.from_array(a.packed()), keep=a)
             ops.append({"T": cache[id(a)], "conj": 0, "sidx": None, "xflag": False})
@@ -331,6 +332,42 @@ def _cell_dev(items, level, mode, kw, cache, plans):
         P.execute(A.buf, cur.buf, dC)
         plans.append(P)
         cur = _DevT(P.out, dC)
+        cflag = 0
+        skipB = set()
+    return cur
+
+
+def _any_complex(items) -> bool:
+    return any((isinstance(a, QSpace) and a.is_complex()) or (_is_cell(a) and _any_complex(a)) for a in items)
+
+
+def _cell_host(items, mode, kw) -> QSpace:
+    """the cell form with complex leaves: the same itag matching and
+    right-to-left fold (contractQS_itags, Source/contractQS.cc:634-808),
+    each step one device contraction of host QSpaces through the complex
+    path of qsx_contract (SkipZeroData after every contraction, as the
+    reference does)"""
+    ops = []
+    for a in items:
+        if isinstance(a, QSpace):
+            ops.append({"T": a, "conj": 0, "sidx": None, "xflag": False})
+        elif _is_cell(a):
+            if len(a) < 2:
+                raise _capi.QsxError(1, "invalid cell-contraction (got cell with fewer than two entries)")
+            ops.append({"T": _cell_host(list(a), mode, kw), "conj": 0, "sidx": None, "xflag": False})
+        else:
+            if not ops:
+                raise _capi.QsxError(1, "invalid QSpace cell-structure (option before any QSpace)")
+            _cell_option(ops[-1], a)
+    if len(ops) < 2:
+        raise _capi.QsxError(1, "invalid usage #2: at least two QSpaces required for each contraction")
+    cur = ops[-1]["T"]
+    cflag = ops[-1]["conj"]
+    skipB = _skipped(ops[-1], cur)
+    for op in reversed(ops[:-1]):
+        A = op["T"]
+        ia, ib, conjA = _itag_match(A, cur, op["conj"], cflag, _skipped(op, A), skipB)
+        cur = _contract_pair(A, ia, conjA, cur, ib, cflag, None, mode=mode, **kw)
         cflag = 0
         skipB = set()
     return cur
@@ -439,6 +476,8 @@ class DeviceBuffer:
 
     @classmethod
     def from_array(cls, a: np.ndarray) -> "DeviceBuffer":
+        if np.iscomplexobj(a):
+            raise TypeError("DeviceBuffer holds real doubles (got complex data)")
         a = np.ascontiguousarray(a, dtype=np.float64)
         b = cls(a.size)
         b.upload(a)

@@ -384,3 +384,18 @@ def test_complex_su2_env(gpu):
     N, NR = _both(A, [0, 1], A, [0, 1])
     assert_same_structure(N, NR)
     assert rel_err(N, NR) < TOL
+
+
+def test_complex_env_update_cell_form(gpu):
+    """the cell form with complex leaves (a complex MPS tensor, real HK and
+    W): {A,'!2*',{{A,HK},W}} against the oracle's explicit complex chain;
+    the '*' option conjugates A's data"""
+    A0, HK, W = gen.u1_env_update(seed=42, dw=2)
+    A = _complexify(A0, 6)
+    C = Q.contractQS([A, "!2*", [[A, HK], W]])
+    X1 = O.contract(A, [0], HK, [0])
+    X2 = O.contract(X1, [1, 3], W, [2, 0])
+    R = O.contract(A, [0, 2], X2, [1, 3], conjA=1)
+    assert C.is_complex() and C.itags == ["R", "R*", "w2*"]
+    assert_same_structure(C, R)
+    assert rel_err(C, R) < TOL
