@@ -185,6 +185,8 @@ typedef struct qsx_ortho_info {
   double gpu_ms;     /* device time (assembly, SVD, result blocks) */
   int32_t sweeps;    /* largest number of Jacobi sweeps of a group */
   int32_t ngroups;   /* symmetry groups (block matrices) */
+  double* svd_qdim;  /* ntot multiplet dimensions of the svd records (S, qdim)
+                        sorted descending; all 1 for abelian symmetries */
 } qsx_ortho_info;
 
 /* orthoQS(PSI, K, Nkmin, Nkeep, stol, itag) <- Source/mpsortho.cc:1031-1078
@@ -195,9 +197,13 @@ typedef struct qsx_ortho_info {
    those above stol, at least nkmin; nkeep < 0: no limit) give
    A1 = U (legs 0..K-1, new leg) and A2 = V S sfac (legs K..r-1, new leg),
    PSI ~ A1 . A2 over the new leg.  itag names the new leg (NULL or "":
-   the reference's default).  Abelian symmetries and real data only
-   (QSX_ERR_UNSUPPORTED otherwise, and for a K = r-1 split whose last leg is
-   incoming, which the reference answers with an identity-CG flip). */
+   the reference's default).  Real data; abelian symmetries, and SU(2)
+   (alone or with abelian factors) for K = r-1 splits of rank <= 3, where
+   each block enters times prod cgw / sqrt(qdim) of the grouping multiplet,
+   the weights count every singular value qdim times, A1 keeps PSI's CGT
+   and A2 is the rank-2 identity, both with weights normExt.
+   QSX_ERR_UNSUPPORTED otherwise, and for a K = r-1 split whose last leg is
+   incoming, which the reference answers with an identity-CG flip. */
 int qsx_ortho(const qsx_tensor* psi, int32_t K, int32_t nkmin, int64_t nkeep, double stol, const char* itag,
               qsx_tensor_out** A1, qsx_tensor_out** A2, qsx_ortho_info** info, char* err, size_t errlen);
 void qsx_ortho_info_free(qsx_ortho_info* info);

@@ -22,6 +22,13 @@ Parity status of the oracle's functions (what pins each one):
                                          with the conjugated operand's data
                                          conjugated, zero imaginary parts =
                                          the real result bit for bit
+  * ortho (orthoQS)                   -- dense singular values, A1 isometry,
+                                         A1.A2 = PSI (kept all) or the scaled
+                                         projection / Eckart-Young (truncated),
+                                         markSet and itag rules by literal
+                                         cases; SU(2) K = r-1 in the full
+                                         M-resolved space, weights other than
+                                         normExt included
   * reference's FP64 summation order / MKL dgemm rounding: parity unpinned
     (no reference build; tolerance 1e-12 relative instead of bit equality)
 """
@@ -545,7 +552,13 @@ def _tag_conj(t):
 def ortho(PSI: QSpace, K, Nkmin=0, Nkeep=-1, stol=0.0, itag=""):
     """orthoQS(PSI,A1,A2,K,Nkmin,Nkeep,stol,tx) -- Source/mpsortho.cc:1031-1078:
     SVD_Data::blockSVD (:545-613) then dmrgTruncate (:615-954), for abelian
-    symmetries.  Returns (A1, A2, info):
+    symmetries, and for SU(2) (alone or with abelian factors) when K = r-1
+    and r <= 3: the group matrix takes each block times prod cgw /
+    sqrt(qdim_tot), the norms weight every singular value by qdim_tot, A1
+    keeps PSI's CGT (data times sqrt(qdim_tot) / normExt, weight normExt)
+    and A2 is the rank-2 identity (weight normExt = sqrt(d)); info carries
+    svd_qdim, the second column of the reference's svd records.
+    Returns (A1, A2, info):
       * blocks grouped by the label sum over legs [0] (K=1), [r-1] (K=r-1) or
         [0..K-1] (QSpace::getQsum, Source/QSpace.hh:2385-2420), groups in
         stable ascending order (groupRecs);
@@ -565,8 +578,13 @@ def ortho(PSI: QSpace, K, Nkmin=0, Nkeep=-1, stol=0.0, itag=""):
     info: svd (all singular values, descending), svd2tr, sv2tot, sfac,
     Nkeep, Ntot, flag (dmrgTruncate's count of notes)."""
     r, n = PSI.rank, PSI.nblk
-    if any(s != "A" for s in _syms(PSI)):
-        raise OracleError(5, "non-abelian symmetries (the CGT of the new leg is not restated)")
+    syms = _syms(PSI)
+    su2 = [j for j, s in enumerate(syms) if s == "SU2"]
+    if any(s not in ("A", "SU2") for s in syms):
+        raise OracleError(5, f"symmetry {PSI.qtype} not restated")
+    if su2 and (K != r - 1 or r > 3):
+        raise OracleError(5, "non-abelian symmetries need K = r-1 and rank <= 3 here (the reference's "
+                             "K = 1 split of a rank-3 tensor and outer multiplicities are not restated)")
     if PSI.is_complex():
         raise OracleError(5, "complex data")
     if n == 0:
@@ -621,21 +639,39 @@ def ortho(PSI: QSpace, K, Nkmin=0, Nkeep=-1, stol=0.0, itag=""):
         D2 = [int(np.prod(s)) for s in S2]
         cD1 = np.concatenate([[0], np.cumsum(D1)]).astype(int)
         cD2 = np.concatenate([[0], np.cumsum(D2)]).astype(int)
+        # multiplet dimension of the grouping leg (QBlock::toBlockMatrix,
+        # Source/mpsortho.cc:62-90: leg 0 for K=1, leg K otherwise)
+        l1 = 0 if K == 1 else K
+        qd = 1
+        for j in su2:
+            qd *= int(PSI.Q[l1][g[0], j]) + 1
         MM = np.zeros((cD1[-1], cD2[-1]))
         for t, i in enumerate(g):
             a, b = ri[t], ci[t]
-            MM[cD1[a]:cD1[a + 1], cD2[b]:cD2[b + 1]] = np.asarray(PSI.data[i]).reshape(D1[a], D2[b], order="F")
+            # block scale cfac = prod cgw / sqrt(qdim_tot) (mpsortho.cc:192-210)
+            cf = 1.0
+            for j in su2:
+                cf *= _cgw_of(PSI, i, j, dirs)
+            cf /= np.sqrt(float(qd))
+            blk = np.asarray(PSI.data[i]).reshape(D1[a], D2[b], order="F")
+            MM[cD1[a]:cD1[a + 1], cD2[b]:cD2[b + 1]] = blk if cf == 1.0 else blk * cf
         U, S, Vt = np.linalg.svd(MM, full_matrices=False)
         QB.append(dict(g=g, rows=[rows[x[0]] for x in g1], cols=[cols[x[0]] for x in g2], S1=S1, S2=S2,
-                       cD1=cD1, cD2=cD2, U=U, S=S, Vc=Vt.T, key=Qb[g[0]]))
+                       cD1=cD1, cD2=cD2, U=U, S=S, Vc=Vt.T, key=Qb[g[0]], qd=qd))
     SV = [b["S"] for b in QB]
     N = sum(len(s) for s in SV)
     if N == 0:
         raise OracleError(1, "got empty SVD data")
-    snorm2 = sum(float(np.sum(s * s)) for s in SV)
-    SM = np.sort(np.concatenate(SV))[::-1]
+    # norms weighted by the multiplet dimension (dmrgTruncate :668-684,
+    # QBlock::get_s2t Source/mpsortho.hh:98-106); the svd records are
+    # (S, qdim) sorted descending (SortRecs(P,-1))
+    snorm2 = sum(float(np.sum(b["S"] * b["S"])) * b["qd"] for b in QB)
+    s_all = np.concatenate(SV)
+    d_all = np.concatenate([np.full(len(b["S"]), float(b["qd"])) for b in QB])
+    order = np.lexsort((-d_all, -s_all))
+    SM, SMd = s_all[order], d_all[order]
     marks, NK = _mark_set(SV, max(Nkmin, 0), Nkeep, stol)
-    s2t = sum(float(np.sum(s[~m] ** 2)) for s, m in zip(SV, marks))
+    s2t = sum(float(np.sum(b["S"][~m] ** 2)) * b["qd"] for b, m in zip(QB, marks))
     nk = sum(int(m.sum()) for m in marks)
     flag = 0
     xtol2 = stol * 1e-14 if stol > 0 else 0.0
@@ -678,7 +714,15 @@ def ortho(PSI: QSpace, K, Nkmin=0, Nkeep=-1, stol=0.0, itag=""):
     tU = tags[:K] + [tu]
     tV = tags[K:] + [_tag_conj(tu)]
     QU, DU, QV, DV = [[] for _ in range(K + 1)], [], [[] for _ in range(r - K + 1)], []
+    WU, WV = [], []
+    dU, dV = _tag_dirs(tU, K + 1), _tag_dirs(tV, r - K + 1)
     q = PSI.qdim
+
+    def weights(recs, dd):
+        """output weights: normExt per SU(2) symmetry (NormSignW with
+        useExt, Source/clebsch.cc:5035-5052; the identity CGT of a rank-2
+        A2 has cgw = sqrt(d) = normExt, CRef::initIdentityR), 1 otherwise"""
+        return [norm_ext([rr[j] for rr in recs], dd) if j in su2 else 1.0 for j in range(q)]
     for b, m in zip(QB, marks):
         Ik = np.nonzero(m)[0]
         if not len(Ik):
@@ -691,10 +735,16 @@ def ortho(PSI: QSpace, K, Nkmin=0, Nkeep=-1, stol=0.0, itag=""):
         for a, rec in enumerate(b["rows"]):
             if b["cD1"][a + 1] == b["cD1"][a]:
                 continue  # empty sub-block: no block (initFromBlockMatrix)
-            for j in range(K):
-                QU[j].append(rec[j * q:(j + 1) * q])
-            QU[K].append(new)
+            recs = [rec[j * q:(j + 1) * q] for j in range(K)] + [new]
+            for j in range(K + 1):
+                QU[j].append(recs[j])
             blk = b["U"][b["cD1"][a]:b["cD1"][a + 1], Ik]
+            w = weights(recs, dU)
+            if su2:
+                # U keeps PSI's CGT: weight sqrt(qdim_tot) renormalised to
+                # normExt (init_wId + NormSignW, mpsortho.cc:447-452)
+                blk = blk * (np.sqrt(float(b["qd"])) / float(np.prod([w[j] for j in su2])))
+            WU.append(w)
             DU.append(np.asfortranarray(blk.reshape(b["S1"][a] + (len(Ik),), order="F")))
         VS = b["Vc"][:, Ik] * (b["S"][Ik] * sfac)[None, :]
         XQ, XD = [], []
@@ -710,14 +760,26 @@ def ortho(PSI: QSpace, K, Nkmin=0, Nkeep=-1, stol=0.0, itag=""):
                 continue
             for j in range(r - K + 1):
                 QV[j].append(xq[j])
+            WV.append(weights(xq, dV))
             DV.append(xd)
 
-    def mk(Qs, D, rank, tg):
-        return QSpace([np.array(x, dtype=np.int32).reshape(len(D), q) for x in Qs], D, PSI.qtype, tg)
+    def mk(Qs, D, W, tg):
+        cg = np.array(W).reshape(len(D), q) if su2 and D else None
+        return QSpace([np.array(x, dtype=np.int32).reshape(len(D), q) for x in Qs], D, PSI.qtype, tg, cg)
 
     info = dict(svd=SM, svd2tr=s2t / snorm2, sv2tot=snorm2, sfac=sfac, Nkeep=nk, Ntot=len(SM), flag=flag,
                 Nkmin=max(Nkmin, 0), stol=stol)
-    return mk(QU, DU, K + 1, tU), mk(QV, DV, r - K + 1, tV), info
+    if su2:
+        info["svd_qdim"] = SMd
+    return mk(QU, DU, WU, tU), mk(QV, DV, WV, tV), info
+
+
+def _cgw_of(T: QSpace, i, j, dirs) -> float:
+    """CGT weight of block i for symmetry j (normExt when T carries none)"""
+    cgw = T.info.get("cgw")
+    if cgw is not None:
+        return float(cgw[i, j])
+    return norm_ext(_block_labels(T, i, j), dirs)
 
 
 def find_openblas():

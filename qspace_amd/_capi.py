@@ -16,7 +16,7 @@ class QsxOrthoInfo(C.Structure):
     _fields_ = [("ntot", C.c_int64), ("nkeep", C.c_int64), ("nkmin", C.c_int32), ("flag", C.c_int32),
                 ("stol", C.c_double), ("svd2tr", C.c_double), ("sv2tot", C.c_double), ("sfac", C.c_double),
                 ("svd", C.POINTER(C.c_double)), ("gpu_ms", C.c_double), ("sweeps", C.c_int32),
-                ("ngroups", C.c_int32)]
+                ("ngroups", C.c_int32), ("svd_qdim", C.POINTER(C.c_double))]
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libqsx.so")

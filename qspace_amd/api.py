@@ -443,7 +443,9 @@ def orthoQS(PSI: QSpace, K: int, Nkeep: int = -1, stol: float = 0.0, Nkmin: int 
     A2 = V S sfac with legs (PSI legs K..r-1, new leg), PSI ~ A1 . A2 over
     the new leg; info holds svd (all singular values, descending), Nkeep,
     Ntot, svd2tr, sv2tot, sfac, flag, and the device time.  ``Nkeep < 0``:
-    no limit.  Abelian symmetries, real data."""
+    no limit.  Real data; abelian symmetries, and SU(2) for K = r-1 splits
+    of rank <= 3 (info then holds svd_qdim, the multiplet dimension of each
+    singular value)."""
     L = _capi.lib()
     ca, ka = to_c(PSI)
     o1 = C.POINTER(QsxTensorOut)()
@@ -459,6 +461,8 @@ def orthoQS(PSI: QSpace, K: int, Nkeep: int = -1, stol: float = 0.0, Nkmin: int 
                     Ntot=int(x.ntot), Nkmin=int(x.nkmin), flag=int(x.flag), stol=float(x.stol),
                     svd2tr=float(x.svd2tr), sv2tot=float(x.sv2tot), sfac=float(x.sfac), gpu_ms=float(x.gpu_ms),
                     sweeps=int(x.sweeps), ngroups=int(x.ngroups))
+        if any(s.strip() == "SU2" for s in (PSI.qtype or "").split(",")):
+            info["svd_qdim"] = np.ctypeslib.as_array(x.svd_qdim, shape=(max(1, x.ntot),))[:x.ntot].copy()
     finally:
         L.qsx_ortho_info_free(oi)
     return _out_to_qspace(o1), _out_to_qspace(o2), info
@@ -471,6 +475,7 @@ class DeviceBuffer:
         L = _capi.lib()
         self.n = int(n)
         self.ptr = C.c_void_p()
+        self._release = L.qsx_free_device  # held so teardown at exit need not look up the module
         err = C.create_string_buffer(256)
         _capi.check(L.qsx_malloc_device(C.byref(self.ptr), 8 * max(1, self.n), err, 256), err)
 
@@ -494,7 +499,7 @@ class DeviceBuffer:
 
     def __del__(self):
         if getattr(self, "ptr", None) and self.ptr.value:
-            _capi.lib().qsx_free_device(self.ptr)
+            self._release(self.ptr)
             self.ptr = None
 
 
@@ -512,6 +517,7 @@ class CtrPlan:
         ib_ = _ivec(ib)
         p = _ivec(perm if perm is not None else [])
         self.h = C.c_void_p()
+        self._release = L.qsx_ctr_plan_destroy
         err = C.create_string_buffer(1024)
         rc = L.qsx_ctr_plan_create(C.byref(cA), ia_.ctypes.data_as(C.POINTER(C.c_int32)), len(ia), ca, C.byref(cB),
                                    ib_.ctypes.data_as(C.POINTER(C.c_int32)), len(ib), cb,
@@ -550,7 +556,7 @@ class CtrPlan:
 
     def __del__(self):
         if getattr(self, "h", None) and self.h.value:
-            _capi.lib().qsx_ctr_plan_destroy(self.h)
+            self._release(self.h)
             self.h = None
 
 
@@ -566,6 +572,7 @@ class HPsi:
         self._opts, self._okeep = make_opts(mode=mode, hconj=hconj, fermA=fermA, nslices=nslices, skip_zero=0,
                                             verbose=verbose)
         self.h = C.c_void_p()
+        self._release = L.qsx_hpsi_plan_destroy
         err = C.create_string_buffer(1024)
         rc = L.qsx_hpsi_plan_create(C.byref(cL), C.byref(cP), C.byref(cR), C.byref(self._opts), C.byref(self.h),
                                     err, 1024)
@@ -653,7 +660,7 @@ class HPsi:
 
     def __del__(self):
         if getattr(self, "h", None) and self.h.value:
-            _capi.lib().qsx_hpsi_plan_destroy(self.h)
+            self._release(self.h)
             self.h = None
 
 

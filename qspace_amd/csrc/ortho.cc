@@ -27,10 +27,21 @@
 //    V(:,kept) * S(kept) * sfac; A2's blocks of norm <= stol*1e-14 are
 //    dropped unless A2 is a rank-2 block-diagonal matrix (SkipZeroData
 //    with bflag 'b', Source/QSpace.cc:3201-3270).
-//  * Not restated: non-abelian symmetries (the CGT of the new leg,
-//    CRef::init_wId / NormSignW) and the identity-CG flip of a K = r-1
+//  * SU(2) (alone or with abelian factors) for K = r-1 splits of rank <= 3
+//    (outer multiplicity 1): toBlockMatrix (:62-90, 192-210) scales each
+//    block by prod cgw / sqrt(qdim_tot), qdim_tot the multiplet dimension
+//    of the grouping leg; dmrgTruncate (:668-684, get_s2t in
+//    Source/mpsortho.hh:98-106) weights each squared singular value by
+//    qdim_tot and sorts the records (S, qdim_tot) descending; A1 keeps
+//    PSI's CGT with weight sqrt(qdim_tot) renormalised to normExt
+//    (QBlock::initFromBlockMatrix :447-452, CRef::init_wId and NormSignW,
+//    Source/clebsch.hh:3287-3306, clebsch.cc:5035-5052), so its data carry
+//    sqrt(qdim_tot) / normExt; A2 is the rank-2 identity, weight sqrt(d) =
+//    normExt (CRef::initIdentityR).
+//  * Not restated: the non-abelian K = 1 split of a rank-3 tensor (A2 with
+//    a permuted CGT), ranks above 3, and the identity-CG flip of a K = r-1
 //    split with an incoming last leg (revert_Uidx, :786-797 and 887-916):
-//    both are QSX_ERR_UNSUPPORTED.
+//    QSX_ERR_UNSUPPORTED.
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -65,6 +76,7 @@ struct Group {
   int64_t m = 0, n = 0;                 // stored m x n, m >= n
   int64_t a_off = 0, v_off = 0, s_off = 0;
   int item = -1;                        // SVD item (-1: no singular values)
+  int64_t qd = 1;                       // multiplet dimension (SU(2) factors)
   std::vector<double> S;                // sorted descending
   std::vector<int32_t> col;             // column of the i-th sorted value
 };
@@ -117,6 +129,14 @@ int64_t mark_desc(const std::vector<double>& E, int64_t Nkmin, int64_t Nkeep, do
   return N - 1 - l;
 }
 
+// the CGT weight of block i for symmetry j (normExt when PSI carries none)
+double block_cgw(const TensorDesc& P, int i, int j, const std::vector<int>& dir) {
+  if (P.cgw_given) return P.cgw[(size_t)i * P.nsym + j];
+  std::vector<int> tj(P.r);
+  for (int l = 0; l < P.r; ++l) tj[l] = P.label(i, l, j);
+  return su2_norm_ext(tj, dir);
+}
+
 struct OrthoResult {
   qsx_tensor_out* A1 = nullptr;
   qsx_tensor_out* A2 = nullptr;
@@ -127,11 +147,15 @@ OrthoResult ortho_oneshot(const qsx_tensor* pt, int K, int Nkmin, int64_t Nkeep,
   if (pt && pt->data_im) fail(QSX_ERR_UNSUPPORTED, "orthoQS: complex data not supported (real QSpaces only)");
   TensorDesc P = make_desc(pt, "PSI");
   const int r = P.r, q = P.qdim;
-  for (int t : P.symtype)
-    if (t != SYM_A)
-      fail(QSX_ERR_UNSUPPORTED,
-           "orthoQS: non-abelian symmetries (%s) not supported (the CGT of the new leg is not restated)",
-           P.qtype.c_str());
+  bool su2 = false;
+  for (int t : P.symtype) {
+    if (t == SYM_SU2) su2 = true;
+    else if (t != SYM_A) fail(QSX_ERR_UNSUPPORTED, "orthoQS: symmetry %s not supported", P.qtype.c_str());
+  }
+  if (su2 && (K != r - 1 || r > 3))
+    fail(QSX_ERR_UNSUPPORTED,
+         "orthoQS: non-abelian symmetries (%s) need K = r-1 and rank <= 3 (got K=%d/%d)", P.qtype.c_str(), K, r);
+  if (su2 && P.nsym != q) fail(QSX_ERR_UNSUPPORTED, "orthoQS: symmetries with several labels (%s)", P.qtype.c_str());
   if (P.n == 0) fail(QSX_ERR_ARG, "orthoQS: got empty PSI");
   if (K < 1 || K >= r) fail(QSX_ERR_ARG, "orthoQS: index specs out of bounds (%d/%d)", K, r);
   if ((int)P.itags.size() != r) fail(QSX_ERR_ITAG, "orthoQS: got invalid/empty itags (%d/%d)", (int)P.itags.size(), r);
@@ -200,6 +224,9 @@ OrthoResult ortho_oneshot(const qsx_tensor* pt, int K, int Nkmin, int64_t Nkeep,
     }
     b.dim1 = b.cD1.back();
     b.dim2 = b.cD2.back();
+    // multiplet dimension of the grouping leg (leg 0 for K = 1, leg K else)
+    for (int j = 0; j < P.nsym && su2; ++j)
+      if (P.symtype[j] == SYM_SU2) b.qd *= (int64_t)P.label(g[0], K == 1 ? 0 : K, j) + 1;
     b.trans = b.dim1 < b.dim2;
     b.m = std::max(b.dim1, b.dim2);
     b.n = std::min(b.dim1, b.dim2);
@@ -247,6 +274,12 @@ OrthoResult ortho_oneshot(const qsx_tensor* pt, int K, int Nkmin, int64_t Nkeep,
       gg.d2 = (int32_t)(b.cD2[c + 1] - b.cD2[c]);
       gg.ld = (int32_t)b.m;
       gg.trans = b.trans ? 1 : 0;
+      gg.scale = 1.0;
+      if (su2) {  // cfac = prod cgw / sqrt(qdim_tot)
+        for (int j = 0; j < P.nsym; ++j)
+          if (P.symtype[j] == SYM_SU2) gg.scale *= block_cgw(P, i, j, dir);
+        gg.scale /= std::sqrt((double)b.qd);
+      }
       gg.dst_off = b.trans ? b.a_off + b.cD2[c] + b.cD1[a] * b.m : b.a_off + b.cD1[a] + b.cD2[c] * b.m;
       if ((int64_t)gg.d1 * gg.d2 != P.numel(i)) fail(QSX_ERR_STRUCT, "orthoQS: block %d dims", i + 1);
       gath.push_back(gg);
@@ -418,7 +451,7 @@ OrthoResult ortho_oneshot(const qsx_tensor* pt, int K, int Nkmin, int64_t Nkeep,
   }
 
   // ---- truncation (dmrgTruncate)
-  std::vector<double> E;
+  std::vector<double> E, Ed;  // singular values and their multiplet dimensions
   for (auto& b : G) {
     if (b.item < 0) continue;
     b.col.resize(b.n);
@@ -428,12 +461,22 @@ OrthoResult ortho_oneshot(const qsx_tensor* pt, int K, int Nkmin, int64_t Nkeep,
     b.S.resize(b.n);
     for (int64_t i = 0; i < b.n; ++i) b.S[i] = sg[b.col[i]];
     E.insert(E.end(), b.S.begin(), b.S.end());
+    Ed.insert(Ed.end(), b.S.size(), (double)b.qd);
   }
   const int64_t N = (int64_t)E.size();
   double snorm2 = 0.0;
-  for (double x : E) snorm2 += x * x;
-  std::vector<double> SM(E);
-  std::sort(SM.begin(), SM.end(), [](double a, double b) { return a > b; });
+  for (int64_t i = 0; i < N; ++i) snorm2 += E[i] * E[i] * Ed[i];
+  // the records (S, qdim) sorted descending (SortRecs(P,-1))
+  std::vector<int64_t> Ps(N);
+  std::iota(Ps.begin(), Ps.end(), 0);
+  std::sort(Ps.begin(), Ps.end(), [&](int64_t a, int64_t b) {
+    return E[a] != E[b] ? E[a] > E[b] : Ed[a] > Ed[b];
+  });
+  std::vector<double> SM(N), SMd(N);
+  for (int64_t i = 0; i < N; ++i) {
+    SM[i] = E[Ps[i]];
+    SMd[i] = Ed[Ps[i]];
+  }
   // markSet over all values (stable ascending sort, marks mapped back)
   std::vector<char> markAll(N, 0);
   int64_t NK = Nkeep < 0 ? N : Nkeep;
@@ -455,7 +498,7 @@ OrthoResult ortho_oneshot(const qsx_tensor* pt, int K, int Nkmin, int64_t Nkeep,
   int64_t nk = 0;
   for (int64_t i = 0; i < N; ++i) {
     if (markAll[i]) ++nk;
-    else s2t += E[i] * E[i];
+    else s2t += E[i] * E[i] * Ed[i];
   }
   int flag = 0;
   double xtol2 = stol > 0 ? stol * 1e-14 : 0.0;
@@ -501,6 +544,7 @@ OrthoResult ortho_oneshot(const qsx_tensor* pt, int K, int Nkmin, int64_t Nkeep,
   std::vector<uint64_t> tU(P.itags.begin(), P.itags.begin() + K), tV(P.itags.begin() + K, P.itags.end());
   tU.push_back(tu);
   tV.push_back(tu ^ 0x80);
+  const std::vector<int> dirU = leg_dirs(tU, K + 1), dirV = leg_dirs(tV, r - K + 1);
 
   // ---- result blocks
   struct OutBlk {
@@ -537,6 +581,18 @@ OrthoResult ortho_oneshot(const qsx_tensor* pt, int K, int Nkmin, int64_t Nkeep,
       for (int j = 0; j < K; ++j)
         for (int c = 0; c < q; ++c) nl[c] += b.rows[0][(size_t)j * q + c];
     }
+    // A1 keeps PSI's CGT: the data carry sqrt(qdim_tot) / prod normExt
+    auto u_scale = [&](const std::vector<int32_t>& Qrec) {
+      if (!su2) return 1.0;
+      double ne = 1.0;
+      for (int j = 0; j < P.nsym; ++j) {
+        if (P.symtype[j] != SYM_SU2) continue;
+        std::vector<int> tj(K + 1);
+        for (int l = 0; l <= K; ++l) tj[l] = Qrec[(size_t)l * q + j];
+        ne *= su2_norm_ext(tj, dirU);
+      }
+      return std::sqrt((double)b.qd) / ne;
+    };
     // U columns: the left vectors of M (A part, or V part when transposed)
     const int64_t u_base = b.trans ? b.v_off : b.a_off, u_ld = b.trans ? b.n : b.m;
     const int64_t v_base = b.trans ? b.a_off : b.v_off, v_ld = b.trans ? b.m : b.n;
@@ -554,7 +610,7 @@ OrthoResult ortho_oneshot(const qsx_tensor* pt, int K, int Nkmin, int64_t Nkeep,
       ob.off = totU;
       ob.grp = (int)gi;
       const int64_t rows = b.cD1[a + 1] - b.cD1[a];
-      sc.push_back(SvdScatter{totU, u_base + b.cD1[a], (int32_t)rows, nkg, (int32_t)u_ld, c0u});
+      sc.push_back(SvdScatter{totU, u_base + b.cD1[a], (int32_t)rows, nkg, (int32_t)u_ld, c0u, u_scale(ob.Q)});
       totU += rows * nkg;
       U.push_back(std::move(ob));
     }
@@ -568,7 +624,7 @@ OrthoResult ortho_oneshot(const qsx_tensor* pt, int K, int Nkmin, int64_t Nkeep,
       ob.off = totX;
       ob.grp = (int)gi;
       const int64_t rows = b.cD2[c + 1] - b.cD2[c];
-      sc.push_back(SvdScatter{-1 - totX, v_base + b.cD2[c], (int32_t)rows, nkg, (int32_t)v_ld, c0v});
+      sc.push_back(SvdScatter{-1 - totX, v_base + b.cD2[c], (int32_t)rows, nkg, (int32_t)v_ld, c0v, 1.0});
       totX += rows * nkg;
       X.push_back(std::move(ob));
     }
@@ -622,7 +678,7 @@ OrthoResult ortho_oneshot(const qsx_tensor* pt, int K, int Nkmin, int64_t Nkeep,
     }
   }
   auto make = [&](const std::vector<OutBlk>& B, const std::vector<char>* keep, int rank,
-                  const std::vector<uint64_t>& tags, const double* base) {
+                  const std::vector<uint64_t>& tags, const std::vector<int>& dd, const double* base) {
     int n = 0;
     int64_t tot = 0;
     for (size_t i = 0; i < B.size(); ++i) {
@@ -632,7 +688,9 @@ OrthoResult ortho_oneshot(const qsx_tensor* pt, int K, int Nkmin, int64_t Nkeep,
       for (int64_t d : B[i].dims) ne *= d;
       tot += ne;
     }
-    qsx_tensor_out* o = alloc_out(n, rank, q, 0, P.qtype, tot);  // abelian: no weights
+    // abelian: no weights; SU(2): normExt per block (1 for abelian factors)
+    const int nsym = su2 ? P.nsym : 0;
+    qsx_tensor_out* o = alloc_out(n, rank, q, nsym, P.qtype, tot);
     int k = 0;
     for (size_t i = 0; i < B.size(); ++i) {
       if (keep && !(*keep)[i]) continue;
@@ -642,6 +700,15 @@ OrthoResult ortho_oneshot(const qsx_tensor* pt, int K, int Nkmin, int64_t Nkeep,
       std::copy(B[i].dims.begin(), B[i].dims.end(), o->dims + (size_t)k * rank);
       o->offs[k + 1] = o->offs[k] + ne;
       std::copy(base + B[i].off, base + B[i].off + ne, o->data + o->offs[k]);
+      for (int j = 0; j < nsym; ++j) {
+        double w = 1.0;
+        if (P.symtype[j] == SYM_SU2) {
+          std::vector<int> tj(rank);
+          for (int l = 0; l < rank; ++l) tj[l] = B[i].Q[(size_t)l * q + j];
+          w = su2_norm_ext(tj, dd);
+        }
+        o->cgw[(size_t)k * nsym + j] = w;
+      }
       ++k;
     }
     for (int l = 0; l < rank; ++l) o->itags[l] = tags[l];
@@ -649,8 +716,8 @@ OrthoResult ortho_oneshot(const qsx_tensor* pt, int K, int Nkmin, int64_t Nkeep,
     return o;
   };
   OrthoResult R;
-  R.A1 = make(U, nullptr, K + 1, tU, out.data());
-  R.A2 = make(X, &keepX, r - K + 1, tV, out.data() + totU);
+  R.A1 = make(U, nullptr, K + 1, tU, dirU, out.data());
+  R.A2 = make(X, &keepX, r - K + 1, tV, dirV, out.data() + totU);
   auto* info = (qsx_ortho_info*)calloc(1, sizeof(qsx_ortho_info));
   info->ntot = N;
   info->nkeep = nk;
@@ -662,6 +729,8 @@ OrthoResult ortho_oneshot(const qsx_tensor* pt, int K, int Nkmin, int64_t Nkeep,
   info->sfac = sfac;
   info->svd = (double*)calloc(std::max<int64_t>(1, N), sizeof(double));
   std::copy(SM.begin(), SM.end(), info->svd);
+  info->svd_qdim = (double*)calloc(std::max<int64_t>(1, N), sizeof(double));
+  std::copy(SMd.begin(), SMd.end(), info->svd_qdim);
   info->gpu_ms = ms;
   info->sweeps = maxsw;
   info->ngroups = (int32_t)items.size();
@@ -691,6 +760,7 @@ int qsx_ortho(const qsx_tensor* psi, int32_t K, int32_t nkmin, int64_t nkeep, do
 void qsx_ortho_info_free(qsx_ortho_info* info) {
   if (!info) return;
   free(info->svd);
+  free(info->svd_qdim);
   free(info);
 }
 

@@ -329,6 +329,7 @@ constexpr int SVD_LDS_DOUBLES = 20224;
 struct SvdGather {
   int64_t src_off, dst_off;
   int32_t d1, d2, ld, trans;
+  double scale;  // block factor (1 for abelian symmetries)
 };
 // one group matrix: A (m x n) at a_off, V (n x n) at v_off, singular values
 // (unsorted) at s_off; on return A holds the left vectors
@@ -345,6 +346,7 @@ struct SvdCol {
 struct SvdScatter {
   int64_t dst_off, src_off;
   int32_t rows, ncols, ld, col0;
+  double scale;  // block factor on top of the column scales
 };
 void launch_svd_gather(const SvdGather* ditems, int nitems, const double* src, double* work, hipStream_t s);
 // sweeps[i] = sweeps that rotated (jacobi_max_sweeps(): not converged)

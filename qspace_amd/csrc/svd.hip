@@ -331,7 +331,7 @@ __global__ __launch_bounds__(256) void k_svd_gather(const SvdGather* __restrict_
   for (int64_t e = threadIdx.x; e < n; e += 256) {
     const int64_t i = e % it.d1, j = e / it.d1;
     const int64_t d = it.trans ? it.dst_off + j + i * it.ld : it.dst_off + i + j * it.ld;
-    work[d] = src[it.src_off + e];
+    work[d] = src[it.src_off + e] * it.scale;
   }
 }
 
@@ -456,7 +456,7 @@ __global__ __launch_bounds__(256) void k_svd_scatter(const SvdScatter* __restric
   for (int64_t e = threadIdx.x; e < n; e += 256) {
     const int64_t r = e % it.rows, c = e / it.rows;
     const SvdCol col = cols[it.col0 + c];
-    out[it.dst_off + e] = work[it.src_off + r + (int64_t)col.col * it.ld] * col.scale;
+    out[it.dst_off + e] = work[it.src_off + r + (int64_t)col.col * it.ld] * col.scale * it.scale;
   }
 }
 

@@ -159,7 +159,7 @@ def test_errors():
     assert e.value.code == 2
     A, L = gen.su2_mps(seed=3, tS=(0, 1, 2), dmin=2, dmax=3)
     with pytest.raises(O.OracleError) as e:
-        O.ortho(A, 2)
+        O.ortho(A, 1)  # non-abelian K = 1 split of a rank-3 tensor
     assert e.value.code == 5
     # K = r-1 with an incoming last leg: the identity-CG flip is not restated
     M = gen.u1_mps(seed=2, Lq=(-2, 0, 2), Rq=(-1, 1), dmin=2, dmax=3)
