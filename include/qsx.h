@@ -198,10 +198,12 @@ typedef struct qsx_ortho_info {
    A1 = U (legs 0..K-1, new leg) and A2 = V S sfac (legs K..r-1, new leg),
    PSI ~ A1 . A2 over the new leg.  itag names the new leg (NULL or "":
    the reference's default).  Real data; abelian symmetries, and SU(2)
-   (alone or with abelian factors) for K = r-1 splits of rank <= 3, where
-   each block enters times prod cgw / sqrt(qdim) of the grouping multiplet,
-   the weights count every singular value qdim times, A1 keeps PSI's CGT
-   and A2 is the rank-2 identity, both with weights normExt.
+   (alone or with abelian factors) for K = 1 or r-1 splits of rank <= 3,
+   where each block enters times prod cgw / sqrt(qdim) of the grouping
+   multiplet and the weights count every singular value qdim times; for
+   K = r-1 A1 keeps PSI's CGT and A2 is the rank-2 identity, for K = 1 of
+   a rank-3 PSI A1 is the identity and A2 carries PSI's CGT permuted to
+   (legs 1, 2, new); output weights normExt.
    QSX_ERR_UNSUPPORTED otherwise, and for a K = r-1 split whose last leg is
    incoming, which the reference answers with an identity-CG flip. */
 int qsx_ortho(const qsx_tensor* psi, int32_t K, int32_t nkmin, int64_t nkeep, double stol, const char* itag,

@@ -26,7 +26,7 @@ Parity status of the oracle's functions (what pins each one):
                                          A1.A2 = PSI (kept all) or the scaled
                                          projection / Eckart-Young (truncated),
                                          markSet and itag rules by literal
-                                         cases; SU(2) K = r-1 in the full
+                                         cases; SU(2) K = 1, r-1 in the full
                                          M-resolved space, weights other than
                                          normExt included
   * reference's FP64 summation order / MKL dgemm rounding: parity unpinned
@@ -552,12 +552,14 @@ def _tag_conj(t):
 def ortho(PSI: QSpace, K, Nkmin=0, Nkeep=-1, stol=0.0, itag=""):
     """orthoQS(PSI,A1,A2,K,Nkmin,Nkeep,stol,tx) -- Source/mpsortho.cc:1031-1078:
     SVD_Data::blockSVD (:545-613) then dmrgTruncate (:615-954), for abelian
-    symmetries, and for SU(2) (alone or with abelian factors) when K = r-1
-    and r <= 3: the group matrix takes each block times prod cgw /
-    sqrt(qdim_tot), the norms weight every singular value by qdim_tot, A1
-    keeps PSI's CGT (data times sqrt(qdim_tot) / normExt, weight normExt)
-    and A2 is the rank-2 identity (weight normExt = sqrt(d)); info carries
-    svd_qdim, the second column of the reference's svd records.
+    symmetries, and for SU(2) (alone or with abelian factors) when r <= 3
+    and K = r-1 or 1: the group matrix takes each block times prod cgw /
+    sqrt(qdim_tot), the norms weight every singular value by qdim_tot; for
+    K = r-1, A1 keeps PSI's CGT (data times sqrt(qdim_tot) / normExt,
+    weight normExt) and A2 is the rank-2 identity (weight normExt =
+    sqrt(d)); for K = 1 of a rank-3 tensor, A1 is the identity and A2
+    carries PSI's CGT permuted to (legs 1, 2, new); info carries svd_qdim,
+    the second column of the reference's svd records.
     Returns (A1, A2, info):
       * blocks grouped by the label sum over legs [0] (K=1), [r-1] (K=r-1) or
         [0..K-1] (QSpace::getQsum, Source/QSpace.hh:2385-2420), groups in
@@ -582,9 +584,9 @@ def ortho(PSI: QSpace, K, Nkmin=0, Nkeep=-1, stol=0.0, itag=""):
     su2 = [j for j, s in enumerate(syms) if s == "SU2"]
     if any(s not in ("A", "SU2") for s in syms):
         raise OracleError(5, f"symmetry {PSI.qtype} not restated")
-    if su2 and (K != r - 1 or r > 3):
-        raise OracleError(5, "non-abelian symmetries need K = r-1 and rank <= 3 here (the reference's "
-                             "K = 1 split of a rank-3 tensor and outer multiplicities are not restated)")
+    if su2 and (r > 3 or K not in (1, r - 1)):
+        raise OracleError(5, "non-abelian symmetries need K = 1 or r-1 and rank <= 3 here (outer "
+                             "multiplicities are not restated)")
     if PSI.is_complex():
         raise OracleError(5, "complex data")
     if n == 0:
@@ -740,7 +742,7 @@ def ortho(PSI: QSpace, K, Nkmin=0, Nkeep=-1, stol=0.0, itag=""):
                 QU[j].append(recs[j])
             blk = b["U"][b["cD1"][a]:b["cD1"][a + 1], Ik]
             w = weights(recs, dU)
-            if su2:
+            if su2 and K == r - 1:
                 # U keeps PSI's CGT: weight sqrt(qdim_tot) renormalised to
                 # normExt (init_wId + NormSignW, mpsortho.cc:447-452)
                 blk = blk * (np.sqrt(float(b["qd"])) / float(np.prod([w[j] for j in su2])))
@@ -754,6 +756,8 @@ def ortho(PSI: QSpace, K, Nkmin=0, Nkeep=-1, stol=0.0, itag=""):
             XQ.append([rec[j * q:(j + 1) * q] for j in range(r - K)] + [new])
             blk = VS[b["cD2"][c]:b["cD2"][c + 1], :]
             XD.append(np.asfortranarray(blk.reshape(b["S2"][c] + (len(Ik),), order="F")))
+        if su2 and K == 1 and r == 3:
+            XQ, XD = _su2_first_leg_a2(PSI, dirs, b["qd"], XQ, XD, su2)
         blockdiag = r - K + 1 == 2 and all(tuple(x[0]) == tuple(x[1]) for x in XQ)
         for xq, xd in zip(XQ, XD):
             if not blockdiag and float(np.sum(xd * xd)) <= xtol2 * xtol2:
@@ -772,6 +776,27 @@ def ortho(PSI: QSpace, K, Nkmin=0, Nkeep=-1, stol=0.0, itag=""):
     if su2:
         info["svd_qdim"] = SMd
     return mk(QU, DU, WU, tU), mk(QV, DV, WV, tV), info
+
+
+def _su2_first_leg_a2(PSI: QSpace, dirs, qd, XQ, XD, su2):
+    """A2 blocks of a non-abelian K = 1 split of a rank-3 tensor
+    (initFromBlockMatrix with cgp2 = [1,2,0], Source/mpsortho.cc:440-452):
+    PSI's CGT with weight sqrt(qdim_tot) renormalised to normExt on legs
+    (new, 1, 2) -- the labels and directions of PSI's block -- then that CGT
+    permuted to (1, 2, new), restated as permute() of that tensor."""
+    q, n = PSI.qdim, len(XD)
+    Qx = [[xq[2] for xq in XQ], [xq[0] for xq in XQ], [xq[1] for xq in XQ]]
+    data = []
+    for xq, xd in zip(XQ, XD):
+        ne = 1.0
+        for j in su2:
+            ne *= norm_ext([int(xq[2][j]), int(xq[0][j]), int(xq[1][j])], dirs)
+        data.append(np.asfortranarray(np.transpose(xd, (2, 0, 1)) * (np.sqrt(float(qd)) / ne)))
+    X = QSpace([np.array(x, dtype=np.int32).reshape(n, q) for x in Qx], data, PSI.qtype, list(PSI.itags))
+    Y = permute(X, [1, 2, 0])
+    recs = [[tuple(int(v) for v in Y.Q[k][i]) for k in range(3)] for i in range(n)]
+    assert all(tuple(a) == tuple(b) for rec, xq in zip(recs, XQ) for a, b in zip(rec, xq))
+    return recs, [np.asfortranarray(d) for d in Y.data]
 
 
 def _cgw_of(T: QSpace, i, j, dirs) -> float:

@@ -443,9 +443,9 @@ def orthoQS(PSI: QSpace, K: int, Nkeep: int = -1, stol: float = 0.0, Nkmin: int 
     A2 = V S sfac with legs (PSI legs K..r-1, new leg), PSI ~ A1 . A2 over
     the new leg; info holds svd (all singular values, descending), Nkeep,
     Ntot, svd2tr, sv2tot, sfac, flag, and the device time.  ``Nkeep < 0``:
-    no limit.  Real data; abelian symmetries, and SU(2) for K = r-1 splits
-    of rank <= 3 (info then holds svd_qdim, the multiplet dimension of each
-    singular value)."""
+    no limit.  Real data; abelian symmetries, and SU(2) for K = 1 or r-1
+    splits of rank <= 3 (info then holds svd_qdim, the multiplet dimension
+    of each singular value)."""
     L = _capi.lib()
     ca, ka = to_c(PSI)
     o1 = C.POINTER(QsxTensorOut)()

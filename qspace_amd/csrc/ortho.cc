@@ -27,7 +27,7 @@
 //    V(:,kept) * S(kept) * sfac; A2's blocks of norm <= stol*1e-14 are
 //    dropped unless A2 is a rank-2 block-diagonal matrix (SkipZeroData
 //    with bflag 'b', Source/QSpace.cc:3201-3270).
-//  * SU(2) (alone or with abelian factors) for K = r-1 splits of rank <= 3
+//  * SU(2) (alone or with abelian factors) for K = 1 or r-1 splits of rank <= 3
 //    (outer multiplicity 1): toBlockMatrix (:62-90, 192-210) scales each
 //    block by prod cgw / sqrt(qdim_tot), qdim_tot the multiplet dimension
 //    of the grouping leg; dmrgTruncate (:668-684, get_s2t in
@@ -38,8 +38,10 @@
 //    Source/clebsch.hh:3287-3306, clebsch.cc:5035-5052), so its data carry
 //    sqrt(qdim_tot) / normExt; A2 is the rank-2 identity, weight sqrt(d) =
 //    normExt (CRef::initIdentityR).
-//  * Not restated: the non-abelian K = 1 split of a rank-3 tensor (A2 with
-//    a permuted CGT), ranks above 3, and the identity-CG flip of a K = r-1
+//    For K = 1 of a rank-3 tensor A1 is the rank-2 identity and A2 carries
+//    PSI's CGT permuted to (legs 1, 2, new) (cgp2 = [1,2,0]).
+//  * Not restated: ranks above 3 (outer multiplicity), and the identity-CG
+//    flip of a K = r-1
 //    split with an incoming last leg (revert_Uidx, :786-797 and 887-916):
 //    QSX_ERR_UNSUPPORTED.
 #include <algorithm>
@@ -152,9 +154,10 @@ OrthoResult ortho_oneshot(const qsx_tensor* pt, int K, int Nkmin, int64_t Nkeep,
     if (t == SYM_SU2) su2 = true;
     else if (t != SYM_A) fail(QSX_ERR_UNSUPPORTED, "orthoQS: symmetry %s not supported", P.qtype.c_str());
   }
-  if (su2 && (K != r - 1 || r > 3))
+  if (su2 && (r > 3 || (K != 1 && K != r - 1)))
     fail(QSX_ERR_UNSUPPORTED,
-         "orthoQS: non-abelian symmetries (%s) need K = r-1 and rank <= 3 (got K=%d/%d)", P.qtype.c_str(), K, r);
+         "orthoQS: non-abelian symmetries (%s) need K = 1 or r-1 and rank <= 3 (got K=%d/%d)", P.qtype.c_str(), K,
+         r);
   if (su2 && P.nsym != q) fail(QSX_ERR_UNSUPPORTED, "orthoQS: symmetries with several labels (%s)", P.qtype.c_str());
   if (P.n == 0) fail(QSX_ERR_ARG, "orthoQS: got empty PSI");
   if (K < 1 || K >= r) fail(QSX_ERR_ARG, "orthoQS: index specs out of bounds (%d/%d)", K, r);
@@ -583,7 +586,7 @@ OrthoResult ortho_oneshot(const qsx_tensor* pt, int K, int Nkmin, int64_t Nkeep,
     }
     // A1 keeps PSI's CGT: the data carry sqrt(qdim_tot) / prod normExt
     auto u_scale = [&](const std::vector<int32_t>& Qrec) {
-      if (!su2) return 1.0;
+      if (!su2 || K != r - 1) return 1.0;  // K = 1 of rank 3: A1 is the identity
       double ne = 1.0;
       for (int j = 0; j < P.nsym; ++j) {
         if (P.symtype[j] != SYM_SU2) continue;
@@ -592,6 +595,20 @@ OrthoResult ortho_oneshot(const qsx_tensor* pt, int K, int Nkmin, int64_t Nkeep,
         ne *= su2_norm_ext(tj, dirU);
       }
       return std::sqrt((double)b.qd) / ne;
+    };
+    // K = 1 of a rank-3 tensor: A2 carries PSI's CGT (weight sqrt(qdim_tot)
+    // renormalised to normExt) permuted from (new, 1, 2) to (1, 2, new), the
+    // canonical CGT of that order times su2_perm_sign (initFromBlockMatrix
+    // with cgp2 = [1,2,0], Source/mpsortho.cc:440-452)
+    auto v_scale = [&](const std::vector<int32_t>& crec) {
+      if (!su2 || K != 1 || r != 3) return 1.0;
+      double f = std::sqrt((double)b.qd);
+      for (int j = 0; j < P.nsym; ++j) {
+        if (P.symtype[j] != SYM_SU2) continue;
+        const std::vector<int> tj = {nl[j], crec[j], crec[(size_t)q + j]};
+        f *= su2_perm_sign(tj, dir, {1, 2, 0}) / su2_norm_ext(tj, dir);
+      }
+      return f;
     };
     // U columns: the left vectors of M (A part, or V part when transposed)
     const int64_t u_base = b.trans ? b.v_off : b.a_off, u_ld = b.trans ? b.n : b.m;
@@ -624,7 +641,8 @@ OrthoResult ortho_oneshot(const qsx_tensor* pt, int K, int Nkmin, int64_t Nkeep,
       ob.off = totX;
       ob.grp = (int)gi;
       const int64_t rows = b.cD2[c + 1] - b.cD2[c];
-      sc.push_back(SvdScatter{-1 - totX, v_base + b.cD2[c], (int32_t)rows, nkg, (int32_t)v_ld, c0v, 1.0});
+      sc.push_back(SvdScatter{-1 - totX, v_base + b.cD2[c], (int32_t)rows, nkg, (int32_t)v_ld, c0v,
+                              v_scale(b.cols[c])});
       totX += rows * nkg;
       X.push_back(std::move(ob));
     }

@@ -157,10 +157,6 @@ def test_errors():
     with pytest.raises(O.OracleError) as e:
         O.ortho(Q2, 2)  # mixed directions in the right group
     assert e.value.code == 2
-    A, L = gen.su2_mps(seed=3, tS=(0, 1, 2), dmin=2, dmax=3)
-    with pytest.raises(O.OracleError) as e:
-        O.ortho(A, 1)  # non-abelian K = 1 split of a rank-3 tensor
-    assert e.value.code == 5
     # K = r-1 with an incoming last leg: the identity-CG flip is not restated
     M = gen.u1_mps(seed=2, Lq=(-2, 0, 2), Rq=(-1, 1), dmin=2, dmax=3)
     M2 = QSpace(M.Q, M.data, "A", ["L", "s", "R"])

@@ -1,8 +1,9 @@
-"""Pins of the oracle's SU(2) orthoQS (K = r-1; CPU only), all in the full
-M-resolved space, where they hold whatever the CGT conventions are:
+"""Pins of the oracle's SU(2) orthoQS (K = r-1, and K = 1 of a rank-3
+tensor; CPU only), all in the full M-resolved space, where they hold
+whatever the CGT conventions are:
 
   * the singular values, each repeated by its multiplet dimension, are the
-    dense singular values of Full(PSI) split (legs 0..K-1 | leg r-1), and
+    dense singular values of Full(PSI) split (legs 0..K-1 | legs K..r-1), and
     sv2tot = |Full(PSI)|^2 (drops the sqrt(qdim) of the block scale or the
     qdim weight of the norms if either is wrong);
   * Full(A1) is an isometry over the new leg;
@@ -81,27 +82,28 @@ def _tensor(kind):
     return _weighted(gen.su2_mps(seed=12, tS=tuple(range(0, 7)), dmin=2, dmax=6)[0])
 
 
-KINDS = ["mps", "mixed", "matrix", "matrix_op", "weighted"]
+KINDS = [("mps", 2), ("mixed", 2), ("matrix", 1), ("matrix_op", 1), ("weighted", 2), ("mps", 1), ("mixed", 1),
+         ("weighted", 1)]
 
 
-def _full_parts(P, A1, A2):
+def _full_parts(P, A1, A2, K):
     j = _su2_j(P)
     r = P.rank
     lp = [_layout(_sectors(P, k, j)) for k in range(r)]
-    ln = _layout(_sectors(A1, r - 1, j))
+    ln = _layout(_sectors(A1, K, j))
     FP = expand(P, lp)
-    F1 = expand(A1, lp[:r - 1] + [ln])
-    F2 = expand(A2, [lp[r - 1], ln])
+    F1 = expand(A1, lp[:K] + [ln])
+    F2 = expand(A2, lp[K:] + [ln])
     return FP, F1, F2
 
 
-@pytest.mark.parametrize("kind", KINDS)
-def test_singular_values_and_norm(kind):
+@pytest.mark.parametrize("kind,K", KINDS)
+def test_singular_values_and_norm(kind, K):
     P = _tensor(kind)
     r = P.rank
-    A1, A2, info = O.ortho(P, r - 1)
-    FP, _, _ = _full_parts(P, A1, A2)
-    d1 = int(np.prod(FP.shape[:r - 1]))
+    A1, A2, info = O.ortho(P, K)
+    FP, _, _ = _full_parts(P, A1, A2, K)
+    d1 = int(np.prod(FP.shape[:K]))
     s = np.linalg.svd(FP.reshape(d1, -1), compute_uv=False)
     rep = np.sort(np.repeat(info["svd"], info["svd_qdim"].astype(int)))[::-1]
     assert len(rep) <= len(s)
@@ -113,16 +115,16 @@ def test_singular_values_and_norm(kind):
     assert rec == sorted(rec, reverse=True)
 
 
-@pytest.mark.parametrize("kind", KINDS)
-def test_keep_all_reconstructs_with_an_isometry(kind):
+@pytest.mark.parametrize("kind,K", KINDS)
+def test_keep_all_reconstructs_with_an_isometry(kind, K):
     P = _tensor(kind)
     r = P.rank
-    A1, A2, info = O.ortho(P, r - 1)
+    A1, A2, info = O.ortho(P, K)
     assert info["Nkeep"] == info["Ntot"] and info["sfac"] == 1.0
-    FP, F1, F2 = _full_parts(P, A1, A2)
+    FP, F1, F2 = _full_parts(P, A1, A2, K)
     M1 = F1.reshape(-1, F1.shape[-1])
     assert np.allclose(M1.T @ M1, np.eye(M1.shape[1]), atol=1e-12)
-    X = np.tensordot(F1, F2, axes=([r - 1], [1]))
+    X = np.tensordot(F1, F2, axes=([K], [r - K]))
     assert np.allclose(X, FP, atol=1e-12 * np.abs(FP).max())
     # output weights are normExt (the default) on every SU(2) block
     j = _su2_j(P)
@@ -133,17 +135,18 @@ def test_keep_all_reconstructs_with_an_isometry(kind):
             assert T.info["cgw"][i, j] == ne
 
 
-@pytest.mark.parametrize("kind,nkeep", [("mps", 4), ("mixed", 6), ("matrix", 2), ("weighted", 3)])
-def test_truncation_is_the_scaled_projection(kind, nkeep):
+@pytest.mark.parametrize("kind,K,nkeep", [("mps", 2, 4), ("mixed", 2, 6), ("matrix", 1, 2), ("weighted", 2, 3),
+                                           ("mps", 1, 4), ("mixed", 1, 5), ("weighted", 1, 3)])
+def test_truncation_is_the_scaled_projection(kind, K, nkeep):
     P = _tensor(kind)
     r = P.rank
-    A1, A2, info = O.ortho(P, r - 1, Nkeep=nkeep)
+    A1, A2, info = O.ortho(P, K, Nkeep=nkeep)
     assert info["Nkeep"] == nkeep < info["Ntot"]
-    FP, F1, F2 = _full_parts(P, A1, A2)
+    FP, F1, F2 = _full_parts(P, A1, A2, K)
     M1 = F1.reshape(-1, F1.shape[-1])
     assert np.allclose(M1.T @ M1, np.eye(M1.shape[1]), atol=1e-12)
-    X = np.tensordot(F1, F2, axes=([r - 1], [1]))
-    d1 = int(np.prod(FP.shape[:r - 1]))
+    X = np.tensordot(F1, F2, axes=([K], [r - K]))
+    d1 = int(np.prod(FP.shape[:K]))
     proj = (M1 @ (M1.T @ FP.reshape(d1, -1))).reshape(FP.shape)
     assert np.allclose(X, info["sfac"] * proj, atol=1e-12 * np.abs(FP).max())
     assert np.sum(X * X) == pytest.approx(np.sum(FP * FP), rel=1e-12)
@@ -154,7 +157,10 @@ def test_truncation_is_the_scaled_projection(kind, nkeep):
 
 
 def test_unsupported_su2_splits():
-    T = gen.su2_mps(seed=11, tS=tuple(range(0, 5)), dmin=1, dmax=3)[0]
-    with pytest.raises(O.OracleError) as e:
-        O.ortho(T, 1)
-    assert e.value.code == 5
+    # rank 4 (outer multiplicity of the fused legs): not restated
+    T4 = QSpace([np.array([[0], [1]], dtype=np.int32) for _ in range(4)], [np.ones((2, 1, 1, 2))] * 2, "SU2",
+                ["a", "b", "c*", "d*"])
+    for K in (1, 2, 3):
+        with pytest.raises(O.OracleError) as e:
+            O.ortho(T4, K)
+        assert e.value.code == 5
