@@ -365,36 +365,42 @@ def truncation():
     (s2,R): 11 charge sectors per bond, "lds" with U[4,32] states per sector
     (group matrices up to 64 x 64, decomposed in LDS), "bond1000" with
     U[40,140] (about 1000 states per bond, group matrices about 180 x 180,
-    decomposed in place in HBM/L2); Nkeep = 512.  GPU: device time of the
-    call (events: assembly, Jacobi SVD, result blocks) and its wall time,
-    best of 3; CPU: the oracle (numpy LAPACK gesdd), wall time."""
+    decomposed in place in HBM/L2); Nkeep = 512.  And an SU(2) MPS tensor
+    A(L,s,R*) (17 multiplets per bond, U[8,64] per multiplet, group matrices
+    up to 128 x 64) orthonormalised to the left (K = 2) and to the right
+    (K = 1), Nkeep = 64 multiplets.  GPU: device time of the call (events:
+    assembly, Jacobi SVD, result blocks) and its wall time, best of 3; CPU:
+    the oracle (numpy LAPACK gesdd), wall time."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     import qspace_amd as Q
     from qspace_amd import gen
+    cases = [(f"u1_two_site_psi_{name}", gen.u1_two_site_psi(seed=1009, dmin=dmin, dmax=dmax), 2, 512)
+             for name, dmin, dmax in (("lds", 4, 32), ("bond1000", 40, 140))]
+    A = gen.su2_mps(seed=1002)[0]
+    cases += [("su2_mps_left", A, 2, 64), ("su2_mps_right", A, 1, 64)]
     out = {}
-    for name, dmin, dmax in (("lds", 4, 32), ("bond1000", 40, 140)):
-        P = gen.u1_two_site_psi(seed=1009, dmin=dmin, dmax=dmax)
-        Q.orthoQS(P, 2, Nkeep=512)
+    for key, P, K, nkeep in cases:
+        Q.orthoQS(P, K, Nkeep=nkeep)
         best, wall = None, []
         for _ in range(3):
             t0 = time.perf_counter()
-            A1, A2, info = Q.orthoQS(P, 2, Nkeep=512)
+            A1, A2, info = Q.orthoQS(P, K, Nkeep=nkeep)
             wall.append((time.perf_counter() - t0) * 1e3)
             best = info if best is None or info["gpu_ms"] < best["gpu_ms"] else best
         t0 = time.perf_counter()
-        R1, R2, rinfo = O.ortho(P, 2, Nkeep=512)
+        R1, R2, rinfo = O.ortho(P, K, Nkeep=nkeep)
         cpu_ms = (time.perf_counter() - t0) * 1e3
         smax = float(rinfo["svd"][0])
-        X = O.contract(A1, [2], A2, [2])
-        XR = O.contract(R1, [2], R2, [2])
-        out[f"u1_two_site_psi_{name}"] = {
-            "blocks": P.nblk, "elements": P.numel(), "groups": best["ngroups"], "sweeps": best["sweeps"],
+        X = O.contract(A1, [K], A2, [P.rank - K])
+        XR = O.contract(R1, [K], R2, [P.rank - K])
+        out[key] = {
+            "blocks": P.nblk, "elements": P.numel(), "K": K, "groups": best["ngroups"], "sweeps": best["sweeps"],
             "Ntot": best["Ntot"], "Nkeep": best["Nkeep"], "gpu_ms": best["gpu_ms"], "gpu_wall_ms": min(wall),
             "oracle_cpu_ms": cpu_ms,
             "svd_max_err_rel": float(np.abs(best["svd"] - rinfo["svd"]).max() / smax),
             "product_rel_err_vs_oracle": _rel(X, XR)}
-        log(f"[bench] truncation {name}: {out[f'u1_two_site_psi_{name}']}")
+        log(f"[bench] truncation {key}: {out[key]}")
     return out
 
 
