@@ -4,7 +4,8 @@ Each seed draws U(1) tensors (one or two charge labels per leg) of rank 2-4
 whose contracted legs sit at random positions, random conj flags and output
 permutations, and checks contractQS, permuteQS, plusQS and orthoQS through
 the C-ABI against the oracle on the same inputs: structure bit-exact, data
-within 1e-12 relative (bit-exact for permute / plus).
+within 1e-12 relative (bit-exact for permute / plus); SU(2) contractions
+and truncations of rank-3 tensors likewise.
 """
 import itertools
 
@@ -198,3 +199,40 @@ def test_random_su2_contractions(gpu, seed):
     assert_same_structure(C, R)
     if R.nblk:
         assert rel_err(C, R) < 1e-12
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_su2_truncations(gpu, seed):
+    """SU(2) orthoQS of random rank-3 tensors (two legs incoming, the last
+    outgoing for K = 2; any directions for K = 1) and random rank-2 (in,
+    out) matrices"""
+    rng = np.random.RandomState(400 + seed)
+    if seed % 4 == 3:
+        legs = [_su2_leg(rng)]
+        legs.append(dict(legs[0]))
+        for t in legs[1]:
+            legs[1][t] = int(rng.randint(1, 5))
+        T = _su2_tensor(rng, legs, [1, -1], ["a", "b*"])
+        K = 1
+    else:
+        K = 1 + seed % 2
+        dirs = [1, 1, -1] if K == 2 else [1 if rng.rand() < 0.5 else -1 for _ in range(3)]
+        T = _su2_tensor(rng, [_su2_leg(rng) for _ in range(3)], dirs,
+                        [f"x{i}" + ("*" if d < 0 else "") for i, d in enumerate(dirs)])
+    if not T.nblk:
+        pytest.skip("empty draw")
+    nkeep = int(rng.choice([-1, 2, 6]))
+    stol = float(rng.choice([0.0, 0.5]))
+    print(f"seed {seed}: {T.itags} blocks {T.nblk} K {K} Nkeep {nkeep} stol {stol}", flush=True)
+    A1, A2, info = Q.orthoQS(T, K, Nkeep=nkeep, stol=stol)
+    R1, R2, rinfo = O.ortho(T, K, Nkeep=nkeep, stol=stol)
+    assert_same_structure(A1, R1)
+    assert_same_structure(A2, R2)
+    assert (info["Nkeep"], info["Ntot"], info["flag"]) == (rinfo["Nkeep"], rinfo["Ntot"], rinfo["flag"])
+    assert np.array_equal(info["svd_qdim"], rinfo["svd_qdim"])
+    assert np.abs(info["svd"] - rinfo["svd"]).max() <= 1e-12 * max(1.0, rinfo["svd"][0])
+    if A1.nblk:
+        X = O.contract(A1, [K], A2, [T.rank - K])
+        XR = O.contract(R1, [K], R2, [T.rank - K])
+        assert_same_structure(X, XR)
+        assert rel_err(X, XR) < 1e-12

@@ -164,3 +164,23 @@ def test_unsupported_su2_splits():
         with pytest.raises(O.OracleError) as e:
             O.ortho(T4, K)
         assert e.value.code == 5
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_k1_split_any_directions(seed):
+    """K = 1 of rank-3 SU(2) tensors with random leg directions: A1.A2 =
+    PSI in the full space and A1 an isometry (the permutation sign of A2's
+    CGT differs between direction patterns)"""
+    from test_gpu_random import _su2_leg, _su2_tensor
+    rng = np.random.RandomState(500 + seed)
+    dirs = [1 if rng.rand() < 0.5 else -1 for _ in range(3)]
+    P = _su2_tensor(rng, [_su2_leg(rng) for _ in range(3)], dirs,
+                    [f"x{i}" + ("*" if d < 0 else "") for i, d in enumerate(dirs)])
+    if not P.nblk:
+        pytest.skip("empty draw")
+    A1, A2, info = O.ortho(P, 1)
+    FP, F1, F2 = _full_parts(P, A1, A2, 1)
+    M1 = F1.reshape(-1, F1.shape[-1])
+    assert np.allclose(M1.T @ M1, np.eye(M1.shape[1]), atol=1e-12)
+    X = np.tensordot(F1, F2, axes=([1], [2]))
+    assert np.allclose(X, FP, atol=1e-12 * np.abs(FP).max())
