@@ -44,6 +44,10 @@ WORKLOADS = {
                      name="su2_heisenberg_two_site_hpsi_Dstar4000"),
     "heis2000": dict(kind="heis", seed=1003, Dstar=2000, mu=3.0, sigma=0.8, dmax=256,
                      name="su2_heisenberg_two_site_hpsi_Dstar2000"),
+    "heis8000": dict(kind="heis", seed=1003, Dstar=8000, mu=3.0, sigma=0.8, dmax=256,
+                     name="su2_heisenberg_two_site_hpsi_Dstar8000"),
+    "heis16000": dict(kind="heis", seed=1003, Dstar=16000, mu=3.0, sigma=0.8, dmax=256,
+                      name="su2_heisenberg_two_site_hpsi_Dstar16000"),
     "hubbard4000": dict(kind="hubbard", seed=1005, Dstar=4000, mu=3.2, sigma=0.9, dmax=384,
                         name="su2xu1_hubbard_two_site_hpsi_Dstar4000"),
 }
