@@ -279,9 +279,7 @@ int qsx_hpsi_stage_info(const qsx_hpsi_plan* plan, double* info);
    the k-range is split over workgroups, [7] wave tiles: XCD-aware
    workgroup order over this many L2 domains (0 or 1: longest first only),
    [8] wave tiles of at most 2x2 fragments: waves per workgroup = waves
-   sharing one long tile (4, 8 or 16), [9] H*psi, stage-1 policy only: 1 =
-   both stages in ONE launch (partitioned by psi's row label over the XCDs,
-   stage 2 waits per partition on an L2 counter), [10..11] reserved (0).  Any policy gives
+   sharing one long tile (4, 8 or 16), [9..11] reserved (0).  Any policy gives
    the same result to rounding (k-splits change the summation grouping). */
 #define QSX_POLICY_INTS 12
 /* autotune the work decomposition of both stages by timing candidates on
@@ -291,9 +289,6 @@ int qsx_hpsi_stage_info(const qsx_hpsi_plan* plan, double* info);
 int qsx_hpsi_autotune(qsx_hpsi_plan* plan, int32_t reps, double* stage_ms, int32_t* policy);
 /* set the policies of both stages (policy[2*QSX_POLICY_INTS]) */
 int qsx_hpsi_set_policy(qsx_hpsi_plan* plan, const int32_t* policy);
-/* 0, or QSX_ERR_DEVICE if a bounded wait of the single-launch H*psi ever
-   gave up (its results are then invalid); synchronises the device */
-int qsx_hpsi_status(const qsx_hpsi_plan* plan);
 /* capture/replay the H*psi in a hipGraph (0 = direct launches) */
 int qsx_hpsi_use_graph(qsx_hpsi_plan* plan, int32_t on);
 /* per-kernel device timing of the last direct (non-graph) apply, HIP events

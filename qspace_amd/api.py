@@ -16,7 +16,7 @@ import numpy as np
 from . import _capi
 from .qspace import QSpace, QsxOpts, QsxTensorOut, from_out, parse_ctridx, to_c
 
-__all__ = ["contractQS", "permuteQS", "plusQS", "orthoQS", "DeviceBuffer", "CtrPlan", "HPsi", "su2_cg", "su2_x3", "make_opts"]
+__all__ = ["contractQS", "permuteQS", "plusQS", "orthoQS", "ortho_args", "DeviceBuffer", "CtrPlan", "HPsi", "su2_cg", "su2_x3", "make_opts"]
 
 
 def make_opts(mode="batched", nslices=0, skip_zero=1, hconj=0, fermA=None, fermB=None, verbose=1):
@@ -39,11 +39,11 @@ def make_opts(mode="batched", nslices=0, skip_zero=1, hconj=0, fermA=None, fermB
 
 
 POLICY_KEYS = ("engine", "split_steps", "tile_overhead", "slots", "wave_max", "group_mfma", "split_mfma", "xcd_bins",
-               "group_waves", "fused", "r10", "r11")
+               "group_waves", "r9", "r10", "r11")
 NPOL = len(POLICY_KEYS)
 ENGINES = {"grouped": 0, "wave": 1}
 _POLICY_DEFAULT = {"engine": 1, "split_steps": 16, "tile_overhead": 1024, "slots": 1024, "wave_max": 2,
-                   "group_mfma": 64, "split_mfma": 1 << 30, "xcd_bins": 8, "group_waves": 4, "fused": 0,
+                   "group_mfma": 64, "split_mfma": 1 << 30, "xcd_bins": 8, "group_waves": 4, "r9": 0,
                    "r10": 0, "r11": 0}
 
 
@@ -435,17 +435,48 @@ def plusQS(A: QSpace, B: QSpace, bfac: float = 1.0, skip_zero: int = 1) -> QSpac
     return _out_to_qspace(out)
 
 
-def orthoQS(PSI: QSpace, K: int, Nkeep: int = -1, stol: float = 0.0, Nkmin: int = 0, itag: str = ""):
+STOL_SVD = 1e-8  # WB_STOL_SVD, Source/mpsortho.hh:4
+
+
+def ortho_args(PSI: QSpace, K: int, Nkeep=None, stol=None):
+    """(Nkeep, stol) as the orthoQS front end resolves them
+    (MPS_ORTHO_1, Source/orthoQS.cc:137-170): stol defaults to
+    WB_STOL_SVD = 1e-8, or to 0 when Nkeep > 0 is given without stol; a
+    missing or negative Nkeep becomes get_Nkeep_estimate (orthoQS.cc:39-56):
+    with d1 = prod of the leg dims of legs 0..K-1 and d2 that of legs
+    K..r-1 (QSpace::getDim: per leg the sum of the block dims over its
+    distinct labels, multiplets for non-abelian symmetries), min(d1, d2) for
+    abelian symmetries and max(d1, d2) otherwise."""
+    if stol is None:
+        stol = 0.0 if (Nkeep is not None and Nkeep > 0) else STOL_SVD
+    if Nkeep is None or Nkeep < 0:
+        dims = PSI.dims()
+        leg = []
+        for k in range(PSI.rank):
+            seen = {}
+            for i in range(PSI.nblk):
+                seen.setdefault(tuple(np.asarray(PSI.Q[k][i]).ravel()), int(dims[i, k]))
+            leg.append(sum(seen.values()))
+        d1 = int(np.prod(leg[:K], dtype=np.int64))
+        d2 = int(np.prod(leg[K:], dtype=np.int64))
+        abelian = all(x.strip() in ("", "A") for x in (PSI.qtype or "").split(","))
+        Nkeep = min(d1, d2) if abelian else max(d1, d2)
+    return int(Nkeep), float(stol)
+
+
+def orthoQS(PSI: QSpace, K: int, Nkeep=None, stol=None, Nkmin: int = 0, itag: str = ""):
     """orthoQS(PSI, K, Nkeep, stol, Nkmin, itag) -- the truncation core of
-    Source/mpsortho.cc:1031-1078 (blockSVD + dmrgTruncate) on the device.
+    Source/mpsortho.cc:1031-1078 (blockSVD + dmrgTruncate) on the device,
+    with the front end's defaults for Nkeep and stol (``ortho_args``).
 
     Returns (A1, A2, info): A1 = U with legs (PSI legs 0..K-1, new leg),
     A2 = V S sfac with legs (PSI legs K..r-1, new leg), PSI ~ A1 . A2 over
     the new leg; info holds svd (all singular values, descending), Nkeep,
-    Ntot, svd2tr, sv2tot, sfac, flag, and the device time.  ``Nkeep < 0``:
-    no limit.  Real data; abelian symmetries, and SU(2) for K = 1 or r-1
-    splits of rank <= 3 (info then holds svd_qdim, the multiplet dimension
-    of each singular value)."""
+    Ntot, svd2tr, sv2tot, sfac, flag, and the device time.  Real data;
+    abelian symmetries, and SU(2) for K = 1 or r-1 splits of rank <= 3
+    (info then holds svd_qdim, the multiplet dimension of each singular
+    value)."""
+    Nkeep, stol = ortho_args(PSI, K, Nkeep, stol)
     L = _capi.lib()
     ca, ka = to_c(PSI)
     o1 = C.POINTER(QsxTensorOut)()
@@ -608,14 +639,9 @@ class HPsi:
         return {"stage_ms": (ms[0], ms[1]), "policy": (_policy_dict(pol[:NPOL]), _policy_dict(pol[NPOL:]))}
 
     def set_policy(self, stage1: dict, stage2: dict):
-        """work decomposition per stage (keys of POLICY_KEYS); stage1["fused"]
-        = 1 runs both stages in one launch"""
+        """work decomposition per stage (keys of POLICY_KEYS)"""
         v = (C.c_int32 * (2 * NPOL))(*(_policy_ints(stage1) + _policy_ints(stage2)))
         _capi.check(_capi.lib().qsx_hpsi_set_policy(self.h, v), None)
-
-    def status(self) -> int:
-        """0, or QSX_ERR_DEVICE if a bounded wait of the single-launch path gave up"""
-        return _capi.lib().qsx_hpsi_status(self.h)
 
     def weights(self) -> np.ndarray:
         w = np.zeros(self.N)

@@ -1,5 +1,5 @@
-// hpsi.cc -- two-site H*psi as two batched-GEMM launches (or one fused
-// launch, or the split-precision BF16 path), replayed from a hipGraph, and
+// hpsi.cc -- two-site H*psi as two batched-GEMM launches (or the
+// split-precision BF16 path), replayed from a hipGraph, and
 // the FP64 Lanczos / Davidson loops around it.
 //
 // Reference: get_HPsi.m:54-57 (Class/@Hamilton1D/private),
@@ -86,17 +86,7 @@ struct HpsiPlan {
   WorkList swork1, swork2;
   WorkItem *dsitems1 = nullptr, *dsitems2 = nullptr;
   double split_exec_flops[2] = {0, 0};
-  // fused single-launch H*psi (both stages, XCD partitions by psi's row label)
-  bool fused = false;
-  static constexpr int kParts = 8;
-  std::vector<WorkItem> fitems;
-  int n1wg = 0;
-  WorkItem* dfitems = nullptr;
-  unsigned long long* dcnt = nullptr;  // 2 * kParts monotonic counters
-  int32_t* dn12 = nullptr;             // counting workgroups per partition, stage 1 then 2
-  int32_t* derr = nullptr;
   ~HpsiPlan() {
-    for (void* p : {(void*)dfitems, (void*)dcnt, (void*)dn12, (void*)derr}) dev_free(p);
     if (gexec) (void)hipGraphExecDestroy(gexec);
     for (SplitSide* sp : {&spHL, &spPsi, &spT, &spHR}) {
       dev_free(sp->planes);
@@ -222,99 +212,6 @@ static void hpsi_build_work(HpsiPlan& H) {
   }
 }
 
-// ---------------------------------------------------------------- fused launch
-// Work list of the single-launch H*psi: both stages' targets are dealt to
-// kParts partitions by the row label of psi's first leg (longest first over
-// the labels' total cost); per partition and stage the wave-tile work list
-// is built as usual (no k-splits across workgroups: there is no reduction
-// launch); the workgroups are then interleaved so that workgroup i belongs
-// to partition i % kParts, stage 1 before stage 2.
-static void hpsi_build_fused(HpsiPlan& H) {
-  hipStream_t s = default_stream();
-  const int P = HpsiPlan::kParts;
-  const int q = H.psi.qdim;
-  std::vector<std::string> key1, key2;
-  for (int t = 0; t < H.nterm; ++t) {
-    const CtrPlan& C = *H.s1[t];
-    for (int c = 0; c < C.nC; ++c) key1.push_back(rec_key(&C.Qout[(size_t)c * C.rc * q], q));
-  }
-  for (int i = 0; i < H.psi.n; ++i) key2.push_back(rec_key(&H.psi.Q[(size_t)i * H.psi.r * q], q));
-  if (key1.size() != H.targets1.size() || key2.size() != H.targets2.size())
-    fail(QSX_ERR_STRUCT, "fused H*psi: target bookkeeping mismatch");
-  std::map<std::string, double> cost;
-  auto tcost = [](const WorkTarget& t, const std::vector<PairDesc>& pr) {
-    double c = 0;
-    for (int64_t p = t.p0; p < t.p1; ++p) c += (double)t.M * t.N * pr[p].K;
-    return c + 1.0;
-  };
-  for (size_t g = 0; g < H.targets1.size(); ++g) cost[key1[g]] += tcost(H.targets1[g], H.pairs1);
-  for (size_t g = 0; g < H.targets2.size(); ++g) cost[key2[g]] += tcost(H.targets2[g], H.pairs2);
-  std::vector<std::pair<double, std::string>> keys;
-  for (auto& kv : cost) keys.push_back({kv.second, kv.first});
-  std::stable_sort(keys.begin(), keys.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
-  std::map<std::string, int> part;
-  std::vector<double> load(P, 0.0);
-  for (auto& k : keys) {
-    const int b = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-    load[b] += k.first;
-    part[k.second] = b;
-  }
-  WorkPolicy pol[2] = {H.pol1, H.pol2};
-  std::vector<std::vector<WorkItem>> wl[2];
-  std::vector<int32_t> n12(2 * P, 0);
-  for (int st = 0; st < 2; ++st) {
-    WorkPolicy& p = pol[st];
-    p.engine = ENG_WAVE;
-    p.wave_max = std::min(2, std::max(1, p.wave_max));
-    p.group_waves = 4;
-    p.split_mfma = 1 << 30;
-    p.xcd_bins = 1;
-    const auto& tg = st == 0 ? H.targets1 : H.targets2;
-    const auto& ky = st == 0 ? key1 : key2;
-    const auto& pr = st == 0 ? H.pairs1 : H.pairs2;
-    wl[st].resize(P);
-    for (int b = 0; b < P; ++b) {
-      std::vector<WorkTarget> sub;
-      for (size_t g = 0; g < tg.size(); ++g)
-        if (part[ky[g]] == b) sub.push_back(tg[g]);
-      WorkList W = build_work(sub, pr, p);
-      if (W.engine != ENG_WAVE || W.wg_waves != 4 || !W.reds.empty() || W.items.size() % 4)
-        fail(QSX_ERR_STRUCT, "fused H*psi: unexpected work list");
-      for (WorkItem& w : W.items) w.pad |= kCounts;  // these workgroups count for the partition
-      n12[st * P + b] = (int32_t)(W.items.size() / 4);
-      wl[st][b] = std::move(W.items);
-    }
-  }
-  H.fitems.clear();
-  WorkItem pad{};
-  pad.slot = -2;
-  pad.p0 = 0;
-  pad.p1 = -1;
-  pad.shape = wave_shape(1, 1);
-  pad.pad = 0;
-  for (int st = 0; st < 2; ++st) {
-    int depth = 0;
-    for (int b = 0; b < P; ++b) depth = std::max(depth, n12[st * P + b]);
-    for (int r = 0; r < depth; ++r)
-      for (int b = 0; b < P; ++b)
-        for (int v = 0; v < 4; ++v)
-          H.fitems.push_back(r < n12[st * P + b] ? wl[st][b][(size_t)r * 4 + v] : pad);
-    if (st == 0) H.n1wg = depth * P;
-  }
-  for (void* p : {(void*)H.dfitems, (void*)H.dcnt, (void*)H.dn12, (void*)H.derr}) dev_free(p);
-  H.dfitems = upload_vec(H.fitems, s);
-  H.dn12 = upload_vec(n12, s);
-  H.dcnt = (unsigned long long*)dev_alloc(sizeof(unsigned long long) * 2 * P);
-  H.derr = (int32_t*)dev_alloc(sizeof(int32_t));
-  QSX_HIP(hipMemsetAsync(H.dcnt, 0, sizeof(unsigned long long) * 2 * P, s));
-  QSX_HIP(hipMemsetAsync(H.derr, 0, sizeof(int32_t), s));
-  QSX_HIP(hipStreamSynchronize(s));
-  if (H.gexec) {
-    (void)hipGraphExecDestroy(H.gexec);
-    H.gexec = nullptr;
-  }
-}
-
 // ---------------------------------------------------------------- autotuner
 // Tile shapes and k-splits come from timing candidate work decompositions of
 // each stage on the plan itself (HIP events, median of reps).  The result is
@@ -375,21 +272,6 @@ static double time_stage(HpsiPlan& H, int stage, const WorkList& W, const double
 
 static void hpsi_launch(HpsiPlan& H, const double* dpsi, double* dhpsi, hipStream_t s, bool timed, bool sym);
 
-// median device time (events) of whole H*psi applies, direct launches
-static double time_hpsi_total(HpsiPlan& H, const double* dpsi, double* dout, int reps) {
-  hipStream_t s = default_stream();
-  std::vector<float> ms;
-  for (int r = 0; r < reps + 2; ++r) {
-    hpsi_launch(H, dpsi, dout, s, true, false);
-    QSX_HIP(hipEventSynchronize(H.ev[2]));
-    float t = 0;
-    QSX_HIP(hipEventElapsedTime(&t, H.ev[0], H.ev[2]));
-    if (r >= 2) ms.push_back(t);
-  }
-  std::sort(ms.begin(), ms.end());
-  return ms[ms.size() / 2];
-}
-
 static void hpsi_autotune(HpsiPlan& H, int reps, double* out) {
   if (H.mode != QSX_MODE_BATCHED) {  // the other modes have fixed work lists
     if (out) out[0] = out[1] = -1;
@@ -410,8 +292,6 @@ static void hpsi_autotune(HpsiPlan& H, int reps, double* out) {
     H.pol1 = cached.first;
     H.pol2 = cached.second;
     hpsi_build_work(H);
-    H.fused = H.pol1.fused;
-    if (H.fused) hpsi_build_fused(H);
     if (out) out[0] = out[1] = -1;
     return;
   }
@@ -470,17 +350,6 @@ static void hpsi_autotune(HpsiPlan& H, int reps, double* out) {
     (stage == 0 ? H.pol1 : H.pol2) = bp;
     if (out) out[stage] = best;
   }
-  // two launches or one: time the whole H*psi both ways
-  H.pol1.fused = 0;
-  hpsi_build_work(H);
-  H.fused = false;
-  const double t_two = time_hpsi_total(H, dpsi, dout, reps);
-  hpsi_build_fused(H);
-  H.fused = true;
-  const double t_one = time_hpsi_total(H, dpsi, dout, reps);
-  H.pol1.fused = t_one < t_two ? 1 : 0;
-  H.fused = H.pol1.fused;
-  if (H.verbose) fprintf(stderr, "qsx autotune H*psi: two launches %.4f ms, one launch %.4f ms\n", t_two, t_one);
   dev_free(dpsi);
   dev_free(dout);
   {
@@ -488,7 +357,6 @@ static void hpsi_autotune(HpsiPlan& H, int reps, double* out) {
     g_tune_cache[key] = {H.pol1, H.pol2};
   }
   hpsi_build_work(H);
-  if (H.fused) hpsi_build_fused(H);
 }
 
 // ---------------------------------------------------------------- split precision
@@ -600,11 +468,7 @@ static void hpsi_launch(HpsiPlan& H, const double* dpsi, double* dhpsi, hipStrea
   const PairDesc* p2 = sym ? H.dpairs2s : H.dpairs2;
   // events: [0] start, [1] stage 1 done, [2] stage 2 done
   if (timed) QSX_HIP(hipEventRecord(H.ev[0], s));
-  if (H.fused && H.mode == QSX_MODE_BATCHED) {
-    launch_hpsi_fused(H.dfitems, (int)H.fitems.size(), H.n1wg, p1, p2, H.dHL, dpsi, H.dT, H.dHR, dhpsi, H.dcnt,
-                      H.dn12, H.derr, HpsiPlan::kParts, s);
-    if (timed) QSX_HIP(hipEventRecord(H.ev[1], s));
-  } else if (H.mode == QSX_MODE_SPLIT_BF16) {
+  if (H.mode == QSX_MODE_SPLIT_BF16) {
     const int S = H.nsl;
     launch_split_lines(H.spPsi.dlines, H.spPsi.nlines, dpsi, H.spPsi.planes, H.spPsi.plane, S, H.spPsi.exps, s);
     launch_split_gemm(S, H.dsitems1, (int)H.swork1.items.size(), sym ? H.dsp1s : H.dsp1, H.spHL.planes,
@@ -929,17 +793,32 @@ static void lanczos(HpsiPlan& H, const double* psi0, int niter, double* e0, doub
     std::vector<double> b(niter + 1), al(niter);
     QSX_HIP(hipMemcpy(b.data(), beta, sizeof(double) * niter, hipMemcpyDeviceToHost));
     QSX_HIP(hipMemcpy(al.data(), alpha, sizeof(double) * niter, hipMemcpyDeviceToHost));
-    // Ritz values of every T_k, lowest eigenpair of T_niter
+    // breakdown: a beta_j at rounding level means the Krylov space of psi0
+    // is invariant after j+1 vectors (psi0 inside an invariant subspace, or
+    // niter beyond its dimension); the later vectors are then zero or
+    // normalised rounding noise, not Lanczos vectors, so T is cut to the
+    // first m = j+1 rows and columns
+    int m = niter;
+    double scale = 0.0;
+    for (int j = 0; j + 1 < niter; ++j) {
+      scale = std::max(scale, std::fabs(al[j]));
+      if (j > 0) scale = std::max(scale, b[j - 1]);
+      if (!(b[j] > 64.0 * 2.220446049250313e-16 * scale)) {
+        m = j + 1;
+        break;
+      }
+    }
+    // Ritz values of every T_k (T_m beyond a breakdown), lowest eigenpair of T_m
     if (ritz)
-      for (int k = 1; k < niter; ++k) ritz[k - 1] = tri_min_eigenvalue(k, al.data(), b.data());
+      for (int k = 1; k < niter; ++k) ritz[k - 1] = tri_min_eigenvalue(std::min(k, m), al.data(), b.data());
     double lam = 0.0;
     std::vector<double> y;
-    tri_lowest(niter, al.data(), b.data(), lam, y);
+    tri_lowest(m, al.data(), b.data(), lam, y);
     if (ritz) ritz[niter - 1] = lam;
     if (e0) *e0 = lam;
     if (gs) {
       double* dy = upload_vec(y, s);
-      vec_combine(w, V, N, dy, niter, N, s);
+      vec_combine(w, V, N, dy, m, N, s);
       QSX_HIP(hipMemcpyAsync(gs, w, sizeof(double) * N, hipMemcpyDeviceToHost, s));
       QSX_HIP(hipStreamSynchronize(s));
       dev_free(dy);
@@ -1139,8 +1018,6 @@ int qsx_hpsi_set_policy(qsx_hpsi_plan* plan, const int32_t* policy) {
     H.pol1 = policy_from_ints(policy);
     H.pol2 = policy_from_ints(policy + QSX_POLICY_INTS);
     hpsi_build_work(H);
-    H.fused = H.pol1.fused && H.mode == QSX_MODE_BATCHED;
-    if (H.fused) hpsi_build_fused(H);
     return 0;
   } catch (const Error& e) {
     fprintf(stderr, "qsx_hpsi_set_policy: %s\n", e.msg.c_str());
@@ -1148,15 +1025,6 @@ int qsx_hpsi_set_policy(qsx_hpsi_plan* plan, const int32_t* policy) {
   }
 }
 
-int qsx_hpsi_status(const qsx_hpsi_plan* plan) {
-  if (!plan) return QSX_ERR_ARG;
-  const HpsiPlan& H = *plan->h;
-  if (!H.derr) return 0;
-  int32_t e = 0;
-  if (hipDeviceSynchronize() != hipSuccess) return QSX_ERR_DEVICE;
-  if (hipMemcpy(&e, H.derr, sizeof(e), hipMemcpyDeviceToHost) != hipSuccess) return QSX_ERR_DEVICE;
-  return e ? QSX_ERR_DEVICE : 0;
-}
 
 int qsx_hpsi_weights(const qsx_hpsi_plan* plan, double* w) {
   if (!plan || !w) return QSX_ERR_ARG;

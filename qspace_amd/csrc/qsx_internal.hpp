@@ -128,10 +128,8 @@ struct WorkItem {
                      // -2 (wave engine): no work (padding of a workgroup)
   int32_t group;     // wave engine: 1 if the workgroup's 4 waves share this
                      // tile, each over its own k-range, summed through LDS
-  int32_t pad;       // wave engine flags: kCounts (single-launch H*psi),
-                     // kShortItem (at most two k-chunks)
+  int32_t pad;       // wave engine flags: kShortItem (at most two k-chunks)
 };
-constexpr int32_t kCounts = 1;
 constexpr int32_t kShortItem = 2;
 // k-steps of 4 per chunk of the wave kernel's ring for a tile of fm x fn
 // fragments (the kernel's Pipe<fm, fn>::KS)
@@ -160,7 +158,6 @@ struct WorkPolicy {
   int group_mfma = 64;           // wave tiles: MFMAs per tile above which a workgroup's waves share it
   int xcd_bins = 8;              // wave tiles: XCD-aware workgroup order over this many L2s (<= 1: off)
   int group_waves = 4;           // wave tiles of <= 2x2 fragments: waves per workgroup (4, 8 or 16)
-  int fused = 0;                 // H*psi only (stage-1 policy): both stages in one launch
   int split_steps = 16;          // k-steps (of 16) per chunk before a tile is split
   double tile_overhead = 1024;   // per-tile cost (in MACs per k) when choosing shapes
   int shape_mask = (1 << SH_NUM) - 1;
@@ -178,8 +175,7 @@ inline void policy_to_ints(const WorkPolicy& p, int32_t* v) {
   v[6] = p.split_mfma;
   v[7] = p.xcd_bins;
   v[8] = p.group_waves;
-  v[9] = p.fused;
-  v[10] = v[11] = 0;
+  v[9] = v[10] = v[11] = 0;
 }
 inline WorkPolicy policy_from_ints(const int32_t* v) {
   WorkPolicy p;
@@ -192,7 +188,6 @@ inline WorkPolicy policy_from_ints(const int32_t* v) {
   p.split_mfma = v[6] > 0 ? v[6] : 1;
   p.xcd_bins = v[7] >= 0 && v[7] <= 64 ? v[7] : 8;
   p.group_waves = v[8] == 8 || v[8] == 16 ? v[8] : 4;
-  p.fused = v[9] ? 1 : 0;
   return p;
 }
 
@@ -217,12 +212,6 @@ WorkList build_work(const std::vector<WorkTarget>& targets, const std::vector<Pa
 // persistent grouped GEMM over the queues, then (if any tile is split) the reduction
 void launch_work(const WorkQueue* dqueues, int nqueues, const WorkItem* ditems, const ReduceItem* dreds, int nreds,
                  const PairDesc* dpairs, const double* A, const double* B, double* C, double* scratch, hipStream_t s);
-// both H*psi stages in one launch (wave.hip): items = stage-1 workgroups then
-// stage-2 workgroups, each interleaved over nparts XCD partitions
-void launch_hpsi_fused(const WorkItem* ditems, int nitems, int n1wg, const PairDesc* dpairs1,
-                       const PairDesc* dpairs2, const double* HL, const double* psi, double* T, const double* HR,
-                       double* hpsi, unsigned long long* counters, const int32_t* dn12, int32_t* derr, int nparts,
-                       hipStream_t s);
 // one wave per item, then (if any tile is split) the reduction
 void launch_wave(const WorkItem* ditems, int nitems, int maxf, int wg_waves, const ReduceItem* dreds, int nreds,
                  const PairDesc* dpairs, const double* A, const double* B, double* C, double* scratch,

@@ -370,119 +370,7 @@ __global__ __launch_bounds__(NT) void k_wave_reduce(const ReduceItem* __restrict
   }
 }
 
-// ---------------------------------------------------------------- fused H*psi
-// Both H*psi stages in ONE launch.  Workgroup i runs on XCD i % 8 (the
-// dispatcher deals workgroups round-robin over the XCDs, each with its own
-// L2).  The host partitions the work by the row label of psi's first leg:
-// every stage-2 output block (a, b) reads only intermediates T(a, w, b') of
-// the same a, and those are produced by stage-1 blocks of the same a, so a
-// partition's whole dependency chain stays on one XCD.  Workgroups
-// [0, n1wg) run stage 1, [n1wg, grid) stage 2, both interleaved by
-// partition (i % 8).  A stage-1 workgroup, once all its stores have
-// completed, adds 1 to its partition's counter; a stage-2 workgroup waits
-// until the counter shows all stage-1 workgroups of its partition for this
-// call.  The counters live in the XCD's L2 (atomics execute there, and only
-// that XCD's workgroups touch them); stores reach the L2 before the counter
-// moves (s_waitcnt vmcnt(0)), and no CU holds an L1 line of T from before the
-// wait (stage 1 never reads T, stage 2 reads it only after the wait), so
-// plain loads see the intermediates.  Counters only grow: a stage-2
-// workgroup reads the call's generation from a second counter it bumps
-// itself, so no reset is needed between calls or graph replays.  Deadlock is
-// impossible: workgroups of one XCD are dispatched in index order, so every
-// stage-1 workgroup of a partition is resident or done before any stage-2
-// workgroup of it waits; the wait is still bounded (spin limit -> error flag).
-struct FusedSync {
-  unsigned long long* cnt1;   // [nparts] stage-1 workgroups done (all calls)
-  unsigned long long* cnt2;   // [nparts] stage-2 workgroups started (all calls)
-  const int32_t* n1;          // [nparts] counting stage-1 workgroups per call
-  const int32_t* n2;          // [nparts] counting stage-2 workgroups per call
-  int32_t* err;               // set if a wait hit the spin limit
-  int nparts;
-  int spin_limit;
-};
-
-__device__ __forceinline__ unsigned long long l2_add(unsigned long long* p, unsigned long long v) {
-  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-// read a counter with an atomic that executes in the L2 (a compare-exchange
-// whose compare never matches: it writes nothing and returns the value; an
-// idempotent add would be turned into a plain, L1-cacheable load)
-__device__ __forceinline__ unsigned long long l2_read(unsigned long long* p) {
-  unsigned long long expected = ~0ull;
-  __hip_atomic_compare_exchange_strong(p, &expected, ~0ull, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_WORKGROUP);
-  return expected;
-}
-
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4)))
-void k_hpsi_fused(const WorkItem* __restrict__ items, int nitems, int n1wg, const PairDesc* __restrict__ pairs1,
-                  const PairDesc* __restrict__ pairs2, const double* __restrict__ HL, const double* __restrict__ psi,
-                  double* T, const double* __restrict__ HR, double* __restrict__ hpsi, FusedSync sync) {
-  __shared__ GroupLds lds;
-  const int stage = (int)blockIdx.x >= n1wg ? 1 : 0;
-  const int part = (int)(blockIdx.x % (unsigned)sync.nparts);
-  const int first = blockIdx.x * (NT / 64);
-  // bit 0 of pad: the workgroup counts for its partition (not padding)
-  const bool counts = first < nitems && (items[first].pad & kCounts);
-  if (stage == 1 && counts) {
-    if (threadIdx.x == 0) {
-      const unsigned long long gen = l2_add(&sync.cnt2[part], 1ull) / (unsigned long long)sync.n2[part];
-      const unsigned long long target = (gen + 1) * (unsigned long long)sync.n1[part];
-      int it = 0;
-      while (l2_read(&sync.cnt1[part]) < target) {
-        if (++it > sync.spin_limit) {
-          __hip_atomic_fetch_or(sync.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          break;
-        }
-        __builtin_amdgcn_s_sleep(16);
-      }
-    }
-    __syncthreads();
-  }
-  const int wid = uniform(first + (threadIdx.x >> 6));
-  if (wid < nitems) {
-    const WorkItem w = items[wid];
-    if (w.slot != -2) {
-      const PairDesc* pairs = stage ? pairs2 : pairs1;
-      const double* A = stage ? T : HL;
-      const double* B = stage ? HR : psi;
-      double* C = stage ? hpsi : T;
-      switch (w.shape - SH_WAVE) {
-        case 0: run_item<1, 1>(w, pairs, A, B, C, nullptr, lds); break;
-        case 1: run_item<1, 2>(w, pairs, A, B, C, nullptr, lds); break;
-        case 4: run_item<2, 1>(w, pairs, A, B, C, nullptr, lds); break;
-        case 5: run_item<2, 2>(w, pairs, A, B, C, nullptr, lds); break;
-        default: break;
-      }
-    }
-  }
-  if (stage == 0 && counts) {
-    __builtin_amdgcn_s_waitcnt(0);  // this wave's stores have completed
-    __syncthreads();
-    if (threadIdx.x == 0) l2_add(&sync.cnt1[part], 1ull);
-  }
-}
-
 }  // namespace
-
-void launch_hpsi_fused(const WorkItem* ditems, int nitems, int n1wg, const PairDesc* dpairs1,
-                       const PairDesc* dpairs2, const double* HL, const double* psi, double* T, const double* HR,
-                       double* hpsi, unsigned long long* counters, const int32_t* dn12, int32_t* derr, int nparts,
-                       hipStream_t s) {
-  if (nitems <= 0) return;
-  if (nitems % (NT / 64) || (n1wg % nparts)) fail(QSX_ERR_ARG, "fused H*psi: malformed work list");
-  FusedSync sy;
-  sy.cnt1 = counters;
-  sy.cnt2 = counters + nparts;
-  sy.n1 = dn12;
-  sy.n2 = dn12 + nparts;
-  sy.err = derr;
-  sy.nparts = nparts;
-  sy.spin_limit = 1 << 24;
-  hipLaunchKernelGGL(k_hpsi_fused, dim3(nitems / (NT / 64)), dim3(NT), 0, s, ditems, nitems, n1wg, dpairs1, dpairs2,
-                     HL, psi, T, HR, hpsi, sy);
-  QSX_HIP(hipGetLastError());
-}
 
 void launch_wave(const WorkItem* ditems, int nitems, int maxf, int wg_waves, const ReduceItem* dreds, int nreds,
                  const PairDesc* dpairs, const double* A, const double* B, double* C, double* scratch,

@@ -156,12 +156,27 @@ def hubbard_hpsi(seed=1005, Dstar=4000, mu=3.2, sigma=0.9, dmax=384, qmax=12):
     return HL, psi, HR
 
 
-def fermion_parity_signs(HL: QSpace) -> np.ndarray:
-    """Exact +-1 per HL block: parity (-1)^{Q_w} of the operator leg, the
-    Jordan-Wigner string an explicit parity operator Z contributes when a
-    fermionic operator acts through the left block."""
-    qw = HL.Q[2][:, 0]
-    return np.where(qw % 2 == 0, 1.0, -1.0)
+def fermion_parity_signs(HL: QSpace, ket_leg: int = 1, w_leg: int = 2, parity=None) -> np.ndarray:
+    """Exact +-1 per block of a left environment HL(a, a', w) that holds the
+    bare fermion operators of the left part: the Jordan-Wigner string P_L.
+
+    A fermion operator of the right part is P_L (x) c_R, so the cut's
+    hopping c+_L c_R = (c+_L P_L) (x) c_R and c+_R c_L = -(c_L P_L) (x) c+_R:
+    the left operator of every fermionic channel (odd parity of its w label)
+    is followed by P_L, i.e. multiplied by the parity (-1)^N of its ket
+    state a' (ket_leg, the leg contracted with psi).  The sign is applied
+    per block through the contraction's afac = fA[ia] * fB[ib]
+    (Source/QSpace_aux.cc:112-114), and in the Hermitian-conjugate term the
+    same block represents (c+_L P_L)^T = P_L c_L with the same factor.
+    ``parity`` maps a leg's label record to 0/1; the default reads the 2S
+    of an "A,SU2" Hubbard label (Q, 2S): a block of sites has (-1)^N =
+    (-1)^(2S), since only singly occupied sites carry spin 1/2.  Both the
+    rule and the parity map are checked against a full Fock-space
+    Jordan-Wigner Hamiltonian of 2-4 sites in the tests."""
+    if parity is None:
+        parity = lambda rec: int(rec[1]) % 2
+    z = np.array([parity(HL.Q[w_leg][i]) * parity(HL.Q[ket_leg][i]) for i in range(HL.nblk)], dtype=np.int64)
+    return np.where(z % 2 == 0, 1.0, -1.0)
 
 
 # ---------------------------------------------------------------------------

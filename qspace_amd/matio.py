@@ -17,9 +17,17 @@ Layout restated from the reference's MATLAB interface:
 Read: labels, data, itags, qtype and, per block and non-abelian symmetry,
 the scalar weight cgw (outer multiplicity 1).  The CGT itself is not read:
 this library's block CGT is the canonical CGT of the block's labels and
-directions, so a file whose CRef carries another gauge (a CGT permutation,
-``cgp``) reads with that gauge's sign lost -- no reference .mat file exists
-to check this against (parity unpinned for SU(2) files).
+directions in its leg order.  The reference writes only CRefs in that
+standard form: add2MxStruct refuses a CRef whose CGT permutation is not the
+sorting permutation of its labels (isSortedDegQ, Source/clebsch_io.cc:1748-1756,
+Source/clebsch.cc:4804-4824), and writes qset / qdir in the tensor's leg
+order (the CData's labels adapted by the permutation).  A CRef whose legs
+were permuted is brought to that form by CRef::Permute -> SortDegQ ->
+X3Map::contractDegQ, which moves the sign of the transposition into cgw
+(Source/clebsch.cc:5005-5031,4827-4868; Source/clebsch_aux.cc:4512-4600).
+The reader checks each record's qset / qdir against the block's labels and
+leg directions and raises ValueError on a mismatch (a CGT in another gauge,
+which this format cannot represent).
 Write: the same struct with cgr records holding type, qset (the block's
 labels of that symmetry in leg order), qdir, size, nnz and cgw; cid and cgt
 are left empty (the reference recomputes CGTs from its own store).
@@ -112,9 +120,11 @@ def qspace_from_mx(S) -> QSpace:
             else:
                 raise ValueError(f"info.cgr of {cg.size} records for {n} blocks x {len(syms)} symmetries")
             cgw = np.ones((n, len(syms)))
+            dirs = [(-1 if t.endswith("*") else 1) for t in itags] if itags else None
             for i in range(n):
                 for j, sy in enumerate(syms):
                     if sy == "SU2" and j < recs.shape[1]:
+                        _check_cgr_record(recs[i, j], [int(Qi[k][i, j]) for k in range(r)], dirs, i, j)
                         w = np.asarray(recs[i, j]["cgw"], dtype=np.float64).ravel()
                         if w.size > 1:
                             raise ValueError("outer multiplicity > 1 (cgw with several entries) not supported")
@@ -123,6 +133,35 @@ def qspace_from_mx(S) -> QSpace:
             if "SU2" not in syms:
                 cgw = None
     return QSpace(Qi, data, qtype, itags, cgw)
+
+
+def _record_field(rec, name):
+    names = getattr(getattr(rec, "dtype", None), "names", None)
+    if names is not None:
+        return rec[name] if name in names else None
+    if isinstance(rec, dict):
+        return rec.get(name)
+    return None
+
+
+def _check_cgr_record(rec, labels, dirs, i, j):
+    """a cgr record's qset / qdir must be the block's labels of that symmetry
+    and the legs' directions, in leg order (the standard CRef form the
+    reference writes, Source/clebsch_io.cc:1748-1768); an empty field is not
+    checked"""
+    qs = _record_field(rec, "qset")
+    if qs is not None and np.asarray(qs).size:
+        q = np.asarray(qs, dtype=np.float64).ravel()
+        if q.size != len(labels) or not np.array_equal(q, np.asarray(labels, dtype=np.float64)):
+            raise ValueError(f"info.cgr({i + 1},{j + 1}).qset {q.tolist()} differs from the block's labels "
+                             f"{labels} (a CGT in a non-standard gauge)")
+    qd = _record_field(rec, "qdir")
+    if qd is not None and dirs is not None:
+        d = _str(qd)
+        if d:
+            want = "".join("+" if x > 0 else "-" for x in dirs)
+            if d != want:
+                raise ValueError(f"info.cgr({i + 1},{j + 1}).qdir '{d}' differs from the leg directions '{want}'")
 
 
 def _norm_ext(tj, dirs) -> float:

@@ -19,3 +19,14 @@ def assert_same_structure(C, R):
     assert C.itags == R.itags, (C.itags, R.itags)
     if R.info["cgw"] is not None or C.info["cgw"] is not None:
         assert np.array_equal(np.asarray(C.info["cgw"]), np.asarray(R.info["cgw"]))
+
+
+def oracle_ortho(P, K, **kw):
+    """the oracle's orthoQS core called with the Nkeep / stol the front end
+    resolves (qspace_amd.ortho_args, orthoQS.cc:137-170), so that it sees the
+    same arguments as the device call with the same keywords"""
+    import oracle as O
+    import qspace_amd as Q
+    kw = dict(kw)
+    kw["Nkeep"], kw["stol"] = Q.ortho_args(P, K, kw.get("Nkeep"), kw.get("stol"))
+    return O.ortho(P, K, **kw)

@@ -13,7 +13,7 @@ import pytest
 import oracle as O
 import qspace_amd as Q
 from qspace_amd import gen
-from helpers import assert_same_structure
+from helpers import assert_same_structure, oracle_ortho
 from test_oracle_blockops import u1_rank4
 
 pytestmark = pytest.mark.gpu
@@ -46,7 +46,7 @@ def _group_mats(T):
 
 def _check(P, K, **kw):
     A1, A2, info = Q.orthoQS(P, K, **kw)
-    R1, R2, rinfo = O.ortho(P, K, **kw)
+    R1, R2, rinfo = oracle_ortho(P, K, **kw)
     print(f"orthoQS K={K} {kw} blocks={P.nblk} groups={info['ngroups']} sweeps={info['sweeps']} "
           f"keep={info['Nkeep']}/{info['Ntot']}", flush=True)
     assert_same_structure(A1, R1)

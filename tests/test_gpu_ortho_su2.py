@@ -9,6 +9,7 @@ import pytest
 import oracle as O
 import qspace_amd as Q
 from qspace_amd import gen
+from helpers import oracle_ortho
 from test_gpu_ortho import _check
 from test_oracle_ortho_su2 import _mixed_mps, _su2_matrix, _tensor, _weighted
 
@@ -34,7 +35,7 @@ def test_su2_ortho_matches_oracle(gpu, kind, K, kw):
     P = _tensor(kind)
     _check(P, K, **kw)
     _, _, info = Q.orthoQS(P, K, **kw)
-    _, _, rinfo = O.ortho(P, K, **kw)
+    _, _, rinfo = oracle_ortho(P, K, **kw)
     assert np.array_equal(info["svd_qdim"], rinfo["svd_qdim"])
 
 

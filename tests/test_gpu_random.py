@@ -15,7 +15,7 @@ import pytest
 import oracle as O
 import qspace_amd as Q
 from qspace_amd import QSpace
-from helpers import assert_same_structure, rel_err
+from helpers import assert_same_structure, oracle_ortho, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -129,7 +129,7 @@ def test_random_truncations(gpu, seed):
     stol = float(rng.choice([0.0, 0.5]))
     print(f"seed {seed}: rank {r} K {K} blocks {T.nblk} Nkeep {nkeep} stol {stol}", flush=True)
     A1, A2, info = Q.orthoQS(T, K, Nkeep=nkeep, stol=stol)
-    R1, R2, rinfo = O.ortho(T, K, Nkeep=nkeep, stol=stol)
+    R1, R2, rinfo = oracle_ortho(T, K, Nkeep=nkeep, stol=stol)
     assert_same_structure(A1, R1)
     assert_same_structure(A2, R2)
     assert (info["Nkeep"], info["Ntot"], info["flag"]) == (rinfo["Nkeep"], rinfo["Ntot"], rinfo["flag"])
@@ -225,7 +225,7 @@ def test_random_su2_truncations(gpu, seed):
     stol = float(rng.choice([0.0, 0.5]))
     print(f"seed {seed}: {T.itags} blocks {T.nblk} K {K} Nkeep {nkeep} stol {stol}", flush=True)
     A1, A2, info = Q.orthoQS(T, K, Nkeep=nkeep, stol=stol)
-    R1, R2, rinfo = O.ortho(T, K, Nkeep=nkeep, stol=stol)
+    R1, R2, rinfo = oracle_ortho(T, K, Nkeep=nkeep, stol=stol)
     assert_same_structure(A1, R1)
     assert_same_structure(A2, R2)
     assert (info["Nkeep"], info["Ntot"], info["flag"]) == (rinfo["Nkeep"], rinfo["Ntot"], rinfo["flag"])

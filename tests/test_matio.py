@@ -54,6 +54,9 @@ def test_reference_layout_struct():
     cgr = np.zeros((2, 1), dtype=[("type", object), ("qset", object), ("qdir", object), ("cid", object),
                                   ("size", object), ("nnz", object), ("cgw", object), ("cgt", object)])
     cgr[0, 0]["cgw"] = np.array([[1.5]]); cgr[1, 0]["cgw"] = np.array([[2.0]])
+    for i, qs in enumerate(([0.0, 1.0, 1.0], [2.0, 1.0, 1.0])):
+        cgr[i, 0]["qset"] = np.array([qs]); cgr[i, 0]["qdir"] = np.array(["++-"])
+        cgr[i, 0]["type"] = np.array(["SU2"])
     S = {"Q": Q, "data": D, "info": {"qtype": "SU2", "itags": it, "cgr": cgr}}
     T = qspace_from_mx(S)
     assert T.rank == 3 and T.nblk == 2 and T.qtype == "SU2" and T.itags == ["L", "s", "R*"]
@@ -61,6 +64,26 @@ def test_reference_layout_struct():
     assert np.array_equal(T.info["cgw"], [[1.5], [2.0]])
     M = qspace_to_mx(T)
     _same(T, qspace_from_mx(M))
+
+
+def test_cgr_record_in_another_gauge_is_refused():
+    """a cgr record whose qset is not the block's labels in leg order (a CGT
+    permutation the reference would have brought to the standard form before
+    writing, Source/clebsch_io.cc:1748-1756), or whose qdir is not the legs'
+    directions, is refused instead of losing the gauge's sign"""
+    A, _ = gen.su2_mps(seed=3, tS=(0, 1, 2, 3, 4), dmin=2, dmax=3)
+    M = qspace_to_mx(A)
+    qspace_from_mx(M)  # the writer's own records pass the check
+    i = next(i for i in range(A.nblk) if len({int(A.Q[k][i, 0]) for k in range(3)}) == 3)
+    bad = M["info"]["cgr"].copy()
+    bad[i, 0]["qset"] = bad[i, 0]["qset"][:, [1, 0, 2]]
+    with pytest.raises(ValueError, match="non-standard gauge"):
+        qspace_from_mx({"Q": M["Q"], "data": M["data"], "info": dict(M["info"], cgr=bad)})
+    bad = M["info"]["cgr"].copy()
+    d = bad[i, 0]["qdir"]
+    bad[i, 0]["qdir"] = d[::-1] if d[::-1] != d else d.replace("+", "x")
+    with pytest.raises(ValueError, match="qdir"):
+        qspace_from_mx({"Q": M["Q"], "data": M["data"], "info": dict(M["info"], cgr=bad)})
 
 
 def test_invalid_structs():

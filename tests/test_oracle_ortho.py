@@ -212,3 +212,23 @@ def test_rank2_block_diagonal():
     U, s, Vt = np.linalg.svd(F)
     best = (U[:, :5] * s[:5]) @ Vt[:5] * (np.sqrt(np.sum(s * s)) / np.sqrt(np.sum(s[:5] ** 2)))
     assert np.abs(expand(X, lay) - best).max() < 1e-12 * np.abs(best).max()
+
+
+def test_front_end_defaults():
+    """orthoQS front-end defaults (MPS_ORTHO_1, Source/orthoQS.cc:137-170,
+    get_Nkeep_estimate :39-56) on a hand-built rank-3 tensor whose leg dims
+    (sums of block dims over distinct labels) are 5, 2 and 9"""
+    import qspace_amd as Q
+    Qs = [np.array([[0], [2], [0]]), np.array([[1], [1], [-1]]), np.array([[1], [3], [-1]])]
+    data = [np.ones((2, 1, 4)), np.ones((3, 1, 2)), np.ones((2, 1, 3))]
+    P = QSpace(Qs, data, "A", ["a", "b", "c*"])
+    # leg dims: leg 0 {0: 2, 2: 3} = 5, leg 1 {1: 1, -1: 1} = 2, leg 2 {1: 4, 3: 2, -1: 3} = 9
+    assert Q.ortho_args(P, 2) == (min(10, 9), 1e-8)
+    assert Q.ortho_args(P, 1) == (min(5, 18), 1e-8)
+    assert Q.ortho_args(P, 2, Nkeep=-3) == (9, 1e-8)
+    assert Q.ortho_args(P, 2, Nkeep=3) == (3, 0.0)          # Nkeep > 0 without stol: stol = 0
+    assert Q.ortho_args(P, 2, Nkeep=3, stol=0.25) == (3, 0.25)
+    assert Q.ortho_args(P, 2, Nkeep=0) == (0, 1e-8)         # Nkeep = 0 is not > 0
+    S = QSpace(Qs, data, "SU2", ["a", "b", "c*"])
+    assert Q.ortho_args(S, 2) == (max(10, 9), 1e-8)          # non-abelian: the larger side
+    assert Q.ortho_args(S, 1) == (max(5, 18), 1e-8)
