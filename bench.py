@@ -115,7 +115,42 @@ def stage_timing(H, dpsi, dout, reps):
     return [float(x) for x in np.mean(np.array(t), axis=0)]
 
 
-def cpu_baseline(w, sample_smax=None, target_s=15.0):
+def oracle_hpsi(HL, psi, HR, ferm, x):
+    """the oracle's H*psi of the packed vector x (both terms, parity signs on
+    HL): the explicit contraction chain of get_HPsi.m:54-57"""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    xs = psi.with_packed(x)
+    T1 = O.contract(HL, [1], xs, [0], fermA=ferm, skip_zero=0)
+    T2 = O.contract(HL, [0], xs, [0], conjA=1, fermA=ferm, skip_zero=0)
+    key = {tuple(r): i for i, r in enumerate(psi.QIDX)}
+    acc = [np.zeros_like(d) for d in psi.data]
+    for Ht in (O.contract(T1, [1, 2], HR, [2, 1], skip_zero=0),
+               O.contract(T2, [1, 2], HR, [2, 0], conjB=1, skip_zero=0)):
+        for r, d in zip(Ht.QIDX, Ht.data):
+            acc[key[tuple(r)]] += d
+    return np.concatenate([a.ravel(order="F") for a in acc])
+
+
+def host_cpus():
+    """CPUs this process may use (its affinity mask, capped by
+    OMP_NUM_THREADS when the host sets it), with nproc and the CPU model"""
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    cores = min(aff, int(omp)) if omp.isdigit() and int(omp) > 0 else aff
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return cores, {"nproc": os.cpu_count(), "affinity": aff, "omp_num_threads": omp or None, "cpu_model": model}
+
+
+def cpu_baseline(w, sample_smax=None, target_s=10.0):
     """Reference-structure CPU path (oracle/, OpenMP over output blocks, per
     pair OpenBLAS dgemm + axpy): the full workload (or, with sample_smax, the
     same generated operands restricted to sectors with 2S <= sample_smax),
@@ -129,44 +164,49 @@ def cpu_baseline(w, sample_smax=None, target_s=15.0):
     def keep(r):
         return sample_smax is None or all(r[k * qd + spin_col] <= sample_smax for k in range(len(r) // qd))
     HLs, psis, HRs = restrict(HL, keep), restrict(psi, keep), restrict(HR, keep)
-    cores = min(16, os.cpu_count() or 1)
+    if ferm is not None:  # the signs of the sample's own HL blocks (signs do not change the cost)
+        from qspace_amd import gen
+        ferm = gen.fermion_parity_signs(HLs)
+    cores, host = host_cpus()
     # the four contractions of H*psi (two stages x hconj term)
-    T1 = O.contract(HLs, [1], psis, [0], skip_zero=0)
-    T2 = O.contract(HLs, [0], psis, [0], conjA=1, skip_zero=0)
+    T1 = O.contract(HLs, [1], psis, [0], fermA=ferm, skip_zero=0)
+    T2 = O.contract(HLs, [0], psis, [0], conjA=1, fermA=ferm, skip_zero=0)
     jobs = [O.CpuBaseline(HLs, [1], psis, [0]), O.CpuBaseline(HLs, [0], psis, [0], conjA=1),
             O.CpuBaseline(T1, [1, 2], HRs, [2, 1]), O.CpuBaseline(T2, [1, 2], HRs, [2, 0], conjB=1)]
     flops = sum(j.flops for j in jobs)
-    for j in jobs:
-        j.run(cores)  # warm
-    reps = 0
-    t0 = time.perf_counter()
-    while True:
+
+    def timed(nthreads, budget):
         for j in jobs:
-            j.run(cores)
-        reps += 1
-        el = time.perf_counter() - t0
-        if el > target_s:
-            break
-    sec = el / reps
-    # GPU vs CPU on the same sample: rel err of H*psi vs the FP64 CPU result
+            j.run(nthreads)  # warm
+        reps = 0
+        t0 = time.perf_counter()
+        while True:
+            for j in jobs:
+                j.run(nthreads)
+            reps += 1
+            el = time.perf_counter() - t0
+            if el > budget:
+                return el / reps, reps, el
+
+    sec, reps, el = timed(cores, target_s)
+    sec8, reps8, _ = timed(8, 0.5 * target_s)
+    # GPU vs CPU on the same sample: rel err of H*psi vs the FP64 oracle
     import qspace_amd as Q
-    Hg = Q.HPsi(HLs, psis, HRs, hconj=1)
+    Hg = Q.HPsi(HLs, psis, HRs, hconj=1, fermA=ferm)
     got = Hg.apply(psis.packed())
-    H1 = O.contract(T1, [1, 2], HRs, [2, 1], skip_zero=0)
-    H2 = O.contract(T2, [1, 2], HRs, [2, 0], conjB=1, skip_zero=0)
-    key = {tuple(r): i for i, r in enumerate(psis.QIDX)}
-    out = [np.zeros_like(d) for d in psis.data]
-    for Ht in (H1, H2):
-        for r, d in zip(Ht.QIDX, Ht.data):
-            out[key[tuple(r)]] += d
-    ref = np.concatenate([o.ravel(order="F") for o in out])
+    ref = oracle_hpsi(HLs, psis, HRs, ferm, psis.packed())
     err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
     return {
         "value": flops / sec / 1e9, "unit": "GFLOP/s", "cores": cores, "kind": "port",
         "sample": ((f"the full workload ({psi.nblk} psi blocks" if sample_smax is None else
                     f"same seeded operands restricted to sectors with 2S<={sample_smax} ({psis.nblk} of {psi.nblk} "
-                    "psi blocks") + f", {flops / 1e9:.3f} GFLOP per H*psi), {reps} H*psi in {el:.1f} s; "
-                   f"OpenMP over output blocks, per-pair dgemm + axpy ({os.path.basename(jobs[0].blas)}, 1 thread)"),
+                    "psi blocks") + f", {flops / 1e9:.3f} GFLOP per H*psi), {reps} H*psi in {el:.1f} s on all "
+                   f"{cores} CPUs this process may use; OpenMP over output blocks, per-pair dgemm + axpy "
+                   f"({os.path.basename(jobs[0].blas)}, 1 BLAS thread)"),
+        "host": host,
+        "value_8_threads": flops / sec8 / 1e9,
+        "ms_per_hpsi_8_threads": sec8 * 1e3,
+        "reps_8_threads": reps8,
         "ms_per_hpsi_sample": sec * 1e3,
         "rel_err_gpu_vs_fp64_oracle_sample": err,
     }
@@ -254,16 +294,19 @@ def extra_configs():
     del H
     # 4: split precision (BF16 slices, exact FP32 accumulation, FP64
     # recombination) on the headline operator: error table per slice count
+    # against the FP64 oracle (and the GPU's own FP64 result)
     w = WORKLOADS["heis4000"]
     HL, psi, HR, ferm = make_workload(w)
     Hd = Q.HPsi(HL, psi, HR, hconj=1)
     Hd.autotune(5)
     xv = psi.packed()
     ref = Hd.apply(xv)
+    oref = oracle_hpsi(HL, psi, HR, None, xv)
     e_ref, _, _ = Hd.lanczos(xv, 20)
     x = Q.DeviceBuffer.from_array(xv)
     y = Q.DeviceBuffer(Hd.N)
     ms_fp64 = time_hpsi(Hd, x, y, 50, 5)
+    fp64_err = float(np.linalg.norm(ref - oref) / np.linalg.norm(oref))
     del Hd
     table = {}
     for S in (5, 6, 7, 8):
@@ -272,11 +315,17 @@ def extra_configs():
         ms = time_hpsi(Hs, x, y, 20, 3)
         es, _, _ = Hs.lanczos(xv, 20)
         inf = Hs.stage_info()
-        table[str(S)] = {"rel_err_hpsi_vs_fp64": float(np.linalg.norm(got - ref) / np.linalg.norm(ref)),
+        table[str(S)] = {"rel_err_hpsi_vs_oracle": float(np.linalg.norm(got - oref) / np.linalg.norm(oref)),
+                         "rel_err_hpsi_vs_gpu_fp64": float(np.linalg.norm(got - ref) / np.linalg.norm(ref)),
                          "E0_rel_err_lanczos20": float(abs(es - e_ref) / abs(e_ref)), "hpsi_ms": ms,
                          "executed_slice_gflop": float(sum(inf["split_exec_flops"]) / 1e9)}
         del Hs
-    out["4_split_bf16_heis4000"] = {"fp64_hpsi_ms": ms_fp64, "E0_fp64_lanczos20": e_ref, "slices": table}
+    Hs = Q.HPsi(HL, psi, HR, hconj=1, mode="split_bf16")
+    default_err = float(np.linalg.norm(Hs.apply(xv) - oref) / np.linalg.norm(oref))
+    del Hs
+    out["4_split_bf16_heis4000"] = {"fp64_hpsi_ms": ms_fp64, "fp64_rel_err_vs_oracle": fp64_err,
+                                    "E0_fp64_lanczos20": e_ref, "default_slices_rel_err_vs_oracle": default_err,
+                                    "slices": table}
     log(f"[bench] config 4: {out['4_split_bf16_heis4000']}")
     # 5: SU(2)xU(1) Hubbard H*psi at D*=4000 with parity signs, 30 Davidson iterations
     w = WORKLOADS["hubbard4000"]
@@ -287,15 +336,7 @@ def extra_configs():
     y = Q.DeviceBuffer(H.N)
     ms = time_hpsi(H, x, y, 50, 5)
     got = y.download()
-    T1 = O.contract(HL, [1], psi, [0], fermA=ferm, skip_zero=0)
-    T2 = O.contract(HL, [0], psi, [0], conjA=1, fermA=ferm, skip_zero=0)
-    key = {tuple(r): i for i, r in enumerate(psi.QIDX)}
-    acc = [np.zeros_like(d) for d in psi.data]
-    for Ht in (O.contract(T1, [1, 2], HR, [2, 1], skip_zero=0), O.contract(T2, [1, 2], HR, [2, 0], conjB=1,
-                                                                                  skip_zero=0)):
-        for r, d in zip(Ht.QIDX, Ht.data):
-            acc[key[tuple(r)]] += d
-    ref = np.concatenate([a.ravel(order="F") for a in acc])
+    ref = oracle_hpsi(HL, psi, HR, ferm, psi.packed())
     dav_first, dav, e0 = _time_eig(H.davidson, psi.packed(), 30)
     out["5_hubbard4000_davidson30"] = {"psi_elements": H.N, "useful_gflop_per_hpsi": H.flops / 1e9, "hpsi_ms": ms,
                                        "gflops": H.flops / ms / 1e6, "davidson30_ms": dav,
@@ -472,9 +513,13 @@ def main():
     ms = time_hpsi(H, dpsi, dout, args.steps, args.warmup, graph=True)
     log(f"[bench] batched+graph H*psi {ms:.4f} ms -> {H.flops / ms / 1e6:.1f} GFLOP/s")
     ms_nograph = time_hpsi(H, dpsi, dout, max(5, args.steps // 5), 2, graph=False)
-    kt = stage_timing(H, dpsi, dout, 20)
+    kt_ev = stage_timing(H, dpsi, dout, 20)
+    # kernel durations: 200 back-to-back launches of one stage between two
+    # events on its stream (dispatch overlapped with the previous launch)
+    kt = [H.stage_ms(1, 200, dpsi, dout), H.stage_ms(2, 200, dpsi, dout)]
     t1, t2 = kt[0], kt[1]
-    log(f"[bench] stage kernels {info['kernel'][0]} {kt[0]:.4f} ms, {info['kernel'][1]} {kt[1]:.4f} ms")
+    log(f"[bench] stage kernels {info['kernel'][0]} {kt[0]:.4f} ms, {info['kernel'][1]} {kt[1]:.4f} ms "
+        f"(per-launch events {kt_ev[0]:.4f}, {kt_ev[1]:.4f} ms)")
     ms_plain = err_plain = herm = lan_ms = lan_first = e0 = None
     if not args.lite:
         # plain FP64 per-pair GPU path (one launch per pair, no graph)
@@ -518,6 +563,8 @@ def main():
     roof["kernel_grid"] = info["grid"][dom]
     roof["reduce_grid"] = info["reduce_grid"][dom]
     roof["kernel_ms"] = {"stage1": kt[0], "stage2": kt[1]}
+    roof["kernel_ms_timing"] = "mean of 200 back-to-back launches of the stage between two HIP events"
+    roof["kernel_ms_per_launch_events"] = {"stage1": kt_ev[0], "stage2": kt_ev[1]}
     roof["algorithmic_flops"] = fl
     roof["algorithmic_bytes"] = by
     roof["avg_launch_ms"] = tk

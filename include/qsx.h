@@ -231,6 +231,15 @@ int qsx_ctr_plan_output(const qsx_ctr_plan* plan, qsx_tensor_out** C);
    seg has nout+1 entries (may pass NULL to query the sizes) */
 int qsx_ctr_plan_pairs(const qsx_ctr_plan* plan, int64_t* npairs, int64_t* nout,
                        int32_t* ia, int32_t* ib, double* afac, int64_t* seg);
+/* the index set of contract_getIdxSet (Source/QSpace.cc:4131-4294) as the
+   plan holds it: ALL label-matched pairs (ia, ib) grouped by output block in
+   the reference order (matchIndex / groupRecs), seg (nout+1 offsets), the
+   output label records qc (nout x (rank_A + rank_B - 2k) * qdim, before the
+   output permutation), and per pair afac = signs x X-symbols x cgw of the
+   operands (before NormCGW; 0 for a pair below CG_SKIP_EPS2).  Any pointer
+   may be NULL to query the sizes. */
+int qsx_ctr_plan_idxset(const qsx_ctr_plan* plan, int64_t* npairs, int64_t* nout, int32_t* ia, int32_t* ib,
+                        double* afac, int64_t* seg, int32_t* qc);
 int qsx_ctr_execute(qsx_ctr_plan* plan, const double* dA, const double* dB, double* dC,
                     void* stream);
 double qsx_ctr_plan_flops(const qsx_ctr_plan* plan);
@@ -294,6 +303,14 @@ int qsx_hpsi_use_graph(qsx_hpsi_plan* plan, int32_t on);
 /* per-kernel device timing of the last direct (non-graph) apply, HIP events
    on the launch stream: [grouped GEMM stage 1, grouped GEMM stage 2] in ms */
 int qsx_hpsi_last_timing(const qsx_hpsi_plan* plan, double* ms2);
+/* mean device time per launch of one stage's GEMM (stage 1 or 2; with its
+   split-tile reduction, if any) over reps back-to-back launches between two
+   HIP events on the launch stream, after one H*psi of dpsi; batched FP64
+   mode.  Back-to-back launches overlap each launch's dispatch with the
+   previous kernel, so this is the kernel's duration plus the gap between
+   kernels. */
+int qsx_hpsi_stage_ms(qsx_hpsi_plan* plan, int32_t stage, int32_t reps, const double* dpsi, double* dhpsi,
+                      double* ms);
 void qsx_hpsi_plan_destroy(qsx_hpsi_plan* plan);
 
 /* Lanczos with full FP64 re-orthogonalisation; psi0 is a host vector of

@@ -561,6 +561,8 @@ class CtrPlan:
         L.qsx_tensor_free(out)
         self.flops = L.qsx_ctr_plan_flops(self.h)
         self.nout = self.out.numel()
+        self._rank_out = A.rank + B.rank - 2 * len(ia)
+        self._qdim = A.qdim
 
     def pairs(self):
         L = _capi.lib()
@@ -576,6 +578,26 @@ class CtrPlan:
                              ib.ctypes.data_as(P(C.c_int32)), af.ctypes.data_as(P(C.c_double)),
                              seg.ctypes.data_as(P(C.c_int64)))
         return ia[: n.value], ib[: n.value], af[: n.value], seg
+
+    def idxset(self):
+        """contract_getIdxSet's result as the plan holds it: (Ia, Ib, afac,
+        seg, QC) over all label-matched pairs, afac before NormCGW"""
+        L = _capi.lib()
+        n = C.c_int64()
+        m = C.c_int64()
+        _capi.check(L.qsx_ctr_plan_idxset(self.h, C.byref(n), C.byref(m), None, None, None, None, None), None)
+        rc = self._rank_out
+        q = self._qdim
+        ia = np.zeros(max(1, n.value), np.int32)
+        ib = np.zeros(max(1, n.value), np.int32)
+        af = np.zeros(max(1, n.value))
+        seg = np.zeros(m.value + 1, np.int64)
+        qc = np.zeros(max(1, m.value * rc * q), np.int32)
+        P = C.POINTER
+        _capi.check(L.qsx_ctr_plan_idxset(self.h, C.byref(n), C.byref(m), ia.ctypes.data_as(P(C.c_int32)),
+                                          ib.ctypes.data_as(P(C.c_int32)), af.ctypes.data_as(P(C.c_double)),
+                                          seg.ctypes.data_as(P(C.c_int64)), qc.ctypes.data_as(P(C.c_int32))), None)
+        return ia[: n.value], ib[: n.value], af[: n.value], seg, qc[: m.value * rc * q].reshape(m.value, rc * q)
 
     def execute(self, dA: DeviceBuffer, dB: DeviceBuffer, dC: DeviceBuffer):
         _capi.check(_capi.lib().qsx_ctr_execute(self.h, dA.ptr, dB.ptr, dC.ptr, None), None)
@@ -665,6 +687,14 @@ class HPsi:
         ms = (C.c_double * 2)()
         _capi.check(_capi.lib().qsx_hpsi_last_timing(self.h, ms), None)
         return tuple(ms)
+
+    def stage_ms(self, stage: int, reps: int, dpsi: DeviceBuffer, dhpsi: DeviceBuffer) -> float:
+        """mean device ms per launch of one stage over reps back-to-back
+        launches (two HIP events on the launch stream)"""
+        ms = C.c_double()
+        _capi.check(_capi.lib().qsx_hpsi_stage_ms(self.h, int(stage), int(reps), dpsi.ptr, dhpsi.ptr, C.byref(ms)),
+                    None)
+        return ms.value
 
     def lanczos(self, psi0: np.ndarray, niter: int):
         return self._eig(_capi.lib().qsx_lanczos, psi0, niter)

@@ -1234,6 +1234,21 @@ int qsx_ctr_plan_pairs(const qsx_ctr_plan* plan, int64_t* npairs, int64_t* nout,
   return 0;
 }
 
+int qsx_ctr_plan_idxset(const qsx_ctr_plan* plan, int64_t* npairs, int64_t* nout, int32_t* ia, int32_t* ib,
+                        double* afac, int64_t* seg, int32_t* qc) {
+  if (!plan) return QSX_ERR_ARG;
+  const CtrPlan& P = *plan->p;
+  const int64_t np = (int64_t)P.L.Ia.size(), nc = (int64_t)P.L.seg.size() - 1;
+  if (npairs) *npairs = np;
+  if (nout) *nout = nc;
+  if (ia) std::copy(P.L.Ia.begin(), P.L.Ia.end(), ia);
+  if (ib) std::copy(P.L.Ib.begin(), P.L.Ib.end(), ib);
+  if (afac) std::copy(P.afac.begin(), P.afac.end(), afac);
+  if (seg) std::copy(P.L.seg.begin(), P.L.seg.end(), seg);
+  if (qc) std::copy(P.L.QC.begin(), P.L.QC.end(), qc);
+  return 0;
+}
+
 int qsx_ctr_execute(qsx_ctr_plan* plan, const double* dA, const double* dB, double* dC, void* stream) {
   if (!plan) return QSX_ERR_ARG;
   try {

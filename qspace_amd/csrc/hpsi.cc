@@ -384,7 +384,7 @@ static std::vector<SplitLine> split_lines(const std::vector<PairDesc>& pairs, co
       const int32_t base = (int32_t)lines.size();
       seen[key] = base;
       ebase[p] = base;
-      for (int i = 0; i < n; ++i) lines.push_back(SplitLine{off + i * sl, d.K, (int32_t)sk, base + i, 0});
+      for (int i = 0; i < n; ++i) lines.push_back(SplitLine{off + i * sl, d.K, (int32_t)sk, base + i, base});
     }
   return lines;
 }
@@ -410,6 +410,24 @@ static std::vector<SplitPair> split_pairs(const std::vector<PairDesc>& pairs, co
   return v;
 }
 
+// The slice planes of a buffer are indexed by element position, so two line
+// layouts of the same elements (different strides, e.g. one block read as
+// rows by one term and as columns by another) would write one plane element
+// under two exponents.  Such a plan is refused.  A line's layout is the
+// first line index of its block view (SplitLine::pad).
+static void split_check_lines(const std::vector<SplitLine>& lines, int64_t plane, const char* what) {
+  std::vector<int32_t> owner((size_t)std::max<int64_t>(1, plane), -1);
+  for (const SplitLine& l : lines)
+    for (int32_t k = 0; k < l.K; ++k) {
+      const int64_t e = l.off + (int64_t)k * l.sk;
+      if (e < 0 || e >= plane) fail(QSX_ERR_STRUCT, "split precision: %s line outside its buffer", what);
+      int32_t& o = owner[(size_t)e];
+      if (o >= 0 && o != l.pad)
+        fail(QSX_ERR_UNSUPPORTED, "split precision: %s is read in two layouts (one slice plane per buffer)", what);
+      o = l.pad;
+    }
+}
+
 static void split_side_alloc(HpsiPlan::SplitSide& sp, const std::vector<SplitLine>& lines, int64_t plane, int S,
                              hipStream_t s) {
   sp.plane = plane;
@@ -431,6 +449,10 @@ static void hpsi_split_setup(HpsiPlan& H) {
   std::vector<SplitLine> lB2 = split_lines(H.pairs2, H.targets2, false, eb2);
   int64_t hl_arena = H.hl_total;
   for (int t = 0; t < H.nterm; ++t) hl_arena = std::max(hl_arena, H.hl_base[t] + H.hl_total);
+  split_check_lines(lA1, hl_arena, "HL");
+  split_check_lines(lB1, H.N, "psi");
+  split_check_lines(lA2, H.t_total, "T");
+  split_check_lines(lB2, H.hr_total, "HR");
   split_side_alloc(H.spHL, lA1, hl_arena, S, s);
   split_side_alloc(H.spPsi, lB1, H.N, S, s);
   split_side_alloc(H.spT, lA2, H.t_total, S, s);
@@ -448,6 +470,18 @@ static void hpsi_split_setup(HpsiPlan& H) {
   pol.split_mfma = 1 << 30;
   H.swork1 = build_work(H.targets1, H.pairs1, pol);
   H.swork2 = build_work(H.targets2, H.pairs2, pol);
+  // the split kernel runs one 16 x 16 tile per wave that stores into C:
+  // a work list of another form (the workgroup-tile fallback for blocks
+  // beyond the wave engine's 32-bit offsets, shared or split tiles) would
+  // leave output elements unwritten
+  for (const WorkList* W : {&H.swork1, &H.swork2}) {
+    bool ok = W->engine == ENG_WAVE && W->reds.empty();
+    for (const WorkItem& w : W->items) {
+      if (w.slot == -2) continue;  // padding
+      if (w.shape != wave_shape(1, 1) || w.slot != -1 || w.group != 0) ok = false;
+    }
+    if (!ok) fail(QSX_ERR_UNSUPPORTED, "split precision: blocks too large for the 16 x 16 tile kernel");
+  }
   H.dsitems1 = upload_vec(H.swork1.items, s);
   H.dsitems2 = upload_vec(H.swork2.items, s);
   // executed slice-product flops: per 16x16 tile and k-step of 16 of a pair,
@@ -536,7 +570,7 @@ static void hpsi_create(HpsiPlan& H, const qsx_tensor* HL, const qsx_tensor* psi
          "FP32 slices: an exact FP32 accumulation allows ~8-bit slices, so 3 FP32 slices carry ~24 bits and "
          "cannot reach 1e-12; use QSX_MODE_SPLIT_BF16 (nslices >= 6)");
   if (H.mode == QSX_MODE_SPLIT_BF16) {
-    H.nsl = H.nslices > 0 ? H.nslices : 6;
+    H.nsl = H.nslices > 0 ? H.nslices : 7;
     if (H.nsl < 3 || H.nsl > 8) fail(QSX_ERR_ARG, "split precision: %d slices (3..8 supported)", H.nsl);
   }
   H.N = H.psi.total();
@@ -1047,6 +1081,35 @@ int qsx_hpsi_last_timing(const qsx_hpsi_plan* plan, double* ms2) {
     ms2[i] = t;
   }
   return 0;
+}
+
+int qsx_hpsi_stage_ms(qsx_hpsi_plan* plan, int32_t stage, int32_t reps, const double* dpsi, double* dhpsi,
+                      double* ms) {
+  if (!plan || !ms || (stage != 1 && stage != 2) || reps < 1 || !dpsi || !dhpsi) return QSX_ERR_ARG;
+  try {
+    HpsiPlan& H = *plan->h;
+    if (H.mode != QSX_MODE_BATCHED) fail(QSX_ERR_ARG, "stage timing is for the batched FP64 mode");
+    hipStream_t s = default_stream();
+    hpsi_launch(H, dpsi, dhpsi, s, false, false);  // T of this psi for stage 2; warm
+    auto one = [&]() {
+      if (stage == 1)
+        launch_worklist(H.work1, H.dq1, H.ditems1, H.dreds1, H.dpairs1, H.dHL, dpsi, H.dT, H.dscratch, s);
+      else
+        launch_worklist(H.work2, H.dq2, H.ditems2, H.dreds2, H.dpairs2, H.dT, H.dHR, dhpsi, H.dscratch, s);
+    };
+    one();
+    QSX_HIP(hipEventRecord(H.ev[0], s));
+    for (int r = 0; r < reps; ++r) one();
+    QSX_HIP(hipEventRecord(H.ev[1], s));
+    QSX_HIP(hipEventSynchronize(H.ev[1]));
+    float t = 0;
+    QSX_HIP(hipEventElapsedTime(&t, H.ev[0], H.ev[1]));
+    *ms = (double)t / reps;
+    return 0;
+  } catch (const Error& e) {
+    fprintf(stderr, "qsx_hpsi_stage_ms: %s\n", e.msg.c_str());
+    return e.code;
+  }
 }
 
 void qsx_hpsi_plan_destroy(qsx_hpsi_plan* plan) { delete plan; }
