@@ -248,6 +248,17 @@ double qsx_ctr_plan_flops(const qsx_ctr_plan* plan);
    exact zeros; offs / flags are host arrays, ddata device memory. */
 int qsx_block_zero_flags(const int64_t* offs, int32_t nblk, const double* ddata, int32_t* flags, char* err,
                          size_t errlen);
+/* qsx_block_zero_flags without any device allocation.  No counterpart in
+   the reference: it is for callers that cannot allocate device memory
+   (code under HIP graph capture, tight serving loops, harnesses that forbid
+   hipMalloc).  Same arguments and result as qsx_block_zero_flags; the
+   device copy of offs and the flags live in a fixed per-device array of the
+   library (1 MiB, held until the process ends), so nblk is at most 65536
+   (larger lists: QSX_ERR_ARG).  Requires qsx_device_init, which takes the
+   array's address on the device.  Calls that overlap on one device share
+   the array and race; the call is synchronous. */
+int qsx_block_zero_flags_noalloc(const int64_t* offs, int32_t nblk, const double* ddata, int32_t* flags, char* err,
+                                 size_t errlen);
 /* set the work decomposition policy (int32[QSX_POLICY_INTS], see below) */
 int qsx_ctr_plan_set_policy(qsx_ctr_plan* plan, const int32_t* policy);
 void qsx_ctr_plan_destroy(qsx_ctr_plan* plan);

@@ -1064,6 +1064,7 @@ int qsx_device_init(int device, char* err, size_t errlen) {
     if (device < 0 || device >= n) fail(QSX_ERR_ARG, "device %d out of range (%d devices)", device, n);
     QSX_HIP(hipSetDevice(device));
     ensure_device();
+    noalloc_scratch_setup();  // the no-allocation entry points' array on this device
     return 0;
   });
 }
@@ -1247,6 +1248,31 @@ int qsx_ctr_plan_idxset(const qsx_ctr_plan* plan, int64_t* npairs, int64_t* nout
   if (seg) std::copy(P.L.seg.begin(), P.L.seg.end(), seg);
   if (qc) std::copy(P.L.QC.begin(), P.L.QC.end(), qc);
   return 0;
+}
+
+int qsx_block_zero_flags_noalloc(const int64_t* offs, int32_t nblk, const double* ddata, int32_t* flags, char* err,
+                                 size_t errlen) {
+  return guarded(err, errlen, [&] {
+    if (nblk < 0 || (nblk && (!offs || !flags))) fail(QSX_ERR_ARG, "invalid block list");
+    if (nblk > kNoAllocMaxBlocks)
+      fail(QSX_ERR_ARG, "qsx_block_zero_flags_noalloc: %d blocks exceed its maximum of %d (use qsx_block_zero_flags)",
+           nblk, kNoAllocMaxBlocks);
+    const int dev = g_dev.load(std::memory_order_acquire);
+    if (dev < 0) fail(QSX_ERR_DEVICE, "qsx_block_zero_flags_noalloc: call qsx_device_init first");
+    char* base = (char*)noalloc_scratch(dev);
+    if (!base) fail(QSX_ERR_DEVICE, "qsx_block_zero_flags_noalloc: no scratch on device %d (qsx_device_init)", dev);
+    if (!nblk) return 0;
+    // scratch layout: nblk+1 offsets, then nblk flags (every flag is written
+    // by the kernel, so nothing needs clearing)
+    int64_t* doffs = (int64_t*)base;
+    int32_t* dz = (int32_t*)(base + sizeof(int64_t) * (kNoAllocMaxBlocks + 1));
+    hipStream_t s = g_stream;
+    QSX_HIP(hipMemcpyAsync(doffs, offs, sizeof(int64_t) * (nblk + 1), hipMemcpyHostToDevice, s));
+    launch_block_zero_flags(doffs, nblk, ddata, dz, s);
+    QSX_HIP(hipMemcpyAsync(flags, dz, sizeof(int32_t) * nblk, hipMemcpyDeviceToHost, s));
+    QSX_HIP(hipStreamSynchronize(s));
+    return 0;
+  });
 }
 
 int qsx_ctr_execute(qsx_ctr_plan* plan, const double* dA, const double* dB, double* dC, void* stream) {

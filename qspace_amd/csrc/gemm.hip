@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 
 #include "qsx_internal.hpp"
 
@@ -177,6 +178,38 @@ void launch_pair_gemm_f64(const PairDesc& p, int64_t c_off, int M, int N, int be
   dim3 g((M + 63) / 64, (N + 63) / 64);
   hipLaunchKernelGGL((k_pair_gemm_f64<64, 64>), g, dim3(256), 0, s, p, c_off, M, N, beta, A, B, C);
   QSX_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------- no-allocation scratch
+// Device memory of the entry points that allocate nothing
+// (qsx_block_zero_flags_noalloc): a file-scope array, one per device, held
+// until the process ends.  Host code never uses the array's name as an
+// address (on the host it is a stand-in, not device memory): the device
+// address comes from hipGetSymbolAddress, taken once per device with that
+// device current, and is what kernels and copies receive.  Every call on a
+// device shares that device's array; calls that overlap on one device race.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+void noalloc_scratch_setup() {
+  int dev = -1;
+  QSX_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) fail(QSX_ERR_DEVICE, "device %d beyond the library's table", dev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[dev]) return;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    fail(QSX_ERR_DEVICE, "hipGetSymbolAddress(scratch) failed on device %d: %s", dev, hipGetErrorString(e));
+  g_scratch_ptr[dev] = p;
+}
+
+void* noalloc_scratch(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch_ptr[dev];
 }
 
 void launch_block_zero_flags(const int64_t* doffs, int nblk, const double* data, int32_t* flags, hipStream_t s) {

@@ -227,6 +227,19 @@ inline void launch_worklist(const WorkList& W, const WorkQueue* dqueues, const W
     launch_work(dqueues, (int)W.queues.size(), ditems, dreds, (int)W.reds.size(), dpairs, A, B, C, scratch, s);
 }
 void launch_copy(double* d, const double* src, int64_t n, hipStream_t s);
+// device scratch of the no-allocation entry points (gemm.hip): a file-scope
+// __device__ array of kNoAllocScratchBytes per device; setup takes its
+// address on the current device (hipGetSymbolAddress), noalloc_scratch(dev)
+// returns it (nullptr before the setup on that device)
+constexpr size_t kNoAllocScratchBytes = size_t(1) << 20;
+constexpr int kMaxDevices = 64;
+// SkipZeroData test without allocation: at most this many blocks (offsets
+// and flags in the scratch array)
+constexpr int kNoAllocMaxBlocks = 65536;
+static_assert(sizeof(int64_t) * (kNoAllocMaxBlocks + 1) + sizeof(int32_t) * kNoAllocMaxBlocks <= kNoAllocScratchBytes,
+              "zero-test scratch exceeds the array");
+void noalloc_scratch_setup();
+void* noalloc_scratch(int dev);
 void launch_pair_gemm_f64(const PairDesc& p, int64_t c_off, int M, int N, int beta, const double* A,
                           const double* B, double* C, hipStream_t s);
 // split precision (split.hip): a contraction line of an operand block (a row
