@@ -16,6 +16,7 @@
  *                          Source/QSpace_aux.cc:27-355); the split mirrors the
  *                          pair-list caching of CPAT (Source/cpat.hh:572-593)
  *   qsx_hpsi_*          <- two-site H*psi of Class/@Hamilton1D/private/get_HPsi.m:54-57
+ *                          (full MPO) and :62-83 (pseudo MPO, qsx_hpsi_plan_create_pseudo)
  *                          (contractQS({HK_L,Psi},HK_R)) and the legacy C++
  *                          CPAT::HZTimes, Source/cpat.hh:487-527
  *   qsx_lanczos         <- RunEigs Lanczos loop, Source/eigs2SiteQS.cc:98-334
@@ -271,6 +272,19 @@ void qsx_ctr_plan_destroy(qsx_ctr_plan* plan);
 typedef struct qsx_hpsi_plan qsx_hpsi_plan;
 int qsx_hpsi_plan_create(const qsx_tensor* HL, const qsx_tensor* psi, const qsx_tensor* HR,
                          const qsx_opts* opts, qsx_hpsi_plan** plan, char* err, size_t errlen);
+/* H*psi of the pseudo MPO (Class/@Hamilton1D/private/get_HPsi.m:62-83, the
+   form update_psi_2site.m:63-66 takes when the Hamiltonian has no full
+   MPO): HK_L.psi + psi.HK_R + sum_j contractQS({OP_L(j),'*',psi},OP_R(j))
+   [+ contractQS({OP_L(j),psi},OP_R(j),'*') when hconj[j]], summed in that
+   order, all contractions by itag matching.  HKL / HKR (rank 2 block
+   Hamiltonians) may be NULL; OPL / OPR are nop operator tensors (rank 3,
+   OP_L(j) acting on psi's left leg, OP_R(j) on its right leg, their
+   operator legs matched by itag); the caller passes the operators of the
+   terms J it selects.  Same plan type as qsx_hpsi_plan_create (apply,
+   Lanczos, Davidson, autotune); batched FP64 or plain mode. */
+int qsx_hpsi_plan_create_pseudo(const qsx_tensor* HKL, const qsx_tensor* HKR, int32_t nop, const qsx_tensor* OPL,
+                                const qsx_tensor* OPR, const int32_t* hconj, const qsx_tensor* psi,
+                                const qsx_opts* opts, qsx_hpsi_plan** plan, char* err, size_t errlen);
 int64_t qsx_hpsi_size(const qsx_hpsi_plan* plan);      /* doubles in psi */
 double qsx_hpsi_flops(const qsx_hpsi_plan* plan);      /* useful flops per H*psi */
 double qsx_hpsi_bytes(const qsx_hpsi_plan* plan);      /* algorithmic bytes per H*psi */

@@ -79,6 +79,8 @@ def lib():
         "qsx_hpsi_set_policy": ([vp, P(i32)], C.c_int),
         "qsx_ctr_plan_destroy": ([vp], None),
         "qsx_hpsi_plan_create": ([P(QsxTensor), P(QsxTensor), P(QsxTensor), P(QsxOpts), P(vp), cs, sz], C.c_int),
+        "qsx_hpsi_plan_create_pseudo": ([P(QsxTensor), P(QsxTensor), i32, P(QsxTensor), P(QsxTensor), P(i32),
+                                         P(QsxTensor), P(QsxOpts), P(vp), cs, sz], C.c_int),
         "qsx_hpsi_size": ([vp], i64),
         "qsx_hpsi_flops": ([vp], f64),
         "qsx_hpsi_bytes": ([vp], f64),
@@ -110,7 +112,7 @@ def exported_symbols() -> list[str]:
             "qsx_permute", "qsx_plus", "qsx_ortho", "qsx_ortho_info_free",
             "qsx_parse_ctridx", "qsx_packed_offsets", "qsx_ctr_plan_create", "qsx_ctr_plan_output",
             "qsx_ctr_plan_pairs", "qsx_ctr_plan_idxset", "qsx_ctr_execute", "qsx_ctr_plan_flops", "qsx_ctr_plan_set_policy", "qsx_ctr_plan_destroy", "qsx_block_zero_flags", "qsx_block_zero_flags_noalloc",
-            "qsx_hpsi_plan_create", "qsx_hpsi_size", "qsx_hpsi_flops", "qsx_hpsi_bytes", "qsx_hpsi_apply",
+            "qsx_hpsi_plan_create", "qsx_hpsi_plan_create_pseudo", "qsx_hpsi_size", "qsx_hpsi_flops", "qsx_hpsi_bytes", "qsx_hpsi_apply",
             "qsx_hpsi_use_graph", "qsx_hpsi_weights", "qsx_hpsi_stage_info", "qsx_hpsi_autotune", "qsx_hpsi_set_policy", "qsx_hpsi_last_timing", "qsx_hpsi_stage_ms", "qsx_hpsi_plan_destroy", "qsx_lanczos", "qsx_davidson",
             "qsx_su2_cg", "qsx_su2_x3"]
 

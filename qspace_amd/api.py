@@ -14,7 +14,7 @@ import ctypes as C
 import numpy as np
 
 from . import _capi
-from .qspace import QSpace, QsxOpts, QsxTensorOut, from_out, parse_ctridx, to_c
+from .qspace import QSpace, QsxOpts, QsxTensor, QsxTensorOut, from_out, parse_ctridx, to_c
 
 __all__ = ["contractQS", "permuteQS", "plusQS", "orthoQS", "ortho_args", "DeviceBuffer", "CtrPlan", "HPsi", "su2_cg", "su2_x3", "make_opts"]
 
@@ -634,6 +634,53 @@ class HPsi:
         self.N = int(L.qsx_hpsi_size(self.h))
         self.flops = float(L.qsx_hpsi_flops(self.h))
         self.bytes = float(L.qsx_hpsi_bytes(self.h))
+
+    @classmethod
+    def pseudo(cls, HKL, psi: QSpace, HKR, OPL, OPR, hconj=None, mode="batched", verbose=0) -> "HPsi":
+        """H*psi of the pseudo MPO (get_HPsi.m:62-83): HK_L.psi + psi.HK_R +
+        sum_j contractQS({OP_L[j],'*',psi},OP_R[j]) [+ contractQS({OP_L[j],psi},
+        OP_R[j],'*') where hconj[j]]; HKL / HKR may be None, hconj defaults to
+        1 for every operator."""
+        L = _capi.lib()
+        if len(OPL) != len(OPR):
+            raise ValueError("OP_L and OP_R lists differ in length")
+        self = cls.__new__(cls)
+        keep = []
+        cL = cR = None
+        if HKL is not None:
+            cL, k = to_c(HKL)
+            keep.append(k)
+        if HKR is not None:
+            cR, k = to_c(HKR)
+            keep.append(k)
+        n = len(OPL)
+        arrL = (QsxTensor * max(1, n))()
+        arrR = (QsxTensor * max(1, n))()
+        for j in range(n):
+            arrL[j], k1 = to_c(OPL[j])
+            arrR[j], k2 = to_c(OPR[j])
+            keep += [k1, k2]
+        hc = np.ascontiguousarray([1] * n if hconj is None else [int(x) for x in hconj], dtype=np.int32)
+        if hc.size != n:
+            raise ValueError("hconj needs one flag per operator")
+        cP, kp = to_c(psi, with_data=False)
+        keep.append(kp)
+        self._opts, self._okeep = make_opts(mode=mode, skip_zero=0, verbose=verbose)
+        self._k1 = keep
+        self.h = C.c_void_p()
+        self._release = L.qsx_hpsi_plan_destroy
+        err = C.create_string_buffer(1024)
+        P = C.POINTER
+        rc = L.qsx_hpsi_plan_create_pseudo(C.byref(cL) if cL is not None else None,
+                                           C.byref(cR) if cR is not None else None, n, arrL, arrR,
+                                           hc.ctypes.data_as(P(C.c_int32)), C.byref(cP), C.byref(self._opts),
+                                           C.byref(self.h), err, 1024)
+        _capi.check(rc, err)
+        self.psi = psi
+        self.N = int(L.qsx_hpsi_size(self.h))
+        self.flops = float(L.qsx_hpsi_flops(self.h))
+        self.bytes = float(L.qsx_hpsi_bytes(self.h))
+        return self
 
     def stage_info(self) -> dict:
         """per stage: useful flops, algorithmic bytes, work items, pairs, the

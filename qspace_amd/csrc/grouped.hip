@@ -52,11 +52,12 @@ struct StepPair {  // the pair a k-step reads from
 };
 
 __device__ __forceinline__ StepPair load_pair(const PairDesc* __restrict__ pairs, int p, const double* A,
-                                              const double* B) {
+                                              const double* B, const double* X) {
   const PairDesc pd = pairs[p];
   StepPair s;
-  s.a = A + pd.a_off;
-  s.b = B + pd.b_off;
+  // flags bit 2 / 3: the operand block lies in X (the input vector)
+  s.a = ((pd.flags & kPairAinX) ? X : A) + pd.a_off;
+  s.b = ((pd.flags & kPairBinX) ? X : B) + pd.b_off;
   s.K = pd.K;
   s.lda = pd.lda;
   s.ldb = pd.ldb;
@@ -154,8 +155,8 @@ struct Cursor {
 template <int TM, int TN, int WM, int WN>
 __device__ __forceinline__ void run_queue(const WorkQueue& q, const WorkItem* __restrict__ items,
                                           const PairDesc* __restrict__ pairs, const double* __restrict__ A,
-                                          const double* __restrict__ B, double* __restrict__ C,
-                                          double* __restrict__ scratch, void* smem) {
+                                          const double* __restrict__ B, const double* __restrict__ X,
+                                          double* __restrict__ C, double* __restrict__ scratch, void* smem) {
   static_assert(WM * WN == 4, "4 waves");
   constexpr int FM = TM / (16 * WM), FN = TN / (16 * WN);
   Lds<TM, TN>& sm = *reinterpret_cast<Lds<TM, TN>*>(smem);
@@ -178,7 +179,7 @@ __device__ __forceinline__ void run_queue(const WorkQueue& q, const WorkItem* __
     w = items[it];
   }
   Cursor cur{it, w.p0, w.k0, 0};
-  StepPair sp = load_pair(pairs, cur.p, A, B);
+  StepPair sp = load_pair(pairs, cur.p, A, B, X);
   cur.kend = (cur.p == w.p1) ? w.k1 : sp.K;
   gload<TM, TN>(sp, cur.k, w.M, w.N, w.m0, w.n0, ra, rb);
   lstore<TM, TN>(sm, 0, sp, ra, rb);
@@ -195,7 +196,7 @@ __device__ __forceinline__ void run_queue(const WorkQueue& q, const WorkItem* __
       if (cur.p < w.p1) {
         nxt.p = cur.p + 1;
         nxt.k = 0;
-        nsp = load_pair(pairs, nxt.p, A, B);
+        nsp = load_pair(pairs, nxt.p, A, B, X);
         nxt.kend = (nxt.p == w.p1) ? w.k1 : nsp.K;
       } else {
         newitem = true;
@@ -204,7 +205,7 @@ __device__ __forceinline__ void run_queue(const WorkQueue& q, const WorkItem* __
           nw = items[ni];
           if (nw.p0 <= nw.p1) {
             nxt = Cursor{ni, nw.p0, nw.k0, 0};
-            nsp = load_pair(pairs, nxt.p, A, B);
+            nsp = load_pair(pairs, nxt.p, A, B, X);
             nxt.kend = (nxt.p == nw.p1) ? nw.k1 : nsp.K;
             more = true;
             break;
@@ -249,18 +250,18 @@ constexpr size_t kLdsBytes = sizeof(Lds<64, 64>);
 
 __global__ __launch_bounds__(NT) void k_grouped(const WorkQueue* __restrict__ queues, const WorkItem* __restrict__ items,
                                                 const PairDesc* __restrict__ pairs, const double* __restrict__ A,
-                                                const double* __restrict__ B, double* __restrict__ C,
-                                                double* __restrict__ scratch) {
+                                                const double* __restrict__ B, const double* __restrict__ X,
+                                                double* __restrict__ C, double* __restrict__ scratch) {
   __shared__ __align__(16) unsigned char smem[kLdsBytes];
   const WorkQueue q = queues[blockIdx.x];
   if (q.i0 >= q.i1) return;
   switch (q.shape) {
-    case SH_64x64: run_queue<64, 64, 2, 2>(q, items, pairs, A, B, C, scratch, smem); break;
-    case SH_32x32: run_queue<32, 32, 2, 2>(q, items, pairs, A, B, C, scratch, smem); break;
-    case SH_16x64: run_queue<16, 64, 1, 4>(q, items, pairs, A, B, C, scratch, smem); break;
-    case SH_64x16: run_queue<64, 16, 4, 1>(q, items, pairs, A, B, C, scratch, smem); break;
-    case SH_32x64: run_queue<32, 64, 2, 2>(q, items, pairs, A, B, C, scratch, smem); break;
-    case SH_64x32: run_queue<64, 32, 2, 2>(q, items, pairs, A, B, C, scratch, smem); break;
+    case SH_64x64: run_queue<64, 64, 2, 2>(q, items, pairs, A, B, X, C, scratch, smem); break;
+    case SH_32x32: run_queue<32, 32, 2, 2>(q, items, pairs, A, B, X, C, scratch, smem); break;
+    case SH_16x64: run_queue<16, 64, 1, 4>(q, items, pairs, A, B, X, C, scratch, smem); break;
+    case SH_64x16: run_queue<64, 16, 4, 1>(q, items, pairs, A, B, X, C, scratch, smem); break;
+    case SH_32x64: run_queue<32, 64, 2, 2>(q, items, pairs, A, B, X, C, scratch, smem); break;
+    case SH_64x32: run_queue<64, 32, 2, 2>(q, items, pairs, A, B, X, C, scratch, smem); break;
     default: break;
   }
 }
@@ -294,9 +295,10 @@ __global__ __launch_bounds__(NT) void k_reduce(const ReduceItem* __restrict__ re
 }  // namespace
 
 void launch_work(const WorkQueue* dqueues, int nqueues, const WorkItem* ditems, const ReduceItem* dreds, int nreds,
-                 const PairDesc* dpairs, const double* A, const double* B, double* C, double* scratch, hipStream_t s) {
+                 const PairDesc* dpairs, const double* A, const double* B, const double* X, double* C, double* scratch,
+                 hipStream_t s) {
   if (nqueues > 0) {
-    hipLaunchKernelGGL(k_grouped, dim3(nqueues), dim3(NT), 0, s, dqueues, ditems, dpairs, A, B, C, scratch);
+    hipLaunchKernelGGL(k_grouped, dim3(nqueues), dim3(NT), 0, s, dqueues, ditems, dpairs, A, B, X, C, scratch);
     QSX_HIP(hipGetLastError());
   }
   if (nreds > 0) {

@@ -29,22 +29,32 @@
 
 namespace qsx {
 
+// one two-stage term: contractQS({L [,'*'], psi}, R [,'*']) with L = Ls[li]
+// (conj if cl) and R = Rs[ri] (conj if cr)
+struct HpsiTerm {
+  int li, cl, ri, cr;
+};
+
 struct HpsiPlan {
-  TensorDesc HL, psi, HR;
+  std::vector<TensorDesc> Ls, Rs;  // left / right operators of the terms
+  TensorDesc psi;
+  std::vector<HpsiTerm> terms;
+  TensorDesc HKL, HKR;             // local terms HK_L . psi and psi . HK_R (pseudo MPO), if present
+  bool has_hkl = false, has_hkr = false;
   int nterm = 1;
   int verbose = 0;
   int mode = QSX_MODE_BATCHED;
   int nslices = 0;
-  std::unique_ptr<CtrPlan> s1[2], s2[2];
-  std::vector<int32_t> t1Q[2];
-  std::vector<int64_t> t1dims[2];
-  std::vector<uint64_t> t1tags[2];
+  std::vector<std::unique_ptr<CtrPlan>> s1, s2;  // per term
+  std::unique_ptr<CtrPlan> locL, locR;           // the local terms' contractions
   int64_t N = 0;
-  double* dHL = nullptr;   // arena: [HL | HL permuted copies]
-  double* dHR = nullptr;   // arena: [HR copies in the stage-2 layouts]
-  double* dT = nullptr;    // arena: [T_0 | T_1]
-  int64_t hl_base[2] = {0, 0}, hr_base[2] = {0, 0}, t_base[2] = {0, 0};
-  int64_t t_total = 0, hr_total = 0, hl_total = 0;
+  double* dHL = nullptr;   // arena: [Ls as uploaded | permuted copies per term]
+  double* dHR = nullptr;   // arena: [R copies per term in the stage-2 layouts | HK_R copy]
+  double* dT = nullptr;    // arena: [T_t per term | HK_L copy]
+  std::vector<int64_t> hl_base, hr_base, t_base;  // per term
+  int64_t t_total = 0, hr_total = 0, hl_total = 0;  // T part of dT, whole dHR / dHL arenas
+  int64_t hkl_base = 0, hkr_base = 0, dt_size = 0;
+  bool xpairs = false;     // stage 2 has pairs that read psi itself (local terms)
   std::vector<PairDesc> pairs1, pairs2;
   std::vector<WorkTarget> targets1, targets2;
   WorkList work1, work2;
@@ -141,7 +151,8 @@ void build_multi(const std::vector<CtrPlan*>& plans, const std::vector<int64_t>&
                  const std::vector<int64_t>& b_base, const std::vector<std::vector<int>>& target,
                  const std::vector<int64_t>& t_off, const std::vector<int>& tM, const std::vector<int>& tN,
                  std::vector<PairDesc>& pairs, std::vector<WorkTarget>& targets,
-                 std::vector<int>* pair_target = nullptr, std::vector<std::pair<int, int64_t>>* pair_src = nullptr) {
+                 std::vector<int>* pair_target = nullptr, std::vector<std::pair<int, int64_t>>* pair_src = nullptr,
+                 const std::vector<int32_t>* xflags = nullptr) {
   const int nt = (int)t_off.size();
   std::vector<std::vector<std::pair<int, int>>> src(nt);  // (term, block)
   for (size_t t = 0; t < plans.size(); ++t)
@@ -165,6 +176,7 @@ void build_multi(const std::vector<CtrPlan*>& plans, const std::vector<int64_t>&
         PairDesc pd = P.pairs[p];
         pd.a_off += a_base[t];
         pd.b_off += b_base[t];
+        if (xflags) pd.flags |= (*xflags)[t];
         pairs.push_back(pd);
         if (pair_target) pair_target->push_back(g);
         if (pair_src) pair_src->push_back({(int)t, p});
@@ -253,7 +265,7 @@ static double time_stage(HpsiPlan& H, int stage, const WorkList& W, const double
     if (stage == 0)
       launch_worklist(W, dq, di, dr, p, H.dHL, dpsi, H.dT, sc, s);
     else
-      launch_worklist(W, dq, di, dr, p, H.dT, H.dHR, dout, sc, s);
+      launch_worklist(W, dq, di, dr, p, H.dT, H.dHR, dout, sc, s, dpsi);
     QSX_HIP(hipEventRecord(e1, s));
     QSX_HIP(hipEventSynchronize(e1));
     float t = 0;
@@ -447,8 +459,7 @@ static void hpsi_split_setup(HpsiPlan& H) {
   std::vector<SplitLine> lB1 = split_lines(H.pairs1, H.targets1, false, eb1);
   std::vector<SplitLine> lA2 = split_lines(H.pairs2, H.targets2, true, ea2);
   std::vector<SplitLine> lB2 = split_lines(H.pairs2, H.targets2, false, eb2);
-  int64_t hl_arena = H.hl_total;
-  for (int t = 0; t < H.nterm; ++t) hl_arena = std::max(hl_arena, H.hl_base[t] + H.hl_total);
+  const int64_t hl_arena = H.hl_total;
   split_check_lines(lA1, hl_arena, "HL");
   split_check_lines(lB1, H.N, "psi");
   split_check_lines(lA2, H.t_total, "T");
@@ -514,7 +525,7 @@ static void hpsi_launch(HpsiPlan& H, const double* dpsi, double* dhpsi, hipStrea
   } else {
     launch_worklist(H.work1, H.dq1, H.ditems1, H.dreds1, p1, H.dHL, dpsi, H.dT, H.dscratch, s);
     if (timed) QSX_HIP(hipEventRecord(H.ev[1], s));
-    launch_worklist(H.work2, H.dq2, H.ditems2, H.dreds2, p2, H.dT, H.dHR, dhpsi, H.dscratch, s);
+    launch_worklist(H.work2, H.dq2, H.ditems2, H.dreds2, p2, H.dT, H.dHR, dhpsi, H.dscratch, s, dpsi);
   }
   if (timed) QSX_HIP(hipEventRecord(H.ev[2], s));
 }
@@ -528,8 +539,11 @@ static void hpsi_plain(HpsiPlan& H, const double* dpsi, double* dhpsi, hipStream
         launch_fill_zero(C + t.c_off, (int64_t)t.M * t.N, s);
         continue;
       }
-      for (int64_t p = t.p0; p < t.p1; ++p)
-        launch_pair_gemm_f64(pairs[p], t.c_off, t.M, t.N, p == t.p0 ? 0 : 1, A, B, C, s);
+      for (int64_t p = t.p0; p < t.p1; ++p) {
+        const PairDesc& pd = pairs[p];
+        launch_pair_gemm_f64(pd, t.c_off, t.M, t.N, p == t.p0 ? 0 : 1, (pd.flags & kPairAinX) ? dpsi : A,
+                             (pd.flags & kPairBinX) ? dpsi : B, C, s);
+      }
     }
   };
   run(H.pairs1, H.targets1, H.dHL, dpsi, H.dT);
@@ -548,17 +562,37 @@ struct PhaseClock {
   }
 };
 
-static void hpsi_create(HpsiPlan& H, const qsx_tensor* HL, const qsx_tensor* psi, const qsx_tensor* HR,
-                        const qsx_opts* o, int hconj) {
-  for (const qsx_tensor* t : {HL, psi, HR})
+// The plan of H*psi = [HK_L.psi] + [psi.HK_R] + sum_t contractQS({L_t[*], psi}, R_t[*]):
+// stage 1 computes the intermediates T_t = L_t[*] . psi of every term, stage
+// 2 every block of H*psi as one sum over the local terms' pairs (operand psi
+// read directly) and the terms' T_t . R_t[*] pairs, in that order.
+static void hpsi_create(HpsiPlan& H, const std::vector<const qsx_tensor*>& Ls, const std::vector<const qsx_tensor*>& Rs,
+                        const std::vector<HpsiTerm>& terms, const qsx_tensor* psi, const qsx_tensor* hkl,
+                        const qsx_tensor* hkr, const qsx_opts* o) {
+  for (const qsx_tensor* t : Ls)
     if (t && t->data_im) fail(QSX_ERR_UNSUPPORTED, "H*psi plans take real tensors (complex data not supported)");
+  for (const qsx_tensor* t : Rs)
+    if (t && t->data_im) fail(QSX_ERR_UNSUPPORTED, "H*psi plans take real tensors (complex data not supported)");
+  for (const qsx_tensor* t : {psi, hkl, hkr})
+    if (t && t->data_im) fail(QSX_ERR_UNSUPPORTED, "H*psi plans take real tensors (complex data not supported)");
+  if (!psi) fail(QSX_ERR_ARG, "H*psi: psi missing");
+  for (const HpsiTerm& t : terms)
+    if (t.li < 0 || t.li >= (int)Ls.size() || t.ri < 0 || t.ri >= (int)Rs.size() || !Ls[t.li] || !Rs[t.ri])
+      fail(QSX_ERR_ARG, "H*psi: term operator missing");
+  if (terms.empty() && !hkl && !hkr) fail(QSX_ERR_ARG, "H*psi: no terms");
   ensure_device();
   PhaseClock clk{o ? o->verbose : 0};
   hipStream_t s = default_stream();
-  H.HL = make_desc(HL, "HL");
+  for (size_t i = 0; i < Ls.size(); ++i) H.Ls.push_back(make_desc(Ls[i], "HL"));
+  for (size_t i = 0; i < Rs.size(); ++i) H.Rs.push_back(make_desc(Rs[i], "HR"));
   H.psi = make_desc(psi, "psi");
-  H.HR = make_desc(HR, "HR");
-  H.nterm = hconj ? 2 : 1;
+  H.terms = terms;
+  H.nterm = (int)terms.size();
+  H.has_hkl = hkl != nullptr;
+  H.has_hkr = hkr != nullptr;
+  if (hkl) H.HKL = make_desc(hkl, "HK_L");
+  if (hkr) H.HKR = make_desc(hkr, "HK_R");
+  H.xpairs = H.has_hkl || H.has_hkr;
   if (o) {
     H.mode = o->mode;
     H.nslices = o->nslices;
@@ -570,29 +604,38 @@ static void hpsi_create(HpsiPlan& H, const qsx_tensor* HL, const qsx_tensor* psi
          "FP32 slices: an exact FP32 accumulation allows ~8-bit slices, so 3 FP32 slices carry ~24 bits and "
          "cannot reach 1e-12; use QSX_MODE_SPLIT_BF16 (nslices >= 6)");
   if (H.mode == QSX_MODE_SPLIT_BF16) {
+    if (H.xpairs || Ls.size() != 1 || Rs.size() != 1)
+      fail(QSX_ERR_UNSUPPORTED, "split precision: plans of one environment pair (full MPO) only");
     H.nsl = H.nslices > 0 ? H.nslices : 7;
     if (H.nsl < 3 || H.nsl > 8) fail(QSX_ERR_ARG, "split precision: %d slices (3..8 supported)", H.nsl);
   }
+  const double* fermA = o ? o->fermA : nullptr;  // exact +-1 factors of the blocks of Ls[0] (parity operators)
+  if (fermA && Ls.size() != 1) fail(QSX_ERR_ARG, "H*psi: fermion signs need a single left environment");
   H.N = H.psi.total();
   H.pol1.slots = H.pol2.slots = resident_slots();
   qsx_opts so{};
   so.mode = QSX_MODE_BATCHED;
   so.verbose = 0;
   so.skip_zero = 0;
-  so.fermA = o ? o->fermA : nullptr;  // exact +-1 factors of HL blocks (parity operators)
-  // stage 1 plans: ({HL [,'*'], psi}) with itag matching
+  // stage 1 plans: ({L_t [,'*'], psi}) with itag matching
+  H.s1.resize(H.nterm);
+  H.s2.resize(H.nterm);
   for (int t = 0; t < H.nterm; ++t) {
+    const HpsiTerm& tm = terms[t];
     std::vector<int32_t> ia, ib;
     int cA = 0;
-    match_itags(H.HL, H.psi, t == 1, 0, ia, ib, cA);
-    H.s1[t] = plan_create(HL, ia.data(), (int)ia.size(), cA, psi, ib.data(), (int)ib.size(), 0, nullptr, 0, &so, false,
-                          1);
+    match_itags(H.Ls[tm.li], H.psi, tm.cl, 0, ia, ib, cA);
+    qsx_opts s1o = so;
+    s1o.fermA = fermA;
+    H.s1[t] = plan_create(Ls[tm.li], ia.data(), (int)ia.size(), cA, psi, ib.data(), (int)ib.size(), 0, nullptr, 0,
+                          &s1o, false, 1);
     if (H.s1[t]->lb.permute) fail(QSX_ERR_UNSUPPORTED, "H*psi: psi needs a permute in stage 1");
   }
   clk.lap("stage-1 structure");
-  // stage 2 plans: (T_t, HR [,'*'])
-  HostTensor Tt[2];
+  // stage 2 plans: (T_t, R_t [,'*'])
+  std::vector<HostTensor> Tt(H.nterm);
   for (int t = 0; t < H.nterm; ++t) {
+    const HpsiTerm& tm = terms[t];
     CtrPlan& P = *H.s1[t];
     Tt[t].Q = P.Qout;
     Tt[t].dims = P.dimsOut;
@@ -602,52 +645,136 @@ static void hpsi_create(HpsiPlan& H, const qsx_tensor* HL, const qsx_tensor* psi
     TensorDesc Td = make_desc(&Tt[t].t, "T");
     std::vector<int32_t> ia, ib;
     int cA = 0;
-    match_itags(Td, H.HR, 0, t == 1, ia, ib, cA);
-    qsx_opts s2o = so;
-    s2o.fermA = nullptr;
-    H.s2[t] = plan_create(&Tt[t].t, ia.data(), (int)ia.size(), cA, HR, ib.data(), (int)ib.size(), t == 1, nullptr, 0,
-                          &s2o, false, 2);
+    match_itags(Td, H.Rs[tm.ri], 0, tm.cr, ia, ib, cA);
+    H.s2[t] = plan_create(&Tt[t].t, ia.data(), (int)ia.size(), cA, Rs[tm.ri], ib.data(), (int)ib.size(), tm.cr,
+                          nullptr, 0, &so, false, 2);
     if (H.s2[t]->la.permute) fail(QSX_ERR_UNSUPPORTED, "H*psi: intermediate needs a permute in stage 2");
   }
+  // local terms: HK_L . psi (itag matching, psi as the B operand) and
+  // psi . HK_R (psi as the A operand); psi is read where it lies
+  if (hkl) {
+    std::vector<int32_t> ia, ib;
+    int cA = 0;
+    match_itags(H.HKL, H.psi, 0, 0, ia, ib, cA);
+    H.locL = plan_create(hkl, ia.data(), (int)ia.size(), cA, psi, ib.data(), (int)ib.size(), 0, nullptr, 0, &so,
+                         false, 1);
+    if (H.locL->lb.permute) fail(QSX_ERR_UNSUPPORTED, "H*psi: psi needs a permute in HK_L . psi");
+  }
+  if (hkr) {
+    std::vector<int32_t> ia, ib;
+    int cA = 0;
+    match_itags(H.psi, H.HKR, 0, 0, ia, ib, cA);
+    H.locR = plan_create(psi, ia.data(), (int)ia.size(), cA, hkr, ib.data(), (int)ib.size(), 0, nullptr, 0, &so,
+                         false, 2);
+    if (H.locR->la.permute) fail(QSX_ERR_UNSUPPORTED, "H*psi: psi needs a permute in psi . HK_R");
+  }
   clk.lap("stage-2 structure");
-  // device arenas: HL (+ permuted copies), HR copies, T_t
-  H.hl_total = H.HL.total();
-  int64_t off = H.hl_total;
+  // device arenas.  dHL: every left operator as uploaded, then a permuted
+  // copy per term whose stage-1 layout needs one
+  std::vector<int64_t> l_base(Ls.size(), 0);
+  int64_t off = 0;
+  for (size_t i = 0; i < Ls.size(); ++i) {
+    l_base[i] = off;
+    off += H.Ls[i].total();
+  }
+  H.hl_base.assign(H.nterm, 0);
   for (int t = 0; t < H.nterm; ++t) {
     if (H.s1[t]->la.permute) {
       H.hl_base[t] = off;
-      off += H.hl_total;
-    } else H.hl_base[t] = 0;
+      off += H.Ls[terms[t].li].total();
+    } else H.hl_base[t] = l_base[terms[t].li];
   }
+  H.hl_total = off;
   H.dHL = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, off));
-  upload_blocks(H.HL, H.dHL, s);
+  for (size_t i = 0; i < Ls.size(); ++i) upload_blocks(H.Ls[i], H.dHL + l_base[i], s);
   for (int t = 0; t < H.nterm; ++t)
-    if (H.s1[t]->la.permute) plan_permute_operand(*H.s1[t], 0, H.dHL, H.dHL + H.hl_base[t], s);
+    if (H.s1[t]->la.permute) plan_permute_operand(*H.s1[t], 0, H.dHL + l_base[terms[t].li], H.dHL + H.hl_base[t], s);
+  // dHR: a copy of R_t per term in its stage-2 layout, then HK_R in its layout
   int64_t roff = 0;
+  H.hr_base.assign(H.nterm, 0);
   for (int t = 0; t < H.nterm; ++t) {
     H.hr_base[t] = roff;
-    roff += H.HR.total();
+    roff += H.Rs[terms[t].ri].total();
   }
+  H.hkr_base = roff;
+  if (hkr) roff += H.HKR.total();
   H.hr_total = roff;
   H.dHR = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, roff));
   {
-    double* tmp = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, H.HR.total()));
-    upload_blocks(H.HR, tmp, s);
-    for (int t = 0; t < H.nterm; ++t) {
-      if (H.s2[t]->lb.permute) plan_permute_operand(*H.s2[t], 1, tmp, H.dHR + H.hr_base[t], s);
-      else QSX_HIP(hipMemcpyAsync(H.dHR + H.hr_base[t], tmp, sizeof(double) * H.HR.total(), hipMemcpyDeviceToDevice, s));
+    int64_t tmp_n = hkr ? H.HKR.total() : 0;
+    for (const TensorDesc& R : H.Rs) tmp_n = std::max(tmp_n, R.total());
+    double* tmp = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, tmp_n));
+    try {
+      for (int t = 0; t < H.nterm; ++t) {
+        const TensorDesc& R = H.Rs[terms[t].ri];
+        upload_blocks(R, tmp, s);
+        if (H.s2[t]->lb.permute) plan_permute_operand(*H.s2[t], 1, tmp, H.dHR + H.hr_base[t], s);
+        else QSX_HIP(hipMemcpyAsync(H.dHR + H.hr_base[t], tmp, sizeof(double) * R.total(), hipMemcpyDeviceToDevice, s));
+        QSX_HIP(hipStreamSynchronize(s));
+      }
+      if (hkr) {
+        upload_blocks(H.HKR, tmp, s);
+        if (H.locR->lb.permute) plan_permute_operand(*H.locR, 1, tmp, H.dHR + H.hkr_base, s);
+        else QSX_HIP(hipMemcpyAsync(H.dHR + H.hkr_base, tmp, sizeof(double) * H.HKR.total(), hipMemcpyDeviceToDevice, s));
+      }
+      QSX_HIP(hipStreamSynchronize(s));
+    } catch (...) {
+      dev_free(tmp);
+      throw;
     }
-    QSX_HIP(hipStreamSynchronize(s));
     dev_free(tmp);
   }
+  // dT: the intermediates T_t (rewritten by every apply), then HK_L in its
+  // layout (constant)
   int64_t toff = 0;
+  H.t_base.assign(H.nterm, 0);
   for (int t = 0; t < H.nterm; ++t) {
     H.t_base[t] = toff;
     toff += H.s1[t]->offsOut.back();
   }
   H.t_total = toff;
+  H.hkl_base = toff;
+  if (hkl) toff += H.HKL.total();
+  H.dt_size = toff;
   H.dT = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, toff));
+  if (hkl) {
+    if (H.locL->la.permute) {
+      double* tmp = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, H.HKL.total()));
+      try {
+        upload_blocks(H.HKL, tmp, s);
+        plan_permute_operand(*H.locL, 0, tmp, H.dT + H.hkl_base, s);
+      } catch (...) {
+        dev_free(tmp);
+        throw;
+      }
+      dev_free(tmp);
+    } else {
+      upload_blocks(H.HKL, H.dT + H.hkl_base, s);
+    }
+  }
   clk.lap("environment upload + permute");
+  // weights of psi's blocks: prod_j cgw_j^2 (|CData|_F = 1)
+  std::vector<double> wblk(H.psi.n, 1.0);
+  {
+    std::vector<int> dir(H.psi.r, 1);
+    for (int i = 0; i < H.psi.r && !H.psi.itags.empty(); ++i) dir[i] = (H.psi.itags[i] & 0x80) ? -1 : 1;
+    for (int i = 0; i < H.psi.n; ++i)
+      for (int j = 0; j < H.psi.nsym; ++j) {
+        if (H.psi.symtype[j] != SYM_SU2) continue;
+        double c;
+        if (H.psi.cgw_given) c = H.psi.cgw[(size_t)i * H.psi.nsym + j];
+        else {
+          std::vector<int> tj(H.psi.r);
+          for (int l = 0; l < H.psi.r; ++l) tj[l] = H.psi.label(i, l, j);
+          c = su2_norm_ext(tj, dir);
+        }
+        wblk[i] *= c * c;
+      }
+  }
+  H.welem.assign(H.N, 1.0);
+  for (int i = 0; i < H.psi.n; ++i)
+    for (int64_t e = H.psi.offs[i]; e < H.psi.offs[i + 1]; ++e) H.welem[e] = wblk[i];
+  H.wblk = wblk;
   // stage 1 work: every T_t block is its own target
   {
     std::vector<CtrPlan*> plans;
@@ -670,43 +797,24 @@ static void hpsi_create(HpsiPlan& H, const qsx_tensor* HL, const qsx_tensor* psi
     }
     std::vector<std::pair<int, int64_t>> src1;
     build_multi(plans, ab, bb, target, toffs, tM, tN, H.pairs1, H.targets1, nullptr, &src1);
-    // weights of psi's blocks: prod_j cgw_j^2 (|CData|_F = 1)
-    std::vector<double> wblk(H.psi.n, 1.0);
-    {
-      std::vector<int> dir(H.psi.r, 1);
-      for (int i = 0; i < H.psi.r && !H.psi.itags.empty(); ++i) dir[i] = (H.psi.itags[i] & 0x80) ? -1 : 1;
-      for (int i = 0; i < H.psi.n; ++i)
-        for (int j = 0; j < H.psi.nsym; ++j) {
-          if (H.psi.symtype[j] != SYM_SU2) continue;
-          double c;
-          if (H.psi.cgw_given) c = H.psi.cgw[(size_t)i * H.psi.nsym + j];
-          else {
-            std::vector<int> tj(H.psi.r);
-            for (int l = 0; l < H.psi.r; ++l) tj[l] = H.psi.label(i, l, j);
-            c = su2_norm_ext(tj, dir);
-          }
-          wblk[i] *= c * c;
-        }
-    }
-    H.welem.assign(H.N, 1.0);
-    for (int i = 0; i < H.psi.n; ++i)
-      for (int64_t e = H.psi.offs[i]; e < H.psi.offs[i + 1]; ++e) H.welem[e] = wblk[i];
-    H.wblk = wblk;
     H.pairs1s = H.pairs1;
     for (size_t p = 0; p < H.pairs1s.size(); ++p) {
       int ib = H.s1[src1[p].first]->pIb[src1[p].second];
       H.pairs1s[p].afac /= std::sqrt(wblk[ib]);
     }
   }
-  // stage 2 work: targets are psi's blocks (H*psi in psi's packed layout)
+  // stage 2 work: targets are psi's blocks (H*psi in psi's packed layout);
+  // per target the local terms' pairs first, then the terms' in term order
   {
     const int q = H.psi.qdim, r = H.psi.r;
     std::map<std::string, int> psi_idx;
     for (int i = 0; i < H.psi.n; ++i) psi_idx[rec_key(&H.psi.Q[(size_t)i * r * q], r * q)] = i;
     std::vector<CtrPlan*> plans;
     std::vector<int64_t> ab, bb, toffs(H.psi.n);
+    std::vector<int32_t> xf;
     std::vector<std::vector<int>> target;
     std::vector<int> tM(H.psi.n), tN(H.psi.n);
+    std::vector<int> kind;  // per plan: 0 term, 1 HK_L . psi (psi = B), 2 psi . HK_R (psi = A)
     for (int i = 0; i < H.psi.n; ++i) {
       toffs[i] = H.psi.offs[i];
       int64_t m = H.psi.dims[(size_t)i * r], n = 1;
@@ -714,15 +822,16 @@ static void hpsi_create(HpsiPlan& H, const qsx_tensor* HL, const qsx_tensor* psi
       tM[i] = (int)m;
       tN[i] = (int)n;
     }
-    for (int t = 0; t < H.nterm; ++t) {
-      CtrPlan& P = *H.s2[t];
+    auto add = [&](CtrPlan& P, int64_t a0, int64_t b0, int32_t flags, int kd) {
       if (P.rc != r) fail(QSX_ERR_STRUCT, "H*psi result has rank %d, psi rank %d", P.rc, r);
       for (int l = 0; l < r; ++l)
         if (P.tagsC.size() == (size_t)r && H.psi.itags.size() == (size_t)r && P.tagsC[l] != H.psi.itags[l])
           fail(QSX_ERR_ITAG, "H*psi result itags differ from psi's at leg %d", l + 1);
       plans.push_back(&P);
-      ab.push_back(H.t_base[t]);
-      bb.push_back(H.hr_base[t]);
+      ab.push_back(a0);
+      bb.push_back(b0);
+      xf.push_back(flags);
+      kind.push_back(kd);
       std::vector<int> tg(P.nC);
       for (int c = 0; c < P.nC; ++c) {
         auto it = psi_idx.find(rec_key(&P.Qout[(size_t)c * r * q], r * q));
@@ -730,11 +839,21 @@ static void hpsi_create(HpsiPlan& H, const qsx_tensor* HL, const qsx_tensor* psi
         tg[c] = it->second;
       }
       target.push_back(tg);
-    }
+    };
+    if (hkl) add(*H.locL, H.hkl_base, 0, kPairBinX, 1);
+    if (hkr) add(*H.locR, 0, H.hkr_base, kPairAinX, 2);
+    for (int t = 0; t < H.nterm; ++t) add(*H.s2[t], H.t_base[t], H.hr_base[t], 0, 0);
     std::vector<int> tgt2;
-    build_multi(plans, ab, bb, target, toffs, tM, tN, H.pairs2, H.targets2, &tgt2, nullptr);
+    std::vector<std::pair<int, int64_t>> src2;
+    build_multi(plans, ab, bb, target, toffs, tM, tN, H.pairs2, H.targets2, &tgt2, &src2, &xf);
     H.pairs2s = H.pairs2;
-    for (size_t p = 0; p < H.pairs2s.size(); ++p) H.pairs2s[p].afac *= std::sqrt(H.wblk[tgt2[p]]);
+    for (size_t p = 0; p < H.pairs2s.size(); ++p) {
+      double f = std::sqrt(H.wblk[tgt2[p]]);
+      const int pl = src2[p].first;
+      if (kind[pl] == 1) f /= std::sqrt(wblk[plans[pl]->pIb[src2[p].second]]);
+      if (kind[pl] == 2) f /= std::sqrt(wblk[plans[pl]->pIa[src2[p].second]]);
+      H.pairs2s[p].afac *= f;
+    }
   }
   clk.lap("pair lists");
   H.dpairs1 = upload_vec(H.pairs1, s);
@@ -747,15 +866,21 @@ static void hpsi_create(HpsiPlan& H, const qsx_tensor* HL, const qsx_tensor* psi
   QSX_HIP(hipStreamSynchronize(s));
   H.flops = 0;
   for (int t = 0; t < H.nterm; ++t) {
-    H.flops += H.s1[t]->flops + H.s2[t]->flops;
     H.stage_flops[0] += H.s1[t]->flops;
     H.stage_flops[1] += H.s2[t]->flops;
   }
+  if (H.locL) H.stage_flops[1] += H.locL->flops;
+  if (H.locR) H.stage_flops[1] += H.locR->flops;
+  H.flops = H.stage_flops[0] + H.stage_flops[1];
   // algorithmic bytes per stage: unique operand and result elements
-  H.stage_bytes[0] = 8.0 * ((double)H.HL.total() + (double)H.N + (double)H.t_total);
-  H.stage_bytes[1] = 8.0 * ((double)H.t_total + (double)H.HR.total() + (double)H.N);
-  // algorithmic bytes: unique environments + psi + H*psi (+ the T_t write and re-read)
-  H.bytes = 8.0 * ((double)H.HL.total() + (double)H.HR.total() + 2.0 * (double)H.N + 2.0 * (double)H.t_total);
+  int64_t nl = 0, nr = 0;
+  for (const TensorDesc& L : H.Ls) nl += L.total();
+  for (const TensorDesc& R : H.Rs) nr += R.total();
+  const int64_t nloc = (hkl ? H.HKL.total() : 0) + (hkr ? H.HKR.total() : 0);
+  H.stage_bytes[0] = 8.0 * ((double)nl + (double)H.N + (double)H.t_total);
+  H.stage_bytes[1] = 8.0 * ((double)H.t_total + (double)nr + (double)nloc + (double)H.N * (H.xpairs ? 2 : 1));
+  // algorithmic bytes: unique operators + psi + H*psi (+ the T_t write and re-read)
+  H.bytes = 8.0 * ((double)nl + (double)nr + (double)nloc + 2.0 * (double)H.N + 2.0 * (double)H.t_total);
   for (auto& e : H.ev) QSX_HIP(hipEventCreate(&e));
 }
 
@@ -975,7 +1100,42 @@ int qsx_hpsi_plan_create(const qsx_tensor* HL, const qsx_tensor* psi, const qsx_
     auto* p = new qsx_hpsi_plan;
     p->h = std::make_unique<HpsiPlan>();
     try {
-      hpsi_create(*p->h, HL, psi, HR, opts, opts ? opts->hconj : 0);
+      std::vector<HpsiTerm> terms{{0, 0, 0, 0}};
+      if (opts && opts->hconj) terms.push_back({0, 1, 0, 1});  // get_HPsi.m:55-56
+      hpsi_create(*p->h, {HL}, {HR}, terms, psi, nullptr, nullptr, opts);
+    } catch (...) {
+      delete p;
+      throw;
+    }
+    *plan = p;
+    return 0;
+  } catch (const Error& e) {
+    set_err2(err, errlen, e.msg);
+    return e.code;
+  }
+}
+
+int qsx_hpsi_plan_create_pseudo(const qsx_tensor* HKL, const qsx_tensor* HKR, int32_t nop, const qsx_tensor* OPL,
+                                const qsx_tensor* OPR, const int32_t* hconj, const qsx_tensor* psi,
+                                const qsx_opts* opts, qsx_hpsi_plan** plan, char* err, size_t errlen) {
+  try {
+    if (!plan) fail(QSX_ERR_ARG, "null plan pointer");
+    if (nop < 0 || (nop > 0 && (!OPL || !OPR))) fail(QSX_ERR_ARG, "invalid operator list");
+    if (opts && opts->fermA) fail(QSX_ERR_ARG, "pseudo-MPO H*psi: parity signs belong in the operators");
+    // get_HPsi.m:62-83: HK_L.psi + psi.HK_R + sum_j {OP_L(j),'*',psi}.OP_R(j)
+    // [+ {OP_L(j),psi}.OP_R(j)' when hconj(j)], in that order
+    std::vector<const qsx_tensor*> Ls, Rs;
+    std::vector<HpsiTerm> terms;
+    for (int j = 0; j < nop; ++j) {
+      Ls.push_back(OPL + j);
+      Rs.push_back(OPR + j);
+      terms.push_back({j, 1, j, 0});
+      if (hconj && hconj[j]) terms.push_back({j, 0, j, 1});
+    }
+    auto* p = new qsx_hpsi_plan;
+    p->h = std::make_unique<HpsiPlan>();
+    try {
+      hpsi_create(*p->h, Ls, Rs, terms, psi, HKL, HKR, opts);
     } catch (...) {
       delete p;
       throw;
@@ -1095,7 +1255,7 @@ int qsx_hpsi_stage_ms(qsx_hpsi_plan* plan, int32_t stage, int32_t reps, const do
       if (stage == 1)
         launch_worklist(H.work1, H.dq1, H.ditems1, H.dreds1, H.dpairs1, H.dHL, dpsi, H.dT, H.dscratch, s);
       else
-        launch_worklist(H.work2, H.dq2, H.ditems2, H.dreds2, H.dpairs2, H.dT, H.dHR, dhpsi, H.dscratch, s);
+        launch_worklist(H.work2, H.dq2, H.ditems2, H.dreds2, H.dpairs2, H.dT, H.dHR, dhpsi, H.dscratch, s, dpsi);
     };
     one();
     QSX_HIP(hipEventRecord(H.ev[0], s));

@@ -89,9 +89,13 @@ struct PairDesc {
   int32_t K;       // contracted dimension
   int32_t lda;     // leading dim of A as stored (M if A is MxK, K if KxM)
   int32_t ldb;     // leading dim of B as stored (K if B is KxN, N if NxK)
-  int32_t flags;   // bit0: A stored KxM; bit1: B stored NxK
+  int32_t flags;   // bit0: A stored KxM; bit1: B stored NxK; bit2 / bit3: the A / B block lies in
+                   // the launch's X buffer (the input vector) instead of its A / B buffer
   double afac;     // X-symbol x cgw x sign x output normalisation
 };
+
+constexpr int32_t kPairAinX = 4;
+constexpr int32_t kPairBinX = 8;
 
 // tile shapes of the grouped GEMM (TM x TN, 4 waves per workgroup)
 enum TileShape { SH_64x64 = 0, SH_32x32 = 1, SH_16x64 = 2, SH_64x16 = 3, SH_32x64 = 4, SH_64x32 = 5, SH_NUM = 6 };
@@ -211,20 +215,22 @@ struct WorkList {
 WorkList build_work(const std::vector<WorkTarget>& targets, const std::vector<PairDesc>& pairs, const WorkPolicy& pol);
 // persistent grouped GEMM over the queues, then (if any tile is split) the reduction
 void launch_work(const WorkQueue* dqueues, int nqueues, const WorkItem* ditems, const ReduceItem* dreds, int nreds,
-                 const PairDesc* dpairs, const double* A, const double* B, double* C, double* scratch, hipStream_t s);
+                 const PairDesc* dpairs, const double* A, const double* B, const double* X, double* C, double* scratch,
+                 hipStream_t s);
 // one wave per item, then (if any tile is split) the reduction
 void launch_wave(const WorkItem* ditems, int nitems, int maxf, int wg_waves, const ReduceItem* dreds, int nreds,
-                 const PairDesc* dpairs, const double* A, const double* B, double* C, double* scratch,
-                 hipStream_t s);
-// the launch that matches the work list's engine
+                 const PairDesc* dpairs, const double* A, const double* B, const double* X, double* C,
+                 double* scratch, hipStream_t s);
+// the launch that matches the work list's engine; X is the buffer of pairs
+// flagged kPairAinX / kPairBinX (may be null when no pair is)
 inline void launch_worklist(const WorkList& W, const WorkQueue* dqueues, const WorkItem* ditems,
                             const ReduceItem* dreds, const PairDesc* dpairs, const double* A, const double* B,
-                            double* C, double* scratch, hipStream_t s) {
+                            double* C, double* scratch, hipStream_t s, const double* X = nullptr) {
   if (W.engine == ENG_WAVE)
-    launch_wave(ditems, (int)W.items.size(), W.maxf, W.wg_waves, dreds, (int)W.reds.size(), dpairs, A, B, C, scratch,
-                s);
+    launch_wave(ditems, (int)W.items.size(), W.maxf, W.wg_waves, dreds, (int)W.reds.size(), dpairs, A, B, X, C,
+                scratch, s);
   else
-    launch_work(dqueues, (int)W.queues.size(), ditems, dreds, (int)W.reds.size(), dpairs, A, B, C, scratch, s);
+    launch_work(dqueues, (int)W.queues.size(), ditems, dreds, (int)W.reds.size(), dpairs, A, B, X, C, scratch, s);
 }
 void launch_copy(double* d, const double* src, int64_t n, hipStream_t s);
 // device scratch of the no-allocation entry points (gemm.hip): a file-scope

@@ -73,7 +73,8 @@ struct Cursor {
 
 template <int FM, int FN>
 __device__ __forceinline__ void set_pair(Cursor<FM, FN>& c, const PairDesc& pd, const PairDesc* __restrict__ pairs,
-                                         int p, int k, const WorkItem& w, const double* A, const double* B) {
+                                         int p, int k, const WorkItem& w, const double* A, const double* B,
+                                         const double* X) {
   const int lane = threadIdx.x & 63, lr = lane & 15, lq = lane >> 4;
   c.p = p;
   c.k = k;
@@ -84,8 +85,12 @@ __device__ __forceinline__ void set_pair(Cursor<FM, FN>& c, const PairDesc& pd, 
   const int64_t skb = (pd.flags & 2) ? pd.ldb : 1, scb = (pd.flags & 2) ? 1 : pd.ldb;
   c.sa4 = 4 * ska * 8;
   c.sb4 = 4 * skb * 8;
-  c.pa = reinterpret_cast<const char*>(A + pd.a_off + (int64_t)k * ska);
-  c.pb = reinterpret_cast<const char*>(B + pd.b_off + (int64_t)k * skb);
+  // flags bit 2 / 3: the A / B operand block lies in X (the input vector)
+  // instead of the launch's A / B buffer
+  const double* Ab = (pd.flags & kPairAinX) ? X : A;
+  const double* Bb = (pd.flags & kPairBinX) ? X : B;
+  c.pa = reinterpret_cast<const char*>(Ab + pd.a_off + (int64_t)k * ska);
+  c.pb = reinterpret_cast<const char*>(Bb + pd.b_off + (int64_t)k * skb);
 #pragma unroll
   for (int i = 0; i < FM; ++i) {
     const int r = w.m0 + 16 * i + lr;
@@ -102,7 +107,7 @@ __device__ __forceinline__ void set_pair(Cursor<FM, FN>& c, const PairDesc& pd, 
 // move the cursor to the next chunk of the item; false at its end
 template <int FM, int FN, int KS>
 __device__ __forceinline__ bool advance(Cursor<FM, FN>& c, const PairDesc* __restrict__ pairs, const WorkItem& w,
-                                        const double* A, const double* B) {
+                                        const double* A, const double* B, const double* X) {
   if (c.k + 4 * KS < c.kend) {
     c.k += 4 * KS;
     c.pa += KS * c.sa4;
@@ -111,7 +116,7 @@ __device__ __forceinline__ bool advance(Cursor<FM, FN>& c, const PairDesc* __res
   }
   if (c.p >= w.p1) return false;
   const PairDesc pd = c.nx;
-  set_pair<FM, FN>(c, pd, pairs, c.p + 1, 0, w, A, B);
+  set_pair<FM, FN>(c, pd, pairs, c.p + 1, 0, w, A, B, X);
   return true;
 }
 
@@ -212,7 +217,7 @@ struct Pipe {
 template <int FM, int FN, int D>
 __device__ __forceinline__ void run_k(const WorkItem& w, const PairDesc* __restrict__ pairs,
                                       const double* __restrict__ A, const double* __restrict__ B,
-                                      d4 (&acc)[FM][FN]) {
+                                      const double* __restrict__ X, d4 (&acc)[FM][FN]) {
   constexpr int KS = Pipe<FM, FN>::KS;
 #pragma unroll
   for (int i = 0; i < FM; ++i)
@@ -222,7 +227,7 @@ __device__ __forceinline__ void run_k(const WorkItem& w, const PairDesc* __restr
   Cursor<FM, FN> c;
   {
     const PairDesc pd = pairs[w.p0];
-    set_pair<FM, FN>(c, pd, pairs, w.p0, w.k0, w, A, B);
+    set_pair<FM, FN>(c, pd, pairs, w.p0, w.k0, w, A, B, X);
   }
   bool have = true;
   double ra[D][KS][FM], rb[D][KS][FN];
@@ -233,7 +238,7 @@ __device__ __forceinline__ void run_k(const WorkItem& w, const PairDesc* __restr
     kv[d] = have ? min(4 * KS, c.kend - c.k) : 0;
     af[d] = c.af;
     issue<FM, FN, KS>(c, kv[d], ra[d], rb[d]);
-    if (have) have = advance<FM, FN, KS>(c, pairs, w, A, B);
+    if (have) have = advance<FM, FN, KS>(c, pairs, w, A, B, X);
   }
   for (;;) {
 #pragma unroll
@@ -242,7 +247,7 @@ __device__ __forceinline__ void run_k(const WorkItem& w, const PairDesc* __restr
       kv[nslot] = have ? min(4 * KS, c.kend - c.k) : 0;
       af[nslot] = c.af;
       issue<FM, FN, KS>(c, kv[nslot], ra[nslot], rb[nslot]);
-      if (have) have = advance<FM, FN, KS>(c, pairs, w, A, B);
+      if (have) have = advance<FM, FN, KS>(c, pairs, w, A, B, X);
       if (kv[d] <= 0) return;
       mma<FM, FN, KS>(kv[d], af[d], ra[d], rb[d], acc);
     }
@@ -264,15 +269,16 @@ struct GroupLds {
 template <int FM, int FN>
 __device__ __forceinline__ void run_item(const WorkItem& w, const PairDesc* __restrict__ pairs,
                                          const double* __restrict__ A, const double* __restrict__ B,
-                                         double* __restrict__ C, double* __restrict__ scratch, GroupLds& lds) {
+                                         const double* __restrict__ X, double* __restrict__ C,
+                                         double* __restrict__ scratch, GroupLds& lds) {
   d4 acc[FM][FN];
   // an item of at most two chunks (flag set by the work-list builder) runs a
   // two-deep ring: a deeper ring would issue empty chunks' loads (out of
   // range, no memory access, but address-unit work) for nothing
   if (w.pad & kShortItem)
-    run_k<FM, FN, 2>(w, pairs, A, B, acc);
+    run_k<FM, FN, 2>(w, pairs, A, B, X, acc);
   else
-    run_k<FM, FN, Pipe<FM, FN>::D>(w, pairs, A, B, acc);
+    run_k<FM, FN, Pipe<FM, FN>::D>(w, pairs, A, B, X, acc);
   if constexpr (FM * FN <= 4) {
     if (w.group) {  // uniform over the workgroup: all its waves share this tile
       // the partials are summed in wave order 0, 1, ..., G-1: one fixed order
@@ -310,8 +316,8 @@ __device__ __forceinline__ void run_item(const WorkItem& w, const PairDesc* __re
 template <int MAXF>
 __device__ __forceinline__ void wave_dispatch(const WorkItem* __restrict__ items, int nitems,
                                               const PairDesc* __restrict__ pairs, const double* __restrict__ A,
-                                              const double* __restrict__ B, double* __restrict__ C,
-                                              double* __restrict__ scratch) {
+                                              const double* __restrict__ B, const double* __restrict__ X,
+                                              double* __restrict__ C, double* __restrict__ scratch) {
   __shared__ GroupLds lds;
   const int wid = uniform(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
   if (wid >= nitems) return;
@@ -320,7 +326,7 @@ __device__ __forceinline__ void wave_dispatch(const WorkItem* __restrict__ items
   switch (w.shape - SH_WAVE) {
 #define QSX_WAVE_CASE(fm, fn) \
   case ((fm) - 1) * 4 + (fn) - 1: \
-    if constexpr ((fm) <= MAXF && (fn) <= MAXF) run_item<fm, fn>(w, pairs, A, B, C, scratch, lds); \
+    if constexpr ((fm) <= MAXF && (fn) <= MAXF) run_item<fm, fn>(w, pairs, A, B, X, C, scratch, lds); \
     break;
     QSX_WAVE_CASE(1, 1) QSX_WAVE_CASE(1, 2) QSX_WAVE_CASE(1, 3) QSX_WAVE_CASE(1, 4)
     QSX_WAVE_CASE(2, 1) QSX_WAVE_CASE(2, 2) QSX_WAVE_CASE(2, 3) QSX_WAVE_CASE(2, 4)
@@ -333,16 +339,16 @@ __device__ __forceinline__ void wave_dispatch(const WorkItem* __restrict__ items
 
 __global__ __launch_bounds__(64 * kMaxWaves) __attribute__((amdgpu_waves_per_eu(4)))
 void k_wave_s(const WorkItem* __restrict__ items, int nitems, const PairDesc* __restrict__ pairs,
-              const double* __restrict__ A, const double* __restrict__ B, double* __restrict__ C,
-              double* __restrict__ scratch) {
-  wave_dispatch<2>(items, nitems, pairs, A, B, C, scratch);
+              const double* __restrict__ A, const double* __restrict__ B, const double* __restrict__ X,
+              double* __restrict__ C, double* __restrict__ scratch) {
+  wave_dispatch<2>(items, nitems, pairs, A, B, X, C, scratch);
 }
 
 __global__ __launch_bounds__(NT) void k_wave_l(const WorkItem* __restrict__ items, int nitems,
                                                const PairDesc* __restrict__ pairs, const double* __restrict__ A,
-                                               const double* __restrict__ B, double* __restrict__ C,
-                                               double* __restrict__ scratch) {
-  wave_dispatch<4>(items, nitems, pairs, A, B, C, scratch);
+                                               const double* __restrict__ B, const double* __restrict__ X,
+                                               double* __restrict__ C, double* __restrict__ scratch) {
+  wave_dispatch<4>(items, nitems, pairs, A, B, X, C, scratch);
 }
 
 // sum the chunk partials of split tiles in chunk order (fixed order:
@@ -373,17 +379,17 @@ __global__ __launch_bounds__(NT) void k_wave_reduce(const ReduceItem* __restrict
 }  // namespace
 
 void launch_wave(const WorkItem* ditems, int nitems, int maxf, int wg_waves, const ReduceItem* dreds, int nreds,
-                 const PairDesc* dpairs, const double* A, const double* B, double* C, double* scratch,
-                 hipStream_t s) {
+                 const PairDesc* dpairs, const double* A, const double* B, const double* X, double* C,
+                 double* scratch, hipStream_t s) {
   if (nitems > 0) {
     if (maxf <= 2) {
       if (wg_waves < 1 || wg_waves > kMaxWaves)
         fail(QSX_ERR_ARG, "wave engine: %d waves per workgroup", wg_waves);
       const int nb = (nitems + wg_waves - 1) / wg_waves;
-      hipLaunchKernelGGL(k_wave_s, dim3(nb), dim3(64 * wg_waves), 0, s, ditems, nitems, dpairs, A, B, C, scratch);
+      hipLaunchKernelGGL(k_wave_s, dim3(nb), dim3(64 * wg_waves), 0, s, ditems, nitems, dpairs, A, B, X, C, scratch);
     } else {
       const int nb = (nitems + NT / 64 - 1) / (NT / 64);
-      hipLaunchKernelGGL(k_wave_l, dim3(nb), dim3(NT), 0, s, ditems, nitems, dpairs, A, B, C, scratch);
+      hipLaunchKernelGGL(k_wave_l, dim3(nb), dim3(NT), 0, s, ditems, nitems, dpairs, A, B, X, C, scratch);
     }
     QSX_HIP(hipGetLastError());
   }

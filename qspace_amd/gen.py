@@ -18,6 +18,8 @@ Workloads (sizes at full scale):
                HK(E,E',w); MPO bond w = {2S=0: dim 2, 2S=2: dim 1} (3 multiplets);
                enlarged-bond sectors 2S = 0,2,4,... with lognormal(mu,sigma)
                dims clipped to [1,dmax], drawn until sum(d) >= D*.
+  heis_pseudo_hpsi  the same chain in the pseudo-MPO form of get_HPsi.m:62-83:
+               block Hamiltonians HK_L, HK_R and spin-1 operator pairs.
   hubbard_hpsi SU(2)xU(1) ("A,SU2") Hubbard two-site H*psi, MPO bond
                {(0,0): 2, (+1,1): 2, (-1,1): 2} (6 multiplets), sectors (Q,2S)
                with Q in [-12,12], Q+2S even, ordered by |Q|+2S, lognormal dims.
@@ -132,6 +134,41 @@ def heis_hpsi(seed=1003, Dstar=2000, mu=3.0, sigma=0.8, dmax=256):
     nrm = psi.norm()
     psi = psi.with_packed(psi.packed() / nrm)
     return HL, psi, HR
+
+
+def heis_pseudo_hpsi(seed=1008, Dstar=2000, mu=3.0, sigma=0.8, dmax=256, nop=1):
+    """Operands of the pseudo-MPO two-site H*psi of the SU(2) spin-1/2
+    Heisenberg chain (get_HPsi.m:62-83): psi(E1l,E2r) as in heis_hpsi, the
+    block Hamiltonians HK_L(E1l,E1l*) and HK_R(E2r,E2r*) (symmetric blocks),
+    and nop spin-1 operator pairs OP_L(E1l,E1l*,op_j), OP_R(E2r,E2r*,op_j)
+    (operator leg 2S = 2, incoming on both, matched by its itag).
+    Returns (HKL, psi, HKR, [OP_L], [OP_R])."""
+    rng = _rng(seed)
+    labels = list(range(0, 4000, 2))
+    E1 = lognormal_sectors(rng, labels, Dstar, mu, sigma, dmax)
+    E2 = [(q, int(np.clip(np.rint(np.exp(mu + sigma * rng.standard_normal())), 1, dmax))) for q, _ in E1]
+
+    def block_h(E, tags):
+        data = []
+        for _, d in E:
+            x = rng.standard_normal((d, d))
+            data.append(np.asfortranarray(0.5 * (x + x.T)))
+        return QSpace([np.array([[q] for q, _ in E]), np.array([[q] for q, _ in E])], data, "SU2", tags)
+
+    HKL = block_h(E1, ["E1l", "E1l*"])
+    HKR = block_h(E2, ["E2r", "E2r*"])
+    rule = lambda a, b, w: _tri(a, b, w)
+    OPL, OPR = [], []
+    for j in range(nop):
+        tag = f"op{j + 1}"
+        QL, dL = _env_blocks(rng, E1, E1, [(2, 1)], rule)
+        OPL.append(QSpace([np.array(q) for q in QL], dL, "SU2", ["E1l", "E1l*", tag]))
+        QR, dR = _env_blocks(rng, E2, E2, [(2, 1)], rule)
+        OPR.append(QSpace([np.array(q) for q in QR], dR, "SU2", ["E2r", "E2r*", tag]))
+    psi = QSpace([np.array([[q] for q, _ in E1]), np.array([[q] for q, _ in E2])],
+                 [_normal_block(rng, (d1, d2)) for (_, d1), (_, d2) in zip(E1, E2)], "SU2", ["E1l", "E2r"])
+    psi = psi.with_packed(psi.packed() / psi.norm())
+    return HKL, psi, HKR, OPL, OPR
 
 
 def hubbard_hpsi(seed=1005, Dstar=4000, mu=3.2, sigma=0.9, dmax=384, qmax=12):

@@ -30,3 +30,29 @@ def oracle_ortho(P, K, **kw):
     kw = dict(kw)
     kw["Nkeep"], kw["stol"] = Q.ortho_args(P, K, kw.get("Nkeep"), kw.get("stol"))
     return O.ortho(P, K, **kw)
+
+
+def pseudo_hpsi_oracle(HKL, psi, HKR, OPL, OPR, hconj):
+    """the oracle's pseudo-MPO H*psi (get_HPsi.m:62-83) as a psi-shaped
+    QSpace: HK_L.psi + psi.HK_R + sum_j {OP_L(j),'*',psi}.OP_R(j)
+    [+ {OP_L(j),psi}.OP_R(j)' where hconj[j]], blocks summed by label"""
+    import numpy as np
+    import oracle as O
+    terms = []
+    if HKL is not None:
+        terms.append(O.contract(HKL, [1], psi, [0], skip_zero=0))
+    if HKR is not None:
+        terms.append(O.contract(psi, [1], HKR, [1], skip_zero=0))
+    for L, R, h in zip(OPL, OPR, hconj):
+        T1 = O.contract(L, [0], psi, [0], conjA=1, skip_zero=0)
+        terms.append(O.contract(T1, [1, 2], R, [2, 1], skip_zero=0))
+        if h:
+            T2 = O.contract(L, [1], psi, [0], skip_zero=0)
+            terms.append(O.contract(T2, [1, 2], R, [2, 0], conjB=1, skip_zero=0))
+    key = {tuple(r): i for i, r in enumerate(psi.QIDX)}
+    out = [np.zeros_like(d) for d in psi.data]
+    for Ht in terms:
+        assert Ht.itags == psi.itags, (Ht.itags, psi.itags)
+        for r, d in zip(Ht.QIDX, Ht.data):
+            out[key[tuple(r)]] += d
+    return psi.with_packed(np.concatenate([o.ravel(order="F") for o in out]))

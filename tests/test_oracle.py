@@ -113,6 +113,33 @@ def test_fullspace_su2_hpsi_terms():
     assert check_contraction(T2, [1, 2], HR, [2, 0], H2) < 1e-13
 
 
+def test_fullspace_pseudo_mpo_terms():
+    """every contraction of the pseudo-MPO H*psi (get_HPsi.m:62-83: HK_L.psi,
+    psi.HK_R, {OP_L,'*',psi}.OP_R and {OP_L,psi}.OP_R') equals its full
+    M-space tensordot, and their block-wise sum equals the sum of the full
+    tensors"""
+    from helpers import pseudo_hpsi_oracle
+    from fullspace import _layout, _sectors, expand
+    HKL, psi, HKR, OPL, OPR = gen.heis_pseudo_hpsi(seed=34, Dstar=14, mu=1.0, sigma=0.4, dmax=4, nop=1)
+    L, R = OPL[0], OPR[0]
+    assert check_contraction(HKL, [1], psi, [0], O.contract(HKL, [1], psi, [0], skip_zero=0)) < 1e-13
+    assert check_contraction(psi, [1], HKR, [1], O.contract(psi, [1], HKR, [1], skip_zero=0)) < 1e-13
+    T1 = O.contract(L, [0], psi, [0], conjA=1, skip_zero=0)
+    assert check_contraction(L, [0], psi, [0], T1) < 1e-13
+    X1 = O.contract(T1, [1, 2], R, [2, 1], skip_zero=0)
+    assert check_contraction(T1, [1, 2], R, [2, 1], X1) < 1e-13
+    T2 = O.contract(L, [1], psi, [0], skip_zero=0)
+    assert check_contraction(L, [1], psi, [0], T2) < 1e-13
+    X2 = O.contract(T2, [1, 2], R, [2, 0], conjB=1, skip_zero=0)
+    assert check_contraction(T2, [1, 2], R, [2, 0], X2) < 1e-13
+    # the sum, in the full space of psi's legs
+    H = pseudo_hpsi_oracle(HKL, psi, HKR, OPL, OPR, [1])
+    lay = [_layout(_sectors(psi, k, 0)) for k in range(2)]
+    full = sum(expand(T, lay) for T in (O.contract(HKL, [1], psi, [0], skip_zero=0),
+                                        O.contract(psi, [1], HKR, [1], skip_zero=0), X1, X2))
+    assert np.abs(expand(H, lay) - full).max() < 1e-13 * max(1.0, np.abs(full).max())
+
+
 def test_fullspace_su2_u1_hubbard_terms():
     HL, psi, HR = gen.hubbard_hpsi(seed=33, Dstar=14, mu=1.0, sigma=0.4, dmax=3, qmax=2)
     T = O.contract(HL, [1], psi, [0], skip_zero=0)
