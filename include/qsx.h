@@ -10,6 +10,8 @@
  *   qsx_contract        <- contractQS usage #1, Source/contractQS.cc:505-632
  *                          (contract_plain + CONTRACT_QS), which calls
  *                          QSpace::contract, Source/QSpace.cc:4296-4420
+ *   qsx_contract_cell   <- contractQS usage #2 (cell form, itag matching),
+ *                          contractQS_itags, Source/contractQS.cc:634-808
  *   qsx_ctr_plan_*      <- the same QSpace::contract split into its structure
  *                          half (contract_getIdxSet, Source/QSpace.cc:4131-4294)
  *                          and its data half (contractDATA_group,
@@ -151,6 +153,44 @@ int qsx_contract(const qsx_tensor* A, const int32_t* ia, int32_t na, int32_t con
                  const int32_t* perm, int32_t nperm, const qsx_opts* opts,
                  qsx_tensor_out** C, char* err, size_t errlen);
 void qsx_tensor_free(qsx_tensor_out* t);
+
+/* contractQS usage #2, the cell form with itag auto-contraction
+   (contractQS_itags, Source/contractQS.cc:634-808; the options of icFlags,
+   :159-293; the matching of iTags::getCtr, Source/wbindex.cc:1512-1590):
+   contractQS({A, opts.., {B, opts.., C}, opts..}, D, opts.. [,perm]).
+   An entry is a QSpace (T) or a nested cell (sub, nsub >= 2 entries), with
+   the options that follow it in the reference's argument list: option
+   strings ("*" conj, "23" / "!2" the 1-based legs that may / may not take
+   part in the match, combined as "!2*") and at most one numeric index
+   option (idx, 1-based).  Every level contracts right to left over all
+   legs whose itags match (equal names with opposite conj flags, or equal
+   flags when one side carries "*"; with no match, the match is retried
+   with conj(A), the reference's missing-conj rule).  One top-level entry:
+   its conj flag and index option give a conj / permutation of it.  perm
+   (0-based, may be shorter than the rank) permutes the result; the result
+   takes SkipZeroData unless opts->skip_zero == 0.  Real operands stay on
+   the device for the whole fold (each distinct leaf uploaded once);
+   complex leaves are contracted step by step through the complex path. */
+typedef struct qsx_cell {
+  const qsx_tensor* T;           /* a QSpace, or NULL for a nested cell */
+  int32_t nsub;                  /* nested cell: its entries */
+  const struct qsx_cell* sub;
+  int32_t nopt;                  /* option strings following the entry */
+  const char* const* opt;
+  int32_t nidx;                  /* numeric index option (1-based legs), 0 if none */
+  const int32_t* idx;
+} qsx_cell;
+int qsx_contract_cell(const qsx_cell* items, int32_t nitems, const int32_t* perm, int32_t nperm,
+                      const qsx_opts* opts, qsx_tensor_out** C, char* err, size_t errlen);
+/* one step of that fold on the host, structure only: the options of A and
+   B (as in qsx_cell) and the itag matching give the contracted legs
+   ia / ib (0-based, nctr of them; arrays of at least min(rank) entries)
+   and the conj flags the contraction takes (conjA after the missing-conj
+   rule); needs no device */
+int qsx_itag_match(const qsx_tensor* A, int32_t nopta, const char* const* opta, const int32_t* idxa, int32_t nidxa,
+                   const qsx_tensor* B, int32_t noptb, const char* const* optb, const int32_t* idxb, int32_t nidxb,
+                   int32_t* ia, int32_t* ib, int32_t* nctr, int32_t* conjA, int32_t* conjB, char* err,
+                   size_t errlen);
 
 /* whole-QSpace block operations (device kernels over all blocks) -------- */
 /* permuteQS(A, perm [,'conj']) <- Source/permuteQS.cc:25-87 (QSpace::permute,

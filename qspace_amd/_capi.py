@@ -8,7 +8,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 
-from .qspace import QsxOpts, QsxTensor, QsxTensorOut
+from .qspace import QsxCell, QsxOpts, QsxTensor, QsxTensorOut
 
 
 class QsxOrthoInfo(C.Structure):
@@ -59,6 +59,9 @@ def lib():
         "qsx_contract": ([P(QsxTensor), P(i32), i32, i32, P(QsxTensor), P(i32), i32, i32, P(i32), i32, P(QsxOpts),
                           P(P(QsxTensorOut)), cs, sz], C.c_int),
         "qsx_tensor_free": ([P(QsxTensorOut)], None),
+        "qsx_contract_cell": ([P(QsxCell), i32, P(i32), i32, P(QsxOpts), P(P(QsxTensorOut)), cs, sz], C.c_int),
+        "qsx_itag_match": ([P(QsxTensor), i32, P(C.c_char_p), P(i32), i32, P(QsxTensor), i32, P(C.c_char_p), P(i32),
+                            i32, P(i32), P(i32), P(i32), P(i32), P(i32), cs, sz], C.c_int),
         "qsx_permute": ([P(QsxTensor), P(i32), i32, i32, P(P(QsxTensorOut)), cs, sz], C.c_int),
         "qsx_plus": ([P(QsxTensor), P(QsxTensor), f64, P(QsxOpts), P(P(QsxTensorOut)), cs, sz], C.c_int),
         "qsx_ortho": ([P(QsxTensor), i32, i32, i64, f64, cs, P(P(QsxTensorOut)), P(P(QsxTensorOut)),
@@ -108,7 +111,7 @@ def lib():
 def exported_symbols() -> list[str]:
     """names declared in include/qsx.h that the loaded library must export."""
     return ["qsx_version", "qsx_device_init", "qsx_malloc_device", "qsx_free_device", "qsx_memcpy_h2d",
-            "qsx_memcpy_d2h", "qsx_memset_device", "qsx_synchronize", "qsx_copy_device", "qsx_contract", "qsx_tensor_free",
+            "qsx_memcpy_d2h", "qsx_memset_device", "qsx_synchronize", "qsx_copy_device", "qsx_contract", "qsx_tensor_free", "qsx_contract_cell", "qsx_itag_match",
             "qsx_permute", "qsx_plus", "qsx_ortho", "qsx_ortho_info_free",
             "qsx_parse_ctridx", "qsx_packed_offsets", "qsx_ctr_plan_create", "qsx_ctr_plan_output",
             "qsx_ctr_plan_pairs", "qsx_ctr_plan_idxset", "qsx_ctr_execute", "qsx_ctr_plan_flops", "qsx_ctr_plan_set_policy", "qsx_ctr_plan_destroy", "qsx_block_zero_flags", "qsx_block_zero_flags_noalloc",

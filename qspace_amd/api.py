@@ -14,9 +14,9 @@ import ctypes as C
 import numpy as np
 
 from . import _capi
-from .qspace import QSpace, QsxOpts, QsxTensor, QsxTensorOut, from_out, parse_ctridx, to_c
+from .qspace import QSpace, QsxCell, QsxOpts, QsxTensor, QsxTensorOut, from_out, parse_ctridx, to_c
 
-__all__ = ["contractQS", "permuteQS", "plusQS", "orthoQS", "ortho_args", "DeviceBuffer", "CtrPlan", "HPsi", "su2_cg", "su2_x3", "make_opts"]
+__all__ = ["contractQS", "permuteQS", "plusQS", "orthoQS", "ortho_args", "itag_match", "DeviceBuffer", "CtrPlan", "HPsi", "su2_cg", "su2_x3", "make_opts"]
 
 
 def make_opts(mode="batched", nslices=0, skip_zero=1, hconj=0, fermA=None, fermB=None, verbose=1):
@@ -64,42 +64,6 @@ def _policy_dict(v):
 def _ivec(v):
     a = np.ascontiguousarray(np.asarray(v, dtype=np.int32).reshape(-1))
     return a if a.size else np.zeros(1, np.int32)
-
-
-def _itag_match(A: QSpace, B: QSpace, conjA: int, conjB: int, skipA=(), skipB=()):
-    """iTags::getCtr (Source/wbindex.cc:1512-1590): every pair of legs with
-    equal tag names (two empty names count as equal) and opposite conj bits
-    (equal bits if exactly one side carries a conj flag) is contracted; a
-    leg matching twice is an error; if nothing matches, the match is retried
-    with the other conj relation and conj(A) is applied (the reference's
-    "check missing conj flag" warning).  Legs in skipA / skipB are flagged
-    (itag_::SetFlags) and match nothing."""
-    if len(A.itags) != A.rank or len(B.itags) != B.rank:
-        raise _capi.QsxError(2, "got empty or invalid set of itags")
-    cflag = conjA ^ conjB
-    for it in range(2):
-        ma, mb, err = {}, {}, 0
-        for i, ta in enumerate(A.itags):
-            if i in skipA:
-                continue
-            na, oa = ta.rstrip("*"), ta.endswith("*")
-            for j, tb in enumerate(B.itags):
-                if j in skipB:
-                    continue
-                nb, ob = tb.rstrip("*"), tb.endswith("*")
-                if na == nb and ((oa == ob) == bool(cflag)):
-                    if i in ma or j in mb:
-                        err += 1
-                    else:
-                        ma[i] = j
-                        mb[j] = i
-        if err:
-            raise _capi.QsxError(2, "got non-unique itags")
-        if ma:
-            ia = sorted(ma)
-            return ia, [ma[i] for i in ia], (conjA if it == 0 else 1 - conjA)
-        cflag = 1 - cflag
-    raise _capi.QsxError(2, "empty set of matching itags")
 
 
 def _contract_pair(A, ia, conjA, B, ib, conjB, perm=None, mode="batched", **kw) -> QSpace:
@@ -154,22 +118,98 @@ def contractQS(*args, mode="batched", **kw) -> QSpace:
         if sorted(v) == list(range(1, len(v) + 1)) and (len(v) != 1 or len(args) - 1 < 2):
             perm = [x - 1 for x in v]
             args.pop()
-    # the contractions run on device-resident operands (one upload per leaf,
-    # one download of the result); a single top-level operand is a conj /
-    # permutation of it
-    tops = [a for a in args if isinstance(a, QSpace) or _is_cell(a)]
-    if len(tops) >= 2:
-        if _any_complex(args):
-            C = _cell_host(args, mode, kw)
+    # the fold (itag matching, options, device-resident operands) is the
+    # library's qsx_contract_cell
+    L = _capi.lib()
+    keep = []
+    items, n = _cell_structs(args, {}, keep)
+    opts, okeep = make_opts(mode=mode, **kw)
+    p = _ivec(perm if perm is not None else [])
+    out = C.POINTER(QsxTensorOut)()
+    err = C.create_string_buffer(1024)
+    rc = L.qsx_contract_cell(items, n, p.ctypes.data_as(C.POINTER(C.c_int32)), 0 if perm is None else len(perm),
+                             C.byref(opts), C.byref(out), err, 1024)
+    _capi.check(rc, err)
+    return _out_to_qspace(out)
+
+
+def _cell_structs(args, tensors, keep):
+    """the argument list of the cell form as qsx_cell entries: a QSpace or a
+    nested cell starts an entry, the strings and numeric vectors after it are
+    its options; one QsxTensor per distinct QSpace (the library uploads each
+    leaf once)"""
+    entries = []
+    for a in args:
+        if isinstance(a, QSpace):
+            if id(a) not in tensors:
+                ct, k = to_c(a)
+                tensors[id(a)] = C.pointer(ct)
+                keep += [k, ct, a]
+            entries.append({"T": tensors[id(a)], "sub": None, "opt": [], "idx": None})
+        elif _is_cell(a):
+            sub, n = _cell_structs(list(a), tensors, keep)
+            entries.append({"T": None, "sub": (sub, n), "opt": [], "idx": None})
         else:
-            C = _download_result(_cell_dev(args, 0, mode, kw, {}, []))
-    else:
-        C = _single(args, mode, kw)
-    if perm is not None:
-        if len(perm) > C.rank:
-            raise _capi.QsxError(1, f"invalid input permutation (len={len(perm)}/{C.rank})")
-        C = permuteQS(C, perm)
-    return C
+            if not entries:
+                raise _capi.QsxError(1, "invalid QSpace cell-structure (option before any QSpace)")
+            if isinstance(a, str):
+                entries[-1]["opt"].append(a)
+            elif _is_numvec(a):
+                if entries[-1]["idx"] is not None:
+                    raise _capi.QsxError(1, "cell-contraction option: ctr-idx already set")
+                entries[-1]["idx"] = [int(x) for x in np.atleast_1d(a)]
+            else:
+                raise _capi.QsxError(1, f"unexpected cell-contraction input {type(a).__name__}")
+    arr = (QsxCell * max(1, len(entries)))()
+    for e, c in zip(entries, arr):
+        if e["T"] is not None:
+            c.T = e["T"]
+        else:
+            c.sub, c.nsub = e["sub"]
+        _cell_options(c, e["opt"], e["idx"], keep)
+    keep.append(arr)
+    return arr, len(entries)
+
+
+def _cell_options(c, opt, idx, keep):
+    if opt:
+        b = [o.encode() for o in opt]
+        a = (C.c_char_p * len(b))(*b)
+        keep += [b, a]
+        c.opt, c.nopt = a, len(b)
+    if idx is not None:
+        v = _ivec(idx)
+        keep.append(v)
+        c.idx, c.nidx = v.ctypes.data_as(C.POINTER(C.c_int32)), len(idx)
+
+
+def itag_match(A: QSpace, optA, B: QSpace, optB):
+    """one step of the cell form's fold, structure only (qsx_itag_match, no
+    device): the options of A and B (lists of option strings / one numeric
+    index vector, as in the cell form) and the itag matching give (ia, ib,
+    conjA, conjB), ia / ib 0-based"""
+    L = _capi.lib()
+    keep = []
+    ca, k1 = to_c(A, with_data=False)
+    cb, k2 = to_c(B, with_data=False)
+    cells = (QsxCell * 2)()
+    for c, o in zip(cells, (optA, optB)):
+        strs = [x for x in o if isinstance(x, str)]
+        nums = [x for x in o if not isinstance(x, str)]
+        if len(nums) > 1:
+            raise _capi.QsxError(1, "cell-contraction option: ctr-idx already set")
+        _cell_options(c, strs, [int(x) for x in np.atleast_1d(nums[0])] if nums else None, keep)
+    n = max(A.rank, B.rank, 1)
+    ia, ib = np.zeros(n, np.int32), np.zeros(n, np.int32)
+    nc, cA, cB = C.c_int32(), C.c_int32(), C.c_int32()
+    err = C.create_string_buffer(512)
+    P = C.POINTER
+    rc = L.qsx_itag_match(C.byref(ca), cells[0].nopt, cells[0].opt, cells[0].idx, cells[0].nidx, C.byref(cb),
+                          cells[1].nopt, cells[1].opt, cells[1].idx, cells[1].nidx,
+                          ia.ctypes.data_as(P(C.c_int32)), ib.ctypes.data_as(P(C.c_int32)), C.byref(nc),
+                          C.byref(cA), C.byref(cB), err, 512)
+    _capi.check(rc, err)
+    return list(ia[: nc.value]), list(ib[: nc.value]), cA.value, cB.value
 
 
 def _is_ctridx(a) -> bool:
@@ -194,207 +234,6 @@ def _is_numvec(a) -> bool:
     except (TypeError, ValueError):
         return False
     return v.ndim == 1 and v.size > 0 and np.issubdtype(v.dtype, np.number)
-
-
-def _cell_option(op, a):
-    """icFlags::set (Source/contractQS.cc:159-242): '*' = conj; '[!]##[*]' or
-    a numeric vector = the legs (1-based) that may be contracted ('!': the
-    legs that may not)."""
-    if isinstance(a, str):
-        s = a
-        if s.lstrip("-").lower().startswith("op:") or s.startswith("--itag:s/"):
-            raise _capi.QsxError(5, f"cell-contraction option '{a}' (itag operations) not supported")
-        x = s.startswith("!")
-        if x:
-            s = s[1:]
-        n = 0
-        while n < len(s) and s[n].isdigit():
-            n += 1
-        rest = s[n:]
-        if rest == "*":
-            if op["conj"]:
-                raise _capi.QsxError(1, "conj already set")
-            op["conj"] = 1
-        elif rest or (not n and not rest):
-            raise _capi.QsxError(1, f"invalid cell-contraction option (expecting '[!##][*]': got '{a}')")
-        if n:
-            if op["sidx"] is not None:
-                raise _capi.QsxError(1, f"ctr-idx already set ({a})")
-            op["sidx"] = [int(c) - 1 for c in s[:n]]
-            op["xflag"] = x
-        return
-    if not _is_numvec(a):
-        raise _capi.QsxError(1, f"unexpected cell-contraction input {type(a).__name__}")
-    v = [int(x) for x in np.atleast_1d(a)]
-    if any(x < 1 or x > 9 for x in v):
-        raise _capi.QsxError(1, f"invalid ctr-idx {v}")
-    if op["sidx"] is not None:
-        raise _capi.QsxError(1, "ctr-idx already set")
-    op["sidx"] = [x - 1 for x in v]
-    op["xflag"] = False
-
-
-def _skipped(op, T: QSpace):
-    """legs excluded from the itag match (icFlags::apply,
-    Source/contractQS.cc:244-293): without '!' the legs not listed, with '!'
-    the listed ones"""
-    if op["sidx"] is None:
-        return set()
-    r = T.rank
-    if any(j >= r for j in op["sidx"]):
-        raise _capi.QsxError(1, f"cell-ctr index out of bounds (r={r})")
-    if len(set(op["sidx"])) != len(op["sidx"]):
-        raise _capi.QsxError(1, "invalid cell-ctr index (non-unique)")
-    listed = set(op["sidx"])
-    return {i for i in range(r) if (i in listed) == op["xflag"]}
-
-
-def _single(items, mode, kw) -> QSpace:
-    """a cell form with one top-level operand (contractQS_itags with len 1
-    at level 0, Source/contractQS.cc:676-704): the operand (a nested cell is
-    contracted on the device first), then its conj flag and its index
-    option as a permutation"""
-    op = None
-    for a in items:
-        if isinstance(a, QSpace) or _is_cell(a):
-            if op is not None:
-                raise _capi.QsxError(1, "one top-level operand expected")
-            if isinstance(a, QSpace):
-                T = a
-            else:
-                if len(a) < 2:
-                    raise _capi.QsxError(1, "invalid cell-contraction (got cell with fewer than two entries)")
-                T = _cell_host(list(a), mode, kw) if _any_complex(a) else \
-                    _download_result(_cell_dev(list(a), 1, mode, kw, {}, []))
-            op = {"T": T, "conj": 0, "sidx": None, "xflag": False}
-        else:
-            if op is None:
-                raise _capi.QsxError(1, "invalid QSpace cell-structure (option before any QSpace)")
-            _cell_option(op, a)
-    if op is None:
-        raise _capi.QsxError(1, "invalid usage #2: no QSpace")
-    T = op["T"]
-    if op["conj"]:
-        T = permuteQS(T, [], conj=True)
-    if op["sidx"] is not None:
-        T = permuteQS(T, op["sidx"])
-    return T
-
-
-def _ctr_str(idx, conj):
-    """0-based leg list -> compact 1-based ctr-idx string (digits, then A=10..)."""
-    chars = "".join(str(i + 1) if i < 9 else chr(ord("A") + i - 9) for i in idx)
-    return chars + ("*" if conj else "")
-
-
-class _DevT:
-    """a cell operand resident on the device: its structure (a QSpace whose
-    data are not used) and its packed data"""
-
-    def __init__(self, S: QSpace, buf: "DeviceBuffer", keep=None):
-        self.S = S
-        self.buf = buf
-        self.keep = keep
-
-
-def _cell_dev(items, level, mode, kw, cache, plans):
-    """contractQS_itags (Source/contractQS.cc:634-808) on device-resident
-    operands: every leaf is uploaded once, each contraction is a plan
-    executed on device buffers, and only the result comes back.  The
-    intermediates keep their exactly-zero blocks (the reference drops them
-    after every contraction; they add nothing to the result, whose zero
-    blocks are dropped at the end)."""
-    ops = []
-    for a in items:
-        if isinstance(a, QSpace):
-            if id(a) not in cache:
-                cache[id(a)] = _DevT(a, DeviceBuffer.from_array(a.packed()), keep=a)
-            ops.append({"T": cache[id(a)], "conj": 0, "sidx": None, "xflag": False})
-        elif _is_cell(a):
-            if len(a) < 2:
-                raise _capi.QsxError(1, "invalid cell-contraction (got cell with fewer than two entries)")
-            ops.append({"T": _cell_dev(list(a), level + 1, mode, kw, cache, plans), "conj": 0, "sidx": None,
-                        "xflag": False})
-        else:
-            if not ops:
-                raise _capi.QsxError(1, "invalid QSpace cell-structure (option before any QSpace)")
-            _cell_option(ops[-1], a)
-    if len(ops) < 2:
-        raise _capi.QsxError(1, "invalid usage #2: at least two QSpaces required for each contraction")
-    cur = ops[-1]["T"]
-    cflag = ops[-1]["conj"]
-    skipB = _skipped(ops[-1], cur.S)
-    for op in reversed(ops[:-1]):
-        A = op["T"]
-        ia, ib, conjA = _itag_match(A.S, cur.S, op["conj"], cflag, _skipped(op, A.S), skipB)
-        P = CtrPlan(A.S, _ctr_str(ia, conjA), cur.S, _ctr_str(ib, cflag), mode=mode, **kw)
-        dC = DeviceBuffer(max(1, P.nout))
-        P.execute(A.buf, cur.buf, dC)
-        plans.append(P)
-        cur = _DevT(P.out, dC)
-        cflag = 0
-        skipB = set()
-    return cur
-
-
-def _any_complex(items) -> bool:
-    return any((isinstance(a, QSpace) and a.is_complex()) or (_is_cell(a) and _any_complex(a)) for a in items)
-
-
-def _cell_host(items, mode, kw) -> QSpace:
-    """the cell form with complex leaves: the same itag matching and
-    right-to-left fold (contractQS_itags, Source/contractQS.cc:634-808),
-    each step one device contraction of host QSpaces through the complex
-    path of qsx_contract (SkipZeroData after every contraction, as the
-    reference does)"""
-    ops = []
-    for a in items:
-        if isinstance(a, QSpace):
-            ops.append({"T": a, "conj": 0, "sidx": None, "xflag": False})
-        elif _is_cell(a):
-            if len(a) < 2:
-                raise _capi.QsxError(1, "invalid cell-contraction (got cell with fewer than two entries)")
-            ops.append({"T": _cell_host(list(a), mode, kw), "conj": 0, "sidx": None, "xflag": False})
-        else:
-            if not ops:
-                raise _capi.QsxError(1, "invalid QSpace cell-structure (option before any QSpace)")
-            _cell_option(ops[-1], a)
-    if len(ops) < 2:
-        raise _capi.QsxError(1, "invalid usage #2: at least two QSpaces required for each contraction")
-    cur = ops[-1]["T"]
-    cflag = ops[-1]["conj"]
-    skipB = _skipped(ops[-1], cur)
-    for op in reversed(ops[:-1]):
-        A = op["T"]
-        ia, ib, conjA = _itag_match(A, cur, op["conj"], cflag, _skipped(op, A), skipB)
-        cur = _contract_pair(A, ia, conjA, cur, ib, cflag, None, mode=mode, **kw)
-        cflag = 0
-        skipB = set()
-    return cur
-
-
-def _download_result(R: _DevT, skip_zero=True) -> QSpace:
-    """the device result back as a QSpace, SkipZeroData applied on the device
-    (eps = 0; zero blocks kept in a rank-2 block-diagonal matrix,
-    Source/QSpace.cc:3200-3270)"""
-    S = R.S
-    n = S.nblk
-    sizes = [int(np.prod(d.shape)) for d in S.data]
-    offs = np.zeros(n + 1, np.int64)
-    offs[1:] = np.cumsum(sizes) if n else []
-    keep = list(range(n))
-    if skip_zero and n:
-        blockdiag = S.rank == 2 and all(np.array_equal(S.Q[0][i], S.Q[1][i]) for i in range(n))
-        if not blockdiag:
-            flags = np.zeros(n, np.int32)
-            err = C.create_string_buffer(256)
-            _capi.check(_capi.lib().qsx_block_zero_flags(offs.ctypes.data_as(C.POINTER(C.c_int64)), n, R.buf.ptr,
-                                                         flags.ctypes.data_as(C.POINTER(C.c_int32)), err, 256), err)
-            keep = [i for i in range(n) if not flags[i]]
-    v = R.buf.download()
-    data = [np.asarray(v[offs[i]:offs[i + 1]]).reshape(S.data[i].shape, order="F") for i in keep]
-    cgw = S.info["cgw"][keep] if S.info["cgw"] is not None and keep else None
-    return QSpace([q[keep] for q in S.Q], data, S.qtype, S.itags, cgw)
 
 
 def _out_to_qspace(out) -> QSpace:

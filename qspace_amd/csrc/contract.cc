@@ -179,7 +179,8 @@ static inline uint64_t tag_name(uint64_t t) { return t & ~(uint64_t)0x80; }
 static inline bool tag_out(uint64_t t) { return (t & 0x80) != 0; }
 
 void match_itags(const TensorDesc& A, const TensorDesc& B, int conjA, int conjB, std::vector<int32_t>& ia,
-                 std::vector<int32_t>& ib, int& conjA_out) {
+                 std::vector<int32_t>& ib, int& conjA_out, const std::vector<char>* skipA,
+                 const std::vector<char>* skipB) {
   if ((int)A.itags.size() != A.r || (int)B.itags.size() != B.r)
     fail(QSX_ERR_ITAG, "itag auto-contraction needs itags on both operands");
   conjA_out = conjA;
@@ -191,8 +192,10 @@ void match_itags(const TensorDesc& A, const TensorDesc& B, int conjA, int conjB,
     int e = 0;
     const bool want_same = cflag != 0;  // sameAs sign +1 (same dir) if one side is conjugated
     for (int i = 0; i < A.r; ++i) {
+      if (skipA && (*skipA)[i]) continue;
       int hit = -1;
       for (int j = 0; j < B.r; ++j) {
+        if (skipB && (*skipB)[j]) continue;
         uint64_t a = A.itags[i], b = B.itags[j];
         if (tag_name(a) != tag_name(b)) continue;  // two empty names count as equal (itag_::sameAs)
         bool same = tag_out(a) == tag_out(b);
@@ -705,8 +708,8 @@ void plan_execute(CtrPlan& P, const double* dA, const double* dB, double* dC, hi
 }
 
 // ---------------------------------------------------------------- one-shot
-static qsx_tensor_out* make_out(const CtrPlan& P, const std::vector<int>& keep, const double* data,
-                                const double* data_im = nullptr) {
+qsx_tensor_out* make_out(const CtrPlan& P, const std::vector<int>& keep, const double* data,
+                         const double* data_im) {
   auto* o = (qsx_tensor_out*)calloc(1, sizeof(qsx_tensor_out));
   const int q = P.A.qdim, rc = P.rc;
   int n = (int)keep.size();
@@ -946,9 +949,9 @@ static qsx_tensor_out* contract_oneshot_complex(const qsx_tensor* A, const int32
   return out;
 }
 
-static qsx_tensor_out* contract_oneshot(const qsx_tensor* A, const int32_t* ia, int na, int conjA, const qsx_tensor* B,
-                                        const int32_t* ib, int nb, int conjB, const int32_t* perm, int nperm,
-                                        const qsx_opts* o) {
+qsx_tensor_out* contract_oneshot(const qsx_tensor* A, const int32_t* ia, int na, int conjA, const qsx_tensor* B,
+                                 const int32_t* ib, int nb, int conjB, const int32_t* perm, int nperm,
+                                 const qsx_opts* o) {
   ensure_device();
   hipStream_t s = default_stream();
   if (A && B && (A->nblk == 0 || B->nblk == 0)) {  // empty input -> empty output

@@ -74,9 +74,21 @@ void plan_execute(CtrPlan& P, const double* dA, const double* dB, double* dC, hi
 // permute an operand's packed device data into the plan's layout
 void plan_permute_operand(const CtrPlan& P, int which, const double* src, double* dst, hipStream_t s);
 
-// itag auto-matching (iTags::getCtr, Source/wbindex.cc:1512-1590)
+// itag auto-matching (iTags::getCtr, Source/wbindex.cc:1512-1590); legs
+// flagged in skipA / skipB (itag_::SetFlags of icFlags::apply) match nothing
 void match_itags(const TensorDesc& A, const TensorDesc& B, int conjA, int conjB, std::vector<int32_t>& ia,
-                 std::vector<int32_t>& ib, int& conjA_out);
+                 std::vector<int32_t>& ib, int& conjA_out, const std::vector<char>* skipA = nullptr,
+                 const std::vector<char>* skipB = nullptr);
+// host result of a plan: the output blocks keep[] (plan order) with the
+// packed host data (and imaginary parts, or null)
+qsx_tensor_out* make_out(const CtrPlan& P, const std::vector<int>& keep, const double* data,
+                         const double* data_im = nullptr);
+// contractQS usage #2 (cell.cc)
+qsx_tensor_out* contract_cell(const qsx_cell* items, int n, const int32_t* perm, int nperm, const qsx_opts* o);
+// contractQS usage #1 on host tensors (qsx_contract without the error wrapper)
+qsx_tensor_out* contract_oneshot(const qsx_tensor* A, const int32_t* ia, int na, int conjA, const qsx_tensor* B,
+                                 const int32_t* ib, int nb, int conjB, const int32_t* perm, int nperm,
+                                 const qsx_opts* o);
 
 hipStream_t default_stream();
 // host blocks h[i] (T's dims/offsets) -> the packed device buffer d, through

@@ -20,7 +20,7 @@ import re
 import numpy as np
 
 __all__ = [
-    "QSpace", "QsxTensor", "QsxTensorOut", "QsxOpts", "pack_itag", "unpack_itag",
+    "QSpace", "QsxTensor", "QsxTensorOut", "QsxCell", "QsxOpts", "pack_itag", "unpack_itag",
     "parse_ctridx", "qtype_syms", "to_c", "from_out",
 ]
 
@@ -43,6 +43,16 @@ class QsxTensorOut(C.Structure):
         ("qtype", C.c_char * 32), ("npairs", C.c_int64), ("useful_flops", C.c_double),
         ("gpu_ms", C.c_double), ("data_im", C.POINTER(C.c_double)),
     ]
+
+
+class QsxCell(C.Structure):
+    """one entry of the cell form (qsx_cell): a QSpace or a nested cell and
+    the options after it"""
+
+
+QsxCell._fields_ = [("T", C.POINTER(QsxTensor)), ("nsub", C.c_int32), ("sub", C.POINTER(QsxCell)),
+                    ("nopt", C.c_int32), ("opt", C.POINTER(C.c_char_p)), ("nidx", C.c_int32),
+                    ("idx", C.POINTER(C.c_int32))]
 
 
 class QsxOpts(C.Structure):

@@ -136,51 +136,70 @@ def test_no_device_is_a_clean_error():
 
 
 # ---- cell-form options and itag matching (host logic, no GPU) -------------
-from qspace_amd import api as _api  # noqa: E402
 
 
 def _mk(tags):
     return Q.QSpace([np.array([[0]])] * len(tags), [np.ones((1,) * len(tags))], "A", tags)
 
 
-@pytest.mark.parametrize("opt,sidx,xflag,conj", [("*", None, False, 1), ("23", [1, 2], False, 0),
-                                                 ("!2*", [1], True, 1), ("!13", [0, 2], True, 0),
-                                                 ([1, 3], [0, 2], False, 0)])
-def test_cell_options(opt, sidx, xflag, conj):
-    op = {"T": None, "conj": 0, "sidx": None, "xflag": False}
-    _api._cell_option(op, opt)
-    assert (op["sidx"], op["xflag"], op["conj"]) == (sidx, xflag, conj)
+# the options and the matching run in the library (qsx_itag_match, host only)
+def _legs(opt, tags=("L", "s", "R*")):
+    """legs of a rank-3 operand that a fully matching partner contracts
+    under option opt, and the conj flag the step takes"""
+    A = _mk(list(tags))
+    B = _mk([t[:-1] if t.endswith("*") else t + "*" for t in tags])
+    ia, ib, cA, cB = Q.itag_match(A, opt if isinstance(opt, list) and opt and isinstance(opt[0], str) else
+                                  ([opt] if opt is not None else []), B, [])
+    return ia, cA
 
 
-@pytest.mark.parametrize("bad", ["", "2x", "**", "-op:S", "--itag:s/a/b/", [0], [10]])
-def test_cell_option_errors(bad):
-    op = {"T": None, "conj": 0, "sidx": None, "xflag": False}
-    with pytest.raises(Q.QsxError):
-        _api._cell_option(op, bad)
+@pytest.mark.parametrize("opt,legs,conj", [(None, [0, 1, 2], 0), ("23", [1, 2], 0), ("!2", [0, 2], 0),
+                                           ("!13", [1], 0), ([1, 3], [0, 2], 0)])
+def test_cell_options(opt, legs, conj):
+    ia, cA = _legs(opt)
+    assert (ia, cA) == (legs, conj)
 
 
-def test_cell_skipped_legs():
-    T = _mk(["L", "s", "R*"])
-    op = {"T": T, "conj": 0, "sidx": [1], "xflag": True}     # '!2': leg 2 may not be contracted
-    assert _api._skipped(op, T) == {1}
-    op = {"T": T, "conj": 0, "sidx": [0, 2], "xflag": False}  # '13': only legs 1, 3
-    assert _api._skipped(op, T) == {1}
+def test_cell_option_conj():
+    """'*' on A flips the matching relation: A with itself matches all legs
+    with conj(A) taken (no missing-conj retry needed); '!2*' also excludes
+    leg 2"""
+    A = _mk(["L", "s", "R*"])
+    assert Q.itag_match(A, ["*"], A, []) == ([0, 1, 2], [0, 1, 2], 1, 0)
+    assert Q.itag_match(A, ["!2*"], A, []) == ([0, 2], [0, 2], 1, 0)
+    assert Q.itag_match(A, [], A, ["*"]) == ([0, 1, 2], [0, 1, 2], 0, 1)
+
+
+@pytest.mark.parametrize("bad,code", [("", 1), ("2x", 1), ("**", 1), ("-op:S", 5), ("--itag:s/a/b/", 5),
+                                      ([0], 1), ([10], 1), ("4", 1), ("11", 1)])
+def test_cell_option_errors(bad, code):
+    A = _mk(["L", "s", "R*"])
+    opt = ["*", bad] if bad == "*" else [bad]
+    if bad == "**":
+        opt = ["*", "*"]
+    with pytest.raises(Q.QsxError) as e:
+        Q.itag_match(A, opt, A, [])
+    assert e.value.code == code
 
 
 def test_itag_match_rules():
     A = _mk(["L", "s", "R*"])
     B = _mk(["L*", "s*", "R"])
-    assert _api._itag_match(A, B, 0, 0)[:2] == ([0, 1, 2], [0, 1, 2])
-    assert _api._itag_match(A, B, 0, 0, skipA={1})[:2] == ([0, 2], [0, 2])
+    assert Q.itag_match(A, [], B, [])[:2] == ([0, 1, 2], [0, 1, 2])
+    assert Q.itag_match(A, ["13"], B, [])[:2] == ([0, 2], [0, 2])
     # same directions: no match at first, retried with conj(A) (the
     # reference's "missing conj flag" rule)
-    ia, ib, cA = _api._itag_match(A, A, 0, 0)
-    assert (ia, ib, cA) == ([0, 1, 2], [0, 1, 2], 1)
+    assert Q.itag_match(A, [], A, []) == ([0, 1, 2], [0, 1, 2], 1, 0)
     # a leg that matches twice is an error (iTags::getCtr)
-    with pytest.raises(Q.QsxError):
-        _api._itag_match(_mk(["a", "a"]), _mk(["a*", "b"]), 0, 0)
+    with pytest.raises(Q.QsxError) as e:
+        Q.itag_match(_mk(["a", "a"]), [], _mk(["a*", "b"]), [])
+    assert e.value.code == 2
     # two empty names count as equal
-    assert _api._itag_match(_mk(["", "x"]), _mk(["*", "y"]), 0, 0)[:2] == ([0], [0])
+    assert Q.itag_match(_mk(["", "x"]), [], _mk(["*", "y"]), [])[:2] == ([0], [0])
+    # nothing to match at all
+    with pytest.raises(Q.QsxError) as e:
+        Q.itag_match(_mk(["a"]), [], _mk(["b"]), [])
+    assert e.value.code == 2
 
 
 def test_complex_rejected_where_unsupported():

@@ -355,6 +355,30 @@ def test_cell_form_options(gpu):
     assert rel_err(C, Z) < TOL
 
 
+def test_cell_form_single_operand_and_errors(gpu):
+    """one top-level operand (contractQS_itags at level 0 with one entry,
+    contractQS.cc:676-704): its conj flag and index option give a conj /
+    permutation of it, bit for bit as permuteQS; a nested cell as that
+    operand is contracted first; malformed cells are errors"""
+    A, L = gen.su2_mps(seed=11, tS=tuple(range(5)), dmin=3, dmax=8)
+    for got, ref in ((Q.contractQS(A, "*"), Q.permuteQS(A, [], conj=True)),
+                     (Q.contractQS(A, "312"), Q.permuteQS(A, [2, 0, 1])),
+                     (Q.contractQS(A, "312*"), Q.permuteQS(Q.permuteQS(A, [], conj=True), [2, 0, 1]))):
+        assert_same_structure(got, ref)
+        assert all(np.array_equal(a, b) for a, b in zip(got.data, ref.data))
+    X = Q.contractQS([L, A], "*")
+    Y = Q.permuteQS(Q.contractQS(L, "2", A, "1"), [], conj=True)
+    assert_same_structure(X, Y)
+    assert rel_err(X, Y) < TOL
+    bad = {"one-entry cell": [[A, "*"]], "one-entry nested cell": [A, [L]], "index beyond the rank": [A, A, "5"],
+           "one-entry first cell": [[A], L], "one-entry cell with a cell": [[[L, A], "*"]]}
+    for name, args in bad.items():
+        with pytest.raises(Q.QsxError) as e:
+            print("malformed cell form:", name, flush=True)
+            Q.contractQS(*args)
+        assert e.value.code in (1, 2), name
+
+
 def test_env_update_u1_cell_form(gpu):
     """environment update L' = A'.(HK.W.A) in the cell form of updateHK.m
     ({A,'!2*',{{A,HK},W}}), contracted on device-resident operands, against
