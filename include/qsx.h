@@ -30,7 +30,7 @@
  *   qsx_ortho           <- orthoQS core (SVD_Data::blockSVD + dmrgTruncate),
  *                          Source/mpsortho.cc:545-954,1031-1078; called by
  *                          mpsOrthoQS after the eigensolve, eigs2SiteQS.cc:481-484
- *   qsx_su2_cg / qsx_su2_x3
+ *   qsx_su2_cg / qsx_su2_x3 / qsx_su2_om_basis / qsx_su2_xom
  *                       <- SU(2) Clebsch-Gordan data and X-symbols of
  *                          X3Map::contractCGR, Source/clebsch_aux.cc:4073-4510
  */
@@ -83,6 +83,13 @@ extern "C" {
  *           non-abelian symmetry; entries of abelian symmetries ignored).
  *           NULL = external normalisation |cgw| = sqrt(d) of
  *           CRef::normExt (Source/clebsch.cc:4738-4760).
+ *   om    : nblk outer multiplicities (NULL = 1 for every block).  A block
+ *           with om > 1 carries a trailing data dimension of that size, the
+ *           coefficients of its data on the om basis tensors of
+ *           qsx_su2_om_basis (the extra data dimension of QS_FULL_OM
+ *           blocks, ExpandOM / FuseOM in Source/QSpace_aux.cc:242-345);
+ *           om must equal the size of that basis.  Contractions accept such
+ *           blocks (one SU(2) symmetry); the other entry points reject them.
  */
 typedef struct qsx_tensor {
   int32_t nblk;
@@ -98,6 +105,7 @@ typedef struct qsx_tensor {
      layout as data, or NULL for real data (qsx_contract, qsx_permute,
      qsx_plus; the other entry points reject complex data) */
   const double* const* data_im;
+  const int32_t* om;
 } qsx_tensor;
 
 /* Result tensor, owned by the library; release with qsx_tensor_free. */
@@ -115,6 +123,7 @@ typedef struct qsx_tensor_out {
   double useful_flops;   /* sum over executed pairs of 2*M*N*K */
   double gpu_ms;         /* device time of the data half (events) */
   double* data_im;       /* imaginary parts (complex results), else NULL */
+  int32_t* om;           /* nblk outer multiplicities, or NULL when all are 1 */
 } qsx_tensor_out;
 
 typedef struct qsx_opts {
@@ -268,8 +277,10 @@ int qsx_ctr_plan_create(const qsx_tensor* A, const int32_t* ia, int32_t na, int3
                         qsx_ctr_plan** plan, char* err, size_t errlen);
 /* structure of the result (data==NULL); caller frees with qsx_tensor_free */
 int qsx_ctr_plan_output(const qsx_ctr_plan* plan, qsx_tensor_out** C);
-/* pairs grouped by output block: ia/ib/afac have npairs entries,
-   seg has nout+1 entries (may pass NULL to query the sizes) */
+/* executed GEMM pairs grouped by output block: ia/ib/afac have npairs
+   entries, seg has nout+1 entries (may pass NULL to query the sizes).  With
+   outer multiplicity the groups are the output blocks' components and a
+   pair is one (ia, ib, a, b) term of one component. */
 int qsx_ctr_plan_pairs(const qsx_ctr_plan* plan, int64_t* npairs, int64_t* nout,
                        int32_t* ia, int32_t* ib, double* afac, int64_t* seg);
 /* the index set of contract_getIdxSet (Source/QSpace.cc:4131-4294) as the
@@ -398,6 +409,23 @@ int qsx_su2_x3(int32_t ra, const int32_t* tja, const int32_t* dira,
                int32_t rb, const int32_t* tjb, const int32_t* dirb,
                const int32_t* ia, const int32_t* ib, int32_t nctr, double* x3,
                char* err, size_t errlen);
+/* Outer-multiplicity basis of a block's SU(2) CGT space (rank r, labels
+   2S, dirs +1/-1): the om unit basis tensors, om x prod(tj[k]+1) doubles,
+   each column-major over the legs with index i = j - m, basis index
+   slowest.  For rank <= 3 the single canonical CGT.  Replaces the OM
+   bases of QS_FULL_OM CData (Source/clebsch.hh CData, RC_STORE fusion
+   trees) by this library's fusion-chain convention.  out may be NULL
+   (count only); returns om >= 0, or -code on error. */
+int qsx_su2_om_basis(int32_t r, const int32_t* tj, const int32_t* dir, double* out, int64_t outlen,
+                     char* err, size_t errlen);
+/* Matrix-valued X-symbol of two blocks' bases (X3Map::contractCGR with
+   outer multiplicity, Source/clebsch_aux.cc:4464-4471): x[(a*ob + b)*oc + c]
+   = <C_c | contract(A_a, B_b)>, the kept legs [A | B] permuted by perm
+   (new leg l = old leg perm[l]; NULL = identity).  Writes oa, ob, oc (oc = 0:
+   no invariant output) and up to xlen coefficients. */
+int qsx_su2_xom(int32_t ra, const int32_t* tja, const int32_t* dira, int32_t rb, const int32_t* tjb,
+                const int32_t* dirb, const int32_t* ia, const int32_t* ib, int32_t nctr, const int32_t* perm,
+                int32_t* dims3, double* x, int64_t xlen, char* err, size_t errlen);
 
 #ifdef __cplusplus
 }

@@ -1170,6 +1170,20 @@ double oracle_su2_cg(int32_t tj1, int32_t tm1, int32_t tj2, int32_t tm2, int32_t
   return T[i1 + (size_t)d1 * (i2 + (size_t)d2 * i3)];
 }
 
+// dense CG table <j1 m1; j2 m2 | j3 m3> of the lowering-operator
+// construction, column-major over (i1, i2, i3), i = j - m
+void oracle_cg_table(int32_t tj1, int32_t tj2, int32_t tj3, double* out) {
+  const std::vector<double>& T = cg_cached(tj1, tj2, tj3);
+  std::copy(T.begin(), T.end(), out);
+}
+
+// canonical leg order of a CGT: ord[k] = block leg at canonical position k
+void oracle_canon_order(int32_t r, const int32_t* tj, const int32_t* dir, int32_t* ord) {
+  std::vector<int> t(tj, tj + r), d(dir, dir + r);
+  std::vector<int> o = canon_order(t, d);
+  std::copy(o.begin(), o.end(), ord);
+}
+
 // dense block CGT (column-major in the block's leg order); returns numel
 int64_t oracle_block_cgt(int32_t r, const int32_t* tj, const int32_t* dir, double* out, int64_t outlen, char* err,
                          size_t errlen) {

@@ -22,6 +22,10 @@ Parity status of the oracle's functions (what pins each one):
                                          with the conjugated operand's data
                                          conjugated, zero imaginary parts =
                                          the real result bit for bit
+  * outer multiplicity (om.py)        -- see om.py: basis orthonormal,
+                                         invariant and complete; block-wise
+                                         contraction in the full M-resolved
+                                         space
   * ortho (orthoQS)                   -- dense singular values, A1 isometry,
                                          A1.A2 = PSI (kept all) or the scaled
                                          projection / Eckart-Young (truncated),
@@ -70,6 +74,10 @@ def lib():
         L.oracle_block_cgt.argtypes = [C.c_int32, P(C.c_int32), P(C.c_int32), P(C.c_double), C.c_int64, C.c_char_p,
                                        C.c_size_t]
         L.oracle_block_cgt.restype = C.c_int64
+        L.oracle_cg_table.argtypes = [C.c_int32, C.c_int32, C.c_int32, P(C.c_double)]
+        L.oracle_cg_table.restype = None
+        L.oracle_canon_order.argtypes = [C.c_int32, P(C.c_int32), P(C.c_int32), P(C.c_int32)]
+        L.oracle_canon_order.restype = None
         L.oracle_norm_ext.argtypes = [C.c_int32, P(C.c_int32), P(C.c_int32)]
         L.oracle_norm_ext.restype = C.c_double
         L.oracle_x3.argtypes = [C.c_int32, P(C.c_int32), P(C.c_int32), C.c_int32, P(C.c_int32), P(C.c_int32),
@@ -102,9 +110,24 @@ def _ivec(x):
     return a
 
 
+def _needs_om(A: QSpace, ia, B: QSpace, ib) -> bool:
+    if "SU2" not in _syms(A):
+        return False
+    rc = A.rank + B.rank - 2 * len(ia)
+    return A.rank > 3 or B.rank > 3 or rc > 3 or max(A.om(), default=1) > 1 or max(B.om(), default=1) > 1
+
+
 def contract(A: QSpace, ia, B: QSpace, ib, perm=None, conjA=0, conjB=0, fermA=None, fermB=None,
              skip_zero=1) -> QSpace:
-    """0-based ia/ib; returns the oracle's contractQS result."""
+    """0-based ia/ib; returns the oracle's contractQS result.  SU(2)
+    contractions with outer multiplicity (an operand or the result of rank
+    > 3, or blocks with an OM data dimension) go to om.contract, whose
+    output blocks are in sorted record order."""
+    if _needs_om(A, ia, B, ib):
+        if fermA is not None or fermB is not None:
+            raise OracleError(5, "fermionic signs with outer multiplicity are not restated")
+        import om
+        return om.contract(A, ia, B, ib, perm=perm, conjA=conjA, conjB=conjB, skip_zero=skip_zero)
     if A.is_complex() or B.is_complex():
         return _contract_complex(A, ia, B, ib, perm, conjA, conjB, fermA, fermB, skip_zero)
     L = lib()

@@ -16,7 +16,8 @@ import numpy as np
 from . import _capi
 from .qspace import QSpace, QsxCell, QsxOpts, QsxTensor, QsxTensorOut, from_out, parse_ctridx, to_c
 
-__all__ = ["contractQS", "permuteQS", "plusQS", "orthoQS", "ortho_args", "itag_match", "DeviceBuffer", "CtrPlan", "HPsi", "su2_cg", "su2_x3", "make_opts"]
+__all__ = ["contractQS", "permuteQS", "plusQS", "orthoQS", "ortho_args", "itag_match", "DeviceBuffer", "CtrPlan", "HPsi", "su2_cg", "su2_x3", "su2_om_basis",
+           "su2_xom", "make_opts"]
 
 
 def make_opts(mode="batched", nslices=0, skip_zero=1, hconj=0, fermA=None, fermB=None, verbose=1):
@@ -622,3 +623,46 @@ def su2_x3(tja, dira, tjb, dirb, ia, ib) -> float:
                       cb.ctypes.data_as(P(C.c_int32)), len(ia), C.byref(out), err, 256)
     _capi.check(rc, err)
     return out.value
+
+
+def su2_om_basis(tj, dirs) -> np.ndarray:
+    """outer-multiplicity basis of a block's SU(2) CGT space: array of shape
+    (om, 2j_1+1, ..., 2j_r+1), each basis tensor of unit norm (index i = j - m
+    per leg); om = 0 when the labels admit no invariant."""
+    L = _capi.lib()
+    P = C.POINTER
+    t, d = _ivec(tj), _ivec(dirs)
+    err = C.create_string_buffer(256)
+    om = L.qsx_su2_om_basis(len(tj), t.ctypes.data_as(P(C.c_int32)), d.ctypes.data_as(P(C.c_int32)), None, 0, err, 256)
+    if om < 0:
+        _capi.check(-om, err)
+    shape = tuple(int(x) + 1 for x in tj)
+    n = int(np.prod(shape, dtype=np.int64))
+    out = np.zeros(om * n)
+    if om:
+        rc = L.qsx_su2_om_basis(len(tj), t.ctypes.data_as(P(C.c_int32)), d.ctypes.data_as(P(C.c_int32)),
+                                out.ctypes.data_as(P(C.c_double)), out.size, err, 256)
+        if rc < 0:
+            _capi.check(-rc, err)
+    return np.stack([out[a * n:(a + 1) * n].reshape(shape, order="F") for a in range(om)]) if om else \
+        np.zeros((0,) + shape)
+
+
+def su2_xom(tja, dira, tjb, dirb, ia, ib, perm=None) -> np.ndarray:
+    """matrix-valued X-symbol x[a, b, c] = <C_c | contract(A_a, B_b)> of the
+    blocks' outer-multiplicity bases (output legs [kept A | kept B] permuted
+    by perm)."""
+    L = _capi.lib()
+    P = C.POINTER
+    rc_ = len(tja) + len(tjb) - 2 * len(ia)
+    a, da, b, db, ca, cb = (_ivec(v) for v in (tja, dira, tjb, dirb, ia, ib))
+    pv = _ivec(perm if perm is not None else range(rc_))
+    dims = (C.c_int32 * 3)()
+    err = C.create_string_buffer(256)
+    args = (len(tja), a.ctypes.data_as(P(C.c_int32)), da.ctypes.data_as(P(C.c_int32)), len(tjb),
+            b.ctypes.data_as(P(C.c_int32)), db.ctypes.data_as(P(C.c_int32)), ca.ctypes.data_as(P(C.c_int32)),
+            cb.ctypes.data_as(P(C.c_int32)), len(ia), pv.ctypes.data_as(P(C.c_int32)), dims)
+    _capi.check(L.qsx_su2_xom(*args, None, 0, err, 256), err)
+    x = np.zeros(dims[0] * dims[1] * dims[2])
+    _capi.check(L.qsx_su2_xom(*args, x.ctypes.data_as(P(C.c_double)), x.size, err, 256), err)
+    return x.reshape(dims[0], dims[1], dims[2])

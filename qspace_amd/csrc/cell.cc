@@ -95,6 +95,7 @@ struct Held {
   std::vector<int64_t> dims;
   std::vector<uint64_t> tags;
   std::vector<double> cgw;
+  std::vector<int32_t> om;
   std::string qtype;
   std::vector<const double*> re, im;
   qsx_tensor t{};
@@ -109,6 +110,7 @@ struct Held {
     t.itags = tags.empty() ? nullptr : tags.data();
     t.qtype = qtype.c_str();
     t.cgw = with_cgw && !cgw.empty() ? cgw.data() : nullptr;
+    t.om = om.empty() ? nullptr : om.data();
   }
 };
 
@@ -120,6 +122,7 @@ std::unique_ptr<Held> view_out(const qsx_tensor_out* o) {
   h->dims.assign(o->dims, o->dims + (size_t)n * r);
   if (o->itags) h->tags.assign(o->itags, o->itags + r);
   if (o->cgw && o->nsym) h->cgw.assign(o->cgw, o->cgw + (size_t)n * o->nsym);
+  if (o->om) h->om.assign(o->om, o->om + n);
   h->qtype = o->qtype;
   for (int i = 0; i < n; ++i) {
     h->re.push_back(o->data ? o->data + o->offs[i] : nullptr);
@@ -234,6 +237,7 @@ struct DevFold {
       h->Q = P->Qout;
       h->dims = P->dimsOut;
       h->tags = P->tagsC;
+      h->om = P->omOut;
       h->qtype = P->A.qtype;
       h->bind(P->nC, P->rc, P->A.qdim, false);  // results carry the external normalisation
       DevOperand r;

@@ -75,9 +75,17 @@ TensorDesc make_desc(const qsx_tensor* t, const char* name) {
     T.dims.assign(t->dims, t->dims + (size_t)T.n * T.r);
   }
   if (t->itags) T.itags.assign(t->itags, t->itags + T.r);
+  if (t->om && T.n) {
+    bool any = false;
+    for (int i = 0; i < T.n; ++i) {
+      if (t->om[i] < 1) fail(QSX_ERR_STRUCT, "tensor %s: outer multiplicity %d of block %d", name, t->om[i], i + 1);
+      any |= t->om[i] > 1;
+    }
+    if (any) T.om.assign(t->om, t->om + T.n);
+  }
   T.offs.assign(T.n + 1, 0);
   for (int i = 0; i < T.n; ++i) {
-    int64_t s = 1;
+    int64_t s = T.om_of(i);
     for (int k = 0; k < T.r; ++k) {
       int64_t d = T.dims[(size_t)i * T.r + k];
       if (d < 0) fail(QSX_ERR_STRUCT, "tensor %s: negative block dim", name);
@@ -104,6 +112,10 @@ TensorDesc make_desc(const qsx_tensor* t, const char* name) {
         fail(QSX_ERR_STRUCT, "tensor %s: non-unique Q records (blocks %d and %d)", name, p[i - 1] + 1, p[i] + 1);
   }
   return T;
+}
+
+void require_no_om(const TensorDesc& T, const char* what) {
+  if (T.has_om()) fail(QSX_ERR_UNSUPPORTED, "%s: blocks with outer multiplicity > 1 are supported by contractions only", what);
 }
 
 // ---------------------------------------------------------------- device
@@ -436,45 +448,6 @@ static void build_structure(CtrPlan& P, const int32_t* perm, int nperm, const do
   P.L = device_label_engine(QAc, QAk, A.n, QBc, QBk, B.n, kc, kma, kmb, default_stream());
   const int nL = (int)P.L.seg.size() - 1;
   const size_t np = P.L.Ia.size();
-  // afac per pair (contractDATA_group, QSpace_aux.cc:100-230).  The
-  // X-symbols are exact and memoised; a structure's new label sets are
-  // evaluated by OpenMP threads (the exact-arithmetic tables are grown
-  // first, so the threads only read them; an error is rethrown after the
-  // loop)
-  P.afac.assign(np, 1.0);
-  int max_tj = 0;
-  bool su2 = false;
-  for (int j = 0; j < A.nsym; ++j) {
-    if (A.symtype[j] != SYM_SU2) continue;
-    su2 = true;
-    for (int i = 0; i < A.n; ++i)
-      for (int l = 0; l < A.r; ++l) max_tj = std::max(max_tj, A.label(i, l, j));
-    for (int i = 0; i < B.n; ++i)
-      for (int l = 0; l < B.r; ++l) max_tj = std::max(max_tj, B.label(i, l, j));
-  }
-  if (su2) su2_reserve_labels(max_tj);
-  std::atomic<bool> bad{false};
-  Error first{0, ""};
-  std::mutex err_mu;
-#pragma omp parallel for schedule(dynamic, 32) if (su2 && np > 256)
-  for (size_t p = 0; p < np; ++p) {
-    if (bad.load(std::memory_order_relaxed)) continue;
-    try {
-      const int ia = P.L.Ia[p], ib = P.L.Ib[p];
-      double af = (fA ? fA[ia] : 1.0) * (fB ? fB[ib] : 1.0);
-      for (int j = 0; j < A.nsym && af != 0; ++j) {
-        if (A.symtype[j] != SYM_SU2) continue;
-        double x = su2_x3(su2_col(A, ia, j), P.dirA, su2_col(B, ib, j), P.dirB, P.ica, P.icb);
-        af *= x * A.cgw[(size_t)ia * A.nsym + j] * B.cgw[(size_t)ib * B.nsym + j];
-      }
-      if (std::fabs(af) < 1e-18) af = 0;  // CG_SKIP_EPS2
-      P.afac[p] = af;
-    } catch (const Error& e) {
-      std::lock_guard<std::mutex> lk(err_mu);
-      if (!bad.exchange(true)) first = e;
-    }
-  }
-  if (bad.load()) throw first;
   // output permutation (completed with the identity, bin/contractQS.m:154-170)
   P.perm.resize(P.rc);
   std::iota(P.perm.begin(), P.perm.end(), 0);
@@ -492,6 +465,73 @@ static void build_structure(CtrPlan& P, const int32_t* perm, int nperm, const do
   }
   P.has_perm = false;
   for (int i = 0; i < P.rc; ++i) P.has_perm |= P.perm[i] != i;
+  // outer multiplicity: CGTs of rank > 3 on the operands or the result, or
+  // blocks that carry components
+  int nsu2 = 0, jsu2 = -1;
+  for (int j = 0; j < A.nsym; ++j)
+    if (A.symtype[j] == SYM_SU2) {
+      ++nsu2;
+      jsu2 = j;
+    }
+  if ((A.has_om() || B.has_om()) && !nsu2)
+    fail(QSX_ERR_STRUCT, "outer multiplicity > 1 in a tensor without SU(2) symmetry");
+  P.om_path = nsu2 && (A.r > 3 || B.r > 3 || P.rc > 3 || A.has_om() || B.has_om());
+  if (P.om_path && nsu2 > 1) fail(QSX_ERR_UNSUPPORTED, "outer multiplicity with more than one SU(2) symmetry");
+  std::vector<std::shared_ptr<const SU2XTensor>> xs(P.om_path ? np : 0);
+  // afac per pair (contractDATA_group, QSpace_aux.cc:100-230).  The
+  // X-symbols are exact and memoised; a structure's new label sets are
+  // evaluated by OpenMP threads (the exact-arithmetic tables are grown
+  // first, so the threads only read them; an error is rethrown after the
+  // loop)
+  P.afac.assign(np, 1.0);
+  int max_tj = 0;
+  bool su2 = false;
+  for (int j = 0; j < A.nsym; ++j) {
+    if (A.symtype[j] != SYM_SU2) continue;
+    su2 = true;
+    for (int i = 0; i < A.n; ++i)
+      for (int l = 0; l < A.r; ++l) max_tj = std::max(max_tj, A.label(i, l, j));
+    for (int i = 0; i < B.n; ++i)
+      for (int l = 0; l < B.r; ++l) max_tj = std::max(max_tj, B.label(i, l, j));
+  }
+  if (su2) su2_reserve_labels(max_tj * std::max(3, std::max(std::max(A.r, B.r), P.rc)));
+  std::atomic<bool> bad{false};
+  Error first{0, ""};
+  std::mutex err_mu;
+#pragma omp parallel for schedule(dynamic, 32) if (su2 && np > 256)
+  for (size_t p = 0; p < np; ++p) {
+    if (bad.load(std::memory_order_relaxed)) continue;
+    try {
+      const int ia = P.L.Ia[p], ib = P.L.Ib[p];
+      double af = (fA ? fA[ia] : 1.0) * (fB ? fB[ib] : 1.0);
+      if (P.om_path) {
+        auto X = su2_xom(su2_col(A, ia, jsu2), P.dirA, su2_col(B, ib, jsu2), P.dirB, P.ica, P.icb, P.perm);
+        if (X->oa != A.om_of(ia))
+          fail(QSX_ERR_STRUCT, "block %d of A has outer multiplicity %d, its SU(2) labels give %d", ia + 1,
+               A.om_of(ia), X->oa);
+        if (X->ob != B.om_of(ib))
+          fail(QSX_ERR_STRUCT, "block %d of B has outer multiplicity %d, its SU(2) labels give %d", ib + 1,
+               B.om_of(ib), X->ob);
+        bool nz = false;
+        for (double v : X->x) nz |= v != 0;
+        af = nz ? af * A.cgw[(size_t)ia * A.nsym + jsu2] * B.cgw[(size_t)ib * B.nsym + jsu2] : 0.0;
+        xs[p] = std::move(X);
+        P.afac[p] = af;
+        continue;
+      }
+      for (int j = 0; j < A.nsym && af != 0; ++j) {
+        if (A.symtype[j] != SYM_SU2) continue;
+        double x = su2_x3(su2_col(A, ia, j), P.dirA, su2_col(B, ib, j), P.dirB, P.ica, P.icb);
+        af *= x * A.cgw[(size_t)ia * A.nsym + j] * B.cgw[(size_t)ib * B.nsym + j];
+      }
+      if (std::fabs(af) < 1e-18) af = 0;  // CG_SKIP_EPS2
+      P.afac[p] = af;
+    } catch (const Error& e) {
+      std::lock_guard<std::mutex> lk(err_mu);
+      if (!bad.exchange(true)) first = e;
+    }
+  }
+  if (bad.load()) throw first;
   // layouts (may reorder ica/icb jointly; afac is independent of that order)
   choose_layouts(P);
   // executed output blocks, pair descriptors, output normalisation
@@ -500,7 +540,10 @@ static void build_structure(CtrPlan& P, const int32_t* perm, int nperm, const do
   P.dimsOut.clear();
   P.offsOut.assign(1, 0);
   P.cgwOut.clear();
+  P.omOut.clear();
   P.blkL.clear();
+  P.nT = 0;
+  P.tOff.clear();
   P.pairs.clear();
   P.pIa.clear();
   P.pIb.clear();
@@ -539,10 +582,65 @@ static void build_structure(CtrPlan& P, const int32_t* perm, int nperm, const do
       cfac /= ne;
       // after the user permutation the block's CGT is the canonical CGT of
       // the new leg order: the data take its sign relative to the
-      // transposed one
-      if (P.has_perm) cfac *= su2_perm_sign(tj, P.dirC, P.perm);
+      // transposed one (on the om path the X-symbols are taken on the
+      // permuted output's basis instead)
+      if (P.has_perm && !P.om_path) cfac *= su2_perm_sign(tj, P.dirC, P.perm);
     }
-    for (int64_t p = p0; p < p1; ++p) {
+    auto pair_desc = [&](int64_t p, int64_t aoff, int64_t boff, double af) {
+      const int ia = P.L.Ia[p], ib = P.L.Ib[p];
+      int64_t K = 1, Mp = 1, Np = 1;
+      for (int l = 0; l < k; ++l) {
+        int64_t da = A.dims[(size_t)ia * A.r + P.ica[l]], db = B.dims[(size_t)ib * B.r + P.icb[l]];
+        if (da != db) fail(QSX_ERR_STRUCT, "contracted block dims differ (%ld/%ld)", (long)da, (long)db);
+        K *= da;
+      }
+      for (int l : P.ka) Mp *= A.dims[(size_t)ia * A.r + l];
+      for (int l : P.kb) Np *= B.dims[(size_t)ib * B.r + l];
+      if (Mp != Mc || Np != Nc) fail(QSX_ERR_STRUCT, "inconsistent output block dims within block %d", c + 1);
+      if (K > INT32_MAX) fail(QSX_ERR_UNSUPPORTED, "contracted dim too large");
+      PairDesc pd{};
+      pd.a_off = aoff;
+      pd.b_off = boff;
+      pd.K = (int32_t)K;
+      pd.flags = (P.la.kmajor ? 1 : 0) | (P.lb.kmajor ? 2 : 0);
+      pd.lda = (int32_t)std::max<int64_t>(1, P.la.kmajor ? K : Mc);
+      pd.ldb = (int32_t)std::max<int64_t>(1, P.lb.kmajor ? Nc : K);
+      pd.afac = af;
+      P.pairs.push_back(pd);
+      P.pIa.push_back(ia);
+      P.pIb.push_back(ib);
+      if (af != 0) P.flops += 2.0 * (double)Mc * (double)Nc * (double)K;
+    };
+    int omC = 1;
+    if (P.om_path) {
+      // one GEMM target per output component; a component that no X-symbol
+      // reaches gets one zero-weight pair, so that it is written (zeros)
+      omC = xs[first]->oc;
+      const int64_t cs = Mc * Nc;
+      for (int g = 0; g < omC; ++g) {
+        const size_t before = P.pairs.size();
+        for (int64_t p = p0; p < p1; ++p) {
+          if (P.afac[p] == 0) continue;
+          const SU2XTensor& X = *xs[p];
+          const int ia = P.L.Ia[p], ib = P.L.Ib[p];
+          const int64_t sa = A.slice(ia), sb = B.slice(ib);
+          for (int a = 0; a < X.oa; ++a)
+            for (int b = 0; b < X.ob; ++b) {
+              const double x = X.x[((size_t)a * X.ob + b) * X.oc + g];
+              if (x == 0) continue;
+              pair_desc(p, A.offs[ia] + a * sa, B.offs[ib] + b * sb, P.afac[p] * x * cfac);
+            }
+        }
+        if (P.pairs.size() == before) pair_desc(first, A.offs[P.L.Ia[first]], B.offs[P.L.Ib[first]], 0.0);
+        P.pseg.push_back((int64_t)P.pairs.size());
+        P.M.push_back((int32_t)Mc);
+        P.N.push_back((int32_t)Nc);
+        P.tOff.push_back(P.offsOut.back() + g * cs);
+        ++P.nT;
+      }
+      P.omOut.push_back(omC);
+    }
+    for (int64_t p = p0; p < p1 && !P.om_path; ++p) {
       if (P.afac[p] == 0) continue;
       const int ia = P.L.Ia[p], ib = P.L.Ib[p];
       int64_t K = 1, Mp = 1, Np = 1;
@@ -568,9 +666,13 @@ static void build_structure(CtrPlan& P, const int32_t* perm, int nperm, const do
       P.pIb.push_back(ib);
       P.flops += 2.0 * (double)Mc * (double)Nc * (double)K;
     }
-    P.pseg.push_back((int64_t)P.pairs.size());
-    P.M.push_back((int32_t)Mc);
-    P.N.push_back((int32_t)Nc);
+    if (!P.om_path) {
+      P.pseg.push_back((int64_t)P.pairs.size());
+      P.M.push_back((int32_t)Mc);
+      P.N.push_back((int32_t)Nc);
+      P.tOff.push_back(P.offsOut.back());
+      ++P.nT;
+    }
     // output record / dims after the user permutation
     std::vector<int32_t> r2(mc);
     std::vector<int64_t> d2(rc);
@@ -580,7 +682,7 @@ static void build_structure(CtrPlan& P, const int32_t* perm, int nperm, const do
     }
     P.Qout.insert(P.Qout.end(), r2.begin(), r2.end());
     P.dimsOut.insert(P.dimsOut.end(), d2.begin(), d2.end());
-    P.offsOut.push_back(P.offsOut.back() + Mc * Nc);
+    P.offsOut.push_back(P.offsOut.back() + Mc * Nc * omC);
     P.cgwOut.insert(P.cgwOut.end(), cgw.begin(), cgw.end());
     P.blkL.push_back(c);
     ++P.nC;
@@ -592,14 +694,32 @@ static void build_structure(CtrPlan& P, const int32_t* perm, int nperm, const do
   }
 }
 
+// block permutation plan of n blocks of rank r whose outer-multiplicity
+// components (om, empty = 1) form a trailing dimension that stays last
+static void perm_plan_om(PermPlan& pp, int n, int r, const int64_t* dims, const std::vector<int32_t>& om,
+                         const int64_t* offs, const std::vector<int>& perm) {
+  if (om.empty()) {
+    perm_plan_build(pp, n, r, dims, offs, offs, perm.data(), nullptr);
+    return;
+  }
+  std::vector<int64_t> d((size_t)n * (r + 1));
+  for (int b = 0; b < n; ++b) {
+    for (int k = 0; k < r; ++k) d[(size_t)b * (r + 1) + k] = dims[(size_t)b * r + k];
+    d[(size_t)b * (r + 1) + r] = om[b];
+  }
+  std::vector<int> p2(perm.begin(), perm.end());
+  p2.push_back(r);
+  perm_plan_build(pp, n, r + 1, d.data(), offs, offs, p2.data(), nullptr);
+}
+
 void plan_upload(CtrPlan& P) {
   hipStream_t s = default_stream();
   dev_free(P.dpairs);
   P.dpairs = (PairDesc*)dev_alloc(sizeof(PairDesc) * std::max<size_t>(1, P.pairs.size()));
   if (!P.pairs.empty())
     QSX_HIP(hipMemcpyAsync(P.dpairs, P.pairs.data(), sizeof(PairDesc) * P.pairs.size(), hipMemcpyHostToDevice, s));
-  std::vector<WorkTarget> targets(P.nC);
-  for (int c = 0; c < P.nC; ++c) targets[c] = WorkTarget{P.offsOut[c], P.M[c], P.N[c], P.pseg[c], P.pseg[c + 1]};
+  std::vector<WorkTarget> targets(P.nT);
+  for (int t = 0; t < P.nT; ++t) targets[t] = WorkTarget{P.tOff[t], P.M[t], P.N[t], P.pseg[t], P.pseg[t + 1]};
   P.work = build_work(targets, P.pairs, P.policy);
   dev_free(P.ditems);
   dev_free(P.dreds);
@@ -616,15 +736,13 @@ void plan_upload(CtrPlan& P) {
   if (!P.work.items.empty())
     QSX_HIP(hipMemcpyAsync(P.ditems, P.work.items.data(), sizeof(WorkItem) * P.work.items.size(), hipMemcpyHostToDevice, s));
   if (P.la.permute && !P.a_prepermuted) {
-    perm_plan_build(P.ppA, P.A.n, P.A.r, P.A.dims.data(), P.A.offs.data(), P.A.offs.data(), P.la.perm.data(),
-                    nullptr);
+    perm_plan_om(P.ppA, P.A.n, P.A.r, P.A.dims.data(), P.A.om, P.A.offs.data(), P.la.perm);
     perm_plan_upload(P.ppA, s);
     dev_free(P.dAperm);
     P.dAperm = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, P.A.total()));
   }
   if (P.lb.permute && !P.b_prepermuted) {
-    perm_plan_build(P.ppB, P.B.n, P.B.r, P.B.dims.data(), P.B.offs.data(), P.B.offs.data(), P.lb.perm.data(),
-                    nullptr);
+    perm_plan_om(P.ppB, P.B.n, P.B.r, P.B.dims.data(), P.B.om, P.B.offs.data(), P.lb.perm);
     perm_plan_upload(P.ppB, s);
     dev_free(P.dBperm);
     P.dBperm = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, P.B.total()));
@@ -634,7 +752,7 @@ void plan_upload(CtrPlan& P) {
     std::vector<int64_t> d((size_t)P.nC * P.rc);
     for (int c = 0; c < P.nC; ++c)
       for (int l = 0; l < P.rc; ++l) d[(size_t)c * P.rc + P.perm[l]] = P.dimsOut[(size_t)c * P.rc + l];
-    perm_plan_build(P.ppC, P.nC, P.rc, d.data(), P.offsOut.data(), P.offsOut.data(), P.perm.data(), nullptr);
+    perm_plan_om(P.ppC, P.nC, P.rc, d.data(), P.omOut, P.offsOut.data(), P.perm);
     perm_plan_upload(P.ppC, s);
     dev_free(P.dCtmp);
     P.dCtmp = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, P.offsOut.back()));
@@ -647,7 +765,7 @@ void plan_permute_operand(const CtrPlan& P, int which, const double* src, double
   const OperandLayout& L = which == 0 ? P.la : P.lb;
   if (!L.permute) fail(QSX_ERR_ARG, "operand needs no permute");
   PermPlan pp;
-  perm_plan_build(pp, T.n, T.r, T.dims.data(), T.offs.data(), T.offs.data(), L.perm.data(), nullptr);
+  perm_plan_om(pp, T.n, T.r, T.dims.data(), T.om, T.offs.data(), L.perm);
   perm_plan_upload(pp, s);
   launch_perm_plan(pp, src, dst, s);
   QSX_HIP(hipStreamSynchronize(s));
@@ -693,13 +811,13 @@ void plan_execute(CtrPlan& P, const double* dA, const double* dB, double* dC, hi
   }
   double* c = P.has_perm ? P.dCtmp : dC;
   if (P.mode == QSX_MODE_PLAIN) {
-    for (int blk = 0; blk < P.nC; ++blk) {
+    for (int t = 0; t < P.nT; ++t) {
       bool first = true;
-      for (int64_t p = P.pseg[blk]; p < P.pseg[blk + 1]; ++p) {
-        launch_pair_gemm_f64(P.pairs[p], P.offsOut[blk], P.M[blk], P.N[blk], first ? 0 : 1, a, b, c, s);
+      for (int64_t p = P.pseg[t]; p < P.pseg[t + 1]; ++p) {
+        launch_pair_gemm_f64(P.pairs[p], P.tOff[t], P.M[t], P.N[t], first ? 0 : 1, a, b, c, s);
         first = false;
       }
-      if (P.pseg[blk] == P.pseg[blk + 1]) launch_fill_zero(c + P.offsOut[blk], P.offsOut[blk + 1] - P.offsOut[blk], s);
+      if (P.pseg[t] == P.pseg[t + 1]) launch_fill_zero(c + P.tOff[t], (int64_t)P.M[t] * P.N[t], s);
     }
   } else if (P.mode == QSX_MODE_BATCHED) {
     launch_worklist(P.work, P.dqueues, P.ditems, P.dreds, P.dpairs, a, b, c, P.dscratch, s);
@@ -724,6 +842,12 @@ qsx_tensor_out* make_out(const CtrPlan& P, const std::vector<int>& keep, const d
   o->cgw = (double*)malloc(sizeof(double) * std::max<size_t>(1, (size_t)n * P.A.nsym));
   o->itags = (uint64_t*)calloc(std::max(1, rc), sizeof(uint64_t));
   o->offs[0] = 0;
+  bool any_om = false;
+  for (int c : keep) any_om |= !P.omOut.empty() && P.omOut[c] > 1;
+  if (any_om) {
+    o->om = (int32_t*)malloc(sizeof(int32_t) * std::max(1, n));
+    for (int i = 0; i < n; ++i) o->om[i] = P.omOut[keep[i]];
+  }
   for (int i = 0; i < n; ++i) {
     int c = keep[i];
     for (int l = 0; l < rc * q; ++l) o->Q[(size_t)i * rc * q + l] = P.Qout[(size_t)c * rc * q + l];
@@ -1135,6 +1259,7 @@ void qsx_tensor_free(qsx_tensor_out* t) {
   free(t->itags);
   free(t->cgw);
   free(t->data_im);
+  free(t->om);
   free(t);
 }
 
@@ -1229,7 +1354,7 @@ int qsx_ctr_plan_pairs(const qsx_ctr_plan* plan, int64_t* npairs, int64_t* nout,
   if (!plan) return QSX_ERR_ARG;
   const CtrPlan& P = *plan->p;
   if (npairs) *npairs = (int64_t)P.pairs.size();
-  if (nout) *nout = P.nC;
+  if (nout) *nout = P.nT;
   if (ia) std::copy(P.pIa.begin(), P.pIa.end(), ia);
   if (ib) std::copy(P.pIb.begin(), P.pIb.end(), ib);
   if (afac)
@@ -1341,6 +1466,46 @@ int qsx_su2_x3(int32_t ra, const int32_t* tja, const int32_t* dira, int32_t rb, 
     std::vector<int> a(tja, tja + ra), da(dira, dira + ra), b(tjb, tjb + rb), db(dirb, dirb + rb);
     std::vector<int> ca(ia, ia + nctr), cb(ib, ib + nctr);
     *x3 = su2_x3(a, da, b, db, ca, cb);
+    return 0;
+  });
+}
+
+int qsx_su2_om_basis(int32_t r, const int32_t* tj, const int32_t* dir, double* out, int64_t outlen, char* err,
+                     size_t errlen) {
+  int om = 0;
+  const int rc = guarded(err, errlen, [&]() -> int {
+    if (r < 0 || r > QSX_MAX_RANK || (r && (!tj || !dir))) fail(QSX_ERR_ARG, "invalid SU(2) label set");
+    std::vector<int> t(tj, tj + r), d(dir, dir + r);
+    auto B = su2_om_basis(t, d);
+    om = B->om;
+    if (out) {
+      if (outlen < (int64_t)om * B->numel) fail(QSX_ERR_ARG, "basis buffer too small (%lld < %lld)",
+                                                (long long)outlen, (long long)om * B->numel);
+      std::fill(out, out + (int64_t)om * B->numel, 0.0);
+      for (int a = 0; a < om; ++a)
+        for (const auto& e : B->ent[a]) out[(int64_t)a * B->numel + e.first] = e.second;
+    }
+    return 0;
+  });
+  return rc ? -rc : om;
+}
+
+int qsx_su2_xom(int32_t ra, const int32_t* tja, const int32_t* dira, int32_t rb, const int32_t* tjb,
+                const int32_t* dirb, const int32_t* ia, const int32_t* ib, int32_t nctr, const int32_t* perm,
+                int32_t* dims3, double* x, int64_t xlen, char* err, size_t errlen) {
+  return guarded(err, errlen, [&]() -> int {
+    std::vector<int> a(tja, tja + ra), da(dira, dira + ra), b(tjb, tjb + rb), db(dirb, dirb + rb);
+    std::vector<int> ca(ia, ia + nctr), cb(ib, ib + nctr);
+    const int rc = ra + rb - 2 * nctr;
+    if (rc < 0) fail(QSX_ERR_ARG, "invalid contraction");
+    std::vector<int> p(rc);
+    std::iota(p.begin(), p.end(), 0);
+    if (perm) p.assign(perm, perm + rc);
+    auto X = su2_xom(a, da, b, db, ca, cb, p);
+    dims3[0] = X->oa;
+    dims3[1] = X->ob;
+    dims3[2] = X->oc;
+    if (x) std::copy(X->x.begin(), X->x.begin() + std::min<int64_t>(xlen, (int64_t)X->x.size()), x);
     return 0;
   });
 }

@@ -39,11 +39,19 @@ struct CtrPlan {
   std::vector<int> blkL;              // output block -> index into L's blocks
   std::vector<int> perm;              // output permutation (identity if none)
   bool has_perm = false;
-  // GEMM work
-  std::vector<PairDesc> pairs;        // executed pairs, grouped by output block
-  std::vector<int64_t> pseg;          // nC+1 ranges into pairs
+  // outer multiplicity (SU(2) CGTs of rank > 3, su2_om_basis): each pair
+  // expands into one GEMM per non-zero X-symbol entry (a, b, c) on the
+  // operands' components a, b into the output component c
+  bool om_path = false;
+  std::vector<int32_t> omOut;         // nC outer multiplicities (om path), else empty
+  // GEMM work: targets are the output blocks, or (om path) every output
+  // block's components
+  int nT = 0;
+  std::vector<int64_t> tOff;          // nT output offsets of the targets
+  std::vector<PairDesc> pairs;        // executed pairs, grouped by target
+  std::vector<int64_t> pseg;          // nT+1 ranges into pairs
   std::vector<int32_t> pIa, pIb;      // executed pairs' block indices
-  std::vector<int32_t> M, N;          // per output block matrix dims
+  std::vector<int32_t> M, N;          // per target matrix dims
   OperandLayout la, lb;
   bool a_prepermuted = false, b_prepermuted = false;
   double flops = 0;                   // useful flops (2MNK over executed pairs)

@@ -592,6 +592,11 @@ static void hpsi_create(HpsiPlan& H, const std::vector<const qsx_tensor*>& Ls, c
   H.has_hkr = hkr != nullptr;
   if (hkl) H.HKL = make_desc(hkl, "HK_L");
   if (hkr) H.HKR = make_desc(hkr, "HK_R");
+  for (const auto& T : H.Ls) require_no_om(T, "H*psi");
+  for (const auto& T : H.Rs) require_no_om(T, "H*psi");
+  require_no_om(H.psi, "H*psi");
+  if (hkl) require_no_om(H.HKL, "H*psi");
+  if (hkr) require_no_om(H.HKR, "H*psi");
   H.xpairs = H.has_hkl || H.has_hkr;
   if (o) {
     H.mode = o->mode;
@@ -630,6 +635,7 @@ static void hpsi_create(HpsiPlan& H, const std::vector<const qsx_tensor*>& Ls, c
     H.s1[t] = plan_create(Ls[tm.li], ia.data(), (int)ia.size(), cA, psi, ib.data(), (int)ib.size(), 0, nullptr, 0,
                           &s1o, false, 1);
     if (H.s1[t]->lb.permute) fail(QSX_ERR_UNSUPPORTED, "H*psi: psi needs a permute in stage 1");
+    if (H.s1[t]->om_path) fail(QSX_ERR_UNSUPPORTED, "H*psi: SU(2) intermediates of rank > 3 (outer multiplicity)");
   }
   clk.lap("stage-1 structure");
   // stage 2 plans: (T_t, R_t [,'*'])
@@ -649,6 +655,7 @@ static void hpsi_create(HpsiPlan& H, const std::vector<const qsx_tensor*>& Ls, c
     H.s2[t] = plan_create(&Tt[t].t, ia.data(), (int)ia.size(), cA, Rs[tm.ri], ib.data(), (int)ib.size(), tm.cr,
                           nullptr, 0, &so, false, 2);
     if (H.s2[t]->la.permute) fail(QSX_ERR_UNSUPPORTED, "H*psi: intermediate needs a permute in stage 2");
+    if (H.s2[t]->om_path) fail(QSX_ERR_UNSUPPORTED, "H*psi: SU(2) intermediates of rank > 3 (outer multiplicity)");
   }
   // local terms: HK_L . psi (itag matching, psi as the B operand) and
   // psi . HK_R (psi as the A operand); psi is read where it lies

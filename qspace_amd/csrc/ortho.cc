@@ -148,6 +148,7 @@ struct OrthoResult {
 OrthoResult ortho_oneshot(const qsx_tensor* pt, int K, int Nkmin, int64_t Nkeep, double stol, const char* itag) {
   if (pt && pt->data_im) fail(QSX_ERR_UNSUPPORTED, "orthoQS: complex data not supported (real QSpaces only)");
   TensorDesc P = make_desc(pt, "PSI");
+  require_no_om(P, "orthoQS");
   const int r = P.r, q = P.qdim;
   bool su2 = false;
   for (int t : P.symtype) {

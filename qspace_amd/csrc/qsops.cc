@@ -200,6 +200,7 @@ qsx_tensor_out* permute_oneshot(const qsx_tensor* At, const int32_t* perm, int n
   ensure_device();
   hipStream_t s = default_stream();
   TensorDesc A = make_desc(At, "A");
+  require_no_om(A, "permuteQS");
   const int r = A.r, q = A.qdim;
   if (nperm < 0 || nperm > r) fail(QSX_ERR_ARG, "invalid permutation (len=%d/%d)", nperm, r);
   if (nperm && !perm) fail(QSX_ERR_ARG, "null permutation");
@@ -306,6 +307,8 @@ qsx_tensor_out* plus_oneshot(const qsx_tensor* At, const qsx_tensor* Bt, double 
   ensure_device();
   hipStream_t s = default_stream();
   TensorDesc A = make_desc(At, "A"), B = make_desc(Bt, "B");
+  require_no_om(A, "plusQS");
+  require_no_om(B, "plusQS");
   if (!A.n && !B.n) return copy_out(A, packed_host(A), 1.0);
   // an empty operand contributes no block (Cat / plus_plain return the
   // other one, scaled by bfac if it is B); the rest of the path is common

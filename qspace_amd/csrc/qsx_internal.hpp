@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <memory>
 #include <vector>
 
 #include "../../include/qsx.h"
@@ -39,6 +40,7 @@ struct TensorDesc {
   std::vector<int64_t> offs;         // n+1 packed element offsets
   std::vector<uint64_t> itags;       // r or empty
   std::vector<double> cgw;           // n x nsym
+  std::vector<int32_t> om;           // n outer multiplicities, or empty (all 1)
   bool cgw_given = false;
   const double* const* hdata = nullptr;  // host block pointers (may be null)
   const double* const* hdata_im = nullptr;  // imaginary parts (complex data) or null
@@ -50,7 +52,13 @@ struct TensorDesc {
     return false;
   }
   int label(int blk, int leg, int col) const { return Q[(size_t)blk * r * qdim + (size_t)leg * qdim + col]; }
+  int om_of(int blk) const { return om.empty() ? 1 : om[blk]; }
+  bool has_om() const { return !om.empty(); }
+  // block size of one outer-multiplicity component
+  int64_t slice(int blk) const { return numel(blk) / om_of(blk); }
 };
+// entry points without outer-multiplicity support refuse such tensors
+void require_no_om(const TensorDesc& T, const char* what);
 
 TensorDesc make_desc(const qsx_tensor* t, const char* name);
 void parse_qtype(const char* s, int qdim, std::vector<int>& st);
@@ -80,6 +88,27 @@ void su2_reserve_labels(int max_tj);
 // +-1: the CGT implied by the legs in order perm (new leg k = old leg
 // perm[k]) relative to the transposed CGT of the block
 double su2_perm_sign(const std::vector<int>& tj, const std::vector<int>& dir, const std::vector<int>& perm);
+// outer-multiplicity basis of a block's SU(2) CGT space (su2.cc states the
+// convention): om unit tensors, each a sparse list of (block-order
+// column-major index over the legs' 2j+1 states, value); om = 0 when the
+// labels admit no invariant
+struct SU2Basis {
+  int r = 0, om = 0;
+  std::vector<int> tj, dims;
+  int64_t numel = 1;
+  std::vector<std::vector<std::pair<int64_t, double>>> ent;
+};
+std::shared_ptr<const SU2Basis> su2_om_basis(const std::vector<int>& tj, const std::vector<int>& dir);
+int su2_om(const std::vector<int>& tj, const std::vector<int>& dir);
+// contraction coefficients of basis tensors, x[(a*ob + b)*oc + c]
+struct SU2XTensor {
+  int oa = 0, ob = 0, oc = 0;
+  std::vector<double> x;
+};
+std::shared_ptr<const SU2XTensor> su2_xom(const std::vector<int>& tja, const std::vector<int>& dira,
+                                          const std::vector<int>& tjb, const std::vector<int>& dirb,
+                                          const std::vector<int>& ia, const std::vector<int>& ib,
+                                          const std::vector<int>& perm);
 
 // ---------------------------------------------------------------- kernels
 // Work descriptors of the grouped GEMM (gemm.hip)

@@ -15,6 +15,16 @@
 //    (X3Map::contractCGR / contract_x3, Source/clebsch_aux.cc:4073-4510);
 //  * external norm sqrt(d) (CRef::normExt, Source/clebsch.cc:4738-4760).
 //
+// Outer multiplicity (rank >= 4, one SU(2) symmetry): a block's CGT space
+// has dimension OM >= 1 and its data carry a trailing OM dimension (the
+// extra data dimension of the reference's QS_FULL_OM blocks, ExpandOM /
+// FuseOM / tensorRefs in Source/QSpace_aux.cc:242-345, matrix-valued cgw in
+// Source/clebsch_aux.cc:4464-4471).  The basis is this library's own
+// convention (the reference takes its OM bases from RC_STORE fusion trees),
+// stated in su2_om_basis below; the contraction coefficients are the
+// overlaps X[a][b][c] = <C_c | contract(A_a, B_b)> of unit basis tensors.
+// For rank <= 3 the basis is the single canonical CGT above.
+//
 // Method (ours): CG coefficients by the Racah sum evaluated in exact integer
 // arithmetic (prime-exponent bookkeeping + a small big-integer type), so they
 // carry only the final rounding at any spin.  Since C_AB = x3 * C_C for
@@ -24,6 +34,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <functional>
+#include <memory>
 #include <cstdint>
 #include <cstring>
 #include <map>
@@ -325,6 +337,38 @@ inline bool allowed(const CanonCGT& c, const int* tm) {
   return false;
 }
 
+// canonical leg order (QSet::Sort): legs of the majority direction first
+// (flag 0; incoming on a tie), each group by ascending label with ties kept
+// in leg order; equal groups are swapped when the first differing label of
+// the first group is the larger.  ord[k] = block leg at canonical position
+// k, fl[k] its flag.
+void canon_order(const std::vector<int>& tj, const std::vector<int>& dir, std::vector<int>& ord,
+                 std::vector<int>& fl) {
+  const int r = (int)tj.size();
+  int nin = 0;
+  for (int d : dir) nin += d > 0;
+  const bool flip = nin < (r + 1) / 2;
+  auto flag = [&](int k) { return (dir[k] > 0 ? 0 : 1) ^ (flip ? 1 : 0); };
+  ord.resize(r);
+  for (int k = 0; k < r; ++k) ord[k] = k;
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
+    if (flag(a) != flag(b)) return flag(a) < flag(b);
+    return tj[a] < tj[b];
+  });
+  if (2 * nin == r) {
+    int l = r / 2;
+    for (int i = 0; i < l; ++i) {
+      int a = tj[ord[i]], b = tj[ord[i + l]];
+      if (a != b) {
+        if (a > b) std::rotate(ord.begin(), ord.begin() + l, ord.end());
+        break;
+      }
+    }
+  }
+  fl.resize(r);
+  for (int k = 0; k < r; ++k) fl[k] = flag(ord[k]);
+}
+
 CanonCGT make_canon(const std::vector<int>& tj, const std::vector<int>& dir) {
   CanonCGT c;
   int r = (int)tj.size();
@@ -335,32 +379,14 @@ CanonCGT make_canon(const std::vector<int>& tj, const std::vector<int>& dir) {
     c.kind = K_SCALAR;
     return c;
   }
-  if (r > 3) fail(QSX_ERR_UNSUPPORTED, "SU(2) CGT of rank %d (outer multiplicity) not supported", r);
-  int nin = 0;
-  for (int d : dir) nin += d > 0;
-  const bool flip = nin < (r + 1) / 2;
-  auto flag = [&](int k) { return (dir[k] > 0 ? 0 : 1) ^ (flip ? 1 : 0); };
-  std::vector<int> ord(r);
-  for (int k = 0; k < r; ++k) ord[k] = k;
-  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
-    if (flag(a) != flag(b)) return flag(a) < flag(b);
-    return tj[a] < tj[b];
-  });
-  if (2 * nin == r) {  // equal groups (rank 2 in/out): rotate if first > second
-    int l = r / 2;
-    for (int i = 0; i < l; ++i) {
-      int a = tj[ord[i]], b = tj[ord[i + l]];
-      if (a != b) {
-        if (a > b) std::rotate(ord.begin(), ord.begin() + l, ord.end());
-        break;
-      }
-    }
-  }
+  if (r > 3) fail(QSX_ERR_UNSUPPORTED, "SU(2) CGT of rank %d has an outer multiplicity basis (su2_om_basis)", r);
+  std::vector<int> ord, flv;
+  canon_order(tj, dir, ord, flv);
   int fl[3];
   for (int k = 0; k < r; ++k) {
     c.ord[k] = ord[k];
     c.tj[k] = tj[ord[k]];
-    fl[k] = flag(ord[k]);
+    fl[k] = flv[k];
   }
   double nrm;
   if (r == 2) {
@@ -616,6 +642,353 @@ double su2_norm_ext(const std::vector<int>& tj, const std::vector<int>& dir) {
   int d = 0;
   for (int t : tj) d = std::max(d, t + 1);
   return std::sqrt((double)d);
+}
+
+
+// ---------------------------------------------------------------- outer multiplicity
+// Basis of the SU(2) CGT space of one block (labels tj, directions dir in
+// block leg order), this library's convention:
+//  * canonical leg order and flags as for rank <= 3 (canon_order); group 0 =
+//    the leading canonical legs of the first flag (n0 legs), group 1 = the
+//    rest (n1 legs);
+//  * n1 > 0: each group is coupled left to right by CG coefficients,
+//    J_1 = j_1, (J_{i-1} M_{i-1}) x (j_i m_i) -> (J_i M_i), and the two
+//    groups' totals are identified (J, M) = (J', M'):
+//      C[m] = prod_i CG_0(...) * prod_i CG_1(...);
+//  * n1 = 0 (one direction): legs 1..r-1 of the group are coupled as above
+//    to (J, M) = (j_r, -m_r) with the factor (-1)^(j_r - m_r);
+//  * one basis tensor per intermediate path (J_2..J_{n0-1}, J'_2..J'_{n1-1})
+//    with matching totals, ordered lexicographically (group 0's path first,
+//    each intermediate ascending); each normalised to |C|_F = 1 with its
+//    first entry above 1e-14 of the norm, in column-major order over the
+//    canonical legs, positive.
+// For rank <= 3 this is the single canonical CGT (block_entry) and
+// intermediate paths do not occur.
+namespace {
+
+struct OmKey {
+  std::vector<int> v;
+  bool operator==(const OmKey& o) const { return v == o.v; }
+};
+struct OmKeyHash {
+  size_t operator()(const OmKey& k) const { return VecHash()(k.v); }
+};
+
+std::unordered_map<OmKey, std::shared_ptr<const SU2Basis>, OmKeyHash> g_om;
+std::shared_mutex g_om_mu;
+std::unordered_map<OmKey, std::shared_ptr<const SU2XTensor>, OmKeyHash> g_xom;
+std::shared_mutex g_xom_mu;
+
+// couplings of one group: every m-tuple of its legs with its total M and
+// coefficient, for one intermediate path (pathJ[i] = J after leg i)
+struct ChainEntry {
+  std::vector<int> tm;  // two-m per group leg
+  int TM;
+  double v;
+};
+
+void chain_paths(const std::vector<int>& tjs, std::vector<std::vector<int>>& out) {
+  out.clear();
+  const int n = (int)tjs.size();
+  if (n == 0) return;
+  std::vector<int> path(n);
+  path[0] = tjs[0];
+  std::function<void(int)> rec = [&](int i) {
+    if (i == n) {
+      out.push_back(path);
+      return;
+    }
+    const int a = path[i - 1], b = tjs[i];
+    for (int J = std::abs(a - b); J <= a + b; J += 2) {
+      path[i] = J;
+      rec(i + 1);
+    }
+  };
+  rec(1);
+}
+
+void chain_entries(const std::vector<int>& tjs, const std::vector<int>& path, std::vector<ChainEntry>& out) {
+  out.clear();
+  const int n = (int)tjs.size();
+  std::vector<int> tm(n);
+  std::function<void(int, int, double)> rec = [&](int i, int M, double v) {
+    if (i == n) {
+      out.push_back(ChainEntry{tm, M, v});
+      return;
+    }
+    for (int m = -tjs[i]; m <= tjs[i]; m += 2) {
+      tm[i] = m;
+      if (i == 0) {
+        rec(1, m, 1.0);
+        continue;
+      }
+      const int M2 = M + m;
+      if (std::abs(M2) > path[i]) continue;
+      const double c = su2_cg_exact(path[i - 1], M, tjs[i], m, path[i], M2);
+      if (c == 0.0) continue;
+      rec(i + 1, M2, v * c);
+    }
+  };
+  rec(0, 0, 1.0);
+}
+
+std::shared_ptr<const SU2Basis> om_basis_compute(const std::vector<int>& tj, const std::vector<int>& dir) {
+  auto B = std::make_shared<SU2Basis>();
+  const int r = (int)tj.size();
+  B->r = r;
+  B->tj = tj;
+  B->dims.resize(r);
+  int64_t tot = 1;
+  for (int k = 0; k < r; ++k) {
+    if (tj[k] < 0) fail(QSX_ERR_CG, "negative SU(2) label");
+    B->dims[k] = tj[k] + 1;
+    tot *= tj[k] + 1;
+  }
+  B->numel = tot;
+  if (r == 0) {
+    B->om = 1;
+    B->ent.push_back({{0, 1.0}});
+    return B;
+  }
+  std::vector<int> ord, fl;
+  canon_order(tj, dir, ord, fl);
+  int n0 = 1;
+  while (n0 < r && fl[n0] == fl[0]) ++n0;
+  const int n1 = r - n0;
+  // legs of the chains (canonical positions) and the singlet leg
+  std::vector<int> g0, g1;
+  int last = -1;
+  if (n1 > 0) {
+    for (int k = 0; k < n0; ++k) g0.push_back(k);
+    for (int k = n0; k < r; ++k) g1.push_back(k);
+  } else {
+    for (int k = 0; k + 1 < r; ++k) g0.push_back(k);
+    last = r - 1;
+  }
+  auto labels = [&](const std::vector<int>& g) {
+    std::vector<int> v;
+    for (int k : g) v.push_back(tj[ord[k]]);
+    return v;
+  };
+  const std::vector<int> t0 = labels(g0), t1 = labels(g1);
+  std::vector<std::vector<int>> P0, P1;
+  chain_paths(t0, P0);
+  if (n1 > 0) chain_paths(t1, P1);
+  else if (r == 1) P0.assign(1, std::vector<int>{tj[0]});
+  // canonical column-major strides, block strides
+  std::vector<int64_t> cst(r), bst(r);
+  {
+    int64_t a = 1, b = 1;
+    for (int k = 0; k < r; ++k) {
+      cst[k] = a;
+      a *= tj[ord[k]] + 1;
+      bst[k] = b;
+      b *= tj[k] + 1;
+    }
+  }
+  std::vector<ChainEntry> E0, E1;
+  auto emit = [&](const std::vector<std::pair<std::vector<int>, double>>& raw) {
+    // raw: two-m per canonical position -> value
+    double nrm2 = 0;
+    for (auto& e : raw) nrm2 += e.second * e.second;
+    const double nrm = std::sqrt(nrm2);
+    if (!(nrm > 0)) return;
+    int64_t first = -1;
+    double sgn = 1;
+    for (auto& e : raw) {
+      if (std::fabs(e.second) <= 1e-14 * nrm) continue;
+      int64_t cl = 0;
+      for (int k = 0; k < r; ++k) cl += cst[k] * ((tj[ord[k]] - e.first[k]) / 2);
+      if (first < 0 || cl < first) {
+        first = cl;
+        sgn = e.second < 0 ? -1 : 1;
+      }
+    }
+    std::vector<std::pair<int64_t, double>> ent;
+    ent.reserve(raw.size());
+    for (auto& e : raw) {
+      int64_t bl = 0;
+      for (int k = 0; k < r; ++k) bl += bst[ord[k]] * ((tj[ord[k]] - e.first[k]) / 2);
+      ent.push_back({bl, sgn * e.second / nrm});
+    }
+    std::sort(ent.begin(), ent.end());
+    B->ent.push_back(std::move(ent));
+  };
+  if (r == 1) {
+    if (tj[0] == 0) B->ent.push_back({{0, 1.0}});
+    B->om = (int)B->ent.size();
+    return B;
+  }
+  std::vector<std::pair<std::vector<int>, double>> raw;
+  for (const auto& p0 : P0) {
+    if (n1 == 0) {
+      if (p0.back() != tj[ord[last]]) continue;
+      chain_entries(t0, p0, E0);
+      raw.clear();
+      for (const auto& e : E0) {
+        std::vector<int> tm(r);
+        for (int i = 0; i < (int)g0.size(); ++i) tm[g0[i]] = e.tm[i];
+        tm[last] = -e.TM;
+        const double ph = (((tj[ord[last]] - tm[last]) / 2) & 1) ? -1.0 : 1.0;
+        raw.push_back({tm, ph * e.v});
+      }
+      emit(raw);
+      continue;
+    }
+    for (const auto& p1 : P1) {
+      if (p1.back() != p0.back()) continue;
+      chain_entries(t0, p0, E0);
+      chain_entries(t1, p1, E1);
+      std::unordered_map<int, std::vector<int>> byM;
+      for (int i = 0; i < (int)E1.size(); ++i) byM[E1[i].TM].push_back(i);
+      raw.clear();
+      for (const auto& e : E0) {
+        auto it = byM.find(e.TM);
+        if (it == byM.end()) continue;
+        for (int i1 : it->second) {
+          std::vector<int> tm(r);
+          for (int i = 0; i < n0; ++i) tm[g0[i]] = e.tm[i];
+          for (int i = 0; i < n1; ++i) tm[g1[i]] = E1[i1].tm[i];
+          raw.push_back({tm, e.v * E1[i1].v});
+        }
+      }
+      emit(raw);
+    }
+  }
+  B->om = (int)B->ent.size();
+  return B;
+}
+
+}  // namespace
+
+std::shared_ptr<const SU2Basis> su2_om_basis(const std::vector<int>& tj, const std::vector<int>& dir) {
+  OmKey key;
+  key.v = tj;
+  key.v.insert(key.v.end(), dir.begin(), dir.end());
+  {
+    std::shared_lock<std::shared_mutex> lk(g_om_mu);
+    auto it = g_om.find(key);
+    if (it != g_om.end()) return it->second;
+  }
+  int mx = 0;
+  for (int t : tj) mx += t;
+  su2_reserve_labels(mx);
+  auto B = om_basis_compute(tj, dir);
+  std::unique_lock<std::shared_mutex> lk(g_om_mu);
+  return g_om.emplace(std::move(key), B).first->second;
+}
+
+int su2_om(const std::vector<int>& tj, const std::vector<int>& dir) { return su2_om_basis(tj, dir)->om; }
+
+// X[a][b][c] = <C_c | P(contract(A_a, B_b))>: A's legs ia contracted with
+// B's legs ib, the kept legs [A | B] permuted by perm (new leg l = old leg
+// perm[l]), C the basis of the permuted output
+std::shared_ptr<const SU2XTensor> su2_xom(const std::vector<int>& tja, const std::vector<int>& dira,
+                                          const std::vector<int>& tjb, const std::vector<int>& dirb,
+                                          const std::vector<int>& ia, const std::vector<int>& ib,
+                                          const std::vector<int>& perm) {
+  OmKey key;
+  auto put = [&](const std::vector<int>& v) {
+    key.v.push_back((int)v.size());
+    key.v.insert(key.v.end(), v.begin(), v.end());
+  };
+  put(tja);
+  put(dira);
+  put(tjb);
+  put(dirb);
+  put(ia);
+  put(ib);
+  put(perm);
+  {
+    std::shared_lock<std::shared_mutex> lk(g_xom_mu);
+    auto it = g_xom.find(key);
+    if (it != g_xom.end()) return it->second;
+  }
+  const int ra = (int)tja.size(), rb = (int)tjb.size(), k = (int)ia.size();
+  std::vector<char> ca(ra, 0), cb(rb, 0);
+  for (int i = 0; i < k; ++i) {
+    ca[ia[i]] = 1;
+    cb[ib[i]] = 1;
+    if (tja[ia[i]] != tjb[ib[i]]) fail(QSX_ERR_CG, "contracted SU(2) labels differ");
+    if (dira[ia[i]] == dirb[ib[i]]) fail(QSX_ERR_ITAG, "contracted SU(2) legs must have opposite directions");
+  }
+  std::vector<int> ka, kb, tj0, dir0;
+  for (int i = 0; i < ra; ++i)
+    if (!ca[i]) { ka.push_back(i); tj0.push_back(tja[i]); dir0.push_back(dira[i]); }
+  for (int i = 0; i < rb; ++i)
+    if (!cb[i]) { kb.push_back(i); tj0.push_back(tjb[i]); dir0.push_back(dirb[i]); }
+  const int rc = (int)tj0.size();
+  if ((int)perm.size() != rc) fail(QSX_ERR_ARG, "su2_xom: permutation length");
+  std::vector<int> tjc(rc), dirc(rc);
+  for (int l = 0; l < rc; ++l) {
+    tjc[l] = tj0[perm[l]];
+    dirc[l] = dir0[perm[l]];
+  }
+  auto A = su2_om_basis(tja, dira), B = su2_om_basis(tjb, dirb), Cb = su2_om_basis(tjc, dirc);
+  auto X = std::make_shared<SU2XTensor>();
+  X->oa = A->om;
+  X->ob = B->om;
+  X->oc = Cb->om;
+  X->x.assign((size_t)X->oa * X->ob * X->oc, 0.0);
+  if (X->oc > 0 && X->oa > 0 && X->ob > 0) {
+    if (Cb->numel > ((int64_t)1 << 26)) fail(QSX_ERR_UNSUPPORTED, "SU(2) output CGT of %lld entries", (long long)Cb->numel);
+    // per-leg digits of a block-order linear index
+    auto digits = [](int64_t lin, const std::vector<int>& d, int* out) {
+      for (size_t q = 0; q < d.size(); ++q) {
+        out[q] = (int)(lin % d[q]);
+        lin /= d[q];
+      }
+    };
+    // output strides in the permuted order: position of old kept leg o
+    std::vector<int64_t> cst(rc);
+    {
+      std::vector<int64_t> s(rc);
+      int64_t a = 1;
+      for (int l = 0; l < rc; ++l) {
+        s[l] = a;
+        a *= Cb->dims[l];
+      }
+      for (int l = 0; l < rc; ++l) cst[perm[l]] = s[l];
+    }
+    // B entries bucketed by the contracted digits
+    std::vector<std::unordered_map<int64_t, std::vector<std::pair<int64_t, double>>>> Bk(X->ob);
+    std::vector<int> dg(std::max(ra, rb) + 1);
+    for (int b = 0; b < X->ob; ++b)
+      for (const auto& e : B->ent[b]) {
+        digits(e.first, B->dims, dg.data());
+        int64_t key2 = 0, kept = 0;
+        for (int i = 0; i < k; ++i) key2 = key2 * (tjb[ib[i]] + 1) + dg[ib[i]];
+        for (size_t q = 0; q < kb.size(); ++q) kept += cst[ka.size() + q] * dg[kb[q]];
+        Bk[b][key2].push_back({kept, e.second});
+      }
+    std::vector<double> T((size_t)Cb->numel);
+    for (int a = 0; a < X->oa; ++a)
+      for (int b = 0; b < X->ob; ++b) {
+        std::fill(T.begin(), T.end(), 0.0);
+        bool any = false;
+        for (const auto& e : A->ent[a]) {
+          digits(e.first, A->dims, dg.data());
+          int64_t key2 = 0, kept = 0;
+          for (int i = 0; i < k; ++i) key2 = key2 * (tja[ia[i]] + 1) + dg[ia[i]];
+          auto it = Bk[b].find(key2);
+          if (it == Bk[b].end()) continue;
+          for (size_t q = 0; q < ka.size(); ++q) kept += cst[q] * dg[ka[q]];
+          for (const auto& f : it->second) {
+            T[(size_t)(kept + f.first)] += e.second * f.second;
+            any = true;
+          }
+        }
+        if (!any) continue;
+        for (int c = 0; c < X->oc; ++c) {
+          double acc = 0;
+          for (const auto& e : Cb->ent[c]) acc += e.second * T[(size_t)e.first];
+          if (std::fabs(acc) < 1e-14) acc = 0;
+          X->x[((size_t)a * X->ob + b) * X->oc + c] = acc;
+        }
+      }
+  }
+  std::unique_lock<std::shared_mutex> lk(g_xom_mu);
+  return g_xom.emplace(std::move(key), X).first->second;
 }
 
 }  // namespace qsx

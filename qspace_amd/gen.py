@@ -26,6 +26,8 @@ Workloads (sizes at full scale):
 """
 from __future__ import annotations
 
+import itertools
+
 import numpy as np
 
 from .qspace import QSpace
@@ -271,3 +273,56 @@ def u1_two_site_psi(seed=1007, Lq=tuple(range(-10, 11, 2)), Rq=tuple(range(-10, 
                         Q[k].append([v])
                     data.append(_normal_block(rng, (dL[qL], 1, 1, dR[qR])))
     return QSpace([np.array(q) for q in Q], data, "A", list(itags))
+
+
+# ---------------------------------------------------------------------------
+def _su2_invariants(tj) -> int:
+    """number of SU(2) invariants of a leg set (multiplicity of total spin 0,
+    by iterated coupling counts; directions do not change it)"""
+    mult = {int(tj[0]): 1}
+    for t in tj[1:]:
+        nxt = {}
+        for J, m in mult.items():
+            for K in range(abs(J - int(t)), J + int(t) + 1, 2):
+                nxt[K] = nxt.get(K, 0) + m
+        mult = nxt
+    return mult.get(0, 0)
+
+
+def su2_om_tensor(seed, legs, itags, scale=1.0, keep=1.0):
+    """random SU(2) QSpace of any rank: every combination of the legs'
+    (2S, dim) sectors that admits an invariant becomes a block; a block
+    whose CGT space has dimension om > 1 carries a trailing data dimension
+    of size om (outer multiplicity).  keep < 1 drops blocks at random."""
+    rng = _rng(seed)
+    r = len(legs)
+    Q = [[] for _ in range(r)]
+    data = []
+    for combo in itertools.product(*legs):
+        tj = [c[0] for c in combo]
+        om = _su2_invariants(tj)
+        if not om or (keep < 1 and rng.rand() > keep):
+            continue
+        shape = tuple(c[1] for c in combo) + ((om,) if om > 1 else ())
+        for k in range(r):
+            Q[k].append([tj[k]])
+        data.append(scale * _normal_block(rng, shape))
+    return QSpace([np.array(q, dtype=np.int32).reshape(-1, 1) for q in Q], data, "SU2", list(itags))
+
+
+def su2_env_update(seed=1009, tS=tuple(range(0, 7)), dmin=2, dmax=6, dw=2):
+    """Operands of the SU(2) environment update L' = A'.(HK.W.A) in the leg
+    conventions of updateHK.m: A(L,R*,s) (s = spin 1/2), HK(L*,L,w*) and the
+    rank-4 MPO W(w,w2*,s*,s) whose operator legs carry spin 0 and 1 (the
+    identity and spin-operator channels of a Heisenberg MPO), random data;
+    W's blocks and the rank-4 intermediates have outer multiplicity."""
+    rng = _rng(seed)
+    A0, _ = su2_mps(seed=seed, tS=tS, dmin=dmin, dmax=dmax)
+    A = QSpace([A0.Q[0], A0.Q[2], A0.Q[1]], [np.asfortranarray(np.transpose(d, (0, 2, 1))) for d in A0.data],
+               "SU2", ["L", "R*", "s"])
+    dL = {int(q): A.data[i].shape[0] for i, q in enumerate(A.Q[0][:, 0])}
+    secL = sorted(dL.items())
+    wsec = [(0, dw), (2, dw)]
+    HK = su2_om_tensor(seed + 1, [secL, secL, wsec], ["L*", "L", "w*"])
+    W = su2_om_tensor(seed + 2, [wsec, wsec, [(1, 1)], [(1, 1)]], ["w", "w2*", "s*", "s"])
+    return A, HK, W

@@ -5,7 +5,10 @@ mx layout Source/QSpace.hh:3150-3364):
 
 * ``Q``     list of ``rank`` int32 arrays, each ``nblk x qdim`` (MATLAB's 1 x r
             cell of label matrices; SU(2) labels are 2S);
-* ``data``  list of ``nblk`` column-major float64 blocks with ``rank`` dims;
+* ``data``  list of ``nblk`` column-major float64 blocks with ``rank`` dims
+            (a block with outer multiplicity om > 1 has a trailing data
+            dimension of size om: its coefficients on the om basis tensors of
+            ``su2_om_basis``, the extra data dimension of QS_FULL_OM blocks);
 * ``info``  dict with ``qtype`` ("A", "SU2", "A,SU2", ...), ``itags`` (list of
             leg tags, trailing '*' = outgoing leg, Source/wbindex.hh:632-700)
             and ``cgw`` (nblk x nsym outer-multiplicity weights, or None for
@@ -31,7 +34,7 @@ class QsxTensor(C.Structure):
         ("Q", C.POINTER(C.c_int32)), ("dims", C.POINTER(C.c_int64)),
         ("data", C.POINTER(C.POINTER(C.c_double))), ("itags", C.POINTER(C.c_uint64)),
         ("qtype", C.c_char_p), ("cgw", C.POINTER(C.c_double)),
-        ("data_im", C.POINTER(C.POINTER(C.c_double))),
+        ("data_im", C.POINTER(C.POINTER(C.c_double))), ("om", C.POINTER(C.c_int32)),
     ]
 
 
@@ -41,7 +44,7 @@ class QsxTensorOut(C.Structure):
         ("Q", C.POINTER(C.c_int32)), ("dims", C.POINTER(C.c_int64)), ("offs", C.POINTER(C.c_int64)),
         ("data", C.POINTER(C.c_double)), ("itags", C.POINTER(C.c_uint64)), ("cgw", C.POINTER(C.c_double)),
         ("qtype", C.c_char * 32), ("npairs", C.c_int64), ("useful_flops", C.c_double),
-        ("gpu_ms", C.c_double), ("data_im", C.POINTER(C.c_double)),
+        ("gpu_ms", C.c_double), ("data_im", C.POINTER(C.c_double)), ("om", C.POINTER(C.c_int32)),
     ]
 
 
@@ -182,6 +185,12 @@ class QSpace:
     def dims(self) -> np.ndarray:
         return np.array([d.shape[: self.rank] for d in self.data], dtype=np.int64).reshape(self.nblk, self.rank)
 
+    def om(self) -> np.ndarray:
+        """per-block outer multiplicity (size of the trailing data dimension
+        beyond the rank, 1 if there is none)"""
+        r = self.rank
+        return np.array([d.shape[r] if d.ndim > r else 1 for d in self.data], dtype=np.int32)
+
     def numel(self) -> int:
         return int(sum(d.size for d in self.data))
 
@@ -267,6 +276,12 @@ def to_c(T: QSpace, with_data: bool = True):
         ct.cgw = cg.ctypes.data_as(C.POINTER(C.c_double))
     else:
         ct.cgw = None
+    om = T.om() if n and r else np.ones(0, np.int32)
+    if om.size and om.max() > 1:
+        keep.append(om)
+        ct.om = om.ctypes.data_as(C.POINTER(C.c_int32))
+    else:
+        ct.om = None
     return ct, keep
 
 
@@ -277,6 +292,9 @@ def from_out(o: QsxTensorOut, with_data: bool = True) -> QSpace:
     offs = np.ctypeslib.as_array(o.offs, shape=(n + 1,)).copy()
     data = []
     dl = dims.tolist()
+    if o.om:
+        om = np.ctypeslib.as_array(o.om, shape=(max(1, n),))[:n].tolist()
+        dl = [d + [m] if m > 1 else d for d, m in zip(dl, om)]
     if with_data and o.data:
         # one copy of the whole result; the blocks are column-major views of it
         tot = int(offs[-1])

@@ -3,13 +3,16 @@ checks: Full(T) = sum over blocks of data (x) (cgw * CGT), each leg laid out
 sector by sector (ascending label record), multiplet index major, m minor.
 A contraction is correct iff Full(C) = tensordot(Full(A), Full(B)).
 
-The CGT of a block comes from the oracle (block_cgt); any consistent CGT
+The CGT of a block comes from the oracle (block_cgt; for outer
+multiplicity the sum over the om basis tensors of om.om_basis with the
+block's trailing data dimension as coefficients); any consistent CGT
 convention passes, so this pins the X-symbols and the cgw/normalisation
 bookkeeping, not the sign convention (pinned separately).  At most one SU(2)
 symmetry per qtype (plus abelian ones).
 """
 import numpy as np
 
+import om as OM
 import oracle as O
 
 
@@ -59,6 +62,11 @@ def expand(T, layouts):
         if j is not None:
             tj = [rec[j] for rec in recs]
             w = cgw[i, j] if cgw is not None else O.norm_ext(tj, dirs)
+            if r > 3 or np.ndim(d) > r:  # outer multiplicity: sum over the basis
+                blk = OM.expand_block(d, r, tj, dirs, w)
+                idx = tuple(slice(layouts[k][0][recs[k]], layouts[k][0][recs[k]] + blk.shape[k]) for k in range(r))
+                full[idx] += blk
+                continue
             cg = O.block_cgt(tj, dirs) * w
         else:
             cg = np.ones((1,) * r)
