@@ -313,9 +313,9 @@ def orthoQS(PSI: QSpace, K: int, Nkeep=None, stol=None, Nkmin: int = 0, itag: st
     A2 = V S sfac with legs (PSI legs K..r-1, new leg), PSI ~ A1 . A2 over
     the new leg; info holds svd (all singular values, descending), Nkeep,
     Ntot, svd2tr, sv2tot, sfac, flag, and the device time.  Real data;
-    abelian symmetries, and SU(2) for K = 1 or r-1 splits of rank <= 3
-    (info then holds svd_qdim, the multiplet dimension of each singular
-    value)."""
+    abelian symmetries, and SU(2) for K = 1 or r-1 splits of rank <= 3 and
+    K = 2 of a rank-4 tensor (outer multiplicity) (info then holds
+    svd_qdim, the multiplet dimension of each singular value)."""
     Nkeep, stol = ortho_args(PSI, K, Nkeep, stol)
     L = _capi.lib()
     ca, ka = to_c(PSI)

@@ -46,6 +46,9 @@
 //    QSX_ERR_UNSUPPORTED.
 #include <algorithm>
 #include <cmath>
+#include <map>
+#include <memory>
+#include <set>
 #include <cstdlib>
 #include <cstring>
 #include <numeric>
@@ -145,21 +148,18 @@ struct OrthoResult {
   qsx_ortho_info* info = nullptr;
 };
 
+OrthoResult ortho_fused(const qsx_tensor* pt, const TensorDesc& P, int Nkmin, int64_t Nkeep, double stol,
+                        const char* itag);
+
 OrthoResult ortho_oneshot(const qsx_tensor* pt, int K, int Nkmin, int64_t Nkeep, double stol, const char* itag) {
   if (pt && pt->data_im) fail(QSX_ERR_UNSUPPORTED, "orthoQS: complex data not supported (real QSpaces only)");
   TensorDesc P = make_desc(pt, "PSI");
-  require_no_om(P, "orthoQS");
   const int r = P.r, q = P.qdim;
   bool su2 = false;
   for (int t : P.symtype) {
     if (t == SYM_SU2) su2 = true;
     else if (t != SYM_A) fail(QSX_ERR_UNSUPPORTED, "orthoQS: symmetry %s not supported", P.qtype.c_str());
   }
-  if (su2 && (r > 3 || (K != 1 && K != r - 1)))
-    fail(QSX_ERR_UNSUPPORTED,
-         "orthoQS: non-abelian symmetries (%s) need K = 1 or r-1 and rank <= 3 (got K=%d/%d)", P.qtype.c_str(), K,
-         r);
-  if (su2 && P.nsym != q) fail(QSX_ERR_UNSUPPORTED, "orthoQS: symmetries with several labels (%s)", P.qtype.c_str());
   if (P.n == 0) fail(QSX_ERR_ARG, "orthoQS: got empty PSI");
   if (K < 1 || K >= r) fail(QSX_ERR_ARG, "orthoQS: index specs out of bounds (%d/%d)", K, r);
   if ((int)P.itags.size() != r) fail(QSX_ERR_ITAG, "orthoQS: got invalid/empty itags (%d/%d)", (int)P.itags.size(), r);
@@ -171,6 +171,13 @@ OrthoResult ortho_oneshot(const qsx_tensor* pt, int K, int Nkmin, int64_t Nkeep,
       if (dir[i] != dir[K]) fail(QSX_ERR_ITAG, "orthoQS: 2nd/right group is mixed in+out (K=%d)", K);
   }
   if (P.total() && !P.hdata) fail(QSX_ERR_ARG, "orthoQS: PSI without data");
+  if (su2 && r == 4 && K == 2 && P.nsym == q) return ortho_fused(pt, P, Nkmin, Nkeep, stol, itag);
+  require_no_om(P, "orthoQS");
+  if (su2 && (r > 3 || (K != 1 && K != r - 1)))
+    fail(QSX_ERR_UNSUPPORTED,
+         "orthoQS: non-abelian symmetries (%s) need K = 1 or r-1 and rank <= 3, or K = 2 of rank 4 (got K=%d/%d)",
+         P.qtype.c_str(), K, r);
+  if (su2 && P.nsym != q) fail(QSX_ERR_UNSUPPORTED, "orthoQS: symmetries with several labels (%s)", P.qtype.c_str());
 
   // ---- groups and their block matrices
   std::vector<int> Ib;
@@ -754,6 +761,163 @@ OrthoResult ortho_oneshot(const qsx_tensor* pt, int K, int Nkmin, int64_t Nkeep,
   info->sweeps = maxsw;
   info->ngroups = (int32_t)items.size();
   R.info = info;
+  return R;
+}
+
+// a host tensor owned here, as a qsx_tensor view
+struct HostQ {
+  std::vector<int32_t> Q, om;
+  std::vector<int64_t> dims;
+  std::vector<uint64_t> tags;
+  std::vector<double> cgw;
+  std::vector<std::vector<double>> blocks;
+  std::vector<const double*> ptr;
+  std::string qtype;
+  qsx_tensor t{};
+  void bind(int n, int r, int q, bool with_cgw) {
+    ptr.clear();
+    for (auto& b : blocks) ptr.push_back(b.data());
+    t.nblk = n;
+    t.rank = r;
+    t.qdim = q;
+    t.Q = Q.empty() ? nullptr : Q.data();
+    t.dims = dims.empty() ? nullptr : dims.data();
+    t.data = ptr.empty() ? nullptr : ptr.data();
+    t.data_im = nullptr;
+    t.itags = tags.empty() ? nullptr : tags.data();
+    t.qtype = qtype.c_str();
+    t.cgw = with_cgw && !cgw.empty() ? cgw.data() : nullptr;
+    t.om = om.empty() ? nullptr : om.data();
+  }
+};
+
+// a library result as an input view (borrowed data)
+struct OutView {
+  std::vector<const double*> ptr;
+  qsx_tensor t{};
+  explicit OutView(const qsx_tensor_out* o) {
+    for (int i = 0; i < o->nblk; ++i) ptr.push_back(o->data ? o->data + o->offs[i] : nullptr);
+    t.nblk = o->nblk;
+    t.rank = o->rank;
+    t.qdim = o->qdim;
+    t.Q = o->Q;
+    t.dims = o->dims;
+    t.data = ptr.empty() ? nullptr : ptr.data();
+    t.data_im = nullptr;
+    t.itags = o->itags;
+    t.qtype = o->qtype;
+    t.cgw = o->cgw;
+    t.om = o->om;
+  }
+};
+
+struct OutFree {
+  void operator()(qsx_tensor_out* o) const { qsx_tensor_free(o); }
+};
+using OutPtr = std::unique_ptr<qsx_tensor_out, OutFree>;
+
+// K = 2 of a rank-4 SU(2) PSI (the two-site wave function after the
+// eigensolve, mpsOrthoQS at Source/mpsortho.cc:1031-1078 from
+// eigs2SiteQS.cc:481-484): legs (0, 1) are fused into one leg X by an
+// isometry F(0, 1, X*) (data: identity maps of each label pair into the
+// multiplet space of each coupled label, CGT weight normExt, so that F is an
+// isometry in the full M-space), PSI~(X, 2, 3) = conj(F).PSI (a rank-3
+// tensor: outer multiplicity resolved by the X-symbols), the K = 1 split of
+// PSI~ gives U(X, X'*) and A2(2, 3, X'), and A1 = F.U.  F being a complete
+// isometry, the singular values, their multiplets and the truncation are
+// those of PSI split (0, 1 | 2, 3) per total spin of legs (0, 1).
+OrthoResult ortho_fused(const qsx_tensor* pt, const TensorDesc& P, int Nkmin, int64_t Nkeep, double stol,
+                        const char* itag) {
+  (void)pt;
+  const int q = P.qdim;
+  // label records of legs 0 and 1 with their dims, and the fused labels of
+  // each pair that a block of PSI couples to (SU(2): J in (j0 x j1) and in
+  // (j2 x j3); abelian: the sum of legs 0 and 1)
+  using Rec = std::vector<int32_t>;
+  std::map<std::pair<Rec, Rec>, std::pair<std::pair<int64_t, int64_t>, std::set<Rec>>> pairs;
+  for (int i = 0; i < P.n; ++i) {
+    Rec r0(q), r1(q), x(q);
+    for (int c = 0; c < q; ++c) {
+      r0[c] = P.label(i, 0, c);
+      r1[c] = P.label(i, 1, c);
+    }
+    auto& e = pairs[{r0, r1}];
+    e.first = {P.dims[(size_t)i * 4 + 0], P.dims[(size_t)i * 4 + 1]};
+    std::vector<Rec> xs(1, Rec(q, 0));
+    for (int c = 0; c < q; ++c) {
+      std::vector<Rec> nx;
+      for (const Rec& base : xs) {
+        if (P.symtype[c] == SYM_A) {
+          Rec b = base;
+          b[c] = r0[c] + r1[c];
+          nx.push_back(b);
+          continue;
+        }
+        const int a = r0[c], b1 = r1[c], c2 = P.label(i, 2, c), c3 = P.label(i, 3, c);
+        for (int J = std::abs(a - b1); J <= a + b1; J += 2) {
+          if (J < std::abs(c2 - c3) || J > c2 + c3 || ((J + c2 + c3) & 1)) continue;
+          Rec b = base;
+          b[c] = J;
+          nx.push_back(b);
+        }
+      }
+      xs.swap(nx);
+    }
+    for (auto& x2 : xs) e.second.insert(x2);
+  }
+  // multiplet space of each fused label: the pairs in sorted order
+  std::map<Rec, int64_t> dimX;
+  HostQ F;
+  struct FB {
+    Rec r0, r1, x;
+    int64_t d0, d1, off;
+  };
+  std::vector<FB> fb;
+  for (auto& [k, v] : pairs)
+    for (const Rec& x : v.second) {
+      int64_t& D = dimX[x];
+      fb.push_back(FB{k.first, k.second, x, v.first.first, v.first.second, D});
+      D += v.first.first * v.first.second;
+    }
+  for (const FB& b : fb) {
+    F.Q.insert(F.Q.end(), b.r0.begin(), b.r0.end());
+    F.Q.insert(F.Q.end(), b.r1.begin(), b.r1.end());
+    F.Q.insert(F.Q.end(), b.x.begin(), b.x.end());
+    const int64_t DX = dimX[b.x];
+    F.dims.insert(F.dims.end(), {b.d0, b.d1, DX});
+    std::vector<double> d((size_t)(b.d0 * b.d1 * DX), 0.0);
+    for (int64_t i1 = 0; i1 < b.d1; ++i1)
+      for (int64_t i0 = 0; i0 < b.d0; ++i0) {
+        const int64_t row = i0 + b.d0 * i1;
+        d[(size_t)(row + b.d0 * b.d1 * (b.off + row))] = 1.0;
+      }
+    F.blocks.push_back(std::move(d));
+  }
+  const uint64_t tagF = pack_name("qsxF") | 0x80;
+  F.tags = {P.itags[0], P.itags[1], tagF};
+  F.qtype = P.qtype;
+  F.bind((int)fb.size(), 3, q, false);
+  // PSI~ = conj(F) . PSI over legs (0, 1): legs (X, 2, 3)
+  qsx_opts o{};
+  o.mode = QSX_MODE_BATCHED;
+  o.verbose = 0;
+  o.skip_zero = 1;
+  const int32_t i01[2] = {0, 1}, i2[1] = {2}, i0[1] = {0};
+  OutPtr Pt(contract_oneshot(&F.t, i01, 2, 1, pt, i01, 2, 0, nullptr, 0, &o));
+  OutView Pv(Pt.get());
+  // the new leg's itag: the K = 2 rule (named after itag, outgoing on A1
+  // when leg 0 is incoming)
+  const uint64_t tx = pack_name(itag);
+  const uint64_t tu = tag_out(P.itags[0]) ? tx : (tx | 0x80);
+  OrthoResult R1 = ortho_oneshot(&Pv.t, 1, Nkmin, Nkeep, stol, "qsxX");
+  OutPtr U(R1.A1);
+  OutView Uv(U.get());
+  OrthoResult R;
+  R.A1 = contract_oneshot(&F.t, i2, 1, 0, &Uv.t, i0, 1, 0, nullptr, 0, &o);
+  R.A2 = R1.A2;
+  R.info = R1.info;
+  R.A1->itags[2] = tu;
+  R.A2->itags[R.A2->rank - 1] = tu ^ 0x80;
   return R;
 }
 

@@ -217,8 +217,8 @@ def test_ortho_argument_errors():
     from qspace_amd import gen
     P = gen.u1_two_site_psi(seed=1, Lq=(-2, 0, 2), Rq=(-2, 0, 2), dmin=1, dmax=2)
     A = Q.QSpace([np.array([[0], [1]], dtype=np.int32) for _ in range(4)], [np.ones((2, 1, 1, 2))] * 2, "SU2",
-                 ["a", "b", "c*", "d*"])  # SU(2) of rank 4: not restated
-    cases = [(lambda: Q.orthoQS(A, 2), 5), (lambda: Q.orthoQS(P, 0), 1), (lambda: Q.orthoQS(P, 4), 1),
+                 ["a", "b", "c*", "d*"])  # SU(2) of rank 4 split other than K = 2: not restated
+    cases = [(lambda: Q.orthoQS(A, 1), 5), (lambda: Q.orthoQS(P, 0), 1), (lambda: Q.orthoQS(P, 4), 1),
              (lambda: Q.orthoQS(P, 3), 2),
              (lambda: Q.orthoQS(Q.QSpace(P.Q, P.data, "A", ["L", "s1", "s2*", "R"]), 2), 2),
              (lambda: Q.orthoQS(Q.QSpace(P.Q, [d * (1 + 1j) for d in P.data], "A", P.itags), 2), 5)]

@@ -47,8 +47,63 @@ def test_su2_ortho_full_size(gpu, kind, K, kw):
 
 
 def test_su2_ortho_unsupported(gpu):
-    A = Q.QSpace([np.array([[0], [1]], dtype=np.int32) for _ in range(4)], [np.ones((2, 1, 1, 2))] * 2, "SU2",
-                 ["a", "b", "c*", "d*"])
-    with pytest.raises(Q.QsxError) as e:
-        Q.orthoQS(A, 2)
-    assert e.value.code == 5
+    """rank 4 splits other than K = 2 are refused"""
+    A = gen.su2_om_tensor(5, [[(0, 2), (1, 1)]] * 4, ["a", "b", "c*", "d*"])
+    B = gen.su2_om_tensor(6, [[(0, 2), (1, 1)]] * 4, ["a", "b", "c", "d*"])
+    for T, K in ((A, 1), (B, 3)):
+        with pytest.raises(Q.QsxError) as e:
+            Q.orthoQS(T, K)
+        assert e.value.code == 5
+
+
+def _two_site(seed, labels, dmin, dmax, tags=("L", "s1", "s2*", "R*")):
+    rng = np.random.RandomState(seed)
+    L = [(t, int(rng.randint(dmin, dmax + 1))) for t in labels]
+    R = [(t, int(rng.randint(dmin, dmax + 1))) for t in labels]
+    return gen.su2_om_tensor(seed, [L, [(1, 1)], [(1, 1)], R], list(tags))
+
+
+@pytest.mark.parametrize("kw", [{"Nkeep": 1000}, {"Nkeep": 12}, {"Nkeep": 30, "stol": 1e-2}])
+def test_su2_two_site_rank4_truncation(gpu, kw):
+    """K = 2 of the rank-4 SU(2) two-site wave function (legs with outer
+    multiplicity; mpsortho.cc:1031-1078 after eigs2SiteQS.cc:481-484), in the
+    full M-resolved space: the singular values, each repeated by its
+    multiplet dimension, are the dense singular values of Full(PSI) split
+    (L, s1 | s2, R); A1 is an isometry; A1.A2 = PSI when all are kept and
+    = sfac times the projection onto A1's columns after a truncation"""
+    from test_oracle_ortho_su2 import _full_parts
+    P = _two_site(61, range(0, 7), 2, 6)
+    assert max(P.om()) > 1
+    A1, A2, info = Q.orthoQS(P, 2, itag="X", **kw)
+    print(f"rank-4 K=2 {kw}: Ntot={info['Ntot']} Nkeep={info['Nkeep']}", flush=True)
+    assert A1.itags == ["L", "s1", "X*"] and A2.itags == ["s2*", "R*", "X"]
+    FP, F1, F2 = _full_parts(P, A1, A2, 2)
+    d1 = int(np.prod(FP.shape[:2]))
+    s = np.linalg.svd(FP.reshape(d1, -1), compute_uv=False)
+    full = np.sort(np.repeat(info["svd"], info["svd_qdim"].astype(int)))[::-1]
+    assert np.allclose(full, s[:len(full)], rtol=1e-12, atol=1e-12 * s[0])
+    assert np.all(s[len(full):] < 1e-12 * s[0])
+    assert info["sv2tot"] == pytest.approx(float(np.sum(FP * FP)), rel=1e-12)
+    M1 = F1.reshape(-1, F1.shape[-1])
+    assert np.allclose(M1.T @ M1, np.eye(M1.shape[1]), atol=1e-12)
+    X = np.tensordot(F1, F2, axes=([2], [2]))
+    proj = (M1 @ (M1.T @ FP.reshape(d1, -1))).reshape(FP.shape)
+    assert np.allclose(X, info["sfac"] * proj, atol=1e-12 * np.abs(FP).max())
+    if kw["Nkeep"] >= info["Ntot"]:
+        assert info["Nkeep"] == info["Ntot"] and info["sfac"] == 1.0
+        assert np.allclose(X, FP, atol=1e-12 * np.abs(FP).max())
+    else:
+        assert info["Nkeep"] <= kw["Nkeep"]
+        assert M1.shape[1] == int(np.sum(info["svd_qdim"][:info["Nkeep"]]))
+
+
+def test_su2_two_site_rank4_truncation_large(gpu):
+    """the same at 2S <= 12 with dims up to 24 per multiplet: A1.A2 = PSI
+    block by block (contracting the results on the device, compared with
+    PSI to 1e-12) and the kept weight"""
+    P = _two_site(62, range(0, 13), 8, 24)
+    A1, A2, info = Q.orthoQS(P, 2, Nkeep=100000, itag="X")
+    assert info["Nkeep"] == info["Ntot"]
+    B = Q.contractQS(A1, "3", A2, "3")
+    from test_gpu_om import assert_same_by_record
+    assert_same_by_record(B, P)
