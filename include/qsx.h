@@ -418,6 +418,9 @@ int qsx_su2_x3(int32_t ra, const int32_t* tja, const int32_t* dira,
    (count only); returns om >= 0, or -code on error. */
 int qsx_su2_om_basis(int32_t r, const int32_t* tj, const int32_t* dir, double* out, int64_t outlen,
                      char* err, size_t errlen);
+/* canonical (sorted QSet) leg order of a block CGT (QSet::Sort,
+   Source/clebsch.cc:1435-1505): ord[k] = block leg at position k */
+int qsx_su2_canon_order(int32_t r, const int32_t* tj, const int32_t* dir, int32_t* ord);
 /* Matrix-valued X-symbol of two blocks' bases (X3Map::contractCGR with
    outer multiplicity, Source/clebsch_aux.cc:4464-4471): x[(a*ob + b)*oc + c]
    = <C_c | contract(A_a, B_b)>, the kept legs [A | B] permuted by perm

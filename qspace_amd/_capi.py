@@ -100,6 +100,7 @@ def lib():
         "qsx_su2_cg": ([i32] * 6, f64),
         "qsx_su2_x3": ([i32, P(i32), P(i32), i32, P(i32), P(i32), P(i32), P(i32), i32, P(f64), cs, sz], C.c_int),
         "qsx_su2_om_basis": ([i32, P(i32), P(i32), P(f64), i64, cs, sz], C.c_int),
+        "qsx_su2_canon_order": ([i32, P(i32), P(i32), P(i32)], C.c_int),
         "qsx_su2_xom": ([i32, P(i32), P(i32), i32, P(i32), P(i32), P(i32), P(i32), i32, P(i32), P(i32), P(f64), i64,
                          cs, sz], C.c_int),
     }
@@ -120,7 +121,8 @@ def exported_symbols() -> list[str]:
             "qsx_ctr_plan_pairs", "qsx_ctr_plan_idxset", "qsx_ctr_execute", "qsx_ctr_plan_flops", "qsx_ctr_plan_set_policy", "qsx_ctr_plan_destroy", "qsx_block_zero_flags", "qsx_block_zero_flags_noalloc",
             "qsx_hpsi_plan_create", "qsx_hpsi_plan_create_pseudo", "qsx_hpsi_size", "qsx_hpsi_flops", "qsx_hpsi_bytes", "qsx_hpsi_apply",
             "qsx_hpsi_use_graph", "qsx_hpsi_weights", "qsx_hpsi_stage_info", "qsx_hpsi_autotune", "qsx_hpsi_set_policy", "qsx_hpsi_last_timing", "qsx_hpsi_stage_ms", "qsx_hpsi_plan_destroy", "qsx_lanczos", "qsx_davidson",
-            "qsx_su2_cg", "qsx_su2_x3", "qsx_su2_om_basis", "qsx_su2_xom"]
+            "qsx_su2_cg", "qsx_su2_x3", "qsx_su2_om_basis", "qsx_su2_xom",
+            "qsx_su2_canon_order"]
 
 
 def check(rc: int, err) -> None:

@@ -17,7 +17,7 @@ from . import _capi
 from .qspace import QSpace, QsxCell, QsxOpts, QsxTensor, QsxTensorOut, from_out, parse_ctridx, to_c
 
 __all__ = ["contractQS", "permuteQS", "plusQS", "orthoQS", "ortho_args", "itag_match", "DeviceBuffer", "CtrPlan", "HPsi", "su2_cg", "su2_x3", "su2_om_basis",
-           "su2_xom", "make_opts"]
+           "su2_xom", "su2_canon_order", "make_opts"]
 
 
 def make_opts(mode="batched", nslices=0, skip_zero=1, hconj=0, fermA=None, fermB=None, verbose=1):
@@ -646,6 +646,17 @@ def su2_om_basis(tj, dirs) -> np.ndarray:
             _capi.check(-rc, err)
     return np.stack([out[a * n:(a + 1) * n].reshape(shape, order="F") for a in range(om)]) if om else \
         np.zeros((0,) + shape)
+
+
+def su2_canon_order(tj, dirs) -> list:
+    """canonical leg order of a block CGT: ord[k] = block leg at position k"""
+    t, d = _ivec(tj), _ivec(dirs)
+    o = np.zeros(max(1, len(tj)), np.int32)
+    P = C.POINTER
+    rc = _capi.lib().qsx_su2_canon_order(len(tj), t.ctypes.data_as(P(C.c_int32)), d.ctypes.data_as(P(C.c_int32)),
+                                         o.ctypes.data_as(P(C.c_int32)))
+    _capi.check(rc, None)
+    return [int(x) for x in o[:len(tj)]]
 
 
 def su2_xom(tja, dira, tjb, dirb, ia, ib, perm=None) -> np.ndarray:

@@ -1490,6 +1490,13 @@ int qsx_su2_om_basis(int32_t r, const int32_t* tj, const int32_t* dir, double* o
   return rc ? -rc : om;
 }
 
+int qsx_su2_canon_order(int32_t r, const int32_t* tj, const int32_t* dir, int32_t* ord) {
+  if (r < 0 || r > QSX_MAX_RANK || (r && (!tj || !dir || !ord))) return QSX_ERR_ARG;
+  std::vector<int> o = su2_canon_order(std::vector<int>(tj, tj + r), std::vector<int>(dir, dir + r));
+  std::copy(o.begin(), o.end(), ord);
+  return 0;
+}
+
 int qsx_su2_xom(int32_t ra, const int32_t* tja, const int32_t* dira, int32_t rb, const int32_t* tjb,
                 const int32_t* dirb, const int32_t* ia, const int32_t* ib, int32_t nctr, const int32_t* perm,
                 int32_t* dims3, double* x, int64_t xlen, char* err, size_t errlen) {

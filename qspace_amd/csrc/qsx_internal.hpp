@@ -100,6 +100,7 @@ struct SU2Basis {
 };
 std::shared_ptr<const SU2Basis> su2_om_basis(const std::vector<int>& tj, const std::vector<int>& dir);
 int su2_om(const std::vector<int>& tj, const std::vector<int>& dir);
+std::vector<int> su2_canon_order(const std::vector<int>& tj, const std::vector<int>& dir);
 // contraction coefficients of basis tensors, x[(a*ob + b)*oc + c]
 struct SU2XTensor {
   int oa = 0, ob = 0, oc = 0;

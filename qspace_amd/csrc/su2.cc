@@ -880,6 +880,12 @@ std::shared_ptr<const SU2Basis> su2_om_basis(const std::vector<int>& tj, const s
 
 int su2_om(const std::vector<int>& tj, const std::vector<int>& dir) { return su2_om_basis(tj, dir)->om; }
 
+std::vector<int> su2_canon_order(const std::vector<int>& tj, const std::vector<int>& dir) {
+  std::vector<int> ord, fl;
+  canon_order(tj, dir, ord, fl);
+  return ord;
+}
+
 // X[a][b][c] = <C_c | P(contract(A_a, B_b))>: A's legs ia contracted with
 // B's legs ib, the kept legs [A | B] permuted by perm (new leg l = old leg
 // perm[l]), C the basis of the permuted output
