@@ -28,6 +28,11 @@ X3Map::contractDegQ, which moves the sign of the transposition into cgw
 The reader checks each record's qset / qdir against the block's labels and
 leg directions and raises ValueError on a mismatch (a CGT in another gauge,
 which this format cannot represent).
+Outer multiplicity: a block with one data dimension more than the rank
+holds coefficients on a basis of its CGT space; this library reads and
+writes them on its own basis (files marked info.otype = OM_OTYPE) and
+refuses them in other files, where they refer to the reference's RC_STORE
+basis (rcstore.basis_overlap converts them).
 Write: the same struct with cgr records holding type, qset (the block's
 labels of that symmetry in leg order), qdir, size, nnz and cgw; cid and cgt
 are left empty (the reference recomputes CGTs from its own store).
@@ -38,7 +43,12 @@ import numpy as np
 
 from .qspace import QSpace
 
-__all__ = ["qspace_from_mx", "qspace_to_mx", "load_qspace", "save_qspace"]
+__all__ = ["qspace_from_mx", "qspace_to_mx", "load_qspace", "save_qspace", "OM_OTYPE"]
+
+# info.otype of files whose blocks carry outer-multiplicity coefficients on
+# this library's basis (qsx_su2_om_basis); such a trailing data dimension in
+# any other file holds coefficients on the reference's own (RC_STORE) basis
+OM_OTYPE = "qsx-om"
 
 
 def _cell_list(a):
@@ -94,13 +104,18 @@ def qspace_from_mx(S) -> QSpace:
     dcells = _cell_list(aD)
     if len(dcells) != n:
         raise ValueError(f"dimension mismatch (QIDX: {n} records, data: {len(dcells)})")
+    ot = _field(aI, "otype") if aI is not None and np.asarray(aI).size else None
+    own_om = ot is not None and _str(ot) == OM_OTYPE
     data = []
     for d in dcells:
         a = np.asarray(d)
         a = a.astype(np.complex128 if np.iscomplexobj(a) else np.float64, copy=False)
-        if a.ndim > r:
-            raise ValueError(f"data block of rank {a.ndim} > {r}")
-        data.append(np.asfortranarray(a.reshape(a.shape + (1,) * (r - a.ndim), order="F")))
+        if a.ndim > r + 1 or (a.ndim == r + 1 and not own_om):
+            raise ValueError(f"data block of rank {a.ndim} > {r}: outer-multiplicity coefficients on the "
+                             f"reference's RC_STORE basis (convert them with rcstore.basis_overlap)")
+        if a.ndim <= r:
+            a = a.reshape(a.shape + (1,) * (r - a.ndim), order="F")
+        data.append(np.asfortranarray(a))
     qtype, itags, cgw = "", [], None
     if aI is not None and np.asarray(aI).size:
         qtype = _str(_field(aI, "qtype") if _field(aI, "qtype") is not None else "")
@@ -193,7 +208,9 @@ def qspace_to_mx(T: QSpace) -> dict:
     for k, t in enumerate(T.itags):
         itags[0, k] = t
     syms = [x.strip() for x in T.qtype.split(",")] if T.qtype else []
-    info = {"qtype": T.qtype, "otype": "", "itags": itags, "fdir": np.zeros((0, 0)), "ctime": 0.0}
+    has_om = n and r and int(max(T.om())) > 1
+    info = {"qtype": T.qtype, "otype": OM_OTYPE if has_om else "", "itags": itags, "fdir": np.zeros((0, 0)),
+            "ctime": 0.0}
     if "SU2" in syms and n:
         dirs = [(-1 if t.endswith("*") else 1) for t in T.itags] if T.itags else [1] * r
         fields = ["type", "qset", "qdir", "cid", "size", "nnz", "cgw", "cgt"]

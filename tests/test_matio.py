@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from qspace_amd import QSpace, gen
+from qspace_amd import QSpace, gen, matio
 from qspace_amd.matio import _norm_ext, load_qspace, qspace_from_mx, qspace_to_mx, save_qspace
 
 
@@ -110,3 +110,21 @@ def test_complex_round_trip(tmp_path):
     assert R.is_complex()
     for a, b in zip(R.data, T.data):
         assert np.array_equal(a, b)
+
+
+def test_outer_multiplicity_round_trip_and_foreign_files(tmp_path):
+    """blocks with an outer-multiplicity dimension round-trip through files
+    marked as this library's basis; the same data without the mark (the
+    reference's RC_STORE basis) are refused"""
+    from qspace_amd import gen
+    T = gen.su2_om_tensor(3, [[(0, 2), (1, 1), (2, 2)]] * 4, ["a", "b", "c*", "d*"])
+    assert max(T.om()) > 1
+    p = str(tmp_path / "om.mat")
+    matio.save_qspace(p, {"T": T})
+    U = matio.load_qspace(p, "T")
+    assert all(np.array_equal(a, b) for a, b in zip(T.data, U.data))
+    assert np.array_equal(U.om(), T.om())
+    S = matio.qspace_to_mx(T)
+    S["info"]["otype"] = ""
+    with pytest.raises(ValueError, match="outer-multiplicity"):
+        matio.qspace_from_mx(S)
