@@ -37,6 +37,7 @@
 #include <dlfcn.h>
 #include <map>
 #include <mutex>
+#include <sstream>
 #include <numeric>
 #include <omp.h>
 #include <string>
@@ -86,33 +87,33 @@ struct Tensor {
 };
 
 // qtype string -> symmetry list (QType::init_s, clebsch.cc:118-164; only
-// U(1) "A" and "SU2" are in scope)
+// U(1) "A" and "SU2" are in scope): comma separated names, blanks ignored,
+// empty = QDIM abelian labels
 static void parse_qtype(const char* s, int qdim, std::vector<int>& st) {
   st.clear();
-  std::string q = s ? s : "";
-  if (q.empty()) {
-    for (int i = 0; i < qdim; ++i) st.push_back(SYM_A);
+  std::string all;
+  for (const char* c = s; c && *c; ++c)
+    if (*c != ' ') all.push_back(*c);
+  if (all.empty()) {
+    st.assign(qdim, SYM_A);
     return;
   }
-  size_t p = 0;
-  while (p <= q.size()) {
-    size_t e = q.find(',', p);
-    if (e == std::string::npos) e = q.size();
-    std::string t = q.substr(p, e - p);
-    while (!t.empty() && t.front() == ' ') t.erase(t.begin());
-    while (!t.empty() && t.back() == ' ') t.pop_back();
-    if (t == "A" || t == "U1") st.push_back(SYM_A);
-    else if (t == "SU2") st.push_back(SYM_SU2);
-    else fail(QSX_ERR_UNSUPPORTED, "unsupported symmetry '%s' in qtype '%s'", t.c_str(), q.c_str());
-    p = e + 1;
+  std::istringstream in(all);
+  std::string name;
+  while (std::getline(in, name, ',')) {
+    static const std::map<std::string, int> known = {{"A", SYM_A}, {"U1", SYM_A}, {"SU2", SYM_SU2}};
+    auto it = known.find(name);
+    if (it == known.end()) fail(QSX_ERR_UNSUPPORTED, "unsupported symmetry '%s' in qtype '%s'", name.c_str(), s);
+    st.push_back(it->second);
   }
-  if ((int)st.size() != qdim)
-    fail(QSX_ERR_QDIM, "qtype '%s' has %d symmetries but QDIM=%d", q.c_str(), (int)st.size(), qdim);
+  if (!all.empty() && all.back() == ',') fail(QSX_ERR_UNSUPPORTED, "empty symmetry name in qtype '%s'", s);
+  if ((int)st.size() != qdim) fail(QSX_ERR_QDIM, "qtype '%s' has %d symmetries but QDIM=%d", s, (int)st.size(), qdim);
 }
 
 static Tensor from_c(const qsx_tensor* t, const char* name) {
   if (!t) fail(QSX_ERR_ARG, "null tensor %s", name);
   if (t->data_im) fail(QSX_ERR_UNSUPPORTED, "tensor %s: complex data (combined from real contractions by the caller)", name);
+  if (t->om) fail(QSX_ERR_UNSUPPORTED, "tensor %s: outer multiplicity (restated in om.py)", name);
   Tensor T;
   T.n = t->nblk;
   T.r = t->rank;

@@ -33,6 +33,10 @@ Parity status of the oracle's functions (what pins each one):
                                          cases; SU(2) K = 1, r-1 in the full
                                          M-resolved space, weights other than
                                          normExt included
+  * struct encoding / decoding         -- its own (abi_io.py), pinned against
+                                         the reference's itag layout by known
+                                         answers and against the product's
+                                         encoder byte for byte
   * reference's FP64 summation order / MKL dgemm rounding: parity unpinned
     (no reference build; tolerance 1e-12 relative instead of bit equality)
 """
@@ -44,7 +48,8 @@ import subprocess
 
 import numpy as np
 
-from qspace_amd.qspace import QSpace, QsxTensor, QsxTensorOut, from_out, to_c
+from abi_io import Tensor as QsxTensor, TensorOut as QsxTensorOut, decode as from_out, encode as to_c
+from qspace_amd.qspace import QSpace
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
