@@ -427,6 +427,7 @@ static void choose_layouts(CtrPlan& P) {
 }
 
 static void build_structure(CtrPlan& P, const int32_t* perm, int nperm, const double* fA, const double* fB) {
+  PhaseClock clk{P.verbose, "contract plan"};
   check_and_tags(P);
   TensorDesc &A = P.A, &B = P.B;
   const int q = A.qdim, k = (int)P.ica.size();
@@ -445,7 +446,9 @@ static void build_structure(CtrPlan& P, const int32_t* perm, int nperm, const do
     for (size_t l = 0; l < P.kb.size(); ++l)
       for (int j = 0; j < q; ++j) QBk[(size_t)i * kmb + l * q + j] = B.label(i, P.kb[l], j);
   }
+  clk.lap("label records");
   P.L = device_label_engine(QAc, QAk, A.n, QBc, QBk, B.n, kc, kma, kmb, default_stream());
+  clk.lap("device label engine");
   const int nL = (int)P.L.seg.size() - 1;
   const size_t np = P.L.Ia.size();
   // output permutation (completed with the identity, bin/contractQS.m:154-170)
@@ -532,6 +535,7 @@ static void build_structure(CtrPlan& P, const int32_t* perm, int nperm, const do
     }
   }
   if (bad.load()) throw first;
+  clk.lap("X-symbols");
   // layouts (may reorder ica/icb jointly; afac is independent of that order)
   choose_layouts(P);
   // executed output blocks, pair descriptors, output normalisation
@@ -692,6 +696,7 @@ static void build_structure(CtrPlan& P, const int32_t* perm, int nperm, const do
     for (int l = 0; l < rc; ++l) t2[l] = P.tagsC[P.perm[l]];
     P.tagsC = t2;
   }
+  clk.lap("pairs and output blocks");
 }
 
 // block permutation plan of n blocks of rank r whose outer-multiplicity
@@ -713,6 +718,7 @@ static void perm_plan_om(PermPlan& pp, int n, int r, const int64_t* dims, const 
 }
 
 void plan_upload(CtrPlan& P) {
+  PhaseClock clk{P.verbose, "contract plan upload"};
   hipStream_t s = default_stream();
   dev_free(P.dpairs);
   P.dpairs = (PairDesc*)dev_alloc(sizeof(PairDesc) * std::max<size_t>(1, P.pairs.size()));
@@ -720,7 +726,9 @@ void plan_upload(CtrPlan& P) {
     QSX_HIP(hipMemcpyAsync(P.dpairs, P.pairs.data(), sizeof(PairDesc) * P.pairs.size(), hipMemcpyHostToDevice, s));
   std::vector<WorkTarget> targets(P.nT);
   for (int t = 0; t < P.nT; ++t) targets[t] = WorkTarget{P.tOff[t], P.M[t], P.N[t], P.pseg[t], P.pseg[t + 1]};
+  clk.lap("pairs to device");
   P.work = build_work(targets, P.pairs, P.policy);
+  clk.lap("build_work");
   dev_free(P.ditems);
   dev_free(P.dreds);
   dev_free(P.dqueues);
@@ -735,6 +743,7 @@ void plan_upload(CtrPlan& P) {
   P.dscratch = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, P.work.scratch_elems));
   if (!P.work.items.empty())
     QSX_HIP(hipMemcpyAsync(P.ditems, P.work.items.data(), sizeof(WorkItem) * P.work.items.size(), hipMemcpyHostToDevice, s));
+  clk.lap("work list to device");
   if (P.la.permute && !P.a_prepermuted) {
     perm_plan_om(P.ppA, P.A.n, P.A.r, P.A.dims.data(), P.A.om, P.A.offs.data(), P.la.perm);
     perm_plan_upload(P.ppA, s);
@@ -757,7 +766,9 @@ void plan_upload(CtrPlan& P) {
     dev_free(P.dCtmp);
     P.dCtmp = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, P.offsOut.back()));
   }
+  clk.lap("permutation plans");
   QSX_HIP(hipStreamSynchronize(s));
+  clk.lap("sync");
 }
 
 void plan_permute_operand(const CtrPlan& P, int which, const double* src, double* dst, hipStream_t s) {
@@ -794,7 +805,9 @@ std::unique_ptr<CtrPlan> plan_create(const qsx_tensor* A, const int32_t* ia, int
   if (P->mode == QSX_MODE_SPLIT_BF16 || P->mode == QSX_MODE_SPLIT_F32)
     fail(QSX_ERR_UNSUPPORTED, "split precision is available for H*psi plans only (QSX_MODE_SPLIT_BF16)");
   build_structure(*P, perm, nperm, o ? o->fermA : nullptr, o ? o->fermB : nullptr);
+  PhaseClock clk{P->verbose, "contract plan"};
   if (upload) plan_upload(*P);
+  clk.lap("work list + upload");
   return P;
 }
 
@@ -1086,7 +1099,9 @@ qsx_tensor_out* contract_oneshot(const qsx_tensor* A, const int32_t* ia, int na,
   }
   if (A && B && (A->data_im || B->data_im))
     return contract_oneshot_complex(A, ia, na, conjA, B, ib, nb, conjB, perm, nperm, o);
+  PhaseClock clk{o ? o->verbose : 0, "contract"};
   auto P = plan_create(A, ia, na, conjA, B, ib, nb, conjB, perm, nperm, o, true);
+  clk.lap("plan (structure + upload)");
   double *dA = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, P->A.total()));
   double *dB = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, P->B.total()));
   double *dC = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, P->offsOut.back()));
@@ -1094,8 +1109,10 @@ qsx_tensor_out* contract_oneshot(const qsx_tensor* A, const int32_t* ia, int na,
   int64_t* doffs = nullptr;
   qsx_tensor_out* out = nullptr;
   try {
+    clk.lap("device buffers");
     upload_tensor(P->A, dA, s);
     upload_tensor(P->B, dB, s);
+    clk.lap("upload A, B");
     hipEvent_t e0, e1;
     QSX_HIP(hipEventCreate(&e0));
     QSX_HIP(hipEventCreate(&e1));
@@ -1107,6 +1124,7 @@ qsx_tensor_out* contract_oneshot(const qsx_tensor* A, const int32_t* ia, int na,
     QSX_HIP(hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    clk.lap("execute");
     std::vector<int> keep(P->nC);
     std::iota(keep.begin(), keep.end(), 0);
     const int skip_zero = o ? o->skip_zero : 1;
@@ -1134,10 +1152,12 @@ qsx_tensor_out* contract_oneshot(const qsx_tensor* A, const int32_t* ia, int na,
           if (!z[c]) keep.push_back(c);
       }
     }
+    clk.lap("SkipZeroData");
     static const double kNone = 0.0;
     download_staged(dC, P->offsOut.back(), s,
                     [&](const double* h) { out = make_out(*P, keep, h ? h : &kNone); });
     out->gpu_ms = ms;
+    clk.lap("download + result");
   } catch (...) {
     dev_free(dA);
     dev_free(dB);

@@ -550,18 +550,6 @@ static void hpsi_plain(HpsiPlan& H, const double* dpsi, double* dhpsi, hipStream
   run(H.pairs2, H.targets2, H.dT, H.dHR, dhpsi);
 }
 
-// host wall-clock phases of the plan build (printed with verbose >= 2)
-struct PhaseClock {
-  int verbose;
-  std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
-  void lap(const char* what) {
-    auto n = std::chrono::steady_clock::now();
-    if (verbose >= 2)
-      fprintf(stderr, "qsx H*psi plan: %-28s %8.3f ms\n", what, std::chrono::duration<double, std::milli>(n - t).count());
-    t = n;
-  }
-};
-
 // The plan of H*psi = [HK_L.psi] + [psi.HK_R] + sum_t contractQS({L_t[*], psi}, R_t[*]):
 // stage 1 computes the intermediates T_t = L_t[*] . psi of every term, stage
 // 2 every block of H*psi as one sum over the local terms' pairs (operand psi
@@ -581,7 +569,7 @@ static void hpsi_create(HpsiPlan& H, const std::vector<const qsx_tensor*>& Ls, c
       fail(QSX_ERR_ARG, "H*psi: term operator missing");
   if (terms.empty() && !hkl && !hkr) fail(QSX_ERR_ARG, "H*psi: no terms");
   ensure_device();
-  PhaseClock clk{o ? o->verbose : 0};
+  PhaseClock clk{o ? o->verbose : 0, "H*psi plan"};
   hipStream_t s = default_stream();
   for (size_t i = 0; i < Ls.size(); ++i) H.Ls.push_back(make_desc(Ls[i], "HL"));
   for (size_t i = 0; i < Rs.size(); ++i) H.Rs.push_back(make_desc(Rs[i], "HR"));

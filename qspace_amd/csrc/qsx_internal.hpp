@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cstdint>
 #include <cstdio>
 #include <string>
@@ -57,6 +58,20 @@ struct TensorDesc {
   // block size of one outer-multiplicity component
   int64_t slice(int blk) const { return numel(blk) / om_of(blk); }
 };
+// host wall-clock phases of a call (printed to stderr with verbose >= 2)
+struct PhaseClock {
+  int verbose;
+  const char* what;
+  std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+  void lap(const char* phase) {
+    auto n = std::chrono::steady_clock::now();
+    if (verbose >= 2)
+      fprintf(stderr, "qsx %s: %-28s %8.3f ms\n", what, phase,
+              std::chrono::duration<double, std::milli>(n - t).count());
+    t = n;
+  }
+};
+
 // entry points without outer-multiplicity support refuse such tensors
 void require_no_om(const TensorDesc& T, const char* what);
 

@@ -183,13 +183,25 @@ class QSpace:
         return np.ascontiguousarray(np.concatenate(self.Q, axis=1))
 
     def dims(self) -> np.ndarray:
-        return np.array([d.shape[: self.rank] for d in self.data], dtype=np.int64).reshape(self.nblk, self.rank)
+        r = self.rank
+        return np.array([d.shape[:r] for d in self.data], dtype=np.int64).reshape(len(self.data), r)
 
     def om(self) -> np.ndarray:
         """per-block outer multiplicity (size of the trailing data dimension
         beyond the rank, 1 if there is none)"""
         r = self.rank
         return np.array([d.shape[r] if d.ndim > r else 1 for d in self.data], dtype=np.int32)
+
+    @classmethod
+    def _from_parts(cls, Q, data, qtype, itags, cgw) -> "QSpace":
+        """a QSpace from parts the library produced (int32 label matrices,
+        column-major float64 / complex128 blocks of full rank): no per-block
+        conversion"""
+        T = cls.__new__(cls)
+        T.Q = Q
+        T.data = data
+        T.info = {"qtype": qtype, "itags": list(itags), "cgw": cgw}
+        return T
 
     def numel(self) -> int:
         return int(sum(d.size for d in self.data))
@@ -198,7 +210,7 @@ class QSpace:
         return float(np.sqrt(sum(float(np.sum(np.abs(d) ** 2)) for d in self.data)))
 
     def is_complex(self) -> bool:
-        return any(np.iscomplexobj(d) for d in self.data)
+        return any(d.dtype.kind == "c" for d in self.data)
 
     def packed(self) -> np.ndarray:
         """blocks back to back, column-major (the device layout)."""
@@ -276,7 +288,7 @@ def to_c(T: QSpace, with_data: bool = True):
         ct.cgw = cg.ctypes.data_as(C.POINTER(C.c_double))
     else:
         ct.cgw = None
-    om = T.om() if n and r else np.ones(0, np.int32)
+    om = T.om() if n and r and any(d.ndim > r for d in T.data) else np.ones(0, np.int32)
     if om.size and om.max() > 1:
         keep.append(om)
         ct.om = om.ctypes.data_as(C.POINTER(C.c_int32))
@@ -312,9 +324,10 @@ def from_out(o: QsxTensorOut, with_data: bool = True) -> QSpace:
     cgw = None
     if o.nsym and o.cgw:
         cgw = np.ctypeslib.as_array(o.cgw, shape=(max(1, n * o.nsym),))[: n * o.nsym].copy().reshape(n, o.nsym)
-    Qlist = [Qflat[:, k * q:(k + 1) * q] for k in range(r)]
-    T = QSpace(Qlist, data, o.qtype.decode(), itags, cgw) if r else _scalar_qspace(data, o.qtype.decode(), cgw)
-    return T
+    Qlist = [np.ascontiguousarray(Qflat[:, k * q:(k + 1) * q]) for k in range(r)]
+    if not r:
+        return _scalar_qspace(data, o.qtype.decode(), cgw)
+    return QSpace._from_parts(Qlist, data, o.qtype.decode(), itags, cgw)
 
 
 def _scalar_qspace(data, qtype, cgw):
