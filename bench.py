@@ -323,9 +323,21 @@ def extra_configs():
     Hs = Q.HPsi(HL, psi, HR, hconj=1, mode="split_bf16")
     default_err = float(np.linalg.norm(Hs.apply(xv) - oref) / np.linalg.norm(oref))
     del Hs
+    # FP32 x 3 slices (FP32 matrix cores, FP32 accumulation, FP64 recombination)
+    Hf = Q.HPsi(HL, psi, HR, hconj=1, mode="split_f32")
+    got = Hf.apply(xv)
+    ms_f32 = time_hpsi(Hf, x, y, 20, 3)
+    ef, _, _ = Hf.lanczos(xv, 20)
+    inf = Hf.stage_info()
+    f32 = {"rel_err_hpsi_vs_oracle": float(np.linalg.norm(got - oref) / np.linalg.norm(oref)),
+           "rel_err_hpsi_vs_gpu_fp64": float(np.linalg.norm(got - ref) / np.linalg.norm(ref)),
+           "E0_rel_err_lanczos20": float(abs(ef - e_ref) / abs(e_ref)), "hpsi_ms": ms_f32,
+           "executed_slice_gflop": float(sum(inf["split_exec_flops"]) / 1e9),
+           "stated_bound_rel_err": 5e-10}
+    del Hf
     out["4_split_bf16_heis4000"] = {"fp64_hpsi_ms": ms_fp64, "fp64_rel_err_vs_oracle": fp64_err,
                                     "E0_fp64_lanczos20": e_ref, "default_slices_rel_err_vs_oracle": default_err,
-                                    "slices": table}
+                                    "slices": table, "fp32x3": f32}
     log(f"[bench] config 4: {out['4_split_bf16_heis4000']}")
     # 5: SU(2)xU(1) Hubbard H*psi at D*=4000 with parity signs, 30 Davidson iterations
     w = WORKLOADS["hubbard4000"]
@@ -449,6 +461,25 @@ def truncation():
     return out
 
 
+def split_profile(mode, steps, warmup):
+    """only the split-precision H*psi of the headline operator (for profiler
+    runs of its kernels): time per apply and executed slice flops"""
+    import qspace_amd as Q
+    w = WORKLOADS["heis4000"]
+    HL, psi, HR, ferm = make_workload(w)
+    H = Q.HPsi(HL, psi, HR, hconj=1, mode=f"split_{mode}")
+    calib = calibration_copy()
+    x = Q.DeviceBuffer.from_array(psi.packed())
+    y = Q.DeviceBuffer(H.N)
+    ms = time_hpsi(H, x, y, steps, warmup, graph=False)
+    inf = H.stage_info()
+    fl = float(sum(inf["split_exec_flops"]))
+    peak = 2500.0 if mode == "bf16" else 157.3
+    return {"split_mode": mode, "workload": w["name"], "hpsi_ms": ms, "executed_slice_gflop": fl / 1e9,
+            "slice_tflops": fl / (ms * 1e-3) / 1e12, "peak_tflops": peak,
+            "frac_of_peak": fl / (ms * 1e-3) / 1e12 / peak, "useful_gflop": H.flops / 1e9, "calibration": calib}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -462,6 +493,9 @@ def main():
                     help="only the block operations (permuteQS / plusQS, environment update): one JSON line")
     ap.add_argument("--policy", default=None,
                     help="JSON file {stage1: {...}, stage2: {...}} fixing the work decomposition (no autotune)")
+    ap.add_argument("--save-policy", default=None, help="write the autotuned policy to this JSON file")
+    ap.add_argument("--split-profile", default=None, choices=["bf16", "f32"],
+                    help="only the split-precision H*psi (for profiler runs): one JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (got --gpus {args.gpus}); "
@@ -474,6 +508,9 @@ def main():
     if rc:
         print(f"bench.py: no GPU: {err.value.decode()}", file=sys.stderr)
         sys.exit(1)
+    if args.split_profile:
+        print(json.dumps(split_profile(args.split_profile, args.steps, args.warmup)), flush=True)
+        return
     if args.block_ops:
         bops = block_ops()
         bops["u1_env_update"] = env_update()
@@ -502,6 +539,9 @@ def main():
         tune = H.autotune(5)
     tune_s = time.perf_counter() - t0
     log(f"[bench] autotune {tune_s:.2f} s: {tune}")
+    if args.save_policy:
+        with open(args.save_policy, "w") as f:
+            json.dump({"stage1": tune["policy"][0], "stage2": tune["policy"][1]}, f)
     info = H.stage_info()
     log(f"[bench] plan {plan_s:.2f} s; N={H.N} flops/H*psi={H.flops / 1e9:.3f} GFLOP pairs={info['pairs']} "
         f"items={info['tiles']} kernels={info['kernel']} grid={info['grid']} reduce={info['reduce_grid']}")

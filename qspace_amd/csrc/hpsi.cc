@@ -81,29 +81,37 @@ struct HpsiPlan {
   bool g_sym = false;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   float last_ms[2] = {0, 0};
-  // split precision (QSX_MODE_SPLIT_BF16): BF16 slice planes and line
-  // exponents of the four operand arenas, slice-addressed pairs, 16x16 tiles
+  // split precision (QSX_MODE_SPLIT_BF16 / _F32): BF16 or FP32 slice planes
+  // and line exponents of the four operand arenas, slice-addressed pairs,
+  // 16x16 tiles
   struct SplitSide {
     uint16_t* planes = nullptr;
+    float* fplanes = nullptr;
     int64_t plane = 0;
     int32_t* exps = nullptr;
     SplitLine* dlines = nullptr;
     int nlines = 0;
+    bool lane = false;  // 16 lines per wave (lines adjacent, strided along k)
   };
   int nsl = 6;
   SplitSide spHL, spPsi, spT, spHR;
   SplitPair *dsp1 = nullptr, *dsp2 = nullptr, *dsp1s = nullptr, *dsp2s = nullptr;
   WorkList swork1, swork2;
   WorkItem *dsitems1 = nullptr, *dsitems2 = nullptr;
+  ReduceItem *dsreds1 = nullptr, *dsreds2 = nullptr;  // k-chunked tiles of the split work lists
+  double* dsscratch = nullptr;
   double split_exec_flops[2] = {0, 0};
   ~HpsiPlan() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
     for (SplitSide* sp : {&spHL, &spPsi, &spT, &spHR}) {
       dev_free(sp->planes);
+      dev_free(sp->fplanes);
       dev_free(sp->exps);
       dev_free(sp->dlines);
     }
-    for (void* p : {(void*)dsp1, (void*)dsp2, (void*)dsp1s, (void*)dsp2s, (void*)dsitems1, (void*)dsitems2}) dev_free(p);
+    for (void* p : {(void*)dsp1, (void*)dsp2, (void*)dsp1s, (void*)dsp2s, (void*)dsitems1, (void*)dsitems2,
+                    (void*)dsreds1, (void*)dsreds2, (void*)dsscratch})
+      dev_free(p);
     dev_free(dHL);
     dev_free(dHR);
     dev_free(dT);
@@ -441,10 +449,14 @@ static void split_check_lines(const std::vector<SplitLine>& lines, int64_t plane
 }
 
 static void split_side_alloc(HpsiPlan::SplitSide& sp, const std::vector<SplitLine>& lines, int64_t plane, int S,
-                             hipStream_t s) {
+                             hipStream_t s, bool f32) {
   sp.plane = plane;
   sp.nlines = (int)lines.size();
-  sp.planes = (uint16_t*)dev_alloc(sizeof(uint16_t) * std::max<int64_t>(1, plane * S));
+  int64_t strided = 0;
+  for (const SplitLine& l : lines) strided += l.sk != 1;
+  sp.lane = 2 * strided > (int64_t)lines.size();
+  if (f32) sp.fplanes = (float*)dev_alloc(sizeof(float) * std::max<int64_t>(1, plane * S));
+  else sp.planes = (uint16_t*)dev_alloc(sizeof(uint16_t) * std::max<int64_t>(1, plane * S));
   sp.exps = (int32_t*)dev_alloc(sizeof(int32_t) * std::max<size_t>(1, lines.size()));
   sp.dlines = upload_vec(lines, s);
 }
@@ -464,46 +476,66 @@ static void hpsi_split_setup(HpsiPlan& H) {
   split_check_lines(lB1, H.N, "psi");
   split_check_lines(lA2, H.t_total, "T");
   split_check_lines(lB2, H.hr_total, "HR");
-  split_side_alloc(H.spHL, lA1, hl_arena, S, s);
-  split_side_alloc(H.spPsi, lB1, H.N, S, s);
-  split_side_alloc(H.spT, lA2, H.t_total, S, s);
-  split_side_alloc(H.spHR, lB2, H.hr_total, S, s);
-  launch_split_lines(H.spHL.dlines, H.spHL.nlines, H.dHL, H.spHL.planes, H.spHL.plane, S, H.spHL.exps, s);
-  launch_split_lines(H.spHR.dlines, H.spHR.nlines, H.dHR, H.spHR.planes, H.spHR.plane, S, H.spHR.exps, s);
+  const bool f32 = H.mode == QSX_MODE_SPLIT_F32;
+  split_side_alloc(H.spHL, lA1, hl_arena, S, s, f32);
+  split_side_alloc(H.spPsi, lB1, H.N, S, s, f32);
+  split_side_alloc(H.spT, lA2, H.t_total, S, s, f32);
+  split_side_alloc(H.spHR, lB2, H.hr_total, S, s, f32);
+  if (f32) {
+    launch_split_lines_f32(H.spHL.dlines, H.spHL.nlines, H.spHL.lane, H.dHL, H.spHL.fplanes, H.spHL.plane, H.spHL.exps, s);
+    launch_split_lines_f32(H.spHR.dlines, H.spHR.nlines, H.spHR.lane, H.dHR, H.spHR.fplanes, H.spHR.plane, H.spHR.exps, s);
+  } else {
+    launch_split_lines(H.spHL.dlines, H.spHL.nlines, H.spHL.lane, H.dHL, H.spHL.planes, H.spHL.plane, S, H.spHL.exps, s);
+    launch_split_lines(H.spHR.dlines, H.spHR.nlines, H.spHR.lane, H.dHR, H.spHR.planes, H.spHR.plane, S, H.spHR.exps, s);
+  }
   H.dsp1 = upload_vec(split_pairs(H.pairs1, ea1, eb1), s);
   H.dsp2 = upload_vec(split_pairs(H.pairs2, ea2, eb2), s);
   H.dsp1s = upload_vec(split_pairs(H.pairs1s, ea1, eb1), s);
   H.dsp2s = upload_vec(split_pairs(H.pairs2s, ea2, eb2), s);
+  // one 16 x 16 tile per wave; a tile longer than 32 k-steps of 4 is cut
+  // into k-chunks whose partial tiles the wave engine's reduction sums
   WorkPolicy pol;
   pol.engine = ENG_WAVE;
   pol.wave_max = 1;
   pol.group_mfma = 1 << 30;
-  pol.split_mfma = 1 << 30;
+  pol.split_mfma = 32;
   H.swork1 = build_work(H.targets1, H.pairs1, pol);
   H.swork2 = build_work(H.targets2, H.pairs2, pol);
-  // the split kernel runs one 16 x 16 tile per wave that stores into C:
-  // a work list of another form (the workgroup-tile fallback for blocks
-  // beyond the wave engine's 32-bit offsets, shared or split tiles) would
-  // leave output elements unwritten
+  // the split kernel knows only 16 x 16 tiles stored into C or into a
+  // partial slot: a work list of another form (the workgroup-tile fallback
+  // for blocks beyond the wave engine's 32-bit offsets, shared tiles)
+  // would leave output elements unwritten
   for (const WorkList* W : {&H.swork1, &H.swork2}) {
-    bool ok = W->engine == ENG_WAVE && W->reds.empty();
+    bool ok = W->engine == ENG_WAVE;
     for (const WorkItem& w : W->items) {
       if (w.slot == -2) continue;  // padding
-      if (w.shape != wave_shape(1, 1) || w.slot != -1 || w.group != 0) ok = false;
+      if (w.shape != wave_shape(1, 1) || w.group != 0) ok = false;
     }
+    for (const ReduceItem& r : W->reds)
+      if (r.shape != wave_shape(1, 1)) ok = false;
     if (!ok) fail(QSX_ERR_UNSUPPORTED, "split precision: blocks too large for the 16 x 16 tile kernel");
   }
   H.dsitems1 = upload_vec(H.swork1.items, s);
   H.dsitems2 = upload_vec(H.swork2.items, s);
-  // executed slice-product flops: per 16x16 tile and k-step of 16 of a pair,
-  // S(S+1)/2 MFMAs of 2*16*16*16
+  H.dsreds1 = upload_vec(H.swork1.reds, s);
+  H.dsreds2 = upload_vec(H.swork2.reds, s);
+  H.dsscratch = (double*)dev_alloc(sizeof(double) *
+                                   std::max<int64_t>(1, std::max(H.swork1.scratch_elems, H.swork2.scratch_elems)));
+  // executed slice-product flops: per 16x16 tile and k-step of a pair,
+  // BF16: S(S+1)/2 MFMAs of 2*16*16*16 per 16 k; FP32 x 3: 6 MFMAs of
+  // 2*16*16*4 per 4 k
   const WorkList* W[2] = {&H.swork1, &H.swork2};
   const std::vector<PairDesc>* P[2] = {&H.pairs1, &H.pairs2};
+  const int ks = f32 ? 4 : 16;
+  const double per_step = f32 ? 6 * 2.0 * 16 * 16 * 4 : (S * (S + 1) / 2) * 2.0 * 16 * 16 * 16;
   for (int st = 0; st < 2; ++st) {
     double steps = 0;
     for (const WorkItem& w : W[st]->items)
-      for (int p = w.p0; p <= w.p1 && w.slot != -2; ++p) steps += ((*P[st])[p].K + 15) / 16;
-    H.split_exec_flops[st] = steps * (S * (S + 1) / 2) * 2.0 * 16 * 16 * 16;
+      for (int p = w.p0; p <= w.p1 && w.slot != -2; ++p) {
+        const int kb = p == w.p0 ? w.k0 : 0, ke = p == w.p1 ? w.k1 : (*P[st])[p].K;
+        steps += (ke - kb + ks - 1) / ks;
+      }
+    H.split_exec_flops[st] = steps * per_step;
   }
   QSX_HIP(hipStreamSynchronize(s));
 }
@@ -513,15 +545,28 @@ static void hpsi_launch(HpsiPlan& H, const double* dpsi, double* dhpsi, hipStrea
   const PairDesc* p2 = sym ? H.dpairs2s : H.dpairs2;
   // events: [0] start, [1] stage 1 done, [2] stage 2 done
   if (timed) QSX_HIP(hipEventRecord(H.ev[0], s));
-  if (H.mode == QSX_MODE_SPLIT_BF16) {
-    const int S = H.nsl;
-    launch_split_lines(H.spPsi.dlines, H.spPsi.nlines, dpsi, H.spPsi.planes, H.spPsi.plane, S, H.spPsi.exps, s);
-    launch_split_gemm(S, H.dsitems1, (int)H.swork1.items.size(), sym ? H.dsp1s : H.dsp1, H.spHL.planes,
-                      H.spHL.plane, H.spHL.exps, H.spPsi.planes, H.spPsi.plane, H.spPsi.exps, H.dT, s);
+  if (H.mode == QSX_MODE_SPLIT_F32) {
+    launch_split_lines_f32(H.spPsi.dlines, H.spPsi.nlines, H.spPsi.lane, dpsi, H.spPsi.fplanes, H.spPsi.plane, H.spPsi.exps, s);
+    launch_split_gemm_f32(H.dsitems1, (int)H.swork1.items.size(), sym ? H.dsp1s : H.dsp1, H.spHL.fplanes,
+                          H.spHL.plane, H.spHL.exps, H.spPsi.fplanes, H.spPsi.plane, H.spPsi.exps, H.dT, H.dsscratch,
+                          s);
+    launch_wave_reduce(H.dsreds1, (int)H.swork1.reds.size(), H.dsscratch, H.dT, s);
     if (timed) QSX_HIP(hipEventRecord(H.ev[1], s));
-    launch_split_lines(H.spT.dlines, H.spT.nlines, H.dT, H.spT.planes, H.spT.plane, S, H.spT.exps, s);
+    launch_split_lines_f32(H.spT.dlines, H.spT.nlines, H.spT.lane, H.dT, H.spT.fplanes, H.spT.plane, H.spT.exps, s);
+    launch_split_gemm_f32(H.dsitems2, (int)H.swork2.items.size(), sym ? H.dsp2s : H.dsp2, H.spT.fplanes,
+                          H.spT.plane, H.spT.exps, H.spHR.fplanes, H.spHR.plane, H.spHR.exps, dhpsi, H.dsscratch, s);
+    launch_wave_reduce(H.dsreds2, (int)H.swork2.reds.size(), H.dsscratch, dhpsi, s);
+  } else if (H.mode == QSX_MODE_SPLIT_BF16) {
+    const int S = H.nsl;
+    launch_split_lines(H.spPsi.dlines, H.spPsi.nlines, H.spPsi.lane, dpsi, H.spPsi.planes, H.spPsi.plane, S, H.spPsi.exps, s);
+    launch_split_gemm(S, H.dsitems1, (int)H.swork1.items.size(), sym ? H.dsp1s : H.dsp1, H.spHL.planes,
+                      H.spHL.plane, H.spHL.exps, H.spPsi.planes, H.spPsi.plane, H.spPsi.exps, H.dT, H.dsscratch, s);
+    launch_wave_reduce(H.dsreds1, (int)H.swork1.reds.size(), H.dsscratch, H.dT, s);
+    if (timed) QSX_HIP(hipEventRecord(H.ev[1], s));
+    launch_split_lines(H.spT.dlines, H.spT.nlines, H.spT.lane, H.dT, H.spT.planes, H.spT.plane, S, H.spT.exps, s);
     launch_split_gemm(S, H.dsitems2, (int)H.swork2.items.size(), sym ? H.dsp2s : H.dsp2, H.spT.planes, H.spT.plane,
-                      H.spT.exps, H.spHR.planes, H.spHR.plane, H.spHR.exps, dhpsi, s);
+                      H.spT.exps, H.spHR.planes, H.spHR.plane, H.spHR.exps, dhpsi, H.dsscratch, s);
+    launch_wave_reduce(H.dsreds2, (int)H.swork2.reds.size(), H.dsscratch, dhpsi, s);
   } else {
     launch_worklist(H.work1, H.dq1, H.ditems1, H.dreds1, p1, H.dHL, dpsi, H.dT, H.dscratch, s);
     if (timed) QSX_HIP(hipEventRecord(H.ev[1], s));
@@ -592,15 +637,16 @@ static void hpsi_create(HpsiPlan& H, const std::vector<const qsx_tensor*>& Ls, c
     H.verbose = o->verbose;
   }
   if (H.mode < QSX_MODE_BATCHED || H.mode > QSX_MODE_SPLIT_F32) fail(QSX_ERR_ARG, "invalid mode %d", H.mode);
-  if (H.mode == QSX_MODE_SPLIT_F32)
-    fail(QSX_ERR_UNSUPPORTED,
-         "FP32 slices: an exact FP32 accumulation allows ~8-bit slices, so 3 FP32 slices carry ~24 bits and "
-         "cannot reach 1e-12; use QSX_MODE_SPLIT_BF16 (nslices >= 6)");
-  if (H.mode == QSX_MODE_SPLIT_BF16) {
+  if (H.mode == QSX_MODE_SPLIT_BF16 || H.mode == QSX_MODE_SPLIT_F32) {
     if (H.xpairs || Ls.size() != 1 || Rs.size() != 1)
       fail(QSX_ERR_UNSUPPORTED, "split precision: plans of one environment pair (full MPO) only");
-    H.nsl = H.nslices > 0 ? H.nslices : 7;
-    if (H.nsl < 3 || H.nsl > 8) fail(QSX_ERR_ARG, "split precision: %d slices (3..8 supported)", H.nsl);
+    if (H.mode == QSX_MODE_SPLIT_F32) {
+      if (H.nslices != 0 && H.nslices != 3) fail(QSX_ERR_ARG, "FP32 split precision uses 3 slices (got %d)", H.nslices);
+      H.nsl = 3;
+    } else {
+      H.nsl = H.nslices > 0 ? H.nslices : 7;
+      if (H.nsl < 3 || H.nsl > 8) fail(QSX_ERR_ARG, "split precision: %d slices (3..8 supported)", H.nsl);
+    }
   }
   const double* fermA = o ? o->fermA : nullptr;  // exact +-1 factors of the blocks of Ls[0] (parity operators)
   if (fermA && Ls.size() != 1) fail(QSX_ERR_ARG, "H*psi: fermion signs need a single left environment");
@@ -857,7 +903,7 @@ static void hpsi_create(HpsiPlan& H, const std::vector<const qsx_tensor*>& Ls, c
   H.dpairs2s = upload_vec(H.pairs2s, s);
   hpsi_build_work(H);
   clk.lap("work lists");
-  if (H.mode == QSX_MODE_SPLIT_BF16) hpsi_split_setup(H);
+  if (H.mode == QSX_MODE_SPLIT_BF16 || H.mode == QSX_MODE_SPLIT_F32) hpsi_split_setup(H);
   QSX_HIP(hipStreamSynchronize(s));
   H.flops = 0;
   for (int t = 0; t < H.nterm; ++t) {
@@ -1177,7 +1223,7 @@ int qsx_hpsi_stage_info(const qsx_hpsi_plan* plan, double* info) {
     info[12 + t] = W[t]->maxf;
     info[14 + t] = (double)W[t]->reds.size();
     info[16 + t] = H.split_exec_flops[t];
-    if (H.mode == QSX_MODE_SPLIT_BF16) {
+    if (H.mode == QSX_MODE_SPLIT_BF16 || H.mode == QSX_MODE_SPLIT_F32) {
       info[10 + t] = (double)((H.swork1.items.size() * (t == 0) + H.swork2.items.size() * (t == 1) + 3) / 4);
       info[8 + t] = 2;
     }

@@ -311,11 +311,23 @@ struct SplitPair {
   int32_t pad;
   double afac;
 };
-void launch_split_lines(const SplitLine* dlines, int nlines, const double* X, uint16_t* planes, int64_t plane,
-                        int S, int32_t* exps, hipStream_t s);
+// per_lane: 16 lines per wave, 4 lanes per line (neighbouring lines
+// adjacent in memory), else one wave per line (lines contiguous along k)
+void launch_split_lines(const SplitLine* dlines, int nlines, bool per_lane, const double* X, uint16_t* planes,
+                        int64_t plane, int S, int32_t* exps, hipStream_t s);
+// items with slot >= 0 store partial tiles into scratch (reduced by
+// launch_wave_reduce)
 void launch_split_gemm(int S, const WorkItem* ditems, int nitems, const SplitPair* dpairs, const uint16_t* Ap,
                        int64_t apl, const int32_t* expA, const uint16_t* Bp, int64_t bpl, const int32_t* expB,
-                       double* C, hipStream_t s);
+                       double* C, double* scratch, hipStream_t s);
+// FP32 x 3 variant: three FP32 slice planes per buffer
+void launch_split_lines_f32(const SplitLine* dlines, int nlines, bool per_lane, const double* X, float* planes,
+                            int64_t plane, int32_t* exps, hipStream_t s);
+void launch_split_gemm_f32(const WorkItem* ditems, int nitems, const SplitPair* dpairs, const float* Ap, int64_t apl,
+                           const int32_t* expA, const float* Bp, int64_t bpl, const int32_t* expB, double* C,
+                           double* scratch, hipStream_t s);
+// sum the partial tiles of the wave engine's split items (k_wave_reduce)
+void launch_wave_reduce(const ReduceItem* dreds, int nreds, const double* scratch, double* C, hipStream_t s);
 
 // ---------------------------------------------------------------- block ops
 // Block permutation of a whole QSpace (blockops.hip): per block the fused

@@ -378,6 +378,12 @@ __global__ __launch_bounds__(NT) void k_wave_reduce(const ReduceItem* __restrict
 
 }  // namespace
 
+void launch_wave_reduce(const ReduceItem* dreds, int nreds, const double* scratch, double* C, hipStream_t s) {
+  if (nreds <= 0) return;
+  hipLaunchKernelGGL(k_wave_reduce, dim3(nreds), dim3(NT), 0, s, dreds, scratch, C);
+  QSX_HIP(hipGetLastError());
+}
+
 void launch_wave(const WorkItem* ditems, int nitems, int maxf, int wg_waves, const ReduceItem* dreds, int nreds,
                  const PairDesc* dpairs, const double* A, const double* B, const double* X, double* C,
                  double* scratch, hipStream_t s) {

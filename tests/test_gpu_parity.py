@@ -450,3 +450,33 @@ def test_complex_env_update_cell_form(gpu):
     assert C.is_complex() and C.itags == ["R", "R*", "w2*"]
     assert_same_structure(C, R)
     assert rel_err(C, R) < TOL
+
+
+def test_split_f32x3_vs_oracle(gpu, heis4000):
+    """the FP32 x 3 variant of config 4 (three FP32 slices on the FP32
+    matrix cores, FP32 accumulation, FP64 recombination) at D* = 1000 and at
+    the headline D* = 4000 against the oracle, to its stated bound of 5e-10
+    (the u = 1 products accumulate in FP32: about 2^-35 of |row| |col| per
+    term); weighted symmetry and the 20-step Lanczos energy (relative 1e-10)"""
+    BOUND = 5e-10
+    HL, psi, HR = gen.heis_hpsi(seed=27, Dstar=1000)
+    v = np.random.RandomState(2).standard_normal(psi.numel())
+    ref = _hpsi_oracle(HL, psi.with_packed(v), HR, 1).packed()
+    got = Q.HPsi(HL, psi, HR, hconj=1, mode="split_f32").apply(v)
+    err1 = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    HL, psi, HR, x, ref = heis4000
+    H = Q.HPsi(HL, psi, HR, hconj=1, mode="split_f32")
+    got = H.apply(x)
+    err4 = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    print(f"split_f32x3: rel err vs oracle D*=1000 {err1:.3e}, D*=4000 {err4:.3e}", flush=True)
+    assert err1 < BOUND and err4 < BOUND
+    assert err4 > 1e-13  # FP32 accumulation really is in the path
+    w = H.weights()
+    y = np.random.RandomState(8).standard_normal(H.N)
+    hy = H.apply(y)
+    assert abs((w * x) @ hy - (w * got) @ y) < 10 * BOUND * np.sqrt(((w * x) @ x) * ((w * hy) @ hy))
+    e_ref = Q.HPsi(HL, psi, HR, hconj=1).lanczos(psi.packed(), 20)[0]
+    e_f32 = H.lanczos(psi.packed(), 20)[0]
+    assert abs(e_f32 - e_ref) < 1e-10 * abs(e_ref), (e_f32, e_ref)
+    with pytest.raises(Q.QsxError):
+        Q.HPsi(HL, psi, HR, hconj=1, mode="split_f32", nslices=4)
