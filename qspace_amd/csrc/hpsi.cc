@@ -404,24 +404,21 @@ static std::vector<SplitLine> split_lines(const std::vector<PairDesc>& pairs, co
       const int32_t base = (int32_t)lines.size();
       seen[key] = base;
       ebase[p] = base;
-      for (int i = 0; i < n; ++i) lines.push_back(SplitLine{off + i * sl, d.K, (int32_t)sk, base + i, base});
+      for (int i = 0; i < n; ++i) lines.push_back(SplitLine{off + i * sl, 0, d.K, (int32_t)sk, base + i, base});
     }
   return lines;
 }
 
 static std::vector<SplitPair> split_pairs(const std::vector<PairDesc>& pairs, const std::vector<int32_t>& ea,
-                                          const std::vector<int32_t>& eb) {
+                                          const std::vector<int32_t>& eb, const std::vector<SplitLine>& la,
+                                          const std::vector<SplitLine>& lb) {
   std::vector<SplitPair> v(pairs.size());
   for (size_t p = 0; p < pairs.size(); ++p) {
     const PairDesc& d = pairs[p];
     SplitPair& q = v[p];
-    q.a_off = d.a_off;
-    q.b_off = d.b_off;
+    q.pa = la[ea[p]].pbase;
+    q.pb = lb[eb[p]].pbase;
     q.K = d.K;
-    q.sra = (d.flags & 1) ? d.lda : 1;
-    q.ska = (d.flags & 1) ? 1 : d.lda;
-    q.skb = (d.flags & 2) ? d.ldb : 1;
-    q.scb = (d.flags & 2) ? 1 : d.ldb;
     q.ea = ea[p];
     q.eb = eb[p];
     q.pad = 0;
@@ -430,33 +427,43 @@ static std::vector<SplitPair> split_pairs(const std::vector<PairDesc>& pairs, co
   return v;
 }
 
-// The slice planes of a buffer are indexed by element position, so two line
-// layouts of the same elements (different strides, e.g. one block read as
-// rows by one term and as columns by another) would write one plane element
-// under two exponents.  Such a plan is refused.  A line's layout is the
-// first line index of its block view (SplitLine::pad).
-static void split_check_lines(const std::vector<SplitLine>& lines, int64_t plane, const char* what) {
-  std::vector<int32_t> owner((size_t)std::max<int64_t>(1, plane), -1);
-  for (const SplitLine& l : lines)
-    for (int32_t k = 0; k < l.K; ++k) {
-      const int64_t e = l.off + (int64_t)k * l.sk;
-      if (e < 0 || e >= plane) fail(QSX_ERR_STRUCT, "split precision: %s line outside its buffer", what);
-      int32_t& o = owner[(size_t)e];
-      if (o >= 0 && o != l.pad)
-        fail(QSX_ERR_UNSUPPORTED, "split precision: %s is read in two layouts (one slice plane per buffer)", what);
-      o = l.pad;
-    }
+// packed storage of each line (quads of 4 k, S slices each, the last quad
+// padded): offsets in line order; returns the total
+static int64_t split_pack(std::vector<SplitLine>& lines, int S) {
+  int64_t tot = 0;
+  for (SplitLine& l : lines) {
+    l.pbase = tot;
+    tot += (int64_t)((l.K + 3) / 4) * 4 * S;
+  }
+  return tot;
 }
 
-static void split_side_alloc(HpsiPlan::SplitSide& sp, const std::vector<SplitLine>& lines, int64_t plane, int S,
+// every line element lies inside its buffer (each line has its own packed
+// slices, so one element read in two layouts is sliced twice)
+static void split_check_lines(const std::vector<SplitLine>& lines, int64_t plane, const char* what) {
+  for (const SplitLine& l : lines) {
+    const int64_t last = l.off + (int64_t)(l.K - 1) * l.sk;
+    if (l.K > 0 && (l.off < 0 || last < 0 || l.off >= plane || last >= plane))
+      fail(QSX_ERR_STRUCT, "split precision: %s line outside its buffer", what);
+  }
+}
+
+static void split_side_alloc(HpsiPlan::SplitSide& sp, const std::vector<SplitLine>& lines, int64_t packed, int S,
                              hipStream_t s, bool f32) {
-  sp.plane = plane;
+  (void)S;
+  sp.plane = packed;
   sp.nlines = (int)lines.size();
   int64_t strided = 0;
   for (const SplitLine& l : lines) strided += l.sk != 1;
   sp.lane = 2 * strided > (int64_t)lines.size();
-  if (f32) sp.fplanes = (float*)dev_alloc(sizeof(float) * std::max<int64_t>(1, plane * S));
-  else sp.planes = (uint16_t*)dev_alloc(sizeof(uint16_t) * std::max<int64_t>(1, plane * S));
+  // zero once: the tails of the last quads are never written
+  if (f32) {
+    sp.fplanes = (float*)dev_alloc(sizeof(float) * std::max<int64_t>(1, packed));
+    QSX_HIP(hipMemsetAsync(sp.fplanes, 0, sizeof(float) * std::max<int64_t>(1, packed), s));
+  } else {
+    sp.planes = (uint16_t*)dev_alloc(sizeof(uint16_t) * std::max<int64_t>(1, packed));
+    QSX_HIP(hipMemsetAsync(sp.planes, 0, sizeof(uint16_t) * std::max<int64_t>(1, packed), s));
+  }
   sp.exps = (int32_t*)dev_alloc(sizeof(int32_t) * std::max<size_t>(1, lines.size()));
   sp.dlines = upload_vec(lines, s);
 }
@@ -477,21 +484,21 @@ static void hpsi_split_setup(HpsiPlan& H) {
   split_check_lines(lA2, H.t_total, "T");
   split_check_lines(lB2, H.hr_total, "HR");
   const bool f32 = H.mode == QSX_MODE_SPLIT_F32;
-  split_side_alloc(H.spHL, lA1, hl_arena, S, s, f32);
-  split_side_alloc(H.spPsi, lB1, H.N, S, s, f32);
-  split_side_alloc(H.spT, lA2, H.t_total, S, s, f32);
-  split_side_alloc(H.spHR, lB2, H.hr_total, S, s, f32);
+  split_side_alloc(H.spHL, lA1, split_pack(lA1, S), S, s, f32);
+  split_side_alloc(H.spPsi, lB1, split_pack(lB1, S), S, s, f32);
+  split_side_alloc(H.spT, lA2, split_pack(lA2, S), S, s, f32);
+  split_side_alloc(H.spHR, lB2, split_pack(lB2, S), S, s, f32);
   if (f32) {
-    launch_split_lines_f32(H.spHL.dlines, H.spHL.nlines, H.spHL.lane, H.dHL, H.spHL.fplanes, H.spHL.plane, H.spHL.exps, s);
-    launch_split_lines_f32(H.spHR.dlines, H.spHR.nlines, H.spHR.lane, H.dHR, H.spHR.fplanes, H.spHR.plane, H.spHR.exps, s);
+    launch_split_lines_f32(H.spHL.dlines, H.spHL.nlines, H.spHL.lane, H.dHL, H.spHL.fplanes, H.spHL.exps, s);
+    launch_split_lines_f32(H.spHR.dlines, H.spHR.nlines, H.spHR.lane, H.dHR, H.spHR.fplanes, H.spHR.exps, s);
   } else {
-    launch_split_lines(H.spHL.dlines, H.spHL.nlines, H.spHL.lane, H.dHL, H.spHL.planes, H.spHL.plane, S, H.spHL.exps, s);
-    launch_split_lines(H.spHR.dlines, H.spHR.nlines, H.spHR.lane, H.dHR, H.spHR.planes, H.spHR.plane, S, H.spHR.exps, s);
+    launch_split_lines(H.spHL.dlines, H.spHL.nlines, H.spHL.lane, H.dHL, H.spHL.planes, S, H.spHL.exps, s);
+    launch_split_lines(H.spHR.dlines, H.spHR.nlines, H.spHR.lane, H.dHR, H.spHR.planes, S, H.spHR.exps, s);
   }
-  H.dsp1 = upload_vec(split_pairs(H.pairs1, ea1, eb1), s);
-  H.dsp2 = upload_vec(split_pairs(H.pairs2, ea2, eb2), s);
-  H.dsp1s = upload_vec(split_pairs(H.pairs1s, ea1, eb1), s);
-  H.dsp2s = upload_vec(split_pairs(H.pairs2s, ea2, eb2), s);
+  H.dsp1 = upload_vec(split_pairs(H.pairs1, ea1, eb1, lA1, lB1), s);
+  H.dsp2 = upload_vec(split_pairs(H.pairs2, ea2, eb2, lA2, lB2), s);
+  H.dsp1s = upload_vec(split_pairs(H.pairs1s, ea1, eb1, lA1, lB1), s);
+  H.dsp2s = upload_vec(split_pairs(H.pairs2s, ea2, eb2, lA2, lB2), s);
   // one 16 x 16 tile per wave; a tile longer than 32 k-steps of 4 is cut
   // into k-chunks whose partial tiles the wave engine's reduction sums
   WorkPolicy pol;
@@ -546,26 +553,25 @@ static void hpsi_launch(HpsiPlan& H, const double* dpsi, double* dhpsi, hipStrea
   // events: [0] start, [1] stage 1 done, [2] stage 2 done
   if (timed) QSX_HIP(hipEventRecord(H.ev[0], s));
   if (H.mode == QSX_MODE_SPLIT_F32) {
-    launch_split_lines_f32(H.spPsi.dlines, H.spPsi.nlines, H.spPsi.lane, dpsi, H.spPsi.fplanes, H.spPsi.plane, H.spPsi.exps, s);
+    launch_split_lines_f32(H.spPsi.dlines, H.spPsi.nlines, H.spPsi.lane, dpsi, H.spPsi.fplanes, H.spPsi.exps, s);
     launch_split_gemm_f32(H.dsitems1, (int)H.swork1.items.size(), sym ? H.dsp1s : H.dsp1, H.spHL.fplanes,
-                          H.spHL.plane, H.spHL.exps, H.spPsi.fplanes, H.spPsi.plane, H.spPsi.exps, H.dT, H.dsscratch,
-                          s);
+                          H.spHL.exps, H.spPsi.fplanes, H.spPsi.exps, H.dT, H.dsscratch, s);
     launch_wave_reduce(H.dsreds1, (int)H.swork1.reds.size(), H.dsscratch, H.dT, s);
     if (timed) QSX_HIP(hipEventRecord(H.ev[1], s));
-    launch_split_lines_f32(H.spT.dlines, H.spT.nlines, H.spT.lane, H.dT, H.spT.fplanes, H.spT.plane, H.spT.exps, s);
+    launch_split_lines_f32(H.spT.dlines, H.spT.nlines, H.spT.lane, H.dT, H.spT.fplanes, H.spT.exps, s);
     launch_split_gemm_f32(H.dsitems2, (int)H.swork2.items.size(), sym ? H.dsp2s : H.dsp2, H.spT.fplanes,
-                          H.spT.plane, H.spT.exps, H.spHR.fplanes, H.spHR.plane, H.spHR.exps, dhpsi, H.dsscratch, s);
+                          H.spT.exps, H.spHR.fplanes, H.spHR.exps, dhpsi, H.dsscratch, s);
     launch_wave_reduce(H.dsreds2, (int)H.swork2.reds.size(), H.dsscratch, dhpsi, s);
   } else if (H.mode == QSX_MODE_SPLIT_BF16) {
     const int S = H.nsl;
-    launch_split_lines(H.spPsi.dlines, H.spPsi.nlines, H.spPsi.lane, dpsi, H.spPsi.planes, H.spPsi.plane, S, H.spPsi.exps, s);
-    launch_split_gemm(S, H.dsitems1, (int)H.swork1.items.size(), sym ? H.dsp1s : H.dsp1, H.spHL.planes,
-                      H.spHL.plane, H.spHL.exps, H.spPsi.planes, H.spPsi.plane, H.spPsi.exps, H.dT, H.dsscratch, s);
+    launch_split_lines(H.spPsi.dlines, H.spPsi.nlines, H.spPsi.lane, dpsi, H.spPsi.planes, S, H.spPsi.exps, s);
+    launch_split_gemm(S, H.dsitems1, (int)H.swork1.items.size(), sym ? H.dsp1s : H.dsp1, H.spHL.planes, H.spHL.exps,
+                      H.spPsi.planes, H.spPsi.exps, H.dT, H.dsscratch, s);
     launch_wave_reduce(H.dsreds1, (int)H.swork1.reds.size(), H.dsscratch, H.dT, s);
     if (timed) QSX_HIP(hipEventRecord(H.ev[1], s));
-    launch_split_lines(H.spT.dlines, H.spT.nlines, H.spT.lane, H.dT, H.spT.planes, H.spT.plane, S, H.spT.exps, s);
-    launch_split_gemm(S, H.dsitems2, (int)H.swork2.items.size(), sym ? H.dsp2s : H.dsp2, H.spT.planes, H.spT.plane,
-                      H.spT.exps, H.spHR.planes, H.spHR.plane, H.spHR.exps, dhpsi, H.dsscratch, s);
+    launch_split_lines(H.spT.dlines, H.spT.nlines, H.spT.lane, H.dT, H.spT.planes, S, H.spT.exps, s);
+    launch_split_gemm(S, H.dsitems2, (int)H.swork2.items.size(), sym ? H.dsp2s : H.dsp2, H.spT.planes, H.spT.exps,
+                      H.spHR.planes, H.spHR.exps, dhpsi, H.dsscratch, s);
     launch_wave_reduce(H.dsreds2, (int)H.swork2.reds.size(), H.dsscratch, dhpsi, s);
   } else {
     launch_worklist(H.work1, H.dq1, H.ditems1, H.dreds1, p1, H.dHL, dpsi, H.dT, H.dscratch, s);

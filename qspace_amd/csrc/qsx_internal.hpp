@@ -297,35 +297,35 @@ void launch_pair_gemm_f64(const PairDesc& p, int64_t c_off, int M, int N, int be
 // of A or a column of B over its k-range) and a pair in slice addressing
 struct SplitLine {
   int64_t off;    // element offset of the line's first element
+  int64_t pbase;  // offset of the line's packed slices (quads of 4 k, S slices each)
   int32_t K;      // elements on the line
   int32_t sk;     // element stride along the line
-  int32_t eidx;   // index of the line's exponent
+  int32_t eidx;   // index of the line's exponent (= its index in the line list)
   int32_t pad;
 };
 struct SplitPair {
-  int64_t a_off, b_off;
+  int64_t pa, pb;     // packed offset of A row 0 / B column 0 (lines of a block are consecutive)
   int32_t K;
-  int32_t sra, ska;   // A element (i, k) at a_off + i*sra + k*ska
-  int32_t skb, scb;   // B element (k, j) at b_off + k*skb + j*scb
-  int32_t ea, eb;     // exponent index of A row 0 / B column 0 of the pair's blocks
+  int32_t ea, eb;     // line (exponent) index of A row 0 / B column 0 of the pair's blocks
   int32_t pad;
   double afac;
 };
 // per_lane: 16 lines per wave, 4 lanes per line (neighbouring lines
-// adjacent in memory), else one wave per line (lines contiguous along k)
+// adjacent in memory), else one wave per line (lines contiguous along k);
+// the slices go to the lines' packed storage
 void launch_split_lines(const SplitLine* dlines, int nlines, bool per_lane, const double* X, uint16_t* planes,
-                        int64_t plane, int S, int32_t* exps, hipStream_t s);
+                        int S, int32_t* exps, hipStream_t s);
 // items with slot >= 0 store partial tiles into scratch (reduced by
 // launch_wave_reduce)
 void launch_split_gemm(int S, const WorkItem* ditems, int nitems, const SplitPair* dpairs, const uint16_t* Ap,
-                       int64_t apl, const int32_t* expA, const uint16_t* Bp, int64_t bpl, const int32_t* expB,
-                       double* C, double* scratch, hipStream_t s);
-// FP32 x 3 variant: three FP32 slice planes per buffer
+                       const int32_t* expA, const uint16_t* Bp, const int32_t* expB, double* C, double* scratch,
+                       hipStream_t s);
+// FP32 x 3 variant: three FP32 slices per element
 void launch_split_lines_f32(const SplitLine* dlines, int nlines, bool per_lane, const double* X, float* planes,
-                            int64_t plane, int32_t* exps, hipStream_t s);
-void launch_split_gemm_f32(const WorkItem* ditems, int nitems, const SplitPair* dpairs, const float* Ap, int64_t apl,
-                           const int32_t* expA, const float* Bp, int64_t bpl, const int32_t* expB, double* C,
-                           double* scratch, hipStream_t s);
+                            int32_t* exps, hipStream_t s);
+void launch_split_gemm_f32(const WorkItem* ditems, int nitems, const SplitPair* dpairs, const float* Ap,
+                           const int32_t* expA, const float* Bp, const int32_t* expB, double* C, double* scratch,
+                           hipStream_t s);
 // sum the partial tiles of the wave engine's split items (k_wave_reduce)
 void launch_wave_reduce(const ReduceItem* dreds, int nreds, const double* scratch, double* C, hipStream_t s);
 
