@@ -92,6 +92,7 @@ struct HpsiPlan {
     SplitLine* dlines = nullptr;
     int nlines = 0;
     bool lane = false;  // 16 lines per wave (lines adjacent, strided along k)
+    int maxk = 0;       // longest line
   };
   int nsl = 6;
   SplitSide spHL, spPsi, spT, spHR;
@@ -454,7 +455,10 @@ static void split_side_alloc(HpsiPlan::SplitSide& sp, const std::vector<SplitLin
   sp.plane = packed;
   sp.nlines = (int)lines.size();
   int64_t strided = 0;
-  for (const SplitLine& l : lines) strided += l.sk != 1;
+  for (const SplitLine& l : lines) {
+    strided += l.sk != 1;
+    sp.maxk = std::max(sp.maxk, l.K);
+  }
   sp.lane = 2 * strided > (int64_t)lines.size();
   // zero once: the tails of the last quads are never written
   if (f32) {
@@ -489,11 +493,11 @@ static void hpsi_split_setup(HpsiPlan& H) {
   split_side_alloc(H.spT, lA2, split_pack(lA2, S), S, s, f32);
   split_side_alloc(H.spHR, lB2, split_pack(lB2, S), S, s, f32);
   if (f32) {
-    launch_split_lines_f32(H.spHL.dlines, H.spHL.nlines, H.spHL.lane, H.dHL, H.spHL.fplanes, H.spHL.exps, s);
-    launch_split_lines_f32(H.spHR.dlines, H.spHR.nlines, H.spHR.lane, H.dHR, H.spHR.fplanes, H.spHR.exps, s);
+    launch_split_lines_f32(H.spHL.dlines, H.spHL.nlines, H.spHL.maxk, H.spHL.lane, H.dHL, H.spHL.fplanes, H.spHL.exps, s);
+    launch_split_lines_f32(H.spHR.dlines, H.spHR.nlines, H.spHR.maxk, H.spHR.lane, H.dHR, H.spHR.fplanes, H.spHR.exps, s);
   } else {
-    launch_split_lines(H.spHL.dlines, H.spHL.nlines, H.spHL.lane, H.dHL, H.spHL.planes, S, H.spHL.exps, s);
-    launch_split_lines(H.spHR.dlines, H.spHR.nlines, H.spHR.lane, H.dHR, H.spHR.planes, S, H.spHR.exps, s);
+    launch_split_lines(H.spHL.dlines, H.spHL.nlines, H.spHL.maxk, H.spHL.lane, H.dHL, H.spHL.planes, S, H.spHL.exps, s);
+    launch_split_lines(H.spHR.dlines, H.spHR.nlines, H.spHR.maxk, H.spHR.lane, H.dHR, H.spHR.planes, S, H.spHR.exps, s);
   }
   H.dsp1 = upload_vec(split_pairs(H.pairs1, ea1, eb1, lA1, lB1), s);
   H.dsp2 = upload_vec(split_pairs(H.pairs2, ea2, eb2, lA2, lB2), s);
@@ -553,23 +557,23 @@ static void hpsi_launch(HpsiPlan& H, const double* dpsi, double* dhpsi, hipStrea
   // events: [0] start, [1] stage 1 done, [2] stage 2 done
   if (timed) QSX_HIP(hipEventRecord(H.ev[0], s));
   if (H.mode == QSX_MODE_SPLIT_F32) {
-    launch_split_lines_f32(H.spPsi.dlines, H.spPsi.nlines, H.spPsi.lane, dpsi, H.spPsi.fplanes, H.spPsi.exps, s);
+    launch_split_lines_f32(H.spPsi.dlines, H.spPsi.nlines, H.spPsi.maxk, H.spPsi.lane, dpsi, H.spPsi.fplanes, H.spPsi.exps, s);
     launch_split_gemm_f32(H.dsitems1, (int)H.swork1.items.size(), sym ? H.dsp1s : H.dsp1, H.spHL.fplanes,
                           H.spHL.exps, H.spPsi.fplanes, H.spPsi.exps, H.dT, H.dsscratch, s);
     launch_wave_reduce(H.dsreds1, (int)H.swork1.reds.size(), H.dsscratch, H.dT, s);
     if (timed) QSX_HIP(hipEventRecord(H.ev[1], s));
-    launch_split_lines_f32(H.spT.dlines, H.spT.nlines, H.spT.lane, H.dT, H.spT.fplanes, H.spT.exps, s);
+    launch_split_lines_f32(H.spT.dlines, H.spT.nlines, H.spT.maxk, H.spT.lane, H.dT, H.spT.fplanes, H.spT.exps, s);
     launch_split_gemm_f32(H.dsitems2, (int)H.swork2.items.size(), sym ? H.dsp2s : H.dsp2, H.spT.fplanes,
                           H.spT.exps, H.spHR.fplanes, H.spHR.exps, dhpsi, H.dsscratch, s);
     launch_wave_reduce(H.dsreds2, (int)H.swork2.reds.size(), H.dsscratch, dhpsi, s);
   } else if (H.mode == QSX_MODE_SPLIT_BF16) {
     const int S = H.nsl;
-    launch_split_lines(H.spPsi.dlines, H.spPsi.nlines, H.spPsi.lane, dpsi, H.spPsi.planes, S, H.spPsi.exps, s);
+    launch_split_lines(H.spPsi.dlines, H.spPsi.nlines, H.spPsi.maxk, H.spPsi.lane, dpsi, H.spPsi.planes, S, H.spPsi.exps, s);
     launch_split_gemm(S, H.dsitems1, (int)H.swork1.items.size(), sym ? H.dsp1s : H.dsp1, H.spHL.planes, H.spHL.exps,
                       H.spPsi.planes, H.spPsi.exps, H.dT, H.dsscratch, s);
     launch_wave_reduce(H.dsreds1, (int)H.swork1.reds.size(), H.dsscratch, H.dT, s);
     if (timed) QSX_HIP(hipEventRecord(H.ev[1], s));
-    launch_split_lines(H.spT.dlines, H.spT.nlines, H.spT.lane, H.dT, H.spT.planes, S, H.spT.exps, s);
+    launch_split_lines(H.spT.dlines, H.spT.nlines, H.spT.maxk, H.spT.lane, H.dT, H.spT.planes, S, H.spT.exps, s);
     launch_split_gemm(S, H.dsitems2, (int)H.swork2.items.size(), sym ? H.dsp2s : H.dsp2, H.spT.planes, H.spT.exps,
                       H.spHR.planes, H.spHR.exps, dhpsi, H.dsscratch, s);
     launch_wave_reduce(H.dsreds2, (int)H.swork2.reds.size(), H.dsscratch, dhpsi, s);

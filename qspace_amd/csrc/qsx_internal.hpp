@@ -310,19 +310,20 @@ struct SplitPair {
   int32_t pad;
   double afac;
 };
-// per_lane: 16 lines per wave, 4 lanes per line (neighbouring lines
-// adjacent in memory), else one wave per line (lines contiguous along k);
+// per_lane: 16 lines per wave, 4 lanes per line, in two passes (exponents,
+// then slices over k-chunks; neighbouring lines adjacent in memory), else one
+// wave per line (lines contiguous along k); maxk bounds the lines' lengths;
 // the slices go to the lines' packed storage
-void launch_split_lines(const SplitLine* dlines, int nlines, bool per_lane, const double* X, uint16_t* planes,
-                        int S, int32_t* exps, hipStream_t s);
+void launch_split_lines(const SplitLine* dlines, int nlines, int maxk, bool per_lane, const double* X,
+                        uint16_t* planes, int S, int32_t* exps, hipStream_t s);
 // items with slot >= 0 store partial tiles into scratch (reduced by
 // launch_wave_reduce)
 void launch_split_gemm(int S, const WorkItem* ditems, int nitems, const SplitPair* dpairs, const uint16_t* Ap,
                        const int32_t* expA, const uint16_t* Bp, const int32_t* expB, double* C, double* scratch,
                        hipStream_t s);
 // FP32 x 3 variant: three FP32 slices per element
-void launch_split_lines_f32(const SplitLine* dlines, int nlines, bool per_lane, const double* X, float* planes,
-                            int32_t* exps, hipStream_t s);
+void launch_split_lines_f32(const SplitLine* dlines, int nlines, int maxk, bool per_lane, const double* X,
+                            float* planes, int32_t* exps, hipStream_t s);
 void launch_split_gemm_f32(const WorkItem* ditems, int nitems, const SplitPair* dpairs, const float* Ap,
                            const int32_t* expA, const float* Bp, const int32_t* expB, double* C, double* scratch,
                            hipStream_t s);

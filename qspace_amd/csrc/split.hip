@@ -34,6 +34,7 @@
 // a k-chunk of it whose partial tile the wave engine's reduction sums.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "qsx_internal.hpp"
@@ -108,31 +109,59 @@ __global__ __launch_bounds__(NT) void k_split_lines(const SplitLine* __restrict_
   }
 }
 
-// 16 contraction lines per wave, 4 lanes per line (lines whose elements are
-// strided while neighbouring lines are adjacent, e.g. the rows of a
-// column-major block): lane l takes line l%16 of the wave's group and the
-// quads q = l/16 (mod 4) of 4 consecutive k, so each load instruction reads
-// 16 adjacent elements and each lane writes a quad's slices as one run; the
-// line maximum is reduced over the 4 lanes of a line
-template <typename T, bool F32>
-__global__ __launch_bounds__(NT) void k_split_lines_grp(const SplitLine* __restrict__ lines, int nlines,
-                                                        const double* __restrict__ X, T* __restrict__ P, int S,
-                                                        int32_t* __restrict__ exps) {
+// Lines whose elements are strided while neighbouring lines are adjacent
+// (e.g. the rows of a column-major block) run in two passes over a 2-D grid
+// of (16 lines, 64 k) waves, 4 lanes per line, so that each load instruction
+// reads 16 adjacent elements: the exponents (chunk maxima merged by an
+// integer maximum), then the slices, each lane writing a quad's slices as
+// one run.
+constexpr int kQuadsPerWave = 16;  // quads of 4 k per wave of both passes
+constexpr int kNoExp = -100000;    // exponent of a line not yet seen non-zero
+
+__global__ __launch_bounds__(NT) void k_exps_fill(int32_t* __restrict__ exps, int n) {
+  const int i = blockIdx.x * NT + threadIdx.x;
+  if (i < n) exps[i] = kNoExp;
+}
+
+// per (16 lines, chunk of k) wave: the chunk's maximum of each line, its
+// exponent merged into the line's by an integer maximum (the exponent is
+// monotonic in the maximum)
+__global__ __launch_bounds__(NT) void k_split_exps_grp(const SplitLine* __restrict__ lines, int nlines,
+                                                       const double* __restrict__ X, int32_t* __restrict__ exps) {
   const int lane = threadIdx.x & 63;
   const int li = (blockIdx.x * (NT / 64) + (threadIdx.x >> 6)) * 16 + (lane & 15), kq = lane >> 4;
   const bool ok = li < nlines;
   SplitLine L{};
   if (ok) L = lines[li];
   double mx = 0.0;
-  if (ok)
-    for (int k = kq; k < L.K; k += 4) mx = fmax(mx, fabs(X[L.off + (int64_t)k * L.sk]));
+  if (ok) {
+    const int nq = (L.K + 3) >> 2, q1 = min(nq, (int)(blockIdx.y + 1) * kQuadsPerWave);
+    for (int q = (int)blockIdx.y * kQuadsPerWave + kq; q < q1; q += 4) {
+      double x[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int k = 4 * q + j;
+        x[j] = k < L.K ? fabs(X[L.off + (int64_t)k * L.sk]) : 0.0;
+      }
+      mx = fmax(mx, fmax(fmax(x[0], x[1]), fmax(x[2], x[3])));
+    }
+  }
   mx = fmax(mx, __shfl_xor(mx, 16, 64));
   mx = fmax(mx, __shfl_xor(mx, 32, 64));
-  const int e = mx > 0.0 ? ilogb(mx) + 1 : 0;
-  if (!ok) return;
-  if (kq == 0) exps[L.eidx] = e;
-  const int nq = (L.K + 3) >> 2;
-  for (int q = kq; q < nq; q += 4) {
+  if (ok && kq == 0 && mx > 0.0) atomicMax(exps + L.eidx, ilogb(mx) + 1);
+}
+
+template <typename T, bool F32>
+__global__ __launch_bounds__(NT) void k_split_quads_grp(const SplitLine* __restrict__ lines, int nlines,
+                                                        const double* __restrict__ X, T* __restrict__ P, int S,
+                                                        const int32_t* __restrict__ exps) {
+  const int lane = threadIdx.x & 63;
+  const int li = (blockIdx.x * (NT / 64) + (threadIdx.x >> 6)) * 16 + (lane & 15), kq = lane >> 4;
+  if (li >= nlines) return;
+  const SplitLine L = lines[li];
+  const int e = exps[L.eidx] == kNoExp ? 0 : exps[L.eidx];
+  const int nq = (L.K + 3) >> 2, q1 = min(nq, (int)(blockIdx.y + 1) * kQuadsPerWave);
+  for (int q = (int)blockIdx.y * kQuadsPerWave + kq; q < q1; q += 4) {
     double x[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -351,12 +380,17 @@ __global__ __launch_bounds__(NT) void k_split_gemm_f32(const WorkItem* __restric
 }
 
 template <typename T, bool F32>
-void lines_launch(const SplitLine* dlines, int nlines, bool grouped, const double* X, T* P, int S, int32_t* exps,
-                  hipStream_t s) {
+void lines_launch(const SplitLine* dlines, int nlines, int maxk, bool grouped, const double* X, T* P, int S,
+                  int32_t* exps, hipStream_t s) {
   if (nlines <= 0) return;
   if (grouped) {
     const int nb = (nlines + 16 * (NT / 64) - 1) / (16 * (NT / 64));
-    hipLaunchKernelGGL((k_split_lines_grp<T, F32>), dim3(nb), dim3(NT), 0, s, dlines, nlines, X, P, S, exps);
+    const int ny = std::max(1, ((maxk + 3) / 4 + kQuadsPerWave - 1) / kQuadsPerWave);
+    // line indices are the exponent indices 0 .. nlines-1
+    hipLaunchKernelGGL(k_exps_fill, dim3((nlines + NT - 1) / NT), dim3(NT), 0, s, exps, nlines);
+    hipLaunchKernelGGL(k_split_exps_grp, dim3(nb, ny), dim3(NT), 0, s, dlines, nlines, X, exps);
+    QSX_HIP(hipGetLastError());
+    hipLaunchKernelGGL((k_split_quads_grp<T, F32>), dim3(nb, ny), dim3(NT), 0, s, dlines, nlines, X, P, S, exps);
   } else {
     const int nb = (nlines + NT / 64 - 1) / (NT / 64);
     hipLaunchKernelGGL((k_split_lines<T, F32>), dim3(nb), dim3(NT), 0, s, dlines, nlines, X, P, S, exps);
@@ -366,14 +400,14 @@ void lines_launch(const SplitLine* dlines, int nlines, bool grouped, const doubl
 
 }  // namespace
 
-void launch_split_lines(const SplitLine* dlines, int nlines, bool per_lane, const double* X, uint16_t* planes,
-                        int S, int32_t* exps, hipStream_t s) {
-  lines_launch<uint16_t, false>(dlines, nlines, per_lane, X, planes, S, exps, s);
+void launch_split_lines(const SplitLine* dlines, int nlines, int maxk, bool per_lane, const double* X,
+                        uint16_t* planes, int S, int32_t* exps, hipStream_t s) {
+  lines_launch<uint16_t, false>(dlines, nlines, maxk, per_lane, X, planes, S, exps, s);
 }
 
-void launch_split_lines_f32(const SplitLine* dlines, int nlines, bool per_lane, const double* X, float* planes,
-                            int32_t* exps, hipStream_t s) {
-  lines_launch<float, true>(dlines, nlines, per_lane, X, planes, 3, exps, s);
+void launch_split_lines_f32(const SplitLine* dlines, int nlines, int maxk, bool per_lane, const double* X,
+                            float* planes, int32_t* exps, hipStream_t s) {
+  lines_launch<float, true>(dlines, nlines, maxk, per_lane, X, planes, 3, exps, s);
 }
 
 void launch_split_gemm(int S, const WorkItem* ditems, int nitems, const SplitPair* dpairs, const uint16_t* Ap,
