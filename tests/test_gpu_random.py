@@ -236,3 +236,44 @@ def test_random_su2_truncations(gpu, seed):
         XR = O.contract(R1, [K], R2, [T.rank - K])
         assert_same_structure(X, XR)
         assert rel_err(X, XR) < 1e-12
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_random_su2_om_contractions(gpu, seed):
+    """SU(2) contractions with outer multiplicity: operands of rank 3-4 (blocks
+    with an OM dimension), results of rank 2-5, random directions, leg
+    positions, conj flags and output permutations, against the oracle's
+    block-wise restatement (blocks compared by record) and, for the small
+    draws, the full M-resolved space"""
+    from fullspace import check_contraction
+    from test_gpu_om import assert_same_by_record
+    from qspace_amd import gen
+    rng = np.random.RandomState(700 + seed)
+    ra, rb, k = [(4, 3, 1), (4, 4, 2), (3, 3, 1), (4, 3, 2), (4, 4, 3), (3, 4, 1)][seed % 6]
+    def leg():
+        tss = rng.choice(5, size=rng.randint(2, 4), replace=False)
+        return [(int(t), int(rng.randint(1, 4))) for t in sorted(tss)]
+    shared = [leg() for _ in range(k)]
+    la = shared + [leg() for _ in range(ra - k)]
+    lb = shared + [leg() for _ in range(rb - k)]
+    da = [1 if rng.rand() < 0.5 else -1 for _ in la]
+    db = [-da[i] if i < k else (1 if rng.rand() < 0.5 else -1) for i in range(rb)]
+    pa, pb = rng.permutation(ra), rng.permutation(rb)
+    ta = [(f"c{i}" if i < k else f"a{i}") + ("*" if da[i] < 0 else "") for i in range(ra)]
+    tb = [(f"c{i}" if i < k else f"b{i}") + ("*" if db[i] < 0 else "") for i in range(rb)]
+    A = gen.su2_om_tensor(800 + seed, [la[i] for i in pa], [ta[i] for i in pa], keep=0.8)
+    B = gen.su2_om_tensor(900 + seed, [lb[i] for i in pb], [tb[i] for i in pb], keep=0.8)
+    if not A.nblk or not B.nblk:
+        pytest.skip("empty draw")
+    ia = [int(np.nonzero(pa == i)[0][0]) for i in range(k)]
+    ib = [int(np.nonzero(pb == i)[0][0]) for i in range(k)]
+    rc = ra + rb - 2 * k
+    perm = [int(x) for x in rng.permutation(rc)] if rc and rng.rand() < 0.5 else None
+    conj = int(rng.rand() < 0.3)
+    R = O.contract(A, ia, B, ib, perm=perm, conjA=conj, conjB=conj)
+    print(f"seed {seed}: A {A.itags} {A.nblk} ia {ia}, B {B.itags} {B.nblk} ib {ib}, perm {perm}, conj {conj}"
+          f" -> {R.nblk}", flush=True)
+    C = Q.api._contract_pair(A, ia, conj, B, ib, conj, perm)
+    assert_same_by_record(C, R)
+    if perm is None and not conj and rc <= 4 and A.numel() * B.numel() < 4e6:
+        assert check_contraction(A, ia, B, ib, C) < 1e-13
