@@ -416,6 +416,41 @@ def env_update():
     return out
 
 
+def env_update_su2():
+    """SU(2) environment update L' = A'.(HK.W.A), {A,'!2*',{{A,HK},W}}: bond
+    labels 2S in 0..16 with 8..64 states per multiplet, a rank-4 MPO (operator
+    legs of spin 0 and 1, 4 states each) and rank-4 intermediates with outer
+    multiplicity.  GPU: the whole contractQS call as above; the oracle's
+    chain (numpy restatement of the outer-multiplicity contraction) for the
+    error, compared block by block (record-keyed)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    import qspace_amd as Q
+    from qspace_amd import gen
+    A, HK, W = gen.su2_env_update(seed=1010, tS=tuple(range(17)), dmin=8, dmax=64, dw=4)
+    Q.contractQS([A, "!2*", [[A, HK], W]])
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        C = Q.contractQS([A, "!2*", [[A, HK], W]])
+        ts.append((time.perf_counter() - t0) * 1e3)
+    t0 = time.perf_counter()
+    X1 = O.contract(A, [0], HK, [0])
+    X2 = O.contract(X1, [1, 3], W, [2, 0])
+    R = O.contract(A, [0, 2], X2, [1, 3], conjA=1)
+    cpu_ms = (time.perf_counter() - t0) * 1e3
+    key = lambda T: [tuple(tuple(int(x) for x in T.Q[k][i]) for k in range(T.rank)) for i in range(T.nblk)]
+    dr = dict(zip(key(R), R.data))
+    num = sum(float(np.sum((d - dr[k]) ** 2)) for k, d in zip(key(C), C.data) if k in dr)
+    den = sum(float(np.sum(d ** 2)) for d in R.data)
+    out = {"bond_multiplets": int(sum(A.data[i].shape[0] for i in range(A.nblk))),
+           "blocks": [A.nblk, HK.nblk, W.nblk], "max_om_intermediate": int(max(X2.om())),
+           "same_records": sorted(key(C)) == sorted(key(R)), "gpu_wall_ms": sum(ts) / len(ts),
+           "oracle_numpy_ms": cpu_ms, "rel_err_vs_oracle": float(np.sqrt(num / den))}
+    log(f"[bench] env update SU(2): {out}")
+    return out
+
+
 def truncation():
     """orthoQS (block SVD truncation after the eigensolve, mpsOrthoQS) of a
     U(1) two-site wave function psi(L,s1,s2*,R*) split between (L,s1) and
@@ -514,6 +549,7 @@ def main():
     if args.block_ops:
         bops = block_ops()
         bops["u1_env_update"] = env_update()
+        bops["su2_env_update"] = env_update_su2()
         bops["truncation"] = truncation()
         print(json.dumps({"block_ops": bops}), flush=True)
         return
@@ -615,6 +651,7 @@ def main():
         configs = extra_configs()
         bops = block_ops()
         bops["u1_env_update"] = env_update()
+        bops["su2_env_update"] = env_update_su2()
         bops["truncation"] = truncation()
     cpu = None
     if not args.no_cpu and not args.lite:

@@ -178,6 +178,11 @@ struct WorkItem {
   int32_t group;     // wave engine: 1 if the workgroup's 4 waves share this
                      // tile, each over its own k-range, summed through LDS
   int32_t pad;       // wave engine flags: kShortItem (at most two k-chunks)
+  // wave engine: the addressing of the item's first pair (pairs[p0] but its
+  // afac, which may differ between pair arrays sharing the work list), so
+  // the wave issues its first operand loads without a descriptor load
+  int64_t fa_off, fb_off;
+  int32_t fK, flda, fldb, fflags;
 };
 constexpr int32_t kShortItem = 2;
 // k-steps of 4 per chunk of the wave kernel's ring for a tile of fm x fn

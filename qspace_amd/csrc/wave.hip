@@ -226,7 +226,16 @@ __device__ __forceinline__ void run_k(const WorkItem& w, const PairDesc* __restr
   if (w.p0 > w.p1) return;  // no pairs (zero tile or empty share of a group)
   Cursor<FM, FN> c;
   {
-    const PairDesc pd = pairs[w.p0];
+    // the first pair's addressing comes with the item; its afac (needed
+    // only by the first MFMAs) is loaded beside the operands
+    PairDesc pd;
+    pd.a_off = w.fa_off;
+    pd.b_off = w.fb_off;
+    pd.K = w.fK;
+    pd.lda = w.flda;
+    pd.ldb = w.fldb;
+    pd.flags = w.fflags;
+    pd.afac = pairs[w.p0].afac;
     set_pair<FM, FN>(c, pd, pairs, w.p0, w.k0, w, A, B, X);
   }
   bool have = true;

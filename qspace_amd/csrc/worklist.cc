@@ -267,6 +267,16 @@ static WorkList build_wave_work(const std::vector<WorkTarget>& targets, const st
         }
       }
   }
+  for (WorkItem& w : out.items) {
+    if (w.slot == -2 || w.p0 > w.p1) continue;
+    const PairDesc& d = pairs[w.p0];
+    w.fa_off = d.a_off;
+    w.fb_off = d.b_off;
+    w.fK = d.K;
+    w.flda = d.lda;
+    w.fldb = d.ldb;
+    w.fflags = d.flags;
+  }
   out.scratch_elems = slot * 64 * 64;
   return out;
 }
