@@ -59,7 +59,8 @@ def _policy_ints(p: dict):
 
 
 def _policy_dict(v):
-    return {k: int(x) for k, x in zip(POLICY_KEYS[:10], v)}
+    """named form of an int32 policy vector (the reserved slots dropped)"""
+    return {k: int(x) for k, x in zip(POLICY_KEYS[:9], v)}
 
 
 def _ivec(v):
