@@ -646,6 +646,11 @@ def main():
     roof["avg_launch_ms"] = tk
     roof["fp64_mfma_frac"] = fl / (tk * 1e-3) / 1e12 / PEAK_FP64_TFLOPS
     roof["hbm_frac"] = by / (tk * 1e-3) / 1e9 / PEAK_HBM_GBS
+    # FP64 MFMAs the kernel issues (tile and k padding included)
+    ex = info.get("mfma_exec_flops", (0.0, 0.0))[dom]
+    if ex:
+        roof["mfma_issued_flops"] = ex
+        roof["mfma_issued_frac"] = ex / (tk * 1e-3) / 1e12 / PEAK_FP64_TFLOPS
     configs = bops = None
     if not args.lite:
         configs = extra_configs()

@@ -351,8 +351,9 @@ int qsx_hpsi_weights(const qsx_hpsi_plan* plan, double* w);
    the main GEMM launch; [12..13] largest fragments per tile side (wave
    tiles); [14..15] reduction workgroups (0: no split tiles); [16..17]
    executed slice-product flops (split precision, else 0); engine 2 = the
-   split-precision kernel */
-#define QSX_STAGE_INFO 18
+   split-precision kernel; [18..19] flops of the FP64 MFMAs the wave engine
+   issues (tile and k padding included; 0 for the other engines) */
+#define QSX_STAGE_INFO 20
 int qsx_hpsi_stage_info(const qsx_hpsi_plan* plan, double* info);
 /* Work decomposition policy of one batched contraction, as
    int32[QSX_POLICY_INTS]: [0] engine (0 workgroup tiles, 1 wave tiles),

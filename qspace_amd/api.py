@@ -526,7 +526,7 @@ class HPsi:
     def stage_info(self) -> dict:
         """per stage: useful flops, algorithmic bytes, work items, pairs, the
         GEMM kernel that runs it, its workgroups and reduction workgroups"""
-        v = (C.c_double * 18)()
+        v = (C.c_double * 20)()
         _capi.check(_capi.lib().qsx_hpsi_stage_info(self.h, v), None)
         kern = []
         for t in (0, 1):
@@ -538,7 +538,8 @@ class HPsi:
                 kern.append("k_wave_s" if int(v[12 + t]) <= 2 else "k_wave_l")
         return {"flops": (v[0], v[1]), "bytes": (v[2], v[3]), "tiles": (int(v[4]), int(v[5])),
                 "pairs": (int(v[6]), int(v[7])), "kernel": tuple(kern), "grid": (int(v[10]), int(v[11])),
-                "reduce_grid": (int(v[14]), int(v[15])), "split_exec_flops": (v[16], v[17])}
+                "reduce_grid": (int(v[14]), int(v[15])), "split_exec_flops": (v[16], v[17]),
+                "mfma_exec_flops": (v[18], v[19])}
 
     def autotune(self, reps: int = 5):
         """time candidate work decompositions of both stages and keep the

@@ -1233,6 +1233,7 @@ int qsx_hpsi_stage_info(const qsx_hpsi_plan* plan, double* info) {
     info[12 + t] = W[t]->maxf;
     info[14 + t] = (double)W[t]->reds.size();
     info[16 + t] = H.split_exec_flops[t];
+    info[18 + t] = 2.0 * 16 * 16 * 4 * wave_exec_mfma(*W[t], t == 0 ? H.pairs1 : H.pairs2);
     if (H.mode == QSX_MODE_SPLIT_BF16 || H.mode == QSX_MODE_SPLIT_F32) {
       info[10 + t] = (double)((H.swork1.items.size() * (t == 0) + H.swork2.items.size() * (t == 1) + 3) / 4);
       info[8 + t] = 2;

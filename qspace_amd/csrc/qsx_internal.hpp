@@ -263,6 +263,10 @@ struct WorkList {
 };
 
 WorkList build_work(const std::vector<WorkTarget>& targets, const std::vector<PairDesc>& pairs, const WorkPolicy& pol);
+// v_mfma_f64_16x16x4f64 instructions one launch of a wave-engine work list
+// issues (padding of rows, columns and k to the tile included; 0 for the
+// workgroup-tile engine)
+double wave_exec_mfma(const WorkList& W, const std::vector<PairDesc>& pairs);
 // persistent grouped GEMM over the queues, then (if any tile is split) the reduction
 void launch_work(const WorkQueue* dqueues, int nqueues, const WorkItem* ditems, const ReduceItem* dreds, int nreds,
                  const PairDesc* dpairs, const double* A, const double* B, const double* X, double* C, double* scratch,

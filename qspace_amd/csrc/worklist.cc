@@ -451,4 +451,21 @@ WorkList build_work(const std::vector<WorkTarget>& targets, const std::vector<Pa
   return out;
 }
 
+double wave_exec_mfma(const WorkList& W, const std::vector<PairDesc>& pairs) {
+  if (W.engine != ENG_WAVE) return 0.0;
+  double n = 0.0;
+  for (const WorkItem& w : W.items) {
+    if (w.slot == -2 || w.p0 > w.p1) continue;
+    int tm, tn;
+    shape_dims(w.shape, tm, tn);
+    int64_t steps = 0;
+    for (int p = w.p0; p <= w.p1; ++p) {
+      const int kb = p == w.p0 ? w.k0 : 0, ke = p == w.p1 ? w.k1 : pairs[p].K;
+      if (ke > kb) steps += (ke - kb + 3) / 4;
+    }
+    n += (double)steps * (tm / 16) * (tn / 16);
+  }
+  return n;
+}
+
 }  // namespace qsx
