@@ -350,11 +350,25 @@ def extra_configs():
     got = y.download()
     ref = oracle_hpsi(HL, psi, HR, ferm, psi.packed())
     dav_first, dav, e0 = _time_eig(H.davidson, psi.packed(), 30)
+    # the same H*psi on the plain per-pair FP64 path and in both split modes
+    Hp = Q.HPsi(HL, psi, HR, hconj=1, fermA=ferm, mode="plain")
+    ms_plain = time_hpsi(Hp, x, y, 3, 1, graph=False)
+    del Hp
+    split = {}
+    for mode in ("split_bf16", "split_f32"):
+        Hs = Q.HPsi(HL, psi, HR, hconj=1, fermA=ferm, mode=mode)
+        ms_s = time_hpsi(Hs, x, y, 20, 3)
+        gs = y.download()
+        split[mode] = {"hpsi_ms": ms_s, "rel_err_hpsi_vs_oracle": float(np.linalg.norm(gs - ref) / np.linalg.norm(ref)),
+                       "executed_slice_gflop": float(sum(Hs.stage_info()["split_exec_flops"]) / 1e9)}
+        del Hs
     out["5_hubbard4000_davidson30"] = {"psi_elements": H.N, "useful_gflop_per_hpsi": H.flops / 1e9, "hpsi_ms": ms,
                                        "gflops": H.flops / ms / 1e6, "davidson30_ms": dav,
                                        "davidson30_first_call_ms": dav_first, "E0": e0,
                                        "rel_err_hpsi_vs_oracle": float(np.linalg.norm(got - ref) /
-                                                                       np.linalg.norm(ref))}
+                                                                       np.linalg.norm(ref)),
+                                       "plain_fp64_per_pair_ms": ms_plain, "speedup_vs_plain_fp64": ms_plain / ms,
+                                       "split": split}
     log(f"[bench] config 5: {out['5_hubbard4000_davidson30']}")
     return out
 

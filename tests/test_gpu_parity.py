@@ -209,6 +209,24 @@ def test_hpsi_hubbard_small(gpu):
     assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < TOL
 
 
+def test_split_precision_hubbard_signs(gpu):
+    """config 5 in split precision: the SU(2)xU(1) Hubbard operator with
+    exact +-1 parity signs on HL (D* = 1500) against the oracle, BF16 with
+    the default 7 slices to 1e-12 and FP32 x 3 to its stated 5e-10; the
+    signs enter through the pairs' afac like the X-symbols"""
+    HL, psi, HR = gen.hubbard_hpsi(seed=28, Dstar=1500, mu=3.0, sigma=0.8, dmax=200)
+    ferm = gen.fermion_parity_signs(HL)
+    assert (ferm < 0).any() and (ferm > 0).any()
+    x = np.random.RandomState(12).standard_normal(psi.numel())
+    ref = _hpsi_oracle(HL, psi.with_packed(x), HR, 1, fermA=ferm).packed()
+    errs = {}
+    for mode, bound in (("split_bf16", TOL), ("split_f32", 5e-10)):
+        got = Q.HPsi(HL, psi, HR, hconj=1, fermA=ferm, mode=mode).apply(x)
+        errs[mode] = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+        assert errs[mode] < bound, (mode, errs[mode])
+    print("split precision, Hubbard D*=1500: rel err vs oracle", errs, flush=True)
+
+
 def _dense_weighted(H):
     """H as a dense matrix in the orthonormal basis sqrt(w) x (symmetric)."""
     N = H.N
