@@ -183,14 +183,39 @@ class QSpace:
         return np.ascontiguousarray(np.concatenate(self.Q, axis=1))
 
     def dims(self) -> np.ndarray:
+        lay = self._packed_layout()
+        if lay is not None:
+            return lay[2].copy()
         r = self.rank
         return np.array([d.shape[:r] for d in self.data], dtype=np.int64).reshape(len(self.data), r)
 
     def om(self) -> np.ndarray:
         """per-block outer multiplicity (size of the trailing data dimension
         beyond the rank, 1 if there is none)"""
+        lay = self._packed_layout()
+        if lay is not None:
+            return lay[3].copy()
         r = self.rank
         return np.array([d.shape[r] if d.ndim > r else 1 for d in self.data], dtype=np.int32)
+
+    def _set_packed(self, buf: np.ndarray, offs: np.ndarray, dims: np.ndarray, om: np.ndarray):
+        """record that the blocks are the column-major views of one float64
+        buffer, block i at element offset offs[i] (the form the library's
+        results and with_packed produce): the marshalling then takes the
+        block addresses from the buffer instead of visiting every block"""
+        self._pk = (tuple(self.data), buf, offs, dims, om)
+
+    def _packed_layout(self):
+        """(buf, offs, dims, om) while the blocks are still the views
+        _set_packed recorded (same objects, same order), else None"""
+        pk = getattr(self, "_pk", None)
+        if pk is None:
+            return None
+        views = pk[0]
+        if len(views) != len(self.data) or any(a is not b for a, b in zip(views, self.data)):
+            self._pk = None
+            return None
+        return pk[1:]
 
     @classmethod
     def _from_parts(cls, Q, data, qtype, itags, cgw) -> "QSpace":
@@ -223,7 +248,14 @@ class QSpace:
         for d in self.data:
             out.append(np.asarray(v[o: o + d.size]).reshape(d.shape, order="F"))
             o += d.size
-        return QSpace(self.Q, out, self.qtype, self.itags, self.info["cgw"])
+        T = QSpace(self.Q, out, self.qtype, self.itags, self.info["cgw"])
+        v = np.asarray(v)
+        if (v.dtype == np.float64 and v.ndim == 1 and v.flags.c_contiguous
+                and all(a.base is v or a.size == 0 for a in out) and all(a is b for a, b in zip(out, T.data))):
+            offs = np.zeros(len(out) + 1, dtype=np.int64)
+            np.cumsum([a.size for a in out], out=offs[1:])
+            T._set_packed(v, offs, self.dims(), self.om())
+        return T
 
     def copy(self) -> "QSpace":
         return QSpace([q.copy() for q in self.Q], [d.copy(order="F") for d in self.data], self.qtype,
@@ -249,7 +281,15 @@ def to_c(T: QSpace, with_data: bool = True):
     ct.Q = Q.ctypes.data_as(C.POINTER(C.c_int32))
     ct.dims = dims.ctypes.data_as(C.POINTER(C.c_int64))
     ct.data_im = None
-    if with_data:
+    lay = T._packed_layout() if with_data else None
+    if lay is not None and lay[0].dtype == np.float64:
+        # the blocks are views of one buffer: addresses from the offsets
+        buf, offs = lay[0], lay[1]
+        addrs = (np.uint64(buf.__array_interface__["data"][0]) +
+                 8 * offs[:-1].astype(np.uint64) if n else np.zeros(1, np.uint64)).astype(np.uintp)
+        keep += [addrs, buf, T.data]
+        ct.data = addrs.ctypes.data_as(C.POINTER(C.POINTER(C.c_double)))
+    elif with_data:
         # block addresses gathered into one array (no per-block ctypes
         # objects); a block that is not column-major float64 gets a copy
         cplx = T.is_complex()
@@ -288,7 +328,10 @@ def to_c(T: QSpace, with_data: bool = True):
         ct.cgw = cg.ctypes.data_as(C.POINTER(C.c_double))
     else:
         ct.cgw = None
-    om = T.om() if n and r and any(d.ndim > r for d in T.data) else np.ones(0, np.int32)
+    if lay is not None:
+        om = lay[3]
+    else:
+        om = T.om() if n and r and any(d.ndim > r for d in T.data) else np.ones(0, np.int32)
     if om.size and om.max() > 1:
         keep.append(om)
         ct.om = om.ctypes.data_as(C.POINTER(C.c_int32))
@@ -315,7 +358,9 @@ def from_out(o: QsxTensorOut, with_data: bool = True) -> QSpace:
             buf = buf + 1j * np.ctypeslib.as_array(o.data_im, shape=(max(1, tot),))[:tot]
         ol = offs.tolist()
         data = [buf[ol[i]: ol[i + 1]].reshape(dl[i] if r else (), order="F") for i in range(n)]
+        packed = (buf, offs, dims) if buf.dtype == np.float64 else None
     else:
+        packed = None
         for i in range(n):
             data.append(np.zeros(tuple(dl[i]), order="F"))
     itags = [unpack_itag(int(o.itags[i])) for i in range(r)] if o.itags else []
@@ -327,7 +372,12 @@ def from_out(o: QsxTensorOut, with_data: bool = True) -> QSpace:
     Qlist = [np.ascontiguousarray(Qflat[:, k * q:(k + 1) * q]) for k in range(r)]
     if not r:
         return _scalar_qspace(data, o.qtype.decode(), cgw)
-    return QSpace._from_parts(Qlist, data, o.qtype.decode(), itags, cgw)
+    T = QSpace._from_parts(Qlist, data, o.qtype.decode(), itags, cgw)
+    if packed is not None:
+        om = (np.ctypeslib.as_array(o.om, shape=(max(1, n),))[:n].astype(np.int32) if o.om
+              else np.ones(n, np.int32))
+        T._set_packed(packed[0], packed[1], packed[2], om)
+    return T
 
 
 def _scalar_qspace(data, qtype, cgw):

@@ -42,9 +42,18 @@ def _arr(p, n, dt):
     return np.ctypeslib.as_array(C.cast(p, C.POINTER(dt)), shape=(n,)).copy() if p else None
 
 
-@pytest.mark.parametrize("k", range(6))
+@pytest.mark.parametrize("k", range(12))
 def test_encoders_agree(k):
-    T = _tensors()[k]
+    """the product's and the oracle's encoders give the same struct; cases
+    6.. are the same tensors re-made with with_packed (blocks as views of
+    one buffer: the product takes their addresses from the offsets)"""
+    ts = _tensors()
+    T = ts[k % len(ts)]
+    if k >= len(ts):
+        if T.is_complex():
+            pytest.skip("complex blocks are not views of one float64 buffer")
+        T = T.with_packed(np.random.RandomState(k).standard_normal(T.numel()))
+        assert T._packed_layout() is not None
     a, ka = to_c(T)
     b, kb = IO.encode(T)
     n, r, q = T.nblk, T.rank, T.qdim
@@ -87,3 +96,30 @@ def test_decoders_agree():
     assert X.itags == Y.itags == ["R", "R*"]
     assert all(np.array_equal(x, y) for x, y in zip(X.Q, Y.Q))
     assert all(np.array_equal(x, y) for x, y in zip(X.data, Y.data))
+    # a decoded result is one buffer: the product's encoder takes the block
+    # addresses from its offsets, and they are the blocks' own
+    assert Y._packed_layout() is not None
+    a, ka = to_c(Y)
+    for i, d in enumerate(Y.data):
+        assert C.cast(a.data[i], C.c_void_p).value == d.__array_interface__["data"][0]
+
+
+def test_packed_layout_follows_the_blocks():
+    """the buffer layout recorded by with_packed is used only while the
+    blocks are the recorded views: replacing a block drops it, and the
+    struct then carries the new block's address"""
+    T = gen.u1_mps(seed=5, Lq=(-2, 0, 2), Rq=(-3, -1, 1, 3), dmin=2, dmax=4)
+    v = np.arange(T.numel(), dtype=np.float64)
+    P = T.with_packed(v)
+    lay = P._packed_layout()
+    assert lay is not None and lay[0] is v
+    assert np.array_equal(P.dims(), T.dims()) and np.array_equal(P.om(), T.om())
+    a, ka = to_c(P)
+    for i, d in enumerate(P.data):
+        assert C.cast(a.data[i], C.c_void_p).value == d.__array_interface__["data"][0]
+    new = np.full(P.data[1].shape, -1.0, order="F")
+    P.data[1] = new
+    assert P._packed_layout() is None
+    b, kb = to_c(P)
+    assert C.cast(b.data[1], C.c_void_p).value == new.__array_interface__["data"][0]
+    assert np.array_equal(_arr(b.data[1], new.size, C.c_double), new.ravel(order="F"))
