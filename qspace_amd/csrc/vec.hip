@@ -1,7 +1,7 @@
 // vec.hip -- FP64 vector kernels of the Lanczos / Davidson loops
 // (RunEigs Source/eigs2SiteQS.cc:98-334, update_psi_2site.m:223-391).
-// Reductions are two-stage with a fixed partition (DOT_BLOCKS partials
-// summed in index order) so every dot product is deterministic; scalars
+// Reductions are two-stage with a fixed partition (DOT_BLOCKS partials,
+// each summed in one fixed order) so every dot product is deterministic; scalars
 // stay in device memory so an iteration needs no host round trip and can
 // be captured in a hipGraph.
 #include <hip/hip_runtime.h>
@@ -29,6 +29,28 @@ __device__ __forceinline__ double block_sum(double v) {
   return s;  // valid in thread 0
 }
 
+// the DOT_BLOCKS partials of one reduction summed by one wave (two per lane,
+// then a fixed shuffle tree): every block that needs the value sums it in
+// the same order, so all of them hold the same bits; the result is in every
+// lane
+static_assert(DOT_BLOCKS == 128, "wave_sum_partials reads two partials per lane");
+__device__ __forceinline__ double wave_sum_partials(const double* __restrict__ p) {
+  const int lane = threadIdx.x & 63;
+  double s = p[lane] + p[lane + 64];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+  return __shfl(s, 0, 64);
+}
+
+// c[j] = sum_b partial[j][b] for j < m, one wave per j
+__device__ __forceinline__ void sum_coefficients(const double* __restrict__ partial, int m, double* c) {
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int j = wv; j < m; j += RT / 64) {
+    const double s = wave_sum_partials(partial + (int64_t)j * DOT_BLOCKS);
+    if (lane == 0) c[j] = s;
+  }
+  __syncthreads();
+}
+
 // partial[j*DOT_BLOCKS + b] = sum over the b-th slice of V_j . w
 __global__ __launch_bounds__(RT) void k_mdot_partial(const double* __restrict__ V, int64_t ldv,
                                                       const double* __restrict__ w, int64_t n,
@@ -49,16 +71,6 @@ __global__ __launch_bounds__(RT) void k_mdot_final(const double* __restrict__ pa
   for (int b = threadIdx.x; b < DOT_BLOCKS; b += RT) s += partial[(int64_t)j * DOT_BLOCKS + b];
   s = block_sum(s);
   if (threadIdx.x == 0) out[j] = accumulate ? out[j] + s : s;
-}
-
-// w -= sum_j c[j] V_j   (j < m)
-__global__ __launch_bounds__(RT) void k_gs_update(double* __restrict__ w, const double* __restrict__ V, int64_t ldv,
-                                                   const double* __restrict__ c, int m, int64_t n) {
-  for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < n; i += (int64_t)gridDim.x * RT) {
-    double s = 0;
-    for (int j = 0; j < m; ++j) s += c[j] * V[(int64_t)j * ldv + i];
-    w[i] -= s;
-  }
 }
 
 // y = sum_j c[j] V_j
@@ -98,47 +110,25 @@ __global__ __launch_bounds__(RT) void k_normalize(const double* __restrict__ src
   for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < n; i += (int64_t)gridDim.x * RT) dst[i] = src[i] * inv;
 }
 
-// w -= sum_j c[j] V_j with c[j] = sum_b partial[j][b] summed by every block
-// (the same fixed order everywhere: deterministic); block 0 also writes
-// c[cj] to *cout when cout is given
-constexpr int kMaxM = 512;
-__global__ __launch_bounds__(RT) void k_gs_update_p(double* __restrict__ w, const double* __restrict__ V, int64_t ldv,
-                                                     const double* __restrict__ partial, int m, int64_t n,
-                                                     double* __restrict__ cout, int cj) {
-  __shared__ double c[kMaxM];
-  for (int j = threadIdx.x; j < m; j += RT) {
-    double s = 0;
-    for (int b = 0; b < DOT_BLOCKS; ++b) s += partial[(int64_t)j * DOT_BLOCKS + b];
-    c[j] = s;
-  }
-  __syncthreads();
-  if (cout && blockIdx.x == 0 && threadIdx.x == 0) *cout = c[cj];
-  for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < n; i += (int64_t)gridDim.x * RT) {
-    double s = 0;
-    for (int j = 0; j < m; ++j) s += c[j] * V[(int64_t)j * ldv + i];
-    w[i] -= s;
-  }
-}
-
 // One Gram-Schmidt step fused with the next reduction: w -= sum_j c[j] V_j
 // (c from the partials pin[j][b], every block summing them in the same
 // order), then pout[j][b] = <V_j, w> over block b's slice (jn vectors from
 // V, or <w, w> if V2 is null).  The grid is exactly DOT_BLOCKS blocks with
-// the slice layout of k_mdot_partial, so the partials are the same sums in
-// the same order: deterministic.
+// the slice layout of k_mdot_partial; each partial is its slice's sum in a
+// fixed order (elements in index order per thread, a fixed shuffle tree,
+// the waves in order): deterministic.  The dots of kDotGroup vectors are
+// accumulated in one pass over the slice (their loads in flight together)
+// and reduced with one barrier.
+constexpr int kMaxM = 512;  // largest subspace (coefficients held in LDS)
+constexpr int kDotGroup = 8;
 __global__ __launch_bounds__(RT) void k_gs_update_mdot(double* __restrict__ w, const double* __restrict__ V,
                                                         int64_t ldv, const double* __restrict__ pin, int m,
                                                         int64_t n, double* __restrict__ cout, int cj,
                                                         const double* __restrict__ V2, int jn,
                                                         double* __restrict__ pout) {
   __shared__ double c[kMaxM];
-  __shared__ double red[RT / 64];
-  for (int j = threadIdx.x; j < m; j += RT) {
-    double s = 0;
-    for (int b = 0; b < DOT_BLOCKS; ++b) s += pin[(int64_t)j * DOT_BLOCKS + b];
-    c[j] = s;
-  }
-  __syncthreads();
+  __shared__ double red[RT / 64][kDotGroup];
+  sum_coefficients(pin, m, c);
   if (cout && blockIdx.x == 0 && threadIdx.x == 0) *cout = c[cj];
   const int b = blockIdx.x;
   for (int64_t i = (int64_t)b * RT + threadIdx.x; i < n; i += (int64_t)DOT_BLOCKS * RT) {
@@ -147,20 +137,31 @@ __global__ __launch_bounds__(RT) void k_gs_update_mdot(double* __restrict__ w, c
     w[i] -= s;
   }
   // the slice's updated w is this thread's own elements: no barrier needed
-  // before the dots; one block-wide sum per output vector
+  // before the dots
   const int nv = V2 ? jn : 1;
-  for (int j = 0; j < nv; ++j) {
-    const double* v = V2 ? V2 + (int64_t)j * ldv : w;
-    double s = 0;
-    for (int64_t i = (int64_t)b * RT + threadIdx.x; i < n; i += (int64_t)DOT_BLOCKS * RT) s += v[i] * w[i];
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (lane == 0) red[wv] = s;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int j0 = 0; j0 < nv; j0 += kDotGroup) {
+    const int ng = min(kDotGroup, nv - j0);
+    double acc[kDotGroup];
+#pragma unroll
+    for (int q = 0; q < kDotGroup; ++q) acc[q] = 0.0;
+    for (int64_t i = (int64_t)b * RT + threadIdx.x; i < n; i += (int64_t)DOT_BLOCKS * RT) {
+      const double wi = w[i];
+#pragma unroll
+      for (int q = 0; q < kDotGroup; ++q)
+        if (q < ng) acc[q] += (V2 ? V2[(int64_t)(j0 + q) * ldv + i] : wi) * wi;
+    }
+#pragma unroll
+    for (int q = 0; q < kDotGroup; ++q) {
+      double s = acc[q];
+      for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+      if (lane == 0) red[wv][q] = s;
+    }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < ng) {
       double t = 0;
-      for (int q = 0; q < RT / 64; ++q) t += red[q];
-      pout[(int64_t)j * DOT_BLOCKS + b] = t;
+      for (int q = 0; q < RT / 64; ++q) t += red[q][threadIdx.x];
+      pout[(int64_t)(j0 + threadIdx.x) * DOT_BLOCKS + b] = t;
     }
     __syncthreads();
   }
@@ -172,23 +173,15 @@ __global__ __launch_bounds__(RT) void k_normalize_p(const double* __restrict__ s
                                                      const double* __restrict__ partial, double* __restrict__ beta,
                                                      int64_t n) {
   __shared__ double nb;
-  if (threadIdx.x == 0) {
-    double s = 0;
-    for (int b = 0; b < DOT_BLOCKS; ++b) s += partial[b];
-    nb = s;
+  if (threadIdx.x < 64) {
+    const double s = wave_sum_partials(partial);
+    if (threadIdx.x == 0) nb = s;
   }
   __syncthreads();
   const double b = sqrt(nb);
   const double inv = b > 0 ? 1.0 / b : 0.0;
   if (blockIdx.x == 0 && threadIdx.x == 0 && beta) *beta = b;
   for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < n; i += (int64_t)gridDim.x * RT) dst[i] = src[i] * inv;
-}
-
-// y = x - a * z  with scalar a on the device
-__global__ __launch_bounds__(RT) void k_axmy(double* __restrict__ y, const double* __restrict__ x,
-                                              const double* __restrict__ z, const double* __restrict__ a, int64_t n) {
-  const double s = *a;
-  for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < n; i += (int64_t)gridDim.x * RT) y[i] = x[i] - s * z[i];
 }
 
 int grid_for(int64_t n) { return (int)std::min<int64_t>(2048, std::max<int64_t>(1, (n + RT - 1) / RT)); }
@@ -210,13 +203,6 @@ void vec_mdot_partial(const double* V, int64_t ldv, int m, const double* w, int6
   hipLaunchKernelGGL(k_mdot_partial, dim3(DOT_BLOCKS, m), dim3(RT), 0, s, V, ldv, w, n, partial);
 }
 
-void vec_gs_update_partial(double* w, const double* V, int64_t ldv, const double* partial, int m, int64_t n,
-                           double* cout, int cj, hipStream_t s) {
-  if (m <= 0) return;
-  if (m > kMaxM) fail_vec(m);
-  hipLaunchKernelGGL(k_gs_update_p, dim3(grid_for(n)), dim3(RT), 0, s, w, V, ldv, partial, m, n, cout, cj);
-}
-
 void vec_gs_update_mdot(double* w, const double* V, int64_t ldv, const double* pin, int m, int64_t n, double* cout,
                         int cj, const double* V2, int jn, double* pout, hipStream_t s) {
   if (m <= 0) return;
@@ -230,11 +216,6 @@ void vec_normalize_partial(const double* src, double* dst, const double* partial
   hipLaunchKernelGGL(k_normalize_p, dim3(grid_for(n)), dim3(RT), 0, s, src, dst, partial, beta, n);
 }
 
-void vec_gs_update(double* w, const double* V, int64_t ldv, const double* c, int m, int64_t n, hipStream_t s) {
-  if (m <= 0) return;
-  hipLaunchKernelGGL(k_gs_update, dim3(grid_for(n)), dim3(RT), 0, s, w, V, ldv, c, m, n);
-}
-
 void vec_combine(double* y, const double* V, int64_t ldv, const double* c, int m, int64_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_combine, dim3(grid_for(n)), dim3(RT), 0, s, y, V, ldv, c, m, n);
 }
@@ -246,10 +227,6 @@ void vec_combine_res(double* u, double* r, const double* V, const double* HV, in
 
 void vec_normalize(const double* src, double* dst, const double* nrm2, double* beta, int64_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_normalize, dim3(grid_for(n)), dim3(RT), 0, s, src, dst, nrm2, beta, n);
-}
-
-void vec_axmy(double* y, const double* x, const double* z, const double* a, int64_t n, hipStream_t s) {
-  hipLaunchKernelGGL(k_axmy, dim3(grid_for(n)), dim3(RT), 0, s, y, x, z, a, n);
 }
 
 }  // namespace qsx

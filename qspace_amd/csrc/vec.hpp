@@ -11,13 +11,10 @@ constexpr int DOT_BLOCKS = 128;  // fixed partition of every reduction
 // out[j] (+)= <V_j, w> for j < m; partial needs m*DOT_BLOCKS doubles
 void vec_mdot(const double* V, int64_t ldv, int m, const double* w, int64_t n, double* partial, double* out,
               int accumulate, hipStream_t s);
-void vec_gs_update(double* w, const double* V, int64_t ldv, const double* c, int m, int64_t n, hipStream_t s);
 // the same steps with the reduction's final sum folded into the consumer:
 // partial[j*DOT_BLOCKS + b] from vec_mdot_partial
 void vec_mdot_partial(const double* V, int64_t ldv, int m, const double* w, int64_t n, double* partial,
                       hipStream_t s);
-void vec_gs_update_partial(double* w, const double* V, int64_t ldv, const double* partial, int m, int64_t n,
-                           double* cout, int cj, hipStream_t s);
 void vec_normalize_partial(const double* src, double* dst, const double* partial, double* beta, int64_t n,
                            hipStream_t s);
 // w -= V c(pin) (block 0 writes c[cj] to cout if given), then pout = the
@@ -30,6 +27,5 @@ void vec_combine(double* y, const double* V, int64_t ldv, const double* c, int m
 void vec_combine_res(double* u, double* r, const double* V, const double* HV, int64_t ldv, const double* c, int m,
                      int64_t n, hipStream_t s);
 void vec_normalize(const double* src, double* dst, const double* nrm2, double* beta, int64_t n, hipStream_t s);
-void vec_axmy(double* y, const double* x, const double* z, const double* a, int64_t n, hipStream_t s);
 
 }  // namespace qsx
