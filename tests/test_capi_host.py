@@ -226,3 +226,24 @@ def test_ortho_argument_errors():
         with pytest.raises(Q.QsxError) as e:
             f()
         assert e.value.code == code
+
+
+def test_plain_c_caller(tmp_path):
+    """include/qsx.h is a C99 header and a C program links against
+    libqsx.so: tests/c/capi_example.c (host-only entry points: ctr-idx
+    parsing, CG coefficients, outer-multiplicity count, packed offsets)
+    built with -std=c99 -pedantic -Werror and run"""
+    import shutil
+    import subprocess
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("no C compiler")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    lib = os.path.join(root, "qspace_amd")
+    exe = str(tmp_path / "capi_example")
+    subprocess.run([gcc, "-std=c99", "-Wall", "-Wextra", "-Werror", "-pedantic", "-I", os.path.join(root, "include"),
+                    os.path.join(root, "tests", "c", "capi_example.c"), "-L", lib, "-lqsx", f"-Wl,-rpath,{lib}", "-lm",
+                    "-o", exe], check=True, capture_output=True, text=True)
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stderr
+    assert "capi_example: ok" in out.stdout
