@@ -16,11 +16,15 @@ byte counts of the GEMM kernels against it.
 Besides the headline H*psi the JSON line carries "configs": the benchmark
 configurations 1 (U(1) MPS overlap, CPU oracle and GPU), 2 (SU(2)
 environment step), 3 (Heisenberg D*=2000, 20 Lanczos iterations), 4 (split
-precision: BF16-slice H*psi error/time table on the headline operator) and 5
-(SU(2)xU(1) Hubbard D*=4000 with parity signs, 30 Davidson iterations),
-and "block_ops": permuteQS / plusQS bandwidth on the SU(2) environments,
-the U(1) environment update in the cell form of updateHK.m and the
-orthoQS truncation of a U(1) two-site wave function.
+precision: BF16-slice and FP32x3 H*psi error/time on the headline operator),
+5 (SU(2)xU(1) Hubbard D*=4000 with parity signs, 30 Davidson iterations, the
+plain per-pair path and both split modes) and the pseudo-MPO H*psi of
+get_HPsi.m:62-83 at D*=4000; and "block_ops": permuteQS / plusQS bandwidth
+on the SU(2) environments, the U(1) and SU(2) environment updates in the
+cell form of updateHK.m and the orthoQS truncations.
+
+    python bench.py --split-profile bf16|f32    # only the split H*psi (profiler runs)
+    python bench.py --save-policy FILE          # also write the autotuned stage policies
 """
 from __future__ import annotations
 
@@ -130,6 +134,55 @@ def oracle_hpsi(HL, psi, HR, ferm, x):
         for r, d in zip(Ht.QIDX, Ht.data):
             acc[key[tuple(r)]] += d
     return np.concatenate([a.ravel(order="F") for a in acc])
+
+
+def oracle_pseudo_hpsi(HKL, psi, HKR, OPL, OPR, hconj, x):
+    """the oracle's pseudo-MPO H*psi of the packed vector x: the explicit
+    contraction chain of get_HPsi.m:62-83 (HK_L.psi + psi.HK_R + per
+    operator pair {OP_L,'*',psi}.OP_R [+ {OP_L,psi}.OP_R' where hconj])"""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    xs = psi.with_packed(x)
+    terms = [O.contract(HKL, [1], xs, [0], skip_zero=0), O.contract(xs, [1], HKR, [1], skip_zero=0)]
+    for L, R, h in zip(OPL, OPR, hconj):
+        T1 = O.contract(L, [0], xs, [0], conjA=1, skip_zero=0)
+        terms.append(O.contract(T1, [1, 2], R, [2, 1], skip_zero=0))
+        if h:
+            T2 = O.contract(L, [1], xs, [0], skip_zero=0)
+            terms.append(O.contract(T2, [1, 2], R, [2, 0], conjB=1, skip_zero=0))
+    key = {tuple(r): i for i, r in enumerate(psi.QIDX)}
+    acc = [np.zeros_like(d) for d in psi.data]
+    for Ht in terms:
+        for r, d in zip(Ht.QIDX, Ht.data):
+            acc[key[tuple(r)]] += d
+    return np.concatenate([a.ravel(order="F") for a in acc])
+
+
+def pseudo_mpo():
+    """pseudo-MPO two-site H*psi (get_HPsi.m:62-83, the form update_psi_2site
+    uses without a full MPO): SU(2) Heisenberg block Hamiltonians HK_L, HK_R
+    and one spin operator pair with its Hermitian-conjugate term, D* = 4000;
+    one multi-term plan, hipGraph replay, against the oracle's chain and the
+    per-pair plain path"""
+    import qspace_amd as Q
+    from qspace_amd import gen
+    HKL, psi, HKR, OPL, OPR = gen.heis_pseudo_hpsi(seed=1008, Dstar=4000, nop=1)
+    hc = [1]
+    H = Q.HPsi.pseudo(HKL, psi, HKR, OPL, OPR, hc)
+    H.autotune(5)
+    x = Q.DeviceBuffer.from_array(psi.packed())
+    y = Q.DeviceBuffer(H.N)
+    ms = time_hpsi(H, x, y, 50, 5)
+    got = y.download()
+    ref = oracle_pseudo_hpsi(HKL, psi, HKR, OPL, OPR, hc, psi.packed())
+    Hp = Q.HPsi.pseudo(HKL, psi, HKR, OPL, OPR, hc, mode="plain")
+    ms_plain = time_hpsi(Hp, x, y, 3, 1, graph=False)
+    out = {"workload": "su2_heisenberg_pseudo_mpo_two_site_hpsi_Dstar4000", "psi_elements": H.N,
+           "useful_gflop_per_hpsi": H.flops / 1e9, "hpsi_ms": ms, "gflops": H.flops / ms / 1e6,
+           "rel_err_hpsi_vs_oracle": float(np.linalg.norm(got - ref) / np.linalg.norm(ref)),
+           "plain_fp64_per_pair_ms": ms_plain, "speedup_vs_plain_fp64": ms_plain / ms}
+    log(f"[bench] pseudo-MPO H*psi: {out}")
+    return out
 
 
 def host_cpus():
@@ -674,6 +727,7 @@ def main():
     configs = bops = None
     if not args.lite:
         configs = extra_configs()
+        configs["pseudo_mpo_heis4000"] = pseudo_mpo()
         bops = block_ops()
         bops["u1_env_update"] = env_update()
         bops["su2_env_update"] = env_update_su2()
