@@ -15,6 +15,7 @@
 #include "contract.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <functional>
 #include <mutex>
 #include <atomic>
@@ -119,6 +120,14 @@ void require_no_om(const TensorDesc& T, const char* what) {
 }
 
 // ---------------------------------------------------------------- device
+// Kernel arguments in device memory (HIP_FORCE_DEV_KERNARG): the command
+// processor then reads a launch's arguments from HBM instead of host memory
+// over PCIe, which shortens every dispatch (the eigensolver loops are bound
+// by their launches).  Set when the library is loaded, before its first HIP
+// call, unless the host program chose a value itself; no effect when the
+// process initialised the HIP runtime earlier.
+__attribute__((constructor)) static void kernarg_in_device_memory() { setenv("HIP_FORCE_DEV_KERNARG", "1", 0); }
+
 // the library's device and stream, set up once per process (the first call
 // from any thread; the others wait)
 static std::atomic<int> g_dev{-1};

@@ -247,3 +247,14 @@ def test_plain_c_caller(tmp_path):
     out = subprocess.run([exe], capture_output=True, text=True, timeout=60)
     assert out.returncode == 0, out.stderr
     assert "capi_example: ok" in out.stdout
+
+
+def test_kernel_arguments_in_device_memory():
+    """loading the library asks the HIP runtime for kernel arguments in
+    device memory (HIP_FORCE_DEV_KERNARG=1) unless the host program set the
+    variable itself"""
+    _capi.lib()
+    libc = C.CDLL(None)
+    libc.getenv.restype = C.c_char_p
+    user = os.environ.get("HIP_FORCE_DEV_KERNARG")
+    assert libc.getenv(b"HIP_FORCE_DEV_KERNARG") == (user.encode() if user is not None else b"1")
