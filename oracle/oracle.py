@@ -796,7 +796,8 @@ def ortho(PSI: QSpace, K, Nkmin=0, Nkeep=-1, stol=0.0, itag=""):
             DV.append(xd)
 
     def mk(Qs, D, W, tg):
-        cg = np.array(W).reshape(len(D), q) if su2 and D else None
+        # an SU(2) result keeps its weight columns when nothing is kept
+        cg = np.array(W, dtype=np.float64).reshape(len(D), q) if su2 else None
         return QSpace([np.array(x, dtype=np.int32).reshape(len(D), q) for x in Qs], D, PSI.qtype, tg, cg)
 
     info = dict(svd=SM, svd2tr=s2t / snorm2, sv2tot=snorm2, sfac=sfac, Nkeep=nk, Ntot=len(SM), flag=flag,

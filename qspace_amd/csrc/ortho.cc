@@ -437,7 +437,7 @@ OrthoResult ortho_oneshot(const qsx_tensor* pt, int K, int Nkmin, int64_t Nkeep,
         QSX_HIP(hipMemcpyAsync(dsteps, steps.data(), sizeof(SvdBlockStep) * steps.size(), hipMemcpyHostToDevice, s));
       QSX_HIP(hipMemsetAsync(drot, 0, sizeof(int32_t) * items.size(), s));
       for (size_t st = 0; st + 1 < off.size(); ++st)
-        launch_jacobi_block(ditems, dsteps + off[st], (int)(off[st + 1] - off[st]), dwork.p, drot, s);
+        launch_jacobi_block(ditems, dsteps + off[st], (int)(off[st + 1] - off[st]), dwork.p, drot, sw, s);
       QSX_HIP(hipMemcpyAsync(rot.data(), drot, sizeof(int32_t) * items.size(), hipMemcpyDeviceToHost, s));
       QSX_HIP(hipStreamSynchronize(s));
       std::vector<int> next;

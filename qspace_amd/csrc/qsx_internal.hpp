@@ -436,7 +436,7 @@ struct SvdBlockStep {
   int32_t grp, p0, p1, q0, q1, pad;
 };
 void launch_jacobi_block(const SvdItem* dgroups, const SvdBlockStep* dsteps, int nsteps, double* work, int32_t* rot,
-                         hipStream_t s);
+                         int sweep, hipStream_t s);
 void launch_svd_finalize(const SvdItem* dgroups, const int32_t* dwhich, int n, double* work, double* sig,
                          hipStream_t s);
 void launch_svd_init_v(const SvdItem* dgroups, const int32_t* dwhich, int n, double* work, hipStream_t s);

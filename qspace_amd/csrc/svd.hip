@@ -20,6 +20,15 @@ constexpr int SVD_NT = 1024;          // threads per workgroup
 constexpr int SVD_NT_BLOCK = 512;     // threads per workgroup of the block steps
 constexpr int SVD_SEG = 16;           // lanes per column pair
 constexpr int SVD_MAX_SWEEPS = 60;
+// Orthogonality target of a column pair: sqrt(m) eps (as LAPACK's dgesvj)
+// for the first kTightSweeps sweeps, then m eps, the rounding bound of the
+// length-m dot products that decide a rotation -- a pair whose computed
+// cosine stays above the tight target only through that rounding (e.g.
+// columns of nearly equal norm) would otherwise rotate forever.
+constexpr int kTightSweeps = 20;
+__host__ __device__ inline double jacobi_tol(int m, int sweep) {
+  return (sweep < kTightSweeps ? sqrt((double)m) : (double)m) * DBL_EPSILON;
+}
 
 // v + (v of the lane that the DPP row pattern CTRL names); a segment is one
 // DPP row of 16 lanes, active or inactive as a whole
@@ -44,9 +53,15 @@ __device__ __forceinline__ double seg_sum(double v) {
 }
 
 // the rotation of the Gram matrix [al ga; ga be] that zeroes ga (smaller
-// angle); false when |ga| <= tol sqrt(al be): the pair is orthogonal
+// angle); false when |ga| <= tol sqrt(al be): the pair is orthogonal.  A
+// pair whose smaller column is below tol times the larger (norms) is left
+// alone as well: such a column is rounding noise of a rank-deficient matrix
+// (its singular value is below what FP64 resolves against the larger one),
+// and rotating it against the larger column only makes new noise of the
+// same size, so the sweeps would never end.
 __device__ __forceinline__ bool jacobi_cs(double al, double be, double ga, double tol, double& c, double& s) {
   if (!(fabs(ga) > tol * sqrt(al) * sqrt(be))) return false;
+  if (fmin(al, be) <= tol * tol * fmax(al, be)) return false;
   const double zeta = (be - al) / (2.0 * ga);
   const double az = fabs(zeta);
   const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (az + (az > 1e150 ? az : sqrt(1.0 + zeta * zeta)));
@@ -350,12 +365,11 @@ __global__ __launch_bounds__(SVD_NT) void k_jacobi_svd(const SvdItem* __restrict
   for (int64_t e = tid; e < (int64_t)n * n; e += SVD_NT) V[e] = (e % (n + 1)) == 0 ? 1.0 : 0.0;
   __syncthreads();
 
-  const double tol = (double)m * DBL_EPSILON;
   const int seg = tid / SVD_SEG, lane = tid % SVD_SEG;
   constexpr int nseg = SVD_NT / SVD_SEG;
   int sw = 0;
   for (; sw < SVD_MAX_SWEEPS; ++sw)
-    if (!jacobi_pairs_m<SVD_NT, false>(A, V, n, 0, m, n, tol, &rotated)) break;
+    if (!jacobi_pairs_m<SVD_NT, false>(A, V, n, 0, m, n, jacobi_tol(m, sw), &rotated)) break;
   // sigma_j = |a_j|, u_j = a_j / sigma_j (a zero column stays zero)
   for (int j = seg; j < n; j += nseg) {
     double* aj = A + (int64_t)j * m;
@@ -380,7 +394,8 @@ __global__ __launch_bounds__(SVD_NT) void k_jacobi_svd(const SvdItem* __restrict
 // if anything rotated
 __global__ __launch_bounds__(SVD_NT_BLOCK) void k_jacobi_block(const SvdItem* __restrict__ groups,
                                                          const SvdBlockStep* __restrict__ steps,
-                                                         double* __restrict__ work, int32_t* __restrict__ rot) {
+                                                         double* __restrict__ work, int32_t* __restrict__ rot,
+                                                         int sweep) {
   __shared__ double lds[SVD_LDS_DOUBLES];
   __shared__ int rotated;
   const SvdBlockStep st = steps[blockIdx.x];
@@ -402,7 +417,7 @@ __global__ __launch_bounds__(SVD_NT_BLOCK) void k_jacobi_block(const SvdItem* __
   }
   __syncthreads();
   // one block (q0 == q1): all its pairs; two blocks: the pairs across them
-  const double tol = (double)m * DBL_EPSILON;
+  const double tol = jacobi_tol(m, sweep);
   const bool any = w == np_ ? jacobi_pairs_m<SVD_NT_BLOCK, false>(A, V, w, 0, m, n, tol, &rotated)
                             : jacobi_pairs_m<SVD_NT_BLOCK, true>(A, V, np_, w - np_, m, n, tol, &rotated);
   for (int64_t e = tid; e < (int64_t)w * m; e += SVD_NT_BLOCK) {
@@ -478,9 +493,9 @@ void launch_jacobi_svd(const SvdItem* ditems, int nitems, double* work, double* 
 int jacobi_max_sweeps() { return SVD_MAX_SWEEPS; }
 
 void launch_jacobi_block(const SvdItem* dgroups, const SvdBlockStep* dsteps, int nsteps, double* work, int32_t* rot,
-                         hipStream_t s) {
+                         int sweep, hipStream_t s) {
   if (nsteps <= 0) return;
-  hipLaunchKernelGGL(k_jacobi_block, dim3(nsteps), dim3(SVD_NT_BLOCK), 0, s, dgroups, dsteps, work, rot);
+  hipLaunchKernelGGL(k_jacobi_block, dim3(nsteps), dim3(SVD_NT_BLOCK), 0, s, dgroups, dsteps, work, rot, sweep);
   QSX_HIP(hipGetLastError());
 }
 

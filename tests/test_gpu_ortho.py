@@ -129,3 +129,33 @@ def test_ortho_edge_cases(gpu):
     _check(Z, 2, Nkeep=20)
     _check(u1_rank2(), 1, Nkeep=5)
     _check(u1_rank2(seed=6, qs=(-4, -2, 0, 2, 4), dmin=20, dmax=150), 1, Nkeep=200, stol=1e-3)
+
+
+def test_ortho_rank_deficient_groups(gpu):
+    """group matrices of exactly deficient rank (every block of a two-site
+    psi a product of thin factors, several blocks per group): the zero
+    singular values come out at rounding level, the sweeps end (a column of
+    rounding noise is not rotated against a larger one), and the truncated
+    and the full products agree with the oracle"""
+    rng = np.random.RandomState(17)
+    P = gen.u1_two_site_psi(seed=6, Lq=(-4, -2, 0, 2, 4), Rq=(-4, -2, 0, 2, 4), dmin=3, dmax=7)
+    data = []
+    for d in P.data:
+        s = d.shape
+        L = rng.standard_normal((s[0] * s[1], 2))
+        R = rng.standard_normal((2, s[2] * s[3]))
+        data.append(np.asfortranarray((L @ R).reshape(s, order="F")))
+    D = Q.QSpace(P.Q, data, P.qtype, P.itags)
+    for kw in ({}, {"Nkeep": 6}):
+        A1, A2, info = Q.orthoQS(D, 2, **kw)
+        R1, R2, rinfo = oracle_ortho(D, 2, **kw)
+        assert (info["Nkeep"], info["Ntot"]) == (rinfo["Nkeep"], rinfo["Ntot"])
+        sv, rsv = np.asarray(info["svd"]), np.asarray(rinfo["svd"])
+        assert np.abs(sv - rsv).max() <= 1e-12 * rsv.max()
+        assert (rsv < 1e-13 * rsv.max()).sum() > 0  # the draw is rank-deficient
+        X = O.contract(A1, [2], A2, [2], skip_zero=0)
+        XR = O.contract(R1, [2], R2, [2], skip_zero=0)
+        assert_same_structure(X, XR)
+        num = sum(float(np.sum((a - b) ** 2)) for a, b in zip(X.data, XR.data))
+        den = sum(float(np.sum(b * b)) for b in XR.data)
+        assert np.sqrt(num / den) < 1e-12
