@@ -5,7 +5,8 @@ whose contracted legs sit at random positions, random conj flags and output
 permutations, and checks contractQS, permuteQS, plusQS and orthoQS through
 the C-ABI against the oracle on the same inputs: structure bit-exact, data
 within 1e-12 relative (bit-exact for permute / plus); SU(2) contractions
-and truncations of rank-3 tensors likewise.
+and truncations of rank-3 tensors likewise; complex operands with
+independent conj flags.
 """
 import itertools
 
@@ -82,6 +83,34 @@ def test_random_contractions(gpu, seed):
     assert_same_structure(C2, R2)
     if R2.nblk:
         assert rel_err(C2, R2) < 1e-12
+
+
+def _complexify(T, rng):
+    """complex data on T's structure; now and then every other block real-valued in a complex dtype"""
+    mix = rng.rand() < 0.3
+    return QSpace(T.Q, [np.asfortranarray(np.asarray(d, dtype=np.complex128) if mix and i % 2 else
+                                          d + 1j * rng.standard_normal(d.shape)) for i, d in enumerate(T.data)],
+                  T.qtype, T.itags)
+
+
+@pytest.mark.parametrize("seed", range(3000, 3040))
+def test_random_complex_contractions(gpu, seed):
+    """complex A and/or B with independent conj flags: conj acts on the data"""
+    rng, A, ia, B, ib = _pair(seed)
+    if not A.nblk or not B.nblk:
+        pytest.skip("empty draw")
+    A = _complexify(A, rng) if rng.rand() < 0.7 else A
+    B = _complexify(B, rng) if rng.rand() < 0.7 else B
+    rc = A.rank + B.rank - 2 * len(ia)
+    perm = list(rng.permutation(rc)) if rc and rng.rand() < 0.5 else None
+    cA, cB = int(rng.randint(0, 2)), int(rng.randint(0, 2))
+    print(f"seed {seed}: A {A.rank}/{A.nblk} ia {ia} conj {cA}, B {B.rank}/{B.nblk} ib {ib} conj {cB}, "
+          f"perm {perm}", flush=True)
+    C = Q.api._contract_pair(A, ia, cA, B, ib, cB, perm)
+    R = O.contract(A, ia, B, ib, perm=perm, conjA=cA, conjB=cB)
+    assert_same_structure(C, R)
+    if R.nblk:
+        assert rel_err(C, R) < 1e-12
 
 
 @pytest.mark.parametrize("seed", range(12))
