@@ -102,6 +102,12 @@ struct HpsiPlan {
   ReduceItem *dsreds1 = nullptr, *dsreds2 = nullptr;  // k-chunked tiles of the split work lists
   double* dsscratch = nullptr;
   double split_exec_flops[2] = {0, 0};
+  // eigensolver state kept between calls: sqrt(welem) on the device and one
+  // workspace (Krylov vectors, partials, scalars), grown on demand
+  double* dsw = nullptr;
+  double* dws = nullptr;
+  size_t ws_doubles = 0;
+  double* hstage = nullptr;  // pinned host staging of psi0 and the ground state (N doubles)
   ~HpsiPlan() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
     for (SplitSide* sp : {&spHL, &spPsi, &spT, &spHR}) {
@@ -127,6 +133,9 @@ struct HpsiPlan {
     dev_free(dq1);
     dev_free(dq2);
     dev_free(dscratch);
+    dev_free(dsw);
+    dev_free(dws);
+    if (hstage) (void)hipHostFree(hstage);
     for (auto e : ev)
       if (e) (void)hipEventDestroy(e);
   }
@@ -662,6 +671,15 @@ static void hpsi_create(HpsiPlan& H, const std::vector<const qsx_tensor*>& Ls, c
   if (fermA && Ls.size() != 1) fail(QSX_ERR_ARG, "H*psi: fermion signs need a single left environment");
   H.N = H.psi.total();
   H.pol1.slots = H.pol2.slots = resident_slots();
+  // default decomposition of an untuned plan: the k-range of a wave item
+  // split beyond 256 MFMAs in both stages; stage 2 (T . R into psi's ragged
+  // blocks) in single 16x16 wave tiles.  Measured on the Heisenberg (D* =
+  // 2000, 4000, 8000) and Hubbard (D* = 4000) operators, this is within the
+  // autotuner's timing noise of its choice, where the generic contraction
+  // default (2x2-fragment tiles, no split) left stage 2 up to 1.5x slower.
+  H.pol1.split_mfma = 256;
+  H.pol2.split_mfma = 256;
+  H.pol2.wave_max = 1;
   qsx_opts so{};
   so.mode = QSX_MODE_BATCHED;
   so.verbose = 0;
@@ -966,79 +984,144 @@ static void hpsi_apply(HpsiPlan& H, const double* dpsi, double* dhpsi, hipStream
 // ---------------------------------------------------------------- eigensolvers
 // symmetric eigen-decomposition by cyclic Jacobi (small dense matrices)
 
+// eigensolver set-up shared by Lanczos and Davidson: sqrt(welem) on the
+// device and a pinned host staging buffer (made once per plan), and a
+// workspace of at least `doubles` elements (grown on demand, kept until the
+// plan is destroyed)
+static double* eig_workspace(HpsiPlan& H, size_t doubles, hipStream_t s) {
+  if (!H.dsw) {
+    std::vector<double> sw(H.N);
+    for (int64_t i = 0; i < H.N; ++i) sw[i] = std::sqrt(H.welem[i]);
+    H.dsw = upload_vec(sw, s);
+  }
+  if (!H.hstage) {
+    hipError_t e = hipHostMalloc((void**)&H.hstage, sizeof(double) * std::max<int64_t>(1, H.N), hipHostMallocDefault);
+    if (e != hipSuccess) {
+      H.hstage = nullptr;
+      fail(QSX_ERR_NOMEM, "hipHostMalloc(%lld doubles) failed: %s", (long long)H.N, hipGetErrorString(e));
+    }
+  }
+  if (H.ws_doubles < doubles) {
+    QSX_HIP(hipStreamSynchronize(s));
+    dev_free(H.dws);
+    H.dws = nullptr;
+    H.ws_doubles = 0;
+    H.dws = (double*)dev_alloc(sizeof(double) * doubles);
+    H.ws_doubles = doubles;
+  }
+  return H.dws;
+}
+
+// host <-> device copies of N-element vectors through the pinned staging
+// buffer (a DMA each way instead of the runtime's pageable path); the
+// stream orders the staging buffer's uses
+static void eig_upload(HpsiPlan& H, double* d, const double* h, hipStream_t s) {
+  std::memcpy(H.hstage, h, sizeof(double) * H.N);
+  QSX_HIP(hipMemcpyAsync(d, H.hstage, sizeof(double) * H.N, hipMemcpyHostToDevice, s));
+}
+static void eig_download(HpsiPlan& H, double* h, const double* d, hipStream_t s) {
+  QSX_HIP(hipMemcpyAsync(H.hstage, d, sizeof(double) * H.N, hipMemcpyDeviceToHost, s));
+  QSX_HIP(hipStreamSynchronize(s));
+  std::memcpy(h, H.hstage, sizeof(double) * H.N);
+}
+
+// carve consecutive regions out of the workspace, 256-byte aligned
+struct Carve {
+  double* base;
+  size_t used = 0;
+  double* take(size_t n) {
+    double* p = base + used;
+    used += (n + 31) / 32 * 32;
+    return p;
+  }
+  static size_t size(std::initializer_list<size_t> parts) {
+    size_t t = 0;
+    for (size_t n : parts) t += (n + 31) / 32 * 32;
+    return t;
+  }
+};
+
+// H*psi in the symmetric basis inside the eigensolvers: the stage kernels
+// launched directly (the input and output move through the Krylov vectors,
+// so a graph would be re-instantiated on every step); plain-mode plans run
+// their batched work lists here, the per-pair path having no symmetric-basis
+// variant
+static void eig_hpsi(HpsiPlan& H, const double* x, double* y, hipStream_t s) {
+  hpsi_launch(H, x, y, s, false, true);
+}
+
 static void lanczos(HpsiPlan& H, const double* psi0, int niter, double* e0, double* ritz, double* gs) {
   hipStream_t s = default_stream();
   const int64_t N = H.N;
   if (niter < 1 || niter > 500) fail(QSX_ERR_ARG, "niter must be in 1..500");
-  double* V = (double*)dev_alloc(sizeof(double) * N * (niter + 1));
-  double* w = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, N));
-  double* part = (double*)dev_alloc(sizeof(double) * DOT_BLOCKS * 2 * (niter + 2));
-  double* alpha = (double*)dev_alloc(sizeof(double) * (niter + 1));
-  double* beta = (double*)dev_alloc(sizeof(double) * (niter + 1));
-  double* nrm = (double*)dev_alloc(sizeof(double) * 2);
-  try {
-    std::vector<double> x0(N);
-    for (int64_t i = 0; i < N; ++i) x0[i] = psi0[i] * std::sqrt(H.welem[i]);
-    QSX_HIP(hipMemcpyAsync(w, x0.data(), sizeof(double) * N, hipMemcpyHostToDevice, s));
-    vec_mdot(w, 0, 1, w, N, part, nrm, 0, s);
-    vec_normalize(w, V, nrm, nullptr, N, s);
-    QSX_HIP(hipStreamSynchronize(s));
-    // the iteration sequence is launched directly: the host enqueues all
-    // steps while the device runs them (capturing it as one graph measured
-    // slower: the capture and instantiation of ~6 nodes per step cost more
-    // than the direct launches, which overlap the device work)
-    for (int j = 0; j < niter; ++j) {
-      double* vj = V + (int64_t)j * N;
-      hpsi_launch(H, vj, w, s, false, true);
-      // full re-orthogonalisation, classical Gram-Schmidt twice; the first
-      // pass's coefficient of v_j is alpha_j = <v_j, H v_j>
-      double* pa = part;
-      double* pb = part + (int64_t)DOT_BLOCKS * (niter + 1);
-      vec_mdot_partial(V, N, j + 1, w, N, pa, s);
-      vec_gs_update_mdot(w, V, N, pa, j + 1, N, alpha + j, j, V, j + 1, pb, s);
-      vec_gs_update_mdot(w, V, N, pb, j + 1, N, nullptr, 0, nullptr, 1, pa, s);
-      vec_normalize_partial(w, V + (int64_t)(j + 1) * N, pa, beta + j, N, s);
-    }
-    QSX_HIP(hipStreamSynchronize(s));
-    std::vector<double> b(niter + 1), al(niter);
-    QSX_HIP(hipMemcpy(b.data(), beta, sizeof(double) * niter, hipMemcpyDeviceToHost));
-    QSX_HIP(hipMemcpy(al.data(), alpha, sizeof(double) * niter, hipMemcpyDeviceToHost));
-    // breakdown: a beta_j at rounding level means the Krylov space of psi0
-    // is invariant after j+1 vectors (psi0 inside an invariant subspace, or
-    // niter beyond its dimension); the later vectors are then zero or
-    // normalised rounding noise, not Lanczos vectors, so T is cut to the
-    // first m = j+1 rows and columns
-    int m = niter;
-    double scale = 0.0;
-    for (int j = 0; j + 1 < niter; ++j) {
-      scale = std::max(scale, std::fabs(al[j]));
-      if (j > 0) scale = std::max(scale, b[j - 1]);
-      if (!(b[j] > 64.0 * 2.220446049250313e-16 * scale)) {
-        m = j + 1;
-        break;
-      }
-    }
-    // Ritz values of every T_k (T_m beyond a breakdown), lowest eigenpair of T_m
-    if (ritz)
-      for (int k = 1; k < niter; ++k) ritz[k - 1] = tri_min_eigenvalue(std::min(k, m), al.data(), b.data());
-    double lam = 0.0;
-    std::vector<double> y;
-    tri_lowest(m, al.data(), b.data(), lam, y);
-    if (ritz) ritz[niter - 1] = lam;
-    if (e0) *e0 = lam;
-    if (gs) {
-      double* dy = upload_vec(y, s);
-      vec_combine(w, V, N, dy, m, N, s);
-      QSX_HIP(hipMemcpyAsync(gs, w, sizeof(double) * N, hipMemcpyDeviceToHost, s));
-      QSX_HIP(hipStreamSynchronize(s));
-      dev_free(dy);
-      for (int64_t i = 0; i < N; ++i) gs[i] /= std::sqrt(H.welem[i]);
-    }
-  } catch (...) {
-    for (void* p : {(void*)V, (void*)w, (void*)part, (void*)alpha, (void*)beta, (void*)nrm}) dev_free(p);
-    throw;
+  PhaseClock clk{H.verbose, "Lanczos"};
+  const size_t nparts = (size_t)DOT_BLOCKS * (niter + 1);
+  Carve ws{eig_workspace(H, Carve::size({(size_t)N * (niter + 1), (size_t)std::max<int64_t>(1, N), nparts, nparts,
+                                         2 * (size_t)niter, (size_t)niter}),
+                         s)};
+  double* V = ws.take((size_t)N * (niter + 1));
+  double* w = ws.take(std::max<int64_t>(1, N));
+  double* pa = ws.take(nparts);
+  double* pb = ws.take(nparts);
+  double* ab = ws.take(2 * (size_t)niter);  // [alpha | beta]
+  double* dy = ws.take(niter);
+  double* alpha = ab;
+  double* beta = ab + niter;
+  // v_0 = sqrt(w) psi0 / |sqrt(w) psi0| (psi0 lands in the unused slot v_1)
+  eig_upload(H, V + N, psi0, s);
+  vec_scale_norm_partial(V + N, H.dsw, w, N, pa, s);
+  vec_normalize_partial(w, V, pa, nullptr, N, s);
+  clk.lap("upload psi0");
+  // the iteration sequence is launched directly: the host enqueues all
+  // steps while the device runs the earlier ones (5 launches per step)
+  for (int j = 0; j < niter; ++j) {
+    double* vj = V + (int64_t)j * N;
+    eig_hpsi(H, vj, w, s);
+    // full re-orthogonalisation, classical Gram-Schmidt twice; the first
+    // pass's coefficient of v_j is alpha_j = <v_j, H v_j>; the second pass
+    // is fused with the normalisation (beta_j from |w'|^2 - |c'|^2)
+    vec_mdot_partial(V, N, j + 1, w, N, pa, s);
+    vec_gs_update_mdot(w, V, N, pa, j + 1, N, alpha + j, j, V, j + 1, 1, pb, s);
+    vec_gs_update_normalize(w, V, N, pb, j + 1, N, V + (int64_t)(j + 1) * N, beta + j, s);
   }
-  for (void* p : {(void*)V, (void*)w, (void*)part, (void*)alpha, (void*)beta, (void*)nrm}) dev_free(p);
+  clk.lap("enqueue the steps");
+  std::vector<double> abh(2 * (size_t)niter);
+  QSX_HIP(hipMemcpyAsync(abh.data(), ab, sizeof(double) * 2 * niter, hipMemcpyDeviceToHost, s));
+  QSX_HIP(hipStreamSynchronize(s));
+  clk.lap("device drain");
+  std::vector<double> al(abh.begin(), abh.begin() + niter), b(abh.begin() + niter, abh.end());
+  b.push_back(0.0);
+  // breakdown: a beta_j at rounding level means the Krylov space of psi0
+  // is invariant after j+1 vectors (psi0 inside an invariant subspace, or
+  // niter beyond its dimension); the later vectors are then zero or
+  // normalised rounding noise, not Lanczos vectors, so T is cut to the
+  // first m = j+1 rows and columns
+  int m = niter;
+  double scale = 0.0;
+  for (int j = 0; j + 1 < niter; ++j) {
+    scale = std::max(scale, std::fabs(al[j]));
+    if (j > 0) scale = std::max(scale, b[j - 1]);
+    if (!(b[j] > 64.0 * 2.220446049250313e-16 * scale)) {
+      m = j + 1;
+      break;
+    }
+  }
+  // Ritz values of every T_k (T_m beyond a breakdown), lowest eigenpair of T_m
+  if (ritz)
+    for (int k = 1; k < niter; ++k) ritz[k - 1] = tri_min_eigenvalue(std::min(k, m), al.data(), b.data());
+  double lam = 0.0;
+  std::vector<double> y;
+  tri_lowest(m, al.data(), b.data(), lam, y);
+  if (ritz) ritz[niter - 1] = lam;
+  if (e0) *e0 = lam;
+  clk.lap("tridiagonal eigenproblems");
+  if (gs) {
+    // the Ritz vector back in the caller's basis: (V y) / sqrt(w)
+    QSX_HIP(hipMemcpyAsync(dy, y.data(), sizeof(double) * m, hipMemcpyHostToDevice, s));
+    vec_combine(w, V, N, dy, m, N, s, H.dsw);
+    eig_download(H, gs, w, s);
+  }
+  clk.lap("Ritz vector");
 }
 
 static void davidson(HpsiPlan& H, const double* psi0, int niter, double* e0, double* ritz, double* gs) {
@@ -1046,88 +1129,75 @@ static void davidson(HpsiPlan& H, const double* psi0, int niter, double* e0, dou
   const int64_t N = H.N;
   if (niter < 1 || niter > 500) fail(QSX_ERR_ARG, "niter must be in 1..500");
   const int mmax = niter + 1;
-  double* V = (double*)dev_alloc(sizeof(double) * N * mmax);
-  double* HV = (double*)dev_alloc(sizeof(double) * N * mmax);
-  double* u = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, N));
-  double* hu = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, N));
-  double* part = (double*)dev_alloc(sizeof(double) * DOT_BLOCKS * 2 * (mmax + 1));
-  // ret = [new column of the projected matrix (mmax) | |r|^2 partials]: one
+  const size_t nparts = (size_t)DOT_BLOCKS * (mmax + 1);
+  const size_t n1 = std::max<int64_t>(1, N);
+  Carve ws{eig_workspace(
+      H, Carve::size({(size_t)N * mmax, (size_t)N * mmax, n1, n1, nparts, nparts, (size_t)mmax + 1, (size_t)mmax + 1}),
+      s)};
+  double* V = ws.take((size_t)N * mmax);
+  double* HV = ws.take((size_t)N * mmax);
+  double* u = ws.take(n1);
+  double* hu = ws.take(n1);
+  double* pa = ws.take(nparts);
+  double* pb = ws.take(nparts);
+  // ret = [new column of the projected matrix (mmax) | |r|] : one
   // device-to-host copy per step
-  double* ret = (double*)dev_alloc(sizeof(double) * (mmax + DOT_BLOCKS));
-  double* nrm = (double*)dev_alloc(sizeof(double) * 2);
-  double* dy = (double*)dev_alloc(sizeof(double) * (mmax + 1));
-  try {
-    std::vector<double> x0(N);
-    for (int64_t i = 0; i < N; ++i) x0[i] = psi0[i] * std::sqrt(H.welem[i]);
-    QSX_HIP(hipMemcpyAsync(u, x0.data(), sizeof(double) * N, hipMemcpyHostToDevice, s));
-    vec_mdot(u, 0, 1, u, N, part, nrm, 0, s);
-    vec_normalize(u, V, nrm, nullptr, N, s);
-    hpsi_apply(H, V, HV, s, true);
-    std::vector<double> Hk(1);
-    vec_mdot(V, 0, 1, HV, N, part, ret, 0, s);
-    QSX_HIP(hipMemcpyAsync(&Hk[0], ret, sizeof(double), hipMemcpyDeviceToHost, s));
+  double* ret = ws.take(mmax + 1);
+  double* dy = ws.take(mmax + 1);
+  eig_upload(H, hu, psi0, s);
+  vec_scale_norm_partial(hu, H.dsw, u, N, pa, s);
+  vec_normalize_partial(u, V, pa, nullptr, N, s);
+  eig_hpsi(H, V, HV, s);
+  std::vector<double> Hk(1);
+  vec_mdot(V, 0, 1, HV, N, pa, ret, 0, s);
+  QSX_HIP(hipMemcpyAsync(&Hk[0], ret, sizeof(double), hipMemcpyDeviceToHost, s));
+  QSX_HIP(hipStreamSynchronize(s));
+  int k = 1;
+  std::vector<double> back(mmax + 1);
+  for (int it = 0; it < niter; ++it) {
+    double th = 0.0;
+    std::vector<double> y;
+    sym_lowest(k, Hk, th, y);
+    if (ritz) ritz[it] = th;
+    if (e0) *e0 = th;
+    // u = V y and the residual r = H u - theta u in one pass (y and theta
+    // in one host-to-device copy)
+    y.push_back(th);
+    QSX_HIP(hipMemcpyAsync(dy, y.data(), sizeof(double) * (k + 1), hipMemcpyHostToDevice, s));
+    if (it + 1 == niter) break;
+    vec_combine_res(u, hu, V, HV, N, dy, k, N, s);
+    // r orthogonalised twice against V and normalised: classical
+    // Gram-Schmidt, the reductions' final sums folded into their consumers,
+    // the second pass fused with the normalisation (|r| to ret[mmax])
+    vec_mdot_partial(V, N, k, hu, N, pa, s);
+    vec_gs_update_mdot(hu, V, N, pa, k, N, nullptr, 0, V, k, 1, pb, s);
+    vec_gs_update_normalize(hu, V, N, pb, k, N, V + (int64_t)k * N, ret + mmax, s);
+    eig_hpsi(H, V + (int64_t)k * N, HV + (int64_t)k * N, s);
+    // grow the projected matrix by one row/column; one host round trip
+    // brings the new column and the residual norm
+    vec_mdot(V, N, k + 1, HV + (int64_t)k * N, N, pa, ret, 0, s);
+    QSX_HIP(hipMemcpyAsync(back.data(), ret, sizeof(double) * (mmax + 1), hipMemcpyDeviceToHost, s));
     QSX_HIP(hipStreamSynchronize(s));
-    int k = 1;
-    std::vector<double> back(mmax + DOT_BLOCKS);
-    for (int it = 0; it < niter; ++it) {
-      double th = 0.0;
-      std::vector<double> y;
-      sym_lowest(k, Hk, th, y);
-      if (ritz) ritz[it] = th;
-      if (e0) *e0 = th;
-      // u = V y and the residual r = H u - theta u in one pass (y and theta
-      // in one host-to-device copy)
-      y.push_back(th);
-      QSX_HIP(hipMemcpyAsync(dy, y.data(), sizeof(double) * (k + 1), hipMemcpyHostToDevice, s));
-      if (it + 1 == niter) {
-        vec_combine(u, V, N, dy, k, N, s);
-        break;
-      }
-      vec_combine_res(u, hu, V, HV, N, dy, k, N, s);
-      // r orthogonalised twice against V, then normalised: classical
-      // Gram-Schmidt with each reduction's final sum folded into its
-      // consumer (the second update also forms the |r|^2 partials)
-      double* pa = part;
-      double* pb = part + (int64_t)DOT_BLOCKS * (mmax + 1);
-      double* rpart = ret + mmax;
-      vec_mdot_partial(V, N, k, hu, N, pa, s);
-      vec_gs_update_mdot(hu, V, N, pa, k, N, nullptr, 0, V, k, pb, s);
-      vec_gs_update_mdot(hu, V, N, pb, k, N, nullptr, 0, nullptr, 1, rpart, s);
-      vec_normalize_partial(hu, V + (int64_t)k * N, rpart, nullptr, N, s);
-      hpsi_apply(H, V + (int64_t)k * N, HV + (int64_t)k * N, s, true);
-      // grow the projected matrix by one row/column; one host round trip
-      // brings the new column and the residual norm
-      vec_mdot(V, N, k + 1, HV + (int64_t)k * N, N, part, ret, 0, s);
-      QSX_HIP(hipMemcpyAsync(back.data(), ret, sizeof(double) * (mmax + DOT_BLOCKS), hipMemcpyDeviceToHost, s));
-      QSX_HIP(hipStreamSynchronize(s));
-      double r2 = 0;
-      for (int b = 0; b < DOT_BLOCKS; ++b) r2 += back[mmax + b];
-      if (r2 < 1e-28) {  // converged: the new direction is noise, keep u
-        for (int j = it + 1; j < niter && ritz; ++j) ritz[j] = th;
-        break;
-      }
-      std::vector<double> Hn((size_t)(k + 1) * (k + 1));
-      for (int i = 0; i < k; ++i)
-        for (int j = 0; j < k; ++j) Hn[(size_t)i * (k + 1) + j] = Hk[(size_t)i * k + j];
-      for (int i = 0; i <= k; ++i) {
-        Hn[(size_t)i * (k + 1) + k] = back[i];
-        Hn[(size_t)k * (k + 1) + i] = back[i];
-      }
-      Hk.swap(Hn);
-      ++k;
+    const double rn = back[mmax];
+    if (rn * rn < 1e-28) {  // converged: the new direction is noise, keep u = V y
+      for (int j = it + 1; j < niter && ritz; ++j) ritz[j] = th;
+      break;
     }
-    if (gs) {
-      QSX_HIP(hipMemcpyAsync(gs, u, sizeof(double) * N, hipMemcpyDeviceToHost, s));
-      QSX_HIP(hipStreamSynchronize(s));
-      for (int64_t i = 0; i < N; ++i) gs[i] /= std::sqrt(H.welem[i]);
+    std::vector<double> Hn((size_t)(k + 1) * (k + 1));
+    for (int i = 0; i < k; ++i)
+      for (int j = 0; j < k; ++j) Hn[(size_t)i * (k + 1) + j] = Hk[(size_t)i * k + j];
+    for (int i = 0; i <= k; ++i) {
+      Hn[(size_t)i * (k + 1) + k] = back[i];
+      Hn[(size_t)k * (k + 1) + i] = back[i];
     }
-  } catch (...) {
-    for (void* p : {(void*)V, (void*)HV, (void*)u, (void*)hu, (void*)part, (void*)ret, (void*)nrm, (void*)dy})
-      dev_free(p);
-    throw;
+    Hk.swap(Hn);
+    ++k;
   }
-  for (void* p : {(void*)V, (void*)HV, (void*)u, (void*)hu, (void*)part, (void*)ret, (void*)nrm, (void*)dy})
-    dev_free(p);
+  if (gs) {
+    // the Ritz vector V y (dy holds the last y) back in the caller's basis
+    vec_combine(u, V, N, dy, k, N, s, H.dsw);
+    eig_download(H, gs, u, s);
+  }
 }
 
 }  // namespace qsx

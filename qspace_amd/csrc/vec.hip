@@ -29,24 +29,44 @@ __device__ __forceinline__ double block_sum(double v) {
   return s;  // valid in thread 0
 }
 
-// the DOT_BLOCKS partials of one reduction summed by one wave (two per lane,
-// then a fixed shuffle tree): every block that needs the value sums it in
-// the same order, so all of them hold the same bits; the result is in every
-// lane
-static_assert(DOT_BLOCKS == 128, "wave_sum_partials reads two partials per lane");
-__device__ __forceinline__ double wave_sum_partials(const double* __restrict__ p) {
-  const int lane = threadIdx.x & 63;
-  double s = p[lane] + p[lane + 64];
+// the DOT_BLOCKS partials of one reduction summed by one wave (four per
+// lane in a fixed pairing, then a fixed shuffle tree): every block that
+// needs the value sums it in the same order, so all of them hold the same
+// bits
+static_assert(DOT_BLOCKS == 256, "the partial sums read four partials per lane");
+__device__ __forceinline__ double lane_partials(const double* __restrict__ p, int lane) {
+  return (p[lane] + p[lane + 64]) + (p[lane + 128] + p[lane + 192]);
+}
+__device__ __forceinline__ double wave_reduce(double s) {
   for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+  return s;  // valid in lane 0
+}
+// the result is in every lane
+__device__ __forceinline__ double wave_sum_partials(const double* __restrict__ p) {
+  const double s = wave_reduce(lane_partials(p, threadIdx.x & 63));
   return __shfl(s, 0, 64);
 }
 
-// c[j] = sum_b partial[j][b] for j < m, one wave per j
+// c[j] = sum_b partial[j][b] for j < m, one wave per j; a wave takes up to
+// kSumGroup coefficients per round with all their loads issued together
+// (one L2 round trip for up to RT/64 * kSumGroup coefficients)
+constexpr int kSumGroup = 8;
 __device__ __forceinline__ void sum_coefficients(const double* __restrict__ partial, int m, double* c) {
+  constexpr int NW = RT / 64;
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int j = wv; j < m; j += RT / 64) {
-    const double s = wave_sum_partials(partial + (int64_t)j * DOT_BLOCKS);
-    if (lane == 0) c[j] = s;
+  for (int j0 = wv; j0 < m; j0 += NW * kSumGroup) {
+    double v[kSumGroup];
+#pragma unroll
+    for (int q = 0; q < kSumGroup; ++q) {
+      const int j = j0 + q * NW;
+      v[q] = j < m ? lane_partials(partial + (int64_t)j * DOT_BLOCKS, lane) : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < kSumGroup; ++q) {
+      const double s = wave_reduce(v[q]);
+      const int j = j0 + q * NW;
+      if (lane == 0 && j < m) c[j] = s;
+    }
   }
   __syncthreads();
 }
@@ -73,13 +93,30 @@ __global__ __launch_bounds__(RT) void k_mdot_final(const double* __restrict__ pa
   if (threadIdx.x == 0) out[j] = accumulate ? out[j] + s : s;
 }
 
-// y = sum_j c[j] V_j
+// dst = src .* w, partial[b] = sum over the b-th slice of dst^2 (the slice
+// layout of k_mdot_partial)
+__global__ __launch_bounds__(RT) void k_scale_norm_partial(const double* __restrict__ src,
+                                                            const double* __restrict__ w, double* __restrict__ dst,
+                                                            int64_t n, double* __restrict__ partial) {
+  const int b = blockIdx.x;
+  double s = 0;
+  for (int64_t i = (int64_t)b * RT + threadIdx.x; i < n; i += (int64_t)DOT_BLOCKS * RT) {
+    const double x = src[i] * w[i];
+    dst[i] = x;
+    s += x * x;
+  }
+  s = block_sum(s);
+  if (threadIdx.x == 0) partial[b] = s;
+}
+
+// y = sum_j c[j] V_j (divided elementwise by div if given)
 __global__ __launch_bounds__(RT) void k_combine(double* __restrict__ y, const double* __restrict__ V, int64_t ldv,
-                                                 const double* __restrict__ c, int m, int64_t n) {
+                                                 const double* __restrict__ c, int m, int64_t n,
+                                                 const double* __restrict__ div) {
   for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < n; i += (int64_t)gridDim.x * RT) {
     double s = 0;
     for (int j = 0; j < m; ++j) s += c[j] * V[(int64_t)j * ldv + i];
-    y[i] = s;
+    y[i] = div ? s / div[i] : s;
   }
 }
 
@@ -100,20 +137,10 @@ __global__ __launch_bounds__(RT) void k_combine_res(double* __restrict__ u, doub
   }
 }
 
-// beta = sqrt(*nrm2); dst = src / beta
-__global__ __launch_bounds__(RT) void k_normalize(const double* __restrict__ src, double* __restrict__ dst,
-                                                   const double* __restrict__ nrm2, double* __restrict__ beta,
-                                                   int64_t n) {
-  const double b = sqrt(*nrm2);
-  const double inv = b > 0 ? 1.0 / b : 0.0;
-  if (blockIdx.x == 0 && threadIdx.x == 0 && beta) *beta = b;
-  for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < n; i += (int64_t)gridDim.x * RT) dst[i] = src[i] * inv;
-}
-
 // One Gram-Schmidt step fused with the next reduction: w -= sum_j c[j] V_j
 // (c from the partials pin[j][b], every block summing them in the same
-// order), then pout[j][b] = <V_j, w> over block b's slice (jn vectors from
-// V, or <w, w> if V2 is null).  The grid is exactly DOT_BLOCKS blocks with
+// order), then pout[j][b] = <V2_j, w> over block b's slice for j < jn,
+// followed by <w, w> at j = jn if wnorm is set.  The grid is exactly DOT_BLOCKS blocks with
 // the slice layout of k_mdot_partial; each partial is its slice's sum in a
 // fixed order (elements in index order per thread, a fixed shuffle tree,
 // the waves in order): deterministic.  The dots of kDotGroup vectors are
@@ -124,7 +151,7 @@ constexpr int kDotGroup = 8;
 __global__ __launch_bounds__(RT) void k_gs_update_mdot(double* __restrict__ w, const double* __restrict__ V,
                                                         int64_t ldv, const double* __restrict__ pin, int m,
                                                         int64_t n, double* __restrict__ cout, int cj,
-                                                        const double* __restrict__ V2, int jn,
+                                                        const double* __restrict__ V2, int jn, int wnorm,
                                                         double* __restrict__ pout) {
   __shared__ double c[kMaxM];
   __shared__ double red[RT / 64][kDotGroup];
@@ -133,12 +160,13 @@ __global__ __launch_bounds__(RT) void k_gs_update_mdot(double* __restrict__ w, c
   const int b = blockIdx.x;
   for (int64_t i = (int64_t)b * RT + threadIdx.x; i < n; i += (int64_t)DOT_BLOCKS * RT) {
     double s = 0;
+#pragma unroll 8
     for (int j = 0; j < m; ++j) s += c[j] * V[(int64_t)j * ldv + i];
     w[i] -= s;
   }
   // the slice's updated w is this thread's own elements: no barrier needed
   // before the dots
-  const int nv = V2 ? jn : 1;
+  const int nv = jn + (wnorm ? 1 : 0);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   for (int j0 = 0; j0 < nv; j0 += kDotGroup) {
     const int ng = min(kDotGroup, nv - j0);
@@ -149,7 +177,7 @@ __global__ __launch_bounds__(RT) void k_gs_update_mdot(double* __restrict__ w, c
       const double wi = w[i];
 #pragma unroll
       for (int q = 0; q < kDotGroup; ++q)
-        if (q < ng) acc[q] += (V2 ? V2[(int64_t)(j0 + q) * ldv + i] : wi) * wi;
+        if (q < ng) acc[q] += (j0 + q < jn ? V2[(int64_t)(j0 + q) * ldv + i] : wi) * wi;
     }
 #pragma unroll
     for (int q = 0; q < kDotGroup; ++q) {
@@ -164,6 +192,38 @@ __global__ __launch_bounds__(RT) void k_gs_update_mdot(double* __restrict__ w, c
       pout[(int64_t)(j0 + threadIdx.x) * DOT_BLOCKS + b] = t;
     }
     __syncthreads();
+  }
+}
+
+// The second Gram-Schmidt step fused with the normalisation: c[j] (j < m)
+// and |w|^2 (at j = m) from the partials pin, then dst = (w - sum_j c[j] V_j)
+// / beta with beta^2 = |w|^2 - sum_j c[j]^2, the squared norm of the update
+// for orthonormal V_j.  After a first pass the c[j] are at rounding level
+// relative to |w| and the difference is exact to rounding; when they are
+// not (|c|^2 > |w|^2 / 2: w is rounding noise inside span V, a breakdown)
+// beta = 0 and dst = 0.  Every block sums the partials and the squares in
+// the same order.
+__global__ __launch_bounds__(RT) void k_gs_update_normalize(const double* __restrict__ w,
+                                                             const double* __restrict__ V, int64_t ldv,
+                                                             const double* __restrict__ pin, int m, int64_t n,
+                                                             double* __restrict__ dst, double* __restrict__ beta) {
+  __shared__ double c[kMaxM + 1];
+  __shared__ double inv;
+  sum_coefficients(pin, m + 1, c);
+  if (threadIdx.x == 0) {
+    double cs = 0;
+    for (int j = 0; j < m; ++j) cs += c[j] * c[j];
+    const double b = cs <= 0.5 * c[m] ? sqrt(c[m] - cs) : 0.0;
+    inv = b > 0 ? 1.0 / b : 0.0;
+    if (blockIdx.x == 0 && beta) *beta = b;
+  }
+  __syncthreads();
+  const double f = inv;
+  for (int64_t i = (int64_t)blockIdx.x * RT + threadIdx.x; i < n; i += (int64_t)gridDim.x * RT) {
+    double s = 0;
+#pragma unroll 8
+    for (int j = 0; j < m; ++j) s += c[j] * V[(int64_t)j * ldv + i];
+    dst[i] = (w[i] - s) * f;
   }
 }
 
@@ -204,11 +264,24 @@ void vec_mdot_partial(const double* V, int64_t ldv, int m, const double* w, int6
 }
 
 void vec_gs_update_mdot(double* w, const double* V, int64_t ldv, const double* pin, int m, int64_t n, double* cout,
-                        int cj, const double* V2, int jn, double* pout, hipStream_t s) {
+                        int cj, const double* V2, int jn, int wnorm, double* pout, hipStream_t s) {
   if (m <= 0) return;
   if (m > kMaxM) fail_vec(m);
+  if (jn > 0 && !V2) fail(QSX_ERR_ARG, "vec_gs_update_mdot: %d dots without vectors", jn);
   hipLaunchKernelGGL(k_gs_update_mdot, dim3(DOT_BLOCKS), dim3(RT), 0, s, w, V, ldv, pin, m, n, cout, cj, V2, jn,
-                     pout);
+                     wnorm, pout);
+}
+
+void vec_gs_update_normalize(const double* w, const double* V, int64_t ldv, const double* pin, int m, int64_t n,
+                             double* dst, double* beta, hipStream_t s) {
+  if (m <= 0) return;
+  if (m > kMaxM) fail_vec(m);
+  hipLaunchKernelGGL(k_gs_update_normalize, dim3(grid_for(n)), dim3(RT), 0, s, w, V, ldv, pin, m, n, dst, beta);
+}
+
+void vec_scale_norm_partial(const double* src, const double* w, double* dst, int64_t n, double* partial,
+                            hipStream_t s) {
+  hipLaunchKernelGGL(k_scale_norm_partial, dim3(DOT_BLOCKS), dim3(RT), 0, s, src, w, dst, n, partial);
 }
 
 void vec_normalize_partial(const double* src, double* dst, const double* partial, double* beta, int64_t n,
@@ -216,17 +289,14 @@ void vec_normalize_partial(const double* src, double* dst, const double* partial
   hipLaunchKernelGGL(k_normalize_p, dim3(grid_for(n)), dim3(RT), 0, s, src, dst, partial, beta, n);
 }
 
-void vec_combine(double* y, const double* V, int64_t ldv, const double* c, int m, int64_t n, hipStream_t s) {
-  hipLaunchKernelGGL(k_combine, dim3(grid_for(n)), dim3(RT), 0, s, y, V, ldv, c, m, n);
+void vec_combine(double* y, const double* V, int64_t ldv, const double* c, int m, int64_t n, hipStream_t s,
+                 const double* div) {
+  hipLaunchKernelGGL(k_combine, dim3(grid_for(n)), dim3(RT), 0, s, y, V, ldv, c, m, n, div);
 }
 
 void vec_combine_res(double* u, double* r, const double* V, const double* HV, int64_t ldv, const double* c, int m,
                      int64_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_combine_res, dim3(grid_for(n)), dim3(RT), 0, s, u, r, V, HV, ldv, c, m, n);
-}
-
-void vec_normalize(const double* src, double* dst, const double* nrm2, double* beta, int64_t n, hipStream_t s) {
-  hipLaunchKernelGGL(k_normalize, dim3(grid_for(n)), dim3(RT), 0, s, src, dst, nrm2, beta, n);
 }
 
 }  // namespace qsx
