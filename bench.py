@@ -24,6 +24,7 @@ on the SU(2) environments, the U(1) and SU(2) environment updates in the
 cell form of updateHK.m and the orthoQS truncations.
 
     python bench.py --split-profile bf16|f32    # only the split H*psi (profiler runs)
+    python bench.py --eig-profile [--workload W] # only the eigensolver of an untuned plan (profiler runs)
     python bench.py --save-policy FILE          # also write the autotuned stage policies
 """
 from __future__ import annotations
@@ -275,8 +276,7 @@ def _rel(C, R):
 
 def _time_eig(fn, x0, niter, reps=3):
     """(first call ms, steady-state ms, E0): the steady state is the best of
-    `reps` further calls (the loop is bound by the host's kernel launches, so
-    a single call is exposed to host scheduling noise)"""
+    `reps` further calls (a single call is exposed to host scheduling noise)"""
     t0 = time.perf_counter()
     fn(x0, niter)
     first = (time.perf_counter() - t0) * 1e3
@@ -588,6 +588,20 @@ def split_profile(mode, steps, warmup):
             "frac_of_peak": fl / (ms * 1e-3) / 1e12 / peak, "useful_gflop": H.flops / 1e9, "calibration": calib}
 
 
+def eig_profile(workload):
+    """only the eigensolver of an untuned plan (for profiler runs of its
+    loop): Lanczos 20 on the Heisenberg workloads, Davidson 30 on the Hubbard
+    one; first call and best of three further calls"""
+    import qspace_amd as Q
+    w = WORKLOADS[workload]
+    HL, psi, HR, ferm = make_workload(w)
+    H = Q.HPsi(HL, psi, HR, hconj=1, fermA=ferm)
+    solver, niter = (H.davidson, 30) if w["kind"] == "hubbard" else (H.lanczos, 20)
+    first, best, e0 = _time_eig(solver, psi.packed(), niter)
+    return {"eig_profile": w["name"], "solver": solver.__name__, "niter": niter, "first_call_ms": first,
+            "ms": best, "E0": e0, "psi_elements": H.N, "policy": "untuned plan (default decomposition)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -604,6 +618,8 @@ def main():
     ap.add_argument("--save-policy", default=None, help="write the autotuned policy to this JSON file")
     ap.add_argument("--split-profile", default=None, choices=["bf16", "f32"],
                     help="only the split-precision H*psi (for profiler runs): one JSON line")
+    ap.add_argument("--eig-profile", action="store_true",
+                    help="only the eigensolver of an untuned plan of --workload (for profiler runs): one JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (got --gpus {args.gpus}); "
@@ -618,6 +634,9 @@ def main():
         sys.exit(1)
     if args.split_profile:
         print(json.dumps(split_profile(args.split_profile, args.steps, args.warmup)), flush=True)
+        return
+    if args.eig_profile:
+        print(json.dumps(eig_profile(args.workload)), flush=True)
         return
     if args.block_ops:
         bops = block_ops()
