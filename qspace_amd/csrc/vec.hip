@@ -227,6 +227,56 @@ __global__ __launch_bounds__(RT) void k_gs_update_normalize(const double* __rest
   }
 }
 
+// k_gs_update_mdot for the first Gram-Schmidt pass of the eigensolvers (the
+// dots against the same m <= MC vectors as the update): one pass over V, a
+// thread's V_j[i] held in registers between the update and the dots.  Each
+// partial is its slice's sum in a fixed order (the thread's elements in
+// index order, the shuffle tree, the waves in order): deterministic.
+template <int MC>
+__global__ __launch_bounds__(RT) void k_gs_update_mdot_reg(double* __restrict__ w, const double* __restrict__ V,
+                                                            int64_t ldv, const double* __restrict__ pin, int m,
+                                                            int64_t n, double* __restrict__ cout, int cj, int wnorm,
+                                                            double* __restrict__ pout) {
+  __shared__ double c[MC];
+  __shared__ double red[RT / 64][MC + 1];
+  sum_coefficients(pin, m, c);
+  if (cout && blockIdx.x == 0 && threadIdx.x == 0) *cout = c[cj];
+  const int b = blockIdx.x;
+  double acc[MC + 1];
+#pragma unroll
+  for (int q = 0; q <= MC; ++q) acc[q] = 0.0;
+  for (int64_t i = (int64_t)b * RT + threadIdx.x; i < n; i += (int64_t)DOT_BLOCKS * RT) {
+    double v[MC];
+#pragma unroll
+    for (int j = 0; j < MC; ++j) v[j] = j < m ? V[(int64_t)j * ldv + i] : 0.0;
+    double s = 0;
+#pragma unroll
+    for (int j = 0; j < MC; ++j)
+      if (j < m) s += c[j] * v[j];
+    const double wi = w[i] - s;
+    w[i] = wi;
+#pragma unroll
+    for (int j = 0; j < MC; ++j) acc[j] += v[j] * wi;
+    acc[MC] += wi * wi;
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q <= MC; ++q) {
+    if (q < m || (q == MC && wnorm)) {
+      const double t = wave_reduce(acc[q]);
+      if (lane == 0) red[wv][q] = t;
+    }
+  }
+  __syncthreads();
+  const int nv = m + (wnorm ? 1 : 0);
+  if (threadIdx.x < nv) {
+    const int q = threadIdx.x < m ? threadIdx.x : MC;
+    double t = 0;
+    for (int k = 0; k < RT / 64; ++k) t += red[k][q];
+    pout[(int64_t)threadIdx.x * DOT_BLOCKS + b] = t;
+  }
+}
+
 // beta = sqrt(sum_b partial[b]); dst = src / beta (every block sums the
 // partials in the same order)
 __global__ __launch_bounds__(RT) void k_normalize_p(const double* __restrict__ src, double* __restrict__ dst,
@@ -268,6 +318,18 @@ void vec_gs_update_mdot(double* w, const double* V, int64_t ldv, const double* p
   if (m <= 0) return;
   if (m > kMaxM) fail_vec(m);
   if (jn > 0 && !V2) fail(QSX_ERR_ARG, "vec_gs_update_mdot: %d dots without vectors", jn);
+  if (V2 == V && jn == m && m <= 32) {  // dots against the update's own vectors: one pass over V
+    if (m <= 8)
+      hipLaunchKernelGGL(k_gs_update_mdot_reg<8>, dim3(DOT_BLOCKS), dim3(RT), 0, s, w, V, ldv, pin, m, n, cout, cj,
+                         wnorm, pout);
+    else if (m <= 16)
+      hipLaunchKernelGGL(k_gs_update_mdot_reg<16>, dim3(DOT_BLOCKS), dim3(RT), 0, s, w, V, ldv, pin, m, n, cout, cj,
+                         wnorm, pout);
+    else
+      hipLaunchKernelGGL(k_gs_update_mdot_reg<32>, dim3(DOT_BLOCKS), dim3(RT), 0, s, w, V, ldv, pin, m, n, cout, cj,
+                         wnorm, pout);
+    return;
+  }
   hipLaunchKernelGGL(k_gs_update_mdot, dim3(DOT_BLOCKS), dim3(RT), 0, s, w, V, ldv, pin, m, n, cout, cj, V2, jn,
                      wnorm, pout);
 }
