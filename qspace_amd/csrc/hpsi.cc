@@ -1095,17 +1095,19 @@ static void lanczos(HpsiPlan& H, const double* psi0, int niter, double* e0, doub
   // is invariant after j+1 vectors (psi0 inside an invariant subspace, or
   // niter beyond its dimension); the later vectors are then zero or
   // normalised rounding noise, not Lanczos vectors, so T is cut to the
-  // first m = j+1 rows and columns
+  // first m = j+1 rows and columns.  Rounding level is relative to |H|: the
+  // noise of H v is ~eps |H| even when |H v| is small (psi0 an eigenvector
+  // of a small eigenvalue), so the scale is the largest entry of the whole
+  // T (every alpha_j and beta_j is bounded by |H|, and the entries after a
+  // breakdown, from noise directions, reveal it)
   int m = niter;
   double scale = 0.0;
-  for (int j = 0; j + 1 < niter; ++j) {
-    scale = std::max(scale, std::fabs(al[j]));
-    if (j > 0) scale = std::max(scale, b[j - 1]);
+  for (int j = 0; j < niter; ++j) scale = std::max(scale, std::max(std::fabs(al[j]), std::fabs(b[j])));
+  for (int j = 0; j + 1 < niter; ++j)
     if (!(b[j] > 64.0 * 2.220446049250313e-16 * scale)) {
       m = j + 1;
       break;
     }
-  }
   // Ritz values of every T_k (T_m beyond a breakdown), lowest eigenpair of T_m
   if (ritz)
     for (int k = 1; k < niter; ++k) ritz[k - 1] = tri_min_eigenvalue(std::min(k, m), al.data(), b.data());

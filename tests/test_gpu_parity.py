@@ -286,6 +286,30 @@ def test_lanczos_invariant_start(gpu):
     assert abs(abs(g @ U[:, -1]) / np.linalg.norm(g) - 1.0) < 1e-10
 
 
+@pytest.mark.parametrize("niter", [3, 8, 25])
+def test_lanczos_invariant_start_interior(gpu, niter):
+    """Lanczos started on an exact eigenvector of an interior eigenvalue much
+    smaller than |H| (the Hubbard operator's eigenvalue nearest 0, and one
+    of magnitude ~|H|/60): the rounding noise of H v is ~eps |H|, far above
+    eps |lambda|, and the breakdown is still recognised (scale from the whole
+    tridiagonal matrix), so the result is that eigenvalue, not the ground
+    energy reached through normalised noise directions."""
+    HL, psi, HR = gen.hubbard_hpsi(seed=26, Dstar=33, mu=1.0, sigma=0.4, dmax=4, qmax=3)
+    H = Q.HPsi(HL, psi, HR, hconj=1, fermA=gen.fermion_parity_signs(HL))
+    S = _dense_weighted(H)
+    ev, U = np.linalg.eigh(0.5 * (S + S.T))
+    hn = np.abs(ev).max()
+    sw = np.sqrt(H.weights())
+    k_small = int(np.argmin(np.abs(ev)))
+    k_mid = int(np.argmin(np.abs(np.abs(ev) - hn / 60)))
+    for k in (k_small, k_mid):
+        assert abs(ev[k]) < hn / 30 and ev[k] - ev[0] > 1.0
+        e0, ritz, gs = H.lanczos(U[:, k] / sw, niter)
+        assert abs(e0 - ev[k]) < 1e-10 * hn, (k, e0, ev[k])
+        g = gs * sw
+        assert abs(abs(g @ U[:, k]) / np.linalg.norm(g) - 1.0) < 1e-10
+
+
 def test_split_precision_slices(gpu):
     """BF16-slice H*psi (exact FP32 accumulation, FP64 recombination) against
     the oracle at D* = 1000: the error falls by ~2^-7 per slice and 7
