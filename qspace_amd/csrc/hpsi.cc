@@ -1180,8 +1180,14 @@ static void davidson(HpsiPlan& H, const double* psi0, int niter, double* e0, dou
     vec_mdot(V, N, k + 1, HV + (int64_t)k * N, N, pa, ret, 0, s);
     QSX_HIP(hipMemcpyAsync(back.data(), ret, sizeof(double) * (mmax + 1), hipMemcpyDeviceToHost, s));
     QSX_HIP(hipStreamSynchronize(s));
+    // converged when the orthogonalised residual is at rounding level
+    // relative to the projected matrix (its entries are bounded by |H|):
+    // the new direction would be noise; keep u = V y
     const double rn = back[mmax];
-    if (rn * rn < 1e-28) {  // converged: the new direction is noise, keep u = V y
+    double hs = 0.0;
+    for (double x : Hk) hs = std::max(hs, std::fabs(x));
+    for (int i = 0; i <= k; ++i) hs = std::max(hs, std::fabs(back[i]));
+    if (!(rn > 64.0 * 2.220446049250313e-16 * hs)) {
       for (int j = it + 1; j < niter && ritz; ++j) ritz[j] = th;
       break;
     }
