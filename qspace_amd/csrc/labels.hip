@@ -24,6 +24,8 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "qsx_internal.hpp"
@@ -278,14 +280,67 @@ __global__ void k_key_heads(const uint64_t* kh, const uint64_t* kl, int n, uint3
 
 // ---- host helpers ----------------------------------------------------------
 
+// The engine's temporaries (sort buffers, match and group arrays: about 25
+// per contraction structure, ~150 per H*psi plan) come from a free list by
+// power-of-two size class.  Every use of such a buffer is on the library
+// stream inside one call and the host reads results only after a
+// synchronisation, so a buffer returned here can be handed out again at
+// once: stream order separates its old and new uses.  At most
+// kTempPoolBytes stay cached; the rest go back to hipFree.
+constexpr size_t kTempPoolBytes = size_t(256) << 20;
+struct TempPool {
+  std::mutex mu;
+  std::multimap<size_t, void*> free;
+  size_t held = 0;
+};
+TempPool& temp_pool() {
+  static TempPool* pool = new TempPool;  // lives until the process ends
+  return *pool;
+}
+size_t temp_class(size_t b) {
+  size_t c = 4096;
+  while (c < b) c <<= 1;
+  return c;
+}
+void* temp_alloc(size_t b) {
+  const size_t c = temp_class(b);
+  {
+    TempPool& P = temp_pool();
+    std::lock_guard<std::mutex> lk(P.mu);
+    auto it = P.free.find(c);
+    if (it != P.free.end()) {
+      void* p = it->second;
+      P.free.erase(it);
+      P.held -= c;
+      return p;
+    }
+  }
+  void* p = nullptr;
+  QSX_HIP(hipMalloc(&p, c));
+  return p;
+}
+void temp_release(void* p, size_t b) {
+  const size_t c = temp_class(b);
+  TempPool& P = temp_pool();
+  {
+    std::lock_guard<std::mutex> lk(P.mu);
+    if (P.held + c <= kTempPoolBytes) {
+      P.free.emplace(c, p);
+      P.held += c;
+      return;
+    }
+  }
+  (void)hipFree(p);
+}
+
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
   explicit DevBuf(size_t b) : bytes(b) {
-    if (b) QSX_HIP(hipMalloc(&p, b));
+    if (b) p = temp_alloc(b);
   }
   ~DevBuf() {
-    if (p) (void)hipFree(p);
+    if (p) temp_release(p, bytes);
   }
   DevBuf(const DevBuf&) = delete;
   template <class T>
