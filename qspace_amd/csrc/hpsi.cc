@@ -108,6 +108,8 @@ struct HpsiPlan {
   double* dws = nullptr;
   size_t ws_doubles = 0;
   double* hstage = nullptr;  // pinned host staging of psi0 and the ground state (N doubles)
+  double* hsmall = nullptr;  // pinned host scalars of the eigensolvers (coefficients, T, projected columns)
+  size_t hsmall_n = 0;
   ~HpsiPlan() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
     for (SplitSide* sp : {&spHL, &spPsi, &spT, &spHR}) {
@@ -136,6 +138,7 @@ struct HpsiPlan {
     dev_free(dsw);
     dev_free(dws);
     if (hstage) (void)hipHostFree(hstage);
+    if (hsmall) (void)hipHostFree(hsmall);
     for (auto e : ev)
       if (e) (void)hipEventDestroy(e);
   }
@@ -988,7 +991,7 @@ static void hpsi_apply(HpsiPlan& H, const double* dpsi, double* dhpsi, hipStream
 // device and a pinned host staging buffer (made once per plan), and a
 // workspace of at least `doubles` elements (grown on demand, kept until the
 // plan is destroyed)
-static double* eig_workspace(HpsiPlan& H, size_t doubles, hipStream_t s) {
+static double* eig_workspace(HpsiPlan& H, size_t doubles, size_t small, hipStream_t s) {
   if (!H.dsw) {
     std::vector<double> sw(H.N);
     for (int64_t i = 0; i < H.N; ++i) sw[i] = std::sqrt(H.welem[i]);
@@ -1000,6 +1003,18 @@ static double* eig_workspace(HpsiPlan& H, size_t doubles, hipStream_t s) {
       H.hstage = nullptr;
       fail(QSX_ERR_NOMEM, "hipHostMalloc(%lld doubles) failed: %s", (long long)H.N, hipGetErrorString(e));
     }
+  }
+  if (H.hsmall_n < small) {
+    QSX_HIP(hipStreamSynchronize(s));
+    if (H.hsmall) (void)hipHostFree(H.hsmall);
+    H.hsmall = nullptr;
+    H.hsmall_n = 0;
+    hipError_t e = hipHostMalloc((void**)&H.hsmall, sizeof(double) * small, hipHostMallocDefault);
+    if (e != hipSuccess) {
+      H.hsmall = nullptr;
+      fail(QSX_ERR_NOMEM, "hipHostMalloc(%zu doubles) failed: %s", small, hipGetErrorString(e));
+    }
+    H.hsmall_n = small;
   }
   if (H.ws_doubles < doubles) {
     QSX_HIP(hipStreamSynchronize(s));
@@ -1058,7 +1073,7 @@ static void lanczos(HpsiPlan& H, const double* psi0, int niter, double* e0, doub
   const size_t nparts = (size_t)DOT_BLOCKS * (niter + 1);
   Carve ws{eig_workspace(H, Carve::size({(size_t)N * (niter + 1), (size_t)std::max<int64_t>(1, N), nparts, nparts,
                                          2 * (size_t)niter, (size_t)niter}),
-                         s)};
+                         2 * (size_t)niter, s)};
   double* V = ws.take((size_t)N * (niter + 1));
   double* w = ws.take(std::max<int64_t>(1, N));
   double* pa = ws.take(nparts);
@@ -1085,11 +1100,10 @@ static void lanczos(HpsiPlan& H, const double* psi0, int niter, double* e0, doub
     vec_gs_update_normalize(w, V, N, pb, j + 1, N, V + (int64_t)(j + 1) * N, beta + j, s);
   }
   clk.lap("enqueue the steps");
-  std::vector<double> abh(2 * (size_t)niter);
-  QSX_HIP(hipMemcpyAsync(abh.data(), ab, sizeof(double) * 2 * niter, hipMemcpyDeviceToHost, s));
+  QSX_HIP(hipMemcpyAsync(H.hsmall, ab, sizeof(double) * 2 * niter, hipMemcpyDeviceToHost, s));
   QSX_HIP(hipStreamSynchronize(s));
   clk.lap("device drain");
-  std::vector<double> al(abh.begin(), abh.begin() + niter), b(abh.begin() + niter, abh.end());
+  std::vector<double> al(H.hsmall, H.hsmall + niter), b(H.hsmall + niter, H.hsmall + 2 * niter);
   b.push_back(0.0);
   // breakdown: a beta_j at rounding level means the Krylov space of psi0
   // is invariant after j+1 vectors (psi0 inside an invariant subspace, or
@@ -1119,7 +1133,8 @@ static void lanczos(HpsiPlan& H, const double* psi0, int niter, double* e0, doub
   clk.lap("tridiagonal eigenproblems");
   if (gs) {
     // the Ritz vector back in the caller's basis: (V y) / sqrt(w)
-    QSX_HIP(hipMemcpyAsync(dy, y.data(), sizeof(double) * m, hipMemcpyHostToDevice, s));
+    std::copy(y.begin(), y.begin() + m, H.hsmall);
+    QSX_HIP(hipMemcpyAsync(dy, H.hsmall, sizeof(double) * m, hipMemcpyHostToDevice, s));
     vec_combine(w, V, N, dy, m, N, s, H.dsw);
     eig_download(H, gs, w, s);
   }
@@ -1135,7 +1150,12 @@ static void davidson(HpsiPlan& H, const double* psi0, int niter, double* e0, dou
   const size_t n1 = std::max<int64_t>(1, N);
   Carve ws{eig_workspace(
       H, Carve::size({(size_t)N * mmax, (size_t)N * mmax, n1, n1, nparts, nparts, (size_t)mmax + 1, (size_t)mmax + 1}),
-      s)};
+      2 * ((size_t)mmax + 1), s)};
+  // pinned scalars: [y and theta for the device (mmax + 1) | the projected
+  // column and |r| back (mmax + 1)]; a step's host writes follow the
+  // previous step's synchronisation, so the copies never overlap them
+  double* hy = H.hsmall;
+  double* hback = H.hsmall + mmax + 1;
   double* V = ws.take((size_t)N * mmax);
   double* HV = ws.take((size_t)N * mmax);
   double* u = ws.take(n1);
@@ -1152,10 +1172,10 @@ static void davidson(HpsiPlan& H, const double* psi0, int niter, double* e0, dou
   eig_hpsi(H, V, HV, s);
   std::vector<double> Hk(1);
   vec_mdot(V, 0, 1, HV, N, pa, ret, 0, s);
-  QSX_HIP(hipMemcpyAsync(&Hk[0], ret, sizeof(double), hipMemcpyDeviceToHost, s));
+  QSX_HIP(hipMemcpyAsync(hback, ret, sizeof(double), hipMemcpyDeviceToHost, s));
   QSX_HIP(hipStreamSynchronize(s));
+  Hk[0] = hback[0];
   int k = 1;
-  std::vector<double> back(mmax + 1);
   for (int it = 0; it < niter; ++it) {
     double th = 0.0;
     std::vector<double> y;
@@ -1164,8 +1184,9 @@ static void davidson(HpsiPlan& H, const double* psi0, int niter, double* e0, dou
     if (e0) *e0 = th;
     // u = V y and the residual r = H u - theta u in one pass (y and theta
     // in one host-to-device copy)
-    y.push_back(th);
-    QSX_HIP(hipMemcpyAsync(dy, y.data(), sizeof(double) * (k + 1), hipMemcpyHostToDevice, s));
+    std::copy(y.begin(), y.begin() + k, hy);
+    hy[k] = th;
+    QSX_HIP(hipMemcpyAsync(dy, hy, sizeof(double) * (k + 1), hipMemcpyHostToDevice, s));
     if (it + 1 == niter) break;
     vec_combine_res(u, hu, V, HV, N, dy, k, N, s);
     // r orthogonalised twice against V and normalised: classical
@@ -1178,8 +1199,9 @@ static void davidson(HpsiPlan& H, const double* psi0, int niter, double* e0, dou
     // grow the projected matrix by one row/column; one host round trip
     // brings the new column and the residual norm
     vec_mdot(V, N, k + 1, HV + (int64_t)k * N, N, pa, ret, 0, s);
-    QSX_HIP(hipMemcpyAsync(back.data(), ret, sizeof(double) * (mmax + 1), hipMemcpyDeviceToHost, s));
+    QSX_HIP(hipMemcpyAsync(hback, ret, sizeof(double) * (mmax + 1), hipMemcpyDeviceToHost, s));
     QSX_HIP(hipStreamSynchronize(s));
+    const double* back = hback;
     // converged when the orthogonalised residual is at rounding level
     // relative to the projected matrix (its entries are bounded by |H|):
     // the new direction would be noise; keep u = V y
