@@ -974,6 +974,28 @@ void upload_packed(const TensorDesc& T, const double* const* h, double* d, hipSt
   }
 }
 
+void upload_many(const std::vector<UploadPart>& parts, hipStream_t s) {
+  int64_t tot = 0;  // doubles, each part starting on a 256-byte boundary
+  for (const UploadPart& p : parts) tot += ((int64_t)p.bytes + 255) / 256 * 32;
+  if (!tot) return;
+  if (tot > kStageMax) {  // beyond the staging buffer: the runtime's own path
+    for (const UploadPart& p : parts)
+      if (p.bytes) QSX_HIP(hipMemcpyAsync(p.d, p.h, p.bytes, hipMemcpyHostToDevice, s));
+    QSX_HIP(hipStreamSynchronize(s));
+    return;
+  }
+  std::lock_guard<std::mutex> lk(g_stage_mu);
+  stage_reserve(tot);
+  char* st = (char*)g_stage;
+  for (const UploadPart& p : parts) {
+    if (!p.bytes) continue;
+    memcpy(st, p.h, p.bytes);
+    QSX_HIP(hipMemcpyAsync(p.d, st, p.bytes, hipMemcpyHostToDevice, s));
+    st += (p.bytes + 255) / 256 * 256;
+  }
+  QSX_HIP(hipStreamSynchronize(s));  // the staging buffer is reused by the next call
+}
+
 static void upload_tensor(const TensorDesc& T, double* d, hipStream_t s, bool imag = false) {
   upload_packed(T, imag ? T.hdata_im : T.hdata, d, s);
 }

@@ -102,6 +102,15 @@ hipStream_t default_stream();
 // host blocks h[i] (T's dims/offsets) -> the packed device buffer d, through
 // the library's pinned staging buffer; returns when the copy is complete
 void upload_packed(const TensorDesc& T, const double* const* h, double* d, hipStream_t s);
+// host arrays -> device arrays through the pinned staging buffer (each part
+// in its own region, one DMA each, one synchronisation at the end); returns
+// when the copies are complete
+struct UploadPart {
+  void* d;
+  const void* h;
+  size_t bytes;
+};
+void upload_many(const std::vector<UploadPart>& parts, hipStream_t s);
 // n doubles of device memory -> host, through the pinned staging buffer:
 // use(h) reads them (h = nullptr when n == 0) while the buffer is held, so
 // use must not upload or download itself

@@ -207,6 +207,14 @@ void build_multi(const std::vector<CtrPlan*>& plans, const std::vector<int64_t>&
   }
 }
 
+// a device array for v, and its upload as a part of upload_many
+template <class T>
+T* alloc_for(const std::vector<T>& v, std::vector<UploadPart>& parts) {
+  T* d = (T*)dev_alloc(sizeof(T) * std::max<size_t>(1, v.size()));
+  parts.push_back({d, v.data(), sizeof(T) * v.size()});
+  return d;
+}
+
 template <class T>
 T* upload_vec(const std::vector<T>& v, hipStream_t s) {
   T* d = (T*)dev_alloc(sizeof(T) * std::max<size_t>(1, v.size()));
@@ -231,14 +239,18 @@ static void hpsi_build_work(HpsiPlan& H) {
   for (void* p : {(void*)H.ditems1, (void*)H.ditems2, (void*)H.dreds1, (void*)H.dreds2, (void*)H.dq1, (void*)H.dq2,
                   (void*)H.dscratch})
     dev_free(p);
-  H.ditems1 = upload_vec(H.work1.items, s);
-  H.ditems2 = upload_vec(H.work2.items, s);
-  H.dreds1 = upload_vec(H.work1.reds, s);
-  H.dreds2 = upload_vec(H.work2.reds, s);
-  H.dq1 = upload_vec(H.work1.queues, s);
-  H.dq2 = upload_vec(H.work2.queues, s);
+  for (auto* p : {(void**)&H.ditems1, (void**)&H.ditems2, (void**)&H.dreds1, (void**)&H.dreds2, (void**)&H.dq1,
+                  (void**)&H.dq2, (void**)&H.dscratch})
+    *p = nullptr;
+  std::vector<UploadPart> parts;
+  H.ditems1 = alloc_for(H.work1.items, parts);
+  H.ditems2 = alloc_for(H.work2.items, parts);
+  H.dreds1 = alloc_for(H.work1.reds, parts);
+  H.dreds2 = alloc_for(H.work2.reds, parts);
+  H.dq1 = alloc_for(H.work1.queues, parts);
+  H.dq2 = alloc_for(H.work2.queues, parts);
   H.dscratch = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, std::max(H.work1.scratch_elems, H.work2.scratch_elems)));
-  QSX_HIP(hipStreamSynchronize(s));
+  upload_many(parts, s);
   if (H.gexec) {
     (void)hipGraphExecDestroy(H.gexec);
     H.gexec = nullptr;
@@ -928,10 +940,14 @@ static void hpsi_create(HpsiPlan& H, const std::vector<const qsx_tensor*>& Ls, c
     }
   }
   clk.lap("pair lists");
-  H.dpairs1 = upload_vec(H.pairs1, s);
-  H.dpairs2 = upload_vec(H.pairs2, s);
-  H.dpairs1s = upload_vec(H.pairs1s, s);
-  H.dpairs2s = upload_vec(H.pairs2s, s);
+  {
+    std::vector<UploadPart> parts;
+    H.dpairs1 = alloc_for(H.pairs1, parts);
+    H.dpairs2 = alloc_for(H.pairs2, parts);
+    H.dpairs1s = alloc_for(H.pairs1s, parts);
+    H.dpairs2s = alloc_for(H.pairs2s, parts);
+    upload_many(parts, s);
+  }
   hpsi_build_work(H);
   clk.lap("work lists");
   if (H.mode == QSX_MODE_SPLIT_BF16 || H.mode == QSX_MODE_SPLIT_F32) hpsi_split_setup(H);
