@@ -729,30 +729,27 @@ static void perm_plan_om(PermPlan& pp, int n, int r, const int64_t* dims, const 
 void plan_upload(CtrPlan& P) {
   PhaseClock clk{P.verbose, "contract plan upload"};
   hipStream_t s = default_stream();
-  dev_free(P.dpairs);
-  P.dpairs = (PairDesc*)dev_alloc(sizeof(PairDesc) * std::max<size_t>(1, P.pairs.size()));
-  if (!P.pairs.empty())
-    QSX_HIP(hipMemcpyAsync(P.dpairs, P.pairs.data(), sizeof(PairDesc) * P.pairs.size(), hipMemcpyHostToDevice, s));
   std::vector<WorkTarget> targets(P.nT);
   for (int t = 0; t < P.nT; ++t) targets[t] = WorkTarget{P.tOff[t], P.M[t], P.N[t], P.pseg[t], P.pseg[t + 1]};
-  clk.lap("pairs to device");
   P.work = build_work(targets, P.pairs, P.policy);
   clk.lap("build_work");
-  dev_free(P.ditems);
-  dev_free(P.dreds);
-  dev_free(P.dqueues);
-  dev_free(P.dscratch);
+  for (void** q : {(void**)&P.dpairs, (void**)&P.ditems, (void**)&P.dreds, (void**)&P.dqueues, (void**)&P.dscratch}) {
+    dev_free(*q);
+    *q = nullptr;
+  }
+  // pairs and work lists in one staged upload (pinned buffer, one synchronisation)
+  std::vector<UploadPart> parts;
+  P.dpairs = (PairDesc*)dev_alloc(sizeof(PairDesc) * std::max<size_t>(1, P.pairs.size()));
+  parts.push_back({P.dpairs, P.pairs.data(), sizeof(PairDesc) * P.pairs.size()});
   P.ditems = (WorkItem*)dev_alloc(sizeof(WorkItem) * std::max<size_t>(1, P.work.items.size()));
+  parts.push_back({P.ditems, P.work.items.data(), sizeof(WorkItem) * P.work.items.size()});
   P.dreds = (ReduceItem*)dev_alloc(sizeof(ReduceItem) * std::max<size_t>(1, P.work.reds.size()));
+  parts.push_back({P.dreds, P.work.reds.data(), sizeof(ReduceItem) * P.work.reds.size()});
   P.dqueues = (WorkQueue*)dev_alloc(sizeof(WorkQueue) * std::max<size_t>(1, P.work.queues.size()));
-  if (!P.work.queues.empty())
-    QSX_HIP(hipMemcpyAsync(P.dqueues, P.work.queues.data(), sizeof(WorkQueue) * P.work.queues.size(), hipMemcpyHostToDevice, s));
-  if (!P.work.reds.empty())
-    QSX_HIP(hipMemcpyAsync(P.dreds, P.work.reds.data(), sizeof(ReduceItem) * P.work.reds.size(), hipMemcpyHostToDevice, s));
+  parts.push_back({P.dqueues, P.work.queues.data(), sizeof(WorkQueue) * P.work.queues.size()});
   P.dscratch = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, P.work.scratch_elems));
-  if (!P.work.items.empty())
-    QSX_HIP(hipMemcpyAsync(P.ditems, P.work.items.data(), sizeof(WorkItem) * P.work.items.size(), hipMemcpyHostToDevice, s));
-  clk.lap("work list to device");
+  upload_many(parts, s);
+  clk.lap("pairs and work list to device");
   if (P.la.permute && !P.a_prepermuted) {
     perm_plan_om(P.ppA, P.A.n, P.A.r, P.A.dims.data(), P.A.om, P.A.offs.data(), P.la.perm);
     perm_plan_upload(P.ppA, s);

@@ -148,9 +148,9 @@ void perm_plan_upload(PermPlan& pp, hipStream_t s) {
   if (pp.items.empty()) return;
   pp.ddescs = (PermDesc*)dev_alloc(sizeof(PermDesc) * pp.descs.size());
   pp.ditems = (PermItem*)dev_alloc(sizeof(PermItem) * pp.items.size());
-  QSX_HIP(hipMemcpyAsync(pp.ddescs, pp.descs.data(), sizeof(PermDesc) * pp.descs.size(), hipMemcpyHostToDevice, s));
-  QSX_HIP(hipMemcpyAsync(pp.ditems, pp.items.data(), sizeof(PermItem) * pp.items.size(), hipMemcpyHostToDevice, s));
-  QSX_HIP(hipStreamSynchronize(s));
+  upload_many({{pp.ddescs, pp.descs.data(), sizeof(PermDesc) * pp.descs.size()},
+               {pp.ditems, pp.items.data(), sizeof(PermItem) * pp.items.size()}},
+              s);
 }
 
 namespace {
