@@ -265,6 +265,16 @@ class QSpace:
         return f"QSpace(rank={self.rank}, nblk={self.nblk}, qtype='{self.qtype}', itags={self.itags})"
 
 
+def _addr(a: np.ndarray) -> int:
+    """data address of a Fortran-contiguous array: a ctypes view of its
+    C-contiguous transpose (same first element) costs about 40% of an
+    __array_interface__ lookup; read-only and empty arrays take the lookup"""
+    try:
+        return C.addressof(C.c_char.from_buffer(a.T))
+    except (TypeError, ValueError):
+        return a.__array_interface__["data"][0]
+
+
 def to_c(T: QSpace, with_data: bool = True):
     """QSpace -> (QsxTensor, keepalive list)."""
     keep = []
@@ -300,12 +310,12 @@ def to_c(T: QSpace, with_data: bool = True):
                 a = np.asfortranarray(np.real(d), dtype=np.float64)
                 b = np.asfortranarray(np.imag(d), dtype=np.float64)
                 keep += [a, b]
-                aim[i] = b.__array_interface__["data"][0]
+                aim[i] = _addr(b)
             else:
                 a = d if (d.dtype == np.float64 and d.flags.f_contiguous) else np.asfortranarray(d, dtype=np.float64)
                 if a is not d:
                     keep.append(a)
-            addrs[i] = a.__array_interface__["data"][0]
+            addrs[i] = _addr(a)
         keep += [addrs, T.data]
         ct.data = addrs.ctypes.data_as(C.POINTER(C.POINTER(C.c_double)))
         if cplx:
