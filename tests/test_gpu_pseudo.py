@@ -74,3 +74,24 @@ def test_pseudo_hermitian_and_eigensolvers(gpu):
     for solver in (H.lanczos, H.davidson):
         e0, _, gs = solver(rng.standard_normal(H.N), H.N)
         assert abs(e0 - e_ref) < 1e-10 * max(1.0, abs(e_ref)), (solver.__name__, e0, e_ref)
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_random_pseudo_operators(gpu, seed):
+    """random pseudo-MPO operators (sizes, 1-3 operator pairs, random hconj
+    per pair) under the default decomposition: the oracle bar, and a second
+    apply bitwise equal"""
+    rng = np.random.RandomState(500 + seed)
+    Dstar = int(rng.choice([20, 60, 150, 400]))
+    nop = int(rng.randint(1, 4))
+    HKL, psi, HKR, OPL, OPR = gen.heis_pseudo_hpsi(seed=500 + seed, Dstar=Dstar, mu=float(rng.uniform(1.0, 3.0)),
+                                                   sigma=float(rng.uniform(0.3, 0.9)),
+                                                   dmax=int(rng.choice([6, 30, 90])), nop=nop)
+    hc = [int(rng.randint(0, 2)) for _ in range(nop)]
+    print(f"seed {seed}: D*={Dstar} nop={nop} hconj={hc}", flush=True)
+    H = Q.HPsi.pseudo(HKL, psi, HKR, OPL, OPR, hc)
+    x = rng.standard_normal(H.N)
+    ref = pseudo_hpsi_oracle(HKL, psi.with_packed(x), HKR, OPL, OPR, hc).packed()
+    got = H.apply(x)
+    assert _err(got, ref) < TOL
+    assert np.array_equal(H.apply(x), got)
