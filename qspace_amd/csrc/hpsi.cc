@@ -1008,6 +1008,9 @@ static void hpsi_apply(HpsiPlan& H, const double* dpsi, double* dhpsi, hipStream
 // workspace of at least `doubles` elements (grown on demand, kept until the
 // plan is destroyed)
 static double* eig_workspace(HpsiPlan& H, size_t doubles, size_t small, hipStream_t s) {
+  // nothing of an earlier call (one that failed part-way included) may
+  // still read the pinned buffers the host is about to write
+  QSX_HIP(hipStreamSynchronize(s));
   if (!H.dsw) {
     std::vector<double> sw(H.N);
     for (int64_t i = 0; i < H.N; ++i) sw[i] = std::sqrt(H.welem[i]);
@@ -1021,7 +1024,6 @@ static double* eig_workspace(HpsiPlan& H, size_t doubles, size_t small, hipStrea
     }
   }
   if (H.hsmall_n < small) {
-    QSX_HIP(hipStreamSynchronize(s));
     if (H.hsmall) (void)hipHostFree(H.hsmall);
     H.hsmall = nullptr;
     H.hsmall_n = 0;
@@ -1033,7 +1035,6 @@ static double* eig_workspace(HpsiPlan& H, size_t doubles, size_t small, hipStrea
     H.hsmall_n = small;
   }
   if (H.ws_doubles < doubles) {
-    QSX_HIP(hipStreamSynchronize(s));
     dev_free(H.dws);
     H.dws = nullptr;
     H.ws_doubles = 0;
