@@ -109,6 +109,7 @@ struct HpsiPlan {
   size_t ws_doubles = 0;
   double* hstage = nullptr;  // pinned host staging of psi0 and the ground state (N doubles)
   double* hsmall = nullptr;  // pinned host scalars of the eigensolvers (coefficients, T, projected columns)
+  double* dsmall = nullptr;  // the same memory as kernels address it (hipHostGetDevicePointer)
   size_t hsmall_n = 0;
   ~HpsiPlan() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -1027,10 +1028,19 @@ static double* eig_workspace(HpsiPlan& H, size_t doubles, size_t small, hipStrea
     if (H.hsmall) (void)hipHostFree(H.hsmall);
     H.hsmall = nullptr;
     H.hsmall_n = 0;
-    hipError_t e = hipHostMalloc((void**)&H.hsmall, sizeof(double) * small, hipHostMallocDefault);
+    H.dsmall = nullptr;
+    // mapped and coherent: kernels store the Davidson scalars here and the
+    // host reads them after the stream's synchronisation
+    hipError_t e = hipHostMalloc((void**)&H.hsmall, sizeof(double) * small, hipHostMallocMapped | hipHostMallocCoherent);
     if (e != hipSuccess) {
       H.hsmall = nullptr;
       fail(QSX_ERR_NOMEM, "hipHostMalloc(%zu doubles) failed: %s", small, hipGetErrorString(e));
+    }
+    e = hipHostGetDevicePointer((void**)&H.dsmall, H.hsmall, 0);
+    if (e != hipSuccess || !H.dsmall) {
+      (void)hipHostFree(H.hsmall);
+      H.hsmall = H.dsmall = nullptr;
+      fail(QSX_ERR_DEVICE, "hipHostGetDevicePointer failed: %s", hipGetErrorString(e));
     }
     H.hsmall_n = small;
   }
@@ -1166,30 +1176,29 @@ static void davidson(HpsiPlan& H, const double* psi0, int niter, double* e0, dou
   const size_t nparts = (size_t)DOT_BLOCKS * (mmax + 1);
   const size_t n1 = std::max<int64_t>(1, N);
   Carve ws{eig_workspace(
-      H, Carve::size({(size_t)N * mmax, (size_t)N * mmax, n1, n1, nparts, nparts, (size_t)mmax + 1, (size_t)mmax + 1}),
+      H, Carve::size({(size_t)N * mmax, (size_t)N * mmax, n1, n1, nparts, nparts, (size_t)mmax + 1}),
       2 * ((size_t)mmax + 1), s)};
   // pinned scalars: [y and theta for the device (mmax + 1) | the projected
-  // column and |r| back (mmax + 1)]; a step's host writes follow the
+  // column and |r| (mmax + 1)]; the kernels that form the column and |r|
+  // store them there directly (a few doubles over the link, read by the
+  // host after the step's synchronisation); a step's host writes follow the
   // previous step's synchronisation, so the copies never overlap them
   double* hy = H.hsmall;
   double* hback = H.hsmall + mmax + 1;
+  double* dback = H.dsmall + mmax + 1;
   double* V = ws.take((size_t)N * mmax);
   double* HV = ws.take((size_t)N * mmax);
   double* u = ws.take(n1);
   double* hu = ws.take(n1);
   double* pa = ws.take(nparts);
   double* pb = ws.take(nparts);
-  // ret = [new column of the projected matrix (mmax) | |r|] : one
-  // device-to-host copy per step
-  double* ret = ws.take(mmax + 1);
   double* dy = ws.take(mmax + 1);
   eig_upload(H, hu, psi0, s);
   vec_scale_norm_partial(hu, H.dsw, u, N, pa, s);
   vec_normalize_partial(u, V, pa, nullptr, N, s);
   eig_hpsi(H, V, HV, s);
   std::vector<double> Hk(1);
-  vec_mdot(V, 0, 1, HV, N, pa, ret, 0, s);
-  QSX_HIP(hipMemcpyAsync(hback, ret, sizeof(double), hipMemcpyDeviceToHost, s));
+  vec_mdot(V, 0, 1, HV, N, pa, dback, 0, s);
   QSX_HIP(hipStreamSynchronize(s));
   Hk[0] = hback[0];
   int k = 1;
@@ -1211,12 +1220,11 @@ static void davidson(HpsiPlan& H, const double* psi0, int niter, double* e0, dou
     // the second pass fused with the normalisation (|r| to ret[mmax])
     vec_mdot_partial(V, N, k, hu, N, pa, s);
     vec_gs_update_mdot(hu, V, N, pa, k, N, nullptr, 0, V, k, 1, pb, s);
-    vec_gs_update_normalize(hu, V, N, pb, k, N, V + (int64_t)k * N, ret + mmax, s);
+    vec_gs_update_normalize(hu, V, N, pb, k, N, V + (int64_t)k * N, dback + mmax, s);
     eig_hpsi(H, V + (int64_t)k * N, HV + (int64_t)k * N, s);
     // grow the projected matrix by one row/column; one host round trip
     // brings the new column and the residual norm
-    vec_mdot(V, N, k + 1, HV + (int64_t)k * N, N, pa, ret, 0, s);
-    QSX_HIP(hipMemcpyAsync(hback, ret, sizeof(double) * (mmax + 1), hipMemcpyDeviceToHost, s));
+    vec_mdot(V, N, k + 1, HV + (int64_t)k * N, N, pa, dback, 0, s);
     QSX_HIP(hipStreamSynchronize(s));
     const double* back = hback;
     // converged when the orthogonalised residual is at rounding level
