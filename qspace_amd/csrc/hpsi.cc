@@ -1099,16 +1099,18 @@ static void lanczos(HpsiPlan& H, const double* psi0, int niter, double* e0, doub
   PhaseClock clk{H.verbose, "Lanczos"};
   const size_t nparts = (size_t)DOT_BLOCKS * (niter + 1);
   Carve ws{eig_workspace(H, Carve::size({(size_t)N * (niter + 1), (size_t)std::max<int64_t>(1, N), nparts, nparts,
-                                         2 * (size_t)niter, (size_t)niter}),
+                                         (size_t)niter}),
                          2 * (size_t)niter, s)};
   double* V = ws.take((size_t)N * (niter + 1));
   double* w = ws.take(std::max<int64_t>(1, N));
   double* pa = ws.take(nparts);
   double* pb = ws.take(nparts);
-  double* ab = ws.take(2 * (size_t)niter);  // [alpha | beta]
   double* dy = ws.take(niter);
-  double* alpha = ab;
-  double* beta = ab + niter;
+  // alpha_j and beta_j are stored by the kernels into the mapped pinned
+  // buffer [alpha | beta] and read by the host after the loop's
+  // synchronisation
+  double* alpha = H.dsmall;
+  double* beta = H.dsmall + niter;
   // v_0 = sqrt(w) psi0 / |sqrt(w) psi0| (psi0 lands in the unused slot v_1)
   eig_upload(H, V + N, psi0, s);
   vec_scale_norm_partial(V + N, H.dsw, w, N, pa, s);
@@ -1127,7 +1129,6 @@ static void lanczos(HpsiPlan& H, const double* psi0, int niter, double* e0, doub
     vec_gs_update_normalize(w, V, N, pb, j + 1, N, V + (int64_t)(j + 1) * N, beta + j, s);
   }
   clk.lap("enqueue the steps");
-  QSX_HIP(hipMemcpyAsync(H.hsmall, ab, sizeof(double) * 2 * niter, hipMemcpyDeviceToHost, s));
   QSX_HIP(hipStreamSynchronize(s));
   clk.lap("device drain");
   std::vector<double> al(H.hsmall, H.hsmall + niter), b(H.hsmall + niter, H.hsmall + 2 * niter);
