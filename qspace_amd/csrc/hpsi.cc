@@ -317,6 +317,31 @@ static double time_stage(HpsiPlan& H, int stage, const WorkList& W, const double
 }
 
 static void hpsi_launch(HpsiPlan& H, const double* dpsi, double* dhpsi, hipStream_t s, bool timed, bool sym);
+static void hpsi_build_work(HpsiPlan& H);
+
+// device ms of one whole H*psi (both stages, as uploaded), from runs of 8
+// back-to-back applies: median over reps
+static double time_whole(HpsiPlan& H, const double* dpsi, double* dout, int reps) {
+  hipStream_t s = default_stream();
+  constexpr int kRun = 8;
+  std::vector<float> ms;
+  hipEvent_t e0, e1;
+  QSX_HIP(hipEventCreate(&e0));
+  QSX_HIP(hipEventCreate(&e1));
+  for (int r = 0; r < reps + 1; ++r) {
+    QSX_HIP(hipEventRecord(e0, s));
+    for (int i = 0; i < kRun; ++i) hpsi_launch(H, dpsi, dout, s, false, false);
+    QSX_HIP(hipEventRecord(e1, s));
+    QSX_HIP(hipEventSynchronize(e1));
+    float t = 0;
+    QSX_HIP(hipEventElapsedTime(&t, e0, e1));
+    if (r >= 1) ms.push_back(t / kRun);
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  std::sort(ms.begin(), ms.end());
+  return ms[ms.size() / 2];
+}
 
 static void hpsi_autotune(HpsiPlan& H, int reps, double* out) {
   if (H.mode != QSX_MODE_BATCHED) {  // the other modes have fixed work lists
@@ -341,6 +366,7 @@ static void hpsi_autotune(HpsiPlan& H, int reps, double* out) {
     if (out) out[0] = out[1] = -1;
     return;
   }
+  const WorkPolicy in1 = H.pol1, in2 = H.pol2;
   double* dpsi = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, H.N));
   double* dout = (double*)dev_alloc(sizeof(double) * std::max<int64_t>(1, H.N));
   launch_fill_zero(dpsi, H.N, default_stream());
@@ -395,6 +421,31 @@ static void hpsi_autotune(HpsiPlan& H, int reps, double* out) {
     }
     (stage == 0 ? H.pol1 : H.pol2) = bp;
     if (out) out[stage] = best;
+  }
+  // each stage was timed alone; a stage's cost inside an H*psi (after the
+  // other stage, with the caches holding that stage's operands) can rank the
+  // candidates differently, so the tuned pair is kept only if the whole
+  // H*psi beats the plan's decomposition on entry (the default one of a
+  // fresh plan, or the caller's)
+  try {
+    const WorkPolicy t1 = H.pol1, t2 = H.pol2;
+    hpsi_build_work(H);
+    const double t_tuned = time_whole(H, dpsi, dout, reps);
+    H.pol1 = in1;
+    H.pol2 = in2;
+    hpsi_build_work(H);
+    const double t_in = time_whole(H, dpsi, dout, reps);
+    if (H.verbose)
+      fprintf(stderr, "qsx autotune: whole H*psi %.4f ms tuned, %.4f ms with the entry decomposition\n", t_tuned,
+              t_in);
+    if (t_tuned < t_in) {
+      H.pol1 = t1;
+      H.pol2 = t2;
+    }
+  } catch (...) {
+    dev_free(dpsi);
+    dev_free(dout);
+    throw;
   }
   dev_free(dpsi);
   dev_free(dout);
